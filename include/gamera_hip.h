@@ -1,0 +1,143 @@
+/*
+ * gamera_hip.h -- C-ABI of the MI355X GAMERA MHD step (libgamera_hip.so).
+ *
+ * Plain C types only: dimensions, double pointers, sizes.  Every host array is
+ * a contiguous column-major (Fortran) array with the reference's index bounds:
+ *   cell arrays  (isg:ieg,   jsg:jeg,   ksg:keg,   ncomp)   "G"
+ *   face arrays  (isg:ieg+1, jsg:jeg+1, ksg:keg+1, ncomp)   "F"
+ * i.e. exactly what c_loc() of the reference's allocatables gives
+ * (reference src/base/types/gamtypes.F90:163-264).
+ *
+ * Threading: one host thread per handle; all calls on a handle are ordered on
+ * the handle's HIP stream.  Calls that return host data synchronise.
+ * Errors: every entry point returns 0 on success, > 0 for a recoverable
+ * numerical condition (the reference's "bad timestep" path), < 0 for a fatal
+ * error (bad argument, HIP failure).  gam_last_error() has the text.  The
+ * library never exits the process.
+ */
+#ifndef GAMERA_HIP_H
+#define GAMERA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gam_handle gam_handle;
+
+/* Grid_T sizes: active cells and ghost depth (reference gamtypes.F90:163-180,
+ * init.F90:327 nG = 4). */
+typedef struct {
+    int32_t Nip, Njp, Nkp, nG;
+} gam_dims;
+
+/* Model_T numerical knobs (reference gamtypes.F90:71-146, defaults read in
+ * init.F90:312-492 initModel). */
+typedef struct {
+    double gamma, CFL, pdmb, Vd0, cHogH, cHogM, Ca, dFloor, pFloor, tFin, limDT0, limCPR;
+    int32_t rmeth;          /* 0 = 8CENT (Cen8LRs), 1 = 7UP (Up7LRs), init.F90:450-462 */
+    int32_t doLFMLim;       /* limit on state n (gamtypes.F90:123)                    */
+    int32_t doBoris;        /* Boris correction with speed Ca                         */
+    int32_t doMHD;
+    int32_t do25D;          /* Nkp == 1 (init.F90:612-616)                             */
+    int32_t doCPR;
+    int32_t fixedTimestep;
+    int32_t precision;      /* 0 = FP64 everywhere; 1 = FP32 bulk, FP64 CT/energy/dt   */
+} gam_params;
+
+/* Model_T / State_T time bookkeeping that the step reads and writes. */
+typedef struct {
+    double t;           /* Model%t                                   */
+    double dt;          /* Model%dt                                  */
+    double dt0;         /* Model%dt0 (0 until captured)              */
+    double stateTime;   /* State%time                                */
+    double oStateTime;  /* oState%time                               */
+    int32_t ts;         /* Model%ts                                  */
+    int32_t status;     /* last CalcDT outcome: 0 ok, 1 bad, 2 disaster */
+    int32_t dt0_after;  /* capture dt0 at the first step with ts >= dt0_after (0: never) */
+    int32_t pad;
+} gam_clock;
+
+/* --- lifetime ------------------------------------------------------------ */
+/* Replaces the reference's allocation in Hatch/initSolver (init.F90:38-59,
+ * 494-516): device State, oState, predictor state and solver scratch. */
+int  gam_create(const gam_dims *dims, const gam_params *params, gam_handle **out);
+int  gam_destroy(gam_handle *h);
+const char *gam_last_error(const gam_handle *h);
+int  gam_sync(gam_handle *h);
+void *gam_stream(gam_handle *h);         /* the handle's hipStream_t */
+
+/* --- grid ----------------------------------------------------------------- */
+/* Upload Grid_T geometry computed by the caller (the arrays of
+ * gamtypes.F90:183-221).  Sizes: xyz F*3, xyzcc G*3, xfc F*9, face F*3, Tf F*27,
+ * edge F*3, Te F*27, Teb F*12, volume/di/dj/dk G. */
+int  gam_set_grid(gam_handle *h, const double *xyz, const double *xyzcc, const double *xfc,
+                  const double *face, const double *Tf, const double *edge, const double *Te,
+                  const double *Teb, const double *volume, const double *di, const double *dj,
+                  const double *dk);
+/* Upload the corners only and derive the rest on the GPU
+ * (replaces Corners2Grid + ebGeom, reference init.F90:801-1145). */
+int  gam_set_corners(gam_handle *h, const double *xyz);
+/* Download the geometry (same sizes as gam_set_grid; NULL skips an array). */
+int  gam_get_grid(gam_handle *h, double *xyz, double *xyzcc, double *xfc, double *face,
+                  double *Tf, double *edge, double *Te, double *Teb, double *volume,
+                  double *di, double *dj, double *dk);
+
+/* --- state ---------------------------------------------------------------- */
+/* which: 0 = State, 1 = oState, 2 = predictor (half-step) state (read only).
+ * Gas G*5, magFlux F*3, Bxyz G*3; NULL skips an array. */
+int  gam_put_state(gam_handle *h, int which, double time, const double *Gas,
+                   const double *magFlux, const double *Bxyz);
+int  gam_get_state(gam_handle *h, int which, double *time, double *Gas,
+                   double *magFlux, double *Bxyz);
+int  gam_set_clock(gam_handle *h, const gam_clock *c);
+int  gam_get_clock(gam_handle *h, gam_clock *c);
+
+/* --- initial conditions on the GPU (setup producers) ----------------------- */
+/* prob.F90:74-119 initOT2D with wavenumber factor kscale. */
+int  gam_ic_orszag_tang(gam_handle *h, double rho0, double p0, double b0, double kscale);
+/* prob.F90:190-298 initBW via GasIC2State and VectorPot2Flux (gridutils.F90:144-237). */
+int  gam_ic_blast_wave(gam_handle *h, double x0, double y0, double z0, double D0, double P0,
+                       double pRat, double dRat, double rC, double B0);
+/* init.F90:204-244: first Bxyz, oState = State, initial BCs, first dt. */
+int  gam_hatch(gam_handle *h);
+
+/* --- the step ------------------------------------------------------------- */
+/* AdvanceMHD(Model,Grid,State,oState,ooState,Solver,Model%dt), mhdgroup.F90:16-76.
+ * Leaves State (pre-BC) = new state, oState = old state. */
+int  gam_advance_mhd(gam_handle *h);
+/* CalcDT (step.F90:58-168) after UpdateStateData's ts += 1, t += dt
+ * (gamapp.F90:112-114).  Writes Model%dt into the clock; *status as gam_clock. */
+int  gam_calc_dt(gam_handle *h, double *dt, int *status);
+/* EnforceBCs + bFlux2Fld (step.F90:17-55, gamapp.F90:86-90), periodic BCs. */
+int  gam_enforce_bcs(gam_handle *h);
+/* nsteps x stepGamera (gamapp.F90:73-94), fully on the GPU with dt kept in
+ * device memory; returns the first non-zero CalcDT status. */
+int  gam_step(gam_handle *h, int nsteps);
+/* DivB total (gridutils.F90:717-770). */
+int  gam_divb(gam_handle *h, double *tot);
+
+/* --- diagnostics ---------------------------------------------------------- */
+/* Copy a solver work array of the last step to the host, for stage-level
+ * checks (the reference keeps them in gamSolver_T, gamtypes.F90:267-286):
+ * which 0 = interface fluxes, G*24 (8 per direction: 5 gas then 3 magnetic,
+ * already times face area); 1 = edge EMFs times edge length, G*3; 2 = face
+ * velocities of Mom2Face, G*9 (3 per sweep direction I, J, K). */
+int  gam_get_work(gam_handle *h, int which, double *out);
+
+/* --- instrumentation ------------------------------------------------------ */
+/* Per-stage HIP-event timers, like the reference's Tic/Toc clocks
+ * (base/clocks.F90:84-182).  Stage ids are listed by gam_stage_name. */
+int  gam_set_timing(gam_handle *h, int enable);
+int  gam_get_timing(gam_handle *h, int stage, double *ms_total, int64_t *launches);
+const char *gam_stage_name(int stage);
+int  gam_num_stages(void);
+/* Bytes each stage's kernels must move per launch by the algorithm (inputs
+ * read once, outputs written once); used for roofline reporting. */
+int  gam_stage_bytes(gam_handle *h, int stage, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,108 @@
+"""Build the native parts in-tree.
+
+* ``kaiju_amd/lib/libgamera_hip.so`` -- the HIP step for gfx950 (hipcc).
+* ``kaiju_amd/bin/gamerax_gpu`` and ``kaiju_amd/lib/libgamhost.so`` -- the
+  Fortran host (amdflang) that keeps the reference's Model/Grid/State types and
+  calls the C-ABI.
+
+Usage: ``python -m kaiju_amd.build [hip|fortran|all]``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+FSRC = os.path.join(PKG, "fortran")
+LIBDIR = os.path.join(PKG, "lib")
+BINDIR = os.path.join(PKG, "bin")
+HIP_LIB = os.path.join(LIBDIR, "libgamera_hip.so")
+
+# one translation unit: gam_api.hip includes gam_step.hip and gam_setup.hip
+HIP_SOURCES = ["gam_api.hip"]
+HIP_DEPS = ["gam_api.hip", "gam_step.hip", "gam_setup.hip", "gam_common.h"]
+
+# -ffp-contract=off keeps every multiply and add separately rounded, as the
+# reference's CPU build does; packed FP32 ops are disabled on this toolchain.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _flang() -> str:
+    for c in (os.environ.get("FC"), shutil.which("amdflang"), "/opt/rocm/bin/amdflang", shutil.which("flang")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("amdflang not found")
+
+
+def build_hip(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = HIP_LIB + ".tmp"
+    cmd = [_hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HIP_LIB)
+    return HIP_LIB
+
+
+FORTRAN_MODULES = ["gamera_hip_c.F90", "gamtypes_gpu.F90", "gamapp_gpu.F90", "gamprob_gpu.F90"]
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Fortran host: modules -> libgamhost.so (+ the gamerax_gpu driver)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(BINDIR, exist_ok=True)
+    objdir = os.path.join(PKG, "_fobj")
+    os.makedirs(objdir, exist_ok=True)
+    fc = _flang()
+    fflags = ["-O2", "-fPIC", "-cpp", "-ffree-form", "-module-dir", objdir, "-I", objdir]
+    objs = []
+    for m in FORTRAN_MODULES:
+        src = os.path.join(FSRC, m)
+        obj = os.path.join(objdir, m.replace(".F90", ".o"))
+        cmd = [fc, *fflags, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    lib = os.path.join(LIBDIR, "libgamhost.so")
+    rpath = "-Wl,-rpath,$ORIGIN"
+    subprocess.run([fc, "-shared", "-fPIC", *objs, "-o", lib, "-L", LIBDIR, "-lgamera_hip", rpath], check=True)
+    exe = os.path.join(BINDIR, "gamerax_gpu")
+    drv = os.path.join(FSRC, "gamerax_gpu.F90")
+    subprocess.run([fc, *fflags, drv, *objs, "-o", exe, "-L", LIBDIR, "-lgamera_hip",
+                    "-Wl,-rpath,$ORIGIN/../lib"], check=True)
+    return exe
+
+
+def build_all(verbose: bool = False):
+    build_hip(verbose)
+    if os.path.isdir(FSRC) and all(os.path.exists(os.path.join(FSRC, m)) for m in FORTRAN_MODULES):
+        build_fortran(verbose)
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    if what == "hip":
+        build_hip(True)
+    elif what == "fortran":
+        build_fortran(True)
+    else:
+        build_all(True)

@@ -1,0 +1,103 @@
+"""Problem specifications for the GAMERA step (grid, initial condition, model knobs).
+
+These are plain data shared by the HIP host mirror (``kaiju_amd.gamera``), the
+tests and ``bench.py``.  Each field maps onto a key the reference reads from its
+XML deck (reference ``src/gamera/init.F90:312-492`` ``initModel`` and
+``init.F90:572-798`` ``genGridXML``); values that the reference derives from
+single-precision literals are written out explicitly so every consumer sees the
+same double.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+from typing import Tuple
+
+# init.F90:32 MaxCFL = 0.3 is a default-real literal, widened to double.
+CFL_DEFAULT = 0.30000001192092896
+
+
+@dataclasses.dataclass
+class ModelSpec:
+    """Model_T knobs (reference base/types/gamtypes.F90:71-146)."""
+    gamma: float = 5.0 / 3.0
+    CFL: float = 0.3
+    pdmb: float = 0.75
+    Vd0: float = 0.5
+    cHogH: float = 0.25
+    cHogM: float = 0.25
+    Ca: float = 1.0e12          # reference: Ca = HUGE when Boris is off
+    dFloor: float = 1.0e-12     # gdefs.F90:13 TINY
+    pFloor: float = 1.0e-12
+    tFin: float = 1.0e30
+    limDT0: float = 1.0e-3
+    limCPR: float = 1.0e-1
+    rmeth: str = "7UP"          # "7UP" or "8CENT" (init.F90:450-462)
+    doLFMLim: bool = True
+    doBoris: bool = False
+    doMHD: bool = True
+    doCPR: bool = False
+
+
+@dataclasses.dataclass
+class CaseSpec:
+    name: str
+    N: Tuple[int, int, int]
+    bounds: Tuple[float, float, float, float, float, float]
+    grid: str = "cart"          # "cart" | "warp3" | "refwarp"
+    warp_amp: float = 0.0
+    ic: str = "ot"              # "ot" | "bw"
+    ic_params: dict = dataclasses.field(default_factory=dict)
+    model: ModelSpec = dataclasses.field(default_factory=ModelSpec)
+    nG: int = 4
+    dt0_after: int = 1          # console output (dt0 capture) after this many steps
+    steps: int = 100
+
+    @property
+    def zones(self) -> int:
+        return self.N[0] * self.N[1] * self.N[2]
+
+    @property
+    def do25D(self) -> bool:
+        return self.N[2] == 1
+
+
+def orszag_tang(n: int = 256, nk: int = 4, steps: int = 200, **mkw) -> CaseSpec:
+    """2D Orszag-Tang vortex on [0,2pi]^2 x [0, nk*2pi/n], gamma = 5/3.
+
+    rho = gamma^2, p = gamma, v = (-sin y, sin x, 0), B = (-sin y, sin 2x, 0);
+    the reference ``initOT2D`` (prob.F90:74-119) with a unit wavenumber.
+    """
+    g = 5.0 / 3.0
+    L = 2.0 * math.pi
+    m = ModelSpec(gamma=g, CFL=0.3, **mkw)
+    return CaseSpec(name=f"ot{n}x{n}x{nk}", N=(n, n, nk),
+                    bounds=(0.0, L, 0.0, L, 0.0, nk * L / n), grid="cart", ic="ot",
+                    ic_params=dict(rho0=g * g, p0=g, b0=1.0, kscale=1.0),
+                    model=m, steps=steps)
+
+
+def blast_wave(n: int = 256, steps: int = 100, warp_amp: float = 0.05, B0: float = 10.0,
+               pin: float = 10.0, pout: float = 0.1, **mkw) -> CaseSpec:
+    """3D MHD blast wave on a warped non-orthogonal grid over the unit cube.
+
+    x = xi + a sin(2 pi xi) sin(2 pi eta) sin(2 pi zeta) (same for y, z);
+    rho = 1, p = pout outside / pin inside r < 0.1, v = 0, B = B0 (1,1,0)/sqrt 2;
+    the reference ``initBW`` (prob.F90:190-298) with x0 = y0 = z0 = 0.5.
+    """
+    m = ModelSpec(gamma=5.0 / 3.0, CFL=0.3, **mkw)
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+    return CaseSpec(name=f"bw{nx}x{ny}x{nz}", N=(nx, ny, nz), bounds=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0),
+                    grid="warp3" if warp_amp else "cart", warp_amp=warp_amp, ic="bw",
+                    ic_params=dict(x0=0.5, y0=0.5, z0=0.5, D0=1.0, P0=pout,
+                                   pRat=pin / pout, dRat=1.0, rC=0.1, B0=B0),
+                    model=m, steps=steps)
+
+
+# The two workloads named by the benchmark configuration list.
+def config_c1() -> CaseSpec:
+    return orszag_tang(256, 4, 200)
+
+
+def config_c2() -> CaseSpec:
+    return blast_wave(256, 100)

@@ -1,0 +1,629 @@
+// gam_api.hip -- the C-ABI of libgamera_hip.so: handle, device memory, stream,
+// launch sequencing and per-stage timers.  Declared in include/gamera_hip.h.
+#include "gamera_hip.h"
+#include "gam_common.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// The kernels are part of this translation unit.
+#include "gam_step.hip"
+#include "gam_setup.hip"
+
+// ----------------------------------------------------------------- stages
+enum Stage {
+    ST_CLOCK = 0, ST_PREDICT, ST_BHALF, ST_MOM2FACE, ST_EMF, ST_FLUX_I, ST_FLUX_J, ST_FLUX_K,
+    ST_CT, ST_UPDATE, ST_DTFINAL, ST_BC, ST_BPOST, ST_CELLDT, ST_N
+};
+static const char *kStageNames[ST_N] = {
+    "clock", "predictor", "bflux_half", "mom2face", "emf", "flux_i", "flux_j", "flux_k",
+    "ct_update", "update", "dt_final", "bcs", "bflux_post", "celldt"};
+
+enum { G_XYZ = 0, G_XYZCC, G_XFC, G_FACE, G_TF, G_EDGE, G_TE, G_TEB, G_VOL, G_DI, G_DJ, G_DK, G_N };
+
+struct PendingEv { int stage; hipEvent_t a, b; };
+
+struct gam_handle {
+    gam_dims dims;
+    gam_params par;
+    GV gv;
+    MP mp;
+    hipStream_t stream = nullptr;
+    double *grid[G_N] = {};
+    double *Gas[2] = {}, *mag[2] = {}, *B[2] = {};
+    int s = 0;                 // index of State in the {Gas,mag,B} pairs
+    double *hGas = nullptr, *hmag = nullptr, *hB = nullptr;
+    double *Vf = nullptr, *E = nullptr, *Flx = nullptr, *partial = nullptr;
+    int npart = 0;
+    DClock *clk = nullptr;      // device
+    DClock *clk_host = nullptr; // pinned
+    std::string err;
+    bool timing = false;
+    std::vector<PendingEv> pending;
+    std::vector<hipEvent_t> evpool;
+    double ms[ST_N] = {};
+    int64_t launches[ST_N] = {};
+};
+
+static int fail(gam_handle *h, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, x)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (x);                                                                \
+        if (e_ != hipSuccess) return fail(h, -2, "%s: %s", #x, hipGetErrorString(e_));      \
+    } while (0)
+
+static size_t gsz(const gam_handle *h) { return (size_t)h->gv.nc; }
+static size_t fsz(const gam_handle *h) { return (size_t)h->gv.nf; }
+
+// ----------------------------------------------------------------- timers
+static hipEvent_t ev_get(gam_handle *h) {
+    if (!h->evpool.empty()) { hipEvent_t e = h->evpool.back(); h->evpool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) { h->timing = false; return nullptr; }
+    return e;
+}
+
+static void collect_timers(gam_handle *h) {
+    for (auto &p : h->pending) {
+        float t = 0.f;
+        if (hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) { h->ms[p.stage] += t; h->launches[p.stage]++; }
+        h->evpool.push_back(p.a);
+        h->evpool.push_back(p.b);
+    }
+    h->pending.clear();
+}
+
+struct StageTimer {
+    gam_handle *h; int st; hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(gam_handle *h_, int st_) : h(h_), st(st_) {
+        if (h->timing) {
+            a = ev_get(h);
+            if (!a || hipEventRecord(a, h->stream) != hipSuccess) { h->timing = false; a = nullptr; }
+        }
+    }
+    ~StageTimer() {
+        if (h->timing && a) {
+            b = ev_get(h);
+            if (b && hipEventRecord(b, h->stream) == hipSuccess) h->pending.push_back({st, a, b});
+            else h->timing = false;
+        }
+    }
+};
+
+static inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// ----------------------------------------------------------------- lifetime
+extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handle **out) {
+    if (!dims || !par || !out) return -1;
+    *out = nullptr;
+    if (dims->Nip < 1 || dims->Njp < 1 || dims->Nkp < 1 || dims->nG != 4)
+        return -1;   // the stencils assume 4 ghost layers (init.F90:327)
+    if (par->precision != 0) return -1;
+    if (par->doMHD != 0 && par->doMHD != 1) return -1;
+    long Ni = dims->Nip + 2L * dims->nG, Nj = dims->Njp + 2L * dims->nG, Nk = dims->Nkp + 2L * dims->nG;
+    long nf = (Ni + 1) * (Nj + 1) * (Nk + 1);
+    if (nf * 27 > 0x7fffffffL * 8L) return -1;
+    if (nf > 0x7fffffffL) return -1;
+    gam_handle *h = new gam_handle();
+    h->dims = *dims;
+    h->par = *par;
+    GV &g = h->gv;
+    g.Ni = (int)Ni; g.Nj = (int)Nj; g.Nk = (int)Nk; g.nG = dims->nG;
+    g.Nip = dims->Nip; g.Njp = dims->Njp; g.Nkp = dims->Nkp;
+    g.nc = (int)(Ni * Nj * Nk); g.nf = (int)nf;
+    MP &m = h->mp;
+    m.gamma = par->gamma; m.CFL = par->CFL; m.pdmb = par->pdmb; m.Vd0 = par->Vd0;
+    m.cHogH = par->cHogH; m.cHogM = par->cHogM; m.Ca = par->Ca; m.dFloor = par->dFloor; m.pFloor = par->pFloor;
+    m.rmeth = par->rmeth; m.doLFMLim = par->doLFMLim; m.doBoris = par->doBoris; m.doMHD = par->doMHD;
+    m.do25D = par->do25D;
+    auto cleanup = [&](int code) { gam_destroy(h); return code; };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(-2);
+    const size_t G = gsz(h), F = fsz(h);
+    const size_t gcomp[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
+    for (int q = 0; q < G_N; q++)
+        if (hipMalloc(&h->grid[q], gcomp[q] * sizeof(double)) != hipSuccess) return cleanup(-3);
+    for (int q = 0; q < 2; q++) {
+        if (hipMalloc(&h->Gas[q], 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+        if (hipMalloc(&h->mag[q], 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
+        if (hipMalloc(&h->B[q], 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+        if (hipMemset(h->Gas[q], 0, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-2);
+        if (hipMemset(h->mag[q], 0, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-2);
+        if (hipMemset(h->B[q], 0, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-2);
+    }
+    if (hipMalloc(&h->hGas, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->hmag, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->hB, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->Vf, 9 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->E, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->Flx, 24 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    {
+        double *z[6] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx};
+        // K fluxes stay 0 in 2.5D (init.F90:500-501 zero the flux work arrays)
+        const size_t zn[6] = {5 * G, 3 * F, 3 * G, 9 * G, 3 * G, 24 * G};
+        for (int q = 0; q < 6; q++)
+            if (hipMemset(z[q], 0, zn[q] * sizeof(double)) != hipSuccess) return cleanup(-2);
+    }
+    h->npart = (int)((G + 255) / 256);
+    if (hipMalloc(&h->partial, (size_t)h->npart * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->clk, sizeof(DClock)) != hipSuccess) return cleanup(-3);
+    if (hipHostMalloc(&h->clk_host, sizeof(DClock), hipHostMallocDefault) != hipSuccess) return cleanup(-3);
+    DClock c = {};
+    c.tFin = par->tFin; c.limDT0 = par->limDT0; c.limCPR = par->limCPR;
+    c.fixedTimestep = par->fixedTimestep; c.doCPR = par->doCPR;
+    *h->clk_host = c;
+    if (hipMemcpy(h->clk, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess) return cleanup(-2);
+    // geometry defaults (init.F90:846-850,973-976,1098)
+    double *gp[4] = {h->grid[G_FACE], h->grid[G_EDGE], h->grid[G_TEB], nullptr};
+    size_t gn[3] = {3 * F, 3 * F, 12 * F};
+    for (int q = 0; q < 3; q++) hipLaunchKernelGGL(k_fill, grid1((long)gn[q]), 256, 0, h->stream, gp[q], gn[q], 1.0);
+    if (hipMemsetAsync(h->grid[G_XFC], 0, 9 * F * sizeof(double), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->grid[G_TF], 0, 27 * F * sizeof(double), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->grid[G_TE], 0, 27 * F * sizeof(double), h->stream) != hipSuccess)
+        return cleanup(-2);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(-2);
+    // device views
+    g.xyz = h->grid[G_XYZ]; g.xyzcc = h->grid[G_XYZCC]; g.xfc = h->grid[G_XFC]; g.face = h->grid[G_FACE];
+    g.Tf = h->grid[G_TF]; g.edge = h->grid[G_EDGE]; g.Te = h->grid[G_TE]; g.Teb = h->grid[G_TEB];
+    g.vol = h->grid[G_VOL]; g.di = h->grid[G_DI]; g.dj = h->grid[G_DJ]; g.dk = h->grid[G_DK];
+    *out = h;
+    return 0;
+}
+
+extern "C" int gam_destroy(gam_handle *h) {
+    if (!h) return 0;
+    // teardown: failures here cannot be acted on, the handle is gone either way
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    collect_timers(h);
+    for (auto e : h->evpool) (void)hipEventDestroy(e);
+    for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
+    for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
+    double *bufs[8] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, (double *)h->clk};
+    for (double *b : bufs) if (b) (void)hipFree(b);
+    if (h->clk_host) (void)hipHostFree(h->clk_host);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+extern "C" const char *gam_last_error(const gam_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+extern "C" int gam_sync(gam_handle *h) {
+    if (!h) return -1;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    collect_timers(h);
+    return 0;
+}
+
+extern "C" void *gam_stream(gam_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+// ----------------------------------------------------------------- copies
+static int h2d(gam_handle *h, double *dst, const double *src, size_t n) {
+    if (!src) return 0;
+    HIPCHK(h, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    return 0;
+}
+static int d2h(gam_handle *h, double *dst, const double *src, size_t n) {
+    if (!dst) return 0;
+    HIPCHK(h, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    return 0;
+}
+
+static GVw gvw(gam_handle *h) {
+    const GV &g = h->gv;
+    GVw w;
+    w.Ni = g.Ni; w.Nj = g.Nj; w.Nk = g.Nk; w.nG = g.nG; w.Nip = g.Nip; w.Njp = g.Njp; w.Nkp = g.Nkp;
+    w.nc = g.nc; w.nf = g.nf;
+    w.xyz = h->grid[G_XYZ]; w.xyzcc = h->grid[G_XYZCC]; w.xfc = h->grid[G_XFC]; w.face = h->grid[G_FACE];
+    w.Tf = h->grid[G_TF]; w.edge = h->grid[G_EDGE]; w.Te = h->grid[G_TE]; w.Teb = h->grid[G_TEB];
+    w.vol = h->grid[G_VOL]; w.di = h->grid[G_DI]; w.dj = h->grid[G_DJ]; w.dk = h->grid[G_DK];
+    return w;
+}
+
+extern "C" int gam_set_grid(gam_handle *h, const double *xyz, const double *xyzcc, const double *xfc,
+                            const double *face, const double *Tf, const double *edge, const double *Te,
+                            const double *Teb, const double *volume, const double *di, const double *dj,
+                            const double *dk) {
+    if (!h) return -1;
+    const size_t G = gsz(h), F = fsz(h);
+    const double *src[G_N] = {xyz, xyzcc, xfc, face, Tf, edge, Te, Teb, volume, di, dj, dk};
+    const size_t n[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
+    for (int q = 0; q < G_N; q++) { int rc = h2d(h, h->grid[q], src[q], n[q]); if (rc) return rc; }
+    return gam_sync(h);
+}
+
+extern "C" int gam_get_grid(gam_handle *h, double *xyz, double *xyzcc, double *xfc, double *face, double *Tf,
+                            double *edge, double *Te, double *Teb, double *volume, double *di, double *dj, double *dk) {
+    if (!h) return -1;
+    const size_t G = gsz(h), F = fsz(h);
+    double *dst[G_N] = {xyz, xyzcc, xfc, face, Tf, edge, Te, Teb, volume, di, dj, dk};
+    const size_t n[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
+    for (int q = 0; q < G_N; q++) { int rc = d2h(h, dst[q], h->grid[q], n[q]); if (rc) return rc; }
+    return gam_sync(h);
+}
+
+extern "C" int gam_set_corners(gam_handle *h, const double *xyz) {
+    // init.F90:801-1080 Corners2Grid + ebGeom on the GPU
+    if (!h || !xyz) return -1;
+    const GV &g = h->gv;
+    if (g.Nip < 8) return fail(h, -1, "gam_set_corners: Nip must be >= 8 for the 8-point stencils");
+    int rc = h2d(h, h->grid[G_XYZ], xyz, 3 * fsz(h));
+    if (rc) return rc;
+    GVw w = gvw(h);
+    for (int d = 0; d < 3; d++) {
+        long n = (long)(g.Ni + (d == 0)) * (g.Nj + (d == 1)) * (g.Nk + (d == 2));
+        hipLaunchKernelGGL(k_faces, grid1(n), 256, 0, h->stream, w, d);
+    }
+    hipLaunchKernelGGL(k_volume, grid1(g.nc), 256, 0, h->stream, w);
+    for (int d = 0; d < 3; d++) {
+        long nx = (d == 1) ? g.Ni - 1 : (d == 0 ? g.Ni : g.Ni + 1);
+        long ny = (d == 0) ? g.Nj + 1 : (d == 1 ? g.Nj : g.Nj - 1);
+        long nz = (d == 0) ? g.Nk - 1 : (d == 1 ? g.Nk + 1 : g.Nk);
+        hipLaunchKernelGGL(k_edges, grid1(nx * ny * nz), 256, 0, h->stream, w, d);
+    }
+    long ne = (long)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
+    for (int dN = 0; dN < 3; dN++) {
+        int T1 = (dN + 1) % 3, T2 = (dN + 2) % 3;
+        hipLaunchKernelGGL(k_teb, grid1(ne), 256, 0, h->stream, w, dN, T1, T2, XNQI);
+        hipLaunchKernelGGL(k_teb, grid1(ne), 256, 0, h->stream, w, dN, T2, T1, XNQJ);
+    }
+    HIPCHK(h, hipGetLastError());
+    return gam_sync(h);
+}
+
+// which: 0 State, 1 oState, 2 predictor state
+static void state_ptrs(gam_handle *h, int which, double **Gas, double **mag, double **B) {
+    if (which == 2) { *Gas = h->hGas; *mag = h->hmag; *B = h->hB; return; }
+    int q = which == 0 ? h->s : 1 - h->s;
+    *Gas = h->Gas[q]; *mag = h->mag[q]; *B = h->B[q];
+}
+
+extern "C" int gam_put_state(gam_handle *h, int which, double time, const double *Gas, const double *magFlux,
+                             const double *Bxyz) {
+    if (!h || which < 0 || which > 1) return -1;
+    double *g, *m, *b;
+    state_ptrs(h, which, &g, &m, &b);
+    int rc;
+    if ((rc = h2d(h, g, Gas, 5 * gsz(h)))) return rc;
+    if ((rc = h2d(h, m, magFlux, 3 * fsz(h)))) return rc;
+    if ((rc = h2d(h, b, Bxyz, 3 * gsz(h)))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost));
+    if (which == 0) h->clk_host->stime = time; else h->clk_host->otime = time;
+    HIPCHK(h, hipMemcpy(h->clk, h->clk_host, sizeof(DClock), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int gam_get_state(gam_handle *h, int which, double *time, double *Gas, double *magFlux, double *Bxyz) {
+    if (!h || which < 0 || which > 2) return -1;
+    double *g, *m, *b;
+    state_ptrs(h, which, &g, &m, &b);
+    int rc;
+    if ((rc = d2h(h, Gas, g, 5 * gsz(h)))) return rc;
+    if ((rc = d2h(h, magFlux, m, 3 * fsz(h)))) return rc;
+    if ((rc = d2h(h, Bxyz, b, 3 * gsz(h)))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost, h->stream));
+    if ((rc = gam_sync(h))) return rc;
+    if (time) *time = which == 0 ? h->clk_host->stime : (which == 1 ? h->clk_host->otime : h->clk_host->stime + h->clk_host->pdt);
+    return 0;
+}
+
+extern "C" int gam_set_clock(gam_handle *h, const gam_clock *c) {
+    if (!h || !c) return -1;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost));
+    DClock &d = *h->clk_host;
+    d.t = c->t; d.dt = c->dt; d.dt0 = c->dt0; d.stime = c->stateTime; d.otime = c->oStateTime;
+    d.ts = c->ts; d.status = c->status; d.dt0_after = c->dt0_after;
+    d.halt = 0;
+    HIPCHK(h, hipMemcpy(h->clk, h->clk_host, sizeof(DClock), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int gam_get_clock(gam_handle *h, gam_clock *c) {
+    if (!h || !c) return -1;
+    HIPCHK(h, hipMemcpyAsync(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost, h->stream));
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    const DClock &d = *h->clk_host;
+    c->t = d.t; c->dt = d.dt; c->dt0 = d.dt0; c->stateTime = d.stime; c->oStateTime = d.otime;
+    c->ts = d.ts; c->status = d.status; c->dt0_after = d.dt0_after; c->pad = 0;
+    return 0;
+}
+
+// ----------------------------------------------------------------- ICs
+extern "C" int gam_ic_orszag_tang(gam_handle *h, double rho0, double p0, double b0, double kscale) {
+    if (!h) return -1;
+    const GV &g = h->gv;
+    long n = (long)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
+    hipLaunchKernelGGL(k_ic_ot, grid1(n), 256, 0, h->stream, g, h->mp.gamma, rho0, p0, b0, kscale, h->Gas[h->s], h->mag[h->s]);
+    HIPCHK(h, hipGetLastError());
+    return gam_sync(h);
+}
+
+extern "C" int gam_ic_blast_wave(gam_handle *h, double x0, double y0, double z0, double D0, double P0,
+                                 double pRat, double dRat, double rC, double B0) {
+    if (!h) return -1;
+    const GV &g = h->gv;
+    GVw w = gvw(h);
+    BWp bw = {x0, y0, z0, D0, P0, pRat, dRat, rC};
+    long n = (long)g.Nip * g.Njp * g.Nkp;
+    hipLaunchKernelGGL(k_ic_bw_gas, grid1(n), 256, 0, h->stream, w, h->mp.gamma, bw, h->Gas[h->s]);
+    if (h->mp.doMHD) {
+        // prob.F90:234 bScl = B0/sqrt(2.0) with a default-real sqrt(2.0)
+        double bScl = B0 / (double)sqrtf(2.0f), bSclz = 0.0;
+        double *lA = h->E;   // scratch, zeroed like the reference's allocGridVec
+        HIPCHK(h, hipMemsetAsync(lA, 0, 3 * gsz(h) * sizeof(double), h->stream));
+        long m = (long)(g.Ni - 1) * (g.Nj - 1) * (g.Nk - 1);
+        hipLaunchKernelGGL(k_ic_bw_la, grid1(m), 256, 0, h->stream, w, bScl, bSclz, lA);
+        hipLaunchKernelGGL(k_ic_bw_flux, grid1(m), 256, 0, h->stream, w, lA, h->mag[h->s]);
+    }
+    HIPCHK(h, hipGetLastError());
+    return gam_sync(h);
+}
+
+// ----------------------------------------------------------------- step pieces
+static const GV &G(gam_handle *h) { return h->gv; }
+
+// CellDT min over the DT box of (Gas, B) into h->partial (CalcDT's loop).
+static int launch_celldt(gam_handle *h, const double *Gas, const double *B) {
+    if (!Gas || !B) return fail(h, -1, "launch_celldt: null state");
+    StageTimer t(h, ST_CELLDT);
+    if (h->mp.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<1>), dim3(h->npart), 256, 0, h->stream, G(h), h->mp, h->clk, Gas, B, h->partial);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<0>), dim3(h->npart), 256, 0, h->stream, G(h), h->mp, h->clk, Gas, B, h->partial);
+    return 0;
+}
+
+// Final reduction of h->partial + CalcDT checks (calc), after the
+// UpdateStateData time update (advance).  h->partial must hold a complete set
+// of block minima when calc != 0 (from k_update or launch_celldt).
+static void launch_dtfinal(gam_handle *h, int advance, int calc) {
+    StageTimer t(h, ST_DTFINAL);
+    hipLaunchKernelGGL(k_dtfinal, 1, 1024, 0, h->stream, h->clk, h->partial, h->npart, advance, calc);
+}
+
+template <int RM, int BO, int MH>
+static void launch_flux(gam_handle *h, int d) {
+    const GV &g = h->gv;
+    int S = h->s;
+    long nx = g.Nip + (d == 0), ny = g.Njp + (d == 1), nz = (h->mp.do25D ? 1 : g.Nkp) + (d == 2);
+    dim3 gr = grid1(nx * ny * nz);
+    StageTimer t(h, ST_FLUX_I + d);
+    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->Gas[S], h->B[S], h->hGas, h->hB, h->hmag, h->Flx);
+    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->Gas[S], h->B[S], h->hGas, h->hB, h->hmag, h->Flx);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->Gas[S], h->B[S], h->hGas, h->hB, h->hmag, h->Flx);
+}
+
+template <int RM, int BO, int MH>
+static void launch_fluxes(gam_handle *h) {
+    launch_flux<RM, BO, MH>(h, 0);
+    launch_flux<RM, BO, MH>(h, 1);
+    if (!h->mp.do25D) launch_flux<RM, BO, MH>(h, 2);
+}
+
+// AdvanceMHD + UpdateStateData time update; the dt min of the fused path is
+// left in h->partial when fuse_dt != 0.
+static int advance(gam_handle *h, int fuse_dt) {
+    const GV &g = h->gv;
+    const MP &m = h->mp;
+    const int S = h->s, O = 1 - h->s;
+    { StageTimer t(h, ST_CLOCK); hipLaunchKernelGGL(k_clock_begin, 1, 64, 0, h->stream, h->clk); }
+    { StageTimer t(h, ST_PREDICT);
+      hipLaunchKernelGGL(k_predict, grid1(g.nf), 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->Gas[O], h->mag[S], h->mag[O], h->hGas, h->hmag); }
+    { StageTimer t(h, ST_BHALF); hipLaunchKernelGGL(k_b2f, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->hmag, h->hB); }
+    if (m.doMHD) {
+        { StageTimer t(h, ST_MOM2FACE); hipLaunchKernelGGL(k_vf, grid1(g.nc), 256, 0, h->stream, g, m, h->clk, h->hGas, h->Vf); }
+        long ne = (long)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
+        StageTimer t(h, ST_EMF);
+        if (m.rmeth == 1) {
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
+        } else {
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
+        }
+    }
+    if (m.doMHD) {
+        if (m.rmeth == 1) { if (m.doBoris) launch_fluxes<1, 1, 1>(h); else launch_fluxes<1, 0, 1>(h); }
+        else { if (m.doBoris) launch_fluxes<0, 1, 1>(h); else launch_fluxes<0, 0, 1>(h); }
+    } else {
+        if (m.rmeth == 1) launch_fluxes<1, 0, 0>(h); else launch_fluxes<0, 0, 0>(h);
+    }
+    {
+        StageTimer t(h, ST_CT);
+        if (m.doMHD) hipLaunchKernelGGL(k_ct, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mag[S], h->E, h->mag[O]);
+        else HIPCHK(h, hipMemcpyAsync(h->mag[O], h->mag[S], 3 * fsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    }
+    {
+        StageTimer t(h, ST_UPDATE);
+        dim3 gr(h->npart);
+        if (m.doMHD) {
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1>), gr, 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1>), gr, 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+        } else {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0>), gr, 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+        }
+    }
+    // CopyState State->oState is the buffer swap
+    h->s = O;
+    if (!fuse_dt) launch_dtfinal(h, /*advance*/ 1, /*calc*/ 0);
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+static int bcs(gam_handle *h) {
+    const GV &g = h->gv;
+    const int S = h->s;
+    { StageTimer t(h, ST_BC); hipLaunchKernelGGL(k_bc, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, h->Gas[S], h->mag[S], nullptr); }
+    { StageTimer t(h, ST_BPOST); hipLaunchKernelGGL(k_b2f, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->mag[S], h->B[S]); }
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+extern "C" int gam_advance_mhd(gam_handle *h) {
+    if (!h) return -1;
+    int rc = advance(h, 0);
+    if (rc) return rc;
+    return gam_sync(h);
+}
+
+extern "C" int gam_calc_dt(gam_handle *h, double *dt, int *status) {
+    if (!h) return -1;
+    int rc = launch_celldt(h, h->Gas[h->s], h->B[h->s]);
+    if (rc) return rc;
+    launch_dtfinal(h, 0, 1);
+    gam_clock c;
+    rc = gam_get_clock(h, &c);
+    if (rc) return rc;
+    if (dt) *dt = c.dt;
+    if (status) *status = c.status;
+    return c.status;
+}
+
+extern "C" int gam_enforce_bcs(gam_handle *h) {
+    if (!h) return -1;
+    int rc = bcs(h);
+    if (rc) return rc;
+    return gam_sync(h);
+}
+
+extern "C" int gam_step(gam_handle *h, int nsteps) {
+    if (!h || nsteps < 0) return -1;
+    HIPCHK(h, hipMemcpyAsync(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int ts0 = h->clk_host->ts;
+    const int s0 = h->s;
+    for (int n = 0; n < nsteps; n++) {
+        int rc = advance(h, 1);
+        if (rc) return rc;
+        launch_dtfinal(h, 1, 1);   // block minima were left in h->partial by k_update
+        if ((rc = bcs(h))) return rc;
+    }
+    gam_clock c;
+    int rc = gam_get_clock(h, &c);
+    if (rc) return rc;
+    // steps after a halt launched but did nothing: undo their buffer swaps
+    int done = c.ts - ts0;
+    if (((s0 + done) & 1) != h->s) h->s = 1 - h->s;
+    return c.status;
+}
+
+extern "C" int gam_hatch(gam_handle *h) {
+    // init.F90:204-244: Bxyz from fluxes, initial BCs (Gas, Bxyz and magFlux
+    // ghosts copied; no Bxyz recompute after them), oState = State, first dt
+    // (ts = 0), oState%time = State%time - dt.
+    if (!h) return -1;
+    const GV &g = h->gv;
+    const int S = h->s, O = 1 - h->s;
+    gam_clock c0;
+    int rc = gam_get_clock(h, &c0);
+    if (rc) return rc;
+    c0.dt0 = 0.0; c0.ts = 0; c0.status = 0;
+    if ((rc = gam_set_clock(h, &c0))) return rc;
+    hipLaunchKernelGGL(k_b2f, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->mag[S], h->B[S]);
+    hipLaunchKernelGGL(k_bc, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, h->Gas[S], h->mag[S], h->B[S]);
+    HIPCHK(h, hipMemcpyAsync(h->Gas[O], h->Gas[S], 5 * gsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->mag[O], h->mag[S], 3 * fsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->B[O], h->B[S], 3 * gsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipGetLastError());
+    if ((rc = launch_celldt(h, h->Gas[S], h->B[S]))) return rc;
+    launch_dtfinal(h, 0, 1);
+    gam_clock c;
+    if ((rc = gam_get_clock(h, &c))) return rc;
+    c.oStateTime = c.stateTime - c.dt;
+    int st = c.status;
+    if ((rc = gam_set_clock(h, &c))) return rc;
+    return st;
+}
+
+extern "C" int gam_divb(gam_handle *h, double *tot) {
+    if (!h || !tot) return -1;
+    const GV &g = h->gv;
+    long n = (long)g.Nip * g.Njp * g.Nkp;
+    int nb = (int)((n + 255) / 256);
+    if (nb > h->npart) return fail(h, -1, "gam_divb: partial buffer too small");
+    hipLaunchKernelGGL(k_divb, nb, 256, 0, h->stream, g, h->mag[h->s], h->partial);
+    HIPCHK(h, hipGetLastError());
+    std::vector<double> hp(nb);
+    HIPCHK(h, hipMemcpyAsync(hp.data(), h->partial, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    double s = 0.0;
+    for (double v : hp) s += v;
+    *tot = s;
+    return 0;
+}
+
+extern "C" int gam_get_work(gam_handle *h, int which, double *out) {
+    if (!h || !out) return -1;
+    const size_t G = gsz(h);
+    int rc;
+    if (which == 0) rc = d2h(h, out, h->Flx, 24 * G);
+    else if (which == 1) rc = d2h(h, out, h->E, 3 * G);
+    else if (which == 2) rc = d2h(h, out, h->Vf, 9 * G);
+    else return fail(h, -1, "gam_get_work: which must be 0, 1 or 2");
+    if (rc) return rc;
+    return gam_sync(h);
+}
+
+// ----------------------------------------------------------------- timers API
+extern "C" int gam_set_timing(gam_handle *h, int enable) {
+    if (!h) return -1;
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    h->timing = enable != 0;
+    for (int q = 0; q < ST_N; q++) { h->ms[q] = 0; h->launches[q] = 0; }
+    return 0;
+}
+
+extern "C" int gam_get_timing(gam_handle *h, int stage, double *ms_total, int64_t *launches) {
+    if (!h || stage < 0 || stage >= ST_N) return -1;
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    if (ms_total) *ms_total = h->ms[stage];
+    if (launches) *launches = h->launches[stage];
+    return 0;
+}
+
+extern "C" const char *gam_stage_name(int stage) { return (stage >= 0 && stage < ST_N) ? kStageNames[stage] : ""; }
+extern "C" int gam_num_stages(void) { return ST_N; }
+
+extern "C" int gam_stage_bytes(gam_handle *h, int stage, double *bytes) {
+    // Unique bytes each stage must move: every input array read once over the
+    // stage's domain, every output written once (FP64).
+    if (!h || !bytes || stage < 0 || stage >= ST_N) return -1;
+    const GV &g = h->gv;
+    const double G = g.nc, F = g.nf;
+    const double nI = (double)(g.Nip + 1) * g.Njp * g.Nkp, nJ = (double)g.Nip * (g.Njp + 1) * g.Nkp,
+                 nK = (double)g.Nip * g.Njp * (g.Nkp + 1), nA = (double)g.Nip * g.Njp * g.Nkp;
+    const double nE = (double)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
+    double b = 0;
+    switch (stage) {
+    case ST_PREDICT: b = F * (3 + 3 + 3) + G * (5 + 5 + 5); break;                 // mag n,o -> hf ; Gas n,o -> hf
+    case ST_BHALF: case ST_BPOST: b = F * (3 + 9) + G * (3 + 1 + 3); break;      // mag, xfc ; xyzcc, vol -> B
+    case ST_MOM2FACE: b = G * (4 + 1 + 9); break;                                 // rho,M, vol -> Vf (9)
+    case ST_EMF: b = G * (9 + 1) + nE * (3 + 3 + 3 + 6 * 3 + 4 * 3 + 3 + 3); break; // Vf, rho ; face, mag n/h, Te, Teb, edge -> E
+    case ST_FLUX_I: b = (G * (5 + 5 + 3 + 3 + 1)) + nI * (9 + 1 + 1 + 8); break;
+    case ST_FLUX_J: b = (G * (5 + 5 + 3 + 3 + 1)) + nJ * (9 + 1 + 1 + 8); break;
+    case ST_FLUX_K: b = (G * (5 + 5 + 3 + 3 + 1)) + nK * (9 + 1 + 1 + 8); break;
+    case ST_CT: b = F * (3 + 3) + G * 3; break;
+    case ST_UPDATE: b = G * (5 + 3 + 3 + 1 + 3 + 3 + 5 + 3 + 3) + F * (3 + 9) + nA * 0; break;
+    case ST_BC: b = 2.0 * (G - nA) * 5 + 2.0 * (F - nE) * 3; break;
+    case ST_CELLDT: b = G * (5 + 3 + 3); break;
+    default: b = 0;
+    }
+    *bytes = 8.0 * b;
+    return 0;
+}

@@ -1,0 +1,770 @@
+// gam_step.hip -- FP64 GAMERA MHD step kernels for gfx950.
+//
+// One step (reference gamapp.F90:73-116 stepGamera) is, with S = State,
+// O = oState, H = predictor (half-step) state:
+//   k_clock_begin     time trap + pdt/odt            mhdgroup.F90:25-31,459
+//   k_predict   (F)   AB2 magFlux, PredictCell       mhdgroup.F90:445-616
+//   k_b2f       (G)   H.Bxyz = bFlux2Fld(H.magFlux)  gridutils.F90:281-301,648-680
+//   k_vf        (G)   Mom2Face for the 3 sweeps      fields.F90:110-179,235-285
+//   k_emf       (E)   edge EMFs, all 3 directions    fields.F90:186-226,288-517
+//   k_flux<d>   (I)   interface fluxes, 3 launches   stress.F90:84-128,204-599
+//   k_ct        (F)   O.magFlux = E2Flux(S.magFlux)  fields.F90:18-51
+//   k_update    (G)   Flux2Deltas + Reynolds + Bxyz + Maxwell/Boris + CellDT min
+//                                                    stress.F90:152-194, mhdgroup.F90:79-442,
+//                                                    gridutils.F90:49-106
+//   k_dtfinal   (1)   time update + CalcDT checks    gamapp.F90:112-114, step.F90:58-168
+//   k_bc        (F)   periodic ghost fill            bcs.F90:279-450
+//   k_b2f       (G)   O.Bxyz = bFlux2Fld(O.magFlux)  gamapp.F90:89
+// after which the host swaps the S and O buffer sets (CopyState becomes a
+// pointer swap, mhdgroup.F90:202-257).
+#include "gam_common.h"
+
+// ----------------------------------------------------------------- helpers
+__device__ __forceinline__ void decode_f(const GV &g, int idx, int &i, int &j, int &k) {
+    int nx = g.Ni + 1, ny = g.Nj + 1;
+    i = idx % nx; int r = idx / nx; j = r % ny; k = r / ny;
+}
+__device__ __forceinline__ void decode_c(const GV &g, int idx, int &i, int &j, int &k) {
+    i = idx % g.Ni; int r = idx / g.Ni; j = r % g.Nj; k = r / g.Nj;
+}
+
+// mhdgroup.F90:710-718 ExtrapAB2
+__device__ __forceinline__ double ab2(double dt, double Q, double oQ, double odt) {
+    double dQ = (Q - oQ) / odt;
+    return Q + dt * dQ;
+}
+
+// gridutils.F90:648-680 gamCellBxyz
+__device__ __forceinline__ void cell_bxyz(const GV &g, const double *mag, int i, int j, int k, double *B) {
+    const int nf = g.nf;
+    int c = ci(g, i, j, k), f0 = fi(g, i, j, k);
+    int fi1 = f0 + 1, fj1 = f0 + (g.Ni + 1), fk1 = f0 + (g.Ni + 1) * (g.Nj + 1);
+    double dV = g.vol[c];
+    double fI = mag[f0], fJ = mag[nf + f0], fK = mag[2 * nf + f0];
+    double fIp = mag[fi1], fJp = mag[nf + fj1], fKp = mag[2 * (size_t)nf + fk1];
+    double Div = fIp - fI + fJp - fJ + fKp - fK;
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        const double *xf = g.xfc;
+        double xfI = xf[(size_t)nf * (x + 0) + f0], xfIp = xf[(size_t)nf * (x + 0) + fi1];
+        double xfJ = xf[(size_t)nf * (x + 3) + f0], xfJp = xf[(size_t)nf * (x + 3) + fj1];
+        double xfK = xf[(size_t)nf * (x + 6) + f0], xfKp = xf[(size_t)nf * (x + 6) + fk1];
+        double b = (fIp * xfIp + fJp * xfJp + fKp * xfKp - fI * xfI - fJ * xfJ - fK * xfK) / dV;
+        B[x] = b - Div * g.xyzcc[(size_t)g.nc * x + c] / dV;
+    }
+}
+
+// ----------------------------------------------------------------- clock
+__global__ void k_clock_begin(DClock *clk) {
+    // mhdgroup.F90:25-28 trap and :31,459 pdt/odt
+    if (threadIdx.x == 0 && blockIdx.x == 0 && !clk->halt) {
+        if (fabs(clk->otime - clk->stime) < GAM_TINY) clk->otime = clk->stime - clk->dt;
+        clk->odt = clk->stime - clk->otime;
+        clk->pdt = 0.5 * clk->dt;
+    }
+}
+
+// ----------------------------------------------------------------- predictor
+__global__ void __launch_bounds__(256) k_predict(GV g, MP m, const DClock *__restrict__ clk,
+                                                 const double *__restrict__ Gas, const double *__restrict__ oGas,
+                                                 const double *__restrict__ mag, const double *__restrict__ omag,
+                                                 double *__restrict__ hGas, double *__restrict__ hmag) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nf || clk->halt) return;
+    const double pdt = clk->pdt, odt = clk->odt;
+    const int nf = g.nf, nc = g.nc;
+    if (m.doMHD) {
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            hmag[(size_t)nf * d + idx] = ab2(pdt, mag[(size_t)nf * d + idx], omag[(size_t)nf * d + idx], odt);
+    }
+    int i, j, k;
+    decode_f(g, idx, i, j, k);
+    if (i >= g.Ni || j >= g.Nj || k >= g.Nk) return;
+    int c = ci(g, i, j, k);
+    double Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) { Q[v] = Gas[(size_t)nc * v + c]; oQ[v] = oGas[(size_t)nc * v + c]; }
+    cell_c2p(m, oQ, oW);
+    cell_c2p(m, Q, W);
+#pragma unroll
+    for (int v = 0; v < 5; v++) pW[v] = ab2(pdt, W[v], oW[v], odt);
+    pW[0] = fmax(pW[0], m.dFloor);
+    pW[4] = fmax(pW[4], m.pFloor);
+    cell_p2c(m, pW, pQ);
+#pragma unroll
+    for (int v = 0; v < 5; v++) hGas[(size_t)nc * v + c] = pQ[v];
+}
+
+__global__ void __launch_bounds__(256) k_b2f(GV g, const DClock *__restrict__ clk, const double *__restrict__ mag,
+                                             double *__restrict__ B) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc || clk->halt) return;
+    int i, j, k;
+    decode_c(g, idx, i, j, k);
+    double b[3];
+    cell_bxyz(g, mag, i, j, k, b);
+#pragma unroll
+    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = b[x];
+}
+
+// ----------------------------------------------------------------- E field
+// fields.F90:235-285 Mom2Face at (i,j,k) along dT, c = cell index
+__device__ __forceinline__ void mom2face(const GV &g, const MP &m, const double *__restrict__ Gas, int c, int dT,
+                                         double *Vf) {
+    const int st = cstride(g, dT), nc = g.nc;
+    double VolB[8], Vx[8], Vy[8], Vz[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        int cc = c + (n - 4) * st;
+        double vol = g.vol[cc];
+        double D = fmax(Gas[cc], m.dFloor);
+        VolB[n] = vol;
+        Vx[n] = vol * Gas[(size_t)nc * 1 + cc] / D;
+        Vy[n] = vol * Gas[(size_t)nc * 2 + cc] / D;
+        Vz[n] = vol * Gas[(size_t)nc * 3 + cc] / D;
+    }
+    double dV = fmax(c8(VolB), GAM_TINY);
+    Vf[0] = c8(Vx) / dV;
+    Vf[1] = c8(Vy) / dV;
+    Vf[2] = c8(Vz) / dV;
+}
+
+// Vf arrays: VfA[dT1][xyz] stored as slab (3*dT1 + xyz)
+__global__ void __launch_bounds__(256) k_vf(GV g, MP m, const DClock *__restrict__ clk, const double *__restrict__ hGas,
+                                            double *__restrict__ Vf) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc || clk->halt) return;
+    int i, j, k;
+    decode_c(g, idx, i, j, k);
+    const int a0 = g.nG;
+    const size_t nc = g.nc;
+    double v[3];
+    // dT1 = I (used by K edges): i in is..ie+1   (fields.F90:161-175)
+    if (i >= a0 && i <= a0 + g.Nip) {
+        mom2face(g, m, hGas, idx, 0, v);
+        Vf[nc * 0 + idx] = v[0]; Vf[nc * 1 + idx] = v[1]; Vf[nc * 2 + idx] = v[2];
+    }
+    // dT1 = J (used by I edges): j in js..je+1   (fields.F90:119-133)
+    if (j >= a0 && j <= a0 + g.Njp) {
+        mom2face(g, m, hGas, idx, 1, v);
+        Vf[nc * 3 + idx] = v[0]; Vf[nc * 4 + idx] = v[1]; Vf[nc * 5 + idx] = v[2];
+    }
+    // dT1 = K (used by J edges): k in ks..ke+1   (fields.F90:140-154)
+    if (k >= a0 && k <= a0 + g.Nkp) {
+        mom2face(g, m, hGas, idx, 2, v);
+        Vf[nc * 6 + idx] = v[0]; Vf[nc * 7 + idx] = v[1]; Vf[nc * 8 + idx] = v[2];
+    }
+}
+
+// fields.F90:288-362 EdgeLRs: dT1 faces reconstructed along dT2 at face point f
+template <int RM>
+__device__ __forceinline__ void edge_lrs(const GV &g, const MP &m, const double *__restrict__ magN,
+                                         const double *__restrict__ magH, int f, int dT1, int dT2,
+                                         double &bL, double &bR, double &fA) {
+    const int st = fstride(g, dT2);
+    const size_t off = (size_t)g.nf * dT1;
+    double AreaB[8], MagB[8], nAreaB[4], nMagB[4];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        int ff = f + (n - 4) * st;
+        AreaB[n] = g.face[off + ff];
+        MagB[n] = magH[off + ff];
+    }
+    const double *limMag = m.doLFMLim ? magN : magH;
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        int ff = f + (n - 2) * st;
+        nAreaB[n] = g.face[off + ff];
+        nMagB[n] = limMag[off + ff];
+    }
+#pragma unroll
+    for (int n = 0; n < 8; n++) MagB[n] = MagB[n] / fmax(AreaB[n], GAM_TINY);
+#pragma unroll
+    for (int n = 0; n < 4; n++) nMagB[n] = nMagB[n] / fmax(nAreaB[n], GAM_TINY);
+    double Vi = c8(AreaB);
+    double l, r;
+    get_lrs<RM>(m.pdmb, AreaB, nMagB, MagB, Vi, l, r);
+    bL = l * Vi; bR = r * Vi; fA = Vi;
+}
+
+// One edge EMF (fields.F90:196-221) for edge direction eD at corner (i,j,k)
+template <int RM, int BORIS>
+__device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, const double *__restrict__ Vf,
+                                           const double *__restrict__ hGas, const double *__restrict__ magN,
+                                           const double *__restrict__ magH, int eD, int i, int j, int k) {
+    const int dT1 = (eD + 1) % 3, dT2 = (eD + 2) % 3;
+    const int f = fi(g, i, j, k), c = ci(g, i, j, k);
+    const size_t nf = g.nf, nc = g.nc;
+    // GetCornerV fields.F90:457-517 (Vf of the dT1 faces interpolated along dT2)
+    const int fst = fstride(g, dT2), cst = cstride(g, dT2);
+    const double *VfA = Vf + nc * 3 * dT1;
+    double AreaB[8], Vx[8], Vy[8], Vz[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) {
+        AreaB[n] = g.face[nf * dT1 + f + (n - 4) * fst];
+        int cc = c + (n - 4) * cst;
+        Vx[n] = AreaB[n] * VfA[cc];
+        Vy[n] = AreaB[n] * VfA[nc + cc];
+        Vz[n] = AreaB[n] * VfA[2 * nc + cc];
+    }
+    double dA = fmax(c8(AreaB), GAM_TINY);
+    double Vcx = c8(Vx) / dA, Vcy = c8(Vy) / dA, Vcz = c8(Vz) / dA;
+    const double *Te = g.Te + nf * 9 * eD;
+    double v1 = Vcx * Te[nf * TAN1X + f] + Vcy * Te[nf * TAN1Y + f] + Vcz * Te[nf * TAN1Z + f];
+    double v2 = Vcx * Te[nf * TAN2X + f] + Vcy * Te[nf * TAN2Y + f] + Vcz * Te[nf * TAN2Z + f];
+    double vDiff = sqrt(sq(v1) + sq(v2));
+    // GetCornerD fields.F90:427-454
+    // normal plane basis (metric.F90:151-167): I -> (j,k), J -> (i,k), K -> (i,j)
+    const int s1 = cstride(g, eD == 0 ? 1 : 0), s2 = cstride(g, eD == 2 ? 1 : 2);
+    double Dc = 0.25 * (hGas[c] + hGas[c - s1] + hGas[c - s2] + hGas[c - s1 - s2]);
+    // GetCornerB fields.F90:365-424
+    double bT1L, bT1R, fA1, bT2L, bT2R, fA2;
+    edge_lrs<RM>(g, m, magN, magH, f, dT1, dT2, bT1L, bT1R, fA1);
+    edge_lrs<RM>(g, m, magN, magH, f, dT2, dT1, bT2L, bT2R, fA2);
+    const double *Teb = g.Teb + nf * 4 * eD;
+    double xni = Teb[nf * XNQI + f], yni = Teb[nf * YNQI + f];
+    double xnj = Teb[nf * XNQJ + f], ynj = Teb[nf * YNQJ + f];
+    double detT = 1.0 / (xni * ynj - xnj * yni);
+    double bT1 = 0.5 * (bT1L + bT1R) / fA1;
+    double bT2 = 0.5 * (bT2L + bT2R) / fA2;
+    double db1 = (bT1R - bT1L) / fA1;
+    double db2 = (bT2R - bT2L) / fA2;
+    double Jd = db2 - db1;
+    double b1 = detT * (ynj * bT1 - yni * bT2);
+    double b2 = detT * (-xnj * bT1 + xni * bT2);
+    double vA = sqrt(sq(b1) + sq(b2)) / sqrt(Dc);
+    if (BORIS) vA = m.Ca * vA / sqrt(sq(m.Ca) + sq(vA));
+    vDiff = vDiff + vA;
+    double ed = g.edge[nf * eD + f];
+    vDiff = fmin(vDiff, m.CFL * ed / dt);
+    double e = -(v1 * b2 - v2 * b1) + m.Vd0 * vDiff * Jd;
+    return e * ed;
+}
+
+// Edge box [is,ie+1]x[js,je+1]x[ks,ke+1]; each thread does the (up to) three
+// edges at its lower corner.  E slabs are written only on active edges; k_ct
+// treats the rest as zero (CalcElecField zeroes E, fields.F90:89).
+template <int RM, int BORIS>
+__global__ void __launch_bounds__(256) k_emf(GV g, MP m, const DClock *__restrict__ clk, const double *__restrict__ Vf,
+                                             const double *__restrict__ hGas, const double *__restrict__ magN,
+                                             const double *__restrict__ magH, double *__restrict__ E) {
+    const int nx = g.Nip + 1, ny = g.Njp + 1, nz = g.Nkp + 1;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * ny * nz || clk->halt) return;
+    int a = idx % nx, r = idx / nx, b = r % ny, cz = r / ny;
+    const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
+    const double dt = clk->dt;
+    const int c = ci(g, i, j, k);
+    const size_t nc = g.nc;
+    if (a < g.Nip) E[c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, magN, magH, 0, i, j, k);
+    if (b < g.Njp) E[nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, magN, magH, 1, i, j, k);
+    if (cz < g.Nkp) E[2 * nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, magN, magH, 2, i, j, k);
+}
+
+// ----------------------------------------------------------------- fluxes
+// stress.F90:513-599 MagKinFlux (no background field), q = -1/+1
+template <int BORIS>
+__device__ __forceinline__ void magkin(const MP &m, double sg, const double *P5, const double *B, double Bn,
+                                       const double *T, double *mF, double &bbD, double &VaD) {
+    double D = P5[0], P = P5[4];
+    double Bx = B[0], By = B[1], Bz = B[2];
+    double dPb = 0.5 * (sq(Bx) + sq(By) + sq(Bz));
+    double Va2 = 2 * dPb / D;
+    bbD = bbD + 0.5 * (sq(Bx) + sq(By) + sq(Bz));
+    double Vac2 = BORIS ? Va2 / (1 + (Va2 / sq(m.Ca))) : Va2;
+    double Nx = T[NORMX], Ny = T[NORMY], Nz = T[NORMZ];
+    double Vn = Nx * P5[1] + Ny * P5[2] + Nz * P5[3];
+    double lambda = 1 / (2 * P / D + Vac2);
+    double Vn0 = 0.5 * erfc(sg * sqrt(lambda) * Vn);
+    VaD = VaD + 0.5 * sqrt(Vac2);
+    mF[0] = mF[0] + Vn0 * (dPb * Nx - Bx * Bn);
+    mF[1] = mF[1] + Vn0 * (dPb * Ny - By * Bn);
+    mF[2] = mF[2] + Vn0 * (dPb * Nz - Bz * Bn);
+}
+
+// stress.F90:432-511 GasKinFlux
+__device__ __forceinline__ void gaskin(const MP &m, double sg, const double *P5, const double *T, double *gF, double *dW) {
+    double D = P5[0], P = P5[4], Vx = P5[1], Vy = P5[2], Vz = P5[3];
+    double E = 0.5 * D * (sq(Vx) + sq(Vy) + sq(Vz)) + P / (m.gamma - 1);
+    double lambda = D / (2 * P);
+    double Vn = T[NORMX] * Vx + T[NORMY] * Vy + T[NORMZ] * Vz;
+    double Vt1 = T[TAN1X] * Vx + T[TAN1Y] * Vy + T[TAN1Z] * Vz;
+    double Vt2 = T[TAN2X] * Vx + T[TAN2Y] * Vy + T[TAN2Z] * Vz;
+    double Vn0 = 0.5 * erfc(sg * sqrt(lambda) * Vn);
+    double Vn1 = Vn * Vn0 - 0.5 * sg * exp(-lambda * Vn * Vn) / sqrt(GAM_PI * lambda);
+    gF[0] = gF[0] + D * Vn1;
+    gF[4] = gF[4] + (E + 0.5 * P) * Vn1 + 0.5 * P * Vn0 * Vn;
+    double fMn = D * Vn * Vn1 + P * Vn0;
+    double fMt1 = D * Vt1 * Vn1;
+    double fMt2 = D * Vt2 * Vn1;
+    gF[1] = gF[1] + (T[NORMX] * fMn + T[TAN1X] * fMt1 + T[TAN2X] * fMt2);
+    gF[2] = gF[2] + (T[NORMY] * fMn + T[TAN1Y] * fMt1 + T[TAN2Y] * fMt2);
+    gF[3] = gF[3] + (T[NORMZ] * fMn + T[TAN1Z] * fMt1 + T[TAN2Z] * fMt2);
+    dW[0] = dW[0] + sg * D;
+    dW[1] = dW[1] + sg * D * Vx;
+    dW[2] = dW[2] + sg * D * Vy;
+    dW[3] = dW[3] + sg * D * Vz;
+    dW[4] = dW[4] + sg * E;
+}
+
+// Interface fluxes in direction DN (stress.F90:204-429) for the interface box
+// DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
+// Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
+template <int DN, int RM, int BORIS, int MHD>
+__global__ void __launch_bounds__(256) k_flux(GV g, MP m, const DClock *__restrict__ clk,
+                                              const double *__restrict__ nGas, const double *__restrict__ nB,
+                                              const double *__restrict__ hGas, const double *__restrict__ hB,
+                                              const double *__restrict__ hmag, double *__restrict__ Flx) {
+    const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * ny * nz || clk->halt) return;
+    int a = idx % nx, r = idx / nx, b = r % ny, cz = r / ny;
+    const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
+    const size_t nc = g.nc, nf = g.nf;
+    const int c = ci(g, i, j, k), f = fi(g, i, j, k);
+    const int st = cstride(g, DN);
+
+    double VolB[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) VolB[n] = g.vol[c + (n - 4) * st];
+    const double Vi = c8(VolB);
+    double T[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) T[t] = g.Tf[nf * (t + 9 * DN) + f];
+    const double faB = g.face[nf * DN + f];
+
+    double PL[5], PR[5];
+    double MagL[3] = {0, 0, 0}, MagR[3] = {0, 0, 0}, Bn = 0.0;
+    if (MHD) {
+        const double *limB = m.doLFMLim ? nB : hB;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double Q[8], nQ[4];
+#pragma unroll
+            for (int n = 0; n < 8; n++) Q[n] = hB[nc * d + c + (n - 4) * st];
+#pragma unroll
+            for (int n = 0; n < 4; n++) nQ[n] = limB[nc * d + c + (n - 2) * st];
+            get_lrs<RM>(m.pdmb, VolB, nQ, Q, Vi, MagL[d], MagR[d]);
+        }
+        Bn = hmag[nf * DN + f] / faB;
+    }
+    {
+        const double *limG = m.doLFMLim ? nGas : hGas;
+        double Prim[8][5], nPrim[4][5];
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            double cv[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) cv[v] = hGas[nc * v + c + (n - 4) * st];
+            block_c2p(m, cv, Prim[n]);
+        }
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            double cv[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) cv[v] = limG[nc * v + c + (n - 2) * st];
+            block_c2p(m, cv, nPrim[n]);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            double Q[8], nQ[4];
+#pragma unroll
+            for (int n = 0; n < 8; n++) Q[n] = Prim[n][v];
+#pragma unroll
+            for (int n = 0; n < 4; n++) nQ[n] = nPrim[n][v];
+            get_lrs<RM>(m.pdmb, VolB, nQ, Q, Vi, PL[v], PR[v]);
+        }
+    }
+    double VaD = 0.0;
+    if (MHD) {
+        double mF[3] = {0, 0, 0}, bbD = 0.0;
+        magkin<BORIS>(m, -1.0, PL, MagL, Bn, T, mF, bbD, VaD);
+        magkin<BORIS>(m, 1.0, PR, MagR, Bn, T, mF, bbD, VaD);
+        if (BORIS) {   // stress.F90:335-351
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                double deeV = PR[1 + d] - PL[1 + d];
+                double deeP = bbD * deeV / sq(m.Ca);
+                mF[d] = mF[d] - m.cHogM * VaD * deeP;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) Flx[nc * (8 * DN + 5 + d) + c] = faB * mF[d];
+    }
+    double gF[5] = {0, 0, 0, 0, 0}, dW[5] = {0, 0, 0, 0, 0};
+    gaskin(m, -1.0, PL, T, gF, dW);
+    gaskin(m, 1.0, PR, T, gF, dW);
+#pragma unroll
+    for (int v = 0; v < 5; v++) gF[v] = gF[v] - 1.0 * m.cHogH * VaD * dW[v];
+    if (BORIS) {   // stress.F90:399-417
+        double vL = sqrt(sq(PL[1]) + sq(PL[2]) + sq(PL[3]));
+        double vR = sqrt(sq(PR[1]) + sq(PR[2]) + sq(PR[3]));
+        double Vn = sqrt(sq(vL) + sq(vR)) / (double)sqrtf(2.0f);
+        if (Vn >= 1.5 * m.Ca) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                double dCon = hGas[nc * v + c] - hGas[nc * v + c - st];
+                gF[v] = gF[v] - m.Ca * dCon;
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) Flx[nc * (8 * DN + v) + c] = faB * gF[v];
+}
+
+// ----------------------------------------------------------------- CT update
+// fields.F90:18-51 E2Flux into the new buffer: faces in [is,ie+1]^3 box get the
+// curl update, all others are copied.  E outside the active edge ranges is 0.
+__device__ __forceinline__ double Eval(const GV &g, const double *E, int d, int i, int j, int k) {
+    const int a0 = g.nG;
+    int ai = i - a0, aj = j - a0, ak = k - a0;
+    int mi = g.Nip - (d == 0), mj = g.Njp - (d == 1), mk = g.Nkp - (d == 2);
+    if (ai < 0 || aj < 0 || ak < 0 || ai > mi || aj > mj || ak > mk) return 0.0;
+    return E[(size_t)g.nc * d + ci(g, i, j, k)];
+}
+
+__global__ void __launch_bounds__(256) k_ct(GV g, const DClock *__restrict__ clk, const double *__restrict__ magN,
+                                            const double *__restrict__ E, double *__restrict__ magO) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nf || clk->halt) return;
+    const size_t nf = g.nf;
+    int i, j, k;
+    decode_f(g, idx, i, j, k);
+    const int a0 = g.nG;
+    double m0 = magN[idx], m1 = magN[nf + idx], m2 = magN[2 * nf + idx];
+    if (i >= a0 && i <= a0 + g.Nip && j >= a0 && j <= a0 + g.Njp && k >= a0 && k <= a0 + g.Nkp) {
+        const double dt = clk->dt;
+        double Ei = Eval(g, E, 0, i, j, k), Ej = Eval(g, E, 1, i, j, k), Ek = Eval(g, E, 2, i, j, k);
+        double dfi = Eval(g, E, 2, i, j + 1, k) - Ek - Eval(g, E, 1, i, j, k + 1) + Ej;
+        double dfj = Eval(g, E, 0, i, j, k + 1) - Ei - Eval(g, E, 2, i + 1, j, k) + Ek;
+        double dfk = Eval(g, E, 1, i + 1, j, k) - Ej - Eval(g, E, 0, i, j + 1, k) + Ei;
+        m0 = m0 - dt * dfi;
+        m1 = m1 - dt * dfj;
+        m2 = m2 - dt * dfk;
+    }
+    magO[idx] = m0; magO[nf + idx] = m1; magO[2 * nf + idx] = m2;
+}
+
+// ----------------------------------------------------------------- update
+// mhdgroup.F90:300-330 CellMaxwell
+__device__ __forceinline__ void cell_maxwell(const MP &m, double *U, const double *dGM, double dt) {
+    double D = fmax(U[0], m.dFloor);
+    double Mx = U[1], My = U[2], Mz = U[3], TotE = U[4];
+    double KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
+    double P = fmax((m.gamma - 1) * (TotE - KinE), m.pFloor);
+    Mx = Mx + dt * dGM[0]; My = My + dt * dGM[1]; Mz = Mz + dt * dGM[2];
+    KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
+    U[0] = D; U[1] = Mx; U[2] = My; U[3] = Mz;
+    U[4] = KinE + P / (m.gamma - 1);
+}
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ void normvec(const double *a, double *o) {   // math.F90:332-343
+    double n = sqrt(dot3(a, a));
+    if (n <= GAM_TINY) { o[0] = o[1] = o[2] = 0.0; }
+    else { o[0] = a[0] / n; o[1] = a[1] / n; o[2] = a[2] / n; }
+}
+
+// mhdgroup.F90:337-389 CellBoris (bulk fluid)
+__device__ __forceinline__ void cell_boris(const MP &m, double *U2, const double *U0, const double *B2, const double *B0,
+                                           const double *dGH, const double *dGM, double Ca, double dt) {
+    double W2[5];
+    cell_c2p(m, U2, W2);
+    double D2 = fmax(W2[0], m.dFloor), P2 = fmax(W2[4], m.pFloor);
+    double D0 = fmax(U0[0], m.dFloor);
+    double V0[3] = {U0[1] / D0, U0[2] / D0, U0[3] / D0};
+    double B1[3] = {0.5 * (B2[0] + B0[0]), 0.5 * (B2[1] + B0[1]), 0.5 * (B2[2] + B0[2])};
+    double alpha = dot3(B1, B1) / (D2 * Ca * Ca);
+    double dRho = dt * dGH[0];
+    double dPGas[3] = {dt * dGH[1], dt * dGH[2], dt * dGH[3]};
+    double dPMag[3] = {dt * dGM[0], dt * dGM[1], dt * dGM[2]};
+    double pCon = 1.0 / (1.0 + alpha);
+    double bh1[3], bh2[3];
+    normvec(B1, bh1);
+    normvec(B2, bh2);
+    double dMom[3];
+    if (alpha > GAM_TINY) {
+        double dp = dot3(dPGas, bh1);
+        double sum[3] = {dPGas[0] + dPMag[0], dPGas[1] + dPMag[1], dPGas[2] + dPMag[2]};
+        double ds = dot3(sum, bh1), dv = dot3(V0, bh1);
+        double s = alpha * pCon * dRho;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double a = dp * bh1[d];
+            double b = sum[d] - ds * bh1[d];
+            double cc = V0[d] - dv * bh1[d];
+            dMom[d] = a + pCon * b + s * cc;
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) dMom[d] = dPGas[d] + dPMag[d];
+    }
+    double W[5];
+    W[0] = D2;
+#pragma unroll
+    for (int d = 0; d < 3; d++) W[1 + d] = (D0 * V0[d] + dMom[d]) / D2;
+    W[4] = P2;
+    cell_p2c(m, W, U2);
+}
+
+// gridutils.F90:49-106 CellDT (single fluid, no resistivity)
+template <int BORIS>
+__device__ __forceinline__ double cell_dt(const GV &g, const MP &m, const double *U, const double *B, int c) {
+    double rho = U[0];
+    double Vx = U[1] / rho, Vy = U[2] / rho, Vz = U[3] / rho;
+    double MagV = sqrt(sq(Vx) + sq(Vy) + sq(Vz));
+    double ke = 0.5 * rho * sq(MagV);
+    double e = U[4] - ke;
+    double P = (m.gamma - 1) * e;
+    double Cs = sqrt(m.gamma * P / rho);
+    double Valf = 0.0;
+    if (m.doMHD) {
+        double MagB = sqrt(sq(B[0]) + sq(B[1]) + sq(B[2]));
+        Valf = MagB / sqrt(rho);
+        if (BORIS) Valf = m.Ca * Valf / sqrt(m.Ca * m.Ca + Valf * Valf);
+    }
+    double vCFL = MagV + sqrt(sq(Cs) + sq(Valf)) + 0.0;
+    double dl = fmin(fmin(g.di[c], g.dj[c]), g.dk[c]);
+    return m.CFL * dl / vCFL;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// block min over 256 threads -> partial[blockIdx.x]
+__device__ __forceinline__ void block_min_store(double v, double *partial) {
+#ifdef GAM_EMU
+    // the CPU emulation runs the threads of a block in order 0..blockDim-1
+    partial[blockIdx.x] = threadIdx.x == 0 ? v : fmin(partial[blockIdx.x], v);
+    return;
+#endif
+    __shared__ double red[4];
+    v = wave_min(v);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = fmin(r, red[q]);
+        partial[blockIdx.x] = r;
+    }
+}
+
+__device__ __forceinline__ bool in_dt_box(const GV &g, int i, int j, int k) {
+    // SetDomain init.F90:519-538: active +-1 layer (single block owns all BCs)
+    const int a0 = g.nG - 1;
+    return i >= a0 && i <= g.nG + g.Nip && j >= a0 && j <= g.nG + g.Njp && k >= a0 && k <= g.nG + g.Nkp;
+}
+
+// Flux2Deltas + applyUpdate (Reynolds, Bxyz of the new fluxes, Maxwell/Boris)
+// + CellDT min.  Writes the new Gas/Bxyz into the O buffers; ghosts copy Gas n.
+template <int BORIS, int MHD>
+__global__ void __launch_bounds__(256) k_update(GV g, MP m, const DClock *__restrict__ clk,
+                                                const double *__restrict__ nGas, const double *__restrict__ nB,
+                                                const double *__restrict__ Flx, const double *__restrict__ magNew,
+                                                double *__restrict__ oGas, double *__restrict__ oB,
+                                                double *__restrict__ partial) {
+    if (clk->halt) return;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double dtl = GAM_HUGE;
+    if (idx < g.nc) {
+        int i, j, k;
+        decode_c(g, idx, i, j, k);
+        const size_t nc = g.nc;
+        double U[5], Bn[3] = {0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < 5; v++) U[v] = nGas[nc * v + idx];
+        cell_bxyz(g, magNew, i, j, k, Bn);
+        const int a0 = g.nG;
+        bool active = i >= a0 && i < a0 + g.Nip && j >= a0 && j < a0 + g.Njp && k >= a0 && k < a0 + g.Nkp;
+        if (active) {
+            const double dt = clk->dt;
+            const int ci1 = idx + 1, cj1 = idx + g.Ni, ck1 = idx + g.Ni * g.Nj;
+            const double dV = g.vol[idx];
+            double dGH[5], dGM[3] = {0, 0, 0};
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                dGH[v] = (Flx[nc * (0 + v) + idx] - Flx[nc * (0 + v) + ci1]
+                        + Flx[nc * (8 + v) + idx] - Flx[nc * (8 + v) + cj1]
+                        + Flx[nc * (16 + v) + idx] - Flx[nc * (16 + v) + ck1]) / dV;
+            }
+            if (MHD) {
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    dGM[v] = (Flx[nc * (5 + v) + idx] - Flx[nc * (5 + v) + ci1]
+                            + Flx[nc * (13 + v) + idx] - Flx[nc * (13 + v) + cj1]
+                            + Flx[nc * (21 + v) + idx] - Flx[nc * (21 + v) + ck1]) / dV;
+                }
+            }
+            double U0[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) { U0[v] = U[v]; U[v] = U[v] + dt * dGH[v]; }   // CellReynolds, Model%dt
+            if (MHD) {
+                if (BORIS) {
+                    double oBv[3] = {nB[idx], nB[nc + idx], nB[2 * nc + idx]};
+                    cell_boris(m, U, U0, Bn, oBv, dGH, dGM, m.Ca, dt);
+                } else {
+                    cell_maxwell(m, U, dGM, dt);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) oGas[nc * v + idx] = U[v];
+#pragma unroll
+        for (int x = 0; x < 3; x++) oB[nc * x + idx] = Bn[x];
+        if (in_dt_box(g, i, j, k)) dtl = cell_dt<BORIS>(g, m, U, Bn, idx);
+    }
+    block_min_store(dtl, partial);
+}
+
+// Stand-alone CellDT min over the DT box (CalcDT without the fused update)
+template <int BORIS>
+__global__ void __launch_bounds__(256) k_celldt(GV g, MP m, const DClock *__restrict__ clk, const double *__restrict__ Gas,
+                                                const double *__restrict__ B, double *__restrict__ partial) {
+    if (clk->halt) return;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double dtl = GAM_HUGE;
+    if (idx < g.nc) {
+        int i, j, k;
+        decode_c(g, idx, i, j, k);
+        if (in_dt_box(g, i, j, k)) {
+            const size_t nc = g.nc;
+            double U[5], Bv[3];
+#pragma unroll
+            for (int v = 0; v < 5; v++) U[v] = Gas[nc * v + idx];
+#pragma unroll
+            for (int x = 0; x < 3; x++) Bv[x] = B[nc * x + idx];
+            dtl = cell_dt<BORIS>(g, m, U, Bv, idx);
+        }
+    }
+    block_min_store(dtl, partial);
+}
+
+// UpdateStateData time update (gamapp.F90:112-114, mhdgroup.F90:104-105,255)
+// when advance != 0, then CalcDT's checks and clamps (step.F90:103-166).
+__global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int np, int advance, int calc) {
+    if (clk->halt) return;
+#ifdef GAM_EMU
+    if (threadIdx.x != 0) return;
+    double dtMin = GAM_HUGE;
+    for (int q = 0; q < np; q++) dtMin = fmin(dtMin, partial[q]);
+#else
+    __shared__ double red[16];
+    double v = GAM_HUGE;
+    for (int q = threadIdx.x; q < np; q += blockDim.x) v = fmin(v, partial[q]);
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double dtMin = red[0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); q++) dtMin = fmin(dtMin, red[q]);
+#endif
+    if (advance) {
+        clk->otime = clk->stime;
+        clk->stime = clk->stime + clk->dt;
+        clk->ts = clk->ts + 1;
+        clk->t = clk->t + clk->dt;
+    }
+    if (!calc) return;
+    if (clk->fixedTimestep) { clk->status = 0; return; }
+    double dtOld = clk->dt;
+    int disaster = (dtMin == GAM_HUGE), bad = 0;
+    double dt = dtMin;
+    int status = 0;
+    if (clk->ts > 0) {
+        if (dtOld / dtMin >= 10) bad = 1;
+        if (clk->dt0 > GAM_TINY && dtMin / clk->dt0 <= clk->limDT0) bad = 1;
+        if (clk->doCPR && clk->dt0 > GAM_TINY && dtMin / clk->dt0 <= clk->limCPR) bad = 1;
+        if (dtMin <= GAM_TINY) disaster = 1;
+        status = disaster ? 2 : (bad ? 1 : 0);
+    }
+    if ((clk->t + dt) > clk->tFin) dt = fmax(clk->tFin - clk->t, GAM_TINY);
+    if (clk->dt0 > GAM_TINY && dt > (10.0 * clk->dt0)) dt = 10.0 * clk->dt0;
+    clk->dt = dt;
+    clk->status = status;
+    if (status) clk->halt = 1;   // the reference stops here (step.F90:138-154); the host decides
+    // console-output dt0 capture (output.F90:57-59) at the configured cadence
+    if (clk->dt0_after > 0 && clk->ts >= clk->dt0_after && clk->dt0 < GAM_TINY) clk->dt0 = clk->dt;
+}
+
+// ----------------------------------------------------------------- BCs
+// bcs.F90:279-450 periodic BCs, applied in the order INI,OUTI,INJ,OUTJ,INK,OUTK.
+// The sequential copies compose into one per-axis index map (every source index
+// is an owned index along the axis being mapped), so one in-place pass over the
+// ghost points reproduces them.  Cell-centred map:
+//   i < is -> i + Nip ;  i > ie -> i - Nip
+// Face-normal component along the axis:  i < is -> i + Nip ; i > ie+1 -> i - Nip
+// (face ie+1 keeps its own CT-evolved value).  2.5D: every k except ks -> ks,
+// for all components, k up to keg (bcs.F90:387-394,423-429).
+__device__ __forceinline__ int pmap_c(int i, int a0, int n) { return i < a0 ? i + n : (i >= a0 + n ? i - n : i); }
+__device__ __forceinline__ int pmap_f(int i, int a0, int n) { return i < a0 ? i + n : (i > a0 + n ? i - n : i); }
+
+__global__ void __launch_bounds__(256) k_bc(GV g, const DClock *__restrict__ clk, int do25D, double *__restrict__ Gas,
+                                            double *__restrict__ mag, double *__restrict__ B) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nf || clk->halt) return;
+    int i, j, k;
+    decode_f(g, idx, i, j, k);
+    const int a0 = g.nG;
+    const size_t nc = g.nc, nf = g.nf;
+    const int Nk_eff = g.Nkp;
+    auto kmapc = [&](int kk) { return do25D ? ((kk <= g.Nk - 1) ? a0 : kk) : pmap_c(kk, a0, Nk_eff); };
+    // cells
+    if (i < g.Ni && j < g.Nj && k < g.Nk) {
+        int si = pmap_c(i, a0, g.Nip), sj = pmap_c(j, a0, g.Njp), sk = kmapc(k);
+        if (si != i || sj != j || sk != k) {
+            int s = ci(g, si, sj, sk), d = ci(g, i, j, k);
+#pragma unroll
+            for (int v = 0; v < 5; v++) Gas[nc * v + d] = Gas[nc * v + s];
+            if (B) {   // Bxyz ghosts are copied only where no bFlux2Fld follows (init.F90:230)
+#pragma unroll
+                for (int v = 0; v < 3; v++) B[nc * v + d] = B[nc * v + s];
+            }
+        }
+    }
+    // faces, per component
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        int si = (d == 0) ? pmap_f(i, a0, g.Nip) : pmap_c(i, a0, g.Nip);
+        int sj = (d == 1) ? pmap_f(j, a0, g.Njp) : pmap_c(j, a0, g.Njp);
+        int sk;
+        if (do25D) sk = (k <= g.Nk - 1) ? a0 : k;
+        else sk = (d == 2) ? pmap_f(k, a0, g.Nkp) : pmap_c(k, a0, g.Nkp);
+        // the +1 slot of a non-normal axis is never written by the reference's BCs
+        if (d != 0 && i == g.Ni) si = i;
+        if (d != 1 && j == g.Nj) sj = j;
+        if (!do25D && d != 2 && k == g.Nk) sk = k;
+        if (si != i || sj != j || sk != k) mag[nf * d + idx] = mag[nf * d + fi(g, si, sj, sk)];
+    }
+}
+
+// ----------------------------------------------------------------- diagnostics
+__global__ void k_divb(GV g, const double *__restrict__ mag, double *__restrict__ partial) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    const int n = g.Nip * g.Njp * g.Nkp;
+    if (idx < n) {
+        int a = idx % g.Nip, r = idx / g.Nip, b = r % g.Njp, cz = r / g.Njp;
+        int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
+        const size_t nf = g.nf;
+        int f = fi(g, i, j, k);
+        double dFi = mag[f] - mag[f + 1];
+        double dFj = mag[nf + f] - mag[nf + f + (g.Ni + 1)];
+        double dFk = mag[2 * nf + f] - mag[2 * nf + f + (g.Ni + 1) * (g.Nj + 1)];
+        s = fabs(dFi + dFj + dFk);
+    }
+#ifdef GAM_EMU
+    partial[blockIdx.x] = threadIdx.x == 0 ? s : partial[blockIdx.x] + s;
+    return;
+#endif
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}

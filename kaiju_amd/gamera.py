@@ -1,0 +1,347 @@
+"""Host-side mirror of the reference's Gamera app interface over the HIP C-ABI.
+
+``GamApp`` keeps the shape of the reference's ``gamApp_T``
+(reference ``src/base/types/gamtypes.F90:294-314``): a ``Model`` with the time
+bookkeeping, the grid sizes, ``State``/``oState`` access, and the per-step
+calls ``AdvanceModel(dt)`` (``gamtypessub.F90:78-94``) and ``stepGamera()``
+(``gamapp.F90:73-94``).  All arithmetic runs in ``libgamera_hip.so``; this module
+only moves plain arrays across the C boundary and fails loudly when the library
+is missing.
+
+Arrays are numpy arrays in Fortran order with the reference's shapes
+(``Gas`` is ``(Ni, Nj, Nk, 5)``, ``magFlux`` is ``(Ni+1, Nj+1, Nk+1, 3)``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import configs
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libgamera_hip.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+class GamDims(C.Structure):
+    _fields_ = [("Nip", C.c_int32), ("Njp", C.c_int32), ("Nkp", C.c_int32), ("nG", C.c_int32)]
+
+
+class GamParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("gamma", "CFL", "pdmb", "Vd0", "cHogH", "cHogM", "Ca", "dFloor",
+                                          "pFloor", "tFin", "limDT0", "limCPR")] + \
+               [(n, C.c_int32) for n in ("rmeth", "doLFMLim", "doBoris", "doMHD", "do25D", "doCPR",
+                                         "fixedTimestep", "precision")]
+
+
+class GamClock(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("t", "dt", "dt0", "stateTime", "oStateTime")] + \
+               [(n, C.c_int32) for n in ("ts", "status", "dt0_after", "pad")]
+
+
+class HipStepError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def use_library(path: str):
+    """Point the binding at another build of the same C-ABI (the tests use the
+    CPU emulation build to check kernels before they reach a GPU)."""
+    global LIB_PATH, _lib
+    LIB_PATH = path
+    _lib = None
+
+
+def lib():
+    """Load libgamera_hip.so (no fallback: the step exists only as HIP code)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HipStepError(f"{LIB_PATH} is missing; run __graft_entry__.build() (or python -m kaiju_amd.build hip)")
+    L = C.CDLL(LIB_PATH)
+    H = C.c_void_p
+    sig = {
+        "gam_create": (C.c_int, [C.POINTER(GamDims), C.POINTER(GamParams), C.POINTER(H)]),
+        "gam_destroy": (C.c_int, [H]),
+        "gam_last_error": (C.c_char_p, [H]),
+        "gam_sync": (C.c_int, [H]),
+        "gam_stream": (C.c_void_p, [H]),
+        "gam_set_grid": (C.c_int, [H] + [_dp] * 12),
+        "gam_set_corners": (C.c_int, [H, _dp]),
+        "gam_get_grid": (C.c_int, [H] + [_dp] * 12),
+        "gam_put_state": (C.c_int, [H, C.c_int, C.c_double, _dp, _dp, _dp]),
+        "gam_get_state": (C.c_int, [H, C.c_int, C.POINTER(C.c_double), _dp, _dp, _dp]),
+        "gam_set_clock": (C.c_int, [H, C.POINTER(GamClock)]),
+        "gam_get_clock": (C.c_int, [H, C.POINTER(GamClock)]),
+        "gam_ic_orszag_tang": (C.c_int, [H] + [C.c_double] * 4),
+        "gam_ic_blast_wave": (C.c_int, [H] + [C.c_double] * 9),
+        "gam_hatch": (C.c_int, [H]),
+        "gam_advance_mhd": (C.c_int, [H]),
+        "gam_calc_dt": (C.c_int, [H, C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+        "gam_enforce_bcs": (C.c_int, [H]),
+        "gam_step": (C.c_int, [H, C.c_int]),
+        "gam_divb": (C.c_int, [H, C.POINTER(C.c_double)]),
+        "gam_get_work": (C.c_int, [H, C.c_int, _dp]),
+        "gam_set_timing": (C.c_int, [H, C.c_int]),
+        "gam_get_timing": (C.c_int, [H, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+        "gam_stage_name": (C.c_char_p, [C.c_int]),
+        "gam_num_stages": (C.c_int, []),
+        "gam_stage_bytes": (C.c_int, [H, C.c_int, C.POINTER(C.c_double)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not (a.flags.f_contiguous or a.ndim <= 1):
+        raise ValueError("arrays must be float64 in Fortran order")
+    return a.ctypes.data_as(_dp)
+
+
+def params_from_spec(spec: configs.CaseSpec) -> GamParams:
+    m = spec.model
+    p = GamParams()
+    for k in ("gamma", "CFL", "pdmb", "Vd0", "cHogH", "cHogM", "Ca", "dFloor", "pFloor", "tFin", "limDT0", "limCPR"):
+        setattr(p, k, float(getattr(m, k)))
+    p.rmeth = 1 if m.rmeth.upper() == "7UP" else 0
+    p.doLFMLim = int(m.doLFMLim)
+    p.doBoris = int(m.doBoris)
+    p.doMHD = int(m.doMHD)
+    p.do25D = int(spec.do25D)
+    p.doCPR = int(m.doCPR)
+    p.fixedTimestep = 0
+    p.precision = 0
+    return p
+
+
+def gen_corners(spec: configs.CaseSpec) -> np.ndarray:
+    """Corner array xyz(isg:ieg+1, jsg:jeg+1, ksg:keg+1, 3) for the spec's grid
+    (reference genGridXML, init.F90:669-731, and the warp of configs.blast_wave)."""
+    nG = spec.nG
+    Nip, Njp, Nkp = spec.N
+    x0, x1, y0, y1, z0, z1 = spec.bounds
+    dx, dy, dz = (x1 - x0) / Nip, (y1 - y0) / Njp, (z1 - z0) / Nkp
+    ii = np.arange(1 - nG, Nip + nG + 2)
+    jj = np.arange(1 - nG, Njp + nG + 2)
+    kk = np.arange(1 - nG, Nkp + nG + 2)
+    X = x0 + (ii - 1) * dx
+    Y = y0 + (jj - 1) * dy
+    Z = z0 + (kk - 1) * dz
+    xyz = np.empty((len(ii), len(jj), len(kk), 3), order="F")
+    xyz[..., 0] = X[:, None, None]
+    xyz[..., 1] = Y[None, :, None]
+    xyz[..., 2] = Z[None, None, :]
+    if spec.grid == "warp3":
+        Lx, Ly, Lz = x1 - x0, y1 - y0, z1 - z0
+        s = spec.warp_amp * (np.sin(2 * math.pi * ((X - x0) / Lx))[:, None, None]
+                             * np.sin(2 * math.pi * ((Y - y0) / Ly))[None, :, None]
+                             * np.sin(2 * math.pi * ((Z - z0) / Lz))[None, None, :])
+        xyz[..., 0] += s * Lx
+        xyz[..., 1] += s * Ly
+        xyz[..., 2] += s * Lz
+    elif spec.grid != "cart":
+        raise ValueError(f"grid {spec.grid} is generated by the caller; use set_corners()")
+    return xyz
+
+
+class Model:
+    """Model_T time bookkeeping mirrored from the device clock."""
+
+    def __init__(self, spec: configs.CaseSpec):
+        self.spec = spec.model
+        self.t = 0.0
+        self.dt = 0.0
+        self.dt0 = 0.0
+        self.ts = 0
+        self.tFin = spec.model.tFin
+        self.status = 0
+
+
+class GamApp:
+    """gamApp_T over the HIP step (one handle = one block on one GPU)."""
+
+    def __init__(self, spec: configs.CaseSpec):
+        L = lib()
+        self.spec = spec
+        self.Model = Model(spec)
+        Nip, Njp, Nkp = spec.N
+        self.nG = spec.nG
+        self.Ni, self.Nj, self.Nk = Nip + 2 * spec.nG, Njp + 2 * spec.nG, Nkp + 2 * spec.nG
+        self.G = (self.Ni, self.Nj, self.Nk)
+        self.F = (self.Ni + 1, self.Nj + 1, self.Nk + 1)
+        d = GamDims(Nip, Njp, Nkp, spec.nG)
+        self._params = params_from_spec(spec)
+        h = C.c_void_p()
+        rc = L.gam_create(C.byref(d), C.byref(self._params), C.byref(h))
+        if rc != 0 or not h.value:
+            raise HipStepError(f"gam_create failed rc={rc}")
+        self.h = h
+
+    # ---------------------------------------------------------------- utils
+    def _chk(self, rc: int, what: str) -> int:
+        if rc < 0:
+            msg = lib().gam_last_error(self.h).decode()
+            raise HipStepError(f"{what}: rc={rc} {msg}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().gam_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- setup
+    def set_corners(self, xyz: np.ndarray):
+        xyz = np.asfortranarray(xyz, dtype=np.float64)
+        assert xyz.shape == self.F + (3,)
+        self._chk(lib().gam_set_corners(self.h, _p(xyz)), "gam_set_corners")
+
+    def set_grid(self, grid: dict):
+        names = ("xyz", "xyzcc", "xfc", "face", "Tf", "edge", "Te", "Teb", "volume", "di", "dj", "dk")
+        arrs = [np.asfortranarray(grid[n], dtype=np.float64) for n in names]
+        self._keep = arrs
+        self._chk(lib().gam_set_grid(self.h, *[_p(a) for a in arrs]), "gam_set_grid")
+
+    def get_grid(self) -> dict:
+        G, F = self.G, self.F
+        shapes = dict(xyz=F + (3,), xyzcc=G + (3,), xfc=F + (3, 3), face=F + (3,), Tf=F + (9, 3),
+                      edge=F + (3,), Te=F + (9, 3), Teb=F + (4, 3), volume=G, di=G, dj=G, dk=G)
+        out = {k: np.zeros(s, order="F") for k, s in shapes.items()}
+        names = ("xyz", "xyzcc", "xfc", "face", "Tf", "edge", "Te", "Teb", "volume", "di", "dj", "dk")
+        self._chk(lib().gam_get_grid(self.h, *[_p(out[n]) for n in names]), "gam_get_grid")
+        return out
+
+    def init_ic(self):
+        p, sp = self.spec.ic_params, self.spec
+        L = lib()
+        if sp.ic == "ot":
+            self._chk(L.gam_ic_orszag_tang(self.h, p["rho0"], p["p0"], p["b0"], p["kscale"]), "ic_ot")
+        elif sp.ic == "bw":
+            self._chk(L.gam_ic_blast_wave(self.h, p["x0"], p["y0"], p["z0"], p["D0"], p["P0"], p["pRat"],
+                                          p["dRat"], p["rC"], p["B0"]), "ic_bw")
+        else:
+            raise ValueError(sp.ic)
+
+    def hatch(self):
+        """init.F90:204-244: first Bxyz, BCs, oState = State, first dt."""
+        c = self.get_clock()
+        c.dt0_after = self.spec.dt0_after
+        self.set_clock(c)
+        self._chk(lib().gam_hatch(self.h), "gam_hatch")
+        self._sync_model()
+
+    def InitModel(self, corners: Optional[np.ndarray] = None):
+        """Reference Hatch (init.F90:38-59) for the spec: corners -> metric on
+        the GPU, initial condition on the GPU, hatch."""
+        self.set_corners(gen_corners(self.spec) if corners is None else corners)
+        self.init_ic()
+        self.hatch()
+
+    # ---------------------------------------------------------------- state
+    def put_state(self, which: int, time: float, Gas=None, magFlux=None, Bxyz=None):
+        self._chk(lib().gam_put_state(self.h, which, time, _p(Gas), _p(magFlux), _p(Bxyz)), "gam_put_state")
+
+    def get_state(self, which: int = 0):
+        Gas = np.zeros(self.G + (5,), order="F")
+        mag = np.zeros(self.F + (3,), order="F")
+        B = np.zeros(self.G + (3,), order="F")
+        t = C.c_double()
+        self._chk(lib().gam_get_state(self.h, which, C.byref(t), _p(Gas), _p(mag), _p(B)), "gam_get_state")
+        return dict(time=t.value, Gas=Gas, magFlux=mag, Bxyz=B)
+
+    def get_clock(self) -> GamClock:
+        c = GamClock()
+        self._chk(lib().gam_get_clock(self.h, C.byref(c)), "gam_get_clock")
+        return c
+
+    def set_clock(self, c: GamClock):
+        self._chk(lib().gam_set_clock(self.h, C.byref(c)), "gam_set_clock")
+
+    def _sync_model(self):
+        c = self.get_clock()
+        M = self.Model
+        M.t, M.dt, M.dt0, M.ts, M.status = c.t, c.dt, c.dt0, c.ts, c.status
+        return c
+
+    # ---------------------------------------------------------------- stepping
+    def stepGamera(self, fused: bool = True) -> int:
+        """gamapp.F90:73-94.  fused=False issues AdvanceMHD, CalcDT and
+        EnforceBCs as separate C-ABI calls, like the reference's sequence."""
+        L = lib()
+        if fused:
+            st = self._chk(L.gam_step(self.h, 1), "gam_step")
+        else:
+            self._chk(L.gam_advance_mhd(self.h), "gam_advance_mhd")
+            dt, stc = C.c_double(), C.c_int()
+            st = self._chk(L.gam_calc_dt(self.h, C.byref(dt), C.byref(stc)), "gam_calc_dt")
+            if st == 0:
+                self._chk(L.gam_enforce_bcs(self.h), "gam_enforce_bcs")
+        self._sync_model()
+        return st
+
+    def steps(self, n: int) -> int:
+        st = self._chk(lib().gam_step(self.h, n), "gam_step")
+        self._sync_model()
+        return st
+
+    def AdvanceModel(self, dt: float) -> int:
+        """gamtypessub.F90:78-94: step at least once, then until t >= t0 + dt."""
+        target = self.Model.t + dt
+        st = self.stepGamera()
+        while st == 0 and self.Model.t < target:
+            st = self.stepGamera()
+        return st
+
+    def get_work(self, which: int) -> np.ndarray:
+        """Solver work arrays of the last step: 0 fluxes (G,8,3), 1 EMFs (G,3), 2 Vf (G,3,3)."""
+        shape = {0: self.G + (8, 3), 1: self.G + (3,), 2: self.G + (3, 3)}[which]
+        out = np.zeros(shape, order="F")
+        self._chk(lib().gam_get_work(self.h, which, _p(out)), "gam_get_work")
+        return out
+
+    def divb(self) -> float:
+        v = C.c_double()
+        self._chk(lib().gam_divb(self.h, C.byref(v)), "gam_divb")
+        return v.value
+
+    def sync(self):
+        self._chk(lib().gam_sync(self.h), "gam_sync")
+
+    # ---------------------------------------------------------------- timers
+    def set_timing(self, on: bool):
+        self._chk(lib().gam_set_timing(self.h, int(on)), "gam_set_timing")
+
+    def timings(self) -> dict:
+        L = lib()
+        out = {}
+        for s in range(L.gam_num_stages()):
+            ms, n, b = C.c_double(), C.c_int64(), C.c_double()
+            self._chk(L.gam_get_timing(self.h, s, C.byref(ms), C.byref(n)), "gam_get_timing")
+            self._chk(L.gam_stage_bytes(self.h, s, C.byref(b)), "gam_stage_bytes")
+            out[L.gam_stage_name(s).decode()] = dict(ms=ms.value, launches=n.value, bytes=b.value)
+        return out

@@ -1,0 +1,1297 @@
+/*
+ * gamera_oracle.c -- CPU restatement of the GAMERA MHD step (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP step and the timed CPU
+ * baseline.  The product path (kaiju_amd/) never loads it.
+ *
+ * Each routine names the reference routine it restates (file:line relative to
+ * the kaiju tree).  Arithmetic follows the reference's Fortran evaluation
+ * order (left-to-right, x**2.0 as x*x, SIGN(1,x) as copysign) so that results
+ * agree to rounding.  Built with -ffp-contract=off and no fast-math.
+ *
+ * Parity status of each part is listed in the header comment of
+ * tests/test_oracle.py next to the check that pins it.
+ */
+#include "gamera_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define TINY 1.0e-12                       /* base/kdefs.F90:20 */
+#define HUGEV 1.0e+12
+static const double PI = 3.14159265358979323846264338327950288;
+
+enum { DEN = 0, MOMX, MOMY, MOMZ, ENERGY };              /* kdefs.F90:98-112 */
+enum { NORMX = 0, NORMY, NORMZ, TAN1X, TAN1Y, TAN1Z, TAN2X, TAN2Y, TAN2Z }; /* gdefs.F90:27-32 */
+enum { XNQI = 0, YNQI, XNQJ, YNQJ };                      /* gdefs.F90:35-37 */
+enum { ID = 0, JD = 1, KD = 2 };
+
+/* recon.F90:22-23,30 -- integer weights divided in double precision */
+static double W8[8], W6[8], U7[7];
+static int weights_ready = 0;
+static void init_weights(void) {
+    if (weights_ready) return;
+    const int w8[8] = {-3, 29, -139, 533, 533, -139, 29, -3};
+    const int w6[8] = {0, 1, -8, 37, 37, -8, 1, 0};
+    const int u7[7] = {-3, 25, -101, 319, 214, -38, 4};
+    for (int n = 0; n < 8; n++) { W8[n] = (double)w8[n] / 840.0; W6[n] = (double)w6[n] / 60.0; }
+    for (int n = 0; n < 7; n++) U7[n] = (double)u7[n] / 420.0;
+    weights_ready = 1;
+}
+
+/* quadrature.F90:11-26 */
+static const double gA[12] = {
+    0.1252334085114689154724, 0.3678314989981801937527, 0.5873179542866174472967,
+    0.7699026741943046870369, 0.9041172563704748566785, 0.9815606342467192506906,
+    -0.1252334085114689154724, -0.3678314989981801937527, -0.5873179542866174472967,
+    -0.7699026741943046870369, -0.9041172563704748566785, -0.9815606342467192506906};
+static const double gW[12] = {
+    0.2491470458134027850006, 0.233492536538354808761, 0.203167426723065921749,
+    0.160078328543346226335, 0.1069393259953184309603, 0.0471753363865118271946,
+    0.2491470458134027850006, 0.233492536538354808761, 0.203167426723065921749,
+    0.160078328543346226335, 0.1069393259953184309603, 0.0471753363865118271946};
+
+/* ---------------------------------------------------------------- indexing */
+static inline size_t CI(const orc_grid *g, int i, int j, int k) {
+    return (size_t)(i - g->isg) + (size_t)g->Ni * ((size_t)(j - g->jsg) + (size_t)g->Nj * (size_t)(k - g->ksg));
+}
+static inline size_t FI(const orc_grid *g, int i, int j, int k) {
+    return (size_t)(i - g->isg) + (size_t)(g->Ni + 1) * ((size_t)(j - g->jsg) + (size_t)(g->Nj + 1) * (size_t)(k - g->ksg));
+}
+static inline int dlt(int a, int b) { return a == b; }
+
+void orc_grid_derive(orc_grid *g) {
+    /* init.F90:648-663 (genGridXML index setup), SetDomain init.F90:519-538 */
+    g->Ni = g->Nip + 2 * g->nG; g->Nj = g->Njp + 2 * g->nG; g->Nk = g->Nkp + 2 * g->nG;
+    g->is = 1; g->ie = g->Nip; g->js = 1; g->je = g->Njp; g->ks = 1; g->ke = g->Nkp;
+    g->isg = g->is - g->nG; g->ieg = g->ie + g->nG;
+    g->jsg = g->js - g->nG; g->jeg = g->je + g->nG;
+    g->ksg = g->ks - g->nG; g->keg = g->ke + g->nG;
+    g->isDT = g->is - 1; g->ieDT = g->ie + 1;
+    g->jsDT = g->js - 1; g->jeDT = g->je + 1;
+    g->ksDT = g->ks - 1; g->keDT = g->ke + 1;
+    g->nc = (size_t)g->Ni * g->Nj * g->Nk;
+    g->nf = (size_t)(g->Ni + 1) * (g->Nj + 1) * (g->Nk + 1);
+}
+
+/* ------------------------------------------------------------ small maths */
+static inline double sq(double x) { return x * x; }           /* x**2.0 */
+static inline double nrm3(const double *a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+static inline void cross3(const double *a, const double *b, double *c) { /* base/math.F90:265-274 */
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+static inline double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline void normvec(const double *a, double *o) {       /* math.F90:332-343 */
+    double n = sqrt(dot3(a, a));
+    if (n <= TINY) { o[0] = o[1] = o[2] = 0.0; }
+    else { o[0] = a[0] / n; o[1] = a[1] / n; o[2] = a[2] / n; }
+}
+
+/* ----------------------------------------------------- cell conversions */
+void orc_cell_c2p(const orc_model *m, const double *con, double *prim) {
+    /* gamutils.F90:301-321 CellC2P */
+    double rho = con[DEN];
+    double Vx = con[MOMX] / rho, Vy = con[MOMY] / rho, Vz = con[MOMZ] / rho;
+    double E = con[ENERGY];
+    double KinE = 0.5 * rho * (sq(Vx) + sq(Vy) + sq(Vz));
+    prim[DEN] = fmax(rho, m->dFloor);
+    prim[1] = Vx; prim[2] = Vy; prim[3] = Vz;
+    prim[4] = fmax((m->gamma - 1) * (E - KinE), m->pFloor);
+}
+
+void orc_cell_p2c(const orc_model *m, const double *prim, double *con) {
+    /* gamutils.F90:362-383 CellP2C */
+    double rho = fmax(prim[DEN], m->dFloor);
+    double Vx = prim[1], Vy = prim[2], Vz = prim[3];
+    double IntE = fmax(prim[4], m->pFloor) / (m->gamma - 1);
+    double KinE = 0.5 * rho * (sq(Vx) + sq(Vy) + sq(Vz));
+    con[DEN] = rho; con[MOMX] = rho * Vx; con[MOMY] = rho * Vy; con[MOMZ] = rho * Vz;
+    con[ENERGY] = IntE + KinE;
+}
+
+/* recon.F90:66-92 BlockCon2Prim (block form: floor first, divide by floored D) */
+static inline void block_c2p(const orc_model *m, const double *c, double *w) {
+    double D = fmax(c[DEN], m->dFloor);
+    double Mx = c[MOMX], My = c[MOMY], Mz = c[MOMZ], E = c[ENERGY];
+    double KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
+    double P = fmax((m->gamma - 1) * (E - KinE), m->pFloor);
+    w[0] = D; w[1] = Mx / D; w[2] = My / D; w[3] = Mz / D; w[4] = P;
+}
+
+/* ----------------------------------------------------- reconstruction */
+double orc_central8(const double *q) {       /* recon.F90:202-214 */
+    init_weights();
+    double s = 0.0;
+    for (int n = 0; n < 8; n++) s += W8[n] * q[n];
+    return s;
+}
+static inline double c8(const double *q) {
+    double s = 0.0;
+    for (int n = 0; n < 8; n++) s += W8[n] * q[n];
+    return s;
+}
+double orc_up7(const double *q) {            /* recon.F90:303-310 */
+    init_weights();
+    return q[0] * U7[0] + q[1] * U7[1] + q[2] * U7[2] + q[3] * U7[3]
+         + q[4] * U7[4] + q[5] * U7[5] + q[6] * U7[6];
+}
+static inline double up7(double q0, double q1, double q2, double q3, double q4, double q5, double q6) {
+    return q0 * U7[0] + q1 * U7[1] + q2 * U7[2] + q3 * U7[3]
+         + q4 * U7[4] + q5 * U7[5] + q6 * U7[6];
+}
+double orc_pdm(double pdmb, double q0, double q1, double q2, double qI) {   /* recon.F90:275-301 */
+    double maxQ = fmax(q1, q2), minQ = fmin(q1, q2);
+    double qN = fmax(minQ, fmin(qI, maxQ));
+    double dq0 = pdmb * (q1 - q0);
+    double dq1 = pdmb * (q2 - q1);
+    double s0 = copysign(1.0, dq0), s1 = copysign(1.0, dq1);
+    double s01 = fabs(s0 + s1);
+    double dqL = qN - q1;
+    return qN - s1 * fmax(0.0, fabs(dqL) - s01 * fabs(dq0));
+}
+void orc_pdmlr(double pdmb, const double *nQ, double Qi, double *Ql, double *Qr) { /* recon.F90:231-272 */
+    double v0 = nQ[0], v1 = nQ[1], v2 = nQ[2], v3 = nQ[3];
+    double maxV = fmax(v1, v2), minV = fmin(v1, v2);
+    double vN = fmax(minV, fmin(Qi, maxV));
+    double dv0 = pdmb * (v1 - v0), dv1 = pdmb * (v2 - v1), dv2 = pdmb * (v3 - v2);
+    double s0 = copysign(1.0, dv0), s1 = copysign(1.0, dv1), s2 = copysign(1.0, dv2);
+    double q0 = fabs(s0 + s1), q1 = fabs(s1 + s2);
+    double dvL = vN - v1, dvR = v2 - vN;
+    *Ql = vN - s1 * fmax(0.0, fabs(dvL) - q0 * fabs(dv0));
+    *Qr = vN + s1 * fmax(0.0, fabs(dvR) - q1 * fabs(dv2));
+}
+
+/* one variable of BlockLRs/GetLRs: dV[8] volumes, Q[8] values, nQ[4] limiter
+ * values, Vi = Central8(dV).  recon.F90:125-158 (Cen8LRs), 162-198 (Up7LRs) */
+static inline void get_lrs(const orc_model *m, const double *dV, const double *nQ, const double *Q,
+                           double Vi, double *Ql, double *Qr) {
+    double QdV[8];
+    for (int n = 0; n < 8; n++) QdV[n] = dV[n] * Q[n];
+    if (m->rmeth == ORC_7UP) {
+        double l = up7(QdV[0], QdV[1], QdV[2], QdV[3], QdV[4], QdV[5], QdV[6]) / Vi;
+        double r = up7(QdV[7], QdV[6], QdV[5], QdV[4], QdV[3], QdV[2], QdV[1]) / Vi;
+        *Ql = orc_pdm(m->pdmb, nQ[0], nQ[1], nQ[2], l);
+        *Qr = orc_pdm(m->pdmb, nQ[3], nQ[2], nQ[1], r);
+    } else {
+        double Qi = c8(QdV) / Vi;
+        orc_pdmlr(m->pdmb, nQ, Qi, Ql, Qr);
+    }
+}
+
+/* ----------------------------------------------------- kinetic fluxes */
+void orc_gaskinflux(const orc_model *m, int q, const double *P5, const double *T,
+                    double *gF, double *dW) {
+    /* stress.F90:432-511 GasKinFlux, q = -1 (LEFT) or +1 (RIGHT) */
+    double sg = (double)q;
+    double D = P5[0], P = P5[4], Vx = P5[1], Vy = P5[2], Vz = P5[3];
+    double E = 0.5 * D * (sq(Vx) + sq(Vy) + sq(Vz)) + P / (m->gamma - 1);
+    double lambda = D / (2 * P);
+    double Vn  = T[NORMX] * Vx + T[NORMY] * Vy + T[NORMZ] * Vz;
+    double Vt1 = T[TAN1X] * Vx + T[TAN1Y] * Vy + T[TAN1Z] * Vz;
+    double Vt2 = T[TAN2X] * Vx + T[TAN2Y] * Vy + T[TAN2Z] * Vz;
+    double Vn0 = 0.5 * erfc(sg * sqrt(lambda) * Vn);
+    double Vn1 = Vn * Vn0 - 0.5 * sg * exp(-lambda * Vn * Vn) / sqrt(PI * lambda);
+    gF[DEN] = gF[DEN] + D * Vn1;
+    gF[ENERGY] = gF[ENERGY] + (E + 0.5 * P) * Vn1 + 0.5 * P * Vn0 * Vn;
+    double fMn = D * Vn * Vn1 + P * Vn0;
+    double fMt1 = D * Vt1 * Vn1;
+    double fMt2 = D * Vt2 * Vn1;
+    double fMx = T[NORMX] * fMn + T[TAN1X] * fMt1 + T[TAN2X] * fMt2;
+    double fMy = T[NORMY] * fMn + T[TAN1Y] * fMt1 + T[TAN2Y] * fMt2;
+    double fMz = T[NORMZ] * fMn + T[TAN1Z] * fMt1 + T[TAN2Z] * fMt2;
+    gF[MOMX] = gF[MOMX] + fMx;
+    gF[MOMY] = gF[MOMY] + fMy;
+    gF[MOMZ] = gF[MOMZ] + fMz;
+    dW[DEN] = dW[DEN] + sg * D;
+    dW[MOMX] = dW[MOMX] + sg * D * Vx;
+    dW[MOMY] = dW[MOMY] + sg * D * Vy;
+    dW[MOMZ] = dW[MOMZ] + sg * D * Vz;
+    dW[ENERGY] = dW[ENERGY] + sg * E;
+}
+
+void orc_magkinflux(const orc_model *m, int q, const double *P5, const double *B,
+                    double Bn, const double *T, double *mF, double *bbD, double *VaD) {
+    /* stress.F90:513-599 MagKinFlux (no background field) */
+    double sg = (double)q;
+    double D = P5[0], P = P5[4];
+    double Bx = B[0], By = B[1], Bz = B[2];
+    double dPb = 0.5 * (sq(Bx) + sq(By) + sq(Bz));
+    double Va2 = 2 * dPb / D;
+    *bbD = *bbD + 0.5 * (sq(Bx) + sq(By) + sq(Bz));
+    double Vac2 = m->doBoris ? Va2 / (1 + (Va2 / sq(m->Ca))) : Va2;
+    double Nx = T[NORMX], Ny = T[NORMY], Nz = T[NORMZ];
+    double Vn = Nx * P5[1] + Ny * P5[2] + Nz * P5[3];
+    double lambda = 1 / (2 * P / D + Vac2);
+    double Vn0 = 0.5 * erfc(sg * sqrt(lambda) * Vn);
+    *VaD = *VaD + 0.5 * sqrt(Vac2);
+    mF[0] = mF[0] + Vn0 * (dPb * Nx - Bx * Bn);
+    mF[1] = mF[1] + Vn0 * (dPb * Ny - By * Bn);
+    mF[2] = mF[2] + Vn0 * (dPb * Nz - Bz * Bn);
+}
+
+/* ----------------------------------------------------- flux -> field */
+static inline void cell_bxyz(const orc_grid *g, const double *mag, int i, int j, int k, double *B) {
+    /* gridutils.F90:648-680 gamCellBxyz */
+    size_t nf = g->nf;
+    double dV = g->volume[CI(g, i, j, k)];
+    size_t f0 = FI(g, i, j, k);
+    double fI = mag[f0], fJ = mag[f0 + nf], fK = mag[f0 + 2 * nf];
+    size_t fi1 = FI(g, i + 1, j, k), fj1 = FI(g, i, j + 1, k), fk1 = FI(g, i, j, k + 1);
+    double fIp = mag[fi1], fJp = mag[fj1 + nf], fKp = mag[fk1 + 2 * nf];
+    double Div = fIp - fI + fJp - fJ + fKp - fK;
+    size_t c = CI(g, i, j, k);
+    for (int x = 0; x < 3; x++) {
+        double xfI  = g->xfc[f0  + nf * (x + 3 * ID)], xfIp = g->xfc[fi1 + nf * (x + 3 * ID)];
+        double xfJ  = g->xfc[f0  + nf * (x + 3 * JD)], xfJp = g->xfc[fj1 + nf * (x + 3 * JD)];
+        double xfK  = g->xfc[f0  + nf * (x + 3 * KD)], xfKp = g->xfc[fk1 + nf * (x + 3 * KD)];
+        double b = (fIp * xfIp + fJp * xfJp + fKp * xfKp - fI * xfI - fJ * xfJ - fK * xfK) / dV;
+        B[x] = b - Div * g->xyzcc[c + g->nc * x] / dV;
+    }
+}
+
+void orc_bflux2fld(const orc_grid *g, const double *mag, double *Bxyz) {
+    /* gridutils.F90:281-301 bFlux2Fld (no ring fix) */
+#pragma omp parallel for collapse(2)
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int j = g->jsg; j <= g->jeg; j++)
+            for (int i = g->isg; i <= g->ieg; i++) {
+                double B[3];
+                cell_bxyz(g, mag, i, j, k, B);
+                size_t c = CI(g, i, j, k);
+                Bxyz[c] = B[0]; Bxyz[c + g->nc] = B[1]; Bxyz[c + 2 * g->nc] = B[2];
+            }
+}
+
+/* ----------------------------------------------------- predictor */
+static inline double ab2(double dt, double Q, double oQ, double odt) {   /* mhdgroup.F90:710-718 */
+    double dQ = (Q - oQ) / odt;
+    return Q + dt * dQ;
+}
+
+void orc_predictor(const orc_model *m, const orc_grid *g, const orc_state *s,
+                   const orc_state *o, orc_state *p, double pdt) {
+    /* mhdgroup.F90:445-525 Predictor (AB2 path, single fluid) */
+    double odt = s->time - o->time;
+    size_t nf = g->nf, nc = g->nc;
+    if (m->doMHD) {
+#pragma omp parallel for
+        for (size_t f = 0; f < nf; f++)
+            for (int d = 0; d < 3; d++)
+                p->magFlux[f + nf * d] = ab2(pdt, s->magFlux[f + nf * d], o->magFlux[f + nf * d], odt);
+    }
+    orc_bflux2fld(g, p->magFlux, p->Bxyz);
+#pragma omp parallel for
+    for (size_t c = 0; c < nc; c++) {
+        double Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5];
+        for (int v = 0; v < 5; v++) { Q[v] = s->Gas[c + nc * v]; oQ[v] = o->Gas[c + nc * v]; }
+        orc_cell_c2p(m, oQ, oW);   /* mhdgroup.F90:545-546 */
+        orc_cell_c2p(m, Q, W);
+        for (int v = 0; v < 5; v++) pW[v] = ab2(pdt, W[v], oW[v], odt);
+        pW[DEN] = fmax(pW[DEN], m->dFloor);
+        pW[4] = fmax(pW[4], m->pFloor);
+        orc_cell_p2c(m, pW, pQ);
+        for (int v = 0; v < 5; v++) p->Gas[c + nc * v] = pQ[v];
+    }
+}
+
+/* ----------------------------------------------------- electric field */
+/* offsets of the n-th stencil point along direction d from (i,j,k) */
+#define STEP(d, n, i, j, k, oi, oj, ok) do { oi = (i) + (n) * dlt(d, ID); oj = (j) + (n) * dlt(d, JD); ok = (k) + (n) * dlt(d, KD); } while (0)
+
+/* fields.F90:235-285 Mom2Face for one face (i,j,k) along dT */
+static inline void mom2face(const orc_model *m, const orc_grid *g, const double *Gas, int i, int j, int k, int dT, double *Vf3) {
+    size_t nc = g->nc;
+    double VolB[8], VelB[3][8];
+    for (int n = 0; n < 8; n++) {
+        int a, b, cc; STEP(dT, n - 4, i, j, k, a, b, cc);
+        size_t c = CI(g, a, b, cc);
+        double vol = g->volume[c];
+        double D = Gas[c + nc * DEN];
+        VolB[n] = vol;
+        for (int d = 0; d < 3; d++) VelB[d][n] = vol * Gas[c + nc * (1 + d)] / fmax(D, m->dFloor);
+    }
+    double dV = fmax(c8(VolB), TINY);
+    for (int d = 0; d < 3; d++) Vf3[d] = c8(VelB[d]) / dV;
+}
+
+/* fields.F90:288-362 EdgeLRs: dT1 faces reconstructed along dT2 */
+static inline void edge_lrs(const orc_model *m, const orc_grid *g, const double *magN, const double *magH,
+                            int i, int j, int k, int dT1, int dT2, double *bL, double *bR, double *fA) {
+    size_t nf = g->nf;
+    double AreaB[8], MagB[8], nAreaB[4], nMagB[4];
+    for (int n = 0; n < 8; n++) {
+        int a, b, c; STEP(dT2, n - 4, i, j, k, a, b, c);
+        size_t f = FI(g, a, b, c);
+        AreaB[n] = g->face[f + nf * dT1];
+        MagB[n] = magH[f + nf * dT1];
+    }
+    const double *limMag = m->doLFMLim ? magN : magH;
+    for (int n = 0; n < 4; n++) {
+        int a, b, c; STEP(dT2, n - 2, i, j, k, a, b, c);
+        size_t f = FI(g, a, b, c);
+        nAreaB[n] = g->face[f + nf * dT1];
+        nMagB[n] = limMag[f + nf * dT1];
+    }
+    for (int n = 0; n < 8; n++) MagB[n] = MagB[n] / fmax(AreaB[n], TINY);
+    for (int n = 0; n < 4; n++) nMagB[n] = nMagB[n] / fmax(nAreaB[n], TINY);
+    double Vi = c8(AreaB);
+    double l, r;
+    get_lrs(m, AreaB, nMagB, MagB, Vi, &l, &r);
+    double fa = c8(AreaB);
+    *bL = l * fa; *bR = r * fa; *fA = fa;
+}
+
+void orc_calc_efield(const orc_model *m, const orc_grid *g, const orc_state *n,
+                     const orc_state *hf, double *Vf, double *E) {
+    /* fields.F90:70-232 CalcElecField (no resistivity, no background) */
+    init_weights();
+    size_t nc = g->nc, nf = g->nf;
+    memset(E, 0, sizeof(double) * 3 * nc);
+    for (int eD = 0; eD < 3; eD++) {
+        int dT1, dT2, ie, je, ke;
+        int vi0, vi1, vj0, vj1, vk0, vk1;   /* Mom2Face range */
+        if (eD == ID) { dT1 = JD; dT2 = KD; ie = g->ie; je = g->je + 1; ke = g->ke + 1;
+            vi0 = g->isg; vi1 = g->ieg; vj0 = g->js; vj1 = g->je + 1; vk0 = g->ksg; vk1 = g->keg; }
+        else if (eD == JD) { dT1 = KD; dT2 = ID; ie = g->ie + 1; je = g->je; ke = g->ke + 1;
+            vi0 = g->isg; vi1 = g->ieg; vj0 = g->jsg; vj1 = g->jeg; vk0 = g->ks; vk1 = g->ke + 1; }
+        else { dT1 = ID; dT2 = JD; ie = g->ie + 1; je = g->je + 1; ke = g->ke;
+            vi0 = g->is; vi1 = g->ie + 1; vj0 = g->jsg; vj1 = g->jeg; vk0 = g->ksg; vk1 = g->keg; }
+#pragma omp parallel for collapse(2)
+        for (int k = vk0; k <= vk1; k++)
+            for (int j = vj0; j <= vj1; j++)
+                for (int i = vi0; i <= vi1; i++) {
+                    double v[3];
+                    mom2face(m, g, hf->Gas, i, j, k, dT1, v);
+                    size_t c = CI(g, i, j, k);
+                    Vf[c] = v[0]; Vf[c + nc] = v[1]; Vf[c + 2 * nc] = v[2];
+                }
+#pragma omp parallel for collapse(2)
+        for (int k = g->ks; k <= ke; k++)
+            for (int j = g->js; j <= je; j++)
+                for (int i = g->is; i <= ie; i++) {
+                    size_t f = FI(g, i, j, k);
+                    /* GetCornerV fields.F90:457-517 */
+                    double AreaB[8], VelB[3][8];
+                    for (int q = 0; q < 8; q++) {
+                        int a, b, c; STEP(dT2, q - 4, i, j, k, a, b, c);
+                        AreaB[q] = g->face[FI(g, a, b, c) + nf * dT1];
+                        size_t cc = CI(g, a, b, c);
+                        for (int d = 0; d < 3; d++) VelB[d][q] = Vf[cc + nc * d];
+                    }
+                    double dA = fmax(c8(AreaB), TINY);
+                    double Vc[3];
+                    for (int d = 0; d < 3; d++) {
+                        double s = 0.0;
+                        for (int q = 0; q < 8; q++) s += W8[q] * (AreaB[q] * VelB[d][q]);
+                        Vc[d] = s / dA;
+                    }
+                    const double *Te = g->Te;
+                    size_t tb = f + nf * 9 * eD;
+                    double v1 = Vc[0] * Te[tb + nf * TAN1X] + Vc[1] * Te[tb + nf * TAN1Y] + Vc[2] * Te[tb + nf * TAN1Z];
+                    double v2 = Vc[0] * Te[tb + nf * TAN2X] + Vc[1] * Te[tb + nf * TAN2Y] + Vc[2] * Te[tb + nf * TAN2Z];
+                    double vDiff = sqrt(sq(v1) + sq(v2));
+                    /* GetCornerD fields.F90:427-454 */
+                    int e1i = (eD == ID) ? 0 : 1, e1j = (eD == ID) ? 1 : 0, e1k = 0;
+                    int e2i = 0, e2j = (eD == KD) ? 1 : 0, e2k = (eD == KD) ? 0 : 1;
+                    const double *rho = hf->Gas;
+                    double Dc = 0.25 * (rho[CI(g, i, j, k)] + rho[CI(g, i - e1i, j - e1j, k - e1k)]
+                                        + rho[CI(g, i - e2i, j - e2j, k - e2k)]
+                                        + rho[CI(g, i - e1i - e2i, j - e1j - e2j, k - e1k - e2k)]);
+                    /* GetCornerB fields.F90:365-424 */
+                    double bT1L, bT1R, fA1, bT2L, bT2R, fA2;
+                    edge_lrs(m, g, n->magFlux, hf->magFlux, i, j, k, dT1, dT2, &bT1L, &bT1R, &fA1);
+                    edge_lrs(m, g, n->magFlux, hf->magFlux, i, j, k, dT2, dT1, &bT2L, &bT2R, &fA2);
+                    size_t eb = f + nf * 4 * eD;
+                    double xni = g->Teb[eb + nf * XNQI], yni = g->Teb[eb + nf * YNQI];
+                    double xnj = g->Teb[eb + nf * XNQJ], ynj = g->Teb[eb + nf * YNQJ];
+                    double detT = 1.0 / (xni * ynj - xnj * yni);
+                    double bT1 = 0.5 * (bT1L + bT1R) / fA1;
+                    double bT2 = 0.5 * (bT2L + bT2R) / fA2;
+                    double db1 = (bT1R - bT1L) / fA1;
+                    double db2 = (bT2R - bT2L) / fA2;
+                    double Jd = db2 - db1;
+                    double b1 = detT * (ynj * bT1 - yni * bT2);
+                    double b2 = detT * (-xnj * bT1 + xni * bT2);
+                    /* fields.F90:204-221 */
+                    double vA = sqrt(sq(b1) + sq(b2)) / sqrt(Dc);
+                    if (m->doBoris) vA = m->Ca * vA / sqrt(sq(m->Ca) + sq(vA));
+                    vDiff = vDiff + vA;
+                    double ed = g->edge[f + nf * eD];
+                    vDiff = fmin(vDiff, m->CFL * ed / m->dt);
+                    double e = -(v1 * b2 - v2 * b1) + m->Vd0 * vDiff * Jd;
+                    E[CI(g, i, j, k) + nc * eD] = e * ed;
+                }
+    }
+}
+
+/* ----------------------------------------------------- stresses */
+/* stress.F90:204-429 Fluxes for one interface (i,j,k) in direction dN */
+static void fluxes_iface(const orc_model *m, const orc_grid *g, const orc_state *n, const orc_state *hf,
+                         int i, int j, int k, int dN, double *gOut, double *mOut) {
+    size_t nc = g->nc, nf = g->nf;
+    double VolB[8], ConB[8][5], PrimB[8][5], nPrimB[4][5], MagB[8][3], nMagB[4][3];
+    size_t cs[8];
+    for (int q = 0; q < 8; q++) {
+        int a, b, c; STEP(dN, q - 4, i, j, k, a, b, c);
+        cs[q] = CI(g, a, b, c);
+        VolB[q] = g->volume[cs[q]];
+    }
+    size_t f = FI(g, i, j, k);
+    double Tf[9];
+    for (int t = 0; t < 9; t++) Tf[t] = g->Tf[f + nf * (t + 9 * dN)];
+    double faB = g->face[f + nf * dN];
+    double Vi = c8(VolB);
+    double MagL[3] = {0, 0, 0}, MagR[3] = {0, 0, 0}, Bn = 0.0;
+    if (m->doMHD) {
+        const double *limB = m->doLFMLim ? n->Bxyz : hf->Bxyz;
+        for (int q = 0; q < 8; q++) for (int d = 0; d < 3; d++) MagB[q][d] = hf->Bxyz[cs[q] + nc * d];
+        for (int q = 0; q < 4; q++) for (int d = 0; d < 3; d++) nMagB[q][d] = limB[cs[q + 2] + nc * d];
+        for (int d = 0; d < 3; d++) {
+            double Q[8], nQ[4];
+            for (int q = 0; q < 8; q++) Q[q] = MagB[q][d];
+            for (int q = 0; q < 4; q++) nQ[q] = nMagB[q][d];
+            get_lrs(m, VolB, nQ, Q, Vi, &MagL[d], &MagR[d]);
+        }
+        Bn = hf->magFlux[f + nf * dN] / faB;
+    }
+    const double *limG = m->doLFMLim ? n->Gas : hf->Gas;
+    for (int q = 0; q < 8; q++) {
+        for (int v = 0; v < 5; v++) ConB[q][v] = hf->Gas[cs[q] + nc * v];
+        block_c2p(m, ConB[q], PrimB[q]);
+    }
+    for (int q = 0; q < 4; q++) {
+        double c[5];
+        for (int v = 0; v < 5; v++) c[v] = limG[cs[q + 2] + nc * v];
+        block_c2p(m, c, nPrimB[q]);
+    }
+    double PL[5], PR[5];
+    for (int v = 0; v < 5; v++) {
+        double Q[8], nQ[4];
+        for (int q = 0; q < 8; q++) Q[q] = PrimB[q][v];
+        for (int q = 0; q < 4; q++) nQ[q] = nPrimB[q][v];
+        get_lrs(m, VolB, nQ, Q, Vi, &PL[v], &PR[v]);
+    }
+    double VaD = 0.0;
+    if (m->doMHD) {
+        double mF[3] = {0, 0, 0}, bbD = 0.0;
+        orc_magkinflux(m, -1, PL, MagL, Bn, Tf, mF, &bbD, &VaD);
+        orc_magkinflux(m, +1, PR, MagR, Bn, Tf, mF, &bbD, &VaD);
+        if (m->doBoris) {   /* stress.F90:335-351 */
+            for (int d = 0; d < 3; d++) {
+                double deeV = PR[1 + d] - PL[1 + d];
+                double deeP = bbD * deeV / sq(m->Ca);
+                mF[d] = mF[d] - m->cHogM * VaD * deeP;
+            }
+        }
+        for (int d = 0; d < 3; d++) mOut[d] = faB * mF[d];
+    }
+    double gF[5] = {0, 0, 0, 0, 0}, dW[5] = {0, 0, 0, 0, 0};
+    orc_gaskinflux(m, -1, PL, Tf, gF, dW);
+    orc_gaskinflux(m, +1, PR, Tf, gF, dW);
+    for (int v = 0; v < 5; v++) gF[v] = gF[v] - 1.0 * m->cHogH * VaD * dW[v];   /* stress.F90:385-396 */
+    if (m->doBoris) {   /* stress.F90:399-417 nuke */
+        double vL = nrm3(&PL[1]), vR = nrm3(&PR[1]);
+        double Vn = sqrt(sq(vL) + sq(vR)) / (double)sqrtf(2.0f);
+        if (Vn >= 1.5 * m->Ca) {
+            for (int v = 0; v < 5; v++) {
+                double dCon = ConB[4][v] - ConB[3][v];
+                gF[v] = gF[v] - m->Ca * dCon;
+            }
+        }
+    }
+    for (int v = 0; v < 5; v++) gOut[v] = faB * gF[v];
+}
+
+void orc_calc_stress(const orc_model *m, const orc_grid *g, const orc_state *n,
+                     const orc_state *hf, double *gFlx, double *mFlx,
+                     double *dGasH, double *dGasM) {
+    /* stress.F90:41-198 CalcStress; gFlx is (G,5,3), mFlx (G,3,3) */
+    init_weights();
+    size_t nc = g->nc;
+    int nFlx = m->do25D ? 2 : 3;
+    int ks = g->ks, ke = m->do25D ? g->ks : g->ke;
+    for (int d = 0; d < nFlx; d++) {
+        int i1 = g->ie + (d == ID), j1 = g->je + (d == JD), k1 = ke + (d == KD);
+#pragma omp parallel for collapse(2)
+        for (int k = ks; k <= k1; k++)
+            for (int j = g->js; j <= j1; j++)
+                for (int i = g->is; i <= i1; i++) {
+                    double gO[5], mO[3];
+                    fluxes_iface(m, g, n, hf, i, j, k, d, gO, mO);
+                    size_t c = CI(g, i, j, k);
+                    for (int v = 0; v < 5; v++) gFlx[c + nc * (v + 5 * d)] = gO[v];
+                    if (m->doMHD) for (int v = 0; v < 3; v++) mFlx[c + nc * (v + 3 * d)] = mO[v];
+                }
+    }
+    /* Flux2Deltas stress.F90:152-194 */
+#pragma omp parallel for collapse(2)
+    for (int k = g->ks; k <= g->ke; k++)
+        for (int j = g->js; j <= g->je; j++)
+            for (int i = g->is; i <= g->ie; i++) {
+                size_t c = CI(g, i, j, k);
+                size_t ci = CI(g, i + 1, j, k), cj = CI(g, i, j + 1, k), ck = CI(g, i, j, k + 1);
+                double dV = g->volume[c];
+                for (int v = 0; v < 5; v++) {
+                    const double *F = gFlx;
+                    dGasH[c + nc * v] = (F[c + nc * (v + 0)] - F[ci + nc * (v + 0)]
+                                       + F[c + nc * (v + 5)] - F[cj + nc * (v + 5)]
+                                       + F[c + nc * (v + 10)] - F[ck + nc * (v + 10)]) / dV;
+                }
+                if (m->doMHD) {
+                    for (int v = 0; v < 3; v++) {
+                        const double *F = mFlx;
+                        dGasM[c + nc * v] = (F[c + nc * (v + 0)] - F[ci + nc * (v + 0)]
+                                           + F[c + nc * (v + 3)] - F[cj + nc * (v + 3)]
+                                           + F[c + nc * (v + 6)] - F[ck + nc * (v + 6)]) / dV;
+                    }
+                }
+            }
+}
+
+/* ----------------------------------------------------- update */
+void orc_cell_maxwell(const orc_model *m, double *U, const double *dGasM, double dt) {
+    /* mhdgroup.F90:300-330 CellMaxwell */
+    double D = fmax(U[DEN], m->dFloor);
+    double Mx = U[MOMX], My = U[MOMY], Mz = U[MOMZ], TotE = U[ENERGY];
+    double KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
+    double P = fmax((m->gamma - 1) * (TotE - KinE), m->pFloor);
+    Mx = Mx + dt * dGasM[0]; My = My + dt * dGasM[1]; Mz = Mz + dt * dGasM[2];
+    KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
+    U[DEN] = D; U[MOMX] = Mx; U[MOMY] = My; U[MOMZ] = Mz;
+    U[ENERGY] = KinE + P / (m->gamma - 1);
+}
+
+static inline void vpara(const double *A, const double *nh, double *o) {   /* math.F90:364-369 */
+    double d = dot3(A, nh);
+    o[0] = d * nh[0]; o[1] = d * nh[1]; o[2] = d * nh[2];
+}
+static inline void vperp(const double *A, const double *nh, double *o) {   /* math.F90:371-376 */
+    double p[3]; vpara(A, nh, p);
+    o[0] = A[0] - p[0]; o[1] = A[1] - p[1]; o[2] = A[2] - p[2];
+}
+
+void orc_cell_boris(const orc_model *m, double *U2, const double *U0, const double *B2,
+                    const double *B0, const double *dGasH, const double *dGasM, double Ca, double dt) {
+    /* mhdgroup.F90:337-389 CellBoris (bulk fluid) */
+    double W2[5];
+    orc_cell_c2p(m, U2, W2);
+    double D2 = fmax(W2[DEN], m->dFloor);
+    double P2 = fmax(W2[4], m->pFloor);
+    double D0 = fmax(U0[DEN], m->dFloor);
+    double V0[3] = {U0[MOMX] / D0, U0[MOMY] / D0, U0[MOMZ] / D0};
+    double B1[3] = {0.5 * (B2[0] + B0[0]), 0.5 * (B2[1] + B0[1]), 0.5 * (B2[2] + B0[2])};
+    double alpha = dot3(B1, B1) / (D2 * Ca * Ca);
+    double dRho = dt * dGasH[DEN];
+    double dPGas[3] = {dt * dGasH[MOMX], dt * dGasH[MOMY], dt * dGasH[MOMZ]};
+    double dPMag[3] = {dt * dGasM[0], dt * dGasM[1], dt * dGasM[2]};
+    double pCon = 1.0 / (1.0 + alpha);
+    double bhat1[3], bhat2[3];
+    normvec(B1, bhat1); normvec(B2, bhat2);
+    double dMom[3];
+    if (alpha > TINY) {
+        double a[3], sum[3], b[3], c[3];
+        vpara(dPGas, bhat1, a);
+        for (int d = 0; d < 3; d++) sum[d] = dPGas[d] + dPMag[d];
+        vperp(sum, bhat1, b);
+        vperp(V0, bhat1, c);
+        double s = alpha * pCon * dRho;
+        for (int d = 0; d < 3; d++) dMom[d] = a[d] + pCon * b[d] + s * c[d];
+    } else {
+        for (int d = 0; d < 3; d++) dMom[d] = dPGas[d] + dPMag[d];
+    }
+    double W[5];
+    W[0] = D2;
+    for (int d = 0; d < 3; d++) W[1 + d] = (D0 * V0[d] + dMom[d]) / D2;
+    W[4] = P2;
+    orc_cell_p2c(m, W, U2);
+}
+
+static void e2flux(const orc_grid *g, double *mag, const double *E, double dt) {
+    /* fields.F90:18-51 E2Flux */
+    size_t nc = g->nc, nf = g->nf;
+    const double *Ei = E, *Ej = E + nc, *Ek = E + 2 * nc;
+#pragma omp parallel for collapse(2)
+    for (int k = g->ks; k <= g->ke + 1; k++)
+        for (int j = g->js; j <= g->je + 1; j++)
+            for (int i = g->is; i <= g->ie + 1; i++) {
+                size_t c = CI(g, i, j, k);
+                double dfi = Ek[CI(g, i, j + 1, k)] - Ek[c] - Ej[CI(g, i, j, k + 1)] + Ej[c];
+                double dfj = Ei[CI(g, i, j, k + 1)] - Ei[c] - Ek[CI(g, i + 1, j, k)] + Ek[c];
+                double dfk = Ej[CI(g, i + 1, j, k)] - Ej[c] - Ei[CI(g, i, j + 1, k)] + Ei[c];
+                size_t f = FI(g, i, j, k);
+                mag[f] = mag[f] - dt * dfi;
+                mag[f + nf] = mag[f + nf] - dt * dfj;
+                mag[f + 2 * nf] = mag[f + 2 * nf] - dt * dfk;
+            }
+}
+
+static void copy_state(const orc_grid *g, const orc_state *src, orc_state *dst) {
+    /* mhdgroup.F90:202-257 CopyState */
+    memcpy(dst->Gas, src->Gas, sizeof(double) * 5 * g->nc);
+    memcpy(dst->Bxyz, src->Bxyz, sizeof(double) * 3 * g->nc);
+    memcpy(dst->magFlux, src->magFlux, sizeof(double) * 3 * g->nf);
+    dst->time = src->time;
+}
+
+static void apply_update(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o, double dt,
+                         const double *dGasH, const double *dGasM, const double *E) {
+    /* mhdgroup.F90:79-259 applyUpdate (single fluid, no AB3) */
+    size_t nc = g->nc;
+    copy_state(g, s, o);
+    s->time = s->time + dt;
+#pragma omp parallel for collapse(2)
+    for (int k = g->ks; k <= g->ke; k++)
+        for (int j = g->js; j <= g->je; j++)
+            for (int i = g->is; i <= g->ie; i++) {
+                size_t c = CI(g, i, j, k);
+                for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = s->Gas[c + nc * v] + m->dt * dGasH[c + nc * v];
+            }
+    if (!m->doMHD) return;
+    e2flux(g, s->magFlux, E, dt);
+    orc_bflux2fld(g, s->magFlux, s->Bxyz);
+    int boris = m->doBoris;
+    double Ca = boris ? m->Ca : 0.0;
+#pragma omp parallel for collapse(2)
+    for (int k = g->ks; k <= g->ke; k++)
+        for (int j = g->js; j <= g->je; j++)
+            for (int i = g->is; i <= g->ie; i++) {
+                size_t c = CI(g, i, j, k);
+                double U[5], oU[5], B[3], oB[3], dPg[5], dPm[3];
+                for (int v = 0; v < 5; v++) { U[v] = s->Gas[c + nc * v]; oU[v] = o->Gas[c + nc * v]; dPg[v] = dGasH[c + nc * v]; }
+                for (int v = 0; v < 3; v++) { B[v] = s->Bxyz[c + nc * v]; oB[v] = o->Bxyz[c + nc * v]; dPm[v] = dGasM[c + nc * v]; }
+                if (boris) orc_cell_boris(m, U, oU, B, oB, dPg, dPm, Ca, dt);
+                else orc_cell_maxwell(m, U, dPm, dt);
+                for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = U[v];
+            }
+}
+
+/* ----------------------------------------------------- solver object */
+struct orc_solver {
+    size_t nc, nf;
+    double *gFlx, *mFlx, *Vf, *dGasH, *dGasM;
+    orc_state hf;
+};
+
+orc_solver *orc_solver_new(const orc_grid *g) {
+    /* init.F90:494-516 initSolver */
+    orc_solver *sv = calloc(1, sizeof(orc_solver));
+    sv->nc = g->nc; sv->nf = g->nf;
+    sv->gFlx = calloc(15 * g->nc, sizeof(double));
+    sv->mFlx = calloc(9 * g->nc, sizeof(double));
+    sv->Vf = calloc(3 * g->nc, sizeof(double));
+    sv->dGasH = calloc(5 * g->nc, sizeof(double));
+    sv->dGasM = calloc(3 * g->nc, sizeof(double));
+    sv->hf.Gas = calloc(5 * g->nc, sizeof(double));
+    sv->hf.Bxyz = calloc(3 * g->nc, sizeof(double));
+    sv->hf.Efld = calloc(3 * g->nc, sizeof(double));
+    sv->hf.magFlux = calloc(3 * g->nf, sizeof(double));
+    return sv;
+}
+
+void orc_solver_free(orc_solver *sv) {
+    if (!sv) return;
+    free(sv->gFlx); free(sv->mFlx); free(sv->Vf); free(sv->dGasH); free(sv->dGasM);
+    free(sv->hf.Gas); free(sv->hf.Bxyz); free(sv->hf.Efld); free(sv->hf.magFlux);
+    free(sv);
+}
+
+double *orc_solver_array(orc_solver *sv, int which) {
+    switch (which) {
+    case 0: return sv->hf.Gas;
+    case 1: return sv->hf.magFlux;
+    case 2: return sv->hf.Bxyz;
+    case 3: return sv->Vf;
+    case 4: return sv->gFlx;
+    case 5: return sv->mFlx;
+    case 6: return sv->dGasH;
+    case 7: return sv->dGasM;
+    default: return NULL;
+    }
+}
+
+/* ----------------------------------------------------- advance */
+void orc_advance_mhd(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o,
+                     orc_solver *sv, double dt) {
+    /* mhdgroup.F90:16-76 AdvanceMHD (no gravity, no ring average) */
+    if (fabs(o->time - s->time) < TINY) o->time = s->time - m->dt;
+    orc_predictor(m, g, s, o, &sv->hf, (double)0.5f * dt);
+    if (m->doMHD) orc_calc_efield(m, g, s, &sv->hf, sv->Vf, s->Efld);
+    orc_calc_stress(m, g, s, &sv->hf, sv->gFlx, sv->mFlx, sv->dGasH, sv->dGasM);
+    apply_update(m, g, s, o, dt, sv->dGasH, sv->dGasM, s->Efld);
+}
+
+double orc_cell_dt(const orc_model *m, const orc_grid *g, const orc_state *s, int i, int j, int k) {
+    /* gridutils.F90:49-106 CellDT (single fluid, no resistivity) */
+    size_t nc = g->nc, c = CI(g, i, j, k);
+    double rho = s->Gas[c];
+    double Vx = s->Gas[c + nc * MOMX] / rho, Vy = s->Gas[c + nc * MOMY] / rho, Vz = s->Gas[c + nc * MOMZ] / rho;
+    double MagV = sqrt(sq(Vx) + sq(Vy) + sq(Vz));
+    double ke = 0.5 * rho * sq(MagV);
+    double e = s->Gas[c + nc * ENERGY] - ke;
+    double P = (m->gamma - 1) * e;
+    double Cs = sqrt(m->gamma * P / rho);
+    double Valf = 0.0;
+    if (m->doMHD) {
+        double Bx = s->Bxyz[c], By = s->Bxyz[c + nc], Bz = s->Bxyz[c + 2 * nc];
+        double MagB = sqrt(sq(Bx) + sq(By) + sq(Bz));
+        Valf = MagB / sqrt(rho);
+        if (m->doBoris) Valf = m->Ca * Valf / sqrt(m->Ca * m->Ca + Valf * Valf);
+    }
+    double vCFL = MagV + sqrt(sq(Cs) + sq(Valf)) + 0.0;
+    double dl = fmin(fmin(g->di[c], g->dj[c]), g->dk[c]);
+    return m->CFL * dl / vCFL;
+}
+
+double orc_calc_dt(orc_model *m, const orc_grid *g, const orc_state *s) {
+    /* step.F90:58-168 CalcDT; failure paths return a status instead of stopping */
+    m->status = 0;
+    if (m->fixedTimestep) return m->dt;
+    double dtOld = m->dt;
+    double dtMin = HUGEV;
+#pragma omp parallel for collapse(2) reduction(min:dtMin)
+    for (int k = g->ksDT; k <= g->keDT; k++)
+        for (int j = g->jsDT; j <= g->jeDT; j++)
+            for (int i = g->isDT; i <= g->ieDT; i++) {
+                double d = orc_cell_dt(m, g, s, i, j, k);
+                dtMin = fmin(d, dtMin);
+            }
+    int disaster = (dtMin == HUGEV), bad = 0;
+    double dt = dtMin;
+    m->status = 0;
+    if (m->ts > 0) {
+        if (dtOld / dtMin >= 10) bad = 1;
+        if (m->dt0 > TINY && dtMin / m->dt0 <= m->limDT0) bad = 1;
+        if (m->doCPR && m->dt0 > TINY && dtMin / m->dt0 <= m->limCPR) bad = 1;
+        if (dtMin <= TINY) disaster = 1;
+        m->status = disaster ? 2 : (bad ? 1 : 0);
+    }
+    if ((m->t + dt) > m->tFin) dt = fmax(m->tFin - m->t, TINY);
+    if (m->dt0 > TINY && dt > (10.0 * m->dt0)) dt = 10.0 * m->dt0;
+    return dt;
+}
+
+/* ----------------------------------------------------- periodic BCs */
+void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
+    /* bcs.F90:279-450 periodic_{i,o}bc{I,J,K}, applied in the order of
+     * step.F90:27-53 (INI, OUTI, INJ, OUTJ, INK, OUTK) */
+    size_t nc = g->nc, nf = g->nf;
+    int nG = g->nG;
+    double *Gas = s->Gas, *B = s->Bxyz, *mag = s->magFlux;
+#define ISCC(i, j, k) ((k) <= g->keg && (j) <= g->jeg && (i) <= g->ieg && (k) >= g->ksg && (j) >= g->jsg && (i) >= g->isg)
+#define CPC(di, dj, dk, si, sj, sk) do { size_t a_ = CI(g, di, dj, dk), b_ = CI(g, si, sj, sk); \
+        for (int v = 0; v < 5; v++) Gas[a_ + nc * v] = Gas[b_ + nc * v]; \
+        for (int v = 0; v < 3; v++) B[a_ + nc * v] = B[b_ + nc * v]; } while (0)
+#define CPF(di, dj, dk, si, sj, sk, d) do { mag[FI(g, di, dj, dk) + nf * (d)] = mag[FI(g, si, sj, sk) + nf * (d)]; } while (0)
+    /* I */
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int n = 1; n <= nG; n++) {
+                int ig = g->is - n, is_ = g->ie - n + 1;
+                if (ISCC(ig, j, k)) CPC(ig, j, k, is_, j, k);
+                for (int d = 0; d < 3; d++) CPF(ig, j, k, is_, j, k, d);
+            }
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int n = 1; n <= nG; n++) {
+                int ig = g->ie + n;
+                if (ISCC(ig, j, k)) CPC(ig, j, k, g->is + n - 1, j, k);
+                CPF(ig, j, k, g->is + n - 1, j, k, JD);
+                CPF(ig, j, k, g->is + n - 1, j, k, KD);
+                CPF(ig + 1, j, k, g->is + n, j, k, ID);
+            }
+    /* J */
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int i = g->isg; i <= g->ieg + 1; i++)
+            for (int n = 1; n <= nG; n++) {
+                int jg = g->js - n, js_ = g->je - n + 1;
+                if (ISCC(i, jg, k)) CPC(i, jg, k, i, js_, k);
+                for (int d = 0; d < 3; d++) CPF(i, jg, k, i, js_, k, d);
+            }
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int i = g->isg; i <= g->ieg + 1; i++)
+            for (int n = 1; n <= nG; n++) {
+                int jg = g->je + n;
+                if (ISCC(i, jg, k)) CPC(i, jg, k, i, g->js + n - 1, k);
+                CPF(i, jg, k, i, g->js + n - 1, k, ID);
+                CPF(i, jg, k, i, g->js + n - 1, k, KD);
+                CPF(i, jg + 1, k, i, g->js + n, k, JD);
+            }
+    /* K */
+    if (m->do25D) {
+        for (int n = 1; n <= nG; n++) {
+            for (int kk = 0; kk < 2; kk++) {
+                int kg = kk == 0 ? g->ks - n : g->ks + n;
+                for (int j = g->jsg; j <= g->jeg; j++)
+                    for (int i = g->isg; i <= g->ieg; i++) CPC(i, j, kg, i, j, g->ks);
+                for (int j = g->jsg; j <= g->jeg + 1; j++)
+                    for (int i = g->isg; i <= g->ieg + 1; i++)
+                        for (int d = 0; d < 3; d++) CPF(i, j, kg, i, j, g->ks, d);
+            }
+        }
+    } else {
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int i = g->isg; i <= g->ieg + 1; i++)
+                for (int n = 1; n <= nG; n++) {
+                    int kg = g->ks - n, ks_ = g->ke - n + 1;
+                    if (ISCC(i, j, kg)) CPC(i, j, kg, i, j, ks_);
+                    for (int d = 0; d < 3; d++) CPF(i, j, kg, i, j, ks_, d);
+                }
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int i = g->isg; i <= g->ieg + 1; i++)
+                for (int n = 1; n <= nG; n++) {
+                    int kg = g->ke + n;
+                    if (ISCC(i, j, kg)) CPC(i, j, kg, i, j, g->ks + n - 1);
+                    CPF(i, j, kg, i, j, g->ks + n - 1, ID);
+                    CPF(i, j, kg, i, j, g->ks + n - 1, JD);
+                    CPF(i, j, kg + 1, i, j, g->ks + n, KD);
+                }
+    }
+#undef ISCC
+#undef CPC
+#undef CPF
+}
+
+/* ----------------------------------------------------- step / app glue */
+void orc_step(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o, orc_solver *sv) {
+    /* gamapp.F90:73-116 stepGamera + UpdateStateData */
+    orc_advance_mhd(m, g, s, o, sv, m->dt);
+    m->ts = m->ts + 1;
+    m->t = m->t + m->dt;
+    m->dt = orc_calc_dt(m, g, s);
+    orc_enforce_bcs(m, g, s);
+    orc_bflux2fld(g, s->magFlux, s->Bxyz);
+}
+
+int orc_run_steps(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o,
+                  orc_solver *sv, int nsteps, int dt0_after) {
+    /* gamerax-style loop: dt0 is captured at the first console output after
+     * ts > 0 (output.F90:57-59); here that output happens after step dt0_after */
+    for (int n = 0; n < nsteps; n++) {
+        orc_step(m, g, s, o, sv);
+        if (m->status) return m->status;
+        if (dt0_after > 0 && m->ts >= dt0_after && m->dt0 < TINY) m->dt0 = m->dt;
+    }
+    return 0;
+}
+
+void orc_hatch_finish(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o) {
+    /* init.F90:204-244: first Bxyz, oState copy, initial BCs, first dt */
+    size_t nc = g->nc, nf = g->nf;
+    m->dt0 = 0.0;
+    if (m->doMHD) {
+        orc_bflux2fld(g, s->magFlux, s->Bxyz);
+        memcpy(o->magFlux, s->magFlux, sizeof(double) * 3 * nf);
+        memcpy(o->Bxyz, s->Bxyz, sizeof(double) * 3 * nc);
+    }
+    orc_enforce_bcs(m, g, s);
+    copy_state(g, s, o);
+    m->dt = orc_calc_dt(m, g, s);
+    o->time = s->time - m->dt;
+}
+
+double orc_divb(const orc_grid *g, const orc_state *s) {
+    /* gridutils.F90:717-770 DivB (total, no background) */
+    double tot = 0.0;
+    size_t nf = g->nf;
+    const double *mag = s->magFlux;
+#pragma omp parallel for collapse(2) reduction(+:tot)
+    for (int k = g->ks; k <= g->ke; k++)
+        for (int j = g->js; j <= g->je; j++)
+            for (int i = g->is; i <= g->ie; i++) {
+                double dFi = mag[FI(g, i, j, k)] - mag[FI(g, i + 1, j, k)];
+                double dFj = mag[FI(g, i, j, k) + nf] - mag[FI(g, i, j + 1, k) + nf];
+                double dFk = mag[FI(g, i, j, k) + 2 * nf] - mag[FI(g, i, j, k + 1) + 2 * nf];
+                tot += fabs(dFi + dFj + dFk + 0.0 + 0.0 + 0.0);
+            }
+    return tot;
+}
+
+/* ===================================================== setup producers */
+void orc_gen_corners_cart(const orc_grid *g, const double bds[6], int do25D) {
+    /* init.F90:669-731 genGridXML, Cartesian corners */
+    (void)do25D;
+    size_t nf = g->nf;
+    double dx = (bds[1] - bds[0]) / g->Nip, dy = (bds[3] - bds[2]) / g->Njp, dz = (bds[5] - bds[4]) / g->Nkp;
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int i = g->isg; i <= g->ieg + 1; i++) {
+                size_t f = FI(g, i, j, k);
+                g->xyz[f] = bds[0] + (i - 1) * dx;
+                g->xyz[f + nf] = bds[2] + (j - 1) * dy;
+                g->xyz[f + 2 * nf] = bds[4] + (k - 1) * dz;
+            }
+}
+
+void orc_gen_corners_refwarp(const orc_grid *g, const double bds[6], double w0, double R, int Nw, int do25D) {
+    /* init.F90:733-768 doWarp */
+    (void)R;
+    size_t nf = g->nf;
+    orc_gen_corners_cart(g, bds, do25D);
+    double Lx = bds[1] - bds[0], Ly = bds[3] - bds[2], Lz = bds[5] - bds[4];
+    double Ax = Nw * PI / Lx, Ay = Nw * PI / Ly, Az = Nw * PI / Lz;
+    for (size_t f = 0; f < nf; f++) {
+        double xp = g->xyz[f] - bds[0], yp = g->xyz[f + nf] - bds[2], zp = g->xyz[f + 2 * nf] - bds[4];
+        double dsp = w0 * sin(Ax * xp) * sin(Ay * yp);
+        g->xyz[f] = g->xyz[f] + dsp;
+        g->xyz[f + nf] = g->xyz[f + nf] - dsp;
+        if (!do25D) { dsp = w0 * sin(Az * zp); g->xyz[f + 2 * nf] = g->xyz[f + 2 * nf] + dsp; }
+    }
+}
+
+void orc_gen_corners_warp3(const orc_grid *g, const double bds[6], double amp) {
+    /* Non-orthogonal periodic warp x = xi + a sin(2 pi xi') sin(2 pi eta') sin(2 pi zeta')
+     * (same displacement added to x, y and z; primes: coordinates scaled to [0,1]) */
+    size_t nf = g->nf;
+    orc_gen_corners_cart(g, bds, 0);
+    double Lx = bds[1] - bds[0], Ly = bds[3] - bds[2], Lz = bds[5] - bds[4];
+    for (size_t f = 0; f < nf; f++) {
+        double xi = (g->xyz[f] - bds[0]) / Lx, eta = (g->xyz[f + nf] - bds[2]) / Ly, zeta = (g->xyz[f + 2 * nf] - bds[4]) / Lz;
+        double s = amp * sin(2 * PI * xi) * sin(2 * PI * eta) * sin(2 * PI * zeta);
+        g->xyz[f] = g->xyz[f] + s * Lx;
+        g->xyz[f + nf] = g->xyz[f + nf] + s * Ly;
+        g->xyz[f + 2 * nf] = g->xyz[f + 2 * nf] + s * Lz;
+    }
+}
+
+static inline void xyzv(const orc_grid *g, int i, int j, int k, double *p) {
+    size_t f = FI(g, i, j, k);
+    p[0] = g->xyz[f]; p[1] = g->xyz[f + g->nf]; p[2] = g->xyz[f + 2 * g->nf];
+}
+
+/* quadrature.F90:159-221 GaussianFaceIntegral for A = 1 (area) and A = r (centre) */
+static void gauss_face(const double *f0, const double *f1, const double *f2, const double *f3,
+                       double *area_out, double *ctr_out) {
+    double dx[3], dy[3], dz[3];      /* quadrature.F90:373-389 faceDeltas */
+    dx[0] = f1[0] - f0[0]; dx[1] = f2[0] - f0[0]; dx[2] = f3[0] + f0[0] - f2[0] - f1[0];
+    dy[0] = f1[1] - f0[1]; dy[1] = f2[1] - f0[1]; dy[2] = f3[1] + f0[1] - f2[1] - f1[1];
+    dz[0] = f1[2] - f0[2]; dz[1] = f2[2] - f0[2]; dz[2] = f3[2] + f0[2] - f2[2] - f1[2];
+    double A = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int a = 0; a < 12; a++)
+        for (int b = 0; b < 12; b++) {
+            double eta = 0.5 * (1 + gA[a]), psi = 0.5 * (1 + gA[b]);
+            double w = gW[a] * gW[b];
+            double xp = f0[0] + dx[0] * eta + dx[1] * psi + dx[2] * eta * psi;
+            double yp = f0[1] + dy[0] * eta + dy[1] * psi + dy[2] * eta * psi;
+            double zp = f0[2] + dz[0] * eta + dz[1] * psi + dz[2] * eta * psi;
+            double eV[3] = {dx[0] + dx[2] * psi, dy[0] + dy[2] * psi, dz[0] + dz[2] * psi};
+            double pV[3] = {dx[1] + dx[2] * eta, dy[1] + dy[2] * eta, dz[1] + dz[2] * eta};
+            double cr[3]; cross3(eV, pV, cr);
+            double area = 0.25 * nrm3(cr) * w;
+            A = A + area * 1.0;
+            cx = cx + area * xp; cy = cy + area * yp; cz = cz + area * zp;
+        }
+    *area_out = A; ctr_out[0] = cx; ctr_out[1] = cy; ctr_out[2] = cz;
+}
+
+static inline double detj(const double J[3][3]) {   /* math.F90:470-478 */
+    return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1])
+         - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+         + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+}
+
+typedef void (*gasic_fn)(const void *ctx, double x, double y, double z, double *W5);
+
+/* quadrature.F90:42-100 GaussianVolumeIntegral */
+static void gauss_volume(const double C[8][3], gasic_fn W, const void *ctx, double *out5) {
+    static const int iS[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    static const int jS[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    static const int kS[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    for (int v = 0; v < 5; v++) out5[v] = 0.0;
+    for (int a = 0; a < 12; a++)
+        for (int b = 0; b < 12; b++)
+            for (int c = 0; c < 12; c++) {
+                double wijk = gW[a] * gW[b] * gW[c];
+                double iScl[8], jScl[8], kScl[8];
+                for (int n = 0; n < 8; n++) {
+                    iScl[n] = iS[n] < 0 ? 1 - gA[a] : 1 + gA[a];
+                    jScl[n] = jS[n] < 0 ? 1 - gA[b] : 1 + gA[b];
+                    kScl[n] = kS[n] < 0 ? 1 - gA[c] : 1 + gA[c];
+                }
+                double xp[3], J[3][3];
+                for (int x = 0; x < 3; x++) {
+                    double s = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                    for (int n = 0; n < 8; n++) {
+                        s  += C[n][x] * (iScl[n] * jScl[n] * kScl[n]);
+                        s1 += C[n][x] * ((double)iS[n] * jScl[n] * kScl[n]);
+                        s2 += C[n][x] * (iScl[n] * (double)jS[n] * kScl[n]);
+                        s3 += C[n][x] * (iScl[n] * jScl[n] * (double)kS[n]);
+                    }
+                    xp[x] = 0.125 * s; J[x][0] = 0.125 * s1; J[x][1] = 0.125 * s2; J[x][2] = 0.125 * s3;
+                }
+                double dJ = fabs(detj(J));
+                double Wp[5];
+                W(ctx, xp[0], xp[1], xp[2], Wp);
+                for (int v = 0; v < 5; v++) out5[v] = out5[v] + dJ * wijk * Wp[v];
+            }
+}
+
+static void cell_dv(const void *ctx, double x, double y, double z, double *W) {   /* metric.F90:298-307 */
+    (void)ctx; W[0] = 1; W[1] = x; W[2] = y; W[3] = z; W[4] = 0;
+}
+
+static inline void cell_coords(const orc_grid *g, int i, int j, int k, double C[8][3]) {  /* metric.F90:133-148 */
+    xyzv(g, i, j, k, C[0]); xyzv(g, i + 1, j, k, C[1]); xyzv(g, i + 1, j + 1, k, C[2]); xyzv(g, i, j + 1, k, C[3]);
+    xyzv(g, i, j, k + 1, C[4]); xyzv(g, i + 1, j, k + 1, C[5]); xyzv(g, i + 1, j + 1, k + 1, C[6]); xyzv(g, i, j + 1, k + 1, C[7]);
+}
+
+void orc_corners2grid(const orc_grid *g) {
+    /* init.F90:801-1080 Corners2Grid (doQuadFT = F, no ring fix) + ebGeom 1083-1145 */
+    init_weights();
+    size_t nf = g->nf, nc = g->nc;
+    for (size_t f = 0; f < 3 * nf; f++) { g->face[f] = 1.0; g->edge[f] = 1.0; }
+    memset(g->xfc, 0, sizeof(double) * 9 * nf);
+    memset(g->Tf, 0, sizeof(double) * 27 * nf);
+    memset(g->Te, 0, sizeof(double) * 27 * nf);
+    for (int d = 0; d < 3; d++) {
+        int DI = d == ID, DJ = d == JD, DK = d == KD;
+#pragma omp parallel for collapse(2)
+        for (int k = g->ksg; k <= g->keg + DK; k++)
+            for (int j = g->jsg; j <= g->jeg + DJ; j++)
+                for (int i = g->isg; i <= g->ieg + DI; i++) {
+                    double f0[3], f1[3], f2[3], f3[3];     /* metric.F90:79-108 faceCoords */
+                    xyzv(g, i, j, k, f0);
+                    if (d == ID) { xyzv(g, i, j + 1, k, f1); xyzv(g, i, j, k + 1, f2); xyzv(g, i, j + 1, k + 1, f3); }
+                    else if (d == JD) { xyzv(g, i, j, k + 1, f1); xyzv(g, i + 1, j, k, f2); xyzv(g, i + 1, j, k + 1, f3); }
+                    else { xyzv(g, i + 1, j, k, f1); xyzv(g, i, j + 1, k, f2); xyzv(g, i + 1, j + 1, k, f3); }
+                    double fA, ctr[3];
+                    gauss_face(f0, f1, f2, f3, &fA, ctr);
+                    size_t f = FI(g, i, j, k);
+                    for (int x = 0; x < 3; x++) g->xfc[f + nf * (x + 3 * d)] = ctr[x] / fA;
+                    g->face[f + nf * d] = fA;
+                    double a[3], b[3], c[3], e[3], dT1[3], dT2[3];
+                    if (d == ID) {
+                        xyzv(g, i, j + 1, k + 1, a); xyzv(g, i, j, k + 1, b); xyzv(g, i, j + 1, k, c); xyzv(g, i, j, k, e);
+                        for (int x = 0; x < 3; x++) dT1[x] = 0.5 * (a[x] - b[x]) + 0.5 * (c[x] - e[x]);
+                        xyzv(g, i, j + 1, k + 1, a); xyzv(g, i, j + 1, k, b); xyzv(g, i, j, k + 1, c); xyzv(g, i, j, k, e);
+                        for (int x = 0; x < 3; x++) dT2[x] = 0.5 * (a[x] - b[x]) + 0.5 * (c[x] - e[x]);
+                    } else if (d == JD) {
+                        xyzv(g, i + 1, j, k + 1, a); xyzv(g, i + 1, j, k, b); xyzv(g, i, j, k + 1, c); xyzv(g, i, j, k, e);
+                        for (int x = 0; x < 3; x++) dT1[x] = 0.5 * (a[x] - b[x]) + 0.5 * (c[x] - e[x]);
+                        xyzv(g, i + 1, j, k + 1, a); xyzv(g, i, j, k + 1, b); xyzv(g, i + 1, j, k, c); xyzv(g, i, j, k, e);
+                        for (int x = 0; x < 3; x++) dT2[x] = 0.5 * (a[x] - b[x]) + 0.5 * (c[x] - e[x]);
+                    } else {
+                        xyzv(g, i + 1, j + 1, k, a); xyzv(g, i, j + 1, k, b); xyzv(g, i + 1, j, k, c); xyzv(g, i, j, k, e);
+                        for (int x = 0; x < 3; x++) dT1[x] = 0.5 * (a[x] - b[x]) + 0.5 * (c[x] - e[x]);
+                        xyzv(g, i + 1, j + 1, k, a); xyzv(g, i + 1, j, k, b); xyzv(g, i, j + 1, k, c); xyzv(g, i, j, k, e);
+                        for (int x = 0; x < 3; x++) dT2[x] = 0.5 * (a[x] - b[x]) + 0.5 * (c[x] - e[x]);
+                    }
+                    double cr[3], eN[3], eT1[3], eT2[3];
+                    cross3(dT1, dT2, cr);
+                    double ncr = nrm3(cr);
+                    for (int x = 0; x < 3; x++) eN[x] = cr[x] / ncr;
+                    double n2 = nrm3(dT2);
+                    for (int x = 0; x < 3; x++) eT2[x] = dT2[x] / n2;
+                    cross3(eT2, eN, cr);
+                    ncr = nrm3(cr);
+                    for (int x = 0; x < 3; x++) eT1[x] = cr[x] / ncr;
+                    for (int x = 0; x < 3; x++) {
+                        g->Tf[f + nf * (NORMX + x + 9 * d)] = eN[x];
+                        g->Tf[f + nf * (TAN1X + x + 9 * d)] = eT1[x];
+                        g->Tf[f + nf * (TAN2X + x + 9 * d)] = eT2[x];
+                    }
+                }
+    }
+#pragma omp parallel for collapse(2)
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int j = g->jsg; j <= g->jeg; j++)
+            for (int i = g->isg; i <= g->ieg; i++) {
+                double C[8][3], W[5];
+                cell_coords(g, i, j, k, C);
+                gauss_volume(C, cell_dv, NULL, W);
+                size_t c = CI(g, i, j, k);
+                g->volume[c] = W[0];
+                for (int x = 0; x < 3; x++) g->xyzcc[c + nc * x] = W[1 + x] / W[0];
+            }
+#pragma omp parallel for collapse(2)
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int j = g->jsg; j <= g->jeg; j++)
+            for (int i = g->isg; i <= g->ieg; i++) {
+                size_t c = CI(g, i, j, k);
+                double fA = fmax(g->face[FI(g, i + 1, j, k)], g->face[FI(g, i, j, k)]);
+                g->di[c] = g->volume[c] / fA;
+                fA = fmax(g->face[FI(g, i, j + 1, k) + nf], g->face[FI(g, i, j, k) + nf]);
+                g->dj[c] = g->volume[c] / fA;
+                fA = fmax(g->face[FI(g, i, j, k + 1) + 2 * nf], g->face[FI(g, i, j, k) + 2 * nf]);
+                g->dk[c] = g->volume[c] / fA;
+            }
+    /* edge triads init.F90:972-1048 */
+    for (int d = 0; d < 3; d++) {
+        int is = g->isg, ie = g->ieg + 1, js = g->jsg, je = g->jeg + 1, ks = g->ksg, ke = g->keg + 1;
+        if (d == ID) { ie = g->ieg; ks = g->ksg + 1; ke = g->keg; }
+        else if (d == JD) { je = g->jeg; is = g->isg + 1; ie = g->ieg; }
+        else { ke = g->keg; js = g->jsg + 1; je = g->jeg; }
+#pragma omp parallel for collapse(2)
+        for (int k = ks; k <= ke; k++)
+            for (int j = js; j <= je; j++)
+                for (int i = is; i <= ie; i++) {
+                    double p0[3], p1[3], a[3], b[3], c[3], e[3], dEdge[3], eAvg[3];
+                    xyzv(g, i, j, k, p0);
+                    if (d == ID) {
+                        xyzv(g, i + 1, j, k, p1);
+                        xyzv(g, i, j, k + 1, a); xyzv(g, i, j, k - 1, b); xyzv(g, i + 1, j, k + 1, c); xyzv(g, i + 1, j, k - 1, e);
+                    } else if (d == JD) {
+                        xyzv(g, i, j + 1, k, p1);
+                        xyzv(g, i + 1, j, k, a); xyzv(g, i - 1, j, k, b); xyzv(g, i + 1, j + 1, k, c); xyzv(g, i - 1, j + 1, k, e);
+                    } else {
+                        xyzv(g, i, j, k + 1, p1);
+                        xyzv(g, i, j + 1, k, a); xyzv(g, i, j - 1, k, b); xyzv(g, i, j + 1, k + 1, c); xyzv(g, i, j - 1, k + 1, e);
+                    }
+                    for (int x = 0; x < 3; x++) {
+                        dEdge[x] = p1[x] - p0[x];
+                        eAvg[x] = 0.5 * (a[x] - b[x] + c[x] - e[x]);
+                    }
+                    double len = nrm3(dEdge), eN[3], eT1[3], eT2[3], cr[3];
+                    for (int x = 0; x < 3; x++) eN[x] = dEdge[x] / len;
+                    cross3(eAvg, eN, cr);
+                    double n1 = nrm3(cr);
+                    for (int x = 0; x < 3; x++) eT1[x] = cr[x] / n1;
+                    cross3(eN, eT1, cr);
+                    double n2 = nrm3(cr);
+                    for (int x = 0; x < 3; x++) eT2[x] = cr[x] / n2;
+                    size_t f = FI(g, i, j, k);
+                    g->edge[f + nf * d] = len;
+                    for (int x = 0; x < 3; x++) {
+                        g->Te[f + nf * (NORMX + x + 9 * d)] = eN[x];
+                        g->Te[f + nf * (TAN1X + x + 9 * d)] = eT1[x];
+                        g->Te[f + nf * (TAN2X + x + 9 * d)] = eT2[x];
+                    }
+                }
+    }
+    /* Teb: ebGeom init.F90:1056-1073,1083-1145 */
+    for (size_t f = 0; f < 12 * nf; f++) g->Teb[f] = 1.0;
+    for (int dN = 0; dN < 3; dN++) {
+        int T1 = (dN + 1) % 3, T2 = (dN + 2) % 3;
+        for (int pass = 0; pass < 2; pass++) {
+            int dT1 = pass == 0 ? T1 : T2, dT2 = pass == 0 ? T2 : T1;
+            int slot = pass == 0 ? XNQI : XNQJ;
+#pragma omp parallel for collapse(2)
+            for (int k = g->ks; k <= g->ke + 1; k++)
+                for (int j = g->js; j <= g->je + 1; j++)
+                    for (int i = g->is; i <= g->ie + 1; i++) {
+                        double fArB[8], Nx[3][8];
+                        for (int q = 0; q < 8; q++) {
+                            int a, b, c; STEP(dT2, q - 4, i, j, k, a, b, c);
+                            size_t ff = FI(g, a, b, c);
+                            fArB[q] = g->face[ff + nf * dT1];
+                            for (int x = 0; x < 3; x++) Nx[x][q] = g->Tf[ff + nf * (NORMX + x + 9 * dT1)];
+                        }
+                        double fAi = 0.0;
+                        for (int q = 0; q < 8; q++) fAi += W6[q] * fArB[q];
+                        double Nec[3];
+                        for (int x = 0; x < 3; x++) {
+                            double s = 0.0;
+                            for (int q = 0; q < 8; q++) s += (Nx[x][q] * fArB[q]) * W6[q];
+                            Nec[x] = s / fAi;
+                        }
+                        size_t f = FI(g, i, j, k);
+                        double t1[3], t2[3];
+                        for (int x = 0; x < 3; x++) {
+                            t1[x] = g->Te[f + nf * (TAN1X + x + 9 * dN)];
+                            t2[x] = g->Te[f + nf * (TAN2X + x + 9 * dN)];
+                        }
+                        double dxT = dot3(Nec, t1), dyT = dot3(Nec, t2);
+                        double dScl = sqrt(sq(dxT) + sq(dyT));
+                        if (dScl <= TINY) dScl = 1.0;
+                        g->Teb[f + nf * (slot + 4 * dN)] = dxT / dScl;
+                        g->Teb[f + nf * (slot + 1 + 4 * dN)] = dyT / dScl;
+                    }
+        }
+    }
+}
+
+/* ----------------------------------------------------- initial conditions */
+void orc_ic_ot(const orc_model *m, const orc_grid *g, orc_state *s,
+               double rho0, double p0, double b0, double kscale) {
+    /* prob.F90:74-119 initOT2D with the wavenumber factor kscale (2 pi on the
+     * unit square; 1 on [0, 2 pi]^2).  Written over is..ie+1 like the reference. */
+    size_t nc = g->nc, nf = g->nf;
+    for (int k = g->ks; k <= g->ke + 1; k++)
+        for (int j = g->js; j <= g->je + 1; j++)
+            for (int i = g->is; i <= g->ie + 1; i++) {
+                size_t c = CI(g, i, j, k), f = FI(g, i, j, k);
+                double xc = g->xyzcc[c], yc = g->xyzcc[c + nc];
+                s->Gas[c] = rho0;
+                double Vx = -sin(kscale * yc), Vy = sin(kscale * xc), Vz = 0.0;
+                double KinE = 0.5 * rho0 * (sq(Vx) + sq(Vy) + sq(Vz));
+                s->Gas[c + nc * MOMX] = Vx * rho0;
+                s->Gas[c + nc * MOMY] = Vy * rho0;
+                s->Gas[c + nc * MOMZ] = Vz * rho0;
+                s->Gas[c + nc * ENERGY] = KinE + p0 / (m->gamma - 1);
+                s->magFlux[f] = (-b0 * sin(kscale * yc)) * g->face[f];
+                s->magFlux[f + nf] = (b0 * sin((2 * kscale) * xc)) * g->face[f + nf];
+                s->magFlux[f + 2 * nf] = 0.0;
+            }
+}
+
+typedef struct { double x0, y0, z0, D0, P0, pRat, dRat, rC; } bw_ctx;
+static void gasic_bw(const void *cv, double x, double y, double z, double *W) {   /* prob.F90:280-297 */
+    const bw_ctx *c = (const bw_ctx *)cv;
+    double R = sqrt(sq(x - c->x0) + sq(y - c->y0) + sq(z - c->z0));
+    double D = c->D0, P = c->P0;
+    if (R <= c->rC) { D = c->dRat * D; P = c->pRat * P; }
+    W[0] = D; W[1] = 0.0; W[2] = 0.0; W[3] = 0.0; W[4] = P;
+}
+
+void orc_ic_bw(const orc_model *m, const orc_grid *g, orc_state *s,
+               double x0, double y0, double z0, double D0, double P0,
+               double pRat, double dRat, double rC, double B0) {
+    /* prob.F90:190-298 initBW (doBz = F, no background), via GasIC2State
+     * (gridutils.F90:144-189) and VectorPot2Flux (gridutils.F90:193-237) */
+    size_t nc = g->nc, nf = g->nf;
+    bw_ctx ctx = {x0, y0, z0, D0, P0, pRat, dRat, rC};
+#pragma omp parallel for collapse(2)
+    for (int k = g->ks; k <= g->ke; k++)
+        for (int j = g->js; j <= g->je; j++)
+            for (int i = g->is; i <= g->ie; i++) {
+                double C[8][3], W[5];
+                cell_coords(g, i, j, k, C);
+                size_t c = CI(g, i, j, k);
+                double dV = g->volume[c];
+                gauss_volume(C, gasic_bw, &ctx, W);
+                for (int v = 0; v < 5; v++) W[v] = W[v] / dV;
+                double D = W[0], Mx = D * W[1], My = D * W[2], Mz = D * W[3], P = W[4];
+                double KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
+                double IntE = P / (m->gamma - 1.0);
+                s->Gas[c] = D; s->Gas[c + nc] = Mx; s->Gas[c + 2 * nc] = My; s->Gas[c + 3 * nc] = Mz;
+                s->Gas[c + 4 * nc] = IntE + KinE;
+            }
+    if (!m->doMHD) return;
+    double bScl = B0 / (double)sqrtf(2.0f), bSclz = 0.0;
+    double *lA = calloc(3 * nc, sizeof(double));
+#pragma omp parallel for collapse(2)
+    for (int k = g->ksg; k <= g->keg - 1; k++)
+        for (int j = g->jsg; j <= g->jeg - 1; j++)
+            for (int i = g->isg; i <= g->ieg - 1; i++) {
+                double e0[3], eX[3][3];
+                xyzv(g, i, j, k, e0);
+                xyzv(g, i + 1, j, k, eX[0]); xyzv(g, i, j + 1, k, eX[1]); xyzv(g, i, j, k + 1, eX[2]);
+                for (int d = 0; d < 3; d++) {
+                    /* GaussianEdgeIntegral(e1 = eX, e2 = e0) quadrature.F90:345-370 */
+                    const double *e1 = eX[d], *e2 = e0;
+                    double dR2[3], xBar[3], eInt[3] = {0, 0, 0};
+                    for (int x = 0; x < 3; x++) { dR2[x] = 0.5 * (e2[x] - e1[x]); xBar[x] = 0.5 * (e2[x] + e1[x]); }
+                    for (int q = 0; q < 12; q++) {
+                        double xp[3];
+                        for (int x = 0; x < 3; x++) xp[x] = xBar[x] + gA[q] * dR2[x];
+                        double Ap[3] = {0.0, bSclz * xp[0], bScl * (xp[1] - xp[0])};   /* prob.F90:271-278 */
+                        for (int x = 0; x < 3; x++) eInt[x] = eInt[x] + 0.5 * gW[q] * Ap[x];
+                    }
+                    double dl[3] = {e1[0] - e2[0], e1[1] - e2[1], e1[2] - e2[2]};
+                    lA[CI(g, i, j, k) + nc * d] = dot3(eInt, dl);
+                }
+            }
+#pragma omp parallel for collapse(2)
+    for (int k = g->ksg; k <= g->keg - 1; k++)
+        for (int j = g->jsg; j <= g->jeg - 1; j++)
+            for (int i = g->isg; i <= g->ieg - 1; i++) {
+                size_t c = CI(g, i, j, k), f = FI(g, i, j, k);
+                const double *LI = lA, *LJ = lA + nc, *LK = lA + 2 * nc;
+                s->magFlux[f] = LJ[c] - LJ[CI(g, i, j, k + 1)] + LK[CI(g, i, j + 1, k)] - LK[c];
+                s->magFlux[f + nf] = -(LI[c] - LI[CI(g, i, j, k + 1)] + LK[CI(g, i + 1, j, k)] - LK[c]);
+                s->magFlux[f + 2 * nf] = LI[c] - LI[CI(g, i, j + 1, k)] + LJ[CI(g, i + 1, j, k)] - LJ[c];
+            }
+    free(lA);
+}

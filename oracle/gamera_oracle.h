@@ -1,0 +1,125 @@
+/*
+ * gamera_oracle.h -- CPU restatement of the GAMERA finite-volume MHD step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP step in
+ * kaiju_amd/; only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it.  The product path never calls into it.
+ *
+ * Arrays use the reference layout: Fortran column-major, i fastest, one
+ * contiguous slab per component, ghost-inclusive index ranges
+ *   cell arrays  (isg:ieg, jsg:jeg, ksg:keg, ncomp)        "G"
+ *   face arrays  (isg:ieg+1, jsg:jeg+1, ksg:keg+1, ncomp)   "F"
+ * with isg = 1-nG, ieg = Nip+nG (reference base/types/gamtypes.F90:163-245).
+ */
+#ifndef GAMERA_ORACLE_H
+#define GAMERA_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_8CENT = 0, ORC_7UP = 1 };
+
+/* Model_T subset (reference gamtypes.F90:71-146) */
+typedef struct {
+    double gamma, CFL, pdmb, Vd0, cHogH, cHogM, Ca, dFloor, pFloor;
+    double t, tFin, dt, dt0, limDT0, limCPR;
+    int ts, rmeth, doLFMLim, doBoris, doMHD, do25D, doCPR, fixedTimestep;
+    int status;   /* CalcDT outcome: 0 ok, 1 bad (dt drop), 2 disaster */
+} orc_model;
+
+/* Grid_T subset (reference gamtypes.F90:163-245) */
+typedef struct {
+    int Nip, Njp, Nkp, nG;
+    /* derived by orc_grid_derive */
+    int Ni, Nj, Nk;
+    int is, ie, js, je, ks, ke;
+    int isg, ieg, jsg, jeg, ksg, keg;
+    int isDT, ieDT, jsDT, jeDT, ksDT, keDT;
+    size_t nc, nf;
+    /* caller-owned arrays */
+    double *xyz;     /* F x 3   corners */
+    double *xyzcc;   /* G x 3   cell barycentres */
+    double *xfc;     /* F x 3 x 3 face centres (xyz, face dir) */
+    double *face;    /* F x 3   face areas */
+    double *Tf;      /* F x 9 x 3 face triads */
+    double *edge;    /* F x 3   edge lengths */
+    double *Te;      /* F x 9 x 3 edge triads */
+    double *Teb;     /* F x 4 x 3 edge B transforms */
+    double *volume, *di, *dj, *dk;  /* G */
+} orc_grid;
+
+/* State_T subset (reference gamtypes.F90:248-264) */
+typedef struct {
+    double time;
+    double *Gas;      /* G x 5 */
+    double *magFlux;  /* F x 3 */
+    double *Bxyz;     /* G x 3 */
+    double *Efld;     /* G x 3 */
+} orc_state;
+
+typedef struct orc_solver orc_solver;
+
+void   orc_grid_derive(orc_grid *g);
+
+/* setup producers */
+void   orc_gen_corners_cart(const orc_grid *g, const double bds[6], int do25D);
+void   orc_gen_corners_warp3(const orc_grid *g, const double bds[6], double amp);
+void   orc_gen_corners_refwarp(const orc_grid *g, const double bds[6], double w0, double R, int Nw, int do25D);
+void   orc_corners2grid(const orc_grid *g);
+void   orc_ic_ot(const orc_model *m, const orc_grid *g, orc_state *s,
+                 double rho0, double p0, double b0, double kscale);
+void   orc_ic_bw(const orc_model *m, const orc_grid *g, orc_state *s,
+                 double x0, double y0, double z0, double D0, double P0,
+                 double pRat, double dRat, double rC, double B0);
+void   orc_ic_advance_modes(const orc_model *m, const orc_grid *g, orc_state *s,
+                            const double *modes, int nmodes);
+
+/* solver scratch */
+orc_solver *orc_solver_new(const orc_grid *g);
+void        orc_solver_free(orc_solver *sv);
+double     *orc_solver_array(orc_solver *sv, int which);
+
+/* hot path (reference mhdgroup.F90, stress.F90, fields.F90, step.F90, gamapp.F90) */
+void   orc_hatch_finish(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o);
+void   orc_advance_mhd(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o,
+                       orc_solver *sv, double dt);
+double orc_calc_dt(orc_model *m, const orc_grid *g, const orc_state *s);
+void   orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s);
+void   orc_bflux2fld(const orc_grid *g, const double *magFlux, double *Bxyz);
+void   orc_step(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o, orc_solver *sv);
+int    orc_run_steps(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o,
+                     orc_solver *sv, int nsteps, int dt0_after);
+double orc_divb(const orc_grid *g, const orc_state *s);
+
+/* stage entry points used by stage-level parity tests */
+void   orc_predictor(const orc_model *m, const orc_grid *g, const orc_state *s,
+                     const orc_state *o, orc_state *p, double pdt);
+void   orc_calc_efield(const orc_model *m, const orc_grid *g, const orc_state *n,
+                       const orc_state *hf, double *Vf, double *E);
+void   orc_calc_stress(const orc_model *m, const orc_grid *g, const orc_state *n,
+                       const orc_state *hf, double *gFlx, double *mFlx,
+                       double *dGasH, double *dGasM);
+
+/* cell-level helpers exposed for known-answer tests */
+void   orc_cell_c2p(const orc_model *m, const double *con, double *prim);
+void   orc_cell_p2c(const orc_model *m, const double *prim, double *con);
+double orc_pdm(double pdmb, double q0, double q1, double q2, double qI);
+void   orc_pdmlr(double pdmb, const double *nQ, double Qi, double *Ql, double *Qr);
+double orc_up7(const double *q);
+double orc_central8(const double *q);
+void   orc_gaskinflux(const orc_model *m, int q, const double *prim, const double *Tf9,
+                      double *gFlx5, double *dW5);
+void   orc_magkinflux(const orc_model *m, int q, const double *prim, const double *mag,
+                      double Bn, const double *Tf9, double *mFlx3, double *bbD, double *VaD);
+void   orc_cell_boris(const orc_model *m, double *U2, const double *U0, const double *B2,
+                      const double *B0, const double *dGasH, const double *dGasM,
+                      double Ca, double dt);
+void   orc_cell_maxwell(const orc_model *m, double *U, const double *dGasM, double dt);
+double orc_cell_dt(const orc_model *m, const orc_grid *g, const orc_state *s, int i, int j, int k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

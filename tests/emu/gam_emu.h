@@ -1,0 +1,89 @@
+// gam_emu.h -- minimal HIP runtime + device builtins so the kernel sources of
+// kaiju_amd/csrc compile as plain C++ for the CPU.
+//
+// TEST INFRASTRUCTURE ONLY.  Used by tests/test_emu_*.py to run every kernel's
+// indexing under AddressSanitizer and to compare the emulated step with the
+// oracle before anything reaches a GPU.  Each launch runs its blocks in order
+// and the threads of a block in order 0..blockDim-1 (block reductions in the
+// kernels have a sequential form under GAM_EMU).
+#pragma once
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+
+using std::fmax;
+using std::fmin;
+using std::sqrt;
+using std::erfc;
+using std::exp;
+using std::copysign;
+using std::fabs;
+
+#define __global__
+#define __device__
+#define __host__
+#define __constant__
+#define __shared__ static
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+
+struct dim3 {
+    unsigned x, y, z;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct emu_idx { unsigned x, y, z; };
+inline thread_local emu_idx threadIdx{}, blockIdx{}, blockDim{}, gridDim{};
+
+inline void __syncthreads() {}
+template <class T> inline T __shfl_xor(T v, int, int = 64) { return v; }
+
+typedef int hipError_t;
+typedef void *hipStream_t;
+typedef void *hipEvent_t;
+enum { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 };
+enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+#define hipStreamNonBlocking 1
+#define hipHostMallocDefault 0
+
+inline const char *hipGetErrorString(hipError_t e) { return e ? "emulation error" : "no error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+template <class T> inline hipError_t hipMalloc(T **p, size_t n) {
+    *p = (T *)std::malloc(n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+template <class T> inline hipError_t hipHostMalloc(T **p, size_t n, int) { return hipMalloc(p, n); }
+inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemset(void *p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
+inline hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t) { std::memset(p, v, n); return hipSuccess; }
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (void *)0x1; return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)0x1; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float *t, hipEvent_t, hipEvent_t) { *t = 0.f; return hipSuccess; }
+
+template <class F> inline void emu_launch(dim3 g, dim3 b, F &&f) {
+    blockDim = {b.x, b.y, b.z};
+    gridDim = {g.x, g.y, g.z};
+    for (unsigned bz = 0; bz < g.z; bz++)
+        for (unsigned by = 0; by < g.y; by++)
+            for (unsigned bx = 0; bx < g.x; bx++) {
+                blockIdx = {bx, by, bz};
+                for (unsigned tz = 0; tz < b.z; tz++)
+                    for (unsigned ty = 0; ty < b.y; ty++)
+                        for (unsigned tx = 0; tx < b.x; tx++) {
+                            threadIdx = {tx, ty, tz};
+                            f();
+                        }
+            }
+}
+
+#define HIP_KERNEL_NAME(...) __VA_ARGS__
+#define hipLaunchKernelGGL(K, G, B, SH, S, ...) emu_launch(dim3(G), dim3(B), [&]() { K(__VA_ARGS__); })
