@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the GAMERA FP64 MHD step on one MI355X.
+
+Workload: 3D MHD blast wave on a warped non-orthogonal 256^3 grid over the unit
+cube (x = xi + 0.05 sin 2pi xi sin 2pi eta sin 2pi zeta, same for y, z),
+rho = 1, p = 0.1 / 10 inside r < 0.1, v = 0, B = 10 (1,1,0)/sqrt 2, gamma = 5/3,
+CFL 0.3, 7-point upwind reconstruction with the PDM limiter, FP64.  Geometry and
+initial condition are built on the GPU before the timed region.
+
+One step = one stepGamera: predictor, E field, stresses, CT/update, CalcDT,
+periodic BCs.  value = steps * 256^3 / wall time of the timed steps (the
+reference's kZCs definition x 1e3, reference src/gamera/output.F90:64-85).
+
+Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+ALGO_BYTES_PER_ZONE = 824.0      # FP64 step: 192 B state + 632 B geometry per zone-update
+METRIC = "zone-updates/s per MI355X: plain FP64 vs FP32/learned at fixed L1 error"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_run(n: int, steps: int, warmup: int):
+    from kaiju_amd import configs
+    from kaiju_amd.gamera import GamApp
+    spec = configs.blast_wave(n, steps=steps)
+    t0 = time.perf_counter()
+    app = GamApp(spec)
+    app.InitModel()
+    log(f"[bench] setup {spec.name} on GPU: {time.perf_counter() - t0:.2f} s, dt0 = {app.Model.dt:.4e}")
+    st = app.steps(warmup)
+    if st:
+        raise RuntimeError(f"warmup status {st}")
+    app.sync()
+    app.set_timing(True)
+    app.sync()
+    t1 = time.perf_counter()
+    st = app.steps(steps)
+    app.sync()
+    t2 = time.perf_counter()
+    if st:
+        raise RuntimeError(f"timed steps status {st}")
+    tim = app.timings()
+    app.set_timing(False)
+    divb = app.divb()
+    wall = t2 - t1
+    log(f"[bench] {steps} steps in {wall:.3f} s, t = {app.Model.t:.4e}, divB = {divb:.3e}")
+    return app, spec, wall, tim, divb
+
+
+def roofline(tim: dict, ms_per_step: float, zups: float):
+    # dominant kernel = largest share of the timed region
+    stages = {k: v for k, v in tim.items() if v["launches"] > 0}
+    dom = max(stages, key=lambda k: stages[k]["ms"])
+    d = stages[dom]
+    avg_ms = d["ms"] / d["launches"]
+    if avg_ms <= 0:
+        raise RuntimeError("stage timers recorded no time")
+    achieved = d["bytes"] / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(pmc):
+        try:
+            traffic = json.load(open(pmc)).get(dom, {}).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    step_gbs = ALGO_BYTES_PER_ZONE * zups / 1e9
+    return dict(kernel=dom, bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                avg_launch_ms=round(avg_ms, 4), algo_bytes_per_launch=d["bytes"],
+                step_algo_GBs=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4),
+                stage_avg_ms={k: round(v["ms"] / v["launches"], 4) for k, v in stages.items()})
+
+
+def cpu_baseline(n: int, budget_s: float):
+    """The oracle (CPU restatement, OpenMP) timed on this host's cores on a
+    bounded sample: the same blast-wave problem at n^3, inputs built on the GPU
+    and copied to the oracle, timed over whole steps."""
+    import oracle
+    from kaiju_amd import configs
+    from kaiju_amd.gamera import GamApp
+    from tests.helpers import compare_state
+    oracle.build()
+    spec = configs.blast_wave(n, steps=1)
+    app = GamApp(spec)
+    app.InitModel()
+    oc = oracle.OracleCase(spec, setup=False)
+    g = app.get_grid()
+    for k in oc.grid:
+        oc.grid[k][...] = g[k]
+    for which, st in ((0, oc.state), (1, oc.ostate)):
+        s = app.get_state(which)
+        st.Gas[...] = s["Gas"]; st.magFlux[...] = s["magFlux"]; st.Bxyz[...] = s["Bxyz"]
+        st.s.time = s["time"]
+    c = app.get_clock()
+    oc.m.t, oc.m.dt, oc.m.dt0, oc.m.ts = c.t, c.dt, c.dt0, c.ts
+    t_one = oc.timed_steps(1)
+    nsteps = max(1, min(200, int(budget_s / max(t_one, 1e-6))))
+    wall = oc.timed_steps(nsteps)
+    # the same steps on the GPU: the sample doubles as a parity spot check
+    app.steps(nsteps + 1)
+    err = compare_state(app, oc, 0)
+    app.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    zups = nsteps * spec.zones / wall
+    cpu = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=round(zups, 1), unit="zone-updates/s", cores=cores, kind="port",
+                sample=f"oracle (C/OpenMP restatement) blast wave warped {n}^3, {nsteps} steps, {wall:.1f} s on {cores} threads of {cpu}",
+                gpu_vs_oracle_max_rel_err=max(err["Gas"], err["magFlux"], err["Bxyz"]))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="cells per axis")
+    ap.add_argument("--cpu-n", type=int, default=96, help="cells per axis of the CPU-baseline sample")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle stepping")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    app, spec, wall, tim, divb = gpu_run(args.n, args.steps, args.warmup)
+    app.close()
+    zones = spec.zones
+    zups = args.steps * zones / wall
+    ms = wall / args.steps * 1e3
+    rf = roofline(tim, ms, zups)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args.cpu_n, args.cpu_budget)
+        except Exception as e:   # reported, never substituted for the GPU number
+            log(f"[bench] cpu baseline failed: {e!r}")
+            cpu = {"value": None, "error": repr(e)}
+    out = {
+        "metric": METRIC,
+        "value": round(zups, 1),
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (blast-wave initial condition built on the GPU)",
+        "config": {"workload": f"3D MHD blast wave, warped non-orthogonal {args.n}^3, plain FP64, 7UP+PDM, CFL 0.3",
+                   "cells": [args.n] * 3, "ghosts": 4, "precision": "fp64", "divB_total_after": divb},
+        "roofline": rf,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
