@@ -698,10 +698,21 @@ __global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int n
 // ghost points reproduces them.  Cell-centred map:
 //   i < is -> i + Nip ;  i > ie -> i - Nip
 // Face-normal component along the axis:  i < is -> i + Nip ; i > ie+1 -> i - Nip
-// (face ie+1 keeps its own CT-evolved value).  2.5D: every k except ks -> ks,
+// (face ie+1 keeps its own CT-evolved value); both repeated while needed.  2.5D: every k except ks -> ks,
 // for all components, k up to keg (bcs.F90:387-394,423-429).
-__device__ __forceinline__ int pmap_c(int i, int a0, int n) { return i < a0 ? i + n : (i >= a0 + n ? i - n : i); }
-__device__ __forceinline__ int pmap_f(int i, int a0, int n) { return i < a0 ? i + n : (i > a0 + n ? i - n : i); }
+// When the owned extent n is smaller than the ghost depth the reference's
+// loops (n = 1..nG, in order) copy from ghosts they have just filled, so the
+// map is applied until it lands on an owned index.
+__device__ __forceinline__ int pmap_c(int i, int a0, int n) {
+    while (i < a0) i += n;
+    while (i >= a0 + n) i -= n;
+    return i;
+}
+__device__ __forceinline__ int pmap_f(int i, int a0, int n) {   // owned faces a0..a0+n
+    while (i < a0) i += n;
+    while (i > a0 + n) i -= n;
+    return i;
+}
 
 __global__ void __launch_bounds__(256) k_bc(GV g, const DClock *__restrict__ clk, int do25D, double *__restrict__ Gas,
                                             double *__restrict__ mag, double *__restrict__ B) {

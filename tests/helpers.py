@@ -38,10 +38,13 @@ def small_cases():
     bwb = configs.blast_wave(16, steps=8, doBoris=True, Ca=6.0)
     hyd = configs.blast_wave(16, steps=8, doMHD=False)
     nolfm = configs.orszag_tang(24, 4, steps=6, doLFMLim=False)
+    thin = configs.blast_wave((24, 2, 3), steps=6)     # owned extents below the ghost depth
+    thin.bounds = (0.0, 1.0, 0.4, 0.6, 0.3, 0.7)
     for s, n in ((ot, "ot32x32x4"), (ot8, "ot32x32x4_8cent"), (ot25, "ot32x32x1_25d"), (bw, "bw16_warp"),
-                 (bwb, "bw16_warp_boris"), (hyd, "bw16_hydro"), (nolfm, "ot24x24x4_nolfmlim")):
+                 (bwb, "bw16_warp_boris"), (hyd, "bw16_hydro"), (nolfm, "ot24x24x4_nolfmlim"),
+                 (thin, "bw24x2x3_thin")):
         s.name = n
-    return [ot, ot8, ot25, bw, bwb, hyd, nolfm]
+    return [ot, ot8, ot25, bw, bwb, hyd, nolfm, thin]
 
 
 def app_from_oracle(oc):

@@ -1,0 +1,36 @@
+"""The HIP kernel sources, compiled for the CPU (tests/emu), checked before any
+GPU run: the GPU parity tests pass against the oracle on the emulation, and an
+AddressSanitizer build of the same sources runs every kernel in bounds."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+from tests.emu import build_emu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _run(lib, extra_env, kexpr, timeout):
+    env = dict(os.environ)
+    env.update(extra_env)
+    env["KAIJU_EMU_LIB"] = lib
+    cmd = [sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "gpu",
+           os.path.join(ROOT, "tests", "test_gpu_parity.py"), "-k", kexpr]
+    r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    return r.stdout
+
+
+def test_kernels_match_oracle_on_cpu_emulation():
+    lib = build_emu.build(asan=False)
+    out = _run(lib, {}, "bw16_warp_boris or ot32x32x1_25d or bw16_hydro or 8cent", 600)
+    assert " passed" in out
+
+
+def test_kernels_in_bounds_under_asan():
+    lib = build_emu.build(asan=True)
+    env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
+    out = _run(lib, env, "(test_multistep or test_hatch or test_metric or test_initial) and (bw16_warp_boris or ot32x32x4_8cent or 25d or nolfm)", 900)
+    assert " passed" in out

@@ -1,0 +1,452 @@
+"""Pins the oracle (CPU restatement of the reference step) before it is trusted.
+
+Each check comes from outside the oracle's own formulas:
+  * known-answer values of the reference's unit tests
+    (reference tests/gamera/testgamutils.pf:23-112);
+  * closed forms: reconstruction weights reproduce polynomials exactly, the
+    kinetic flux splitting sums to the exact Euler / Maxwell flux when the two
+    sides agree, CellDT of a uniform state, AB2 exact on linear data, the
+    metric of affine cells;
+  * invariants: div.B preserved to rounding by constrained transport, mass and
+    momentum conservation on a periodic box, point symmetry of Orszag-Tang
+    (reference tests/cases/testCases.pf:184-225);
+  * the periodic-BC index-encoding pattern of tests/gamera_mpi/testmpibc.pf:36-181;
+  * the 1D advection accuracy bar of tests/cases/testCases.pf:115-146.
+A plausible slip (a dropped term, a wrong sign or index, a transposed stencil)
+fails at least one of them.
+"""
+import ctypes as C
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from kaiju_amd import configs
+
+L = None
+
+
+def setup_module(_):
+    global L
+    oracle.build()
+    L = oracle.lib()
+
+
+def _arr(v):
+    return (C.c_double * len(v))(*v)
+
+
+def _model(**kw):
+    spec = configs.ModelSpec(**kw)
+    return oracle.model_struct(spec, False)
+
+
+# ----------------------------------------------------------- known answers
+def test_p2c_known_answer():
+    """testgamutils.pf:23-45 PrimitiveToConserved."""
+    m = _model(gamma=5.0 / 3.0)
+    prim = _arr([0.1, 0.2, 0.3, 0.4, 2.0 / 3.0])
+    con = (C.c_double * 5)()
+    L.orc_cell_p2c(C.byref(m), prim, con)
+    assert abs(con[0] - 0.1) <= 0.0
+    assert abs(con[1] - 0.02) <= 1e-16
+    assert abs(con[2] - 0.03) <= 1e-16
+    assert abs(con[3] - 0.04) <= 1e-16
+    assert abs(con[4] - 1.0145) <= 2e-16
+
+
+def test_c2p_known_answer():
+    """testgamutils.pf:47-69 ConservedToPrimitive."""
+    m = _model(gamma=5.0 / 3.0)
+    con = _arr([0.1, 0.02, 0.03, 0.04, 1.0145])
+    prim = (C.c_double * 5)()
+    L.orc_cell_c2p(C.byref(m), con, prim)
+    assert abs(prim[0] - 0.1) <= 0.0
+    assert abs(prim[1] - 0.2) <= 1e-16
+    assert abs(prim[2] - 0.3) <= 1e-16
+    assert abs(prim[3] - 0.4) <= 1e-16
+    assert abs(prim[4] - 2.0 / 3.0) <= 2e-16
+
+
+def test_round_trips():
+    """testgamutils.pf:71-112 ConservedToConserved / PrimitiveToPrimitive."""
+    m = _model(gamma=5.0 / 3.0)
+    con0 = [0.1, 0.02, 0.03, 0.04, 1.0145]
+    prim, con = (C.c_double * 5)(), (C.c_double * 5)()
+    L.orc_cell_c2p(C.byref(m), _arr(con0), prim)
+    L.orc_cell_p2c(C.byref(m), prim, con)
+    tol = [0.0, 2e-16, 2e-16, 2e-16, 4e-16]
+    for v in range(5):
+        assert abs(con[v] - con0[v]) <= tol[v]
+    prim0 = [0.1, 0.2, 0.3, 0.4, 2.0 / 3.0]
+    L.orc_cell_p2c(C.byref(m), _arr(prim0), con)
+    L.orc_cell_c2p(C.byref(m), con, prim)
+    for v in range(5):
+        assert abs(prim[v] - prim0[v]) <= tol[v]
+
+
+# ----------------------------------------------------------- reconstruction
+def _cell_avg_poly(coef, a, b):
+    """Exact average of sum c_d x^d over [a, b] (rationals)."""
+    s = Fraction(0)
+    for d, c in enumerate(coef):
+        s += Fraction(c) * (Fraction(b) ** (d + 1) - Fraction(a) ** (d + 1)) / (d + 1)
+    return s / (Fraction(b) - Fraction(a))
+
+
+@pytest.mark.parametrize("deg", range(0, 9))
+def test_central8_reproduces_degree_7(deg):
+    """Interface value at x = 0 from the 8 cell averages on [-4,4]: exact for
+    polynomials up to degree 7 (recon.F90:22 weights), not for degree 8."""
+    rng = np.random.default_rng(deg)
+    coef = [int(c) for c in rng.integers(-5, 6, deg + 1)]
+    if coef[-1] == 0:
+        coef[-1] = 1
+    q = [float(_cell_avg_poly(coef, n, n + 1)) for n in range(-4, 4)]
+    got = L.orc_central8(_arr(q))
+    exact = float(coef[0])
+    scale = max(1.0, max(abs(x) for x in q))
+    if deg <= 7:
+        assert abs(got - exact) <= 1e-14 * scale
+    else:
+        assert abs(got - exact) > 1e-6
+
+
+@pytest.mark.parametrize("deg", range(0, 8))
+def test_up7_reproduces_degree_6(deg):
+    """Left state from cells -4..2 and right state from cells 3..-3 (recon.F90:190-191):
+    exact for polynomials up to degree 6 (recon.F90:30 weights)."""
+    rng = np.random.default_rng(100 + deg)
+    coef = [int(c) for c in rng.integers(-5, 6, deg + 1)]
+    if coef[-1] == 0:
+        coef[-1] = 1
+    avg = {n: float(_cell_avg_poly(coef, n, n + 1)) for n in range(-4, 4)}
+    left = L.orc_up7(_arr([avg[n] for n in range(-4, 3)]))
+    right = L.orc_up7(_arr([avg[n] for n in range(3, -4, -1)]))
+    scale = max(1.0, max(abs(x) for x in avg.values()))
+    for got in (left, right):
+        if deg <= 6:
+            assert abs(got - coef[0]) <= 1e-14 * scale
+        else:
+            assert abs(got - coef[0]) > 1e-8
+
+
+def test_weights_sum_to_one():
+    assert abs(L.orc_central8(_arr([1.0] * 8)) - 1.0) <= 2e-16
+    assert abs(L.orc_up7(_arr([1.0] * 7)) - 1.0) <= 2e-16
+
+
+def test_pdm_linear_data_untouched():
+    """Monotone linear data, pdmb >= 1/4: the limiter returns the interpolant."""
+    for pdmb in (0.25, 0.75, 4.0):
+        nQ = _arr([-1.5, -0.5, 0.5, 1.5])
+        ql, qr = C.c_double(), C.c_double()
+        L.orc_pdmlr(pdmb, nQ, 0.0, C.byref(ql), C.byref(qr))
+        assert ql.value == 0.0 and qr.value == 0.0
+        assert L.orc_pdm(pdmb, -1.5, -0.5, 0.5, 0.0) == 0.0
+
+
+def test_pdm_extremum_goes_first_order():
+    """Slopes flip on both sides (q0 = q1 = 0): Ql = v1 and Qr = v2 (recon.F90:260-269)."""
+    nQ = _arr([0.0, 1.0, 2.0, 0.0])     # s0 = +, s1 = +, s2 = - -> q1 = 0
+    ql, qr = C.c_double(), C.c_double()
+    L.orc_pdmlr(0.75, nQ, 1.9, C.byref(ql), C.byref(qr))
+    assert qr.value == 2.0
+    nQ = _arr([2.0, 1.0, 2.0, 3.0])     # s0 = -, s1 = + -> q0 = 0
+    L.orc_pdmlr(0.75, nQ, 1.5, C.byref(ql), C.byref(qr))
+    assert ql.value == 1.0
+    # the interpolant is clamped into [min, max] of the two neighbours
+    L.orc_pdmlr(0.75, _arr([0.0, 1.0, 2.0, 3.0]), 7.0, C.byref(ql), C.byref(qr))
+    assert ql.value <= 2.0 and qr.value <= 2.0
+
+
+def test_pdm_sign_of_negative_zero():
+    """SIGN(1,-0.0) is -1 under IEEE Fortran (Appendix hazard): copysign semantics."""
+    # dq1 = pdmb*(q2-q1) = -0.0 when q2 = -0.0, q1 = +0.0
+    a = L.orc_pdm(0.75, 1.0, 0.0, -0.0, 0.5)
+    b = L.orc_pdm(0.75, 1.0, 0.0, 0.0, 0.5)
+    assert a == 0.0 and b == 0.0
+
+
+# ----------------------------------------------------------- fluxes
+def _triad(rng):
+    n = rng.normal(size=3); n /= np.linalg.norm(n)
+    t1 = np.cross(n, rng.normal(size=3)); t1 /= np.linalg.norm(t1)
+    t2 = np.cross(n, t1)
+    return np.concatenate([n, t1, t2])
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_gaskinflux_equal_states_give_euler_flux(seed):
+    """L = R: Vn0(L)+Vn0(R) = 1 and Vn1(L)+Vn1(R) = Vn, so the split flux is the
+    exact Euler flux and dW = 0 (stress.F90:432-511)."""
+    rng = np.random.default_rng(seed)
+    g = 5.0 / 3.0
+    m = _model(gamma=g)
+    D, P = rng.uniform(0.5, 2), rng.uniform(0.2, 3)
+    v = rng.normal(size=3)
+    T = _triad(rng)
+    prim = _arr([D, *v, P])
+    gF, dW = (C.c_double * 5)(), (C.c_double * 5)()
+    L.orc_gaskinflux(C.byref(m), -1, prim, _arr(T), gF, dW)
+    L.orc_gaskinflux(C.byref(m), 1, prim, _arr(T), gF, dW)
+    n = T[:3]
+    vn = v @ n
+    E = 0.5 * D * v @ v + P / (g - 1)
+    exact = [D * vn, *(D * v * vn + P * n), (E + P) * vn]
+    for k in range(5):
+        assert abs(gF[k] - exact[k]) <= 1e-13 * (1 + abs(exact[k]))
+        assert abs(dW[k]) <= 1e-15 * (1 + abs(E))
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_magkinflux_equal_states_give_maxwell_stress(seed):
+    """L = R: the two one-sided Maxwell fluxes sum to 1/2|B|^2 n - B Bn (stress.F90:513-599)."""
+    rng = np.random.default_rng(10 + seed)
+    m = _model()
+    D, P = rng.uniform(0.5, 2), rng.uniform(0.2, 3)
+    prim = _arr([D, *rng.normal(size=3), P])
+    B = rng.normal(size=3)
+    T = _triad(rng)
+    Bn = float(B @ T[:3])
+    mF = (C.c_double * 3)()
+    bbD, VaD = C.c_double(0.0), C.c_double(0.0)
+    L.orc_magkinflux(C.byref(m), -1, prim, _arr(B), Bn, _arr(T), mF, C.byref(bbD), C.byref(VaD))
+    L.orc_magkinflux(C.byref(m), 1, prim, _arr(B), Bn, _arr(T), mF, C.byref(bbD), C.byref(VaD))
+    exact = 0.5 * (B @ B) * T[:3] - B * Bn
+    assert np.allclose(np.array(mF[:]), exact, rtol=0, atol=1e-14 * (1 + B @ B))
+    assert abs(VaD.value - math.sqrt(B @ B / D)) <= 1e-14 * (1 + VaD.value)
+
+
+def test_boris_reduces_to_maxwell_for_huge_ca():
+    """CellBoris with Ca -> infinity (alpha -> 0) is CellMaxwell (mhdgroup.F90:337-389)."""
+    rng = np.random.default_rng(3)
+    m = _model(gamma=5.0 / 3.0)
+    for _ in range(10):
+        U0 = np.array([1.3, 0.2, -0.1, 0.3, 2.5])
+        B2, B0 = rng.normal(size=3), rng.normal(size=3)
+        dGH, dGM = rng.normal(size=5), rng.normal(size=3)
+        dt = 1e-3
+        U2 = U0 + dt * dGH          # CellBoris receives the Reynolds-updated state (mhdgroup.F90:113-126)
+        a = _arr(list(U2))
+        L.orc_cell_boris(C.byref(m), a, _arr(list(U0)), _arr(list(B2)), _arr(list(B0)), _arr(list(dGH)),
+                         _arr(list(dGM)), 1e12, dt)
+        b = _arr(list(U2))
+        L.orc_cell_maxwell(C.byref(m), b, _arr(list(dGM)), dt)
+        assert np.allclose(a[:], b[:], rtol=1e-12, atol=1e-14)
+
+
+# ----------------------------------------------------------- geometry
+def _affine_case(A, N=(8, 8, 8)):
+    spec = configs.CaseSpec(name="affine", N=N, bounds=(0, 1, 0, 1, 0, 1), grid="cart", ic="none")
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.gen_corners()
+    xyz = oc.grid["xyz"]
+    pts = xyz.reshape(-1, 3, order="F") @ A.T
+    xyz[...] = pts.reshape(xyz.shape, order="F")
+    oc.metric()
+    return oc
+
+
+def test_metric_of_affine_cells():
+    """Corners2Grid on a sheared, stretched (affine) grid: cell volume = |det A| dx dy dz,
+    barycentre = mean of the corners, face areas = |A^-T n| |det A| * area."""
+    A = np.array([[1.2, 0.3, -0.1], [0.2, 0.9, 0.25], [-0.15, 0.1, 1.1]])
+    oc = _affine_case(A)
+    g = oc.grid
+    h = 1.0 / 8
+    vol = abs(np.linalg.det(A)) * h ** 3
+    assert np.allclose(g["volume"], vol, rtol=1e-13)
+    xyz = g["xyz"]
+    cc = 0.125 * sum(xyz[a:a + xyz.shape[0] - 1, b:b + xyz.shape[1] - 1, c:c + xyz.shape[2] - 1]
+                     for a in (0, 1) for b in (0, 1) for c in (0, 1))
+    assert np.allclose(g["xyzcc"], cc, rtol=0, atol=1e-13)
+    cof = np.linalg.det(A) * np.linalg.inv(A).T
+    for d in range(3):
+        area = np.linalg.norm(cof[:, d]) * h * h
+        assert np.allclose(g["face"][:-1, :-1, :-1, d], area, rtol=1e-13)
+        n = cof[:, d] / np.linalg.norm(cof[:, d])
+        assert np.allclose(g["Tf"][2:-2, 2:-2, 2:-2, 0:3, d], n, atol=1e-13)
+
+
+def test_bflux2fld_uniform_field_on_affine_cells():
+    """Uniform B on affine cells is reproduced exactly by gamCellBxyz (gridutils.F90:648-680)."""
+    A = np.array([[1.0, 0.4, 0.0], [0.0, 1.0, 0.3], [0.2, 0.0, 1.0]])
+    oc = _affine_case(A)
+    g = oc.grid
+    B = np.array([0.3, -1.2, 0.7])
+    mag = np.zeros(oc.F + (3,), order="F")
+    for d in range(3):
+        mag[..., d] = g["face"][..., d] * np.einsum("...x,x->...", g["Tf"][..., 0:3, d], B)
+    Bx = np.zeros(oc.G + (3,), order="F")
+    L.orc_bflux2fld(C.byref(oc.g), oracle.ptr(mag), oracle.ptr(Bx))
+    assert np.allclose(Bx[:-1, :-1, :-1], B, rtol=0, atol=1e-13)
+
+
+# ----------------------------------------------------------- time step pieces
+def test_celldt_uniform_state_closed_form():
+    spec = configs.orszag_tang(16, 4)
+    oc = oracle.OracleCase(spec)
+    g = oc.state
+    rho, v, p, B = 2.0, np.array([0.3, -0.4, 0.1]), 0.7, np.array([0.5, 0.2, -0.3])
+    gam = spec.model.gamma
+    g.Gas[..., 0] = rho
+    g.Gas[..., 1:4] = rho * v
+    g.Gas[..., 4] = 0.5 * rho * v @ v + p / (gam - 1)
+    g.Bxyz[..., :] = B
+    i, j, k = 6, 7, 5
+    got = L.orc_cell_dt(C.byref(oc.m), C.byref(oc.g), C.byref(g.s), i, j, k)
+    c = oc.g
+    idx = (i - c.isg, j - c.jsg, k - c.ksg)
+    dl = min(oc.grid["di"][idx], oc.grid["dj"][idx], oc.grid["dk"][idx])
+    exact = spec.model.CFL * dl / (np.linalg.norm(v) + math.sqrt(gam * p / rho + B @ B / rho))
+    assert abs(got - exact) <= 1e-14 * exact
+
+
+def test_predictor_exact_for_linear_in_time_data():
+    """AB2 on primitives (mhdgroup.F90:529-563): exact when W is linear in time."""
+    spec = configs.orszag_tang(16, 4)
+    oc = oracle.OracleCase(spec)
+    gam = spec.model.gamma
+    rng = np.random.default_rng(5)
+    W0 = np.stack([rng.uniform(1, 2, oc.G), *rng.normal(size=(3,) + oc.G), rng.uniform(1, 2, oc.G)], -1)
+    dW = 0.1 * np.stack([rng.normal(size=oc.G) for _ in range(5)], -1)
+
+    def p2c(W):
+        U = np.empty_like(W)
+        U[..., 0] = W[..., 0]
+        U[..., 1:4] = W[..., 0:1] * W[..., 1:4]
+        U[..., 4] = W[..., 4] / (gam - 1) + 0.5 * W[..., 0] * (W[..., 1:4] ** 2).sum(-1)
+        return np.asfortranarray(U)
+    odt, pdt = 0.02, 0.013
+    oc.ostate.Gas[...] = p2c(W0)
+    oc.state.Gas[...] = p2c(W0 + dW)
+    oc.state.s.time, oc.ostate.s.time = odt, 0.0
+    hf = oracle.OracleState(oc.G, oc.F)
+    L.orc_predictor(C.byref(oc.m), C.byref(oc.g), C.byref(oc.state.s), C.byref(oc.ostate.s), C.byref(hf.s), pdt)
+    want = p2c(W0 + dW * (1 + pdt / odt))
+    assert np.allclose(hf.Gas, want, rtol=1e-12, atol=1e-13)
+
+
+# ----------------------------------------------------------- BCs
+def test_periodic_bcs_index_encoding():
+    """Fill every slot with its own (i,j,k,v) code, apply the BCs, and check the
+    ghosts hold the periodic image (testmpibc.pf:36-181 pattern)."""
+    spec = configs.orszag_tang(16, 8)
+    spec.N = (16, 12, 8)
+    oc = oracle.OracleCase(spec, setup=False)
+    g = oc.g
+    Ni, Nj, Nk = oc.G
+    I, J, K = np.meshgrid(np.arange(Ni + 1), np.arange(Nj + 1), np.arange(Nk + 1), indexing="ij")
+    code = lambda i, j, k, v: v * 1e6 + k * 1e4 + j * 1e2 + i
+    s = oc.state
+    for v in range(5):
+        s.Gas[..., v] = code(I[:-1, :-1, :-1], J[:-1, :-1, :-1], K[:-1, :-1, :-1], v)
+    for d in range(3):
+        s.magFlux[..., d] = code(I, J, K, d)
+    L.orc_enforce_bcs(C.byref(oc.m), C.byref(g), C.byref(s.s))
+    nG = spec.nG
+    Nip, Njp, Nkp = spec.N
+
+    def cmap(i, n):
+        return i + n if i < nG else (i - n if i >= nG + n else i)
+
+    def fmap(i, n):
+        return i + n if i < nG else (i - n if i > nG + n else i)
+    for i in range(Ni):
+        for j in range(Nj):
+            for k in range(Nk):
+                want = code(cmap(i, Nip), cmap(j, Njp), cmap(k, Nkp), 4)
+                assert s.Gas[i, j, k, 4] == want
+    for i in range(Ni + 1):
+        for j in range(Nj + 1):
+            for k in range(Nk + 1):
+                for d in range(3):
+                    si = fmap(i, Nip) if d == 0 else (i if i == Ni else cmap(i, Nip))
+                    sj = fmap(j, Njp) if d == 1 else (j if j == Nj else cmap(j, Njp))
+                    sk = fmap(k, Nkp) if d == 2 else (k if k == Nk else cmap(k, Nkp))
+                    assert s.magFlux[i, j, k, d] == code(si, sj, sk, d), (i, j, k, d)
+
+
+# ----------------------------------------------------------- invariants of the step
+@pytest.mark.parametrize("name", ["ot", "bw"])
+def test_divb_and_conservation(name):
+    spec = configs.orszag_tang(24, 4, steps=10) if name == "ot" else configs.blast_wave(16, steps=10)
+    oc = oracle.OracleCase(spec)
+    nG = spec.nG
+    A = tuple(slice(nG, nG + n) for n in spec.N)
+    vol = oc.grid["volume"][A]
+    mass0 = float((oc.state.Gas[A + (0,)] * vol).sum())
+    mom0 = [(oc.state.Gas[A + (v,)] * vol).sum() for v in (1, 2, 3)]
+    divb0 = oc.divb()
+    assert oc.step(spec.steps) == 0
+    mass1 = float((oc.state.Gas[A + (0,)] * vol).sum())
+    assert abs(mass1 - mass0) <= 1e-13 * abs(mass0)
+    # momentum: Reynolds + Maxwell stresses are flux differences -> conserved
+    scale = float(np.abs(oc.state.Gas[A + (1,)] * vol).sum() + np.abs(oc.state.Gas[A + (2,)] * vol).sum()) + 1e-300
+    for v, m0 in zip((1, 2, 3), mom0):
+        m1 = (oc.state.Gas[A + (v,)] * vol).sum()
+        assert abs(m1 - m0) <= 1e-12 * max(scale, mass0)
+    flux_scale = float(np.abs(oc.state.magFlux).max()) * spec.zones
+    assert abs(oc.divb() - divb0) <= 1e-14 * flux_scale
+
+
+def test_orszag_tang_point_symmetry():
+    """testCases.pf:184-225: the OT solution keeps its point symmetry (<= 1e-11)."""
+    spec = configs.orszag_tang(48, 1, steps=150)
+    oc = oracle.OracleCase(spec)
+    assert oc.step(spec.steps) == 0
+    nG = spec.nG
+    G = oc.state.Gas[nG:nG + 48, nG:nG + 48, nG, :]
+    Gp = G[::-1, ::-1, :].copy()
+    Gp[..., 1:3] *= -1
+    dG = np.sqrt(((G - Gp) ** 2).sum(-1)).sum() / (48 * 48)
+    assert dG <= 1e-11
+
+
+def adv1d_state(oc, gamma):
+    """initADV1D (prob.F90:671-736) on cells and faces is..ie+1, with the
+    reference's default-real literals widened to double."""
+    f = lambda x: float(np.float32(x))
+    g = oc.g
+    s = oc.state
+    nc = slice(g.nG, g.nG + g.Nip + 1), slice(g.nG, g.nG + g.Njp + 1), slice(g.nG, g.nG + g.Nkp + 1)
+    xc = oc.grid["xyzcc"][nc + (0,)]
+    rho = f(0.1) + np.exp(-((xc + f(0.7)) / f(0.05)) ** 2)
+    rho = rho + np.where(np.abs(xc + f(0.3)) <= f(0.1), 1.0, 0.0)
+    rho = rho + np.where(np.abs(xc - f(0.1)) <= f(0.1), 1.0 - np.abs(10.0 * (xc - f(0.1))), 0.0)
+    arg = f(0.1) ** 2 - (xc - f(0.5)) ** 2
+    rho = rho + np.where(arg >= 0, 10.0 * np.sqrt(np.maximum(arg, 0)), 0.0)
+    s.Gas[nc + (0,)] = rho
+    s.Gas[nc + (1,)] = rho * 1.0
+    s.Gas[nc + (2,)] = 0.0
+    s.Gas[nc + (3,)] = 0.0
+    s.Gas[nc + (4,)] = 0.5 * rho * 1.0 + 1.0 / (gamma - 1)
+
+
+def adv1d_spec(rmeth):
+    pdmb = 4.0
+    m = configs.ModelSpec(gamma=1.4, CFL=min(0.5 / (pdmb + 0.5), configs.CFL_DEFAULT), pdmb=pdmb,
+                          rmeth=rmeth, doMHD=False, tFin=2.0)
+    return configs.CaseSpec(name=f"adv1d_{rmeth}", N=(128, 1, 1), bounds=(-1.0, 1.0, -0.5, 0.5, -0.5, 0.5),
+                            grid="cart", ic="none", model=m, dt0_after=0)
+
+
+@pytest.mark.parametrize("rmeth", ["7UP", "8CENT"])
+def test_advection_1d_accuracy(rmeth):
+    """testCases.pf:115-180: one period of 1D advection, L2 error <= 7e-3."""
+    spec = adv1d_spec(rmeth)
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.gen_corners()
+    oc.metric()
+    adv1d_state(oc, 1.4)
+    oc.hatch_finish()
+    nG = spec.nG
+    sD = oc.state.Gas[nG:nG + 128, nG, nG, 0].copy()
+    while oc.m.t < oc.m.tFin:
+        assert oc.step(1) == 0
+    eD = oc.state.Gas[nG:nG + 128, nG, nG, 0]
+    L2 = np.linalg.norm(sD - eD) / 128.0
+    assert L2 <= 7.0e-3, L2
