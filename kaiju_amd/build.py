@@ -54,7 +54,8 @@ def build_hip(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
     tmp = HIP_LIB + ".tmp"
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp]
+    cmd = [_hipcc(), *HIPCC_FLAGS, "-Wl,-soname,libgamera_hip.so", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+           *srcs, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -65,14 +66,14 @@ def build_hip(verbose: bool = False) -> str:
 FORTRAN_MODULES = ["gamera_hip_c.F90", "gamtypes_gpu.F90", "gamapp_gpu.F90", "gamprob_gpu.F90"]
 
 
-def build_fortran(verbose: bool = False) -> str:
-    """Fortran host: modules -> libgamhost.so (+ the gamerax_gpu driver)."""
-    os.makedirs(LIBDIR, exist_ok=True)
+def build_fortran(verbose: bool = False, lib: str = HIP_LIB, exe: str = None) -> str:
+    """Fortran host (amdflang): the modules and the gamerax_gpu driver, linked
+    against a build of the C-ABI (the HIP library by default)."""
     os.makedirs(BINDIR, exist_ok=True)
     objdir = os.path.join(PKG, "_fobj")
     os.makedirs(objdir, exist_ok=True)
     fc = _flang()
-    fflags = ["-O2", "-fPIC", "-cpp", "-ffree-form", "-module-dir", objdir, "-I", objdir]
+    fflags = ["-O2", "-cpp", "-ffree-form", "-module-dir", objdir, "-I", objdir]
     objs = []
     for m in FORTRAN_MODULES:
         src = os.path.join(FSRC, m)
@@ -82,13 +83,15 @@ def build_fortran(verbose: bool = False) -> str:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
         objs.append(obj)
-    lib = os.path.join(LIBDIR, "libgamhost.so")
-    rpath = "-Wl,-rpath,$ORIGIN"
-    subprocess.run([fc, "-shared", "-fPIC", *objs, "-o", lib, "-L", LIBDIR, "-lgamera_hip", rpath], check=True)
-    exe = os.path.join(BINDIR, "gamerax_gpu")
+    exe = exe or os.path.join(BINDIR, "gamerax_gpu")
+    libdir, libfile = os.path.split(lib)
+    libname = libfile[3:-3] if libfile.startswith("lib") and libfile.endswith(".so") else libfile
     drv = os.path.join(FSRC, "gamerax_gpu.F90")
-    subprocess.run([fc, *fflags, drv, *objs, "-o", exe, "-L", LIBDIR, "-lgamera_hip",
-                    "-Wl,-rpath,$ORIGIN/../lib"], check=True)
+    cmd = [fc, *fflags, drv, *objs, "-o", exe, "-L", libdir, "-l" + libname,
+           "-Wl,-rpath,$ORIGIN/" + os.path.relpath(libdir, os.path.dirname(exe))]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     return exe
 
 
