@@ -38,6 +38,7 @@ struct gam_handle {
     int s = 0;                 // index of State in the {Gas,mag,B} pairs
     double *hGas = nullptr, *hmag = nullptr, *hB = nullptr;
     double *Vf = nullptr, *E = nullptr, *Flx = nullptr, *partial = nullptr;
+    WK wk = {};                // predictor-pass stencil quantities
     int npart = 0;
     DClock *clk = nullptr;      // device
     DClock *clk_host = nullptr; // pinned
@@ -149,11 +150,17 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     if (hipMalloc(&h->Vf, 9 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
     if (hipMalloc(&h->E, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
     if (hipMalloc(&h->Flx, 24 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->wk.Wh, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->wk.Wn, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->wk.VdV, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->wk.bh, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&h->wk.bn, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
     {
-        double *z[6] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx};
+        double *z[11] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->wk.Wh, h->wk.Wn, h->wk.VdV,
+                         h->wk.bh, h->wk.bn};
         // K fluxes stay 0 in 2.5D (init.F90:500-501 zero the flux work arrays)
-        const size_t zn[6] = {5 * G, 3 * F, 3 * G, 9 * G, 3 * G, 24 * G};
-        for (int q = 0; q < 6; q++)
+        const size_t zn[11] = {5 * G, 3 * F, 3 * G, 9 * G, 3 * G, 24 * G, 5 * G, 5 * G, 3 * G, 3 * F, 3 * F};
+        for (int q = 0; q < 11; q++)
             if (hipMemset(z[q], 0, zn[q] * sizeof(double)) != hipSuccess) return cleanup(-2);
     }
     h->npart = (int)((G + 255) / 256);
@@ -190,7 +197,8 @@ extern "C" int gam_destroy(gam_handle *h) {
     for (auto e : h->evpool) (void)hipEventDestroy(e);
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
-    double *bufs[8] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, (double *)h->clk};
+    double *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, (double *)h->clk,
+                        h->wk.Wh, h->wk.Wn, h->wk.VdV, h->wk.bh, h->wk.bn};
     for (double *b : bufs) if (b) (void)hipFree(b);
     if (h->clk_host) (void)hipHostFree(h->clk_host);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -401,9 +409,9 @@ static void launch_flux(gam_handle *h, int d) {
     long nx = g.Nip + (d == 0), ny = g.Njp + (d == 1), nz = (h->mp.do25D ? 1 : g.Nkp) + (d == 2);
     dim3 gr = grid1(nx * ny * nz);
     StageTimer t(h, ST_FLUX_I + d);
-    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->Gas[S], h->B[S], h->hGas, h->hB, h->hmag, h->Flx);
-    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->Gas[S], h->B[S], h->hGas, h->hB, h->hmag, h->Flx);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->Gas[S], h->B[S], h->hGas, h->hB, h->hmag, h->Flx);
+    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
+    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
 }
 
 template <int RM, int BO, int MH>
@@ -421,18 +429,18 @@ static int advance(gam_handle *h, int fuse_dt) {
     const int S = h->s, O = 1 - h->s;
     { StageTimer t(h, ST_CLOCK); hipLaunchKernelGGL(k_clock_begin, 1, 64, 0, h->stream, h->clk); }
     { StageTimer t(h, ST_PREDICT);
-      hipLaunchKernelGGL(k_predict, grid1(g.nf), 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->Gas[O], h->mag[S], h->mag[O], h->hGas, h->hmag); }
+      hipLaunchKernelGGL(k_predict, grid1(g.nf), 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->Gas[O], h->mag[S], h->mag[O], h->hGas, h->hmag, h->wk); }
     { StageTimer t(h, ST_BHALF); hipLaunchKernelGGL(k_b2f, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->hmag, h->hB); }
     if (m.doMHD) {
-        { StageTimer t(h, ST_MOM2FACE); hipLaunchKernelGGL(k_vf, grid1(g.nc), 256, 0, h->stream, g, m, h->clk, h->hGas, h->Vf); }
+        { StageTimer t(h, ST_MOM2FACE); hipLaunchKernelGGL(k_vf, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->wk.VdV, h->Vf); }
         long ne = (long)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
         StageTimer t(h, ST_EMF);
         if (m.rmeth == 1) {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
         } else {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->mag[S], h->hmag, h->E);
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
         }
     }
     if (m.doMHD) {

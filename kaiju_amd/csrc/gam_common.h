@@ -35,6 +35,17 @@ struct DClock {
     int ts, status, dt0_after, fixedTimestep, doCPR, halt;
 };
 
+// Per-cell / per-face quantities the stencil kernels read, computed once per
+// step in the predictor pass.  Each is exactly the expression the reference
+// evaluates per stencil point, so moving it changes no bits:
+//   Wh, Wn  BlockCon2Prim of the predictor / limiter state  (recon.F90:66-92)
+//   VdV     vol*M/max(D,dFloor) of the predictor state      (fields.F90:265-271)
+//   bh, bn  magFlux/max(face,TINY) of predictor / limiter   (fields.F90:331-337)
+struct WK {
+    double *Wh, *Wn, *VdV;   // 5G, 5G, 3G
+    double *bh, *bn;         // 3F, 3F
+};
+
 // Grid view: sizes + geometry pointers (device).
 struct GV {
     int Ni, Nj, Nk, nG, Nip, Njp, Nkp;

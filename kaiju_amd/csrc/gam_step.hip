@@ -68,15 +68,21 @@ __global__ void k_clock_begin(DClock *clk) {
 __global__ void __launch_bounds__(256) k_predict(GV g, MP m, const DClock *__restrict__ clk,
                                                  const double *__restrict__ Gas, const double *__restrict__ oGas,
                                                  const double *__restrict__ mag, const double *__restrict__ omag,
-                                                 double *__restrict__ hGas, double *__restrict__ hmag) {
+                                                 double *__restrict__ hGas, double *__restrict__ hmag, WK w) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nf || clk->halt) return;
     const double pdt = clk->pdt, odt = clk->odt;
     const int nf = g.nf, nc = g.nc;
     if (m.doMHD) {
 #pragma unroll
-        for (int d = 0; d < 3; d++)
-            hmag[(size_t)nf * d + idx] = ab2(pdt, mag[(size_t)nf * d + idx], omag[(size_t)nf * d + idx], odt);
+        for (int d = 0; d < 3; d++) {
+            double mn = mag[(size_t)nf * d + idx];
+            double hm = ab2(pdt, mn, omag[(size_t)nf * d + idx], odt);
+            hmag[(size_t)nf * d + idx] = hm;
+            double fa = fmax(g.face[(size_t)nf * d + idx], GAM_TINY);
+            w.bh[(size_t)nf * d + idx] = hm / fa;
+            w.bn[(size_t)nf * d + idx] = (m.doLFMLim ? mn : hm) / fa;
+        }
     }
     int i, j, k;
     decode_f(g, idx, i, j, k);
@@ -94,6 +100,19 @@ __global__ void __launch_bounds__(256) k_predict(GV g, MP m, const DClock *__res
     cell_p2c(m, pW, pQ);
 #pragma unroll
     for (int v = 0; v < 5; v++) hGas[(size_t)nc * v + c] = pQ[v];
+    // stencil-point quantities of this cell
+    double Wh[5], Wn[5];
+    block_c2p(m, pQ, Wh);
+    if (m.doLFMLim) block_c2p(m, Q, Wn);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        w.Wh[(size_t)nc * v + c] = Wh[v];
+        w.Wn[(size_t)nc * v + c] = m.doLFMLim ? Wn[v] : Wh[v];
+    }
+    const double vol = g.vol[c];
+    const double D = fmax(pQ[0], m.dFloor);
+#pragma unroll
+    for (int x = 0; x < 3; x++) w.VdV[(size_t)nc * x + c] = vol * pQ[1 + x] / D;
 }
 
 __global__ void __launch_bounds__(256) k_b2f(GV g, const DClock *__restrict__ clk, const double *__restrict__ mag,
@@ -109,20 +128,18 @@ __global__ void __launch_bounds__(256) k_b2f(GV g, const DClock *__restrict__ cl
 }
 
 // ----------------------------------------------------------------- E field
-// fields.F90:235-285 Mom2Face at (i,j,k) along dT, c = cell index
-__device__ __forceinline__ void mom2face(const GV &g, const MP &m, const double *__restrict__ Gas, int c, int dT,
-                                         double *Vf) {
+// fields.F90:235-285 Mom2Face at (i,j,k) along dT, c = cell index; the
+// per-cell summand vol*M/max(D,dFloor) comes precomputed in VdV
+__device__ __forceinline__ void mom2face(const GV &g, const double *__restrict__ VdV, int c, int dT, double *Vf) {
     const int st = cstride(g, dT), nc = g.nc;
     double VolB[8], Vx[8], Vy[8], Vz[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
         int cc = c + (n - 4) * st;
-        double vol = g.vol[cc];
-        double D = fmax(Gas[cc], m.dFloor);
-        VolB[n] = vol;
-        Vx[n] = vol * Gas[(size_t)nc * 1 + cc] / D;
-        Vy[n] = vol * Gas[(size_t)nc * 2 + cc] / D;
-        Vz[n] = vol * Gas[(size_t)nc * 3 + cc] / D;
+        VolB[n] = g.vol[cc];
+        Vx[n] = VdV[cc];
+        Vy[n] = VdV[(size_t)nc + cc];
+        Vz[n] = VdV[2 * (size_t)nc + cc];
     }
     double dV = fmax(c8(VolB), GAM_TINY);
     Vf[0] = c8(Vx) / dV;
@@ -131,7 +148,7 @@ __device__ __forceinline__ void mom2face(const GV &g, const MP &m, const double 
 }
 
 // Vf arrays: VfA[dT1][xyz] stored as slab (3*dT1 + xyz)
-__global__ void __launch_bounds__(256) k_vf(GV g, MP m, const DClock *__restrict__ clk, const double *__restrict__ hGas,
+__global__ void __launch_bounds__(256) k_vf(GV g, const DClock *__restrict__ clk, const double *__restrict__ VdV,
                                             double *__restrict__ Vf) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nc || clk->halt) return;
@@ -142,46 +159,38 @@ __global__ void __launch_bounds__(256) k_vf(GV g, MP m, const DClock *__restrict
     double v[3];
     // dT1 = I (used by K edges): i in is..ie+1   (fields.F90:161-175)
     if (i >= a0 && i <= a0 + g.Nip) {
-        mom2face(g, m, hGas, idx, 0, v);
+        mom2face(g, VdV, idx, 0, v);
         Vf[nc * 0 + idx] = v[0]; Vf[nc * 1 + idx] = v[1]; Vf[nc * 2 + idx] = v[2];
     }
     // dT1 = J (used by I edges): j in js..je+1   (fields.F90:119-133)
     if (j >= a0 && j <= a0 + g.Njp) {
-        mom2face(g, m, hGas, idx, 1, v);
+        mom2face(g, VdV, idx, 1, v);
         Vf[nc * 3 + idx] = v[0]; Vf[nc * 4 + idx] = v[1]; Vf[nc * 5 + idx] = v[2];
     }
     // dT1 = K (used by J edges): k in ks..ke+1   (fields.F90:140-154)
     if (k >= a0 && k <= a0 + g.Nkp) {
-        mom2face(g, m, hGas, idx, 2, v);
+        mom2face(g, VdV, idx, 2, v);
         Vf[nc * 6 + idx] = v[0]; Vf[nc * 7 + idx] = v[1]; Vf[nc * 8 + idx] = v[2];
     }
 }
 
-// fields.F90:288-362 EdgeLRs: dT1 faces reconstructed along dT2 at face point f
+// fields.F90:288-362 EdgeLRs: dT1 faces reconstructed along dT2 at face point f;
+// the face-normal field bh/bn = flux/max(area,TINY) comes precomputed
 template <int RM>
-__device__ __forceinline__ void edge_lrs(const GV &g, const MP &m, const double *__restrict__ magN,
-                                         const double *__restrict__ magH, int f, int dT1, int dT2,
+__device__ __forceinline__ void edge_lrs(const GV &g, const MP &m, const double *__restrict__ bn,
+                                         const double *__restrict__ bh, int f, int dT1, int dT2,
                                          double &bL, double &bR, double &fA) {
     const int st = fstride(g, dT2);
     const size_t off = (size_t)g.nf * dT1;
-    double AreaB[8], MagB[8], nAreaB[4], nMagB[4];
+    double AreaB[8], MagB[8], nMagB[4];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
         int ff = f + (n - 4) * st;
         AreaB[n] = g.face[off + ff];
-        MagB[n] = magH[off + ff];
-    }
-    const double *limMag = m.doLFMLim ? magN : magH;
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-        int ff = f + (n - 2) * st;
-        nAreaB[n] = g.face[off + ff];
-        nMagB[n] = limMag[off + ff];
+        MagB[n] = bh[off + ff];
     }
 #pragma unroll
-    for (int n = 0; n < 8; n++) MagB[n] = MagB[n] / fmax(AreaB[n], GAM_TINY);
-#pragma unroll
-    for (int n = 0; n < 4; n++) nMagB[n] = nMagB[n] / fmax(nAreaB[n], GAM_TINY);
+    for (int n = 0; n < 4; n++) nMagB[n] = bn[off + f + (n - 2) * st];
     double Vi = c8(AreaB);
     double l, r;
     get_lrs<RM>(m.pdmb, AreaB, nMagB, MagB, Vi, l, r);
@@ -191,8 +200,8 @@ __device__ __forceinline__ void edge_lrs(const GV &g, const MP &m, const double 
 // One edge EMF (fields.F90:196-221) for edge direction eD at corner (i,j,k)
 template <int RM, int BORIS>
 __device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, const double *__restrict__ Vf,
-                                           const double *__restrict__ hGas, const double *__restrict__ magN,
-                                           const double *__restrict__ magH, int eD, int i, int j, int k) {
+                                           const double *__restrict__ hGas, const double *__restrict__ bn,
+                                           const double *__restrict__ bh, int eD, int i, int j, int k) {
     const int dT1 = (eD + 1) % 3, dT2 = (eD + 2) % 3;
     const int f = fi(g, i, j, k), c = ci(g, i, j, k);
     const size_t nf = g.nf, nc = g.nc;
@@ -220,8 +229,8 @@ __device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, 
     double Dc = 0.25 * (hGas[c] + hGas[c - s1] + hGas[c - s2] + hGas[c - s1 - s2]);
     // GetCornerB fields.F90:365-424
     double bT1L, bT1R, fA1, bT2L, bT2R, fA2;
-    edge_lrs<RM>(g, m, magN, magH, f, dT1, dT2, bT1L, bT1R, fA1);
-    edge_lrs<RM>(g, m, magN, magH, f, dT2, dT1, bT2L, bT2R, fA2);
+    edge_lrs<RM>(g, m, bn, bh, f, dT1, dT2, bT1L, bT1R, fA1);
+    edge_lrs<RM>(g, m, bn, bh, f, dT2, dT1, bT2L, bT2R, fA2);
     const double *Teb = g.Teb + nf * 4 * eD;
     double xni = Teb[nf * XNQI + f], yni = Teb[nf * YNQI + f];
     double xnj = Teb[nf * XNQJ + f], ynj = Teb[nf * YNQJ + f];
@@ -247,8 +256,8 @@ __device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, 
 // treats the rest as zero (CalcElecField zeroes E, fields.F90:89).
 template <int RM, int BORIS>
 __global__ void __launch_bounds__(256) k_emf(GV g, MP m, const DClock *__restrict__ clk, const double *__restrict__ Vf,
-                                             const double *__restrict__ hGas, const double *__restrict__ magN,
-                                             const double *__restrict__ magH, double *__restrict__ E) {
+                                             const double *__restrict__ hGas, const double *__restrict__ bn,
+                                             const double *__restrict__ bh, double *__restrict__ E) {
     const int nx = g.Nip + 1, ny = g.Njp + 1, nz = g.Nkp + 1;
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nx * ny * nz || clk->halt) return;
@@ -257,9 +266,9 @@ __global__ void __launch_bounds__(256) k_emf(GV g, MP m, const DClock *__restric
     const double dt = clk->dt;
     const int c = ci(g, i, j, k);
     const size_t nc = g.nc;
-    if (a < g.Nip) E[c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, magN, magH, 0, i, j, k);
-    if (b < g.Njp) E[nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, magN, magH, 1, i, j, k);
-    if (cz < g.Nkp) E[2 * nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, magN, magH, 2, i, j, k);
+    if (a < g.Nip) E[c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, bn, bh, 0, i, j, k);
+    if (b < g.Njp) E[nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, bn, bh, 1, i, j, k);
+    if (cz < g.Nkp) E[2 * nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, bn, bh, 2, i, j, k);
 }
 
 // ----------------------------------------------------------------- fluxes
@@ -313,9 +322,10 @@ __device__ __forceinline__ void gaskin(const MP &m, double sg, const double *P5,
 // Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
 template <int DN, int RM, int BORIS, int MHD>
 __global__ void __launch_bounds__(256) k_flux(GV g, MP m, const DClock *__restrict__ clk,
-                                              const double *__restrict__ nGas, const double *__restrict__ nB,
-                                              const double *__restrict__ hGas, const double *__restrict__ hB,
-                                              const double *__restrict__ hmag, double *__restrict__ Flx) {
+                                              const double *__restrict__ Wn, const double *__restrict__ nB,
+                                              const double *__restrict__ Wh, const double *__restrict__ hB,
+                                              const double *__restrict__ hmag, const double *__restrict__ hGas,
+                                              double *__restrict__ Flx) {
     const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nx * ny * nz || clk->halt) return;
@@ -349,32 +359,14 @@ __global__ void __launch_bounds__(256) k_flux(GV g, MP m, const DClock *__restri
         }
         Bn = hmag[nf * DN + f] / faB;
     }
-    {
-        const double *limG = m.doLFMLim ? nGas : hGas;
-        double Prim[8][5], nPrim[4][5];
 #pragma unroll
-        for (int n = 0; n < 8; n++) {
-            double cv[5];
+    for (int v = 0; v < 5; v++) {   // BlockStateLRs on the prepass primitives
+        double Q[8], nQ[4];
 #pragma unroll
-            for (int v = 0; v < 5; v++) cv[v] = hGas[nc * v + c + (n - 4) * st];
-            block_c2p(m, cv, Prim[n]);
-        }
+        for (int n = 0; n < 8; n++) Q[n] = Wh[nc * v + c + (n - 4) * st];
 #pragma unroll
-        for (int n = 0; n < 4; n++) {
-            double cv[5];
-#pragma unroll
-            for (int v = 0; v < 5; v++) cv[v] = limG[nc * v + c + (n - 2) * st];
-            block_c2p(m, cv, nPrim[n]);
-        }
-#pragma unroll
-        for (int v = 0; v < 5; v++) {
-            double Q[8], nQ[4];
-#pragma unroll
-            for (int n = 0; n < 8; n++) Q[n] = Prim[n][v];
-#pragma unroll
-            for (int n = 0; n < 4; n++) nQ[n] = nPrim[n][v];
-            get_lrs<RM>(m.pdmb, VolB, nQ, Q, Vi, PL[v], PR[v]);
-        }
+        for (int n = 0; n < 4; n++) nQ[n] = Wn[nc * v + c + (n - 2) * st];
+        get_lrs<RM>(m.pdmb, VolB, nQ, Q, Vi, PL[v], PR[v]);
     }
     double VaD = 0.0;
     if (MHD) {

@@ -1,0 +1,121 @@
+"""Profile helpers: a calibration copy for HBM counters and summaries of
+rocprofv3 CSV output.
+
+  python -m kaiju_amd.profiling copy [GiB]
+      one plain device-to-device copy of GiB (default 1) float64 bytes, the
+      yardstick for FETCH_SIZE / WRITE_SIZE (known read and write bytes).
+  python -m kaiju_amd.profiling trace <kernel_stats.csv> "<command>"
+      markdown table of a --kernel-trace --stats run.
+  python -m kaiju_amd.profiling traffic <fetch.csv> <write.csv> <calib_fetch.csv> <calib_write.csv> <GiB> <out.json> "<commands>"
+      per-stage HBM bytes per launch from two --pmc runs (FETCH_SIZE, WRITE_SIZE,
+      KiB per dispatch), each scaled by what the same counter reads for the
+      calibration copy; writes <out.json> (read by bench.py) and prints markdown.
+"""
+from __future__ import annotations
+
+import csv
+import json
+import sys
+from collections import defaultdict
+
+# kernel-name prefix -> stage name used by the library's stage timers
+STAGE_OF = [
+    ("k_clock_begin", "clock"), ("k_predict", "predictor"), ("k_b2f", "bflux"),
+    ("k_vf", "mom2face"), ("k_emf", "emf"), ("k_flux<0", "flux_i"), ("k_flux<1", "flux_j"),
+    ("k_flux<2", "flux_k"), ("k_ct", "ct_update"), ("k_update", "update"),
+    ("k_dtfinal", "dt_final"), ("k_bc", "bcs"), ("k_celldt", "celldt"),
+]
+
+
+def stage_of(name: str) -> str | None:
+    n = name.replace("void ", "").strip()
+    for pre, st in STAGE_OF:
+        if n.startswith(pre):
+            return st
+    return None
+
+
+def copy(gib: float) -> None:
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(f"copied {n * 8} bytes", flush=True)
+
+
+def trace_summary(stats_csv: str, command: str) -> str:
+    rows = list(csv.DictReader(open(stats_csv)))
+    out = ["# rocprofv3 kernel trace summary", "", "Command:", "", "```", command, "```", "",
+           "| kernel | calls | total (us) | average (us) | % |", "|---|---:|---:|---:|---:|"]
+    for r in rows:
+        short = r["Name"].replace("void ", "").split("(")[0]
+        out.append(f"| `{short}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e3:.1f} | "
+                   f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out) + "\n"
+
+
+def _counter(csv_path: str, counter: str):
+    """(kernel name, KiB) per dispatch for one counter."""
+    per = defaultdict(float)
+    names = {}
+    for r in csv.DictReader(open(csv_path)):
+        if r["Counter_Name"] != counter:
+            continue
+        d = r["Dispatch_Id"]
+        per[d] += float(r["Counter_Value"])
+        names[d] = r["Kernel_Name"]
+    return [(names[d], per[d]) for d in per]
+
+
+def traffic(fetch_csv, write_csv, cf_csv, cw_csv, gib, out_json, command) -> str:
+    nbytes = int(float(gib) * (1 << 30)) // 8 * 8
+    # the calibration copy is the largest dispatch of its run
+    cf = max(v for _, v in _counter(cf_csv, "FETCH_SIZE"))
+    cw = max(v for _, v in _counter(cw_csv, "WRITE_SIZE"))
+    sf, sw = nbytes / cf, nbytes / cw     # bytes per counted KiB
+    acc = defaultdict(lambda: [0, 0.0, 0, 0.0])
+    for name, v in _counter(fetch_csv, "FETCH_SIZE"):
+        st = stage_of(name)
+        if st:
+            acc[st][0] += 1
+            acc[st][1] += v * sf
+    for name, v in _counter(write_csv, "WRITE_SIZE"):
+        st = stage_of(name)
+        if st:
+            acc[st][2] += 1
+            acc[st][3] += v * sw
+    res = {"calibration": {"copy_bytes": nbytes, "fetch_KiB": cf, "write_KiB": cw,
+                           "fetch_bytes_per_KiB": sf, "write_bytes_per_KiB": sw},
+           "command": command}
+    lines = ["# HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)", "", "Commands:", "",
+             "```", command, "```", "",
+             f"Calibration: a plain copy of {nbytes} B reads FETCH_SIZE = {cf:.0f} KiB and "
+             f"WRITE_SIZE = {cw:.0f} KiB; factors {sf:.1f} and {sw:.1f} bytes per counted KiB.", "",
+             "| stage | launches | read MB/launch | write MB/launch | total MB/launch |",
+             "|---|---:|---:|---:|---:|"]
+    for st in sorted(acc):
+        nf, fb, nw, wb = acc[st]
+        rd = fb / max(nf, 1)
+        wr = wb / max(nw, 1)
+        res[st] = {"launches": nf, "read_bytes_per_launch": rd, "write_bytes_per_launch": wr,
+                   "bytes_per_launch": rd + wr}
+        lines.append(f"| {st} | {nf} | {rd / 1e6:.1f} | {wr / 1e6:.1f} | {(rd + wr) / 1e6:.1f} |")
+    with open(out_json, "w") as f:
+        json.dump(res, f, indent=1)
+    return "\n".join(lines) + "\n"
+
+
+if __name__ == "__main__":
+    cmd = sys.argv[1] if len(sys.argv) > 1 else ""
+    if cmd == "copy":
+        copy(float(sys.argv[2]) if len(sys.argv) > 2 else 1.0)
+    elif cmd == "trace":
+        print(trace_summary(sys.argv[2], sys.argv[3]), end="")
+    elif cmd == "traffic":
+        print(traffic(*sys.argv[2:9]), end="")
+    else:
+        print(__doc__)
+        sys.exit(2)
