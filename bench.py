@@ -72,7 +72,7 @@ def roofline(tim: dict, ms_per_step: float, zups: float):
         raise RuntimeError("stage timers recorded no time")
     achieved = d["bytes"] / (avg_ms * 1e-3) / 1e9
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    pmc = os.path.join(ROOT, "profiles", "hbm_traffic_bw256_fp64.json")
     if os.path.exists(pmc):
         try:
             traffic = json.load(open(pmc)).get(dom, {}).get("bytes_per_launch")

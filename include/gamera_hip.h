@@ -137,6 +137,20 @@ int  gam_num_stages(void);
  * read once, outputs written once); used for roofline reporting. */
 int  gam_stage_bytes(gam_handle *h, int stage, double *bytes);
 
+/* --- launch tuning --------------------------------------------------------- */
+/* No counterpart in the reference (its loops are OpenMP over k,j with i
+ * vectorised).  The stencil stages (bflux_half, bflux_post, mom2face, emf,
+ * flux_i, flux_j, flux_k, update) launch 256-thread blocks that each cover
+ * a ti x tj x tk brick of their index box; 'fast' (0 i, 1 j, 2 k) is the axis
+ * along which consecutive bricks of one XCD follow each other, and fast = 3
+ * gives each block 256 consecutive points of the box in i-fastest order.
+ * Adding 4 to fast keeps the hardware's block order (block b to XCD b % 8)
+ * instead of giving each XCD one contiguous run of bricks.  Results do not
+ * depend on the tile; only speed does.  gam_set_tile returns -1 for other
+ * stages, ti*tj*tk != 256 or fast outside 0..7.  tile4 receives {ti, tj, tk, fast}. */
+int  gam_set_tile(gam_handle *h, int stage, int ti, int tj, int tk, int fast);
+int  gam_get_tile(gam_handle *h, int stage, int *tile4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,7 @@
 #include "gamera_hip.h"
 #include "gam_common.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -16,12 +17,12 @@
 
 // ----------------------------------------------------------------- stages
 enum Stage {
-    ST_CLOCK = 0, ST_PREDICT, ST_BHALF, ST_MOM2FACE, ST_EMF, ST_FLUX_I, ST_FLUX_J, ST_FLUX_K,
-    ST_CT, ST_UPDATE, ST_DTFINAL, ST_BC, ST_BPOST, ST_CELLDT, ST_N
+    ST_CLOCK = 0, ST_PREDICT, ST_BHALF, ST_MOM2FACE, ST_EMF_I, ST_EMF_J, ST_EMF_K, ST_FLUX_I, ST_FLUX_J,
+    ST_FLUX_K, ST_CT, ST_UPDATE, ST_DTFINAL, ST_BC, ST_BPOST, ST_CELLDT, ST_N
 };
 static const char *kStageNames[ST_N] = {
-    "clock", "predictor", "bflux_half", "mom2face", "emf", "flux_i", "flux_j", "flux_k",
-    "ct_update", "update", "dt_final", "bcs", "bflux_post", "celldt"};
+    "clock", "predictor", "bflux_half", "mom2face", "emf_i", "emf_j", "emf_k", "flux_i", "flux_j",
+    "flux_k", "ct_update", "update", "dt_final", "bcs", "bflux_post", "celldt"};
 
 enum { G_XYZ = 0, G_XYZCC, G_XFC, G_FACE, G_TF, G_EDGE, G_TE, G_TEB, G_VOL, G_DI, G_DJ, G_DK, G_N };
 
@@ -39,7 +40,9 @@ struct gam_handle {
     double *hGas = nullptr, *hmag = nullptr, *hB = nullptr;
     double *Vf = nullptr, *E = nullptr, *Flx = nullptr, *partial = nullptr;
     WK wk = {};                // predictor-pass stencil quantities
-    int npart = 0;
+    Tile tile[ST_N] = {};       // launch tiles of the tiled stages
+    int npart = 0;             // blocks of the 1-D CellDT launch
+    int pcap = 0;              // capacity of partial (block minima / sums)
     DClock *clk = nullptr;      // device
     DClock *clk_host = nullptr; // pinned
     std::string err;
@@ -106,6 +109,30 @@ struct StageTimer {
 
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
+// stages whose kernels take a Tile, and the default shapes (256 threads each)
+static bool tiled_stage(int st) {
+    return st == ST_BHALF || st == ST_BPOST || st == ST_MOM2FACE || st == ST_EMF_I || st == ST_EMF_J ||
+           st == ST_EMF_K || st == ST_FLUX_I || st == ST_FLUX_J || st == ST_FLUX_K || st == ST_UPDATE;
+}
+static void default_tiles(gam_handle *h) {
+    // measured best on MI355X for the 256^3 blast wave (ms per launch, stage
+    // timers): flux_j/flux_k/emf bricks halve their HBM traffic against
+    // row-shaped blocks; the full-box kernels keep plain row-major blocks
+    h->tile[ST_BHALF] = {256, 1, 1, TILE_LINEAR | TILE_NOXCD};
+    h->tile[ST_BPOST] = {256, 1, 1, TILE_LINEAR | TILE_NOXCD};
+    h->tile[ST_MOM2FACE] = {256, 1, 1, TILE_LINEAR};
+    h->tile[ST_EMF_I] = {64, 1, 4, TILE_NOXCD};
+    h->tile[ST_EMF_J] = {256, 1, 1, TILE_LINEAR | TILE_NOXCD};
+    h->tile[ST_EMF_K] = {32, 8, 1, 1};
+    h->tile[ST_UPDATE] = {256, 1, 1, TILE_LINEAR | TILE_NOXCD};
+    h->tile[ST_FLUX_I] = {256, 1, 1, TILE_LINEAR | TILE_NOXCD};
+    h->tile[ST_FLUX_J] = {32, 8, 1, 0};
+    h->tile[ST_FLUX_K] = {64, 1, 4, 2};
+}
+static inline dim3 tgrid(const gam_handle *h, int st, int nx, int ny, int nz) {
+    return dim3((unsigned)tile_blocks(h->tile[st], nx, ny, nz));
+}
+
 // ----------------------------------------------------------------- lifetime
 extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handle **out) {
     if (!dims || !par || !out) return -1;
@@ -163,8 +190,10 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
         for (int q = 0; q < 11; q++)
             if (hipMemset(z[q], 0, zn[q] * sizeof(double)) != hipSuccess) return cleanup(-2);
     }
+    default_tiles(h);
     h->npart = (int)((G + 255) / 256);
-    if (hipMalloc(&h->partial, (size_t)h->npart * sizeof(double)) != hipSuccess) return cleanup(-3);
+    h->pcap = std::max(h->npart, tile_blocks(h->tile[ST_UPDATE], g.Ni, g.Nj, g.Nk));
+    if (hipMalloc(&h->partial, (size_t)h->pcap * sizeof(double)) != hipSuccess) return cleanup(-3);
     if (hipMalloc(&h->clk, sizeof(DClock)) != hipSuccess) return cleanup(-3);
     if (hipHostMalloc(&h->clk_host, sizeof(DClock), hipHostMallocDefault) != hipSuccess) return cleanup(-3);
     DClock c = {};
@@ -397,9 +426,14 @@ static int launch_celldt(gam_handle *h, const double *Gas, const double *B) {
 // Final reduction of h->partial + CalcDT checks (calc), after the
 // UpdateStateData time update (advance).  h->partial must hold a complete set
 // of block minima when calc != 0 (from k_update or launch_celldt).
-static void launch_dtfinal(gam_handle *h, int advance, int calc) {
+static int update_blocks(const gam_handle *h) {
+    return tile_blocks(h->tile[ST_UPDATE], h->gv.Ni, h->gv.Nj, h->gv.Nk);
+}
+
+// np = number of block minima in h->partial (calc != 0)
+static void launch_dtfinal(gam_handle *h, int advance, int calc, int np) {
     StageTimer t(h, ST_DTFINAL);
-    hipLaunchKernelGGL(k_dtfinal, 1, 1024, 0, h->stream, h->clk, h->partial, h->npart, advance, calc);
+    hipLaunchKernelGGL(k_dtfinal, 1, 1024, 0, h->stream, h->clk, h->partial, np, advance, calc);
 }
 
 template <int RM, int BO, int MH>
@@ -407,11 +441,28 @@ static void launch_flux(gam_handle *h, int d) {
     const GV &g = h->gv;
     int S = h->s;
     long nx = g.Nip + (d == 0), ny = g.Njp + (d == 1), nz = (h->mp.do25D ? 1 : g.Nkp) + (d == 2);
-    dim3 gr = grid1(nx * ny * nz);
+    const Tile tl = h->tile[ST_FLUX_I + d];
+    dim3 gr = tgrid(h, ST_FLUX_I + d, (int)nx, (int)ny, (int)nz);
     StageTimer t(h, ST_FLUX_I + d);
-    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
-    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH>), gr, 256, 0, h->stream, g, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
+    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
+    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
+}
+
+template <int ED, int RM, int BO>
+static void launch_emf(gam_handle *h) {
+    const GV &g = h->gv;
+    const int st = ST_EMF_I + ED;
+    StageTimer t(h, st);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<ED, RM, BO>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
+                       256, 0, h->stream, g, h->tile[st], h->mp, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
+}
+
+template <int RM, int BO>
+static void launch_emfs(gam_handle *h) {
+    launch_emf<0, RM, BO>(h);
+    launch_emf<1, RM, BO>(h);
+    launch_emf<2, RM, BO>(h);
 }
 
 template <int RM, int BO, int MH>
@@ -430,18 +481,11 @@ static int advance(gam_handle *h, int fuse_dt) {
     { StageTimer t(h, ST_CLOCK); hipLaunchKernelGGL(k_clock_begin, 1, 64, 0, h->stream, h->clk); }
     { StageTimer t(h, ST_PREDICT);
       hipLaunchKernelGGL(k_predict, grid1(g.nf), 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->Gas[O], h->mag[S], h->mag[O], h->hGas, h->hmag, h->wk); }
-    { StageTimer t(h, ST_BHALF); hipLaunchKernelGGL(k_b2f, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->hmag, h->hB); }
+    { StageTimer t(h, ST_BHALF); hipLaunchKernelGGL(k_b2f, tgrid(h, ST_BHALF, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_BHALF], h->clk, h->hmag, h->hB); }
     if (m.doMHD) {
-        { StageTimer t(h, ST_MOM2FACE); hipLaunchKernelGGL(k_vf, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->wk.VdV, h->Vf); }
-        long ne = (long)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
-        StageTimer t(h, ST_EMF);
-        if (m.rmeth == 1) {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<1, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
-        } else {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 1>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<0, 0>), grid1(ne), 256, 0, h->stream, g, m, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
-        }
+        { StageTimer t(h, ST_MOM2FACE); hipLaunchKernelGGL(k_vf, tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_MOM2FACE], h->clk, h->wk.VdV, h->Vf); }
+        if (m.rmeth == 1) { if (m.doBoris) launch_emfs<1, 1>(h); else launch_emfs<1, 0>(h); }
+        else { if (m.doBoris) launch_emfs<0, 1>(h); else launch_emfs<0, 0>(h); }
     }
     if (m.doMHD) {
         if (m.rmeth == 1) { if (m.doBoris) launch_fluxes<1, 1, 1>(h); else launch_fluxes<1, 0, 1>(h); }
@@ -456,17 +500,18 @@ static int advance(gam_handle *h, int fuse_dt) {
     }
     {
         StageTimer t(h, ST_UPDATE);
-        dim3 gr(h->npart);
+        dim3 gr(update_blocks(h));
+        const Tile tu = h->tile[ST_UPDATE];
         if (m.doMHD) {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1>), gr, 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1>), gr, 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1>), gr, 256, 0, h->stream, g, tu, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1>), gr, 256, 0, h->stream, g, tu, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
         } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0>), gr, 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0>), gr, 256, 0, h->stream, g, tu, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
         }
     }
     // CopyState State->oState is the buffer swap
     h->s = O;
-    if (!fuse_dt) launch_dtfinal(h, /*advance*/ 1, /*calc*/ 0);
+    if (!fuse_dt) launch_dtfinal(h, /*advance*/ 1, /*calc*/ 0, 0);
     HIPCHK(h, hipGetLastError());
     return 0;
 }
@@ -475,7 +520,7 @@ static int bcs(gam_handle *h) {
     const GV &g = h->gv;
     const int S = h->s;
     { StageTimer t(h, ST_BC); hipLaunchKernelGGL(k_bc, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, h->Gas[S], h->mag[S], nullptr); }
-    { StageTimer t(h, ST_BPOST); hipLaunchKernelGGL(k_b2f, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->mag[S], h->B[S]); }
+    { StageTimer t(h, ST_BPOST); hipLaunchKernelGGL(k_b2f, tgrid(h, ST_BPOST, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_BPOST], h->clk, h->mag[S], h->B[S]); }
     HIPCHK(h, hipGetLastError());
     return 0;
 }
@@ -491,7 +536,7 @@ extern "C" int gam_calc_dt(gam_handle *h, double *dt, int *status) {
     if (!h) return -1;
     int rc = launch_celldt(h, h->Gas[h->s], h->B[h->s]);
     if (rc) return rc;
-    launch_dtfinal(h, 0, 1);
+    launch_dtfinal(h, 0, 1, h->npart);
     gam_clock c;
     rc = gam_get_clock(h, &c);
     if (rc) return rc;
@@ -516,7 +561,7 @@ extern "C" int gam_step(gam_handle *h, int nsteps) {
     for (int n = 0; n < nsteps; n++) {
         int rc = advance(h, 1);
         if (rc) return rc;
-        launch_dtfinal(h, 1, 1);   // block minima were left in h->partial by k_update
+        launch_dtfinal(h, 1, 1, update_blocks(h));   // block minima were left in h->partial by k_update
         if ((rc = bcs(h))) return rc;
     }
     gam_clock c;
@@ -540,14 +585,14 @@ extern "C" int gam_hatch(gam_handle *h) {
     if (rc) return rc;
     c0.dt0 = 0.0; c0.ts = 0; c0.status = 0;
     if ((rc = gam_set_clock(h, &c0))) return rc;
-    hipLaunchKernelGGL(k_b2f, grid1(g.nc), 256, 0, h->stream, g, h->clk, h->mag[S], h->B[S]);
+    hipLaunchKernelGGL(k_b2f, tgrid(h, ST_BPOST, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_BPOST], h->clk, h->mag[S], h->B[S]);
     hipLaunchKernelGGL(k_bc, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, h->Gas[S], h->mag[S], h->B[S]);
     HIPCHK(h, hipMemcpyAsync(h->Gas[O], h->Gas[S], 5 * gsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->mag[O], h->mag[S], 3 * fsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->B[O], h->B[S], 3 * gsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(h, hipGetLastError());
     if ((rc = launch_celldt(h, h->Gas[S], h->B[S]))) return rc;
-    launch_dtfinal(h, 0, 1);
+    launch_dtfinal(h, 0, 1, h->npart);
     gam_clock c;
     if ((rc = gam_get_clock(h, &c))) return rc;
     c.oStateTime = c.stateTime - c.dt;
@@ -561,7 +606,7 @@ extern "C" int gam_divb(gam_handle *h, double *tot) {
     const GV &g = h->gv;
     long n = (long)g.Nip * g.Njp * g.Nkp;
     int nb = (int)((n + 255) / 256);
-    if (nb > h->npart) return fail(h, -1, "gam_divb: partial buffer too small");
+    if (nb > h->pcap) return fail(h, -1, "gam_divb: partial buffer too small");
     hipLaunchKernelGGL(k_divb, nb, 256, 0, h->stream, g, h->mag[h->s], h->partial);
     HIPCHK(h, hipGetLastError());
     std::vector<double> hp(nb);
@@ -605,6 +650,36 @@ extern "C" int gam_get_timing(gam_handle *h, int stage, double *ms_total, int64_
     return 0;
 }
 
+extern "C" int gam_set_tile(gam_handle *h, int stage, int ti, int tj, int tk, int fast) {
+    if (!h) return -1;
+    if (stage < 0 || stage >= ST_N || !tiled_stage(stage))
+        return fail(h, -1, "gam_set_tile: stage %d takes no tile", stage);
+    if (ti < 1 || tj < 1 || tk < 1 || ti * tj * tk != 256 || fast < 0 || fast > (TILE_LINEAR | TILE_NOXCD))
+        return fail(h, -1, "gam_set_tile: need ti*tj*tk == 256 and fast in 0..7 (got %d %d %d %d)", ti, tj, tk, fast);
+    int rc = gam_sync(h);   // launches already queued keep the tile they were given
+    if (rc) return rc;
+    const Tile t = {ti, tj, tk, fast};
+    if (stage == ST_UPDATE) {   // one block minimum per update block
+        const int nb = tile_blocks(t, h->gv.Ni, h->gv.Nj, h->gv.Nk);
+        if (nb > h->pcap) {
+            double *p = nullptr;
+            HIPCHK(h, hipMalloc(&p, (size_t)nb * sizeof(double)));
+            HIPCHK(h, hipFree(h->partial));
+            h->partial = p;
+            h->pcap = nb;
+        }
+    }
+    h->tile[stage] = t;
+    return 0;
+}
+
+extern "C" int gam_get_tile(gam_handle *h, int stage, int *tile4) {
+    if (!h || !tile4 || stage < 0 || stage >= ST_N || !tiled_stage(stage)) return -1;
+    tile4[0] = h->tile[stage].ti; tile4[1] = h->tile[stage].tj; tile4[2] = h->tile[stage].tk;
+    tile4[3] = h->tile[stage].fast;
+    return 0;
+}
+
 extern "C" const char *gam_stage_name(int stage) { return (stage >= 0 && stage < ST_N) ? kStageNames[stage] : ""; }
 extern "C" int gam_num_stages(void) { return ST_N; }
 
@@ -619,15 +694,21 @@ extern "C" int gam_stage_bytes(gam_handle *h, int stage, double *bytes) {
     const double nE = (double)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
     double b = 0;
     switch (stage) {
-    case ST_PREDICT: b = F * (3 + 3 + 3) + G * (5 + 5 + 5); break;                 // mag n,o -> hf ; Gas n,o -> hf
+    case ST_PREDICT:   // mag n,o, face -> hmag, bh, bn ; Gas n,o, vol -> hGas, Wh, Wn, VdV
+        b = F * (3 + 3 + 3 + 3 + 3 + 3) + G * (5 + 5 + 1 + 5 + 5 + 5 + 3); break;
     case ST_BHALF: case ST_BPOST: b = F * (3 + 9) + G * (3 + 1 + 3); break;      // mag, xfc ; xyzcc, vol -> B
-    case ST_MOM2FACE: b = G * (4 + 1 + 9); break;                                 // rho,M, vol -> Vf (9)
-    case ST_EMF: b = G * (9 + 1) + nE * (3 + 3 + 3 + 6 * 3 + 4 * 3 + 3 + 3); break; // Vf, rho ; face, mag n/h, Te, Teb, edge -> E
+    case ST_MOM2FACE: b = G * (3 + 1 + 9); break;                                 // VdV, vol -> Vf (9)
+    case ST_EMF_I: case ST_EMF_J: case ST_EMF_K: {   // Vf (dT1), rho ; face, bn, bh (dT1, dT2), Te, Teb, edge -> E
+        const int d = stage - ST_EMF_I;
+        const double nEd = (double)(g.Nip + (d != 0)) * (g.Njp + (d != 1)) * (g.Nkp + (d != 2));
+        b = G * (3 + 1) + nEd * (2 + 2 + 2 + 6 + 4 + 1 + 1); break;
+    }
     case ST_FLUX_I: b = (G * (5 + 5 + 3 + 3 + 1)) + nI * (9 + 1 + 1 + 8); break;
     case ST_FLUX_J: b = (G * (5 + 5 + 3 + 3 + 1)) + nJ * (9 + 1 + 1 + 8); break;
     case ST_FLUX_K: b = (G * (5 + 5 + 3 + 3 + 1)) + nK * (9 + 1 + 1 + 8); break;
     case ST_CT: b = F * (3 + 3) + G * 3; break;
-    case ST_UPDATE: b = G * (5 + 3 + 3 + 1 + 3 + 3 + 5 + 3 + 3) + F * (3 + 9) + nA * 0; break;
+    case ST_UPDATE:   // Gas n, fluxes (8 per direction), xyzcc, vol, di/dj/dk ; magFlux new, xfc -> Gas, Bxyz
+        b = G * (5 + 24 + 3 + 1 + 3 + 5 + 3 + (h->mp.doBoris ? 3 : 0)) + F * (3 + 9); break;
     case ST_BC: b = 2.0 * (G - nA) * 5 + 2.0 * (F - nE) * 3; break;
     case ST_CELLDT: b = G * (5 + 3 + 3); break;
     default: b = 0;

@@ -154,3 +154,59 @@ __device__ __forceinline__ void block_c2p(const MP &m, const double *c, double *
 }
 
 __host__ __device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Launch tile: a block of ti*tj*tk threads (== blockDim.x) covers a brick of
+// an nx*ny*nz index box, so a stencil along j or k is served from the bytes the
+// block's own waves fetch instead of from 8 separate passes over HBM.  Tiles are
+// numbered with axis (fast & 3) varying fastest; (fast & 3) == TILE_LINEAR
+// instead gives each block 256 consecutive points of the box in i-fastest order
+// (no ragged bricks when nx is not a multiple of ti).  The hardware dispatches
+// block b to XCD b % 8; unless fast & TILE_NOXCD, block b takes tile
+// (b % 8) * per + b / 8, so each XCD walks one contiguous run of tiles and
+// neighbouring tiles meet in the same L2; with TILE_NOXCD block b takes tile b.
+struct Tile { int ti, tj, tk, fast; };
+#define TILE_LINEAR 3
+#define TILE_NOXCD 4
+
+__device__ __forceinline__ int tile_of_block(const Tile &t, int nt) {
+    if (t.fast & TILE_NOXCD) return (int)blockIdx.x;
+    const int per = cdiv(nt, 8);
+    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+__host__ __device__ inline int tile_count(const Tile &t, int nx, int ny, int nz) {
+    if ((t.fast & 3) == TILE_LINEAR) return cdiv(nx * ny * nz, t.ti * t.tj * t.tk);
+    return cdiv(nx, t.ti) * cdiv(ny, t.tj) * cdiv(nz, t.tk);
+}
+__host__ __device__ inline int tile_blocks(const Tile &t, int nx, int ny, int nz) {
+    const int nt = tile_count(t, nx, ny, nz);
+    return (t.fast & TILE_NOXCD) ? nt : 8 * cdiv(nt, 8);
+}
+
+// (a, b, c) of this thread in the box; false if outside
+__device__ __forceinline__ bool tile_pos(const Tile &t, int nx, int ny, int nz, int &a, int &b, int &c) {
+    if ((t.fast & 3) == TILE_LINEAR) {
+        const int n = nx * ny * nz, bs = t.ti * t.tj * t.tk, nt = cdiv(n, bs);
+        const int q = tile_of_block(t, nt);
+        const int idx = q * bs + (int)threadIdx.x;
+        if (q >= nt || idx >= n) return false;
+        a = idx % nx;
+        const int r = idx / nx;
+        b = r % ny;
+        c = r / ny;
+        return true;
+    }
+    const int ntx = cdiv(nx, t.ti), nty = cdiv(ny, t.tj), ntz = cdiv(nz, t.tk);
+    const int q = tile_of_block(t, ntx * nty * ntz);
+    if (q >= ntx * nty * ntz) return false;
+    int tx, ty, tz;
+    const int f = t.fast & 3;
+    if (f == 0) { tx = q % ntx; int r = q / ntx; ty = r % nty; tz = r / nty; }
+    else if (f == 1) { ty = q % nty; int r = q / nty; tx = r % ntx; tz = r / ntx; }
+    else { tz = q % ntz; int r = q / ntz; tx = r % ntx; ty = r / ntx; }
+    const int l = threadIdx.x;
+    a = tx * t.ti + l % t.ti;
+    b = ty * t.tj + (l / t.ti) % t.tj;
+    c = tz * t.tk + l / (t.ti * t.tj);
+    return a < nx && b < ny && c < nz;
+}

@@ -115,12 +115,11 @@ __global__ void __launch_bounds__(256) k_predict(GV g, MP m, const DClock *__res
     for (int x = 0; x < 3; x++) w.VdV[(size_t)nc * x + c] = vol * pQ[1 + x] / D;
 }
 
-__global__ void __launch_bounds__(256) k_b2f(GV g, const DClock *__restrict__ clk, const double *__restrict__ mag,
+__global__ void __launch_bounds__(256) k_b2f(GV g, Tile t, const DClock *__restrict__ clk, const double *__restrict__ mag,
                                              double *__restrict__ B) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.nc || clk->halt) return;
     int i, j, k;
-    decode_c(g, idx, i, j, k);
+    if (!tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
+    const int idx = ci(g, i, j, k);
     double b[3];
     cell_bxyz(g, mag, i, j, k, b);
 #pragma unroll
@@ -148,12 +147,11 @@ __device__ __forceinline__ void mom2face(const GV &g, const double *__restrict__
 }
 
 // Vf arrays: VfA[dT1][xyz] stored as slab (3*dT1 + xyz)
-__global__ void __launch_bounds__(256) k_vf(GV g, const DClock *__restrict__ clk, const double *__restrict__ VdV,
+__global__ void __launch_bounds__(256) k_vf(GV g, Tile t, const DClock *__restrict__ clk, const double *__restrict__ VdV,
                                             double *__restrict__ Vf) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.nc || clk->halt) return;
     int i, j, k;
-    decode_c(g, idx, i, j, k);
+    if (!tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
+    const int idx = ci(g, i, j, k);
     const int a0 = g.nG;
     const size_t nc = g.nc;
     double v[3];
@@ -251,24 +249,20 @@ __device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, 
     return e * ed;
 }
 
-// Edge box [is,ie+1]x[js,je+1]x[ks,ke+1]; each thread does the (up to) three
-// edges at its lower corner.  E slabs are written only on active edges; k_ct
-// treats the rest as zero (CalcElecField zeroes E, fields.F90:89).
-template <int RM, int BORIS>
-__global__ void __launch_bounds__(256) k_emf(GV g, MP m, const DClock *__restrict__ clk, const double *__restrict__ Vf,
+// Edges of direction ED over their active box (fields.F90:110-225): I edges
+// [is,ie]x[js,je+1]x[ks,ke+1], J edges [is,ie+1]x[js,je]x[ks,ke+1], K edges
+// [is,ie+1]x[js,je+1]x[ks,ke]; E of edge ED at its lower corner (i,j,k).  E
+// slabs are written only on active edges; k_ct treats the rest as zero
+// (CalcElecField zeroes E, fields.F90:89).  One kernel per direction, so each
+// gets a launch brick shaped for its two stencil axes (dT1 and dT2).
+template <int ED, int RM, int BORIS>
+__global__ void __launch_bounds__(256) k_emf(GV g, Tile t, MP m, const DClock *__restrict__ clk, const double *__restrict__ Vf,
                                              const double *__restrict__ hGas, const double *__restrict__ bn,
                                              const double *__restrict__ bh, double *__restrict__ E) {
-    const int nx = g.Nip + 1, ny = g.Njp + 1, nz = g.Nkp + 1;
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nx * ny * nz || clk->halt) return;
-    int a = idx % nx, r = idx / nx, b = r % ny, cz = r / ny;
+    int a, b, cz;
+    if (!tile_pos(t, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2), a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
-    const double dt = clk->dt;
-    const int c = ci(g, i, j, k);
-    const size_t nc = g.nc;
-    if (a < g.Nip) E[c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, bn, bh, 0, i, j, k);
-    if (b < g.Njp) E[nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, bn, bh, 1, i, j, k);
-    if (cz < g.Nkp) E[2 * nc + c] = emf_edge<RM, BORIS>(g, m, dt, Vf, hGas, bn, bh, 2, i, j, k);
+    E[(size_t)g.nc * ED + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, clk->dt, Vf, hGas, bn, bh, ED, i, j, k);
 }
 
 // ----------------------------------------------------------------- fluxes
@@ -321,15 +315,14 @@ __device__ __forceinline__ void gaskin(const MP &m, double sg, const double *P5,
 // DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
 // Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
 template <int DN, int RM, int BORIS, int MHD>
-__global__ void __launch_bounds__(256) k_flux(GV g, MP m, const DClock *__restrict__ clk,
+__global__ void __launch_bounds__(256) k_flux(GV g, Tile t, MP m, const DClock *__restrict__ clk,
                                               const double *__restrict__ Wn, const double *__restrict__ nB,
                                               const double *__restrict__ Wh, const double *__restrict__ hB,
                                               const double *__restrict__ hmag, const double *__restrict__ hGas,
                                               double *__restrict__ Flx) {
     const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nx * ny * nz || clk->halt) return;
-    int a = idx % nx, r = idx / nx, b = r % ny, cz = r / ny;
+    int a, b, cz;
+    if (!tile_pos(t, nx, ny, nz, a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
     const size_t nc = g.nc, nf = g.nf;
     const int c = ci(g, i, j, k), f = fi(g, i, j, k);
@@ -555,17 +548,16 @@ __device__ __forceinline__ bool in_dt_box(const GV &g, int i, int j, int k) {
 // Flux2Deltas + applyUpdate (Reynolds, Bxyz of the new fluxes, Maxwell/Boris)
 // + CellDT min.  Writes the new Gas/Bxyz into the O buffers; ghosts copy Gas n.
 template <int BORIS, int MHD>
-__global__ void __launch_bounds__(256) k_update(GV g, MP m, const DClock *__restrict__ clk,
+__global__ void __launch_bounds__(256) k_update(GV g, Tile t, MP m, const DClock *__restrict__ clk,
                                                 const double *__restrict__ nGas, const double *__restrict__ nB,
                                                 const double *__restrict__ Flx, const double *__restrict__ magNew,
                                                 double *__restrict__ oGas, double *__restrict__ oB,
                                                 double *__restrict__ partial) {
     if (clk->halt) return;
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
     double dtl = GAM_HUGE;
-    if (idx < g.nc) {
-        int i, j, k;
-        decode_c(g, idx, i, j, k);
+    int i, j, k;
+    if (tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k)) {
+        const int idx = ci(g, i, j, k);
         const size_t nc = g.nc;
         double U[5], Bn[3] = {0, 0, 0};
 #pragma unroll

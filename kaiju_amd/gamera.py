@@ -95,6 +95,8 @@ def lib():
         "gam_stage_name": (C.c_char_p, [C.c_int]),
         "gam_num_stages": (C.c_int, []),
         "gam_stage_bytes": (C.c_int, [H, C.c_int, C.POINTER(C.c_double)]),
+        "gam_set_tile": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "gam_get_tile": (C.c_int, [H, C.c_int, C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -335,6 +337,23 @@ class GamApp:
     # ---------------------------------------------------------------- timers
     def set_timing(self, on: bool):
         self._chk(lib().gam_set_timing(self.h, int(on)), "gam_set_timing")
+
+    def stage_id(self, name: str) -> int:
+        L = lib()
+        for s in range(L.gam_num_stages()):
+            if L.gam_stage_name(s).decode() == name:
+                return s
+        raise KeyError(name)
+
+    def set_tile(self, stage: str, ti: int, tj: int, tk: int, fast: int):
+        """Launch brick of a stencil stage (ti*tj*tk == 256); speed only."""
+        self._chk(lib().gam_set_tile(self.h, self.stage_id(stage), ti, tj, tk, fast), "gam_set_tile")
+
+    def get_tile(self, stage: str) -> tuple:
+        t = (C.c_int * 4)()
+        if lib().gam_get_tile(self.h, self.stage_id(stage), t) != 0:
+            raise KeyError(f"stage {stage} takes no tile")
+        return tuple(t)
 
     def timings(self) -> dict:
         L = lib()

@@ -155,3 +155,46 @@ def test_multistep_drift_and_divb(case):
         if case.model.doMHD:
             scale = float(np.abs(oc.state.magFlux).max()) * case.zones
             assert abs(app.divb() - divb0) <= 1e-13 * scale
+
+
+TILE_SETS = [
+    # every stencil stage on a row-shaped brick (the plain 1-D launch order)
+    {st: (256, 1, 1, 0) for st in ("bflux_half", "bflux_post", "mom2face", "emf_i", "emf_j", "emf_k", "flux_i",
+                                   "flux_j", "flux_k", "update")},
+    # linear blocks (256 consecutive points) everywhere, hardware block order
+    {st: (256, 1, 1, 7) for st in ("bflux_half", "bflux_post", "mom2face", "emf_i", "emf_j", "emf_k", "flux_i",
+                                   "flux_j", "flux_k", "update")},
+    # bricks that leave partial tiles on every edge of the small boxes
+    {"bflux_half": (8, 8, 4, 1), "bflux_post": (4, 8, 8, 2), "mom2face": (32, 2, 4, 0), "emf_i": (8, 4, 8, 1),
+     "emf_j": (4, 8, 8, 0), "emf_k": (2, 16, 8, 2), "flux_i": (2, 16, 8, 2), "flux_j": (4, 4, 16, 0),
+     "flux_k": (1, 16, 16, 5), "update": (1, 16, 16, 6)},
+]
+
+
+@pytest.mark.parametrize("tset", range(len(TILE_SETS)))
+def test_launch_tiles_change_no_bits(case, tset):
+    """The launch brick of the stencil stages only reorders threads: two steps
+    with other tiles give bit-identical State to the default tiles."""
+    say(f"[tiles] {case.name} set {tset}")
+    oc = oracle.OracleCase(case)
+    with app_from_oracle(oc) as a1, app_from_oracle(oc) as a2:
+        for st, t in TILE_SETS[tset].items():
+            a2.set_tile(st, *t)
+            assert a2.get_tile(st) == t
+        assert a1.steps(2) == 0 and a2.steps(2) == 0
+        s1, s2 = a1.get_state(0), a2.get_state(0)
+        for k in ("Gas", "magFlux", "Bxyz"):
+            assert np.array_equal(s1[k], s2[k]), k
+        assert a1.Model.dt == a2.Model.dt
+
+
+def test_set_tile_rejects_bad_shapes(case):
+    say(f"[tiles-bad] {case.name}")
+    from kaiju_amd.gamera import GamApp, HipStepError
+    with GamApp(case) as app:
+        with pytest.raises(HipStepError):
+            app.set_tile("emf_i", 16, 4, 2, 0)    # 128 threads
+        with pytest.raises(HipStepError):
+            app.set_tile("flux_i", 64, 4, 1, 8)   # no mode 8
+        with pytest.raises(HipStepError):
+            app.set_tile("ct_update", 64, 4, 1, 0)   # not a tiled stage
