@@ -8,7 +8,10 @@ CFL 0.3, 7-point upwind reconstruction with the PDM limiter, FP64.  Geometry and
 initial condition are built on the GPU before the timed region.
 
 One step = one stepGamera: predictor, E field, stresses, CT/update, CalcDT,
-periodic BCs.  value = steps * 256^3 / wall time of the timed steps (the
+periodic BCs.  The headline value is the plain FP64 step; the same workload in
+FP32-bulk form (FP64 kept for magFlux/CT, the cell update and dt) is reported
+beside it under "fp32_bulk" with its L1 error per variable against the FP64
+run after the same steps.  value = steps * 256^3 / wall time of the timed steps (the
 reference's kZCs definition x 1e3, reference src/gamera/output.F90:64-85).
 
 Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs one GPU.
@@ -27,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
 ALGO_BYTES_PER_ZONE = 824.0      # FP64 step: 192 B state + 632 B geometry per zone-update
+# FP32-bulk: Gas 20 B x 3 touches + magFlux (FP64) 24 B x 3 + FP32 bulk geometry
+# (volume, face 3, Tf 27, Te 18, Teb 12, edge 3, di/dj/dk 3: 67 x 4 B) + FP64
+# positions and volume (xfc 9, xyzcc 3, volume 1: 13 x 8 B)
+ALGO_BYTES_PER_ZONE_FP32 = 60.0 + 72.0 + 67 * 4.0 + 13 * 8.0
 METRIC = "zone-updates/s per MI355X: plain FP64 vs FP32/learned at fixed L1 error"
 
 
@@ -34,10 +41,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gpu_run(n: int, steps: int, warmup: int):
+def gpu_run(n: int, steps: int, warmup: int, precision: int = 0):
     from kaiju_amd import configs
     from kaiju_amd.gamera import GamApp
-    spec = configs.blast_wave(n, steps=steps)
+    spec = configs.blast_wave(n, steps=steps, precision=precision)
     t0 = time.perf_counter()
     app = GamApp(spec)
     app.InitModel()
@@ -58,8 +65,44 @@ def gpu_run(n: int, steps: int, warmup: int):
     app.set_timing(False)
     divb = app.divb()
     wall = t2 - t1
-    log(f"[bench] {steps} steps in {wall:.3f} s, t = {app.Model.t:.4e}, divB = {divb:.3e}")
+    log(f"[bench] {'FP32-bulk' if precision else 'FP64'}: {steps} steps in {wall:.3f} s, t = {app.Model.t:.4e}, divB = {divb:.3e}")
     return app, spec, wall, tim, divb
+
+
+def l1_per_variable(a, b, nG):
+    """L1 error of each Gas variable over the active cells, relative to the L1
+    norm of its field (momentum components against the momentum magnitude)."""
+    import numpy as np
+    A = tuple(slice(nG, -nG) for _ in range(3))
+    a, b = a[A], b[A]
+    out = {}
+    for v, name in enumerate(("rho", "momx", "momy", "momz", "energy")):
+        grp = (1, 2, 3) if 1 <= v <= 3 else (v,)
+        den = sum(float(np.abs(b[..., q]).sum()) for q in grp) / len(grp)
+        out[name] = float(np.abs(a[..., v] - b[..., v]).sum()) / max(den, 1e-300)
+    return out
+
+
+def fp32_leg(n: int, steps: int, warmup: int, s64: dict, ms64: float):
+    """The same workload in FP32-bulk form: throughput, and the L1 error per
+    variable against the FP64 run after the same steps."""
+    import numpy as np
+    app, spec, wall, tim, divb = gpu_run(n, steps, warmup, precision=1)
+    s32 = app.get_state(0)
+    app.close()
+    zups = steps * spec.zones / wall
+    ms = wall / steps * 1e3
+    l1 = l1_per_variable(s32["Gas"], s64["Gas"], spec.nG)
+    mf = float(np.abs(s32["magFlux"] - s64["magFlux"]).sum() / np.abs(s64["magFlux"]).sum())
+    bx = float(np.abs(s32["Bxyz"] - s64["Bxyz"]).sum() / np.abs(s64["Bxyz"]).sum())
+    step_gbs = ALGO_BYTES_PER_ZONE_FP32 * zups / 1e9
+    return dict(value=round(zups, 1), ms_per_step=round(ms, 4), speedup_vs_fp64=round(ms64 / ms, 3),
+                l1_rel_vs_fp64={**{k: float(f"{v:.3e}") for k, v in l1.items()},
+                                "magFlux": float(f"{mf:.3e}"), "Bxyz": float(f"{bx:.3e}")},
+                after_steps=warmup + steps, divB_total_after=divb,
+                algo_bytes_per_zone=ALGO_BYTES_PER_ZONE_FP32, step_algo_GBs=round(step_gbs, 1),
+                step_frac=round(step_gbs / HBM_PEAK_GBS, 4),
+                stage_avg_ms={k: round(v["ms"] / v["launches"], 4) for k, v in tim.items() if v["launches"] > 0})
 
 
 def roofline(tim: dict, ms_per_step: float, zups: float):
@@ -139,6 +182,7 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=96, help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle stepping")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -146,11 +190,18 @@ def main():
         sys.exit(2)
 
     app, spec, wall, tim, divb = gpu_run(args.n, args.steps, args.warmup)
-    app.close()
     zones = spec.zones
     zups = args.steps * zones / wall
     ms = wall / args.steps * 1e3
     rf = roofline(tim, ms, zups)
+    fp32 = None
+    if not args.no_fp32:
+        s64 = app.get_state(0)
+        app.close()
+        fp32 = fp32_leg(args.n, args.steps, args.warmup, s64, ms)
+        del s64
+    else:
+        app.close()
     cpu = None
     if not args.no_cpu:
         try:
@@ -175,6 +226,7 @@ def main():
                    "cells": [args.n] * 3, "ghosts": 4, "precision": "fp64", "divB_total_after": divb},
         "roofline": rf,
         "cpu_baseline": cpu,
+        "fp32_bulk": fp32,
     }
     print(json.dumps(out), flush=True)
 

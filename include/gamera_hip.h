@@ -45,6 +45,13 @@ typedef struct {
     int32_t fixedTimestep;
     int32_t precision;      /* 0 = FP64 everywhere; 1 = FP32 bulk, FP64 CT/energy/dt   */
 } gam_params;
+/* precision = 1 (FP32 bulk): Gas, Bxyz, the predictor state, reconstruction,
+ * fluxes, EMFs and the per-face/per-cell geometry they read are FP32.  FP64 is
+ * kept for magFlux and the CT face update (E2Flux), for Flux2Deltas with the
+ * Reynolds/Maxwell/Boris cell update and its energy-to-pressure recovery with
+ * floors, for bFlux2Fld (positions stay FP64) and for the dt reduction.  The
+ * C-ABI still takes and returns FP64 host arrays: gam_put_state rounds Gas and
+ * Bxyz to FP32, gam_get_state widens them. */
 
 /* Model_T / State_T time bookkeeping that the step reads and writes. */
 typedef struct {

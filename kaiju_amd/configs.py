@@ -37,6 +37,7 @@ class ModelSpec:
     doBoris: bool = False
     doMHD: bool = True
     doCPR: bool = False
+    precision: int = 0          # 0 plain FP64; 1 FP32 bulk with FP64 CT, update and dt
 
 
 @dataclasses.dataclass

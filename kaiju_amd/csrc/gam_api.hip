@@ -25,21 +25,31 @@ static const char *kStageNames[ST_N] = {
     "flux_k", "ct_update", "update", "dt_final", "bcs", "bflux_post", "celldt"};
 
 enum { G_XYZ = 0, G_XYZCC, G_XFC, G_FACE, G_TF, G_EDGE, G_TE, G_TEB, G_VOL, G_DI, G_DJ, G_DK, G_N };
+// geometry the bulk kernels read, kept as an FP32 copy in precision 1
+static const int kG32[9] = {G_FACE, G_TF, G_EDGE, G_TE, G_TEB, G_VOL, G_DI, G_DJ, G_DK};
+enum { W_WH = 0, W_WN, W_VDV, W_BH, W_BN, W_N };
 
 struct PendingEv { int stage; hipEvent_t a, b; };
 
 struct gam_handle {
     gam_dims dims;
     gam_params par;
-    GV gv;
+    GV gv;                      // FP64 view (setup, FP64 step)
+    GVT<float> gv32;            // FP32 view of the bulk geometry (precision 1)
     MP mp;
+    int prec = 0;               // 0 FP64, 1 FP32 bulk
+    size_t es = 8;              // bytes per bulk element
     hipStream_t stream = nullptr;
     double *grid[G_N] = {};
-    double *Gas[2] = {}, *mag[2] = {}, *B[2] = {};
+    float *grid32[9] = {};
+    // bulk-typed arrays (double or float by prec); magFlux is always FP64
+    void *Gas[2] = {}, *B[2] = {};
+    double *mag[2] = {};
     int s = 0;                 // index of State in the {Gas,mag,B} pairs
-    double *hGas = nullptr, *hmag = nullptr, *hB = nullptr;
-    double *Vf = nullptr, *E = nullptr, *Flx = nullptr, *partial = nullptr;
-    WK wk = {};                // predictor-pass stencil quantities
+    void *hGas = nullptr, *hmag = nullptr, *hB = nullptr;
+    void *Vf = nullptr, *E = nullptr, *Flx = nullptr;
+    void *wk[W_N] = {};        // predictor-pass stencil quantities
+    double *partial = nullptr;
     Tile tile[ST_N] = {};       // launch tiles of the tiled stages
     int npart = 0;             // blocks of the 1-D CellDT launch
     int pcap = 0;              // capacity of partial (block minima / sums)
@@ -71,6 +81,17 @@ static int fail(gam_handle *h, int code, const char *fmt, ...) {
 
 static size_t gsz(const gam_handle *h) { return (size_t)h->gv.nc; }
 static size_t fsz(const gam_handle *h) { return (size_t)h->gv.nf; }
+
+template <typename R> static R *P(void *p) { return static_cast<R *>(p); }
+template <typename R> static const GVT<R> &view(const gam_handle *h);
+template <> const GVT<double> &view<double>(const gam_handle *h) { return h->gv; }
+template <> const GVT<float> &view<float>(const gam_handle *h) { return h->gv32; }
+template <typename R> static WKT<R> wkt(gam_handle *h) {
+    WKT<R> w;
+    w.Wh = P<R>(h->wk[W_WH]); w.Wn = P<R>(h->wk[W_WN]); w.VdV = P<R>(h->wk[W_VDV]);
+    w.bh = P<R>(h->wk[W_BH]); w.bn = P<R>(h->wk[W_BN]);
+    return w;
+}
 
 // ----------------------------------------------------------------- timers
 static hipEvent_t ev_get(gam_handle *h) {
@@ -133,13 +154,27 @@ static inline dim3 tgrid(const gam_handle *h, int st, int nx, int ny, int nz) {
     return dim3((unsigned)tile_blocks(h->tile[st], nx, ny, nz));
 }
 
+// FP32 copies of the bulk geometry (precision 1), refreshed after every change
+static int sync_grid32(gam_handle *h) {
+    if (!h->prec) return 0;
+    const size_t G = gsz(h), F = fsz(h);
+    const size_t n[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
+    for (int q = 0; q < 9; q++) {
+        const size_t m = n[kG32[q]];
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cvt<double, float>), dim3((unsigned)((m + 255) / 256)), 256, 0, h->stream,
+                           (const double *)h->grid[kG32[q]], h->grid32[q], m);
+    }
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
 // ----------------------------------------------------------------- lifetime
 extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handle **out) {
     if (!dims || !par || !out) return -1;
     *out = nullptr;
     if (dims->Nip < 1 || dims->Njp < 1 || dims->Nkp < 1 || dims->nG != 4)
         return -1;   // the stencils assume 4 ghost layers (init.F90:327)
-    if (par->precision != 0) return -1;
+    if (par->precision != 0 && par->precision != 1) return -1;
     if (par->doMHD != 0 && par->doMHD != 1) return -1;
     long Ni = dims->Nip + 2L * dims->nG, Nj = dims->Njp + 2L * dims->nG, Nk = dims->Nkp + 2L * dims->nG;
     long nf = (Ni + 1) * (Nj + 1) * (Nk + 1);
@@ -148,6 +183,8 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     gam_handle *h = new gam_handle();
     h->dims = *dims;
     h->par = *par;
+    h->prec = par->precision;
+    h->es = h->prec ? sizeof(float) : sizeof(double);
     GV &g = h->gv;
     g.Ni = (int)Ni; g.Nj = (int)Nj; g.Nk = (int)Nk; g.nG = dims->nG;
     g.Nip = dims->Nip; g.Njp = dims->Njp; g.Nkp = dims->Nkp;
@@ -159,36 +196,30 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     m.do25D = par->do25D;
     auto cleanup = [&](int code) { gam_destroy(h); return code; };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(-2);
-    const size_t G = gsz(h), F = fsz(h);
+    const size_t G = gsz(h), F = fsz(h), es = h->es;
     const size_t gcomp[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
     for (int q = 0; q < G_N; q++)
         if (hipMalloc(&h->grid[q], gcomp[q] * sizeof(double)) != hipSuccess) return cleanup(-3);
+    if (h->prec)
+        for (int q = 0; q < 9; q++)
+            if (hipMalloc(&h->grid32[q], gcomp[kG32[q]] * sizeof(float)) != hipSuccess) return cleanup(-3);
     for (int q = 0; q < 2; q++) {
-        if (hipMalloc(&h->Gas[q], 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
+        if (hipMalloc(&h->Gas[q], 5 * G * es) != hipSuccess) return cleanup(-3);
         if (hipMalloc(&h->mag[q], 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
-        if (hipMalloc(&h->B[q], 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-        if (hipMemset(h->Gas[q], 0, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-2);
+        if (hipMalloc(&h->B[q], 3 * G * es) != hipSuccess) return cleanup(-3);
+        if (hipMemset(h->Gas[q], 0, 5 * G * es) != hipSuccess) return cleanup(-2);
         if (hipMemset(h->mag[q], 0, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-2);
-        if (hipMemset(h->B[q], 0, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-2);
+        if (hipMemset(h->B[q], 0, 3 * G * es) != hipSuccess) return cleanup(-2);
     }
-    if (hipMalloc(&h->hGas, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->hmag, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->hB, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->Vf, 9 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->E, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->Flx, 24 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->wk.Wh, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->wk.Wn, 5 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->wk.VdV, 3 * G * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->wk.bh, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (hipMalloc(&h->wk.bn, 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
     {
-        double *z[11] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->wk.Wh, h->wk.Wn, h->wk.VdV,
-                         h->wk.bh, h->wk.bn};
+        void **z[11] = {&h->hGas, &h->hmag, &h->hB, &h->Vf, &h->E, &h->Flx, &h->wk[W_WH], &h->wk[W_WN],
+                        &h->wk[W_VDV], &h->wk[W_BH], &h->wk[W_BN]};
         // K fluxes stay 0 in 2.5D (init.F90:500-501 zero the flux work arrays)
         const size_t zn[11] = {5 * G, 3 * F, 3 * G, 9 * G, 3 * G, 24 * G, 5 * G, 5 * G, 3 * G, 3 * F, 3 * F};
-        for (int q = 0; q < 11; q++)
-            if (hipMemset(z[q], 0, zn[q] * sizeof(double)) != hipSuccess) return cleanup(-2);
+        for (int q = 0; q < 11; q++) {
+            if (hipMalloc(z[q], zn[q] * es) != hipSuccess) return cleanup(-3);
+            if (hipMemset(*z[q], 0, zn[q] * es) != hipSuccess) return cleanup(-2);
+        }
     }
     default_tiles(h);
     h->npart = (int)((G + 255) / 256);
@@ -209,11 +240,19 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
         hipMemsetAsync(h->grid[G_TF], 0, 27 * F * sizeof(double), h->stream) != hipSuccess ||
         hipMemsetAsync(h->grid[G_TE], 0, 27 * F * sizeof(double), h->stream) != hipSuccess)
         return cleanup(-2);
-    if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(-2);
     // device views
     g.xyz = h->grid[G_XYZ]; g.xyzcc = h->grid[G_XYZCC]; g.xfc = h->grid[G_XFC]; g.face = h->grid[G_FACE];
     g.Tf = h->grid[G_TF]; g.edge = h->grid[G_EDGE]; g.Te = h->grid[G_TE]; g.Teb = h->grid[G_TEB];
     g.vol = h->grid[G_VOL]; g.di = h->grid[G_DI]; g.dj = h->grid[G_DJ]; g.dk = h->grid[G_DK];
+    g.dvol = h->grid[G_VOL];
+    GVT<float> &g32 = h->gv32;
+    g32.Ni = g.Ni; g32.Nj = g.Nj; g32.Nk = g.Nk; g32.nG = g.nG; g32.Nip = g.Nip; g32.Njp = g.Njp; g32.Nkp = g.Nkp;
+    g32.nc = g.nc; g32.nf = g.nf;
+    g32.xyz = g.xyz; g32.xyzcc = g.xyzcc; g32.xfc = g.xfc; g32.dvol = g.dvol;
+    g32.face = h->grid32[0]; g32.Tf = h->grid32[1]; g32.edge = h->grid32[2]; g32.Te = h->grid32[3];
+    g32.Teb = h->grid32[4]; g32.vol = h->grid32[5]; g32.di = h->grid32[6]; g32.dj = h->grid32[7]; g32.dk = h->grid32[8];
+    if (sync_grid32(h)) return cleanup(-2);
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(-2);
     *out = h;
     return 0;
 }
@@ -225,10 +264,11 @@ extern "C" int gam_destroy(gam_handle *h) {
     collect_timers(h);
     for (auto e : h->evpool) (void)hipEventDestroy(e);
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
+    for (int q = 0; q < 9; q++) if (h->grid32[q]) (void)hipFree(h->grid32[q]);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
-    double *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, (double *)h->clk,
-                        h->wk.Wh, h->wk.Wn, h->wk.VdV, h->wk.bh, h->wk.bn};
-    for (double *b : bufs) if (b) (void)hipFree(b);
+    void *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, h->clk,
+                      h->wk[0], h->wk[1], h->wk[2], h->wk[3], h->wk[4]};
+    for (void *b : bufs) if (b) (void)hipFree(b);
     if (h->clk_host) (void)hipHostFree(h->clk_host);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -257,6 +297,25 @@ static int d2h(gam_handle *h, double *dst, const double *src, size_t n) {
     HIPCHK(h, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     return 0;
 }
+// host FP64 <-> device bulk array of n elements (converted on the host in FP32 mode)
+static int h2d_bulk(gam_handle *h, void *dst, const double *src, size_t n) {
+    if (!src) return 0;
+    if (!h->prec) return h2d(h, (double *)dst, src, n);
+    std::vector<float> tmp(n);
+    for (size_t q = 0; q < n; q++) tmp[q] = (float)src[q];
+    HIPCHK(h, hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // tmp goes out of scope
+    return 0;
+}
+static int d2h_bulk(gam_handle *h, double *dst, const void *src, size_t n, bool bulk = true) {
+    if (!dst) return 0;
+    if (!h->prec || !bulk) return d2h(h, dst, (const double *)src, n);
+    std::vector<float> tmp(n);
+    HIPCHK(h, hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (size_t q = 0; q < n; q++) dst[q] = (double)tmp[q];
+    return 0;
+}
 
 static GVw gvw(gam_handle *h) {
     const GV &g = h->gv;
@@ -278,6 +337,8 @@ extern "C" int gam_set_grid(gam_handle *h, const double *xyz, const double *xyzc
     const double *src[G_N] = {xyz, xyzcc, xfc, face, Tf, edge, Te, Teb, volume, di, dj, dk};
     const size_t n[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
     for (int q = 0; q < G_N; q++) { int rc = h2d(h, h->grid[q], src[q], n[q]); if (rc) return rc; }
+    int rc = sync_grid32(h);
+    if (rc) return rc;
     return gam_sync(h);
 }
 
@@ -317,11 +378,12 @@ extern "C" int gam_set_corners(gam_handle *h, const double *xyz) {
         hipLaunchKernelGGL(k_teb, grid1(ne), 256, 0, h->stream, w, dN, T2, T1, XNQJ);
     }
     HIPCHK(h, hipGetLastError());
+    if ((rc = sync_grid32(h))) return rc;
     return gam_sync(h);
 }
 
-// which: 0 State, 1 oState, 2 predictor state
-static void state_ptrs(gam_handle *h, int which, double **Gas, double **mag, double **B) {
+// which: 0 State, 1 oState, 2 predictor state (its magFlux is bulk-typed)
+static void state_ptrs(gam_handle *h, int which, void **Gas, void **mag, void **B) {
     if (which == 2) { *Gas = h->hGas; *mag = h->hmag; *B = h->hB; return; }
     int q = which == 0 ? h->s : 1 - h->s;
     *Gas = h->Gas[q]; *mag = h->mag[q]; *B = h->B[q];
@@ -330,12 +392,12 @@ static void state_ptrs(gam_handle *h, int which, double **Gas, double **mag, dou
 extern "C" int gam_put_state(gam_handle *h, int which, double time, const double *Gas, const double *magFlux,
                              const double *Bxyz) {
     if (!h || which < 0 || which > 1) return -1;
-    double *g, *m, *b;
+    void *g, *m, *b;
     state_ptrs(h, which, &g, &m, &b);
     int rc;
-    if ((rc = h2d(h, g, Gas, 5 * gsz(h)))) return rc;
-    if ((rc = h2d(h, m, magFlux, 3 * fsz(h)))) return rc;
-    if ((rc = h2d(h, b, Bxyz, 3 * gsz(h)))) return rc;
+    if ((rc = h2d_bulk(h, g, Gas, 5 * gsz(h)))) return rc;
+    if ((rc = h2d(h, (double *)m, magFlux, 3 * fsz(h)))) return rc;
+    if ((rc = h2d_bulk(h, b, Bxyz, 3 * gsz(h)))) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost));
     if (which == 0) h->clk_host->stime = time; else h->clk_host->otime = time;
@@ -345,12 +407,12 @@ extern "C" int gam_put_state(gam_handle *h, int which, double time, const double
 
 extern "C" int gam_get_state(gam_handle *h, int which, double *time, double *Gas, double *magFlux, double *Bxyz) {
     if (!h || which < 0 || which > 2) return -1;
-    double *g, *m, *b;
+    void *g, *m, *b;
     state_ptrs(h, which, &g, &m, &b);
     int rc;
-    if ((rc = d2h(h, Gas, g, 5 * gsz(h)))) return rc;
-    if ((rc = d2h(h, magFlux, m, 3 * fsz(h)))) return rc;
-    if ((rc = d2h(h, Bxyz, b, 3 * gsz(h)))) return rc;
+    if ((rc = d2h_bulk(h, Gas, g, 5 * gsz(h)))) return rc;
+    if ((rc = d2h_bulk(h, magFlux, m, 3 * fsz(h), which == 2))) return rc;
+    if ((rc = d2h_bulk(h, Bxyz, b, 3 * gsz(h)))) return rc;
     HIPCHK(h, hipMemcpyAsync(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost, h->stream));
     if ((rc = gam_sync(h))) return rc;
     if (time) *time = which == 0 ? h->clk_host->stime : (which == 1 ? h->clk_host->otime : h->clk_host->stime + h->clk_host->pdt);
@@ -381,12 +443,48 @@ extern "C" int gam_get_clock(gam_handle *h, gam_clock *c) {
 }
 
 // ----------------------------------------------------------------- ICs
+// The initial conditions are computed in FP64.  In FP32 mode they go to an
+// FP64 scratch first and are rounded into the FP32 State.
+struct IcScratch {
+    double *buf = nullptr;
+    ~IcScratch() { if (buf) (void)hipFree(buf); }
+};
+
+static int ic_gas_target(gam_handle *h, IcScratch &sc, double **gas, double **lA) {
+    const size_t G = gsz(h);
+    if (!h->prec) { *gas = (double *)h->Gas[h->s]; *lA = (double *)h->E; return 0; }
+    HIPCHK(h, hipMalloc(&sc.buf, 8 * G * sizeof(double)));
+    *gas = sc.buf;
+    *lA = sc.buf + 5 * G;
+    // the FP32 State keeps whatever the scratch holds outside the IC's cells
+    const size_t n = 5 * G;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cvt<float, double>), dim3((unsigned)((n + 255) / 256)), 256, 0, h->stream,
+                       (const float *)h->Gas[h->s], *gas, n);
+    HIPCHK(h, hipGetLastError());
+    return 0;
+}
+
+static int ic_gas_finish(gam_handle *h, IcScratch &sc) {
+    if (!h->prec) return 0;
+    const size_t n = 5 * gsz(h);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cvt<double, float>), dim3((unsigned)((n + 255) / 256)), 256, 0, h->stream,
+                       (const double *)sc.buf, (float *)h->Gas[h->s], n);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // before the scratch is freed
+    return 0;
+}
+
 extern "C" int gam_ic_orszag_tang(gam_handle *h, double rho0, double p0, double b0, double kscale) {
     if (!h) return -1;
     const GV &g = h->gv;
+    IcScratch sc;
+    double *gas, *lA;
+    int rc = ic_gas_target(h, sc, &gas, &lA);
+    if (rc) return rc;
     long n = (long)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
-    hipLaunchKernelGGL(k_ic_ot, grid1(n), 256, 0, h->stream, g, h->mp.gamma, rho0, p0, b0, kscale, h->Gas[h->s], h->mag[h->s]);
+    hipLaunchKernelGGL(k_ic_ot, grid1(n), 256, 0, h->stream, g, h->mp.gamma, rho0, p0, b0, kscale, gas, h->mag[h->s]);
     HIPCHK(h, hipGetLastError());
+    if ((rc = ic_gas_finish(h, sc))) return rc;
     return gam_sync(h);
 }
 
@@ -395,37 +493,43 @@ extern "C" int gam_ic_blast_wave(gam_handle *h, double x0, double y0, double z0,
     if (!h) return -1;
     const GV &g = h->gv;
     GVw w = gvw(h);
+    IcScratch sc;
+    double *gas, *lA;
+    int rc = ic_gas_target(h, sc, &gas, &lA);
+    if (rc) return rc;
     BWp bw = {x0, y0, z0, D0, P0, pRat, dRat, rC};
     long n = (long)g.Nip * g.Njp * g.Nkp;
-    hipLaunchKernelGGL(k_ic_bw_gas, grid1(n), 256, 0, h->stream, w, h->mp.gamma, bw, h->Gas[h->s]);
+    hipLaunchKernelGGL(k_ic_bw_gas, grid1(n), 256, 0, h->stream, w, h->mp.gamma, bw, gas);
     if (h->mp.doMHD) {
         // prob.F90:234 bScl = B0/sqrt(2.0) with a default-real sqrt(2.0)
         double bScl = B0 / (double)sqrtf(2.0f), bSclz = 0.0;
-        double *lA = h->E;   // scratch, zeroed like the reference's allocGridVec
+        // scratch, zeroed like the reference's allocGridVec
         HIPCHK(h, hipMemsetAsync(lA, 0, 3 * gsz(h) * sizeof(double), h->stream));
         long m = (long)(g.Ni - 1) * (g.Nj - 1) * (g.Nk - 1);
         hipLaunchKernelGGL(k_ic_bw_la, grid1(m), 256, 0, h->stream, w, bScl, bSclz, lA);
-        hipLaunchKernelGGL(k_ic_bw_flux, grid1(m), 256, 0, h->stream, w, lA, h->mag[h->s]);
+        hipLaunchKernelGGL(k_ic_bw_flux, grid1(m), 256, 0, h->stream, w, (const double *)lA, h->mag[h->s]);
     }
     HIPCHK(h, hipGetLastError());
+    if ((rc = ic_gas_finish(h, sc))) return rc;
     return gam_sync(h);
 }
 
 // ----------------------------------------------------------------- step pieces
-static const GV &G(gam_handle *h) { return h->gv; }
-
 // CellDT min over the DT box of (Gas, B) into h->partial (CalcDT's loop).
-static int launch_celldt(gam_handle *h, const double *Gas, const double *B) {
+template <typename R>
+static int launch_celldt(gam_handle *h, const void *Gas, const void *B) {
     if (!Gas || !B) return fail(h, -1, "launch_celldt: null state");
     StageTimer t(h, ST_CELLDT);
-    if (h->mp.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<1>), dim3(h->npart), 256, 0, h->stream, G(h), h->mp, h->clk, Gas, B, h->partial);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<0>), dim3(h->npart), 256, 0, h->stream, G(h), h->mp, h->clk, Gas, B, h->partial);
+    const GVT<R> &g = view<R>(h);
+    if (h->mp.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<1, R>), dim3(h->npart), 256, 0, h->stream, g, h->mp, h->clk, (const R *)Gas, (const R *)B, h->partial);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<0, R>), dim3(h->npart), 256, 0, h->stream, g, h->mp, h->clk, (const R *)Gas, (const R *)B, h->partial);
+    HIPCHK(h, hipGetLastError());
     return 0;
 }
+static int celldt(gam_handle *h, const void *Gas, const void *B) {
+    return h->prec ? launch_celldt<float>(h, Gas, B) : launch_celldt<double>(h, Gas, B);
+}
 
-// Final reduction of h->partial + CalcDT checks (calc), after the
-// UpdateStateData time update (advance).  h->partial must hold a complete set
-// of block minima when calc != 0 (from k_update or launch_celldt).
 static int update_blocks(const gam_handle *h) {
     return tile_blocks(h->tile[ST_UPDATE], h->gv.Ni, h->gv.Nj, h->gv.Nk);
 }
@@ -436,77 +540,95 @@ static void launch_dtfinal(gam_handle *h, int advance, int calc, int np) {
     hipLaunchKernelGGL(k_dtfinal, 1, 1024, 0, h->stream, h->clk, h->partial, np, advance, calc);
 }
 
-template <int RM, int BO, int MH>
+template <int RM, int BO, int MH, typename R>
 static void launch_flux(gam_handle *h, int d) {
-    const GV &g = h->gv;
-    int S = h->s;
+    const GVT<R> &g = view<R>(h);
+    const int S = h->s;
     long nx = g.Nip + (d == 0), ny = g.Njp + (d == 1), nz = (h->mp.do25D ? 1 : g.Nkp) + (d == 2);
     const Tile tl = h->tile[ST_FLUX_I + d];
     dim3 gr = tgrid(h, ST_FLUX_I + d, (int)nx, (int)ny, (int)nz);
+    const WKT<R> w = wkt<R>(h);
+    const R *nB = P<R>(h->B[S]), *hB = P<R>(h->hB), *hmag = P<R>(h->hmag), *hGas = P<R>(h->hGas);
+    R *Flx = P<R>(h->Flx);
     StageTimer t(h, ST_FLUX_I + d);
-    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
-    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, h->wk.Wn, h->B[S], h->wk.Wh, h->hB, h->hmag, h->hGas, h->Flx);
+    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
 }
 
-template <int ED, int RM, int BO>
+template <int ED, int RM, int BO, typename R>
 static void launch_emf(gam_handle *h) {
-    const GV &g = h->gv;
+    const GVT<R> &g = view<R>(h);
     const int st = ST_EMF_I + ED;
+    const WKT<R> w = wkt<R>(h);
     StageTimer t(h, st);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<ED, RM, BO>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
-                       256, 0, h->stream, g, h->tile[st], h->mp, h->clk, h->Vf, h->hGas, h->wk.bn, h->wk.bh, h->E);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<ED, RM, BO, R>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
+                       256, 0, h->stream, g, h->tile[st], h->mp, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
 }
 
-template <int RM, int BO>
+template <int RM, int BO, typename R>
 static void launch_emfs(gam_handle *h) {
-    launch_emf<0, RM, BO>(h);
-    launch_emf<1, RM, BO>(h);
-    launch_emf<2, RM, BO>(h);
+    launch_emf<0, RM, BO, R>(h);
+    launch_emf<1, RM, BO, R>(h);
+    launch_emf<2, RM, BO, R>(h);
 }
 
-template <int RM, int BO, int MH>
+template <int RM, int BO, int MH, typename R>
 static void launch_fluxes(gam_handle *h) {
-    launch_flux<RM, BO, MH>(h, 0);
-    launch_flux<RM, BO, MH>(h, 1);
-    if (!h->mp.do25D) launch_flux<RM, BO, MH>(h, 2);
+    launch_flux<RM, BO, MH, R>(h, 0);
+    launch_flux<RM, BO, MH, R>(h, 1);
+    if (!h->mp.do25D) launch_flux<RM, BO, MH, R>(h, 2);
+}
+
+// bFlux2Fld of magFlux (type TM) into the bulk array B
+template <typename R, typename TM>
+static void launch_b2f(gam_handle *h, int st, const TM *mag, void *B) {
+    const GVT<R> &g = view<R>(h);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[st],
+                       h->clk, mag, P<R>(B));
 }
 
 // AdvanceMHD + UpdateStateData time update; the dt min of the fused path is
 // left in h->partial when fuse_dt != 0.
-static int advance(gam_handle *h, int fuse_dt) {
-    const GV &g = h->gv;
+template <typename R>
+static int advance_t(gam_handle *h, int fuse_dt) {
+    const GVT<R> &g = view<R>(h);
     const MP &m = h->mp;
     const int S = h->s, O = 1 - h->s;
     { StageTimer t(h, ST_CLOCK); hipLaunchKernelGGL(k_clock_begin, 1, 64, 0, h->stream, h->clk); }
     { StageTimer t(h, ST_PREDICT);
-      hipLaunchKernelGGL(k_predict, grid1(g.nf), 256, 0, h->stream, g, m, h->clk, h->Gas[S], h->Gas[O], h->mag[S], h->mag[O], h->hGas, h->hmag, h->wk); }
-    { StageTimer t(h, ST_BHALF); hipLaunchKernelGGL(k_b2f, tgrid(h, ST_BHALF, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_BHALF], h->clk, h->hmag, h->hB); }
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_predict<R>), grid1(g.nf), 256, 0, h->stream, g, m, h->clk, P<R>(h->Gas[S]), P<R>(h->Gas[O]),
+                         h->mag[S], h->mag[O], P<R>(h->hGas), P<R>(h->hmag), wkt<R>(h)); }
+    { StageTimer t(h, ST_BHALF); launch_b2f<R, R>(h, ST_BHALF, P<R>(h->hmag), h->hB); }
     if (m.doMHD) {
-        { StageTimer t(h, ST_MOM2FACE); hipLaunchKernelGGL(k_vf, tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_MOM2FACE], h->clk, h->wk.VdV, h->Vf); }
-        if (m.rmeth == 1) { if (m.doBoris) launch_emfs<1, 1>(h); else launch_emfs<1, 0>(h); }
-        else { if (m.doBoris) launch_emfs<0, 1>(h); else launch_emfs<0, 0>(h); }
+        { StageTimer t(h, ST_MOM2FACE);
+          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_MOM2FACE],
+                             h->clk, P<R>(h->wk[W_VDV]), P<R>(h->Vf)); }
+        if (m.rmeth == 1) { if (m.doBoris) launch_emfs<1, 1, R>(h); else launch_emfs<1, 0, R>(h); }
+        else { if (m.doBoris) launch_emfs<0, 1, R>(h); else launch_emfs<0, 0, R>(h); }
     }
     if (m.doMHD) {
-        if (m.rmeth == 1) { if (m.doBoris) launch_fluxes<1, 1, 1>(h); else launch_fluxes<1, 0, 1>(h); }
-        else { if (m.doBoris) launch_fluxes<0, 1, 1>(h); else launch_fluxes<0, 0, 1>(h); }
+        if (m.rmeth == 1) { if (m.doBoris) launch_fluxes<1, 1, 1, R>(h); else launch_fluxes<1, 0, 1, R>(h); }
+        else { if (m.doBoris) launch_fluxes<0, 1, 1, R>(h); else launch_fluxes<0, 0, 1, R>(h); }
     } else {
-        if (m.rmeth == 1) launch_fluxes<1, 0, 0>(h); else launch_fluxes<0, 0, 0>(h);
+        if (m.rmeth == 1) launch_fluxes<1, 0, 0, R>(h); else launch_fluxes<0, 0, 0, R>(h);
     }
     {
         StageTimer t(h, ST_CT);
-        if (m.doMHD) hipLaunchKernelGGL(k_ct, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mag[S], h->E, h->mag[O]);
+        if (m.doMHD) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ct<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mag[S], P<R>(h->E), h->mag[O]);
         else HIPCHK(h, hipMemcpyAsync(h->mag[O], h->mag[S], 3 * fsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     }
     {
         StageTimer t(h, ST_UPDATE);
         dim3 gr(update_blocks(h));
         const Tile tu = h->tile[ST_UPDATE];
+        const R *nGas = P<R>(h->Gas[S]), *nB = P<R>(h->B[S]), *Flx = P<R>(h->Flx);
+        R *oGas = P<R>(h->Gas[O]), *oB = P<R>(h->B[O]);
         if (m.doMHD) {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1>), gr, 256, 0, h->stream, g, tu, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1>), gr, 256, 0, h->stream, g, tu, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
         } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0>), gr, 256, 0, h->stream, g, tu, m, h->clk, h->Gas[S], h->B[S], h->Flx, h->mag[O], h->Gas[O], h->B[O], h->partial);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
         }
     }
     // CopyState State->oState is the buffer swap
@@ -515,15 +637,28 @@ static int advance(gam_handle *h, int fuse_dt) {
     HIPCHK(h, hipGetLastError());
     return 0;
 }
+static int advance(gam_handle *h, int fuse_dt) {
+    return h->prec ? advance_t<float>(h, fuse_dt) : advance_t<double>(h, fuse_dt);
+}
 
-static int bcs(gam_handle *h) {
-    const GV &g = h->gv;
+// periodic BCs on State; when withB, Bxyz ghosts are copied too (hatch)
+template <typename R>
+static void launch_bc(gam_handle *h, bool withB) {
+    const GVT<R> &g = view<R>(h);
     const int S = h->s;
-    { StageTimer t(h, ST_BC); hipLaunchKernelGGL(k_bc, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, h->Gas[S], h->mag[S], nullptr); }
-    { StageTimer t(h, ST_BPOST); hipLaunchKernelGGL(k_b2f, tgrid(h, ST_BPOST, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_BPOST], h->clk, h->mag[S], h->B[S]); }
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, P<R>(h->Gas[S]),
+                       h->mag[S], withB ? P<R>(h->B[S]) : (R *)nullptr);
+}
+
+template <typename R>
+static int bcs_t(gam_handle *h) {
+    const int S = h->s;
+    { StageTimer t(h, ST_BC); launch_bc<R>(h, false); }
+    { StageTimer t(h, ST_BPOST); launch_b2f<R, double>(h, ST_BPOST, h->mag[S], h->B[S]); }
     HIPCHK(h, hipGetLastError());
     return 0;
 }
+static int bcs(gam_handle *h) { return h->prec ? bcs_t<float>(h) : bcs_t<double>(h); }
 
 extern "C" int gam_advance_mhd(gam_handle *h) {
     if (!h) return -1;
@@ -534,7 +669,7 @@ extern "C" int gam_advance_mhd(gam_handle *h) {
 
 extern "C" int gam_calc_dt(gam_handle *h, double *dt, int *status) {
     if (!h) return -1;
-    int rc = launch_celldt(h, h->Gas[h->s], h->B[h->s]);
+    int rc = celldt(h, h->Gas[h->s], h->B[h->s]);
     if (rc) return rc;
     launch_dtfinal(h, 0, 1, h->npart);
     gam_clock c;
@@ -578,20 +713,19 @@ extern "C" int gam_hatch(gam_handle *h) {
     // ghosts copied; no Bxyz recompute after them), oState = State, first dt
     // (ts = 0), oState%time = State%time - dt.
     if (!h) return -1;
-    const GV &g = h->gv;
     const int S = h->s, O = 1 - h->s;
     gam_clock c0;
     int rc = gam_get_clock(h, &c0);
     if (rc) return rc;
     c0.dt0 = 0.0; c0.ts = 0; c0.status = 0;
     if ((rc = gam_set_clock(h, &c0))) return rc;
-    hipLaunchKernelGGL(k_b2f, tgrid(h, ST_BPOST, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_BPOST], h->clk, h->mag[S], h->B[S]);
-    hipLaunchKernelGGL(k_bc, grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, h->Gas[S], h->mag[S], h->B[S]);
-    HIPCHK(h, hipMemcpyAsync(h->Gas[O], h->Gas[S], 5 * gsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (h->prec) { launch_b2f<float, double>(h, ST_BPOST, h->mag[S], h->B[S]); launch_bc<float>(h, true); }
+    else { launch_b2f<double, double>(h, ST_BPOST, h->mag[S], h->B[S]); launch_bc<double>(h, true); }
+    HIPCHK(h, hipMemcpyAsync(h->Gas[O], h->Gas[S], 5 * gsz(h) * h->es, hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->mag[O], h->mag[S], 3 * fsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->B[O], h->B[S], 3 * gsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->B[O], h->B[S], 3 * gsz(h) * h->es, hipMemcpyDeviceToDevice, h->stream));
     HIPCHK(h, hipGetLastError());
-    if ((rc = launch_celldt(h, h->Gas[S], h->B[S]))) return rc;
+    if ((rc = celldt(h, h->Gas[S], h->B[S]))) return rc;
     launch_dtfinal(h, 0, 1, h->npart);
     gam_clock c;
     if ((rc = gam_get_clock(h, &c))) return rc;
@@ -623,9 +757,9 @@ extern "C" int gam_get_work(gam_handle *h, int which, double *out) {
     if (!h || !out) return -1;
     const size_t G = gsz(h);
     int rc;
-    if (which == 0) rc = d2h(h, out, h->Flx, 24 * G);
-    else if (which == 1) rc = d2h(h, out, h->E, 3 * G);
-    else if (which == 2) rc = d2h(h, out, h->Vf, 9 * G);
+    if (which == 0) rc = d2h_bulk(h, out, h->Flx, 24 * G);
+    else if (which == 1) rc = d2h_bulk(h, out, h->E, 3 * G);
+    else if (which == 2) rc = d2h_bulk(h, out, h->Vf, 9 * G);
     else return fail(h, -1, "gam_get_work: which must be 0, 1 or 2");
     if (rc) return rc;
     return gam_sync(h);
@@ -685,34 +819,38 @@ extern "C" int gam_num_stages(void) { return ST_N; }
 
 extern "C" int gam_stage_bytes(gam_handle *h, int stage, double *bytes) {
     // Unique bytes each stage must move: every input array read once over the
-    // stage's domain, every output written once (FP64).
+    // stage's domain, every output written once, each at its stored precision
+    // (b = bulk elements, 8 or 4 bytes; d = FP64 elements).
     if (!h || !bytes || stage < 0 || stage >= ST_N) return -1;
     const GV &g = h->gv;
     const double G = g.nc, F = g.nf;
     const double nI = (double)(g.Nip + 1) * g.Njp * g.Nkp, nJ = (double)g.Nip * (g.Njp + 1) * g.Nkp,
                  nK = (double)g.Nip * g.Njp * (g.Nkp + 1), nA = (double)g.Nip * g.Njp * g.Nkp;
     const double nE = (double)(g.Nip + 1) * (g.Njp + 1) * (g.Nkp + 1);
-    double b = 0;
+    double b = 0, d = 0;
     switch (stage) {
-    case ST_PREDICT:   // mag n,o, face -> hmag, bh, bn ; Gas n,o, vol -> hGas, Wh, Wn, VdV
-        b = F * (3 + 3 + 3 + 3 + 3 + 3) + G * (5 + 5 + 1 + 5 + 5 + 5 + 3); break;
-    case ST_BHALF: case ST_BPOST: b = F * (3 + 9) + G * (3 + 1 + 3); break;      // mag, xfc ; xyzcc, vol -> B
+    case ST_PREDICT:   // mag n,o (d) ; face -> hmag, bh, bn ; Gas n,o, vol -> hGas, Wh, Wn, VdV
+        d = F * 6; b = F * (3 + 3 + 3 + 3) + G * (5 + 5 + 1 + 5 + 5 + 5 + 3); break;
+    case ST_BHALF:   // hmag (b), xfc, xyzcc, vol (d) -> B (b)
+        b = F * 3 + G * 3; d = F * 9 + G * (3 + 1); break;
+    case ST_BPOST:   // mag, xfc, xyzcc, vol (d) -> B (b)
+        b = G * 3; d = F * (3 + 9) + G * (3 + 1); break;
     case ST_MOM2FACE: b = G * (3 + 1 + 9); break;                                 // VdV, vol -> Vf (9)
     case ST_EMF_I: case ST_EMF_J: case ST_EMF_K: {   // Vf (dT1), rho ; face, bn, bh (dT1, dT2), Te, Teb, edge -> E
-        const int d = stage - ST_EMF_I;
-        const double nEd = (double)(g.Nip + (d != 0)) * (g.Njp + (d != 1)) * (g.Nkp + (d != 2));
+        const int e = stage - ST_EMF_I;
+        const double nEd = (double)(g.Nip + (e != 0)) * (g.Njp + (e != 1)) * (g.Nkp + (e != 2));
         b = G * (3 + 1) + nEd * (2 + 2 + 2 + 6 + 4 + 1 + 1); break;
     }
     case ST_FLUX_I: b = (G * (5 + 5 + 3 + 3 + 1)) + nI * (9 + 1 + 1 + 8); break;
     case ST_FLUX_J: b = (G * (5 + 5 + 3 + 3 + 1)) + nJ * (9 + 1 + 1 + 8); break;
     case ST_FLUX_K: b = (G * (5 + 5 + 3 + 3 + 1)) + nK * (9 + 1 + 1 + 8); break;
-    case ST_CT: b = F * (3 + 3) + G * 3; break;
-    case ST_UPDATE:   // Gas n, fluxes (8 per direction), xyzcc, vol, di/dj/dk ; magFlux new, xfc -> Gas, Bxyz
-        b = G * (5 + 24 + 3 + 1 + 3 + 5 + 3 + (h->mp.doBoris ? 3 : 0)) + F * (3 + 9); break;
-    case ST_BC: b = 2.0 * (G - nA) * 5 + 2.0 * (F - nE) * 3; break;
+    case ST_CT: d = F * (3 + 3); b = G * 3; break;
+    case ST_UPDATE:   // Gas n, fluxes (8 per direction), di/dj/dk -> Gas, Bxyz (b) ; magFlux new, xfc, xyzcc, vol (d)
+        b = G * (5 + 24 + 3 + 5 + 3 + (h->mp.doBoris ? 3 : 0)); d = F * (3 + 9) + G * (3 + 1); break;
+    case ST_BC: b = 2.0 * (G - nA) * 5; d = 2.0 * (F - nE) * 3; break;
     case ST_CELLDT: b = G * (5 + 3 + 3); break;
     default: b = 0;
     }
-    *bytes = 8.0 * b;
+    *bytes = (double)h->es * b + 8.0 * d;
     return 0;
 }

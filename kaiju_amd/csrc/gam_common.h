@@ -41,115 +41,149 @@ struct DClock {
 //   Wh, Wn  BlockCon2Prim of the predictor / limiter state  (recon.F90:66-92)
 //   VdV     vol*M/max(D,dFloor) of the predictor state      (fields.F90:265-271)
 //   bh, bn  magFlux/max(face,TINY) of predictor / limiter   (fields.F90:331-337)
-struct WK {
-    double *Wh, *Wn, *VdV;   // 5G, 5G, 3G
-    double *bh, *bn;         // 3F, 3F
+// R is the bulk real type: double for the FP64 step, float for the FP32-bulk
+// step (same kernels; every literal goes through R(), so the double
+// instantiation is the FP64 arithmetic unchanged).
+template <typename R> struct WKT {
+    R *Wh, *Wn, *VdV;   // 5G, 5G, 3G
+    R *bh, *bn;         // 3F, 3F
 };
+typedef WKT<double> WK;
 
-// Grid view: sizes + geometry pointers (device).
-struct GV {
+// Grid view: sizes + geometry pointers (device).  Positions (xyz, xyzcc, xfc)
+// and the FP64 volume dvol stay double in both precisions: bFlux2Fld's lever
+// arms are differences of nearby positions and Flux2Deltas divides by dvol.
+template <typename R> struct GVT {
     int Ni, Nj, Nk, nG, Nip, Njp, Nkp;
     int nc, nf;           // number of cells (G) and face points (F)
-    const double *xyz, *xyzcc, *xfc, *face, *Tf, *edge, *Te, *Teb, *vol, *di, *dj, *dk;
+    const double *xyz, *xyzcc, *xfc, *dvol;
+    const R *face, *Tf, *edge, *Te, *Teb, *vol, *di, *dj, *dk;
 };
+typedef GVT<double> GV;
 
-__host__ __device__ inline int ci(const GV &g, int i, int j, int k) { return i + g.Ni * (j + g.Nj * k); }
-__host__ __device__ inline int fi(const GV &g, int i, int j, int k) { return i + (g.Ni + 1) * (j + (g.Nj + 1) * k); }
-__host__ __device__ inline int cstride(const GV &g, int d) { return d == 0 ? 1 : (d == 1 ? g.Ni : g.Ni * g.Nj); }
-__host__ __device__ inline int fstride(const GV &g, int d) { return d == 0 ? 1 : (d == 1 ? g.Ni + 1 : (g.Ni + 1) * (g.Nj + 1)); }
+template <typename R> __host__ __device__ inline int ci(const GVT<R> &g, int i, int j, int k) { return i + g.Ni * (j + g.Nj * k); }
+template <typename R> __host__ __device__ inline int fi(const GVT<R> &g, int i, int j, int k) { return i + (g.Ni + 1) * (j + (g.Nj + 1) * k); }
+template <typename R> __host__ __device__ inline int cstride(const GVT<R> &g, int d) { return d == 0 ? 1 : (d == 1 ? g.Ni : g.Ni * g.Nj); }
+template <typename R> __host__ __device__ inline int fstride(const GVT<R> &g, int d) { return d == 0 ? 1 : (d == 1 ? g.Ni + 1 : (g.Ni + 1) * (g.Nj + 1)); }
 
-// component access: slab c of an array with n points per slab
-template <typename T> __device__ __forceinline__ T ld(const T *p, int n, int c, int idx) { return p[(size_t)n * c + idx]; }
+template <typename T> __device__ __forceinline__ T sq(T x) { return x * x; }   // Fortran x**2.0
 
-__device__ __forceinline__ double sq(double x) { return x * x; }   // Fortran x**2.0
+// math in the bulk precision
+__device__ __forceinline__ double rmax(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float rmax(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double rmin(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float rmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double rabs(double a) { return fabs(a); }
+__device__ __forceinline__ float rabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double rsgn(double a) { return copysign(1.0, a); }     // SIGN(1.0, a)
+__device__ __forceinline__ float rsgn(float a) { return copysignf(1.0f, a); }
+__device__ __forceinline__ double rroot(double a) { return sqrt(a); }
+__device__ __forceinline__ float rroot(float a) { return sqrtf(a); }
+__device__ __forceinline__ double rerfc(double a) { return erfc(a); }
+__device__ __forceinline__ float rerfc(float a) { return erfcf(a); }
+__device__ __forceinline__ double rexp(double a) { return exp(a); }
+__device__ __forceinline__ float rexp(float a) { return expf(a); }
 
-// recon.F90:22-30 weights (integer numerators / denominator in double)
+// recon.F90:22-30 weights (integer numerators / denominator in double; the
+// FP32 tables are those doubles rounded to float)
 __device__ __constant__ static const double cW8[8] = {
     -3.0 / 840.0, 29.0 / 840.0, -139.0 / 840.0, 533.0 / 840.0, 533.0 / 840.0, -139.0 / 840.0, 29.0 / 840.0, -3.0 / 840.0};
 __device__ __constant__ static const double cW6[8] = {0.0 / 60.0, 1.0 / 60.0, -8.0 / 60.0, 37.0 / 60.0, 37.0 / 60.0, -8.0 / 60.0, 1.0 / 60.0, 0.0 / 60.0};
 __device__ __constant__ static const double cU7[7] = {
     -3.0 / 420.0, 25.0 / 420.0, -101.0 / 420.0, 319.0 / 420.0, 214.0 / 420.0, -38.0 / 420.0, 4.0 / 420.0};
+__device__ __constant__ static const float fW8[8] = {
+    (float)(-3.0 / 840.0), (float)(29.0 / 840.0), (float)(-139.0 / 840.0), (float)(533.0 / 840.0),
+    (float)(533.0 / 840.0), (float)(-139.0 / 840.0), (float)(29.0 / 840.0), (float)(-3.0 / 840.0)};
+__device__ __constant__ static const float fU7[7] = {
+    (float)(-3.0 / 420.0), (float)(25.0 / 420.0), (float)(-101.0 / 420.0), (float)(319.0 / 420.0),
+    (float)(214.0 / 420.0), (float)(-38.0 / 420.0), (float)(4.0 / 420.0)};
+template <typename R> __device__ __forceinline__ R w8(int n);
+template <> __device__ __forceinline__ double w8<double>(int n) { return cW8[n]; }
+template <> __device__ __forceinline__ float w8<float>(int n) { return fW8[n]; }
+template <typename R> __device__ __forceinline__ R u7(int n);
+template <> __device__ __forceinline__ double u7<double>(int n) { return cU7[n]; }
+template <> __device__ __forceinline__ float u7<float>(int n) { return fU7[n]; }
 
 // recon.F90:202-214 Central8 (sequential dot product)
-__device__ __forceinline__ double c8(const double *q) {
-    double s = 0.0;
+template <typename R> __device__ __forceinline__ R c8(const R *q) {
+    R s = R(0.0);
 #pragma unroll
-    for (int n = 0; n < 8; n++) s += cW8[n] * q[n];
+    for (int n = 0; n < 8; n++) s += w8<R>(n) * q[n];
     return s;
 }
 
 // recon.F90:303-310 Up7
-__device__ __forceinline__ double up7(double q0, double q1, double q2, double q3, double q4, double q5, double q6) {
-    return q0 * cU7[0] + q1 * cU7[1] + q2 * cU7[2] + q3 * cU7[3] + q4 * cU7[4] + q5 * cU7[5] + q6 * cU7[6];
+template <typename R>
+__device__ __forceinline__ R up7(R q0, R q1, R q2, R q3, R q4, R q5, R q6) {
+    return q0 * u7<R>(0) + q1 * u7<R>(1) + q2 * u7<R>(2) + q3 * u7<R>(3) + q4 * u7<R>(4) + q5 * u7<R>(5) + q6 * u7<R>(6);
 }
 
 // recon.F90:275-301 PDM (one-sided)
-__device__ __forceinline__ double pdm1(double pdmb, double q0, double q1, double q2, double qI) {
-    double maxQ = fmax(q1, q2), minQ = fmin(q1, q2);
-    double qN = fmax(minQ, fmin(qI, maxQ));
-    double dq0 = pdmb * (q1 - q0);
-    double dq1 = pdmb * (q2 - q1);
-    double s0 = copysign(1.0, dq0), s1 = copysign(1.0, dq1);
-    double s01 = fabs(s0 + s1);
-    double dqL = qN - q1;
-    return qN - s1 * fmax(0.0, fabs(dqL) - s01 * fabs(dq0));
+template <typename R> __device__ __forceinline__ R pdm1(R pdmb, R q0, R q1, R q2, R qI) {
+    R maxQ = rmax(q1, q2), minQ = rmin(q1, q2);
+    R qN = rmax(minQ, rmin(qI, maxQ));
+    R dq0 = pdmb * (q1 - q0);
+    R dq1 = pdmb * (q2 - q1);
+    R s0 = rsgn(dq0), s1 = rsgn(dq1);
+    R s01 = rabs(s0 + s1);
+    R dqL = qN - q1;
+    return qN - s1 * rmax(R(0.0), rabs(dqL) - s01 * rabs(dq0));
 }
 
 // recon.F90:231-272 pdmLR
-__device__ __forceinline__ void pdmlr(double pdmb, double v0, double v1, double v2, double v3, double Qi,
-                                      double &Ql, double &Qr) {
-    double maxV = fmax(v1, v2), minV = fmin(v1, v2);
-    double vN = fmax(minV, fmin(Qi, maxV));
-    double dv0 = pdmb * (v1 - v0), dv1 = pdmb * (v2 - v1), dv2 = pdmb * (v3 - v2);
-    double s0 = copysign(1.0, dv0), s1 = copysign(1.0, dv1), s2 = copysign(1.0, dv2);
-    double q0 = fabs(s0 + s1), q1 = fabs(s1 + s2);
-    double dvL = vN - v1, dvR = v2 - vN;
-    Ql = vN - s1 * fmax(0.0, fabs(dvL) - q0 * fabs(dv0));
-    Qr = vN + s1 * fmax(0.0, fabs(dvR) - q1 * fabs(dv2));
+template <typename R>
+__device__ __forceinline__ void pdmlr(R pdmb, R v0, R v1, R v2, R v3, R Qi, R &Ql, R &Qr) {
+    R maxV = rmax(v1, v2), minV = rmin(v1, v2);
+    R vN = rmax(minV, rmin(Qi, maxV));
+    R dv0 = pdmb * (v1 - v0), dv1 = pdmb * (v2 - v1), dv2 = pdmb * (v3 - v2);
+    R s0 = rsgn(dv0), s1 = rsgn(dv1), s2 = rsgn(dv2);
+    R q0 = rabs(s0 + s1), q1 = rabs(s1 + s2);
+    R dvL = vN - v1, dvR = v2 - vN;
+    Ql = vN - s1 * rmax(R(0.0), rabs(dvL) - q0 * rabs(dv0));
+    Qr = vN + s1 * rmax(R(0.0), rabs(dvR) - q1 * rabs(dv2));
 }
 
 // GetLRs for one variable: recon.F90:125-158 (Cen8LRs) / 162-198 (Up7LRs)
-template <int RM>
-__device__ __forceinline__ void get_lrs(double pdmb, const double *dV, const double *nQ, const double *Q, double Vi,
-                                        double &Ql, double &Qr) {
-    double QdV[8];
+template <int RM, typename R>
+__device__ __forceinline__ void get_lrs(R pdmb, const R *dV, const R *nQ, const R *Q, R Vi, R &Ql, R &Qr) {
+    R QdV[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) QdV[n] = dV[n] * Q[n];
     if (RM == 1) {
-        double l = up7(QdV[0], QdV[1], QdV[2], QdV[3], QdV[4], QdV[5], QdV[6]) / Vi;
-        double r = up7(QdV[7], QdV[6], QdV[5], QdV[4], QdV[3], QdV[2], QdV[1]) / Vi;
+        R l = up7(QdV[0], QdV[1], QdV[2], QdV[3], QdV[4], QdV[5], QdV[6]) / Vi;
+        R r = up7(QdV[7], QdV[6], QdV[5], QdV[4], QdV[3], QdV[2], QdV[1]) / Vi;
         Ql = pdm1(pdmb, nQ[0], nQ[1], nQ[2], l);
         Qr = pdm1(pdmb, nQ[3], nQ[2], nQ[1], r);
     } else {
-        double Qi = c8(QdV) / Vi;
+        R Qi = c8(QdV) / Vi;
         pdmlr(pdmb, nQ[0], nQ[1], nQ[2], nQ[3], Qi, Ql, Qr);
     }
 }
 
 // gamutils.F90:301-321 CellC2P
-__device__ __forceinline__ void cell_c2p(const MP &m, const double *c, double *w) {
-    double rho = c[DEN];
-    double Vx = c[MOMX] / rho, Vy = c[MOMY] / rho, Vz = c[MOMZ] / rho;
-    double KinE = 0.5 * rho * (sq(Vx) + sq(Vy) + sq(Vz));
-    w[0] = fmax(rho, m.dFloor);
+template <typename R> __device__ __forceinline__ void cell_c2p(const MP &m, const R *c, R *w) {
+    R rho = c[DEN];
+    R Vx = c[MOMX] / rho, Vy = c[MOMY] / rho, Vz = c[MOMZ] / rho;
+    R KinE = R(0.5) * rho * (sq(Vx) + sq(Vy) + sq(Vz));
+    w[0] = rmax(rho, R(m.dFloor));
     w[1] = Vx; w[2] = Vy; w[3] = Vz;
-    w[4] = fmax((m.gamma - 1) * (c[ENERGY] - KinE), m.pFloor);
+    w[4] = rmax(R(m.gamma - 1) * (c[ENERGY] - KinE), R(m.pFloor));
 }
 
 // gamutils.F90:362-383 CellP2C
-__device__ __forceinline__ void cell_p2c(const MP &m, const double *w, double *c) {
-    double rho = fmax(w[0], m.dFloor);
-    double Vx = w[1], Vy = w[2], Vz = w[3];
-    double IntE = fmax(w[4], m.pFloor) / (m.gamma - 1);
-    double KinE = 0.5 * rho * (sq(Vx) + sq(Vy) + sq(Vz));
+template <typename R> __device__ __forceinline__ void cell_p2c(const MP &m, const R *w, R *c) {
+    R rho = rmax(w[0], R(m.dFloor));
+    R Vx = w[1], Vy = w[2], Vz = w[3];
+    R IntE = rmax(w[4], R(m.pFloor)) / R(m.gamma - 1);
+    R KinE = R(0.5) * rho * (sq(Vx) + sq(Vy) + sq(Vz));
     c[0] = rho; c[1] = rho * Vx; c[2] = rho * Vy; c[3] = rho * Vz; c[4] = IntE + KinE;
 }
 
 // recon.F90:66-92 BlockCon2Prim
-__device__ __forceinline__ void block_c2p(const MP &m, const double *c, double *w) {
-    double D = fmax(c[DEN], m.dFloor);
-    double KinE = 0.5 * (sq(c[MOMX]) + sq(c[MOMY]) + sq(c[MOMZ])) / D;
-    double P = fmax((m.gamma - 1) * (c[ENERGY] - KinE), m.pFloor);
+template <typename R> __device__ __forceinline__ void block_c2p(const MP &m, const R *c, R *w) {
+    R D = rmax(c[DEN], R(m.dFloor));
+    R KinE = R(0.5) * (sq(c[MOMX]) + sq(c[MOMY]) + sq(c[MOMZ])) / D;
+    R P = rmax(R(m.gamma - 1) * (c[ENERGY] - KinE), R(m.pFloor));
     w[0] = D; w[1] = c[MOMX] / D; w[2] = c[MOMY] / D; w[3] = c[MOMZ] / D; w[4] = P;
 }
 

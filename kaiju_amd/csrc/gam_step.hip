@@ -1,4 +1,4 @@
-// gam_step.hip -- FP64 GAMERA MHD step kernels for gfx950.
+// gam_step.hip -- GAMERA MHD step kernels for gfx950, FP64 and FP32-bulk.
 //
 // One step (reference gamapp.F90:73-116 stepGamera) is, with S = State,
 // O = oState, H = predictor (half-step) state:
@@ -6,7 +6,7 @@
 //   k_predict   (F)   AB2 magFlux, PredictCell       mhdgroup.F90:445-616
 //   k_b2f       (G)   H.Bxyz = bFlux2Fld(H.magFlux)  gridutils.F90:281-301,648-680
 //   k_vf        (G)   Mom2Face for the 3 sweeps      fields.F90:110-179,235-285
-//   k_emf       (E)   edge EMFs, all 3 directions    fields.F90:186-226,288-517
+//   k_emf<d>    (E)   edge EMFs, one direction each  fields.F90:186-226,288-517
 //   k_flux<d>   (I)   interface fluxes, 3 launches   stress.F90:84-128,204-599
 //   k_ct        (F)   O.magFlux = E2Flux(S.magFlux)  fields.F90:18-51
 //   k_update    (G)   Flux2Deltas + Reynolds + Bxyz + Maxwell/Boris + CellDT min
@@ -17,29 +17,37 @@
 //   k_b2f       (G)   O.Bxyz = bFlux2Fld(O.magFlux)  gamapp.F90:89
 // after which the host swaps the S and O buffer sets (CopyState becomes a
 // pointer swap, mhdgroup.F90:202-257).
+//
+// Precision: the kernels are templates on the bulk real type R.  R = double is
+// the FP64 step.  R = float is the FP32-bulk step: Gas, Bxyz, the predictor
+// state, the stencil inputs and the fluxes/EMFs are stored and computed in
+// FP32, while magFlux, the CT face update (E2Flux), Flux2Deltas with the
+// Reynolds/Maxwell/Boris update and its pressure recovery, bFlux2Fld and the
+// CFL bound stay FP64 (values are promoted on load).
 #include "gam_common.h"
 
 // ----------------------------------------------------------------- helpers
-__device__ __forceinline__ void decode_f(const GV &g, int idx, int &i, int &j, int &k) {
+template <typename R> __device__ __forceinline__ void decode_f(const GVT<R> &g, int idx, int &i, int &j, int &k) {
     int nx = g.Ni + 1, ny = g.Nj + 1;
     i = idx % nx; int r = idx / nx; j = r % ny; k = r / ny;
 }
-__device__ __forceinline__ void decode_c(const GV &g, int idx, int &i, int &j, int &k) {
+template <typename R> __device__ __forceinline__ void decode_c(const GVT<R> &g, int idx, int &i, int &j, int &k) {
     i = idx % g.Ni; int r = idx / g.Ni; j = r % g.Nj; k = r / g.Nj;
 }
 
 // mhdgroup.F90:710-718 ExtrapAB2
-__device__ __forceinline__ double ab2(double dt, double Q, double oQ, double odt) {
-    double dQ = (Q - oQ) / odt;
+template <typename T> __device__ __forceinline__ T ab2(T dt, T Q, T oQ, T odt) {
+    T dQ = (Q - oQ) / odt;
     return Q + dt * dQ;
 }
 
-// gridutils.F90:648-680 gamCellBxyz
-__device__ __forceinline__ void cell_bxyz(const GV &g, const double *mag, int i, int j, int k, double *B) {
+// gridutils.F90:648-680 gamCellBxyz, in FP64 from a magFlux of type TM
+template <typename R, typename TM>
+__device__ __forceinline__ void cell_bxyz(const GVT<R> &g, const TM *mag, int i, int j, int k, double *B) {
     const int nf = g.nf;
     int c = ci(g, i, j, k), f0 = fi(g, i, j, k);
     int fi1 = f0 + 1, fj1 = f0 + (g.Ni + 1), fk1 = f0 + (g.Ni + 1) * (g.Nj + 1);
-    double dV = g.vol[c];
+    double dV = g.dvol[c];
     double fI = mag[f0], fJ = mag[nf + f0], fK = mag[2 * nf + f0];
     double fIp = mag[fi1], fJp = mag[nf + fj1], fKp = mag[2 * (size_t)nf + fk1];
     double Div = fIp - fI + fJp - fJ + fKp - fK;
@@ -65,10 +73,13 @@ __global__ void k_clock_begin(DClock *clk) {
 }
 
 // ----------------------------------------------------------------- predictor
-__global__ void __launch_bounds__(256) k_predict(GV g, MP m, const DClock *__restrict__ clk,
-                                                 const double *__restrict__ Gas, const double *__restrict__ oGas,
+// magFlux AB2 (FP64 in, bulk out) on every face point; PredictCell on every
+// cell plus the per-point stencil inputs (WKT).
+template <typename R>
+__global__ void __launch_bounds__(256) k_predict(GVT<R> g, MP m, const DClock *__restrict__ clk,
+                                                 const R *__restrict__ Gas, const R *__restrict__ oGas,
                                                  const double *__restrict__ mag, const double *__restrict__ omag,
-                                                 double *__restrict__ hGas, double *__restrict__ hmag, WK w) {
+                                                 R *__restrict__ hGas, R *__restrict__ hmag, WKT<R> w) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nf || clk->halt) return;
     const double pdt = clk->pdt, odt = clk->odt;
@@ -78,30 +89,31 @@ __global__ void __launch_bounds__(256) k_predict(GV g, MP m, const DClock *__res
         for (int d = 0; d < 3; d++) {
             double mn = mag[(size_t)nf * d + idx];
             double hm = ab2(pdt, mn, omag[(size_t)nf * d + idx], odt);
-            hmag[(size_t)nf * d + idx] = hm;
-            double fa = fmax(g.face[(size_t)nf * d + idx], GAM_TINY);
-            w.bh[(size_t)nf * d + idx] = hm / fa;
-            w.bn[(size_t)nf * d + idx] = (m.doLFMLim ? mn : hm) / fa;
+            const R hmR = R(hm);
+            hmag[(size_t)nf * d + idx] = hmR;
+            R fa = rmax(g.face[(size_t)nf * d + idx], R(GAM_TINY));
+            w.bh[(size_t)nf * d + idx] = hmR / fa;
+            w.bn[(size_t)nf * d + idx] = (m.doLFMLim ? R(mn) : hmR) / fa;
         }
     }
     int i, j, k;
     decode_f(g, idx, i, j, k);
     if (i >= g.Ni || j >= g.Nj || k >= g.Nk) return;
     int c = ci(g, i, j, k);
-    double Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5];
+    R Q[5], oQ[5], W[5], oW[5], pW[5], pQ[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) { Q[v] = Gas[(size_t)nc * v + c]; oQ[v] = oGas[(size_t)nc * v + c]; }
     cell_c2p(m, oQ, oW);
     cell_c2p(m, Q, W);
 #pragma unroll
-    for (int v = 0; v < 5; v++) pW[v] = ab2(pdt, W[v], oW[v], odt);
-    pW[0] = fmax(pW[0], m.dFloor);
-    pW[4] = fmax(pW[4], m.pFloor);
+    for (int v = 0; v < 5; v++) pW[v] = ab2(R(pdt), W[v], oW[v], R(odt));
+    pW[0] = rmax(pW[0], R(m.dFloor));
+    pW[4] = rmax(pW[4], R(m.pFloor));
     cell_p2c(m, pW, pQ);
 #pragma unroll
     for (int v = 0; v < 5; v++) hGas[(size_t)nc * v + c] = pQ[v];
     // stencil-point quantities of this cell
-    double Wh[5], Wn[5];
+    R Wh[5], Wn[5];
     block_c2p(m, pQ, Wh);
     if (m.doLFMLim) block_c2p(m, Q, Wn);
 #pragma unroll
@@ -109,29 +121,32 @@ __global__ void __launch_bounds__(256) k_predict(GV g, MP m, const DClock *__res
         w.Wh[(size_t)nc * v + c] = Wh[v];
         w.Wn[(size_t)nc * v + c] = m.doLFMLim ? Wn[v] : Wh[v];
     }
-    const double vol = g.vol[c];
-    const double D = fmax(pQ[0], m.dFloor);
+    const R vol = g.vol[c];
+    const R D = rmax(pQ[0], R(m.dFloor));
 #pragma unroll
     for (int x = 0; x < 3; x++) w.VdV[(size_t)nc * x + c] = vol * pQ[1 + x] / D;
 }
 
-__global__ void __launch_bounds__(256) k_b2f(GV g, Tile t, const DClock *__restrict__ clk, const double *__restrict__ mag,
-                                             double *__restrict__ B) {
+// bFlux2Fld over all cells, FP64 arithmetic, stored in the bulk type
+template <typename R, typename TM>
+__global__ void __launch_bounds__(256) k_b2f(GVT<R> g, Tile t, const DClock *__restrict__ clk, const TM *__restrict__ mag,
+                                             R *__restrict__ B) {
     int i, j, k;
     if (!tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
     const int idx = ci(g, i, j, k);
     double b[3];
     cell_bxyz(g, mag, i, j, k, b);
 #pragma unroll
-    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = b[x];
+    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = R(b[x]);
 }
 
 // ----------------------------------------------------------------- E field
 // fields.F90:235-285 Mom2Face at (i,j,k) along dT, c = cell index; the
 // per-cell summand vol*M/max(D,dFloor) comes precomputed in VdV
-__device__ __forceinline__ void mom2face(const GV &g, const double *__restrict__ VdV, int c, int dT, double *Vf) {
+template <typename R>
+__device__ __forceinline__ void mom2face(const GVT<R> &g, const R *__restrict__ VdV, int c, int dT, R *Vf) {
     const int st = cstride(g, dT), nc = g.nc;
-    double VolB[8], Vx[8], Vy[8], Vz[8];
+    R VolB[8], Vx[8], Vy[8], Vz[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
         int cc = c + (n - 4) * st;
@@ -140,21 +155,22 @@ __device__ __forceinline__ void mom2face(const GV &g, const double *__restrict__
         Vy[n] = VdV[(size_t)nc + cc];
         Vz[n] = VdV[2 * (size_t)nc + cc];
     }
-    double dV = fmax(c8(VolB), GAM_TINY);
+    R dV = rmax(c8(VolB), R(GAM_TINY));
     Vf[0] = c8(Vx) / dV;
     Vf[1] = c8(Vy) / dV;
     Vf[2] = c8(Vz) / dV;
 }
 
 // Vf arrays: VfA[dT1][xyz] stored as slab (3*dT1 + xyz)
-__global__ void __launch_bounds__(256) k_vf(GV g, Tile t, const DClock *__restrict__ clk, const double *__restrict__ VdV,
-                                            double *__restrict__ Vf) {
+template <typename R>
+__global__ void __launch_bounds__(256) k_vf(GVT<R> g, Tile t, const DClock *__restrict__ clk, const R *__restrict__ VdV,
+                                            R *__restrict__ Vf) {
     int i, j, k;
     if (!tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
     const int idx = ci(g, i, j, k);
     const int a0 = g.nG;
     const size_t nc = g.nc;
-    double v[3];
+    R v[3];
     // dT1 = I (used by K edges): i in is..ie+1   (fields.F90:161-175)
     if (i >= a0 && i <= a0 + g.Nip) {
         mom2face(g, VdV, idx, 0, v);
@@ -174,13 +190,13 @@ __global__ void __launch_bounds__(256) k_vf(GV g, Tile t, const DClock *__restri
 
 // fields.F90:288-362 EdgeLRs: dT1 faces reconstructed along dT2 at face point f;
 // the face-normal field bh/bn = flux/max(area,TINY) comes precomputed
-template <int RM>
-__device__ __forceinline__ void edge_lrs(const GV &g, const MP &m, const double *__restrict__ bn,
-                                         const double *__restrict__ bh, int f, int dT1, int dT2,
-                                         double &bL, double &bR, double &fA) {
+template <int RM, typename R>
+__device__ __forceinline__ void edge_lrs(const GVT<R> &g, const MP &m, const R *__restrict__ bn,
+                                         const R *__restrict__ bh, int f, int dT1, int dT2,
+                                         R &bL, R &bR, R &fA) {
     const int st = fstride(g, dT2);
     const size_t off = (size_t)g.nf * dT1;
-    double AreaB[8], MagB[8], nMagB[4];
+    R AreaB[8], MagB[8], nMagB[4];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
         int ff = f + (n - 4) * st;
@@ -189,24 +205,25 @@ __device__ __forceinline__ void edge_lrs(const GV &g, const MP &m, const double 
     }
 #pragma unroll
     for (int n = 0; n < 4; n++) nMagB[n] = bn[off + f + (n - 2) * st];
-    double Vi = c8(AreaB);
-    double l, r;
-    get_lrs<RM>(m.pdmb, AreaB, nMagB, MagB, Vi, l, r);
+    R Vi = c8(AreaB);
+    R l, r;
+    get_lrs<RM>(R(m.pdmb), AreaB, nMagB, MagB, Vi, l, r);
     bL = l * Vi; bR = r * Vi; fA = Vi;
 }
 
 // One edge EMF (fields.F90:196-221) for edge direction eD at corner (i,j,k)
-template <int RM, int BORIS>
-__device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, const double *__restrict__ Vf,
-                                           const double *__restrict__ hGas, const double *__restrict__ bn,
-                                           const double *__restrict__ bh, int eD, int i, int j, int k) {
+template <int RM, int BORIS, typename R>
+__device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, double dt64, const R *__restrict__ Vf,
+                                      const R *__restrict__ hGas, const R *__restrict__ bn,
+                                      const R *__restrict__ bh, int eD, int i, int j, int k) {
+    const R dt = R(dt64);
     const int dT1 = (eD + 1) % 3, dT2 = (eD + 2) % 3;
     const int f = fi(g, i, j, k), c = ci(g, i, j, k);
     const size_t nf = g.nf, nc = g.nc;
     // GetCornerV fields.F90:457-517 (Vf of the dT1 faces interpolated along dT2)
     const int fst = fstride(g, dT2), cst = cstride(g, dT2);
-    const double *VfA = Vf + nc * 3 * dT1;
-    double AreaB[8], Vx[8], Vy[8], Vz[8];
+    const R *VfA = Vf + nc * 3 * dT1;
+    R AreaB[8], Vx[8], Vy[8], Vz[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) {
         AreaB[n] = g.face[nf * dT1 + f + (n - 4) * fst];
@@ -215,37 +232,38 @@ __device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, 
         Vy[n] = AreaB[n] * VfA[nc + cc];
         Vz[n] = AreaB[n] * VfA[2 * nc + cc];
     }
-    double dA = fmax(c8(AreaB), GAM_TINY);
-    double Vcx = c8(Vx) / dA, Vcy = c8(Vy) / dA, Vcz = c8(Vz) / dA;
-    const double *Te = g.Te + nf * 9 * eD;
-    double v1 = Vcx * Te[nf * TAN1X + f] + Vcy * Te[nf * TAN1Y + f] + Vcz * Te[nf * TAN1Z + f];
-    double v2 = Vcx * Te[nf * TAN2X + f] + Vcy * Te[nf * TAN2Y + f] + Vcz * Te[nf * TAN2Z + f];
-    double vDiff = sqrt(sq(v1) + sq(v2));
+    R dA = rmax(c8(AreaB), R(GAM_TINY));
+    R Vcx = c8(Vx) / dA, Vcy = c8(Vy) / dA, Vcz = c8(Vz) / dA;
+    const R *Te = g.Te + nf * 9 * eD;
+    R v1 = Vcx * Te[nf * TAN1X + f] + Vcy * Te[nf * TAN1Y + f] + Vcz * Te[nf * TAN1Z + f];
+    R v2 = Vcx * Te[nf * TAN2X + f] + Vcy * Te[nf * TAN2Y + f] + Vcz * Te[nf * TAN2Z + f];
+    R vDiff = rroot(sq(v1) + sq(v2));
     // GetCornerD fields.F90:427-454
     // normal plane basis (metric.F90:151-167): I -> (j,k), J -> (i,k), K -> (i,j)
     const int s1 = cstride(g, eD == 0 ? 1 : 0), s2 = cstride(g, eD == 2 ? 1 : 2);
-    double Dc = 0.25 * (hGas[c] + hGas[c - s1] + hGas[c - s2] + hGas[c - s1 - s2]);
+    R Dc = R(0.25) * (hGas[c] + hGas[c - s1] + hGas[c - s2] + hGas[c - s1 - s2]);
     // GetCornerB fields.F90:365-424
-    double bT1L, bT1R, fA1, bT2L, bT2R, fA2;
+    R bT1L, bT1R, fA1, bT2L, bT2R, fA2;
     edge_lrs<RM>(g, m, bn, bh, f, dT1, dT2, bT1L, bT1R, fA1);
     edge_lrs<RM>(g, m, bn, bh, f, dT2, dT1, bT2L, bT2R, fA2);
-    const double *Teb = g.Teb + nf * 4 * eD;
-    double xni = Teb[nf * XNQI + f], yni = Teb[nf * YNQI + f];
-    double xnj = Teb[nf * XNQJ + f], ynj = Teb[nf * YNQJ + f];
-    double detT = 1.0 / (xni * ynj - xnj * yni);
-    double bT1 = 0.5 * (bT1L + bT1R) / fA1;
-    double bT2 = 0.5 * (bT2L + bT2R) / fA2;
-    double db1 = (bT1R - bT1L) / fA1;
-    double db2 = (bT2R - bT2L) / fA2;
-    double Jd = db2 - db1;
-    double b1 = detT * (ynj * bT1 - yni * bT2);
-    double b2 = detT * (-xnj * bT1 + xni * bT2);
-    double vA = sqrt(sq(b1) + sq(b2)) / sqrt(Dc);
-    if (BORIS) vA = m.Ca * vA / sqrt(sq(m.Ca) + sq(vA));
+    const R *Teb = g.Teb + nf * 4 * eD;
+    R xni = Teb[nf * XNQI + f], yni = Teb[nf * YNQI + f];
+    R xnj = Teb[nf * XNQJ + f], ynj = Teb[nf * YNQJ + f];
+    R detT = R(1.0) / (xni * ynj - xnj * yni);
+    R bT1 = R(0.5) * (bT1L + bT1R) / fA1;
+    R bT2 = R(0.5) * (bT2L + bT2R) / fA2;
+    R db1 = (bT1R - bT1L) / fA1;
+    R db2 = (bT2R - bT2L) / fA2;
+    R Jd = db2 - db1;
+    R b1 = detT * (ynj * bT1 - yni * bT2);
+    R b2 = detT * (-xnj * bT1 + xni * bT2);
+    R vA = rroot(sq(b1) + sq(b2)) / rroot(Dc);
+    const R Ca = R(m.Ca);
+    if (BORIS) vA = Ca * vA / rroot(sq(Ca) + sq(vA));
     vDiff = vDiff + vA;
-    double ed = g.edge[nf * eD + f];
-    vDiff = fmin(vDiff, m.CFL * ed / dt);
-    double e = -(v1 * b2 - v2 * b1) + m.Vd0 * vDiff * Jd;
+    R ed = g.edge[nf * eD + f];
+    vDiff = rmin(vDiff, R(m.CFL) * ed / dt);
+    R e = -(v1 * b2 - v2 * b1) + R(m.Vd0) * vDiff * Jd;
     return e * ed;
 }
 
@@ -255,10 +273,10 @@ __device__ __forceinline__ double emf_edge(const GV &g, const MP &m, double dt, 
 // slabs are written only on active edges; k_ct treats the rest as zero
 // (CalcElecField zeroes E, fields.F90:89).  One kernel per direction, so each
 // gets a launch brick shaped for its two stencil axes (dT1 and dT2).
-template <int ED, int RM, int BORIS>
-__global__ void __launch_bounds__(256) k_emf(GV g, Tile t, MP m, const DClock *__restrict__ clk, const double *__restrict__ Vf,
-                                             const double *__restrict__ hGas, const double *__restrict__ bn,
-                                             const double *__restrict__ bh, double *__restrict__ E) {
+template <int ED, int RM, int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, const DClock *__restrict__ clk, const R *__restrict__ Vf,
+                                             const R *__restrict__ hGas, const R *__restrict__ bn,
+                                             const R *__restrict__ bh, R *__restrict__ E) {
     int a, b, cz;
     if (!tile_pos(t, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2), a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
@@ -267,40 +285,41 @@ __global__ void __launch_bounds__(256) k_emf(GV g, Tile t, MP m, const DClock *_
 
 // ----------------------------------------------------------------- fluxes
 // stress.F90:513-599 MagKinFlux (no background field), q = -1/+1
-template <int BORIS>
-__device__ __forceinline__ void magkin(const MP &m, double sg, const double *P5, const double *B, double Bn,
-                                       const double *T, double *mF, double &bbD, double &VaD) {
-    double D = P5[0], P = P5[4];
-    double Bx = B[0], By = B[1], Bz = B[2];
-    double dPb = 0.5 * (sq(Bx) + sq(By) + sq(Bz));
-    double Va2 = 2 * dPb / D;
-    bbD = bbD + 0.5 * (sq(Bx) + sq(By) + sq(Bz));
-    double Vac2 = BORIS ? Va2 / (1 + (Va2 / sq(m.Ca))) : Va2;
-    double Nx = T[NORMX], Ny = T[NORMY], Nz = T[NORMZ];
-    double Vn = Nx * P5[1] + Ny * P5[2] + Nz * P5[3];
-    double lambda = 1 / (2 * P / D + Vac2);
-    double Vn0 = 0.5 * erfc(sg * sqrt(lambda) * Vn);
-    VaD = VaD + 0.5 * sqrt(Vac2);
+template <int BORIS, typename R>
+__device__ __forceinline__ void magkin(const MP &m, R sg, const R *P5, const R *B, R Bn,
+                                       const R *T, R *mF, R &bbD, R &VaD) {
+    R D = P5[0], P = P5[4];
+    R Bx = B[0], By = B[1], Bz = B[2];
+    R dPb = R(0.5) * (sq(Bx) + sq(By) + sq(Bz));
+    R Va2 = 2 * dPb / D;
+    bbD = bbD + R(0.5) * (sq(Bx) + sq(By) + sq(Bz));
+    R Vac2 = BORIS ? Va2 / (1 + (Va2 / sq(R(m.Ca)))) : Va2;
+    R Nx = T[NORMX], Ny = T[NORMY], Nz = T[NORMZ];
+    R Vn = Nx * P5[1] + Ny * P5[2] + Nz * P5[3];
+    R lambda = 1 / (2 * P / D + Vac2);
+    R Vn0 = R(0.5) * rerfc(sg * rroot(lambda) * Vn);
+    VaD = VaD + R(0.5) * rroot(Vac2);
     mF[0] = mF[0] + Vn0 * (dPb * Nx - Bx * Bn);
     mF[1] = mF[1] + Vn0 * (dPb * Ny - By * Bn);
     mF[2] = mF[2] + Vn0 * (dPb * Nz - Bz * Bn);
 }
 
 // stress.F90:432-511 GasKinFlux
-__device__ __forceinline__ void gaskin(const MP &m, double sg, const double *P5, const double *T, double *gF, double *dW) {
-    double D = P5[0], P = P5[4], Vx = P5[1], Vy = P5[2], Vz = P5[3];
-    double E = 0.5 * D * (sq(Vx) + sq(Vy) + sq(Vz)) + P / (m.gamma - 1);
-    double lambda = D / (2 * P);
-    double Vn = T[NORMX] * Vx + T[NORMY] * Vy + T[NORMZ] * Vz;
-    double Vt1 = T[TAN1X] * Vx + T[TAN1Y] * Vy + T[TAN1Z] * Vz;
-    double Vt2 = T[TAN2X] * Vx + T[TAN2Y] * Vy + T[TAN2Z] * Vz;
-    double Vn0 = 0.5 * erfc(sg * sqrt(lambda) * Vn);
-    double Vn1 = Vn * Vn0 - 0.5 * sg * exp(-lambda * Vn * Vn) / sqrt(GAM_PI * lambda);
+template <typename R>
+__device__ __forceinline__ void gaskin(const MP &m, R sg, const R *P5, const R *T, R *gF, R *dW) {
+    R D = P5[0], P = P5[4], Vx = P5[1], Vy = P5[2], Vz = P5[3];
+    R E = R(0.5) * D * (sq(Vx) + sq(Vy) + sq(Vz)) + P / R(m.gamma - 1);
+    R lambda = D / (2 * P);
+    R Vn = T[NORMX] * Vx + T[NORMY] * Vy + T[NORMZ] * Vz;
+    R Vt1 = T[TAN1X] * Vx + T[TAN1Y] * Vy + T[TAN1Z] * Vz;
+    R Vt2 = T[TAN2X] * Vx + T[TAN2Y] * Vy + T[TAN2Z] * Vz;
+    R Vn0 = R(0.5) * rerfc(sg * rroot(lambda) * Vn);
+    R Vn1 = Vn * Vn0 - R(0.5) * sg * rexp(-lambda * Vn * Vn) / rroot(R(GAM_PI) * lambda);
     gF[0] = gF[0] + D * Vn1;
-    gF[4] = gF[4] + (E + 0.5 * P) * Vn1 + 0.5 * P * Vn0 * Vn;
-    double fMn = D * Vn * Vn1 + P * Vn0;
-    double fMt1 = D * Vt1 * Vn1;
-    double fMt2 = D * Vt2 * Vn1;
+    gF[4] = gF[4] + (E + R(0.5) * P) * Vn1 + R(0.5) * P * Vn0 * Vn;
+    R fMn = D * Vn * Vn1 + P * Vn0;
+    R fMt1 = D * Vt1 * Vn1;
+    R fMt2 = D * Vt2 * Vn1;
     gF[1] = gF[1] + (T[NORMX] * fMn + T[TAN1X] * fMt1 + T[TAN2X] * fMt2);
     gF[2] = gF[2] + (T[NORMY] * fMn + T[TAN1Y] * fMt1 + T[TAN2Y] * fMt2);
     gF[3] = gF[3] + (T[NORMZ] * fMn + T[TAN1Z] * fMt1 + T[TAN2Z] * fMt2);
@@ -314,12 +333,12 @@ __device__ __forceinline__ void gaskin(const MP &m, double sg, const double *P5,
 // Interface fluxes in direction DN (stress.F90:204-429) for the interface box
 // DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
 // Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
-template <int DN, int RM, int BORIS, int MHD>
-__global__ void __launch_bounds__(256) k_flux(GV g, Tile t, MP m, const DClock *__restrict__ clk,
-                                              const double *__restrict__ Wn, const double *__restrict__ nB,
-                                              const double *__restrict__ Wh, const double *__restrict__ hB,
-                                              const double *__restrict__ hmag, const double *__restrict__ hGas,
-                                              double *__restrict__ Flx) {
+template <int DN, int RM, int BORIS, int MHD, typename R>
+__global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, const DClock *__restrict__ clk,
+                                              const R *__restrict__ Wn, const R *__restrict__ nB,
+                                              const R *__restrict__ Wh, const R *__restrict__ hB,
+                                              const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                              R *__restrict__ Flx) {
     const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
     int a, b, cz;
     if (!tile_pos(t, nx, ny, nz, a, b, cz) || clk->halt) return;
@@ -327,70 +346,72 @@ __global__ void __launch_bounds__(256) k_flux(GV g, Tile t, MP m, const DClock *
     const size_t nc = g.nc, nf = g.nf;
     const int c = ci(g, i, j, k), f = fi(g, i, j, k);
     const int st = cstride(g, DN);
+    const R pdmb = R(m.pdmb);
 
-    double VolB[8];
+    R VolB[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) VolB[n] = g.vol[c + (n - 4) * st];
-    const double Vi = c8(VolB);
-    double T[9];
+    const R Vi = c8(VolB);
+    R T[9];
 #pragma unroll
-    for (int t = 0; t < 9; t++) T[t] = g.Tf[nf * (t + 9 * DN) + f];
-    const double faB = g.face[nf * DN + f];
+    for (int q = 0; q < 9; q++) T[q] = g.Tf[nf * (q + 9 * DN) + f];
+    const R faB = g.face[nf * DN + f];
 
-    double PL[5], PR[5];
-    double MagL[3] = {0, 0, 0}, MagR[3] = {0, 0, 0}, Bn = 0.0;
+    R PL[5], PR[5];
+    R MagL[3] = {0, 0, 0}, MagR[3] = {0, 0, 0}, Bn = R(0.0);
     if (MHD) {
-        const double *limB = m.doLFMLim ? nB : hB;
+        const R *limB = m.doLFMLim ? nB : hB;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            double Q[8], nQ[4];
+            R Q[8], nQ[4];
 #pragma unroll
             for (int n = 0; n < 8; n++) Q[n] = hB[nc * d + c + (n - 4) * st];
 #pragma unroll
             for (int n = 0; n < 4; n++) nQ[n] = limB[nc * d + c + (n - 2) * st];
-            get_lrs<RM>(m.pdmb, VolB, nQ, Q, Vi, MagL[d], MagR[d]);
+            get_lrs<RM>(pdmb, VolB, nQ, Q, Vi, MagL[d], MagR[d]);
         }
         Bn = hmag[nf * DN + f] / faB;
     }
 #pragma unroll
     for (int v = 0; v < 5; v++) {   // BlockStateLRs on the prepass primitives
-        double Q[8], nQ[4];
+        R Q[8], nQ[4];
 #pragma unroll
         for (int n = 0; n < 8; n++) Q[n] = Wh[nc * v + c + (n - 4) * st];
 #pragma unroll
         for (int n = 0; n < 4; n++) nQ[n] = Wn[nc * v + c + (n - 2) * st];
-        get_lrs<RM>(m.pdmb, VolB, nQ, Q, Vi, PL[v], PR[v]);
+        get_lrs<RM>(pdmb, VolB, nQ, Q, Vi, PL[v], PR[v]);
     }
-    double VaD = 0.0;
+    const R Ca = R(m.Ca);
+    R VaD = R(0.0);
     if (MHD) {
-        double mF[3] = {0, 0, 0}, bbD = 0.0;
-        magkin<BORIS>(m, -1.0, PL, MagL, Bn, T, mF, bbD, VaD);
-        magkin<BORIS>(m, 1.0, PR, MagR, Bn, T, mF, bbD, VaD);
+        R mF[3] = {0, 0, 0}, bbD = R(0.0);
+        magkin<BORIS>(m, R(-1.0), PL, MagL, Bn, T, mF, bbD, VaD);
+        magkin<BORIS>(m, R(1.0), PR, MagR, Bn, T, mF, bbD, VaD);
         if (BORIS) {   // stress.F90:335-351
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                double deeV = PR[1 + d] - PL[1 + d];
-                double deeP = bbD * deeV / sq(m.Ca);
-                mF[d] = mF[d] - m.cHogM * VaD * deeP;
+                R deeV = PR[1 + d] - PL[1 + d];
+                R deeP = bbD * deeV / sq(Ca);
+                mF[d] = mF[d] - R(m.cHogM) * VaD * deeP;
             }
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) Flx[nc * (8 * DN + 5 + d) + c] = faB * mF[d];
     }
-    double gF[5] = {0, 0, 0, 0, 0}, dW[5] = {0, 0, 0, 0, 0};
-    gaskin(m, -1.0, PL, T, gF, dW);
-    gaskin(m, 1.0, PR, T, gF, dW);
+    R gF[5] = {0, 0, 0, 0, 0}, dW[5] = {0, 0, 0, 0, 0};
+    gaskin(m, R(-1.0), PL, T, gF, dW);
+    gaskin(m, R(1.0), PR, T, gF, dW);
 #pragma unroll
-    for (int v = 0; v < 5; v++) gF[v] = gF[v] - 1.0 * m.cHogH * VaD * dW[v];
+    for (int v = 0; v < 5; v++) gF[v] = gF[v] - R(1.0) * R(m.cHogH) * VaD * dW[v];
     if (BORIS) {   // stress.F90:399-417
-        double vL = sqrt(sq(PL[1]) + sq(PL[2]) + sq(PL[3]));
-        double vR = sqrt(sq(PR[1]) + sq(PR[2]) + sq(PR[3]));
-        double Vn = sqrt(sq(vL) + sq(vR)) / (double)sqrtf(2.0f);
-        if (Vn >= 1.5 * m.Ca) {
+        R vL = rroot(sq(PL[1]) + sq(PL[2]) + sq(PL[3]));
+        R vR = rroot(sq(PR[1]) + sq(PR[2]) + sq(PR[3]));
+        R Vn = rroot(sq(vL) + sq(vR)) / R((double)sqrtf(2.0f));
+        if (Vn >= R(1.5) * Ca) {
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                double dCon = hGas[nc * v + c] - hGas[nc * v + c - st];
-                gF[v] = gF[v] - m.Ca * dCon;
+                R dCon = hGas[nc * v + c] - hGas[nc * v + c - st];
+                gF[v] = gF[v] - Ca * dCon;
             }
         }
     }
@@ -399,18 +420,21 @@ __global__ void __launch_bounds__(256) k_flux(GV g, Tile t, MP m, const DClock *
 }
 
 // ----------------------------------------------------------------- CT update
-// fields.F90:18-51 E2Flux into the new buffer: faces in [is,ie+1]^3 box get the
-// curl update, all others are copied.  E outside the active edge ranges is 0.
-__device__ __forceinline__ double Eval(const GV &g, const double *E, int d, int i, int j, int k) {
+// fields.F90:18-51 E2Flux into the new buffer, in FP64: faces in [is,ie+1]^3
+// box get the curl update, all others are copied.  E outside the active edge
+// ranges is 0.
+template <typename R>
+__device__ __forceinline__ double Eval(const GVT<R> &g, const R *E, int d, int i, int j, int k) {
     const int a0 = g.nG;
     int ai = i - a0, aj = j - a0, ak = k - a0;
     int mi = g.Nip - (d == 0), mj = g.Njp - (d == 1), mk = g.Nkp - (d == 2);
     if (ai < 0 || aj < 0 || ak < 0 || ai > mi || aj > mj || ak > mk) return 0.0;
-    return E[(size_t)g.nc * d + ci(g, i, j, k)];
+    return (double)E[(size_t)g.nc * d + ci(g, i, j, k)];
 }
 
-__global__ void __launch_bounds__(256) k_ct(GV g, const DClock *__restrict__ clk, const double *__restrict__ magN,
-                                            const double *__restrict__ E, double *__restrict__ magO) {
+template <typename R>
+__global__ void __launch_bounds__(256) k_ct(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ magN,
+                                            const R *__restrict__ E, double *__restrict__ magO) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nf || clk->halt) return;
     const size_t nf = g.nf;
@@ -431,7 +455,7 @@ __global__ void __launch_bounds__(256) k_ct(GV g, const DClock *__restrict__ clk
     magO[idx] = m0; magO[nf + idx] = m1; magO[2 * nf + idx] = m2;
 }
 
-// ----------------------------------------------------------------- update
+// ----------------------------------------------------------------- update (FP64)
 // mhdgroup.F90:300-330 CellMaxwell
 __device__ __forceinline__ void cell_maxwell(const MP &m, double *U, const double *dGM, double dt) {
     double D = fmax(U[0], m.dFloor);
@@ -493,9 +517,9 @@ __device__ __forceinline__ void cell_boris(const MP &m, double *U2, const double
     cell_p2c(m, W, U2);
 }
 
-// gridutils.F90:49-106 CellDT (single fluid, no resistivity)
-template <int BORIS>
-__device__ __forceinline__ double cell_dt(const GV &g, const MP &m, const double *U, const double *B, int c) {
+// gridutils.F90:49-106 CellDT (single fluid, no resistivity), FP64
+template <int BORIS, typename R>
+__device__ __forceinline__ double cell_dt(const GVT<R> &g, const MP &m, const double *U, const double *B, int c) {
     double rho = U[0];
     double Vx = U[1] / rho, Vy = U[2] / rho, Vz = U[3] / rho;
     double MagV = sqrt(sq(Vx) + sq(Vy) + sq(Vz));
@@ -510,7 +534,7 @@ __device__ __forceinline__ double cell_dt(const GV &g, const MP &m, const double
         if (BORIS) Valf = m.Ca * Valf / sqrt(m.Ca * m.Ca + Valf * Valf);
     }
     double vCFL = MagV + sqrt(sq(Cs) + sq(Valf)) + 0.0;
-    double dl = fmin(fmin(g.di[c], g.dj[c]), g.dk[c]);
+    double dl = fmin(fmin((double)g.di[c], (double)g.dj[c]), (double)g.dk[c]);
     return m.CFL * dl / vCFL;
 }
 
@@ -539,19 +563,20 @@ __device__ __forceinline__ void block_min_store(double v, double *partial) {
     }
 }
 
-__device__ __forceinline__ bool in_dt_box(const GV &g, int i, int j, int k) {
+template <typename R> __device__ __forceinline__ bool in_dt_box(const GVT<R> &g, int i, int j, int k) {
     // SetDomain init.F90:519-538: active +-1 layer (single block owns all BCs)
     const int a0 = g.nG - 1;
     return i >= a0 && i <= g.nG + g.Nip && j >= a0 && j <= g.nG + g.Njp && k >= a0 && k <= g.nG + g.Nkp;
 }
 
 // Flux2Deltas + applyUpdate (Reynolds, Bxyz of the new fluxes, Maxwell/Boris)
-// + CellDT min.  Writes the new Gas/Bxyz into the O buffers; ghosts copy Gas n.
-template <int BORIS, int MHD>
-__global__ void __launch_bounds__(256) k_update(GV g, Tile t, MP m, const DClock *__restrict__ clk,
-                                                const double *__restrict__ nGas, const double *__restrict__ nB,
-                                                const double *__restrict__ Flx, const double *__restrict__ magNew,
-                                                double *__restrict__ oGas, double *__restrict__ oB,
+// + CellDT min, in FP64 from the bulk-typed inputs.  Writes the new Gas/Bxyz
+// into the O buffers; ghosts copy Gas n.
+template <int BORIS, int MHD, typename R>
+__global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, const DClock *__restrict__ clk,
+                                                const R *__restrict__ nGas, const R *__restrict__ nB,
+                                                const R *__restrict__ Flx, const double *__restrict__ magNew,
+                                                R *__restrict__ oGas, R *__restrict__ oB,
                                                 double *__restrict__ partial) {
     if (clk->halt) return;
     double dtl = GAM_HUGE;
@@ -561,27 +586,26 @@ __global__ void __launch_bounds__(256) k_update(GV g, Tile t, MP m, const DClock
         const size_t nc = g.nc;
         double U[5], Bn[3] = {0, 0, 0};
 #pragma unroll
-        for (int v = 0; v < 5; v++) U[v] = nGas[nc * v + idx];
+        for (int v = 0; v < 5; v++) U[v] = (double)nGas[nc * v + idx];
         cell_bxyz(g, magNew, i, j, k, Bn);
         const int a0 = g.nG;
         bool active = i >= a0 && i < a0 + g.Nip && j >= a0 && j < a0 + g.Njp && k >= a0 && k < a0 + g.Nkp;
         if (active) {
             const double dt = clk->dt;
             const int ci1 = idx + 1, cj1 = idx + g.Ni, ck1 = idx + g.Ni * g.Nj;
-            const double dV = g.vol[idx];
+            const double dV = g.dvol[idx];
+            auto F = [&](int slab, int c) { return (double)Flx[nc * slab + c]; };
             double dGH[5], dGM[3] = {0, 0, 0};
 #pragma unroll
             for (int v = 0; v < 5; v++) {
-                dGH[v] = (Flx[nc * (0 + v) + idx] - Flx[nc * (0 + v) + ci1]
-                        + Flx[nc * (8 + v) + idx] - Flx[nc * (8 + v) + cj1]
-                        + Flx[nc * (16 + v) + idx] - Flx[nc * (16 + v) + ck1]) / dV;
+                dGH[v] = (F(0 + v, idx) - F(0 + v, ci1) + F(8 + v, idx) - F(8 + v, cj1)
+                        + F(16 + v, idx) - F(16 + v, ck1)) / dV;
             }
             if (MHD) {
 #pragma unroll
                 for (int v = 0; v < 3; v++) {
-                    dGM[v] = (Flx[nc * (5 + v) + idx] - Flx[nc * (5 + v) + ci1]
-                            + Flx[nc * (13 + v) + idx] - Flx[nc * (13 + v) + cj1]
-                            + Flx[nc * (21 + v) + idx] - Flx[nc * (21 + v) + ck1]) / dV;
+                    dGM[v] = (F(5 + v, idx) - F(5 + v, ci1) + F(13 + v, idx) - F(13 + v, cj1)
+                            + F(21 + v, idx) - F(21 + v, ck1)) / dV;
                 }
             }
             double U0[5];
@@ -589,7 +613,7 @@ __global__ void __launch_bounds__(256) k_update(GV g, Tile t, MP m, const DClock
             for (int v = 0; v < 5; v++) { U0[v] = U[v]; U[v] = U[v] + dt * dGH[v]; }   // CellReynolds, Model%dt
             if (MHD) {
                 if (BORIS) {
-                    double oBv[3] = {nB[idx], nB[nc + idx], nB[2 * nc + idx]};
+                    double oBv[3] = {(double)nB[idx], (double)nB[nc + idx], (double)nB[2 * nc + idx]};
                     cell_boris(m, U, U0, Bn, oBv, dGH, dGM, m.Ca, dt);
                 } else {
                     cell_maxwell(m, U, dGM, dt);
@@ -597,18 +621,26 @@ __global__ void __launch_bounds__(256) k_update(GV g, Tile t, MP m, const DClock
             }
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) oGas[nc * v + idx] = U[v];
+        for (int v = 0; v < 5; v++) oGas[nc * v + idx] = R(U[v]);
 #pragma unroll
-        for (int x = 0; x < 3; x++) oB[nc * x + idx] = Bn[x];
-        if (in_dt_box(g, i, j, k)) dtl = cell_dt<BORIS>(g, m, U, Bn, idx);
+        for (int x = 0; x < 3; x++) oB[nc * x + idx] = R(Bn[x]);
+        if (in_dt_box(g, i, j, k)) {
+            if (sizeof(R) < sizeof(double)) {   // CFL bound of the stored state
+#pragma unroll
+                for (int v = 0; v < 5; v++) U[v] = (double)R(U[v]);
+#pragma unroll
+                for (int x = 0; x < 3; x++) Bn[x] = (double)R(Bn[x]);
+            }
+            dtl = cell_dt<BORIS>(g, m, U, Bn, idx);
+        }
     }
     block_min_store(dtl, partial);
 }
 
 // Stand-alone CellDT min over the DT box (CalcDT without the fused update)
-template <int BORIS>
-__global__ void __launch_bounds__(256) k_celldt(GV g, MP m, const DClock *__restrict__ clk, const double *__restrict__ Gas,
-                                                const double *__restrict__ B, double *__restrict__ partial) {
+template <int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_celldt(GVT<R> g, MP m, const DClock *__restrict__ clk, const R *__restrict__ Gas,
+                                                const R *__restrict__ B, double *__restrict__ partial) {
     if (clk->halt) return;
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     double dtl = GAM_HUGE;
@@ -619,9 +651,9 @@ __global__ void __launch_bounds__(256) k_celldt(GV g, MP m, const DClock *__rest
             const size_t nc = g.nc;
             double U[5], Bv[3];
 #pragma unroll
-            for (int v = 0; v < 5; v++) U[v] = Gas[nc * v + idx];
+            for (int v = 0; v < 5; v++) U[v] = (double)Gas[nc * v + idx];
 #pragma unroll
-            for (int x = 0; x < 3; x++) Bv[x] = B[nc * x + idx];
+            for (int x = 0; x < 3; x++) Bv[x] = (double)B[nc * x + idx];
             dtl = cell_dt<BORIS>(g, m, U, Bv, idx);
         }
     }
@@ -698,8 +730,9 @@ __device__ __forceinline__ int pmap_f(int i, int a0, int n) {   // owned faces a
     return i;
 }
 
-__global__ void __launch_bounds__(256) k_bc(GV g, const DClock *__restrict__ clk, int do25D, double *__restrict__ Gas,
-                                            double *__restrict__ mag, double *__restrict__ B) {
+template <typename R>
+__global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__ clk, int do25D, R *__restrict__ Gas,
+                                            double *__restrict__ mag, R *__restrict__ B) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nf || clk->halt) return;
     int i, j, k;
@@ -735,6 +768,13 @@ __global__ void __launch_bounds__(256) k_bc(GV g, const DClock *__restrict__ clk
         if (!do25D && d != 2 && k == g.Nk) sk = k;
         if (si != i || sj != j || sk != k) mag[nf * d + idx] = mag[nf * d + fi(g, si, sj, sk)];
     }
+}
+
+// ----------------------------------------------------------------- conversions
+template <typename TI, typename TO>
+__global__ void __launch_bounds__(256) k_cvt(const TI *__restrict__ in, TO *__restrict__ out, size_t n) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n) out[idx] = TO(in[idx]);
 }
 
 // ----------------------------------------------------------------- diagnostics

@@ -42,6 +42,9 @@ module gamtypes_gpu
         real(rp) :: pdmb = 0.75_rp, dFloor = TINY, pFloor = TINY
         character(len=8) :: rmeth = "7UP"
         integer :: status = 0      !> last CalcDT outcome (0 ok, 1 bad, 2 disaster)
+        !> FP32-bulk step (FP64 kept for magFlux/CT, the cell update and dt);
+        !> not a reference knob, the reference is FP64 throughout (kdefs.F90:14)
+        logical :: doFP32Bulk = .false.
     end type Model_T
 
     !> gamtypes.F90:163-245 (sizes, index ranges and the geometry arrays)
@@ -144,7 +147,7 @@ contains
         p%do25D = merge(1, 0, Model%do25D)
         p%doCPR = merge(1, 0, Model%doCPR)
         p%fixedTimestep = merge(1, 0, Model%fixedTimestep)
-        p%precision = 0
+        p%precision = merge(1, 0, Model%doFP32Bulk)
     end function ModelParams
 
 end module gamtypes_gpu

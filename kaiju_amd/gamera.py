@@ -126,7 +126,7 @@ def params_from_spec(spec: configs.CaseSpec) -> GamParams:
     p.do25D = int(spec.do25D)
     p.doCPR = int(m.doCPR)
     p.fixedTimestep = 0
-    p.precision = 0
+    p.precision = int(m.precision)
     return p
 
 
