@@ -158,6 +158,24 @@ int  gam_stage_bytes(gam_handle *h, int stage, double *bytes);
 int  gam_set_tile(gam_handle *h, int stage, int ti, int tj, int tk, int fast);
 int  gam_get_tile(gam_handle *h, int stage, int *tile4);
 
+/* --- ensembles of blocks under one captured graph -------------------------- */
+/* No counterpart in the reference, which steps one block per MPI rank
+ * (gamera/mpi/gamapp_mpi.F90); independent blocks of different sizes are
+ * separate runs there.  gam_graph_create captures nsteps (even, >= 2) fused
+ * steps of each of the n handles -- the same device sequence as gam_step --
+ * as independent branches of one hipGraph; gam_graph_launch replays it
+ * `replays` times (nsteps * replays steps per block, each block with its own
+ * dt); gam_graph_sync waits and returns the largest CalcDT status (a block
+ * whose status is nonzero stops where gam_step would have stopped).  Between
+ * gam_graph_launch and gam_graph_sync the handles belong to the graph: make no
+ * other call on them.  Stage timers must be off while capturing. */
+typedef struct gam_graph gam_graph;
+int  gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_graph **out);
+int  gam_graph_launch(gam_graph *g, int replays);
+int  gam_graph_sync(gam_graph *g);
+int  gam_graph_destroy(gam_graph *g);
+const char *gam_graph_error(const gam_graph *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,17 +63,20 @@ class CaseSpec:
         return self.N[2] == 1
 
 
-def orszag_tang(n: int = 256, nk: int = 4, steps: int = 200, **mkw) -> CaseSpec:
-    """2D Orszag-Tang vortex on [0,2pi]^2 x [0, nk*2pi/n], gamma = 5/3.
+def orszag_tang(n=256, nk: int = 4, steps: int = 200, **mkw) -> CaseSpec:
+    """2D Orszag-Tang vortex on [0,2pi]^2 x [0, nk*2pi/ni], gamma = 5/3.
 
     rho = gamma^2, p = gamma, v = (-sin y, sin x, 0), B = (-sin y, sin 2x, 0);
     the reference ``initOT2D`` (prob.F90:74-119) with a unit wavenumber.
+    ``n`` is the cells per axis in x and y, or an (ni, nj, nk) tuple (then
+    ``nk`` is ignored).
     """
     g = 5.0 / 3.0
     L = 2.0 * math.pi
     m = ModelSpec(gamma=g, CFL=0.3, **mkw)
-    return CaseSpec(name=f"ot{n}x{n}x{nk}", N=(n, n, nk),
-                    bounds=(0.0, L, 0.0, L, 0.0, nk * L / n), grid="cart", ic="ot",
+    ni, nj, nk = (n, n, nk) if isinstance(n, int) else tuple(n)
+    return CaseSpec(name=f"ot{ni}x{nj}x{nk}", N=(ni, nj, nk),
+                    bounds=(0.0, L, 0.0, L, 0.0, nk * L / ni), grid="cart", ic="ot",
                     ic_params=dict(rho0=g * g, p0=g, b0=1.0, kscale=1.0),
                     model=m, steps=steps)
 

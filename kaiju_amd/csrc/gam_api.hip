@@ -765,6 +765,136 @@ extern "C" int gam_get_work(gam_handle *h, int which, double *out) {
     return gam_sync(h);
 }
 
+// ----------------------------------------------------------------- ensembles
+// Several handles stepped by one captured graph.  Each handle's step sequence
+// (advance + dt + BCs, all device-resident) is captured on the handle's own
+// stream, forked from and joined back to a capture stream, so the blocks are
+// independent branches of one graph.  nsteps is even: the State/oState buffer
+// pointers baked into the graph are then the same at every replay.
+struct gam_graph {
+    std::vector<gam_handle *> hs;
+    std::vector<int> ts0, s0;
+    int nsteps = 0;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> evs;
+    std::string err;
+};
+
+static int gfail(gam_graph *g, int code, const char *msg, hipError_t e) {
+    if (g) g->err = std::string(msg) + ": " + hipGetErrorString(e);
+    return code;
+}
+
+extern "C" int gam_graph_destroy(gam_graph *g) {
+    if (!g) return 0;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (auto e : g->evs) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return 0;
+}
+
+extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_graph **out) {
+    if (!hs || n < 1 || !out || nsteps < 2 || (nsteps & 1)) return -1;
+    *out = nullptr;
+    for (int q = 0; q < n; q++) {
+        if (!hs[q]) return -1;
+        for (int r = 0; r < q; r++) if (hs[r] == hs[q]) return -1;   // one branch per handle
+        int rc = gam_sync(hs[q]);
+        if (rc) return rc;
+        if (hs[q]->timing) return fail(hs[q], -1, "gam_graph_create: stage timers must be off");
+    }
+    gam_graph *g = new gam_graph();
+    g->hs.assign(hs, hs + n);
+    g->nsteps = nsteps;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking)) != hipSuccess) {
+        gam_graph_destroy(g);
+        return -2;
+    }
+    g->evs.resize(2 * n + 1, nullptr);
+    for (auto &ev : g->evs)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) { gam_graph_destroy(g); return -2; }
+    if ((e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+        gam_graph_destroy(g);
+        return -2;
+    }
+    int rc = 0;
+    if ((e = hipEventRecord(g->evs[0], g->stream)) != hipSuccess) rc = -2;
+    for (int q = 0; q < n && !rc; q++) {
+        gam_handle *h = hs[q];
+        if ((e = hipStreamWaitEvent(h->stream, g->evs[0], 0)) != hipSuccess) { rc = -2; break; }
+        for (int k = 0; k < nsteps && !rc; k++) {
+            rc = advance(h, 1);
+            if (!rc) launch_dtfinal(h, 1, 1, update_blocks(h));
+            if (!rc) rc = bcs(h);
+        }
+        if (!rc && (e = hipEventRecord(g->evs[1 + q], h->stream)) != hipSuccess) rc = -2;
+        if (!rc && (e = hipStreamWaitEvent(g->stream, g->evs[1 + q], 0)) != hipSuccess) rc = -2;
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ee = hipStreamEndCapture(g->stream, &graph);
+    g->graph = graph;
+    if (rc || ee != hipSuccess) {
+        if (!rc) e = ee;
+        gam_graph_destroy(g);
+        for (int q = 0; q < n; q++) (void)hipGetLastError();
+        return rc ? rc : -2;
+    }
+    if ((e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess) {
+        gam_graph_destroy(g);
+        return -2;
+    }
+    g->ts0.assign(n, 0);
+    g->s0.assign(n, 0);
+    *out = g;
+    return 0;
+}
+
+// Replays the graph; every handle's clock and buffers belong to the graph's
+// stream until gam_graph_sync returns.
+extern "C" int gam_graph_launch(gam_graph *g, int replays) {
+    if (!g || replays < 0) return -1;
+    for (size_t q = 0; q < g->hs.size(); q++) {
+        gam_clock c;
+        int rc = gam_get_clock(g->hs[q], &c);   // also waits for the handle's own stream
+        if (rc) return rc;
+        g->ts0[q] = c.ts;
+        g->s0[q] = g->hs[q]->s;
+    }
+    hipError_t e;
+    for (int r = 0; r < replays; r++)
+        if ((e = hipGraphLaunch(g->exec, g->stream)) != hipSuccess) return gfail(g, -2, "hipGraphLaunch", e);
+    return 0;
+}
+
+// Waits for the replays; returns the largest CalcDT status over the handles
+// (a halted handle skipped its remaining steps; its State is where it stopped).
+extern "C" int gam_graph_sync(gam_graph *g) {
+    if (!g) return -1;
+    hipError_t e;
+    if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return gfail(g, -2, "hipStreamSynchronize", e);
+    int worst = 0;
+    for (size_t q = 0; q < g->hs.size(); q++) {
+        gam_handle *h = g->hs[q];
+        gam_clock c;
+        int rc = gam_get_clock(h, &c);
+        if (rc) return rc;
+        const int done = c.ts - g->ts0[q];
+        h->s = (g->s0[q] + done) & 1 ? 1 - g->s0[q] : g->s0[q];
+        g->ts0[q] = c.ts;
+        g->s0[q] = h->s;
+        worst = std::max(worst, c.status);
+    }
+    return worst;
+}
+
+extern "C" const char *gam_graph_error(const gam_graph *g) { return g ? g->err.c_str() : "null graph"; }
+
 // ----------------------------------------------------------------- timers API
 extern "C" int gam_set_timing(gam_handle *h, int enable) {
     if (!h) return -1;

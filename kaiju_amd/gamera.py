@@ -97,6 +97,11 @@ def lib():
         "gam_stage_bytes": (C.c_int, [H, C.c_int, C.POINTER(C.c_double)]),
         "gam_set_tile": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "gam_get_tile": (C.c_int, [H, C.c_int, C.POINTER(C.c_int)]),
+        "gam_graph_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+        "gam_graph_launch": (C.c_int, [C.c_void_p, C.c_int]),
+        "gam_graph_sync": (C.c_int, [C.c_void_p]),
+        "gam_graph_destroy": (C.c_int, [C.c_void_p]),
+        "gam_graph_error": (C.c_char_p, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -13,6 +13,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <functional>
+#include <vector>
 
 using std::fmax;
 using std::fmin;
@@ -60,7 +62,7 @@ inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
 inline hipError_t hipMemset(void *p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
 inline hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t) { std::memset(p, v, n); return hipSuccess; }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
-inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t);
 inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (void *)0x1; return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
@@ -69,7 +71,39 @@ inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *t, hipEvent_t, hipEvent_t) { *t = 0.f; return hipSuccess; }
 
-template <class F> inline void emu_launch(dim3 g, dim3 b, F &&f) {
+// Stream capture: while a capture is open, launches and copies are recorded
+// (arguments by value) instead of run; a graph launch replays them in the
+// recorded order, one valid order of the captured dependencies.
+typedef std::vector<std::function<void()>> emu_graph;
+typedef emu_graph *hipGraph_t;
+typedef emu_graph *hipGraphExec_t;
+inline thread_local emu_graph *emu_capture = nullptr;
+#define hipStreamCaptureModeThreadLocal 1
+#define hipEventDisableTiming 2
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) {
+    if (emu_capture) return (hipError_t)1;
+    emu_capture = new emu_graph();
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+    *g = emu_capture;
+    emu_capture = nullptr;
+    return *g ? hipSuccess : (hipError_t)1;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, void *, size_t) {
+    *e = new emu_graph(*g);
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto &f : *e) f();
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)0x1; return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
+
+template <class F> inline void emu_launch_now(dim3 g, dim3 b, const F &f) {
     blockDim = {b.x, b.y, b.z};
     gridDim = {g.x, g.y, g.z};
     for (unsigned bz = 0; bz < g.z; bz++)
@@ -85,5 +119,19 @@ template <class F> inline void emu_launch(dim3 g, dim3 b, F &&f) {
             }
 }
 
+// The kernel arguments are evaluated once, at the launch, as HIP does; a
+// recorded launch keeps those values.
+template <class F, class... A> inline void emu_launch(dim3 g, dim3 b, F f, A... args) {
+    auto run = [g, b, f, args...]() { emu_launch_now(g, b, [&]() { f(args...); }); };
+    if (emu_capture) emu_capture->push_back(run);
+    else run();
+}
+
 #define HIP_KERNEL_NAME(...) __VA_ARGS__
-#define hipLaunchKernelGGL(K, G, B, SH, S, ...) emu_launch(dim3(G), dim3(B), [&]() { K(__VA_ARGS__); })
+#define hipLaunchKernelGGL(K, G, B, SH, S, ...) emu_launch(dim3(G), dim3(B), [](auto... a) { K(a...); }, __VA_ARGS__)
+
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) {
+    if (emu_capture) emu_capture->push_back([d, s, n]() { std::memmove(d, s, n); });
+    else std::memmove(d, s, n);
+    return hipSuccess;
+}

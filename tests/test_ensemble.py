@@ -1,0 +1,130 @@
+"""Batched ensemble under one captured graph, and the brick autotuner.
+
+CPU: the seeded member draw.  GPU (and CPU emulation): replaying the captured
+graph gives, at every replay, bit-for-bit the state, clock and status of the
+same blocks stepped eagerly with gam_step; a block that halts on a CalcDT
+status inside the graph stops exactly where gam_step stops; the autotuner
+restores the block it tuned and changes no result.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from kaiju_amd import autotune, configs
+from kaiju_amd.ensemble import Ensemble, draw_members
+from tests.helpers import say
+
+
+def test_members_are_seeded_and_in_range():
+    a = draw_members(32, 31337)
+    b = draw_members(32, 31337)
+    c = draw_members(32, 31338)
+    assert [s.name for s in a] == [s.name for s in b]
+    assert [s.N for s in a] != [s.N for s in c]
+    kinds = {s.ic for s in a}
+    assert kinds == {"ot", "bw"}
+    for s in a:
+        assert all(64 <= n <= 160 for n in s.N) and s.N[0] % 8 == 0
+        assert s.model.precision == 1
+        if s.ic == "bw":
+            assert 1.0 <= s.ic_params["B0"] <= 20.0
+            assert 10.0 <= s.ic_params["pRat"] <= 1000.0
+    zones = sum(s.zones for s in a)
+    assert 32 * 64 ** 3 < zones < 32 * 160 ** 3
+
+
+def small_members(precision):
+    """Four small uneven blocks (two of each kind), oracle-sized."""
+    out = []
+    for q, N in enumerate([(16, 20, 12), (24, 16, 18), (16, 18, 20), (8, 24, 1)]):
+        if q % 2 == 0:
+            s = configs.blast_wave(N, B0=3.0 + q, pin=20.0 + 10 * q, pout=0.1, precision=precision)
+        else:
+            s = configs.orszag_tang(N, precision=precision)
+        out.append(dataclasses.replace(s, name=f"s{q}_{s.name}"))
+    return out
+
+
+def eager_twins(specs):
+    from kaiju_amd.gamera import GamApp
+    apps = []
+    for s in specs:
+        a = GamApp(s)
+        a.InitModel()
+        apps.append(a)
+    return apps
+
+
+def same(a, b):
+    sa, sb = a.get_state(0), b.get_state(0)
+    for k in ("Gas", "magFlux", "Bxyz"):
+        assert np.array_equal(sa[k], sb[k]), k
+    ca, cb = a.get_clock(), b.get_clock()
+    assert (ca.t, ca.dt, ca.ts, ca.status, ca.dt0) == (cb.t, cb.dt, cb.ts, cb.status, cb.dt0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [1, 0])
+def test_graph_replays_match_eager_steps(precision):
+    specs = small_members(precision)
+    say(f"[ensemble] graph vs eager, precision {precision}, {[s.N for s in specs]}")
+    twins = eager_twins(specs)
+    with Ensemble(specs, tune=False, nsteps=2) as ens:
+        for rep in range(3):
+            ens.launch(1)
+            assert ens.sync() == 0
+            for t in twins:
+                assert t.steps(2) == 0
+            # eager work between replays: reads, a sync, a foreign launch
+            for a, t in zip(ens.apps, twins):
+                same(a, t)
+                a.divb()
+            say(f"   replay {rep} identical")
+    for t in twins:
+        t.close()
+
+
+@pytest.mark.gpu
+def test_graph_block_halts_where_gam_step_halts():
+    specs = small_members(1)[:2]
+    say("[ensemble] halt inside the graph")
+    twins = eager_twins(specs)
+    with Ensemble(specs, tune=False, nsteps=2) as ens:
+        # dt0 far above dt: CalcDT's dt/dt0 <= limDT0 check fails after step 1
+        for x in (ens.apps[0], twins[0]):
+            c = x.get_clock()
+            c.dt0 = c.dt * 1e6
+            x.set_clock(c)
+        st = ens.run(4)
+        assert st == 1
+        assert twins[0].steps(4) == 1 and twins[1].steps(4) == 0
+        assert ens.apps[0].Model.ts == 1 and ens.apps[1].Model.ts == 4
+        for a, t in zip(ens.apps, twins):
+            same(a, t)
+    for t in twins:
+        t.close()
+
+
+@pytest.mark.gpu
+def test_autotune_restores_state_and_changes_no_result():
+    spec = small_members(1)[0]
+    say(f"[autotune] {spec.N}")
+    a, b = eager_twins([spec, spec])
+    s_before = a.get_state(0)
+    cache = {}
+    best = autotune.tune(a, cache=cache)
+    s_after = a.get_state(0)
+    for k in ("Gas", "magFlux", "Bxyz"):
+        assert np.array_equal(s_before[k], s_after[k]), k
+    assert a.Model.ts == b.Model.ts == 0
+    for st, t in best.items():
+        assert a.get_tile(st) == t and t in autotune.CANDIDATES[st]
+    assert autotune.shape_key(a) in cache
+    assert a.steps(3) == 0 and b.steps(3) == 0
+    same(a, b)
+    # a cached entry is applied without timing
+    c = eager_twins([spec])[0]
+    assert autotune.tune(c, cache=cache) == best
+    for x in (a, b, c):
+        x.close()
