@@ -91,6 +91,23 @@ int  gam_get_grid(gam_handle *h, double *xyz, double *xyzcc, double *xfc, double
                   double *Tf, double *edge, double *Te, double *Teb, double *volume,
                   double *di, double *dj, double *dk);
 
+/* --- background field (Model%doBackground) ------------------------------- */
+/* The split B = B0 + dB of the reference (background.F90:20-167 AddB0 and its
+ * uses in stress.F90:185-188,302-305,550-596, fields.F90:417-420,
+ * gridutils.F90:82-86, mhdgroup.F90:180-183).  Arrays are Grid_T's
+ * (gamtypes.F90:231-238), G-sized: fcB0 G*3*3 (xyz, face dir), edgB0 G*2*3
+ * (T1/T2 slot, edge dir), B0 G*3, bFlux0 G*3, dpB0 G*3.  Setting a background
+ * turns the split on for every later step; State's magFlux and Bxyz then hold
+ * dB only.  gam_set_background uploads arrays the caller computed (AddB0 for
+ * any Model%B0); gam_background_dipole runs AddB0 on the GPU, on the current
+ * geometry, for B0(x) = bu + dipole(x - xc) with moment (0,0,M0)
+ * (dipolehelper.F90:19-30 MagsphereDipole, its rad**3/rad**5 as products). */
+int  gam_set_background(gam_handle *h, const double *fcB0, const double *edgB0, const double *B0,
+                        const double *bFlux0, const double *dpB0);
+int  gam_get_background(gam_handle *h, double *fcB0, double *edgB0, double *B0, double *bFlux0,
+                        double *dpB0);
+int  gam_background_dipole(gam_handle *h, double M0, const double *xc, const double *bu);
+
 /* --- state ---------------------------------------------------------------- */
 /* which: 0 = State, 1 = oState, 2 = predictor (half-step) state (read only).
  * Gas G*5, magFlux F*3, Bxyz G*3; NULL skips an array. */

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import Tuple
+from typing import Optional, Tuple
 
 # init.F90:32 MaxCFL = 0.3 is a default-real literal, widened to double.
 CFL_DEFAULT = 0.30000001192092896
@@ -53,6 +53,9 @@ class CaseSpec:
     nG: int = 4
     dt0_after: int = 1          # console output (dt0 capture) after this many steps
     steps: int = 100
+    # background field B0 = bu + dipole(x - xc) with moment (0,0,M0): a dict
+    # with keys M0, xc, bu (the reference's Model%doBackground + Model%B0)
+    background: Optional[dict] = None
 
     @property
     def zones(self) -> int:

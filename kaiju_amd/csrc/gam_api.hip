@@ -49,6 +49,8 @@ struct gam_handle {
     void *hGas = nullptr, *hmag = nullptr, *hB = nullptr;
     void *Vf = nullptr, *E = nullptr, *Flx = nullptr;
     void *wk[W_N] = {};        // predictor-pass stencil quantities
+    double *bg[5] = {};        // background field fcB0, edgB0, B0, bFlux0, dpB0 (G-sized)
+    BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
     double *partial = nullptr;
     Tile tile[ST_N] = {};       // launch tiles of the tiled stages
     int npart = 0;             // blocks of the 1-D CellDT launch
@@ -265,6 +267,7 @@ extern "C" int gam_destroy(gam_handle *h) {
     for (auto e : h->evpool) (void)hipEventDestroy(e);
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
     for (int q = 0; q < 9; q++) if (h->grid32[q]) (void)hipFree(h->grid32[q]);
+    for (int q = 0; q < 5; q++) if (h->bg[q]) (void)hipFree(h->bg[q]);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
     void *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, h->clk,
                       h->wk[0], h->wk[1], h->wk[2], h->wk[3], h->wk[4]};
@@ -514,6 +517,72 @@ extern "C" int gam_ic_blast_wave(gam_handle *h, double x0, double y0, double z0,
     return gam_sync(h);
 }
 
+// ----------------------------------------------------------------- background field
+// Grid_T fcB0/edgB0/B0/bFlux0/dpB0 (gamtypes.F90:231-238), G-sized, FP64.
+static const int kBgComp[5] = {9, 6, 3, 3, 3};
+
+static int bg_alloc(gam_handle *h) {
+    const size_t G = gsz(h);
+    for (int q = 0; q < 5; q++)
+        if (!h->bg[q]) HIPCHK(h, hipMalloc(&h->bg[q], kBgComp[q] * G * sizeof(double)));
+    h->bgv.fcB0 = h->bg[0]; h->bgv.edgB0 = h->bg[1]; h->bgv.B0 = h->bg[2];
+    h->bgv.bFlux0 = h->bg[3]; h->bgv.dpB0 = h->bg[4];
+    return 0;
+}
+
+extern "C" int gam_set_background(gam_handle *h, const double *fcB0, const double *edgB0, const double *B0,
+                                  const double *bFlux0, const double *dpB0) {
+    if (!h || !fcB0 || !edgB0 || !B0 || !bFlux0 || !dpB0) return -1;
+    int rc = bg_alloc(h);
+    if (rc) return rc;
+    const double *src[5] = {fcB0, edgB0, B0, bFlux0, dpB0};
+    for (int q = 0; q < 5; q++)
+        if ((rc = h2d(h, h->bg[q], src[q], kBgComp[q] * gsz(h)))) return rc;
+    if ((rc = gam_sync(h))) return rc;
+    h->bgv.on = 1;
+    return 0;
+}
+
+extern "C" int gam_get_background(gam_handle *h, double *fcB0, double *edgB0, double *B0, double *bFlux0,
+                                  double *dpB0) {
+    if (!h) return -1;
+    if (!h->bgv.on) return fail(h, -1, "gam_get_background: no background field set");
+    double *dst[5] = {fcB0, edgB0, B0, bFlux0, dpB0};
+    int rc;
+    for (int q = 0; q < 5; q++)
+        if ((rc = d2h(h, dst[q], h->bg[q], kBgComp[q] * gsz(h)))) return rc;
+    return gam_sync(h);
+}
+
+extern "C" int gam_background_dipole(gam_handle *h, double M0, const double *xc, const double *bu) {
+    // background.F90:20-167 AddB0 on the GPU for B0 = bu + dipole(x - xc)
+    if (!h || !xc || !bu) return -1;
+    int rc = bg_alloc(h);
+    if (rc) return rc;
+    const GV &g = h->gv;
+    const size_t G = gsz(h);
+    double *fS = nullptr;
+    HIPCHK(h, hipMalloc(&fS, 9 * G * sizeof(double)));
+    B0P p = {M0, {xc[0], xc[1], xc[2]}, {bu[0], bu[1], bu[2]}};
+    GVw w = gvw(h);
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < 5 && e == hipSuccess; q++) e = hipMemsetAsync(h->bg[q], 0, kBgComp[q] * G * sizeof(double), h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(fS, 0, 9 * G * sizeof(double), h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_b0_faces, grid1(g.nc), 256, 0, h->stream, w, p, h->bg[0], h->bg[3], fS);
+        hipLaunchKernelGGL(k_b0_edges, grid1(g.nc), 256, 0, h->stream, w, p, h->bg[1]);
+        hipLaunchKernelGGL(k_b0_cells, grid1(g.nc), 256, 0, h->stream, w, p, h->bg[2]);
+        hipLaunchKernelGGL(k_b0_dp, grid1((long)g.Nip * g.Njp * g.Nkp), 256, 0, h->stream, w, (const double *)fS, h->bg[4]);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(h->stream);
+    (void)hipFree(fS);
+    if (e != hipSuccess) return fail(h, -2, "gam_background_dipole: %s", hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(h, -2, "gam_background_dipole: %s", hipGetErrorString(e2));
+    h->bgv.on = 1;
+    return 0;
+}
+
 // ----------------------------------------------------------------- step pieces
 // CellDT min over the DT box of (Gas, B) into h->partial (CalcDT's loop).
 template <typename R>
@@ -521,8 +590,8 @@ static int launch_celldt(gam_handle *h, const void *Gas, const void *B) {
     if (!Gas || !B) return fail(h, -1, "launch_celldt: null state");
     StageTimer t(h, ST_CELLDT);
     const GVT<R> &g = view<R>(h);
-    if (h->mp.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<1, R>), dim3(h->npart), 256, 0, h->stream, g, h->mp, h->clk, (const R *)Gas, (const R *)B, h->partial);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<0, R>), dim3(h->npart), 256, 0, h->stream, g, h->mp, h->clk, (const R *)Gas, (const R *)B, h->partial);
+    if (h->mp.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<1, R>), dim3(h->npart), 256, 0, h->stream, g, h->mp, h->bgv, h->clk, (const R *)Gas, (const R *)B, h->partial);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt<0, R>), dim3(h->npart), 256, 0, h->stream, g, h->mp, h->bgv, h->clk, (const R *)Gas, (const R *)B, h->partial);
     HIPCHK(h, hipGetLastError());
     return 0;
 }
@@ -540,8 +609,8 @@ static void launch_dtfinal(gam_handle *h, int advance, int calc, int np) {
     hipLaunchKernelGGL(k_dtfinal, 1, 1024, 0, h->stream, h->clk, h->partial, np, advance, calc);
 }
 
-template <int RM, int BO, int MH, typename R>
-static void launch_flux(gam_handle *h, int d) {
+template <int RM, int BO, int MH, int BG, typename R>
+static void launch_flux_bg(gam_handle *h, int d) {
     const GVT<R> &g = view<R>(h);
     const int S = h->s;
     long nx = g.Nip + (d == 0), ny = g.Njp + (d == 1), nz = (h->mp.do25D ? 1 : g.Nkp) + (d == 2);
@@ -551,9 +620,15 @@ static void launch_flux(gam_handle *h, int d) {
     const R *nB = P<R>(h->B[S]), *hB = P<R>(h->hB), *hmag = P<R>(h->hmag), *hGas = P<R>(h->hGas);
     R *Flx = P<R>(h->Flx);
     StageTimer t(h, ST_FLUX_I + d);
-    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH, BG, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH, BG, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH, BG, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+}
+
+template <int RM, int BO, int MH, typename R>
+static void launch_flux(gam_handle *h, int d) {
+    if (MH && h->bgv.on) launch_flux_bg<RM, BO, MH, MH, R>(h, d);   // background terms only with MHD
+    else launch_flux_bg<RM, BO, MH, 0, R>(h, d);
 }
 
 template <int ED, int RM, int BO, typename R>
@@ -563,7 +638,7 @@ static void launch_emf(gam_handle *h) {
     const WKT<R> w = wkt<R>(h);
     StageTimer t(h, st);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<ED, RM, BO, R>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
-                       256, 0, h->stream, g, h->tile[st], h->mp, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
+                       256, 0, h->stream, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
 }
 
 template <int RM, int BO, typename R>
@@ -625,10 +700,10 @@ static int advance_t(gam_handle *h, int fuse_dt) {
         const R *nGas = P<R>(h->Gas[S]), *nB = P<R>(h->B[S]), *Flx = P<R>(h->Flx);
         R *oGas = P<R>(h->Gas[O]), *oB = P<R>(h->B[O]);
         if (m.doMHD) {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
         } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
         }
     }
     // CopyState State->oState is the buffer swap
@@ -971,12 +1046,13 @@ extern "C" int gam_stage_bytes(gam_handle *h, int stage, double *bytes) {
         const double nEd = (double)(g.Nip + (e != 0)) * (g.Njp + (e != 1)) * (g.Nkp + (e != 2));
         b = G * (3 + 1) + nEd * (2 + 2 + 2 + 6 + 4 + 1 + 1); break;
     }
-    case ST_FLUX_I: b = (G * (5 + 5 + 3 + 3 + 1)) + nI * (9 + 1 + 1 + 8); break;
-    case ST_FLUX_J: b = (G * (5 + 5 + 3 + 3 + 1)) + nJ * (9 + 1 + 1 + 8); break;
-    case ST_FLUX_K: b = (G * (5 + 5 + 3 + 3 + 1)) + nK * (9 + 1 + 1 + 8); break;
+    case ST_FLUX_I: b = (G * (5 + 5 + 3 + 3 + 1)) + nI * (9 + 1 + 1 + 8); d = h->bgv.on ? nI * 4 : 0; break;
+    case ST_FLUX_J: b = (G * (5 + 5 + 3 + 3 + 1)) + nJ * (9 + 1 + 1 + 8); d = h->bgv.on ? nJ * 4 : 0; break;
+    case ST_FLUX_K: b = (G * (5 + 5 + 3 + 3 + 1)) + nK * (9 + 1 + 1 + 8); d = h->bgv.on ? nK * 4 : 0; break;
     case ST_CT: d = F * (3 + 3); b = G * 3; break;
     case ST_UPDATE:   // Gas n, fluxes (8 per direction), di/dj/dk -> Gas, Bxyz (b) ; magFlux new, xfc, xyzcc, vol (d)
-        b = G * (5 + 24 + 3 + 5 + 3 + (h->mp.doBoris ? 3 : 0)); d = F * (3 + 9) + G * (3 + 1); break;
+        b = G * (5 + 24 + 3 + 5 + 3 + (h->mp.doBoris ? 3 : 0)); d = F * (3 + 9) + G * (3 + 1);
+        if (h->bgv.on) d += G * (3 + 3); break;                                  // dpB0, B0
     case ST_BC: b = 2.0 * (G - nA) * 5; d = 2.0 * (F - nE) * 3; break;
     case ST_CELLDT: b = G * (5 + 3 + 3); break;
     default: b = 0;

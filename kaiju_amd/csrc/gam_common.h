@@ -61,6 +61,18 @@ template <typename R> struct GVT {
 };
 typedef GVT<double> GV;
 
+// Background field B0 (reference Grid_T fcB0/edgB0/B0/bFlux0/dpB0,
+// gamtypes.F90:231-238), G-sized FP64 arrays; on == 0: no background.
+//   fcB0  [nc*(xyz + 3*faceDir)]  face-averaged B0 of the face at the lower corner
+//   edgB0 [nc*(slot + 2*edgeDir)] edge-averaged B0 on the edge's (T1, T2)
+//   B0    [nc*xyz]                cell-averaged B0
+//   bFlux0[nc*faceDir]            B0 flux through the face
+//   dpB0  [nc*xyz]                -div(B0 Maxwell stress)/volume
+struct BGV {
+    const double *fcB0, *edgB0, *B0, *bFlux0, *dpB0;
+    int on;
+};
+
 template <typename R> __host__ __device__ inline int ci(const GVT<R> &g, int i, int j, int k) { return i + g.Ni * (j + g.Nj * k); }
 template <typename R> __host__ __device__ inline int fi(const GVT<R> &g, int i, int j, int k) { return i + (g.Ni + 1) * (j + (g.Nj + 1) * k); }
 template <typename R> __host__ __device__ inline int cstride(const GVT<R> &g, int d) { return d == 0 ? 1 : (d == 1 ? g.Ni : g.Ni * g.Nj); }

@@ -119,8 +119,9 @@ __global__ void __launch_bounds__(256) k_faces(GVw g, int d) {
 // 0 = CellDV (metric.F90:298-307), 1 = blast-wave gas (prob.F90:280-297)
 struct BWp { double x0, y0, z0, D0, P0, pRat, dRat, rC; };
 
-template <int W>
-__device__ void gauss_volume(const double C[8][3], const BWp &bw, double *out5) {
+// quadrature.F90:42-100 GaussianVolumeIntegral of the integrand fn(x, y, z, W5)
+template <class Fn>
+__device__ void gauss_volume_fn(const double C[8][3], const Fn &fn, double *out5) {
     const int iS[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
     const int jS[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
     const int kS[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
@@ -149,16 +150,25 @@ __device__ void gauss_volume(const double C[8][3], const BWp &bw, double *out5) 
                                - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
                                + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]));
                 double Wp[5];
-                if (W == 0) { Wp[0] = 1; Wp[1] = xp[0]; Wp[2] = xp[1]; Wp[3] = xp[2]; Wp[4] = 0; }
-                else {
-                    double R = sqrt(sq(xp[0] - bw.x0) + sq(xp[1] - bw.y0) + sq(xp[2] - bw.z0));
-                    double D = bw.D0, P = bw.P0;
-                    if (R <= bw.rC) { D = bw.dRat * D; P = bw.pRat * P; }
-                    Wp[0] = D; Wp[1] = 0.0; Wp[2] = 0.0; Wp[3] = 0.0; Wp[4] = P;
-                }
+                fn(xp[0], xp[1], xp[2], Wp);
 #pragma unroll
                 for (int v = 0; v < 5; v++) out5[v] = out5[v] + dJ * wijk * Wp[v];
             }
+}
+
+// W selects the integrand: 0 = CellDV (metric.F90:298-307), 1 = blast-wave gas
+// (prob.F90:280-297)
+template <int W>
+__device__ void gauss_volume(const double C[8][3], const BWp &bw, double *out5) {
+    gauss_volume_fn(C, [&](double x, double y, double z, double *Wp) {
+        if (W == 0) { Wp[0] = 1; Wp[1] = x; Wp[2] = y; Wp[3] = z; Wp[4] = 0; }
+        else {
+            double R = sqrt(sq(x - bw.x0) + sq(y - bw.y0) + sq(z - bw.z0));
+            double D = bw.D0, P = bw.P0;
+            if (R <= bw.rC) { D = bw.dRat * D; P = bw.pRat * P; }
+            Wp[0] = D; Wp[1] = 0.0; Wp[2] = 0.0; Wp[3] = 0.0; Wp[4] = P;
+        }
+    }, out5);
 }
 
 __device__ __forceinline__ void cell_coords(const GVw &g, int i, int j, int k, double C[8][3]) {   // metric.F90:133-148
@@ -346,4 +356,125 @@ __global__ void __launch_bounds__(256) k_ic_bw_flux(GVw g, const double *lA, dou
     mag[f] = LJ[c] - LJ[ciw(g, i, j, k + 1)] + LK[ciw(g, i, j + 1, k)] - LK[c];
     mag[nf + f] = -(LI[c] - LI[ciw(g, i, j, k + 1)] + LK[ciw(g, i + 1, j, k)] - LK[c]);
     mag[2 * nf + f] = LI[c] - LI[ciw(g, i, j + 1, k)] + LJ[ciw(g, i + 1, j, k)] - LJ[c];
+}
+
+// ----------------------------------------------------------------- background field
+// background.F90:20-167 AddB0 for B0(x) = bu + dipole(x - xc), moment (0,0,M0)
+// (dipolehelper.F90:19-30 MagsphereDipole; rad**5 and rad**3 as products).
+struct B0P { double M0, xc[3], bu[3]; };
+
+__device__ __forceinline__ void b0_eval(const B0P &p, double x, double y, double z, double *B) {
+    double r[3] = {x - p.xc[0], y - p.xc[1], z - p.xc[2]};
+    B[0] = p.bu[0]; B[1] = p.bu[1]; B[2] = p.bu[2];
+    if (p.M0 != 0.0) {
+        double rad = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        double r3 = rad * rad * rad, r5 = r3 * rad * rad;
+        double mdr = p.M0 * r[2];
+        double m3[3] = {0.0, 0.0, p.M0};
+        for (int x = 0; x < 3; x++) B[x] = B[x] + (3 * mdr * r[x] / r5 - m3[x] / r3);
+    }
+}
+
+// Per cell of the ghost-inclusive box, the three faces at its lower corner:
+// B0 flux (GaussianFaceFlux), face-averaged B0 (GaussianFaceIntegral / Face)
+// and the face Maxwell stress of B0 (GaussianFaceStress, into fS), one sweep
+// of the 12x12 Gauss points (quadrature.F90:105-280).
+__global__ void __launch_bounds__(256) k_b0_faces(GVw g, B0P p, double *fcB0, double *bFlux0, double *fS) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc) return;
+    int i = idx % g.Ni, r = idx / g.Ni, j = r % g.Nj, k = r / g.Nj;
+    const size_t nc = g.nc, nf = g.nf;
+    const int f = fiw(g, i, j, k);
+    for (int d = 0; d < 3; d++) {
+        double f0[3], f1[3], f2[3], f3[3];   // metric.F90:79-108 faceCoords
+        xyzv(g, i, j, k, f0);
+        if (d == 0) { xyzv(g, i, j + 1, k, f1); xyzv(g, i, j, k + 1, f2); xyzv(g, i, j + 1, k + 1, f3); }
+        else if (d == 1) { xyzv(g, i, j, k + 1, f1); xyzv(g, i + 1, j, k, f2); xyzv(g, i + 1, j, k + 1, f3); }
+        else { xyzv(g, i + 1, j, k, f1); xyzv(g, i, j + 1, k, f2); xyzv(g, i + 1, j + 1, k, f3); }
+        double dx[3], dy[3], dz[3];
+        dx[0] = f1[0] - f0[0]; dx[1] = f2[0] - f0[0]; dx[2] = f3[0] + f0[0] - f2[0] - f1[0];
+        dy[0] = f1[1] - f0[1]; dy[1] = f2[1] - f0[1]; dy[2] = f3[1] + f0[1] - f2[1] - f1[1];
+        dz[0] = f1[2] - f0[2]; dz[1] = f2[2] - f0[2]; dz[2] = f3[2] + f0[2] - f2[2] - f1[2];
+        double flx = 0.0, fInt[3] = {0, 0, 0}, Mbb[3] = {0, 0, 0};
+        for (int a = 0; a < 12; a++)
+            for (int b = 0; b < 12; b++) {
+                double eta = 0.5 * (1 + gA[a]), psi = 0.5 * (1 + gA[b]);
+                double w = gW[a] * gW[b];
+                double xp = f0[0] + dx[0] * eta + dx[1] * psi + dx[2] * eta * psi;
+                double yp = f0[1] + dy[0] * eta + dy[1] * psi + dy[2] * eta * psi;
+                double zp = f0[2] + dz[0] * eta + dz[1] * psi + dz[2] * eta * psi;
+                double A[3];
+                b0_eval(p, xp, yp, zp, A);
+                double eV[3] = {dx[0] + dx[2] * psi, dy[0] + dy[2] * psi, dz[0] + dz[2] * psi};
+                double pV[3] = {dx[1] + dx[2] * eta, dy[1] + dy[2] * eta, dz[1] + dz[2] * eta};
+                double cr[3];
+                cross3(eV, pV, cr);
+                double N[3] = {0.25 * w * cr[0], 0.25 * w * cr[1], 0.25 * w * cr[2]};
+                flx = flx + (N[0] * A[0] + N[1] * A[1] + N[2] * A[2]);
+                double area = 0.25 * nrm3(cr) * w;
+                for (int x = 0; x < 3; x++) fInt[x] = fInt[x] + area * A[x];
+                double BB = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+                double BdS = A[0] * N[0] + A[1] * N[1] + A[2] * N[2];
+                for (int x = 0; x < 3; x++) Mbb[x] = Mbb[x] + 0.5 * BB * N[x] - A[x] * BdS;
+            }
+        bFlux0[nc * d + idx] = flx;
+        for (int x = 0; x < 3; x++) fcB0[nc * (x + 3 * d) + idx] = fInt[x] / g.face[nf * d + f];
+        for (int x = 0; x < 3; x++) fS[nc * (d + 3 * x) + idx] = Mbb[x];
+    }
+}
+
+// Per cell, the three edges at its lower corner: the mean of B0 along the edge
+// (GaussianEdgeIntegral, quadrature.F90:345-370) on the edge's (T1, T2)
+__global__ void __launch_bounds__(256) k_b0_edges(GVw g, B0P p, double *edgB0) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc) return;
+    int i = idx % g.Ni, r = idx / g.Ni, j = r % g.Nj, k = r / g.Nj;
+    const size_t nc = g.nc, nf = g.nf;
+    const int f = fiw(g, i, j, k);
+    for (int d = 0; d < 3; d++) {
+        double e1[3], e2[3];   // metric.F90:111-130 edgeCoords
+        xyzv(g, i, j, k, e1);
+        if (d == 0) xyzv(g, i + 1, j, k, e2);
+        else if (d == 1) xyzv(g, i, j + 1, k, e2);
+        else xyzv(g, i, j, k + 1, e2);
+        double dR2[3], xBar[3], eInt[3];
+        for (int x = 0; x < 3; x++) { dR2[x] = 0.5 * (e2[x] - e1[x]); xBar[x] = 0.5 * (e2[x] + e1[x]); eInt[x] = 0.0; }
+        for (int a = 0; a < 12; a++) {
+            double A[3];
+            b0_eval(p, xBar[0] + gA[a] * dR2[0], xBar[1] + gA[a] * dR2[1], xBar[2] + gA[a] * dR2[2], A);
+            for (int x = 0; x < 3; x++) eInt[x] = eInt[x] + 0.5 * gW[a] * A[x];
+        }
+        const double *Te = g.Te + nf * 9 * d;
+        edgB0[nc * (0 + 2 * d) + idx] = eInt[0] * Te[nf * TAN1X + f] + eInt[1] * Te[nf * TAN1Y + f] + eInt[2] * Te[nf * TAN1Z + f];
+        edgB0[nc * (1 + 2 * d) + idx] = eInt[0] * Te[nf * TAN2X + f] + eInt[1] * Te[nf * TAN2Y + f] + eInt[2] * Te[nf * TAN2Z + f];
+    }
+}
+
+// Cell-averaged B0 (GaussianVolumeIntegral / volume)
+__global__ void __launch_bounds__(256) k_b0_cells(GVw g, B0P p, double *B0) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc) return;
+    int i = idx % g.Ni, r = idx / g.Ni, j = r % g.Nj, k = r / g.Nj;
+    double C[8][3], W[5];
+    cell_coords(g, i, j, k, C);
+    gauss_volume_fn(C, [&](double x, double y, double z, double *Wp) {   // background.F90:157-165 B0Wrap
+        double B[3];
+        b0_eval(p, x, y, z, B);
+        Wp[0] = B[0]; Wp[1] = B[1]; Wp[2] = B[2]; Wp[3] = 0.0; Wp[4] = 0.0;
+    }, W);
+    for (int x = 0; x < 3; x++) B0[(size_t)g.nc * x + idx] = W[x] / g.vol[idx];
+}
+
+// background.F90:137-150: -div(face stress)/volume on the active cells
+__global__ void __launch_bounds__(256) k_b0_dp(GVw g, const double *fS, double *dpB0) {
+    const int nx = g.Nip, ny = g.Njp, nz = g.Nkp;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * ny * nz) return;
+    int i = g.nG + idx % nx, r = idx / nx, j = g.nG + r % ny, k = g.nG + r / ny;
+    const size_t nc = g.nc;
+    const int c = ciw(g, i, j, k), ci1 = ciw(g, i + 1, j, k), cj1 = ciw(g, i, j + 1, k), ck1 = ciw(g, i, j, k + 1);
+    for (int x = 0; x < 3; x++)
+        dpB0[nc * x + c] = -(fS[nc * (0 + 3 * x) + ci1] - fS[nc * (0 + 3 * x) + c]
+                           + fS[nc * (1 + 3 * x) + cj1] - fS[nc * (1 + 3 * x) + c]
+                           + fS[nc * (2 + 3 * x) + ck1] - fS[nc * (2 + 3 * x) + c]) / g.vol[c];
 }

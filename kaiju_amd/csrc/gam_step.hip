@@ -213,7 +213,7 @@ __device__ __forceinline__ void edge_lrs(const GVT<R> &g, const MP &m, const R *
 
 // One edge EMF (fields.F90:196-221) for edge direction eD at corner (i,j,k)
 template <int RM, int BORIS, typename R>
-__device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, double dt64, const R *__restrict__ Vf,
+__device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, const BGV &bg, double dt64, const R *__restrict__ Vf,
                                       const R *__restrict__ hGas, const R *__restrict__ bn,
                                       const R *__restrict__ bh, int eD, int i, int j, int k) {
     const R dt = R(dt64);
@@ -257,6 +257,10 @@ __device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, double dt64,
     R Jd = db2 - db1;
     R b1 = detT * (ynj * bT1 - yni * bT2);
     R b2 = detT * (-xnj * bT1 + xni * bT2);
+    if (bg.on) {   // fields.F90:417-420
+        b1 = b1 + R(bg.edgB0[nc * (0 + 2 * eD) + c]);
+        b2 = b2 + R(bg.edgB0[nc * (1 + 2 * eD) + c]);
+    }
     R vA = rroot(sq(b1) + sq(b2)) / rroot(Dc);
     const R Ca = R(m.Ca);
     if (BORIS) vA = Ca * vA / rroot(sq(Ca) + sq(vA));
@@ -274,25 +278,31 @@ __device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, double dt64,
 // (CalcElecField zeroes E, fields.F90:89).  One kernel per direction, so each
 // gets a launch brick shaped for its two stencil axes (dT1 and dT2).
 template <int ED, int RM, int BORIS, typename R>
-__global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, const DClock *__restrict__ clk, const R *__restrict__ Vf,
+__global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk, const R *__restrict__ Vf,
                                              const R *__restrict__ hGas, const R *__restrict__ bn,
                                              const R *__restrict__ bh, R *__restrict__ E) {
     int a, b, cz;
     if (!tile_pos(t, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2), a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
-    E[(size_t)g.nc * ED + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, clk->dt, Vf, hGas, bn, bh, ED, i, j, k);
+    E[(size_t)g.nc * ED + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, ED, i, j, k);
 }
 
 // ----------------------------------------------------------------- fluxes
 // stress.F90:513-599 MagKinFlux (no background field), q = -1/+1
 template <int BORIS, typename R>
 __device__ __forceinline__ void magkin(const MP &m, R sg, const R *P5, const R *B, R Bn,
-                                       const R *T, R *mF, R &bbD, R &VaD) {
+                                       const R *T, bool bgon, const R *B0, R B0n, R *mF, R &bbD, R &VaD) {
     R D = P5[0], P = P5[4];
     R Bx = B[0], By = B[1], Bz = B[2];
     R dPb = R(0.5) * (sq(Bx) + sq(By) + sq(Bz));
-    R Va2 = 2 * dPb / D;
-    bbD = bbD + R(0.5) * (sq(Bx) + sq(By) + sq(Bz));
+    R Va2;
+    if (bgon) {   // stress.F90:550-552 total field
+        Va2 = (sq(Bx + B0[0]) + sq(By + B0[1]) + sq(Bz + B0[2])) / D;
+        bbD = bbD + R(0.5) * (sq(Bx + B0[0]) + sq(By + B0[1]) + sq(Bz + B0[2]));
+    } else {
+        Va2 = 2 * dPb / D;
+        bbD = bbD + R(0.5) * (sq(Bx) + sq(By) + sq(Bz));
+    }
     R Vac2 = BORIS ? Va2 / (1 + (Va2 / sq(R(m.Ca)))) : Va2;
     R Nx = T[NORMX], Ny = T[NORMY], Nz = T[NORMZ];
     R Vn = Nx * P5[1] + Ny * P5[2] + Nz * P5[3];
@@ -302,6 +312,12 @@ __device__ __forceinline__ void magkin(const MP &m, R sg, const R *P5, const R *
     mF[0] = mF[0] + Vn0 * (dPb * Nx - Bx * Bn);
     mF[1] = mF[1] + Vn0 * (dPb * Ny - By * Bn);
     mF[2] = mF[2] + Vn0 * (dPb * Nz - Bz * Bn);
+    if (bgon) {   // stress.F90:589-596 cross terms
+        R BdB0 = B0[0] * Bx + B0[1] * By + B0[2] * Bz;
+        mF[0] = mF[0] + Vn0 * (-B0[0] * Bn - Bx * B0n + BdB0 * Nx);
+        mF[1] = mF[1] + Vn0 * (-B0[1] * Bn - By * B0n + BdB0 * Ny);
+        mF[2] = mF[2] + Vn0 * (-B0[2] * Bn - Bz * B0n + BdB0 * Nz);
+    }
 }
 
 // stress.F90:432-511 GasKinFlux
@@ -332,9 +348,11 @@ __device__ __forceinline__ void gaskin(const MP &m, R sg, const R *P5, const R *
 
 // Interface fluxes in direction DN (stress.F90:204-429) for the interface box
 // DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
-// Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
-template <int DN, int RM, int BORIS, int MHD, typename R>
-__global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, const DClock *__restrict__ clk,
+// Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.  BG: the
+// background-field terms are compiled in (a separate instantiation keeps the
+// plain kernel's register count).
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+__global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                               const R *__restrict__ Wn, const R *__restrict__ nB,
                                               const R *__restrict__ Wh, const R *__restrict__ hB,
                                               const R *__restrict__ hmag, const R *__restrict__ hGas,
@@ -372,6 +390,13 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, const DClo
         }
         Bn = hmag[nf * DN + f] / faB;
     }
+    R B0f[3] = {0, 0, 0}, B0n = R(0.0);
+    const bool bgon = MHD && BG;
+    if (bgon) {   // stress.F90:302-305
+#pragma unroll
+        for (int x = 0; x < 3; x++) B0f[x] = R(bg.fcB0[nc * (x + 3 * DN) + c]);
+        B0n = R(bg.bFlux0[nc * DN + c]) / faB;
+    }
 #pragma unroll
     for (int v = 0; v < 5; v++) {   // BlockStateLRs on the prepass primitives
         R Q[8], nQ[4];
@@ -385,8 +410,8 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, const DClo
     R VaD = R(0.0);
     if (MHD) {
         R mF[3] = {0, 0, 0}, bbD = R(0.0);
-        magkin<BORIS>(m, R(-1.0), PL, MagL, Bn, T, mF, bbD, VaD);
-        magkin<BORIS>(m, R(1.0), PR, MagR, Bn, T, mF, bbD, VaD);
+        magkin<BORIS>(m, R(-1.0), PL, MagL, Bn, T, bgon, B0f, B0n, mF, bbD, VaD);
+        magkin<BORIS>(m, R(1.0), PR, MagR, Bn, T, bgon, B0f, B0n, mF, bbD, VaD);
         if (BORIS) {   // stress.F90:335-351
 #pragma unroll
             for (int d = 0; d < 3; d++) {
@@ -573,7 +598,7 @@ template <typename R> __device__ __forceinline__ bool in_dt_box(const GVT<R> &g,
 // + CellDT min, in FP64 from the bulk-typed inputs.  Writes the new Gas/Bxyz
 // into the O buffers; ghosts copy Gas n.
 template <int BORIS, int MHD, typename R>
-__global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, const DClock *__restrict__ clk,
+__global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                 const R *__restrict__ nGas, const R *__restrict__ nB,
                                                 const R *__restrict__ Flx, const double *__restrict__ magNew,
                                                 R *__restrict__ oGas, R *__restrict__ oB,
@@ -606,6 +631,7 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, const DC
                 for (int v = 0; v < 3; v++) {
                     dGM[v] = (F(5 + v, idx) - F(5 + v, ci1) + F(13 + v, idx) - F(13 + v, cj1)
                             + F(21 + v, idx) - F(21 + v, ck1)) / dV;
+                    if (bg.on) dGM[v] = dGM[v] + bg.dpB0[nc * v + idx];   // stress.F90:185-188
                 }
             }
             double U0[5];
@@ -614,7 +640,12 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, const DC
             if (MHD) {
                 if (BORIS) {
                     double oBv[3] = {(double)nB[idx], (double)nB[nc + idx], (double)nB[2 * nc + idx]};
-                    cell_boris(m, U, U0, Bn, oBv, dGH, dGM, m.Ca, dt);
+                    double Bt[3] = {Bn[0], Bn[1], Bn[2]};
+                    if (bg.on) {   // mhdgroup.F90:180-183 total fields
+#pragma unroll
+                        for (int x = 0; x < 3; x++) { Bt[x] = Bt[x] + bg.B0[nc * x + idx]; oBv[x] = oBv[x] + bg.B0[nc * x + idx]; }
+                    }
+                    cell_boris(m, U, U0, Bt, oBv, dGH, dGM, m.Ca, dt);
                 } else {
                     cell_maxwell(m, U, dGM, dt);
                 }
@@ -631,6 +662,10 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, const DC
 #pragma unroll
                 for (int x = 0; x < 3; x++) Bn[x] = (double)R(Bn[x]);
             }
+            if (bg.on) {   // gridutils.F90:82-86
+#pragma unroll
+                for (int x = 0; x < 3; x++) Bn[x] = Bn[x] + bg.B0[nc * x + idx];
+            }
             dtl = cell_dt<BORIS>(g, m, U, Bn, idx);
         }
     }
@@ -639,7 +674,7 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, const DC
 
 // Stand-alone CellDT min over the DT box (CalcDT without the fused update)
 template <int BORIS, typename R>
-__global__ void __launch_bounds__(256) k_celldt(GVT<R> g, MP m, const DClock *__restrict__ clk, const R *__restrict__ Gas,
+__global__ void __launch_bounds__(256) k_celldt(GVT<R> g, MP m, BGV bg, const DClock *__restrict__ clk, const R *__restrict__ Gas,
                                                 const R *__restrict__ B, double *__restrict__ partial) {
     if (clk->halt) return;
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -654,6 +689,10 @@ __global__ void __launch_bounds__(256) k_celldt(GVT<R> g, MP m, const DClock *__
             for (int v = 0; v < 5; v++) U[v] = (double)Gas[nc * v + idx];
 #pragma unroll
             for (int x = 0; x < 3; x++) Bv[x] = (double)B[nc * x + idx];
+            if (bg.on) {   // gridutils.F90:82-86
+#pragma unroll
+                for (int x = 0; x < 3; x++) Bv[x] = Bv[x] + bg.B0[nc * x + idx];
+            }
             dtl = cell_dt<BORIS>(g, m, U, Bv, idx);
         }
     }

@@ -105,6 +105,22 @@ module gamera_hip_c
             real(c_double), value :: x0, y0, z0, D0, P0, pRat, dRat, rC, B0
         end function gam_ic_blast_wave
 
+        !> background field: Grid_T%fcB0/edgB0/B0/bFlux0/dpB0 (gamtypes.F90:231-238)
+        integer(c_int) function gam_set_background(h, fcB0, edgB0, B0, bFlux0, dpB0) &
+                bind(C, name="gam_set_background")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: fcB0(*), edgB0(*), B0(*), bFlux0(*), dpB0(*)
+        end function gam_set_background
+
+        !> AddB0 (background.F90:20-167) on the GPU for B0 = bu + dipole(x - xc)
+        integer(c_int) function gam_background_dipole(h, M0, xc, bu) bind(C, name="gam_background_dipole")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: M0
+            real(c_double), intent(in) :: xc(3), bu(3)
+        end function gam_background_dipole
+
         integer(c_int) function gam_hatch(h) bind(C, name="gam_hatch")
             import :: c_int, c_ptr
             type(c_ptr), value :: h

@@ -97,6 +97,9 @@ def lib():
         "gam_stage_bytes": (C.c_int, [H, C.c_int, C.POINTER(C.c_double)]),
         "gam_set_tile": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "gam_get_tile": (C.c_int, [H, C.c_int, C.POINTER(C.c_int)]),
+        "gam_set_background": (C.c_int, [H, _dp, _dp, _dp, _dp, _dp]),
+        "gam_get_background": (C.c_int, [H, _dp, _dp, _dp, _dp, _dp]),
+        "gam_background_dipole": (C.c_int, [H, C.c_double, _dp, _dp]),
         "gam_graph_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
         "gam_graph_launch": (C.c_int, [C.c_void_p, C.c_int]),
         "gam_graph_sync": (C.c_int, [C.c_void_p]),
@@ -264,10 +267,31 @@ class GamApp:
 
     def InitModel(self, corners: Optional[np.ndarray] = None):
         """Reference Hatch (init.F90:38-59) for the spec: corners -> metric on
-        the GPU, initial condition on the GPU, hatch."""
+        the GPU, background field (AddB0, init.F90:182-184) if the spec has one,
+        initial condition on the GPU, hatch."""
         self.set_corners(gen_corners(self.spec) if corners is None else corners)
+        if self.spec.background is not None:
+            b = self.spec.background
+            self.background_dipole(b.get("M0", 0.0), b.get("xc", (0.0, 0.0, 0.0)), b.get("bu", (0.0, 0.0, 0.0)))
         self.init_ic()
         self.hatch()
+
+    # ---------------------------------------------------------------- background
+    BG_SHAPES = dict(fcB0=(3, 3), edgB0=(2, 3), B0=(3,), bFlux0=(3,), dpB0=(3,))
+
+    def background_dipole(self, M0: float, xc, bu):
+        xc = np.ascontiguousarray(xc, dtype=np.float64)
+        bu = np.ascontiguousarray(bu, dtype=np.float64)
+        self._chk(lib().gam_background_dipole(self.h, float(M0), _p(xc), _p(bu)), "gam_background_dipole")
+
+    def set_background(self, bg: dict):
+        arrs = [np.asfortranarray(bg[k], dtype=np.float64) for k in self.BG_SHAPES]
+        self._chk(lib().gam_set_background(self.h, *[_p(a) for a in arrs]), "gam_set_background")
+
+    def get_background(self) -> dict:
+        out = {k: np.zeros(self.G + shp, order="F") for k, shp in self.BG_SHAPES.items()}
+        self._chk(lib().gam_get_background(self.h, *[_p(out[k]) for k in self.BG_SHAPES]), "gam_get_background")
+        return out
 
     # ---------------------------------------------------------------- state
     def put_state(self, which: int, time: float, Gas=None, magFlux=None, Bxyz=None):

@@ -29,7 +29,7 @@ class OrcModel(C.Structure):
                  "t", "tFin", "dt", "dt0", "limDT0", "limCPR")] + \
                [(n, C.c_int) for n in
                 ("ts", "rmeth", "doLFMLim", "doBoris", "doMHD", "do25D", "doCPR",
-                 "fixedTimestep", "status")]
+                 "fixedTimestep", "status", "doBackground")]
 
 
 class OrcGrid(C.Structure):
@@ -39,7 +39,7 @@ class OrcGrid(C.Structure):
                  "isDT", "ieDT", "jsDT", "jeDT", "ksDT", "keDT")] + \
                [("nc", C.c_size_t), ("nf", C.c_size_t)] + \
                [(n, _dp) for n in ("xyz", "xyzcc", "xfc", "face", "Tf", "edge", "Te", "Teb",
-                                   "volume", "di", "dj", "dk")]
+                                   "volume", "di", "dj", "dk", "fcB0", "edgB0", "B0", "bFlux0", "dpB0")]
 
 
 class OrcState(C.Structure):
@@ -96,6 +96,8 @@ def lib():
             "orc_cell_boris": (None, [M, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]),
             "orc_cell_maxwell": (None, [M, _dp, _dp, C.c_double]),
             "orc_cell_dt": (C.c_double, [M, G, S, C.c_int, C.c_int, C.c_int]),
+            "orc_magkinflux_b0": (None, [M, C.c_int, _dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp, _dp]),
+            "orc_add_b0": (None, [M, G, C.c_double, _dp, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -174,7 +176,15 @@ class OracleCase:
             a = np.zeros(shp, order="F")
             self.grid[k] = a
             setattr(g, k, ptr(a))
+        # background field arrays (G-sized, reference gamtypes.F90:231-238)
+        bshapes = dict(fcB0=G + (3, 3), edgB0=G + (2, 3), B0=G + (3,), bFlux0=G + (3,), dpB0=G + (3,))
+        self.bg = {}
+        for k, shp in bshapes.items():
+            a = np.zeros(shp, order="F")
+            self.bg[k] = a
+            setattr(g, k, ptr(a))
         self.m = model_struct(spec.model, spec.do25D)
+        self.m.doBackground = int(spec.background is not None)
         self.state = OracleState(G, F)
         self.ostate = OracleState(G, F)
         self.solver = L.orc_solver_new(C.byref(g))
@@ -217,10 +227,20 @@ class OracleCase:
         elif sp.ic != "none":
             raise ValueError(sp.ic)
 
+    def add_b0(self):
+        """background.F90:20-167 AddB0 for the spec's analytic background."""
+        b = self.spec.background
+        if b is None:
+            return
+        xc = (C.c_double * 3)(*b.get("xc", (0.0, 0.0, 0.0)))
+        bu = (C.c_double * 3)(*b.get("bu", (0.0, 0.0, 0.0)))
+        lib().orc_add_b0(C.byref(self.m), C.byref(self.g), float(b.get("M0", 0.0)), xc, bu)
+
     def setup(self):
-        """Reference Hatch: corners, Corners2Grid, IC, first BCs and dt."""
+        """Reference Hatch: corners, Corners2Grid, background field, IC, first BCs and dt."""
         self.gen_corners()
         self.metric()
+        self.add_b0()
         self.ic()
         self.hatch_finish()
 

@@ -212,15 +212,21 @@ void orc_gaskinflux(const orc_model *m, int q, const double *P5, const double *T
     dW[ENERGY] = dW[ENERGY] + sg * E;
 }
 
-void orc_magkinflux(const orc_model *m, int q, const double *P5, const double *B,
-                    double Bn, const double *T, double *mF, double *bbD, double *VaD) {
-    /* stress.F90:513-599 MagKinFlux (no background field) */
+static void magkin(const orc_model *m, int q, const double *P5, const double *B, double Bn,
+                   const double *T, const double *B0, double B0n, double *mF, double *bbD, double *VaD) {
+    /* stress.F90:513-599 MagKinFlux; B0 == NULL: no background field */
     double sg = (double)q;
     double D = P5[0], P = P5[4];
     double Bx = B[0], By = B[1], Bz = B[2];
     double dPb = 0.5 * (sq(Bx) + sq(By) + sq(Bz));
-    double Va2 = 2 * dPb / D;
-    *bbD = *bbD + 0.5 * (sq(Bx) + sq(By) + sq(Bz));
+    double Va2;
+    if (B0) {
+        Va2 = (sq(Bx + B0[0]) + sq(By + B0[1]) + sq(Bz + B0[2])) / D;
+        *bbD = *bbD + 0.5 * (sq(Bx + B0[0]) + sq(By + B0[1]) + sq(Bz + B0[2]));
+    } else {
+        Va2 = 2 * dPb / D;
+        *bbD = *bbD + 0.5 * (sq(Bx) + sq(By) + sq(Bz));
+    }
     double Vac2 = m->doBoris ? Va2 / (1 + (Va2 / sq(m->Ca))) : Va2;
     double Nx = T[NORMX], Ny = T[NORMY], Nz = T[NORMZ];
     double Vn = Nx * P5[1] + Ny * P5[2] + Nz * P5[3];
@@ -230,6 +236,22 @@ void orc_magkinflux(const orc_model *m, int q, const double *P5, const double *B
     mF[0] = mF[0] + Vn0 * (dPb * Nx - Bx * Bn);
     mF[1] = mF[1] + Vn0 * (dPb * Ny - By * Bn);
     mF[2] = mF[2] + Vn0 * (dPb * Nz - Bz * Bn);
+    if (B0) {   /* stress.F90:589-596 cross terms */
+        double BdB0 = B0[0] * Bx + B0[1] * By + B0[2] * Bz;
+        mF[0] = mF[0] + Vn0 * (-B0[0] * Bn - Bx * B0n + BdB0 * Nx);
+        mF[1] = mF[1] + Vn0 * (-B0[1] * Bn - By * B0n + BdB0 * Ny);
+        mF[2] = mF[2] + Vn0 * (-B0[2] * Bn - Bz * B0n + BdB0 * Nz);
+    }
+}
+
+void orc_magkinflux(const orc_model *m, int q, const double *P5, const double *B,
+                    double Bn, const double *T, double *mF, double *bbD, double *VaD) {
+    magkin(m, q, P5, B, Bn, T, NULL, 0.0, mF, bbD, VaD);
+}
+
+void orc_magkinflux_b0(const orc_model *m, int q, const double *P5, const double *B, double Bn,
+                       const double *T, const double *B0, double B0n, double *mF, double *bbD, double *VaD) {
+    magkin(m, q, P5, B, Bn, T, B0, B0n, mF, bbD, VaD);
 }
 
 /* ----------------------------------------------------- flux -> field */
@@ -415,6 +437,11 @@ void orc_calc_efield(const orc_model *m, const orc_grid *g, const orc_state *n,
                     double Jd = db2 - db1;
                     double b1 = detT * (ynj * bT1 - yni * bT2);
                     double b2 = detT * (-xnj * bT1 + xni * bT2);
+                    if (m->doBackground) {   /* fields.F90:417-420 */
+                        size_t c0 = CI(g, i, j, k);
+                        b1 = b1 + g->edgB0[c0 + nc * (0 + 2 * eD)];
+                        b2 = b2 + g->edgB0[c0 + nc * (1 + 2 * eD)];
+                    }
                     /* fields.F90:204-221 */
                     double vA = sqrt(sq(b1) + sq(b2)) / sqrt(Dc);
                     if (m->doBoris) vA = m->Ca * vA / sqrt(sq(m->Ca) + sq(vA));
@@ -457,6 +484,13 @@ static void fluxes_iface(const orc_model *m, const orc_grid *g, const orc_state 
         }
         Bn = hf->magFlux[f + nf * dN] / faB;
     }
+    double B0f[3] = {0, 0, 0}, B0n = 0.0;
+    const int bg = m->doMHD && m->doBackground;
+    if (bg) {   /* stress.F90:302-305 */
+        size_t c0 = CI(g, i, j, k);
+        for (int x = 0; x < 3; x++) B0f[x] = g->fcB0[c0 + nc * (x + 3 * dN)];
+        B0n = g->bFlux0[c0 + nc * dN] / faB;
+    }
     const double *limG = m->doLFMLim ? n->Gas : hf->Gas;
     for (int q = 0; q < 8; q++) {
         for (int v = 0; v < 5; v++) ConB[q][v] = hf->Gas[cs[q] + nc * v];
@@ -477,8 +511,8 @@ static void fluxes_iface(const orc_model *m, const orc_grid *g, const orc_state 
     double VaD = 0.0;
     if (m->doMHD) {
         double mF[3] = {0, 0, 0}, bbD = 0.0;
-        orc_magkinflux(m, -1, PL, MagL, Bn, Tf, mF, &bbD, &VaD);
-        orc_magkinflux(m, +1, PR, MagR, Bn, Tf, mF, &bbD, &VaD);
+        magkin(m, -1, PL, MagL, Bn, Tf, bg ? B0f : NULL, B0n, mF, &bbD, &VaD);
+        magkin(m, +1, PR, MagR, Bn, Tf, bg ? B0f : NULL, B0n, mF, &bbD, &VaD);
         if (m->doBoris) {   /* stress.F90:335-351 */
             for (int d = 0; d < 3; d++) {
                 double deeV = PR[1 + d] - PL[1 + d];
@@ -546,6 +580,8 @@ void orc_calc_stress(const orc_model *m, const orc_grid *g, const orc_state *n,
                         dGasM[c + nc * v] = (F[c + nc * (v + 0)] - F[ci + nc * (v + 0)]
                                            + F[c + nc * (v + 3)] - F[cj + nc * (v + 3)]
                                            + F[c + nc * (v + 6)] - F[ck + nc * (v + 6)]) / dV;
+                        /* stress.F90:185-188 background force */
+                        if (m->doBackground) dGasM[c + nc * v] = dGasM[c + nc * v] + g->dpB0[c + nc * v];
                     }
                 }
             }
@@ -662,6 +698,8 @@ static void apply_update(orc_model *m, const orc_grid *g, orc_state *s, orc_stat
                 double U[5], oU[5], B[3], oB[3], dPg[5], dPm[3];
                 for (int v = 0; v < 5; v++) { U[v] = s->Gas[c + nc * v]; oU[v] = o->Gas[c + nc * v]; dPg[v] = dGasH[c + nc * v]; }
                 for (int v = 0; v < 3; v++) { B[v] = s->Bxyz[c + nc * v]; oB[v] = o->Bxyz[c + nc * v]; dPm[v] = dGasM[c + nc * v]; }
+                if (m->doBackground)   /* mhdgroup.F90:180-183 total fields */
+                    for (int v = 0; v < 3; v++) { B[v] = B[v] + g->B0[c + nc * v]; oB[v] = oB[v] + g->B0[c + nc * v]; }
                 if (boris) orc_cell_boris(m, U, oU, B, oB, dPg, dPm, Ca, dt);
                 else orc_cell_maxwell(m, U, dPm, dt);
                 for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = U[v];
@@ -736,6 +774,9 @@ double orc_cell_dt(const orc_model *m, const orc_grid *g, const orc_state *s, in
     double Valf = 0.0;
     if (m->doMHD) {
         double Bx = s->Bxyz[c], By = s->Bxyz[c + nc], Bz = s->Bxyz[c + 2 * nc];
+        if (m->doBackground) {   /* gridutils.F90:82-86 */
+            Bx = Bx + g->B0[c]; By = By + g->B0[c + nc]; Bz = Bz + g->B0[c + 2 * nc];
+        }
         double MagB = sqrt(sq(Bx) + sq(By) + sq(Bz));
         Valf = MagB / sqrt(rho);
         if (m->doBoris) Valf = m->Ca * Valf / sqrt(m->Ca * m->Ca + Valf * Valf);
@@ -1294,4 +1335,136 @@ void orc_ic_bw(const orc_model *m, const orc_grid *g, orc_state *s,
                 s->magFlux[f + 2 * nf] = LI[c] - LI[CI(g, i, j + 1, k)] + LJ[CI(g, i + 1, j, k)] - LJ[c];
             }
     free(lA);
+}
+
+/* ----------------------------------------------------- background field */
+typedef struct { double M0, xc[3], bu[3]; } b0_ctx;
+
+/* B0(x) = bu + dipole(x - xc), dipolehelper.F90:19-30 MagsphereDipole with
+ * m = (0,0,M0); rad**5 and rad**3 written as products */
+static void b0_eval(const b0_ctx *c, double x, double y, double z, double *B) {
+    double r[3] = {x - c->xc[0], y - c->xc[1], z - c->xc[2]};
+    B[0] = c->bu[0]; B[1] = c->bu[1]; B[2] = c->bu[2];
+    if (c->M0 != 0.0) {
+        double rad = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        double r3 = rad * rad * rad, r5 = r3 * rad * rad;
+        double mdr = c->M0 * r[2];              /* dot_product(m, xyz) */
+        double m3[3] = {0.0, 0.0, c->M0};
+        for (int x3 = 0; x3 < 3; x3++) B[x3] = B[x3] + (3 * mdr * r[x3] / r5 - m3[x3] / r3);
+    }
+}
+
+static void b0_wrap(const void *cv, double x, double y, double z, double *W) {   /* background.F90:157-165 */
+    double B[3];
+    b0_eval((const b0_ctx *)cv, x, y, z, B);
+    W[0] = B[0]; W[1] = B[1]; W[2] = B[2]; W[3] = 0.0; W[4] = 0.0;
+}
+
+/* faceCoords (metric.F90:79-108), the corner order of orc_corners2grid */
+static inline void face_coords(const orc_grid *g, int i, int j, int k, int d,
+                               double *f0, double *f1, double *f2, double *f3) {
+    xyzv(g, i, j, k, f0);
+    if (d == ID) { xyzv(g, i, j + 1, k, f1); xyzv(g, i, j, k + 1, f2); xyzv(g, i, j + 1, k + 1, f3); }
+    else if (d == JD) { xyzv(g, i, j, k + 1, f1); xyzv(g, i + 1, j, k, f2); xyzv(g, i + 1, j, k + 1, f3); }
+    else { xyzv(g, i + 1, j, k, f1); xyzv(g, i, j + 1, k, f2); xyzv(g, i + 1, j + 1, k, f3); }
+}
+
+/* quadrature.F90:105-151 GaussianFaceFlux, 159-221 GaussianFaceIntegral (fInt)
+ * and 223-280 GaussianFaceStress of B0 over one face, in one sweep */
+static void b0_face(const b0_ctx *c, const double *f0, const double *f1, const double *f2, const double *f3,
+                    double *flx, double *fInt, double *Mbb) {
+    double dx[3], dy[3], dz[3];      /* quadrature.F90:373-389 faceDeltas */
+    dx[0] = f1[0] - f0[0]; dx[1] = f2[0] - f0[0]; dx[2] = f3[0] + f0[0] - f2[0] - f1[0];
+    dy[0] = f1[1] - f0[1]; dy[1] = f2[1] - f0[1]; dy[2] = f3[1] + f0[1] - f2[1] - f1[1];
+    dz[0] = f1[2] - f0[2]; dz[1] = f2[2] - f0[2]; dz[2] = f3[2] + f0[2] - f2[2] - f1[2];
+    *flx = 0.0;
+    for (int x = 0; x < 3; x++) { fInt[x] = 0.0; Mbb[x] = 0.0; }
+    for (int a = 0; a < 12; a++)
+        for (int b = 0; b < 12; b++) {
+            double eta = 0.5 * (1 + gA[a]), psi = 0.5 * (1 + gA[b]);
+            double w = gW[a] * gW[b];
+            double xp = f0[0] + dx[0] * eta + dx[1] * psi + dx[2] * eta * psi;
+            double yp = f0[1] + dy[0] * eta + dy[1] * psi + dy[2] * eta * psi;
+            double zp = f0[2] + dz[0] * eta + dz[1] * psi + dz[2] * eta * psi;
+            double A[3];
+            b0_eval(c, xp, yp, zp, A);
+            double eV[3] = {dx[0] + dx[2] * psi, dy[0] + dy[2] * psi, dz[0] + dz[2] * psi};
+            double pV[3] = {dx[1] + dx[2] * eta, dy[1] + dy[2] * eta, dz[1] + dz[2] * eta};
+            double cr[3]; cross3(eV, pV, cr);
+            double N[3] = {0.25 * w * cr[0], 0.25 * w * cr[1], 0.25 * w * cr[2]};
+            *flx = *flx + (N[0] * A[0] + N[1] * A[1] + N[2] * A[2]);
+            double area = 0.25 * nrm3(cr) * w;
+            for (int x = 0; x < 3; x++) fInt[x] = fInt[x] + area * A[x];
+            double BB = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+            double BdS = A[0] * N[0] + A[1] * N[1] + A[2] * N[2];
+            for (int x = 0; x < 3; x++) Mbb[x] = Mbb[x] + 0.5 * BB * N[x] - A[x] * BdS;
+        }
+}
+
+/* quadrature.F90:345-370 GaussianEdgeIntegral (the mean of B0 along the edge) */
+static void b0_edge(const b0_ctx *c, const double *e1, const double *e2, double *eInt) {
+    double dR2[3], xBar[3];
+    for (int x = 0; x < 3; x++) { dR2[x] = 0.5 * (e2[x] - e1[x]); xBar[x] = 0.5 * (e2[x] + e1[x]); eInt[x] = 0.0; }
+    for (int a = 0; a < 12; a++) {
+        double A[3];
+        b0_eval(c, xBar[0] + gA[a] * dR2[0], xBar[1] + gA[a] * dR2[1], xBar[2] + gA[a] * dR2[2], A);
+        for (int x = 0; x < 3; x++) eInt[x] = eInt[x] + 0.5 * gW[a] * A[x];
+    }
+}
+
+void orc_add_b0(const orc_model *m, const orc_grid *g, double M0, const double xc[3], const double bu[3]) {
+    /* background.F90:20-167 AddB0 */
+    (void)m;
+    init_weights();
+    const size_t nc = g->nc, nf = g->nf;
+    b0_ctx ctx = {M0, {xc[0], xc[1], xc[2]}, {bu[0], bu[1], bu[2]}};
+    double *fS = calloc(9 * nc, sizeof(double));   /* faceStress(G, face dir, xyz) */
+    memset(g->fcB0, 0, 9 * nc * sizeof(double));
+    memset(g->edgB0, 0, 6 * nc * sizeof(double));
+    memset(g->B0, 0, 3 * nc * sizeof(double));
+    memset(g->bFlux0, 0, 3 * nc * sizeof(double));
+    memset(g->dpB0, 0, 3 * nc * sizeof(double));
+#pragma omp parallel for collapse(2)
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int j = g->jsg; j <= g->jeg; j++)
+            for (int i = g->isg; i <= g->ieg; i++) {
+                const size_t c = CI(g, i, j, k), f = FI(g, i, j, k);
+                for (int d = 0; d < 3; d++) {
+                    double f0[3], f1[3], f2[3], f3[3], flx, fInt[3], Mbb[3];
+                    face_coords(g, i, j, k, d, f0, f1, f2, f3);
+                    b0_face(&ctx, f0, f1, f2, f3, &flx, fInt, Mbb);
+                    g->bFlux0[c + nc * d] = flx;
+                    for (int x = 0; x < 3; x++) g->fcB0[c + nc * (x + 3 * d)] = fInt[x] / g->face[f + nf * d];
+                    for (int x = 0; x < 3; x++) fS[c + nc * (d + 3 * x)] = Mbb[x];
+                }
+                for (int d = 0; d < 3; d++) {   /* metric.F90:111-130 edgeCoords */
+                    double e1[3], e2[3], eInt[3];
+                    xyzv(g, i, j, k, e1);
+                    if (d == ID) xyzv(g, i + 1, j, k, e2);
+                    else if (d == JD) xyzv(g, i, j + 1, k, e2);
+                    else xyzv(g, i, j, k + 1, e2);
+                    b0_edge(&ctx, e1, e2, eInt);
+                    const double *Te = g->Te + nf * 9 * d;
+                    g->edgB0[c + nc * (0 + 2 * d)] = eInt[0] * Te[f + nf * TAN1X] + eInt[1] * Te[f + nf * TAN1Y]
+                                                   + eInt[2] * Te[f + nf * TAN1Z];
+                    g->edgB0[c + nc * (1 + 2 * d)] = eInt[0] * Te[f + nf * TAN2X] + eInt[1] * Te[f + nf * TAN2Y]
+                                                   + eInt[2] * Te[f + nf * TAN2Z];
+                }
+                double C[8][3], W[5];
+                cell_coords(g, i, j, k, C);
+                gauss_volume(C, b0_wrap, &ctx, W);
+                for (int x = 0; x < 3; x++) g->B0[c + nc * x] = W[x] / g->volume[c];
+            }
+#pragma omp parallel for collapse(2)
+    for (int k = g->ks; k <= g->ke; k++)
+        for (int j = g->js; j <= g->je; j++)
+            for (int i = g->is; i <= g->ie; i++) {
+                const size_t c = CI(g, i, j, k);
+                const size_t ci = CI(g, i + 1, j, k), cj = CI(g, i, j + 1, k), ck = CI(g, i, j, k + 1);
+                for (int x = 0; x < 3; x++)
+                    g->dpB0[c + nc * x] = -(fS[ci + nc * (0 + 3 * x)] - fS[c + nc * (0 + 3 * x)]
+                                          + fS[cj + nc * (1 + 3 * x)] - fS[c + nc * (1 + 3 * x)]
+                                          + fS[ck + nc * (2 + 3 * x)] - fS[c + nc * (2 + 3 * x)]) / g->volume[c];
+            }
+    free(fS);
 }

@@ -27,6 +27,7 @@ typedef struct {
     double t, tFin, dt, dt0, limDT0, limCPR;
     int ts, rmeth, doLFMLim, doBoris, doMHD, do25D, doCPR, fixedTimestep;
     int status;   /* CalcDT outcome: 0 ok, 1 bad (dt drop), 2 disaster */
+    int doBackground;   /* background field B0 split (gamtypes.F90:88) */
 } orc_model;
 
 /* Grid_T subset (reference gamtypes.F90:163-245) */
@@ -48,6 +49,12 @@ typedef struct {
     double *Te;      /* F x 9 x 3 edge triads */
     double *Teb;     /* F x 4 x 3 edge B transforms */
     double *volume, *di, *dj, *dk;  /* G */
+    /* background field (gamtypes.F90:231-238), G-sized, filled by orc_add_b0 */
+    double *fcB0;    /* G x 3 x 3 face-averaged B0 (xyz, face dir) */
+    double *edgB0;   /* G x 2 x 3 edge-averaged B0 on the edge's (T1,T2) (slot, edge dir) */
+    double *B0;      /* G x 3 cell-averaged B0 */
+    double *bFlux0;  /* G x 3 B0 flux through each face */
+    double *dpB0;    /* G x 3 -div(B0 Maxwell stress)/volume (active cells) */
 } orc_grid;
 
 /* State_T subset (reference gamtypes.F90:248-264) */
@@ -73,8 +80,9 @@ void   orc_ic_ot(const orc_model *m, const orc_grid *g, orc_state *s,
 void   orc_ic_bw(const orc_model *m, const orc_grid *g, orc_state *s,
                  double x0, double y0, double z0, double D0, double P0,
                  double pRat, double dRat, double rC, double B0);
-void   orc_ic_advance_modes(const orc_model *m, const orc_grid *g, orc_state *s,
-                            const double *modes, int nmodes);
+/* background.F90:20-167 AddB0 for B0(x) = bu + dipole(x - xc) with moment
+ * (0,0,M0) (dipolehelper.F90:19-30, with r^3 and r^5 as products) */
+void   orc_add_b0(const orc_model *m, const orc_grid *g, double M0, const double xc[3], const double bu[3]);
 
 /* solver scratch */
 orc_solver *orc_solver_new(const orc_grid *g);
@@ -113,6 +121,10 @@ void   orc_gaskinflux(const orc_model *m, int q, const double *prim, const doubl
                       double *gFlx5, double *dW5);
 void   orc_magkinflux(const orc_model *m, int q, const double *prim, const double *mag,
                       double Bn, const double *Tf9, double *mFlx3, double *bbD, double *VaD);
+/* MagKinFlux with the background-field terms (B0 at the face, B0n its normal) */
+void   orc_magkinflux_b0(const orc_model *m, int q, const double *prim, const double *mag,
+                         double Bn, const double *Tf9, const double *B0, double B0n,
+                         double *mFlx3, double *bbD, double *VaD);
 void   orc_cell_boris(const orc_model *m, double *U2, const double *U0, const double *B2,
                       const double *B0, const double *dGasH, const double *dGasM,
                       double Ca, double dt);

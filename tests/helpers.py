@@ -40,11 +40,20 @@ def small_cases():
     nolfm = configs.orszag_tang(24, 4, steps=6, doLFMLim=False)
     thin = configs.blast_wave((24, 2, 3), steps=6)     # owned extents below the ghost depth
     thin.bounds = (0.0, 1.0, 0.4, 0.6, 0.3, 0.7)
+    # background field split: a dipole outside the box plus a uniform part
+    dip = configs.blast_wave(16, steps=8)
+    dip.background = DIPOLE
+    dipb = configs.blast_wave(16, steps=8, doBoris=True, Ca=6.0)
+    dipb.background = DIPOLE
     for s, n in ((ot, "ot32x32x4"), (ot8, "ot32x32x4_8cent"), (ot25, "ot32x32x1_25d"), (bw, "bw16_warp"),
                  (bwb, "bw16_warp_boris"), (hyd, "bw16_hydro"), (nolfm, "ot24x24x4_nolfmlim"),
-                 (thin, "bw24x2x3_thin")):
+                 (thin, "bw24x2x3_thin"), (dip, "bw16_warp_dipole"), (dipb, "bw16_warp_dipole_boris")):
         s.name = n
-    return [ot, ot8, ot25, bw, bwb, hyd, nolfm, thin]
+    return [ot, ot8, ot25, bw, bwb, hyd, nolfm, thin, dip, dipb]
+
+
+# B0 = (0.5, -1, 0.25) + dipole of moment (0,0,8) at (-0.7, 0.45, 0.6): |B0| ~ 1-20 in the unit cube
+DIPOLE = dict(M0=8.0, xc=(-0.7, 0.45, 0.6), bu=(0.5, -1.0, 0.25))
 
 
 def app_from_oracle(oc):
@@ -52,6 +61,8 @@ def app_from_oracle(oc):
     from kaiju_amd.gamera import GamApp
     app = GamApp(oc.spec)
     app.set_grid(oc.grid)
+    if oc.spec.background is not None:
+        app.set_background(oc.bg)
     st, os_ = oc.state, oc.ostate
     app.put_state(0, st.time, st.Gas, st.magFlux, st.Bxyz)
     app.put_state(1, os_.time, os_.Gas, os_.magFlux, os_.Bxyz)

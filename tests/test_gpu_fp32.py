@@ -28,6 +28,13 @@ CASES = [dataclasses.replace(c, model=dataclasses.replace(c.model, precision=1))
 # momentum 2e-5 after 8; hydro and OT cases stay at 1e-7).
 TOL_ONE = 1e-5      # one step
 TOL_MULTI = 2e-5    # case.steps steps
+# with the background dipole |B0| reaches ~23 against p = 0.1 (beta ~ 4e-4,
+# five times lower still): measured 1.1e-5 after one step, 7.8e-5 after 8
+TOL_ONE_BG, TOL_MULTI_BG = 3e-5, 2e-4
+
+
+def tols(case):
+    return (TOL_ONE_BG, TOL_MULTI_BG) if case.background is not None else (TOL_ONE, TOL_MULTI)
 
 
 @pytest.fixture(scope="module", params=CASES, ids=[c.name for c in CASES])
@@ -81,7 +88,7 @@ def test_fp32_one_step_within_rounding(case):
         assert a32.steps(1) == 0 and a64.steps(1) == 0
         e = errs(a32.get_state(0), a64.get_state(0))
         say(f"   one step FP32 vs FP64: {e}")
-        assert max(e.values()) <= TOL_ONE, e
+        assert max(e.values()) <= tols(case)[0], e
         assert abs(a32.Model.dt - a64.Model.dt) <= 1e-6 * a64.Model.dt
 
 
@@ -100,7 +107,7 @@ def test_fp32_multistep_divb_mass(case):
         e = errs(s32, s64)
         l1 = l1_per_variable(s32["Gas"][A], s64["Gas"][A])
         say(f"   after {case.steps}: normwise {e}  L1 per variable {l1}")
-        assert max(e.values()) <= TOL_MULTI, e
+        assert max(e.values()) <= tols(case)[1], e
         if case.model.doMHD:
             scale = float(np.abs(s64["magFlux"]).max()) * case.zones
             assert abs(a32.divb() - divb0) <= 1e-13 * scale

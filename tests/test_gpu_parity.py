@@ -198,3 +198,24 @@ def test_set_tile_rejects_bad_shapes(case):
             app.set_tile("flux_i", 64, 4, 1, 8)   # no mode 8
         with pytest.raises(HipStepError):
             app.set_tile("ct_update", 64, 4, 1, 0)   # not a tiled stage
+
+
+def test_background_from_gpu(case):
+    """gam_background_dipole (AddB0 on the GPU) vs orc_add_b0 on the same grid."""
+    if case.background is None:
+        pytest.skip("no background field in this case")
+    say(f"[addb0] {case.name}")
+    from kaiju_amd.gamera import GamApp
+    oc = oracle.OracleCase(case)
+    b = case.background
+    with GamApp(case) as app:
+        app.set_grid(oc.grid)
+        app.background_dipole(b["M0"], b["xc"], b["bu"])
+        bg = app.get_background()
+    for k, ref in oc.bg.items():
+        scale = max(float(np.abs(ref).max()), 1e-300)
+        err = float(np.abs(bg[k] - ref).max()) / scale
+        say(f"   {k}: {err:.2e}")
+        # dpB0 is a difference of O(B0^2) face stresses divided by the volume
+        tol = 1e-9 if k == "dpB0" else TOL_SETUP
+        assert err <= tol, (k, err)
