@@ -108,9 +108,32 @@ int  gam_get_background(gam_handle *h, double *fcB0, double *edgB0, double *B0, 
                         double *dpB0);
 int  gam_background_dipole(gam_handle *h, double M0, const double *xc, const double *bu);
 
+/* --- learned coarse-grid correction ------------------------------------- */
+/* No counterpart in the reference.  A correction field o (device FP32, [G][8]:
+ * 5 gas-flux and 3 edge-EMF channels per cell, e.g. the output of
+ * libgamera_net.so) set with gam_set_correction is applied by every following
+ * AdvanceMHD: scale*face*o[v] is added to gas flux v of each active interface
+ * at the cell's lower faces and scale*edge*o[5+d] to each active edge EMF d,
+ * before E2Flux and the update, so the update stays conservative and div.B
+ * exact.  o is read at launch time: refresh it between steps (o = NULL turns the
+ * correction off).  gam_state_device returns the device arrays of State (0) or
+ * oState (1), bulk-typed by the precision.  Training: gam_correction_target
+ * uploads a target change (Gas G*5, magFlux F*3) and gam_correction_grad
+ * evaluates, for the change o makes to one step of length dt linearised
+ * (Gas += dt*divergence, magFlux -= dt*curl), the loss
+ *   sum_active_cells sum_v w_v (dGas_v - target_v)^2 + w3 sum_CT_faces (dPhi - target)^2
+ * with w4 = {w_rho, w_momentum, w_energy, w_magFlux}, and its gradient dL/do
+ * into dout (device FP32 [G][8]). */
+int  gam_state_device(gam_handle *h, int which, void **Gas, double **magFlux, void **Bxyz);
+int  gam_set_correction(gam_handle *h, const float *o, double scale);
+int  gam_correction_target(gam_handle *h, const double *dGas, const double *dMag);
+int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt, const double *w4,
+                         float *dout, double *loss);
+
 /* --- state ---------------------------------------------------------------- */
 /* which: 0 = State, 1 = oState, 2 = predictor (half-step) state (read only).
- * Gas G*5, magFlux F*3, Bxyz G*3; NULL skips an array. */
+ * Gas G*5, magFlux F*3, Bxyz G*3; NULL skips an array, except that a NULL
+ * Bxyz with a magFlux given is recomputed from it on the device (bFlux2Fld). */
 int  gam_put_state(gam_handle *h, int which, double time, const double *Gas,
                    const double *magFlux, const double *Bxyz);
 int  gam_get_state(gam_handle *h, int which, double *time, double *Gas,

@@ -1,6 +1,8 @@
 """Build the native parts in-tree.
 
 * ``kaiju_amd/lib/libgamera_hip.so`` -- the HIP step for gfx950 (hipcc).
+* ``kaiju_amd/lib/libgamera_net.so`` -- the learned-correction network
+  (BF16 MFMA convolutions, backward and Adam) for gfx950.
 * ``kaiju_amd/bin/gamerax_gpu`` and ``kaiju_amd/lib/libgamhost.so`` -- the
   Fortran host (amdflang) that keeps the reference's Model/Grid/State types and
   calls the C-ABI.
@@ -21,6 +23,7 @@ FSRC = os.path.join(PKG, "fortran")
 LIBDIR = os.path.join(PKG, "lib")
 BINDIR = os.path.join(PKG, "bin")
 HIP_LIB = os.path.join(LIBDIR, "libgamera_hip.so")
+NET_LIB = os.path.join(LIBDIR, "libgamera_net.so")
 
 # one translation unit: gam_api.hip includes gam_step.hip and gam_setup.hip
 HIP_SOURCES = ["gam_api.hip"]
@@ -50,17 +53,24 @@ def _flang() -> str:
     raise RuntimeError("amdflang not found")
 
 
-def build_hip(verbose: bool = False) -> str:
+def _build_so(srcs, out: str, verbose: bool) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    tmp = HIP_LIB + ".tmp"
-    cmd = [_hipcc(), *HIPCC_FLAGS, "-Wl,-soname,libgamera_hip.so", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-           *srcs, "-o", tmp]
+    tmp = out + ".tmp"
+    cmd = [_hipcc(), *HIPCC_FLAGS, f"-Wl,-soname,{os.path.basename(out)}", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+           *[os.path.join(CSRC, s) for s in srcs], "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, HIP_LIB)
-    return HIP_LIB
+    os.replace(tmp, out)
+    return out
+
+
+def build_hip(verbose: bool = False) -> str:
+    return _build_so(HIP_SOURCES, HIP_LIB, verbose)
+
+
+def build_net(verbose: bool = False) -> str:
+    return _build_so(["gam_net.hip"], NET_LIB, verbose)
 
 
 FORTRAN_MODULES = ["gamera_hip_c.F90", "gamtypes_gpu.F90", "gamapp_gpu.F90", "gamprob_gpu.F90"]
@@ -97,6 +107,7 @@ def build_fortran(verbose: bool = False, lib: str = HIP_LIB, exe: str = None) ->
 
 def build_all(verbose: bool = False):
     build_hip(verbose)
+    build_net(verbose)
     if os.path.isdir(FSRC) and all(os.path.exists(os.path.join(FSRC, m)) for m in FORTRAN_MODULES):
         build_fortran(verbose)
 
@@ -105,6 +116,8 @@ if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     if what == "hip":
         build_hip(True)
+    elif what == "net":
+        build_net(True)
     elif what == "fortran":
         build_fortran(True)
     else:

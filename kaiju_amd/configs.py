@@ -38,6 +38,7 @@ class ModelSpec:
     doMHD: bool = True
     doCPR: bool = False
     precision: int = 0          # 0 plain FP64; 1 FP32 bulk with FP64 CT, update and dt
+    fixedTimestep: bool = False  # dt kept as set (gam_set_clock); CalcDT's checks skipped
 
 
 @dataclasses.dataclass

@@ -50,6 +50,9 @@ struct gam_handle {
     void *Vf = nullptr, *E = nullptr, *Flx = nullptr;
     void *wk[W_N] = {};        // predictor-pass stencil quantities
     double *bg[5] = {};        // background field fcB0, edgB0, B0, bFlux0, dpB0 (G-sized)
+    const float *corr = nullptr; // learned correction field [G][8] (device), or none
+    double corr_scale = 0.0;
+    double *ctG = nullptr, *ctM = nullptr, *crG = nullptr, *crM = nullptr;   // correction training buffers
     BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
     double *partial = nullptr;
     Tile tile[ST_N] = {};       // launch tiles of the tiled stages
@@ -268,6 +271,7 @@ extern "C" int gam_destroy(gam_handle *h) {
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
     for (int q = 0; q < 9; q++) if (h->grid32[q]) (void)hipFree(h->grid32[q]);
     for (int q = 0; q < 5; q++) if (h->bg[q]) (void)hipFree(h->bg[q]);
+    for (double *b : {h->ctG, h->ctM, h->crG, h->crM}) if (b) (void)hipFree(b);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
     void *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, h->clk,
                       h->wk[0], h->wk[1], h->wk[2], h->wk[3], h->wk[4]};
@@ -392,6 +396,8 @@ static void state_ptrs(gam_handle *h, int which, void **Gas, void **mag, void **
     *Gas = h->Gas[q]; *mag = h->mag[q]; *B = h->B[q];
 }
 
+template <typename R, typename TM> static void launch_b2f(gam_handle *h, int st, const TM *mag, void *B);
+
 extern "C" int gam_put_state(gam_handle *h, int which, double time, const double *Gas, const double *magFlux,
                              const double *Bxyz) {
     if (!h || which < 0 || which > 1) return -1;
@@ -400,7 +406,14 @@ extern "C" int gam_put_state(gam_handle *h, int which, double time, const double
     int rc;
     if ((rc = h2d_bulk(h, g, Gas, 5 * gsz(h)))) return rc;
     if ((rc = h2d(h, (double *)m, magFlux, 3 * fsz(h)))) return rc;
-    if ((rc = h2d_bulk(h, b, Bxyz, 3 * gsz(h)))) return rc;
+    if (Bxyz) {
+        if ((rc = h2d_bulk(h, b, Bxyz, 3 * gsz(h)))) return rc;
+    } else if (magFlux && h->prec) {   // bFlux2Fld of the uploaded magFlux
+        launch_b2f<float, double>(h, ST_BPOST, (const double *)m, b);
+    } else if (magFlux) {
+        launch_b2f<double, double>(h, ST_BPOST, (const double *)m, b);
+    }
+    HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost));
     if (which == 0) h->clk_host->stime = time; else h->clk_host->otime = time;
@@ -583,6 +596,79 @@ extern "C" int gam_background_dipole(gam_handle *h, double M0, const double *xc,
     return 0;
 }
 
+// ----------------------------------------------------------------- learned correction
+extern "C" int gam_state_device(gam_handle *h, int which, void **Gas, double **magFlux, void **Bxyz) {
+    if (!h || which < 0 || which > 1) return -1;
+    void *g, *m, *b;
+    state_ptrs(h, which, &g, &m, &b);
+    if (Gas) *Gas = g;
+    if (magFlux) *magFlux = (double *)m;
+    if (Bxyz) *Bxyz = b;
+    return 0;
+}
+
+extern "C" int gam_set_correction(gam_handle *h, const float *o, double scale) {
+    if (!h) return -1;
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    h->corr = o;
+    h->corr_scale = scale;
+    return 0;
+}
+
+extern "C" int gam_correction_target(gam_handle *h, const double *dGas, const double *dMag) {
+    if (!h || !dGas || !dMag) return -1;
+    const size_t G = gsz(h), F = fsz(h);
+    if (!h->ctG) {
+        HIPCHK(h, hipMalloc(&h->ctG, 5 * G * sizeof(double)));
+        HIPCHK(h, hipMalloc(&h->ctM, 3 * F * sizeof(double)));
+        HIPCHK(h, hipMalloc(&h->crG, 5 * G * sizeof(double)));
+        HIPCHK(h, hipMalloc(&h->crM, 3 * F * sizeof(double)));
+    }
+    int rc;
+    if ((rc = h2d(h, h->ctG, dGas, 5 * G))) return rc;
+    if ((rc = h2d(h, h->ctM, dMag, 3 * F))) return rc;
+    return gam_sync(h);
+}
+
+template <typename R>
+static int corr_grad_t(gam_handle *h, const float *o, double scale, double dt, const double *w, float *dout, double *loss) {
+    const GVT<R> &g = view<R>(h);
+    const int nbc = (int)grid1(g.nc).x, nbf = (int)grid1(g.nf).x;
+    if (nbc > h->pcap || nbf > h->pcap) {   // the block sums reuse h->partial
+        double *p = nullptr;
+        HIPCHK(h, hipMalloc(&p, (size_t)std::max(nbc, nbf) * sizeof(double)));
+        HIPCHK(h, hipFree(h->partial));
+        h->partial = p;
+        h->pcap = std::max(nbc, nbf);
+    }
+    std::vector<double> hp(std::max(nbc, nbf));
+    double L = 0.0;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_corr_cells<R>), dim3(nbc), 256, 0, h->stream, g, h->mp.do25D, o, scale, dt,
+                       (const double *)h->ctG, w[0], w[1], w[2], h->crG, h->partial);
+    HIPCHK(h, hipMemcpyAsync(hp.data(), h->partial, nbc * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int q = 0; q < nbc; q++) L += hp[q];
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_corr_faces<R>), dim3(nbf), 256, 0, h->stream, g, o, scale, dt,
+                       (const double *)h->ctM, w[3], h->crM, h->partial);
+    HIPCHK(h, hipMemcpyAsync(hp.data(), h->partial, nbf * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int q = 0; q < nbf; q++) L += hp[q];
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_corr_adjoint<R>), dim3(nbc), 256, 0, h->stream, g, h->mp.do25D, scale, dt,
+                       (const double *)h->crG, (const double *)h->crM, dout);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (loss) *loss = L;
+    return 0;
+}
+
+extern "C" int gam_correction_grad(gam_handle *h, const float *o, double scale, double dt, const double *w4,
+                                   float *dout, double *loss) {
+    if (!h || !o || !w4 || !dout) return -1;
+    if (!h->ctG) return fail(h, -1, "gam_correction_grad: no target set (gam_correction_target)");
+    return h->prec ? corr_grad_t<float>(h, o, scale, dt, w4, dout, loss) : corr_grad_t<double>(h, o, scale, dt, w4, dout, loss);
+}
+
 // ----------------------------------------------------------------- step pieces
 // CellDT min over the DT box of (Gas, B) into h->partial (CalcDT's loop).
 template <typename R>
@@ -688,6 +774,9 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     } else {
         if (m.rmeth == 1) launch_fluxes<1, 0, 0, R>(h); else launch_fluxes<0, 0, 0, R>(h);
     }
+    if (h->corr)   // learned correction of fluxes and EMFs, before CT and the update
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_apply_corr<R>), grid1(g.nc), 256, 0, h->stream, g, h->clk, m.do25D, h->corr,
+                           h->corr_scale, P<R>(h->Flx), P<R>(h->E));
     {
         StageTimer t(h, ST_CT);
         if (m.doMHD) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ct<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mag[S], P<R>(h->E), h->mag[O]);

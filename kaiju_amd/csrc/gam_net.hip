@@ -1,0 +1,480 @@
+// gam_net.hip -- the learned coarse-grid correction: a 3-layer 3x3x3 conv net
+// (8 -> 32 -> 32 -> 8 channels, ReLU between) on the ghost-inclusive cell grid,
+// with BF16 weights and activations and FP32 accumulation on gfx950's
+// v_mfma_f32_16x16x32_bf16.  Forward, backward (data and weights) and Adam are
+// hand-written here; libgamera_net.so exports them (include/gamera_net.h).
+//
+// Layouts (nc = cells of the ghost-inclusive box, p = linear cell index, i fastest):
+//   activations  act[p][32]   BF16, channels last (A operand of the convolutions)
+//                actT[c][p]   BF16, channel-major copy (operand of the weight gradients)
+//   weights      W[l][tap][co][ci] BF16 (ci = 32 padded), tap = dz*9 + dy*3 + dx
+//   output       out[p][8]    FP32 (the correction field, channels last)
+// A convolution is an implicit GEMM: M = 16 consecutive cells per MFMA, N = 16
+// output channels per tile, K = 32 input channels of one tap per MFMA.  Output
+// cells cover [lo, lo + 16*ntile) with lo = Ni*Nj + Ni + 1, so every tap's
+// neighbour p + off stays inside the array; cells near the box faces read
+// wrapped neighbours and are never used (the correction is read on the active
+// box plus one layer only, 4 ghost layers deep).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <cstring>
+#include <cmath>
+
+#include "gamera_net.h"
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16;
+
+#define NET_C 32      // padded channel count of every activation
+#define NET_TAPS 27
+
+// round-to-nearest-even FP32 -> BF16 (finite inputs)
+__host__ __device__ inline u16 f2bf(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (u16)(r >> 16);
+}
+__host__ __device__ inline float bf2f(u16 b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+struct NetDims { int Ni, Nj, Nk, nc, lo, ntile; int off[NET_TAPS]; };
+
+__device__ __forceinline__ bf16x8 ld8(const u16 *p) {   // 16-byte aligned
+    return *reinterpret_cast<const bf16x8 *>(p);
+}
+
+// ----------------------------------------------------------------- convolution
+// MODE 0: ReLU, write act_out (channels last) and actT_out (channel-major), NCO = 32
+// MODE 1: linear output, write out[p][0..7] FP32 (NCO = 16, channels 8..15 padding)
+// MODE 2: backward data: multiply by ReLU'(mask act), write both BF16 layouts, NCO = 32
+// One wave = 16 cells x NCO channels; blockDim 256 = 4 waves.
+template <int NCO, int MODE>
+__global__ void __launch_bounds__(256) k_conv(NetDims d, const u16 *__restrict__ in, const u16 *__restrict__ W,
+                                              const float *__restrict__ bias, const u16 *__restrict__ mask,
+                                              u16 *__restrict__ act_out, u16 *__restrict__ actT_out,
+                                              float *__restrict__ out) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile >= d.ntile) return;                       // whole waves exit together
+    const int p0 = d.lo + 16 * tile;
+    const int row = lane & 15, kg = lane >> 4;         // A row (cell), K group of 8 channels
+    f32x4 acc[NCO / 16];
+#pragma unroll
+    for (int t = 0; t < NCO / 16; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < NET_TAPS; tap++) {
+        const int q = p0 + row + d.off[tap];
+        const bf16x8 a = ld8(in + (size_t)q * NET_C + 8 * kg);
+#pragma unroll
+        for (int t = 0; t < NCO / 16; t++) {
+            const int co = 16 * t + row;
+            const bf16x8 b = ld8(W + ((size_t)tap * NCO + co) * NET_C + 8 * kg);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        }
+    }
+    // D layout: lane holds rows 4*kg + r (cells), column row (channel)
+#pragma unroll
+    for (int t = 0; t < NCO / 16; t++) {
+        const int co = 16 * t + row;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int p = p0 + 4 * kg + r;
+            float v = acc[t][r];
+            if (MODE == 0) {
+                v = v + bias[co];
+                v = v > 0.f ? v : 0.f;
+                const u16 b = f2bf(v);
+                act_out[(size_t)p * NET_C + co] = b;
+                actT_out[(size_t)co * d.nc + p] = b;
+            } else if (MODE == 1) {
+                if (co < 8) out[(size_t)p * 8 + co] = v + bias[co];
+            } else {
+                if (bf2f(mask[(size_t)p * NET_C + co]) <= 0.f) v = 0.f;   // ReLU' of the forward output
+                const u16 b = f2bf(v);
+                act_out[(size_t)p * NET_C + co] = b;
+                actT_out[(size_t)co * d.nc + p] = b;
+            }
+        }
+    }
+}
+
+// Weight gradient of one layer: dW[tap][co][ci] += sum_p g[p][co] * a[p + off][ci]
+// over this block's chunk of cells (FP32 atomics into the unpadded parameter
+// layout [tap][nco][nci]).  gT is the channel-major gradient (NCO rows), aT the
+// channel-major input activation (32 rows).  One wave per (tap, chunk); K = 32
+// cells per MFMA, 8 per lane group.
+template <int NCO>
+__global__ void __launch_bounds__(64) k_wgrad(NetDims d, const u16 *__restrict__ gT, const u16 *__restrict__ aT,
+                                              int chunk, int nco, int nci, float *__restrict__ dW) {
+    const int lane = threadIdx.x;
+    const int tap = blockIdx.y;
+    const int c0 = d.lo + blockIdx.x * chunk;          // chunk is a multiple of 32
+    const int cend = min(c0 + chunk, d.lo + 16 * d.ntile);
+    const int row = lane & 15, kg = lane >> 4;
+    const int off = d.off[tap];
+    f32x4 acc[NCO / 16][2];
+#pragma unroll
+    for (int m = 0; m < NCO / 16; m++) { acc[m][0] = f32x4{0.f, 0.f, 0.f, 0.f}; acc[m][1] = f32x4{0.f, 0.f, 0.f, 0.f}; }
+    for (int c = c0; c < cend; c += 32) {
+        const int pk = c + 8 * kg;                     // 8 cells of this lane group
+        bf16x8 bv[2];
+#pragma unroll
+        for (int n = 0; n < 2; n++) {                  // B[k = cell][n = ci]: a at the shifted cell, unaligned
+            const u16 *src = aT + (size_t)(16 * n + row) * d.nc + pk + off;
+            u16 tmp[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) tmp[e] = (pk + e < cend) ? src[e] : (u16)0;
+            memcpy(&bv[n], tmp, 16);
+        }
+#pragma unroll
+        for (int m = 0; m < NCO / 16; m++) {           // A[m = co][k = cell]
+            const u16 *src = gT + (size_t)(16 * m + row) * d.nc + pk;
+            u16 tmp[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) tmp[e] = (pk + e < cend) ? src[e] : (u16)0;
+            bf16x8 av;
+            memcpy(&av, tmp, 16);
+#pragma unroll
+            for (int n = 0; n < 2; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[n], acc[m][n], 0, 0, 0);
+        }
+    }
+    // D[i = co][j = ci]: lane holds co = 16m + 4kg + r, ci = 16n + row
+#pragma unroll
+    for (int m = 0; m < NCO / 16; m++)
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int co = 16 * m + 4 * kg + r, ci = 16 * n + row;
+                if (co < nco && ci < nci) atomicAdd(&dW[((size_t)tap * nco + co) * nci + ci], acc[m][n][r]);
+            }
+}
+
+// bias gradient: db[co] += sum over the tile range of g[p][co]
+template <int NCO>
+__global__ void __launch_bounds__(256) k_bgrad(NetDims d, const u16 *__restrict__ gT, float *__restrict__ db) {
+    __shared__ float red[256];
+    const int co = blockIdx.y;
+    const int n = 16 * d.ntile;
+    float s = 0.f;
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) s += bf2f(gT[(size_t)co * d.nc + d.lo + q]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&db[co], red[0]);
+}
+
+// dL/dout (FP32 [nc][8]) -> BF16 gradient of the last layer, both layouts (16
+// padded channels); zero outside the computed cells, where the output is not defined
+__global__ void __launch_bounds__(256) k_dout(NetDims d, const float *__restrict__ dout, u16 *__restrict__ g, u16 *__restrict__ gT) {
+    const int nc = d.nc;
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nc) return;
+    const bool in = p >= d.lo && p < d.lo + 16 * d.ntile;
+    for (int c = 0; c < NET_C; c++) {
+        const u16 b = (c < 8 && in) ? f2bf(dout[(size_t)p * 8 + c]) : (u16)0;
+        g[(size_t)p * NET_C + c] = b;
+        if (c < 16) gT[(size_t)c * nc + p] = b;
+    }
+}
+
+// state -> input features ((rho - rho0)/s_rho, v/s_v, (p - p0)/s_p, B/s_B), BF16, both layouts
+template <typename R>
+__global__ void __launch_bounds__(256) k_features(int nc, const R *__restrict__ Gas, const R *__restrict__ B, float gamma,
+                                                  float4 sc, float2 off, u16 *__restrict__ act, u16 *__restrict__ actT) {
+    int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= nc) return;
+    const size_t n = nc;
+    float rho = (float)Gas[p], mx = (float)Gas[n + p], my = (float)Gas[2 * n + p], mz = (float)Gas[3 * n + p];
+    float E = (float)Gas[4 * n + p];
+    float r = fmaxf(rho, 1e-12f);
+    float vx = mx / r, vy = my / r, vz = mz / r;
+    float P = (gamma - 1.f) * (E - 0.5f * r * (vx * vx + vy * vy + vz * vz));
+    float f[8] = {(rho - off.x) / sc.x, vx / sc.y, vy / sc.y, vz / sc.y, (P - off.y) / sc.z, (float)B[p] / sc.w, (float)B[n + p] / sc.w,
+                  (float)B[2 * n + p] / sc.w};
+    for (int c = 0; c < NET_C; c++) {
+        const u16 b = c < 8 ? f2bf(f[c]) : (u16)0;
+        act[(size_t)p * NET_C + c] = b;
+        actT[(size_t)c * n + p] = b;
+    }
+}
+
+// Adam on the FP32 master parameters (Kingma & Ba 2015, bias-corrected)
+__global__ void __launch_bounds__(256) k_adam(int n, float *__restrict__ w, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v, float lr, float b1, float b2,
+                                              float eps, float c1, float c2) {
+    int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    const float gq = g[q];
+    const float mq = b1 * m[q] + (1.f - b1) * gq;
+    const float vq = b2 * v[q] + (1.f - b2) * gq * gq;
+    m[q] = mq;
+    v[q] = vq;
+    w[q] = w[q] - lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+}
+
+// master FP32 parameters -> BF16 forward and transposed/flipped backward weights
+// layer l forward: Wf[tap][co][ci]; backward data: Wb[tap'][ci][co] with tap' the mirrored tap
+__global__ void __launch_bounds__(256) k_pack(const float *__restrict__ w, int off, int nco_real, int nci_real, int nco_pad,
+                                              u16 *__restrict__ Wf, u16 *__restrict__ Wb) {
+    int q = blockIdx.x * 256 + threadIdx.x;
+    const int total = NET_TAPS * nco_pad * NET_C;
+    if (q >= total) return;
+    const int ci = q % NET_C, co = (q / NET_C) % nco_pad, tap = q / (NET_C * nco_pad);
+    const float val = (co < nco_real && ci < nci_real) ? w[off + (tap * nco_real + co) * nci_real + ci] : 0.f;
+    Wf[q] = f2bf(val);
+    // backward data kernel has NCO = 32 outputs (= ci) and 32 inputs (= co, padded)
+    const int mt = NET_TAPS - 1 - tap;
+    if (ci < NET_C && co < NET_C) Wb[((size_t)mt * NET_C + ci) * NET_C + co] = f2bf(val);
+}
+
+// ----------------------------------------------------------------- host side
+struct gam_net {
+    NetDims d;
+    hipStream_t stream = nullptr;
+    // parameters: FP32 master, grads, Adam moments
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    u16 *Wf[3] = {}, *Wb[3] = {};
+    u16 *act[4] = {}, *actT[4] = {};     // 0 features, 1..2 hidden, 3 scratch gradient
+    u16 *gr[2] = {}, *grT[2] = {};       // gradient ping-pong
+    float *out = nullptr;
+    float *dbuf = nullptr;               // upstream gradient uploaded from the host
+    float4 scales = {1.f, 1.f, 1.f, 1.f};
+    float gamma = 5.f / 3.f;
+    float2 offs = {0.f, 0.f};
+    std::string err;
+};
+
+// parameter layout (FP32): layer 0 W[27][32][8], b[32]; layer 1 W[27][32][32], b[32]; layer 2 W[27][8][32], b[8]
+static const int kCo[3] = {32, 32, 8}, kCi[3] = {8, 32, 32}, kCoPad[3] = {32, 32, 16};
+static int woff(int l) { int o = 0; for (int q = 0; q < l; q++) o += NET_TAPS * kCo[q] * kCi[q] + kCo[q]; return o; }
+static int boff(int l) { return woff(l) + NET_TAPS * kCo[l] * kCi[l]; }
+
+extern "C" int gam_net_num_params(void) { return woff(3); }
+
+static int nfail(gam_net *n, int code, const char *msg, hipError_t e) {
+    if (n) n->err = std::string(msg) + ": " + hipGetErrorString(e);
+    return code;
+}
+#define NCHK(n, x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return nfail(n, -2, #x, e_); } while (0)
+
+extern "C" int gam_net_destroy(gam_net *n) {
+    if (!n) return 0;
+    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    void *bufs[] = {n->w, n->g, n->m, n->v, n->out, n->dbuf};
+    for (void *b : bufs) if (b) (void)hipFree(b);
+    for (int l = 0; l < 3; l++) { (void)hipFree(n->Wf[l]); (void)hipFree(n->Wb[l]); }
+    for (int q = 0; q < 4; q++) { (void)hipFree(n->act[q]); (void)hipFree(n->actT[q]); }
+    for (int q = 0; q < 2; q++) { (void)hipFree(n->gr[q]); (void)hipFree(n->grT[q]); }
+    if (n->stream) (void)hipStreamDestroy(n->stream);
+    delete n;
+    return 0;
+}
+
+extern "C" int gam_net_create(int Ni, int Nj, int Nk, gam_net **out) {
+    if (!out || Ni < 8 || Nj < 8 || Nk < 8) return -1;   // the 3-layer stencil needs 3 cells around each used cell
+    *out = nullptr;
+    long nc = (long)Ni * Nj * Nk;
+    if (nc * NET_C > 0x7fffffffL) return -1;
+    gam_net *n = new gam_net();
+    NetDims &d = n->d;
+    d.Ni = Ni; d.Nj = Nj; d.Nk = Nk; d.nc = (int)nc;
+    const int O = Ni * Nj + Ni + 1;
+    d.lo = O;
+    d.ntile = (int)((nc - 2L * O) / 16);
+    for (int t = 0; t < NET_TAPS; t++) {
+        int dx = t % 3 - 1, dy = (t / 3) % 3 - 1, dz = t / 9 - 1;
+        d.off[t] = dx + Ni * (dy + Nj * dz);
+    }
+    auto cleanup = [&](int c) { gam_net_destroy(n); return c; };
+    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(-2);
+    const int np = gam_net_num_params();
+    for (float **p : {&n->w, &n->g, &n->m, &n->v}) {
+        if (hipMalloc(p, np * sizeof(float)) != hipSuccess) return cleanup(-3);
+        if (hipMemset(*p, 0, np * sizeof(float)) != hipSuccess) return cleanup(-2);
+    }
+    for (int l = 0; l < 3; l++) {
+        if (hipMalloc(&n->Wf[l], (size_t)NET_TAPS * kCoPad[l] * NET_C * 2) != hipSuccess) return cleanup(-3);
+        if (hipMalloc(&n->Wb[l], (size_t)NET_TAPS * NET_C * NET_C * 2) != hipSuccess) return cleanup(-3);
+        if (hipMemset(n->Wb[l], 0, (size_t)NET_TAPS * NET_C * NET_C * 2) != hipSuccess) return cleanup(-2);
+    }
+    const size_t ab = (size_t)nc * NET_C * 2;
+    for (int q = 0; q < 4; q++) {
+        if (hipMalloc(&n->act[q], ab) != hipSuccess || hipMalloc(&n->actT[q], ab) != hipSuccess) return cleanup(-3);
+        if (hipMemset(n->act[q], 0, ab) != hipSuccess || hipMemset(n->actT[q], 0, ab) != hipSuccess) return cleanup(-2);
+    }
+    for (int q = 0; q < 2; q++) {
+        if (hipMalloc(&n->gr[q], ab) != hipSuccess || hipMalloc(&n->grT[q], ab) != hipSuccess) return cleanup(-3);
+        if (hipMemset(n->gr[q], 0, ab) != hipSuccess || hipMemset(n->grT[q], 0, ab) != hipSuccess) return cleanup(-2);
+    }
+    if (hipMalloc(&n->out, (size_t)nc * 8 * sizeof(float)) != hipSuccess) return cleanup(-3);
+    if (hipMalloc(&n->dbuf, (size_t)nc * 8 * sizeof(float)) != hipSuccess) return cleanup(-3);
+    if (hipMemset(n->out, 0, (size_t)nc * 8 * sizeof(float)) != hipSuccess) return cleanup(-2);
+    *out = n;
+    return 0;
+}
+
+extern "C" const char *gam_net_error(const gam_net *n) { return n ? n->err.c_str() : "null net"; }
+extern "C" void *gam_net_stream(gam_net *n) { return n ? (void *)n->stream : nullptr; }
+extern "C" const float *gam_net_output(gam_net *n) { return n ? n->out : nullptr; }
+extern "C" float *gam_net_grad_buffer(gam_net *n) { return n ? n->dbuf : nullptr; }
+
+static int pack(gam_net *n) {
+    for (int l = 0; l < 3; l++) {
+        const int total = NET_TAPS * kCoPad[l] * NET_C;
+        hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), 256, 0, n->stream, (const float *)n->w, woff(l), kCo[l], kCi[l],
+                           kCoPad[l], n->Wf[l], n->Wb[l]);
+    }
+    NCHK(n, hipGetLastError());
+    return 0;
+}
+
+extern "C" int gam_net_set_params(gam_net *n, const float *w) {
+    if (!n || !w) return -1;
+    NCHK(n, hipMemcpyAsync(n->w, w, gam_net_num_params() * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    int rc = pack(n);
+    if (rc) return rc;
+    NCHK(n, hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+extern "C" int gam_net_get_params(gam_net *n, float *w, float *grad) {
+    if (!n) return -1;
+    if (w) NCHK(n, hipMemcpyAsync(w, n->w, gam_net_num_params() * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    if (grad) NCHK(n, hipMemcpyAsync(grad, n->g, gam_net_num_params() * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    NCHK(n, hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+extern "C" int gam_net_set_scales(gam_net *n, float rho, float v, float p, float b, float gamma) {
+    if (!n || !(rho > 0) || !(v > 0) || !(p > 0) || !(b > 0)) return -1;
+    n->scales = make_float4(rho, v, p, b);
+    n->gamma = gamma;
+    return 0;
+}
+
+extern "C" int gam_net_set_offsets(gam_net *n, float rho0, float p0) {
+    if (!n) return -1;
+    n->offs = make_float2(rho0, p0);
+    return 0;
+}
+
+extern "C" int gam_net_features(gam_net *n, const void *Gas, const void *B, int precision) {
+    if (!n || !Gas || !B) return -1;
+    const int nb = (n->d.nc + 255) / 256;
+    if (precision) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_features<float>), dim3(nb), 256, 0, n->stream, n->d.nc, (const float *)Gas,
+                                      (const float *)B, n->gamma, n->scales, n->offs, n->act[0], n->actT[0]);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_features<double>), dim3(nb), 256, 0, n->stream, n->d.nc, (const double *)Gas,
+                            (const double *)B, n->gamma, n->scales, n->offs, n->act[0], n->actT[0]);
+    NCHK(n, hipGetLastError());
+    return 0;
+}
+
+extern "C" int gam_net_set_features(gam_net *n, const float *feat) {
+    // host features [nc][8] FP32 (tests): rounded to BF16 like k_features
+    if (!n || !feat) return -1;
+    const size_t nc = n->d.nc;
+    std::vector<u16> a(nc * NET_C, 0), aT(nc * NET_C, 0);
+    for (size_t p = 0; p < nc; p++)
+        for (int c = 0; c < 8; c++) { u16 b = f2bf(feat[p * 8 + c]); a[p * NET_C + c] = b; aT[c * nc + p] = b; }
+    NCHK(n, hipMemcpyAsync(n->act[0], a.data(), a.size() * 2, hipMemcpyHostToDevice, n->stream));
+    NCHK(n, hipMemcpyAsync(n->actT[0], aT.data(), aT.size() * 2, hipMemcpyHostToDevice, n->stream));
+    NCHK(n, hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+static dim3 conv_grid(const gam_net *n) { return dim3((unsigned)((n->d.ntile + 3) / 4)); }
+
+extern "C" int gam_net_forward(gam_net *n) {
+    if (!n) return -1;
+    const NetDims &d = n->d;
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 0>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[0], (const u16 *)n->Wf[0],
+                       (const float *)(n->w + boff(0)), (const u16 *)nullptr, n->act[1], n->actT[1], (float *)nullptr);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 0>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[1], (const u16 *)n->Wf[1],
+                       (const float *)(n->w + boff(1)), (const u16 *)nullptr, n->act[2], n->actT[2], (float *)nullptr);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<16, 1>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[2], (const u16 *)n->Wf[2],
+                       (const float *)(n->w + boff(2)), (const u16 *)nullptr, (u16 *)nullptr, (u16 *)nullptr, n->out);
+    NCHK(n, hipGetLastError());
+    return 0;
+}
+
+// dout: device FP32 [nc][8] = dL/d(out).  Accumulates into the gradient buffer
+// (gam_net_zero_grad clears it).  Uses the activations of the last forward.
+extern "C" int gam_net_backward(gam_net *n, const float *dout) {
+    if (!n || !dout) return -1;
+    const NetDims &d = n->d;
+    const int chunk = 2048;
+    const dim3 wg((unsigned)((16 * d.ntile + chunk - 1) / chunk), NET_TAPS);
+    hipLaunchKernelGGL(k_dout, dim3((d.nc + 255) / 256), 256, 0, n->stream, d, dout, n->gr[0], n->grT[0]);
+    // layer 2 (linear, 8 outputs padded to 16)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<16>), wg, 64, 0, n->stream, d, (const u16 *)n->grT[0], (const u16 *)n->actT[2], chunk,
+                       kCo[2], kCi[2], n->g + woff(2));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgrad<16>), dim3(64, 8), 256, 0, n->stream, d, (const u16 *)n->grT[0], n->g + boff(2));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 2>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->gr[0], (const u16 *)n->Wb[2],
+                       (const float *)nullptr, (const u16 *)n->act[2], n->gr[1], n->grT[1], (float *)nullptr);
+    // layer 1
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<32>), wg, 64, 0, n->stream, d, (const u16 *)n->grT[1], (const u16 *)n->actT[1], chunk,
+                       kCo[1], kCi[1], n->g + woff(1));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgrad<32>), dim3(64, 32), 256, 0, n->stream, d, (const u16 *)n->grT[1], n->g + boff(1));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 2>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->gr[1], (const u16 *)n->Wb[1],
+                       (const float *)nullptr, (const u16 *)n->act[1], n->gr[0], n->grT[0], (float *)nullptr);
+    // layer 0 (input gradient not needed)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<32>), wg, 64, 0, n->stream, d, (const u16 *)n->grT[0], (const u16 *)n->actT[0], chunk,
+                       kCo[0], kCi[0], n->g + woff(0));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgrad<32>), dim3(64, 32), 256, 0, n->stream, d, (const u16 *)n->grT[0], n->g + boff(0));
+    NCHK(n, hipGetLastError());
+    return 0;
+}
+
+extern "C" int gam_net_zero_grad(gam_net *n) {
+    if (!n) return -1;
+    NCHK(n, hipMemsetAsync(n->g, 0, gam_net_num_params() * sizeof(float), n->stream));
+    return 0;
+}
+
+extern "C" int gam_net_adam(gam_net *n, float lr, float b1, float b2, float eps, int t) {
+    if (!n || t < 1) return -1;
+    const int np = gam_net_num_params();
+    const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
+    hipLaunchKernelGGL(k_adam, dim3((np + 255) / 256), 256, 0, n->stream, np, n->w, (const float *)n->g, n->m, n->v, lr, b1, b2,
+                       eps, c1, c2);
+    NCHK(n, hipGetLastError());
+    return pack(n);
+}
+
+extern "C" int gam_net_sync(gam_net *n) {
+    if (!n) return -1;
+    NCHK(n, hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+extern "C" int gam_net_get_output(gam_net *n, float *out) {
+    if (!n || !out) return -1;
+    NCHK(n, hipMemcpyAsync(out, n->out, (size_t)n->d.nc * 8 * sizeof(float), hipMemcpyDeviceToHost, n->stream));
+    NCHK(n, hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+// channels-last BF16 activations of the last forward (layer 0 = features), for tests
+extern "C" int gam_net_get_act(gam_net *n, int layer, unsigned short *out) {
+    if (!n || !out || layer < 0 || layer > 2) return -1;
+    NCHK(n, hipMemcpyAsync(out, n->act[layer], (size_t)n->d.nc * NET_C * 2, hipMemcpyDeviceToHost, n->stream));
+    NCHK(n, hipStreamSynchronize(n->stream));
+    return 0;
+}
+
+extern "C" int gam_net_backward_host(gam_net *n, const float *dout) {
+    if (!n || !dout) return -1;
+    NCHK(n, hipMemcpyAsync(n->dbuf, dout, (size_t)n->d.nc * 8 * sizeof(float), hipMemcpyHostToDevice, n->stream));
+    return gam_net_backward(n, n->dbuf);
+}

@@ -588,6 +588,24 @@ __device__ __forceinline__ void block_min_store(double v, double *partial) {
     }
 }
 
+// block sum over 256 threads -> partial[blockIdx.x]
+__device__ __forceinline__ void block_sum_store(double v, double *partial) {
+#ifdef GAM_EMU
+    partial[blockIdx.x] = threadIdx.x == 0 ? v : partial[blockIdx.x] + v;
+    return;
+#endif
+    __shared__ double reds[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) reds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = reds[0];
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) r += reds[q];
+        partial[blockIdx.x] = r;
+    }
+}
+
 template <typename R> __device__ __forceinline__ bool in_dt_box(const GVT<R> &g, int i, int j, int k) {
     // SetDomain init.F90:519-538: active +-1 layer (single block owns all BCs)
     const int a0 = g.nG - 1;
@@ -841,4 +859,169 @@ __global__ void k_divb(GV g, const double *__restrict__ mag, double *__restrict_
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ----------------------------------------------------------------- learned correction
+// A correction field o[c][8] (5 gas-flux and 3 edge-EMF channels per cell,
+// FP32) adds scale*face_d(c)*o[c][v] to the gas flux v of every active
+// interface d at cell c and scale*edge_d(c)*o[c][5+d] to every active edge EMF
+// d at cell c (the edge at the cell's lower corner), before CT and the update:
+// the update stays conservative and E2Flux keeps div.B exact.  A ghost cell
+// uses the o of its periodic image (the step's boundaries are periodic), so
+// the two copies of a boundary face or edge get one correction.
+template <typename R> __device__ __forceinline__ bool flux_box(const GVT<R> &g, int do25D, int d, int i, int j, int k) {
+    const int a0 = g.nG;
+    const int nk = do25D ? 1 : g.Nkp;
+    return i >= a0 && i < a0 + g.Nip + (d == 0) && j >= a0 && j < a0 + g.Njp + (d == 1) && k >= a0 &&
+           k < a0 + nk + (d == 2) && !(do25D && d == 2);
+}
+template <typename R> __device__ __forceinline__ bool edge_active(const GVT<R> &g, int d, int i, int j, int k) {
+    const int a0 = g.nG;
+    return i >= a0 && i <= a0 + g.Nip - (d == 0) && j >= a0 && j <= a0 + g.Njp - (d == 1) && k >= a0 &&
+           k <= a0 + g.Nkp - (d == 2);
+}
+__device__ __forceinline__ int pwrap(int x, int a0, int n) { return a0 + ((x - a0) % n + n) % n; }
+template <typename R> __device__ __forceinline__ size_t pimg(const GVT<R> &g, int i, int j, int k) {
+    return (size_t)ci(g, pwrap(i, g.nG, g.Nip), pwrap(j, g.nG, g.Njp), pwrap(k, g.nG, g.Nkp));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_apply_corr(GVT<R> g, const DClock *__restrict__ clk, int do25D,
+                                                    const float *__restrict__ o, double s, R *__restrict__ Flx,
+                                                    R *__restrict__ E) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc || clk->halt) return;
+    int i, j, k;
+    decode_c(g, idx, i, j, k);
+    const size_t nc = g.nc, nf = g.nf;
+    const int f = fi(g, i, j, k);
+    const float *oc = o + pimg(g, i, j, k) * 8;
+    for (int d = 0; d < 3; d++) {
+        if (flux_box(g, do25D, d, i, j, k)) {
+            const double A = (double)g.face[nf * d + f];
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                const size_t q = nc * (8 * d + v) + idx;
+                Flx[q] = R((double)Flx[q] + s * A * (double)oc[v]);
+            }
+        }
+        if (edge_active(g, d, i, j, k)) {
+            const size_t q = nc * d + idx;
+            E[q] = R((double)E[q] + s * (double)g.edge[nf * d + f] * (double)oc[5 + d]);
+        }
+    }
+}
+
+// Training: the change the correction o makes to one step, linearised
+// (U += dt*dGH for the gas, magFlux -= dt*curl(E)), against a target change.
+// loss = sum_cells sum_v wG[v] (dU_v - tG_v)^2 + sum_faces wM (dPhi - tM)^2 over
+// the active cells and the active faces (face index along its normal in
+// [is, ie+1], the others in [is, ie]).  Pass 1: cell residuals rG = 2 wG (dU - tG) / dV.
+template <typename R>
+__global__ void __launch_bounds__(256) k_corr_cells(GVT<R> g, int do25D, const float *__restrict__ o, double s, double dt,
+                                                    const double *__restrict__ tG, double w0, double w1, double w4,
+                                                    double *__restrict__ rG, double *__restrict__ partial) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (idx < g.nc) {
+        int i, j, k;
+        decode_c(g, idx, i, j, k);
+        const size_t nc = g.nc, nf = g.nf;
+        const int a0 = g.nG;
+        const bool active = i >= a0 && i < a0 + g.Nip && j >= a0 && j < a0 + g.Njp && k >= a0 && k < a0 + g.Nkp;
+        const double wv[5] = {w0, w1, w1, w1, w4};
+        for (int v = 0; v < 5; v++) {
+            double r = 0.0;
+            if (active) {
+                const double V = g.dvol[idx];
+                double sum = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    if (do25D && d == 2) continue;
+                    const int f = fi(g, i, j, k), fs = fstride(g, d);
+                    const size_t up = pimg(g, i + (d == 0), j + (d == 1), k + (d == 2));
+                    sum += s * (double)g.face[nf * d + f] * (double)o[(size_t)idx * 8 + v]
+                         - s * (double)g.face[nf * d + f + fs] * (double)o[up * 8 + v];
+                }
+                const double dU = dt * sum / V;
+                const double e = dU - tG[nc * v + idx];
+                l += wv[v] * e * e;
+                r = 2.0 * wv[v] * e / V;
+            }
+            rG[nc * v + idx] = r;
+        }
+    }
+    block_sum_store(l, partial);
+}
+
+// Pass 2: face residuals rM = 2 wM (dPhi - tM) on the active faces
+template <typename R>
+__global__ void __launch_bounds__(256) k_corr_faces(GVT<R> g, const float *__restrict__ o, double s, double dt,
+                                                    const double *__restrict__ tM, double wM, double *__restrict__ rM,
+                                                    double *__restrict__ partial) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (idx < g.nf) {
+        int i, j, k;
+        decode_f(g, idx, i, j, k);
+        const size_t nf = g.nf;
+        const int a0 = g.nG;
+        auto dE = [&](int d, int a, int b, int c) -> double {   // correction EMF on an active edge
+            if (!edge_active(g, d, a, b, c)) return 0.0;
+            return s * (double)g.edge[nf * d + fi(g, a, b, c)] * (double)o[pimg(g, a, b, c) * 8 + 5 + d];
+        };
+        for (int d = 0; d < 3; d++) {
+            double r = 0.0;
+            if (i >= a0 && i < a0 + g.Nip + (d == 0) && j >= a0 && j < a0 + g.Njp + (d == 1) && k >= a0 &&
+                k < a0 + g.Nkp + (d == 2)) {
+                double df;
+                if (d == 0) df = dE(2, i, j + 1, k) - dE(2, i, j, k) - dE(1, i, j, k + 1) + dE(1, i, j, k);
+                else if (d == 1) df = dE(0, i, j, k + 1) - dE(0, i, j, k) - dE(2, i + 1, j, k) + dE(2, i, j, k);
+                else df = dE(1, i + 1, j, k) - dE(1, i, j, k) - dE(0, i, j + 1, k) + dE(0, i, j, k);
+                const double e = -dt * df - tM[nf * d + idx];
+                l += wM * e * e;
+                r = 2.0 * wM * e;
+            }
+            rM[nf * d + idx] = r;
+        }
+    }
+    block_sum_store(l, partial);
+}
+
+// Pass 3: dL/do per cell (adjoint of passes 1-2).  An active cell gathers the
+// uses of itself and of its periodic images; a ghost cell's o is never read.
+template <typename R>
+__global__ void __launch_bounds__(256) k_corr_adjoint(GVT<R> g, int do25D, double s, double dt,
+                                                      const double *__restrict__ rG, const double *__restrict__ rM,
+                                                      float *__restrict__ dout) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc) return;
+    int i0, j0, k0;
+    decode_c(g, idx, i0, j0, k0);
+    const size_t nc = g.nc, nf = g.nf;
+    const int a0 = g.nG;
+    double go[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const bool active = i0 >= a0 && i0 < a0 + g.Nip && j0 >= a0 && j0 < a0 + g.Njp && k0 >= a0 && k0 < a0 + g.Nkp;
+    const int fI = fstride(g, 0), fJ = fstride(g, 1), fK = fstride(g, 2);
+    for (int q = 0; active && q < 8; q++) {
+        const int i = i0 + (q & 1) * g.Nip, j = j0 + ((q >> 1) & 1) * g.Njp, k = k0 + (q >> 2) * g.Nkp;
+        if (i >= g.Ni || j >= g.Nj || k >= g.Nk) continue;
+        const int c = ci(g, i, j, k), f = fi(g, i, j, k);
+        for (int d = 0; d < 3; d++) {
+            if (!flux_box(g, do25D, d, i, j, k)) continue;
+            const int st = cstride(g, d);
+            const double A = (double)g.face[nf * d + f];
+            // the face is the lower face of cell c and the upper face of c - st
+            for (int v = 0; v < 5; v++) go[v] += s * A * dt * (rG[nc * v + c] - rG[nc * v + c - st]);
+        }
+        for (int d = 0; d < 3; d++) {
+            if (!edge_active(g, d, i, j, k)) continue;
+            const double *R0 = rM, *R1 = rM + nf, *R2 = rM + 2 * nf;
+            double gE;
+            if (d == 0) gE = -dt * (R1[f - fK] - R1[f] - R2[f - fJ] + R2[f]);
+            else if (d == 1) gE = -dt * (-R0[f - fK] + R0[f] + R2[f - fI] - R2[f]);
+            else gE = -dt * (R0[f - fJ] - R0[f] - R1[f - fI] + R1[f]);
+            go[5 + d] += s * (double)g.edge[nf * d + f] * gE;
+        }
+    }
+    for (int c = 0; c < 8; c++) dout[(size_t)idx * 8 + c] = (float)go[c];
 }

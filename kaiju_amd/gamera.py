@@ -105,6 +105,12 @@ def lib():
         "gam_graph_sync": (C.c_int, [C.c_void_p]),
         "gam_graph_destroy": (C.c_int, [C.c_void_p]),
         "gam_graph_error": (C.c_char_p, [C.c_void_p]),
+        "gam_state_device": (C.c_int, [H, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_void_p)]),
+        "gam_set_correction": (C.c_int, [H, C.c_void_p, C.c_double]),
+        "gam_correction_target": (C.c_int, [H, _dp, _dp]),
+        "gam_correction_grad": (C.c_int, [H, C.c_void_p, C.c_double, C.c_double, _dp, C.c_void_p,
+                                          C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -133,7 +139,7 @@ def params_from_spec(spec: configs.CaseSpec) -> GamParams:
     p.doMHD = int(m.doMHD)
     p.do25D = int(spec.do25D)
     p.doCPR = int(m.doCPR)
-    p.fixedTimestep = 0
+    p.fixedTimestep = int(m.fixedTimestep)
     p.precision = int(m.precision)
     return p
 
@@ -347,6 +353,33 @@ class GamApp:
         while st == 0 and self.Model.t < target:
             st = self.stepGamera()
         return st
+
+    # ---------------------------------------------------------------- learned correction
+    def state_device(self, which: int = 0):
+        """Device addresses (Gas, magFlux, Bxyz) of State (0) or oState (1)."""
+        g, m, b = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        self._chk(lib().gam_state_device(self.h, which, C.byref(g), C.byref(m), C.byref(b)), "gam_state_device")
+        return g.value, m.value, b.value
+
+    def set_correction(self, o_ptr: Optional[int], scale: float = 1.0):
+        """Apply the device FP32 correction field o [G][8] in every following step (None: off)."""
+        self._chk(lib().gam_set_correction(self.h, o_ptr, float(scale)), "gam_set_correction")
+
+    def correction_target(self, dGas: np.ndarray, dMag: np.ndarray):
+        """Target one-step change (Gas (G,5), magFlux (F,3)) for correction_grad."""
+        dGas = np.asfortranarray(dGas, dtype=np.float64)
+        dMag = np.asfortranarray(dMag, dtype=np.float64)
+        assert dGas.shape == self.G + (5,) and dMag.shape == self.F + (3,)
+        self._chk(lib().gam_correction_target(self.h, _p(dGas), _p(dMag)), "gam_correction_target")
+
+    def correction_grad(self, o_ptr: int, scale: float, dt: float, weights, dout_ptr: int) -> float:
+        """Loss of the linearised one-step change of o against the target; dL/do into dout_ptr."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        assert w.size == 4
+        loss = C.c_double()
+        self._chk(lib().gam_correction_grad(self.h, o_ptr, float(scale), float(dt), _p(w), dout_ptr, C.byref(loss)),
+                  "gam_correction_grad")
+        return loss.value
 
     def get_work(self, which: int) -> np.ndarray:
         """Solver work arrays of the last step: 0 fluxes (G,8,3), 1 EMFs (G,3), 2 Vf (G,3,3)."""

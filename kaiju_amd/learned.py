@@ -1,0 +1,468 @@
+"""Learned coarse-grid correction: training data, GPU training and rollouts.
+
+The network of ``libgamera_net.so`` (three 3x3x3 convolutions, 8 -> 32 -> 32
+-> 8 channels, BF16 MFMA with FP32 accumulation) maps the features of a coarse
+state (rho, v, p, B) to a correction field o[G][8].  ``gam_set_correction``
+makes every following step add ``scale * face * o[0:5]`` to the gas fluxes of
+each active interface and ``scale * edge * o[5:8]`` to each active edge EMF,
+before E2Flux and the update: a corrected step stays conservative and keeps
+div.B at machine zero.
+
+Training data are the solver's own fine trajectories.  Decaying MHD
+turbulence from seeded Fourier modes (``TurbIC``) on the periodic unit cube
+at (2N)^3 cells is stepped with a fixed dt/2 and every second state is
+downsampled 2x: conserved variables averaged over 2x2x2 fine cells, face
+fluxes summed over 2x2 fine faces (both exact on the Cartesian grid, so the
+coarse magFlux keeps div.B = 0).  The target of the coarse step from sample n
+is the downsampled fine state at n+1 minus the plain coarse step from the
+downsampled states n (and n-1 as the AB2 predictor's old state).  The loss is
+that of the linearised one-step change that o makes (``gam_correction_grad``:
+exact for density, momentum and magFlux, first order for the energy), and its
+gradient runs back through the network (``gam_net_backward``) into Adam
+(``gam_net_adam``); every stage runs on the GPU.
+
+    python -m kaiju_amd.learned [--N 32] [--train-seeds 0,1,2] [--test-seed 7] [--steps 24] [--iters 1500]
+
+prints one JSON line with the plain and corrected rollout errors against the
+downsampled fine run of the held-out seed.
+"""
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import math
+import sys
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from kaiju_amd import configs
+from kaiju_amd.gamera import GamApp
+
+GAMMA = 5.0 / 3.0
+PRIM = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
+
+
+def turb_spec(N: int, precision: int = 0, nG: int = 4) -> configs.CaseSpec:
+    """Periodic unit cube, N^3 cells, fixed time step (set by the caller)."""
+    m = configs.ModelSpec(gamma=GAMMA, CFL=0.3, fixedTimestep=True, precision=precision)
+    return configs.CaseSpec(name=f"turb{N}", N=(N, N, N), bounds=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), grid="cart",
+                            ic="turb", model=m, nG=nG)
+
+
+# ----------------------------------------------------------------- initial state
+@dataclasses.dataclass
+class TurbIC:
+    """rho = 1, p = 1, v and the vector potential A sums of `modes` seeded
+    cosine modes with integer wave numbers |n_x|,|n_y|,|n_z| <= kmax, scaled to
+    the rms v_rms and rms(curl A) = b_rms, plus a uniform field b_mean."""
+    seed: int
+    kmax: int = 3
+    modes: int = 10
+    v_rms: float = 0.8
+    b_rms: float = 0.5
+    b_mean: Sequence[float] = (0.4, 0.0, 0.0)
+
+    def _draw(self, rng):
+        out = []
+        while len(out) < self.modes:
+            n = rng.integers(-self.kmax, self.kmax + 1, size=3)
+            if not n.any():
+                continue
+            out.append((2.0 * math.pi * n.astype(np.float64), rng.normal(size=3), rng.uniform(0, 2 * math.pi)))
+        return out
+
+    def fields(self, N: int, nG: int):
+        """Gas (G,5) and magFlux (F,3), Fortran order, ghost cells holding the
+        periodic images of the active ones."""
+        rng = np.random.default_rng(self.seed)
+        vmodes, amodes = self._draw(rng), self._draw(rng)
+        dx = 1.0 / N
+        Ni = N + 2 * nG
+        xc = (np.arange(Ni) - nG + 0.5) * dx           # cell centres
+        xe = (np.arange(Ni + 1) - nG) * dx             # corners
+        X, Y, Z = np.meshgrid(xc, xc, xc, indexing="ij")
+        v = np.zeros((3,) + X.shape)
+        for k, a, ph in vmodes:
+            v += a[:, None, None, None] * np.cos(k[0] * X + k[1] * Y + k[2] * Z + ph)[None]
+        v *= self.v_rms / math.sqrt(np.mean(np.sum(v[:, nG:-nG, nG:-nG, nG:-nG] ** 2, axis=0)))
+        # edge integrals of A: L[d][i,j,k] = integral of A_d along the d-edge from corner (i,j,k)
+        Xe, Ye, Ze = np.meshgrid(xe, xe, xe, indexing="ij")
+        L = np.zeros((3,) + Xe.shape)
+        for k, a, ph in amodes:
+            ph0 = k[0] * Xe + k[1] * Ye + k[2] * Ze + ph
+            for d in range(3):
+                if abs(k[d]) > 0:
+                    L[d] += a[d] * (np.sin(ph0 + k[d] * dx) - np.sin(ph0)) / k[d]
+                else:
+                    L[d] += a[d] * np.cos(ph0) * dx
+        F = L.shape[1]
+        mag = np.zeros((F, F, F, 3), order="F")
+        s = lambda a, di, dj, dk: np.roll(a, (-di, -dj, -dk), axis=(0, 1, 2))   # a[i+di, j+dj, k+dk]
+        mag[..., 0] = L[1] + s(L[2], 0, 1, 0) - s(L[1], 0, 0, 1) - L[2]
+        mag[..., 1] = s(L[0], 0, 0, 1) - L[0] - s(L[2], 1, 0, 0) + L[2]
+        mag[..., 2] = s(L[1], 1, 0, 0) - L[1] - s(L[0], 0, 1, 0) + L[0]
+        a0, a1 = nG, nG + N
+        brms = math.sqrt(np.mean(mag[a0:a1, a0:a1, a0:a1] ** 2)) / dx ** 2 * math.sqrt(3.0)
+        mag *= self.b_rms / brms
+        for d in range(3):
+            mag[..., d] += self.b_mean[d] * dx * dx
+        mag = wrap_faces(mag[a0:a1, a0:a1, a0:a1], nG)   # the rolled edge layer is invalid: rebuild ghosts
+        v = np.moveaxis(v[:, a0:a1, a0:a1, a0:a1], 0, -1)
+        gas = np.empty((N, N, N, 5))
+        gas[..., 0] = 1.0
+        gas[..., 1:4] = v
+        gas[..., 4] = 1.0 / (GAMMA - 1.0) + 0.5 * np.sum(v ** 2, axis=-1)
+        return wrap_cells(gas, nG), mag
+
+    def max_speed(self) -> float:
+        """Upper estimate of |v| + fast speed for the time step."""
+        b = self.b_rms * 2.5 + float(np.linalg.norm(self.b_mean))
+        return self.v_rms * 2.5 + math.sqrt(GAMMA + b * b)
+
+
+# ----------------------------------------------------------------- 2x downsampling
+def wrap_cells(act: np.ndarray, nG: int) -> np.ndarray:
+    """Active cells (N,N,N,...) -> ghost-inclusive periodic array (Fortran order)."""
+    N = act.shape[0]
+    m = (np.arange(N + 2 * nG) - nG) % N
+    return np.asfortranarray(act[np.ix_(m, m, m)])
+
+
+def wrap_faces(act: np.ndarray, nG: int) -> np.ndarray:
+    """Active faces (N,N,N,3) (face index = lower face of the cell) -> (N+2nG+1)^3 x 3."""
+    N = act.shape[0]
+    m = (np.arange(N + 2 * nG + 1) - nG) % N
+    return np.asfortranarray(act[np.ix_(m, m, m)])
+
+
+def downsample(gas: np.ndarray, mag: np.ndarray, nG: int):
+    """Fine (2N)^3 state -> coarse N^3 state, ghosts by periodic wrap."""
+    Nf = gas.shape[0] - 2 * nG
+    N = Nf // 2
+    g = gas[nG:nG + Nf, nG:nG + Nf, nG:nG + Nf]
+    gc = g.reshape(N, 2, N, 2, N, 2, 5).mean(axis=(1, 3, 5))
+    f = mag[nG:nG + Nf, nG:nG + Nf, nG:nG + Nf]
+    mc = np.empty((N, N, N, 3))
+    mc[..., 0] = f[0::2, :, :, 0].reshape(N, N, 2, N, 2).sum(axis=(2, 4))
+    mc[..., 1] = f[:, 0::2, :, 1].reshape(N, 2, N, N, 2).sum(axis=(1, 4))
+    mc[..., 2] = f[:, :, 0::2, 2].reshape(N, 2, N, 2, N).sum(axis=(1, 3))
+    return wrap_cells(gc, nG), wrap_faces(mc, nG)
+
+
+def primitives(gas: np.ndarray, B: np.ndarray, nG: int) -> Dict[str, np.ndarray]:
+    """Active-cell rho, v, p, B."""
+    a = slice(nG, gas.shape[0] - nG)
+    g = gas[a, a, a]
+    rho = g[..., 0]
+    v = g[..., 1:4] / rho[..., None]
+    p = (GAMMA - 1.0) * (g[..., 4] - 0.5 * rho * np.sum(v * v, axis=-1))
+    b = B[a, a, a]
+    return dict(rho=rho, vx=v[..., 0], vy=v[..., 1], vz=v[..., 2], p=p, bx=b[..., 0], by=b[..., 1], bz=b[..., 2])
+
+
+def cell_b(mag: np.ndarray, nG: int, dx: float) -> np.ndarray:
+    """Cell-centred B of a Cartesian-grid magFlux (mean of the two faces / area)."""
+    N = mag.shape[0] - 1
+    B = np.empty((N, N, N, 3))
+    B[..., 0] = 0.5 * (mag[:-1, :-1, :-1, 0] + mag[1:, :-1, :-1, 0])
+    B[..., 1] = 0.5 * (mag[:-1, :-1, :-1, 1] + mag[:-1, 1:, :-1, 1])
+    B[..., 2] = 0.5 * (mag[:-1, :-1, :-1, 2] + mag[:-1, :-1, 1:, 2])
+    return B / (dx * dx)
+
+
+def l1_errors(gas, mag, ref_gas, ref_mag, nG: int, dx: float) -> Dict[str, float]:
+    """Mean |x - x_ref| over the active cells, per primitive variable."""
+    a = primitives(gas, cell_b(mag, nG, dx), nG)
+    b = primitives(ref_gas, cell_b(ref_mag, nG, dx), nG)
+    return {k: float(np.mean(np.abs(a[k] - b[k]))) for k in PRIM}
+
+
+def l1_total(e: Dict[str, float], norm: Dict[str, float]) -> float:
+    """Sum over variables of the L1 error relative to the variable's scale."""
+    return float(sum(e[k] / norm[k] for k in PRIM))
+
+
+# ----------------------------------------------------------------- trajectories
+@dataclasses.dataclass
+class Trajectory:
+    """Downsampled fine states at t_n = n * dt (coarse step dt)."""
+    N: int
+    nG: int
+    dt: float
+    gas: List[np.ndarray]
+    mag: List[np.ndarray]
+    fine_s: float = 0.0
+
+
+def choose_dt(ic: TurbIC, N: int, cfl: float = 0.3) -> float:
+    return cfl * (1.0 / N) / ic.max_speed()
+
+
+def set_time(app: GamApp, t: float, dt: float, ts: int):
+    c = app.get_clock()
+    c.t, c.dt, c.stateTime, c.oStateTime, c.ts, c.status = t, dt, t, t - dt, ts, 0
+    app.set_clock(c)
+
+
+def fine_trajectory(ic: TurbIC, N: int, steps: int, dt: Optional[float] = None, nG: int = 4) -> Trajectory:
+    """Run the solver at (2N)^3 with dt/2 for `steps` coarse intervals and
+    downsample every second state."""
+    dt = choose_dt(ic, N) if dt is None else dt
+    gas, mag = ic.fields(2 * N, nG)
+    t0 = time.perf_counter()
+    with GamApp(turb_spec(2 * N, 0, nG)) as app:
+        from kaiju_amd.gamera import gen_corners
+        app.set_corners(gen_corners(app.spec))
+        app.put_state(0, 0.0, gas, mag, None)
+        set_time(app, 0.0, 0.5 * dt, 0)
+        app.hatch()
+        out_g, out_m = [], []
+        for n in range(steps + 1):
+            if n:
+                st = app.steps(2)
+                if st != 0:
+                    raise RuntimeError(f"fine run halted with status {st} at coarse step {n}")
+            s = app.get_state(0)
+            g, m = downsample(s["Gas"], s["magFlux"], nG)
+            out_g.append(g)
+            out_m.append(m)
+    return Trajectory(N=N, nG=nG, dt=dt, gas=out_g, mag=out_m, fine_s=time.perf_counter() - t0)
+
+
+def coarse_app(N: int, precision: int = 0, nG: int = 4) -> GamApp:
+    from kaiju_amd.gamera import gen_corners
+    app = GamApp(turb_spec(N, precision, nG))
+    app.set_corners(gen_corners(app.spec))
+    return app
+
+
+def load_pair(app: GamApp, tr: Trajectory, n: int):
+    """State = sample n, oState = sample n-1 (sample 0 for n = 0), clock at t_n."""
+    t, dt = n * tr.dt, tr.dt
+    m = max(n - 1, 0)
+    app.put_state(0, t, tr.gas[n], tr.mag[n], None)
+    app.put_state(1, t - dt, tr.gas[m], tr.mag[m], None)
+    set_time(app, t, dt, n)
+
+
+def targets(app: GamApp, tr: Trajectory):
+    """Per sample n < last: (plain coarse step from n) and target = sample n+1 - plain."""
+    tg, tm = [], []
+    for n in range(len(tr.gas) - 1):
+        load_pair(app, tr, n)
+        app.set_correction(None)
+        if app.steps(1) != 0:
+            raise RuntimeError(f"coarse step from sample {n} halted")
+        s = app.get_state(0)
+        tg.append(tr.gas[n + 1] - s["Gas"])
+        tm.append(tr.mag[n + 1] - s["magFlux"])
+    return tg, tm
+
+
+# ----------------------------------------------------------------- training
+@dataclasses.dataclass
+class Normalisation:
+    scale: float                       # correction scale s
+    weights: np.ndarray                # w_rho, w_mom, w_energy, w_magFlux
+    feat_scales: tuple                 # rho, v, p, B
+    feat_offsets: tuple                # rho0, p0
+
+
+def normalisation(trs: List[Trajectory], tgs, tms) -> Normalisation:
+    N, nG, dt = trs[0].N, trs[0].nG, trs[0].dt
+    dx = 1.0 / N
+    a = slice(nG, nG + N)
+    G = np.concatenate([t[a, a, a].reshape(-1, 5) for tg in tgs for t in tg])
+    M = np.concatenate([t[a, a, a].reshape(-1, 3) for tm in tms for t in tm])
+    r = lambda x: float(np.sqrt(np.mean(x * x))) + 1e-300
+    rho_r, mom_r, en_r, mag_r = r(G[:, 0]), r(G[:, 1:4]), r(G[:, 4]), r(M)
+    ncell = N ** 3
+    w = np.array([1.0 / rho_r ** 2, 1.0 / mom_r ** 2, 1.0 / en_r ** 2, 1.0 / mag_r ** 2]) / ncell
+    scale = max(rho_r, mom_r, en_r, mag_r / dx ** 2) * dx / dt
+    P = [primitives(g, cell_b(m, nG, dx), nG) for tr in trs for g, m in zip(tr.gas, tr.mag)]
+    st = lambda ks: float(np.sqrt(np.mean([np.mean(p[k] ** 2) for p in P for k in ks])))
+    rho0 = float(np.mean([p["rho"].mean() for p in P]))
+    p0 = float(np.mean([p["p"].mean() for p in P]))
+    fs = (float(np.sqrt(np.mean([np.mean((p["rho"] - rho0) ** 2) for p in P]))) + 1e-6, st(("vx", "vy", "vz")),
+          float(np.sqrt(np.mean([np.mean((p["p"] - p0) ** 2) for p in P]))) + 1e-6, st(("bx", "by", "bz")))
+    return Normalisation(scale=scale, weights=w, feat_scales=fs, feat_offsets=(rho0, p0))
+
+
+class Corrector:
+    """A network sized for one coarse block, with its normalisation."""
+
+    def __init__(self, app: GamApp, norm: Normalisation, seed: int = 0, params: Optional[np.ndarray] = None):
+        from kaiju_amd.net import GamNet, init_params
+        self.app, self.norm = app, norm
+        self.net = GamNet(*app.G)
+        self.net.set_params(init_params(seed, out_scale=0.02) if params is None else params)
+        self.net.set_scales(*norm.feat_scales, gamma=GAMMA)
+        self.net.set_offsets(*norm.feat_offsets)
+        self.precision = app.spec.model.precision
+
+    def close(self):
+        self.net.close()
+
+    def infer(self):
+        """Correction field for the app's current State (device, FP32 [G][8])."""
+        g, _, b = self.app.state_device(0)
+        self.net.features_from_device(g, b, self.precision)
+        self.net.forward()
+        self.net.sync()
+        return self.net.output_ptr()
+
+    def sample_grad(self, tr: Trajectory, n: int, tg, tm) -> float:
+        """Loss of sample n at the current parameters; accumulates the gradient."""
+        self.app.put_state(0, n * tr.dt, tr.gas[n], tr.mag[n], None)
+        o = self.infer()
+        self.app.correction_target(tg, tm)
+        dbuf = self.net.grad_buffer_ptr()
+        loss = self.app.correction_grad(o, self.norm.scale, tr.dt, self.norm.weights, dbuf)
+        self.net.backward(dbuf)
+        return loss
+
+    def train(self, trs: List[Trajectory], tgs, tms, iters: int, lr: float = 1e-3, batch: int = 4, seed: int = 0,
+              log=None) -> List[float]:
+        rng = np.random.default_rng(seed)
+        pool = [(q, n) for q, tr in enumerate(trs) for n in range(len(tr.gas) - 1)]
+        hist = []
+        for it in range(1, iters + 1):
+            self.net.zero_grad()
+            tot = 0.0
+            for _ in range(batch):
+                q, n = pool[rng.integers(len(pool))]
+                tot += self.sample_grad(trs[q], n, tgs[q][n], tms[q][n])
+            rate = lr * (0.1 ** (it / iters))          # exponential decay to lr/10
+            self.net.adam(rate, it)
+            hist.append(tot / batch)
+            if log and (it % max(1, iters // 10) == 0 or it == 1):
+                log(f"[learned] iter {it} loss {hist[-1]:.4f}")
+        return hist
+
+    def zero_loss(self, tr: Trajectory, tgs, tms) -> float:
+        """Loss of o = 0 averaged over a trajectory's samples (sum w T^2)."""
+        w = self.norm.weights
+        N, nG = tr.N, tr.nG
+        a = slice(nG, nG + N)
+        out = []
+        for tg, tm in zip(tgs, tms):
+            g = tg[a, a, a]
+            out.append(w[0] * np.sum(g[..., 0] ** 2) + w[1] * np.sum(g[..., 1:4] ** 2) + w[2] * np.sum(g[..., 4] ** 2)
+                       + w[3] * face_sq(tm, N, nG))
+        return float(np.mean(out))
+
+    def eval_loss(self, tr: Trajectory, tgs, tms) -> float:
+        """Mean loss over a trajectory's samples at the current parameters (no update)."""
+        vals = []
+        for n in range(len(tr.gas) - 1):
+            self.app.put_state(0, n * tr.dt, tr.gas[n], tr.mag[n], None)
+            o = self.infer()
+            self.app.correction_target(tgs[n], tms[n])
+            vals.append(self.app.correction_grad(o, self.norm.scale, tr.dt, self.norm.weights, self.net.grad_buffer_ptr()))
+        return float(np.mean(vals))
+
+
+def face_sq(tm: np.ndarray, N: int, nG: int) -> float:
+    """Sum of squares over the active faces (normal index in [nG, nG+N], others in [nG, nG+N))."""
+    a, b = slice(nG, nG + N), slice(nG, nG + N + 1)
+    return float(np.sum(tm[b, a, a, 0] ** 2) + np.sum(tm[a, b, a, 1] ** 2) + np.sum(tm[a, a, b, 2] ** 2))
+
+
+def rollout(app: GamApp, tr: Trajectory, steps: int, corr: Optional[Corrector] = None) -> List[Dict[str, float]]:
+    """Coarse run from sample 0 (oState = State), errors against the samples."""
+    N, nG = tr.N, tr.nG
+    dx = 1.0 / N
+    app.set_correction(None)
+    app.put_state(0, 0.0, tr.gas[0], tr.mag[0], None)
+    app.put_state(1, -tr.dt, tr.gas[0], tr.mag[0], None)
+    set_time(app, 0.0, tr.dt, 0)
+    errs = []
+    try:
+        for n in range(steps):
+            if corr is not None:
+                app.set_correction(corr.infer(), corr.norm.scale)
+            st = app.steps(1)
+            if st != 0:
+                raise RuntimeError(f"coarse rollout halted with status {st} at step {n}")
+            s = app.get_state(0)
+            errs.append(l1_errors(s["Gas"], s["magFlux"], tr.gas[n + 1], tr.mag[n + 1], nG, dx))
+    finally:
+        app.set_correction(None)
+    return errs
+
+
+def var_norm(tr: Trajectory) -> Dict[str, float]:
+    """Per-variable scale for error sums: mean |x - mean x| of the reference at t_0."""
+    p = primitives(tr.gas[0], cell_b(tr.mag[0], tr.nG, 1.0 / tr.N), tr.nG)
+    return {k: float(np.mean(np.abs(p[k] - p[k].mean()))) + 1e-12 for k in PRIM}
+
+
+def experiment(N: int = 32, train_seeds=(0, 1, 2), test_seed: int = 7, steps: int = 24, iters: int = 1500,
+               lr: float = 1e-3, batch: int = 4, precision: int = 0, log=None) -> dict:
+    """Train on the train seeds' trajectories, roll out the test seed with and without the correction."""
+    log = log or (lambda *a: None)
+    ics = [TurbIC(seed=s) for s in train_seeds]
+    dt = min(choose_dt(ic, N) for ic in ics + [TurbIC(seed=test_seed)])
+    t0 = time.perf_counter()
+    trs = [fine_trajectory(ic, N, steps, dt) for ic in ics]
+    test = fine_trajectory(TurbIC(seed=test_seed), N, steps, dt)
+    t_fine = time.perf_counter() - t0
+    log(f"[learned] fine trajectories {t_fine:.1f} s")
+    app = coarse_app(N, precision)
+    try:
+        tgs, tms = [], []
+        for tr in trs:
+            g, m = targets(app, tr)
+            tgs.append(g)
+            tms.append(m)
+        tg_test, tm_test = targets(app, test)
+        norm = normalisation(trs, tgs, tms)
+        corr = Corrector(app, norm)
+        try:
+            L0 = corr.zero_loss(test, tg_test, tm_test)
+            t0 = time.perf_counter()
+            hist = corr.train(trs, tgs, tms, iters, lr=lr, batch=batch, log=log)
+            t_train = time.perf_counter() - t0
+            L1 = corr.eval_loss(test, tg_test, tm_test)
+            plain = rollout(app, test, steps)
+            fixed = rollout(app, test, steps, corr)
+        finally:
+            corr.close()
+    finally:
+        app.close()
+    vn = var_norm(test)
+    ep = [l1_total(e, vn) for e in plain]
+    ec = [l1_total(e, vn) for e in fixed]
+    return dict(N=N, steps=steps, iters=iters, batch=batch, precision=precision, dt=dt,
+                train_seeds=list(train_seeds), test_seed=test_seed,
+                loss_start=hist[0], loss_end=float(np.mean(hist[-max(1, iters // 20):])),
+                test_loss_zero=L0, test_loss=L1, test_loss_ratio=L1 / L0,
+                err_plain_final=ep[-1], err_corrected_final=ec[-1],
+                err_plain_mean=float(np.mean(ep)), err_corrected_mean=float(np.mean(ec)),
+                gain_final=ep[-1] / ec[-1], gain_mean=float(np.mean(ep) / np.mean(ec)),
+                plain_per_var=plain[-1], corrected_per_var=fixed[-1],
+                fine_s=round(t_fine, 2), train_s=round(t_train, 2), ms_per_iter=round(t_train / iters * 1e3, 3))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--N", type=int, default=32)
+    ap.add_argument("--train-seeds", default="0,1,2")
+    ap.add_argument("--test-seed", type=int, default=7)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--iters", type=int, default=1500)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--fp32", action="store_true")
+    a = ap.parse_args(argv)
+    log = lambda *m: print(*m, file=sys.stderr, flush=True)
+    r = experiment(a.N, tuple(int(s) for s in a.train_seeds.split(",")), a.test_seed, a.steps, a.iters, a.lr,
+                   a.batch, 1 if a.fp32 else 0, log)
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -101,3 +101,43 @@ def compare_state(app, oc, which: int = 0):
     o = oc.state if which == 0 else oc.ostate
     return dict(Gas=field_err(s["Gas"], o.Gas, GAS_FIELDS), magFlux=field_err(s["magFlux"], o.magFlux, VEC_FIELD),
                 Bxyz=field_err(s["Bxyz"], o.Bxyz, VEC_FIELD), time=abs(s["time"] - o.time))
+
+
+def on_emulation() -> bool:
+    """True when the binding points at the CPU emulation build (tests/emu)."""
+    import os
+    from kaiju_amd import gamera
+    return "emu" in os.path.basename(gamera.LIB_PATH)
+
+
+class DevF32:
+    """An FP32 array the step library reads as device memory: a torch tensor on
+    the GPU, plain host memory on the CPU emulation (where device = host)."""
+
+    def __init__(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32).copy()
+        self.shape = a.shape
+        if on_emulation():
+            self.a, self.t = a, None
+            self.ptr = a.ctypes.data
+        else:
+            import torch
+            self.a, self.t = None, torch.from_numpy(a).to("cuda")
+            torch.cuda.synchronize()
+            self.ptr = self.t.data_ptr()
+
+    def set(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32).reshape(self.shape)
+        if self.t is None:
+            self.a[...] = a
+        else:
+            import torch
+            self.t.copy_(torch.from_numpy(a.copy()))
+            torch.cuda.synchronize()
+
+    def numpy(self) -> np.ndarray:
+        if self.t is None:
+            return self.a.copy()
+        import torch
+        torch.cuda.synchronize()
+        return self.t.cpu().numpy()

@@ -1,6 +1,7 @@
 """The HIP kernel sources, compiled for the CPU (tests/emu), checked before any
 GPU run: the GPU parity tests pass against the oracle on the emulation, and an
-AddressSanitizer build of the same sources runs every kernel in bounds."""
+AddressSanitizer build of the same sources runs every kernel in bounds; the
+same for the learned-correction kernels of the step library."""
 import os
 import subprocess
 import sys
@@ -12,12 +13,12 @@ from tests.emu import build_emu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _run(lib, extra_env, kexpr, timeout):
+def _run(lib, extra_env, kexpr, timeout, test_file="test_gpu_parity.py"):
     env = dict(os.environ)
     env.update(extra_env)
     env["KAIJU_EMU_LIB"] = lib
     cmd = [sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider", "-m", "gpu",
-           os.path.join(ROOT, "tests", "test_gpu_parity.py"), "-k", kexpr]
+           os.path.join(ROOT, "tests", test_file), "-k", kexpr]
     r = subprocess.run(cmd, cwd=ROOT, env=env, capture_output=True, text=True, timeout=timeout)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     return r.stdout
@@ -33,4 +34,14 @@ def test_kernels_in_bounds_under_asan():
     lib = build_emu.build(asan=True)
     env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
     out = _run(lib, env, "(test_multistep or test_hatch or test_metric or test_initial) and (bw16_warp_boris or ot32x32x4_8cent or 25d or nolfm)", 900)
+    assert " passed" in out
+
+
+def test_correction_kernels_on_cpu_emulation_and_in_bounds():
+    lib = build_emu.build(asan=False)
+    out = _run(lib, {}, "correction or corrected", 600, "test_learned.py")
+    assert " passed" in out
+    lib = build_emu.build(asan=True)
+    env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
+    out = _run(lib, env, "correction or corrected", 900, "test_learned.py")
     assert " passed" in out

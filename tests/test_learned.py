@@ -1,0 +1,184 @@
+"""Learned coarse-grid correction.
+
+CPU: the seeded turbulence initial state has div.B = 0 to rounding; the 2x
+downsampling conserves mass, momentum, energy and every face-plane flux and
+keeps div.B = 0.
+
+GPU (and the CPU emulation of the same kernels): the gradient of the
+correction loss (gam_correction_grad) against central differences of the loss
+itself; a corrected step differs from the plain one by exactly the linearised
+change the loss models (density, momentum and magFlux); corrected steps keep
+div.B at rounding level and conserve mass and momentum.
+
+GPU only (the network is MFMA code): a network trained on the GPU on the
+solver's downsampled fine trajectories lowers the one-step loss and the
+rollout error of a held-out seed.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from kaiju_amd import learned
+from tests.helpers import DevF32, on_emulation, say
+
+NG = 4
+
+
+def div_faces(mag):
+    return (mag[1:, :-1, :-1, 0] - mag[:-1, :-1, :-1, 0] + mag[:-1, 1:, :-1, 1] - mag[:-1, :-1, :-1, 1]
+            + mag[:-1, :-1, 1:, 2] - mag[:-1, :-1, :-1, 2])
+
+
+def test_turbulence_ic_is_divergence_free_and_scaled():
+    ic = learned.TurbIC(seed=3)
+    N = 16
+    gas, mag = ic.fields(N, NG)
+    assert gas.shape == (N + 8,) * 3 + (5,) and mag.shape == (N + 9,) * 3 + (3,)
+    dx = 1.0 / N
+    assert np.max(np.abs(div_faces(mag))) <= 1e-14 * np.max(np.abs(mag))
+    a = slice(NG, NG + N)
+    v = gas[a, a, a, 1:4] / gas[a, a, a, 0:1]
+    assert math.isclose(math.sqrt(np.mean(np.sum(v * v, axis=-1))), ic.v_rms, rel_tol=1e-12)
+    # mean field through every x plane: b_mean_x (the modes add no net flux)
+    for i in (NG, NG + 5, NG + N):
+        assert math.isclose(mag[i, a, a, 0].sum(), ic.b_mean[0] * 1.0, rel_tol=1e-12, abs_tol=1e-14)
+    # ghosts hold the periodic images
+    assert np.array_equal(gas[NG - 1], gas[NG + N - 1]) and np.array_equal(mag[NG + N], mag[NG])
+
+
+def test_downsample_conserves_and_keeps_divb_zero():
+    ic = learned.TurbIC(seed=4)
+    N = 8
+    gas, mag = ic.fields(2 * N, NG)
+    gc, mc = learned.downsample(gas, mag, NG)
+    assert gc.shape == (N + 8,) * 3 + (5,) and mc.shape == (N + 9,) * 3 + (3,)
+    af, ac = slice(NG, NG + 2 * N), slice(NG, NG + N)
+    for v in range(5):   # totals: fine cell volume is 1/8 of a coarse one
+        assert math.isclose(gas[af, af, af, v].sum() / 8.0, gc[ac, ac, ac, v].sum(), rel_tol=1e-12, abs_tol=1e-12)
+    assert np.max(np.abs(div_faces(mc))) <= 1e-14 * np.max(np.abs(mc))
+    # flux through the coarse x-plane I equals the fine plane 2I
+    for I in range(N):
+        assert math.isclose(mc[NG + I, ac, ac, 0].sum(), mag[NG + 2 * I, af, af, 0].sum(), rel_tol=1e-12,
+                            abs_tol=1e-15)
+    # a single coarse cell is the mean of its 8 fine cells
+    assert math.isclose(gc[NG + 2, NG + 1, NG + 3, 4],
+                        gas[NG + 4:NG + 6, NG + 2:NG + 4, NG + 6:NG + 8, 4].mean(), rel_tol=1e-14)
+
+
+# ----------------------------------------------------------------- step-library correction
+def turb_app(N=8, precision=0, seed=3):
+    ic = learned.TurbIC(seed=seed, kmax=2)
+    gas, mag = ic.fields(N, NG)
+    app = learned.coarse_app(N, precision)
+    app.put_state(0, 0.0, gas, mag, None)
+    dt = learned.choose_dt(ic, N)
+    learned.set_time(app, 0.0, dt, 0)
+    app.hatch()
+    assert app.Model.status == 0
+    return app, dt
+
+
+def cell_index(app, i, j, k):
+    Ni, Nj, _ = app.G
+    return i + Ni * (j + Nj * k)
+
+
+@pytest.mark.gpu
+def test_correction_grad_matches_finite_differences():
+    say("[learned] correction loss gradient vs central differences")
+    app, dt = turb_app()
+    with app:
+        nc = int(np.prod(app.G))
+        rng = np.random.default_rng(5)
+        o = np.round(rng.normal(size=(nc, 8)) * 64.0) / 64.0       # exact in FP32, and o +- h too
+        tg = np.asfortranarray(rng.normal(size=app.G + (5,)) * 1e-3)
+        tm = np.asfortranarray(rng.normal(size=app.F + (3,)) * 1e-5)
+        app.correction_target(tg, tm)
+        w, s = [1.0, 2.0, 0.5, 3.0e4], 0.7
+        do, dd = DevF32(o), DevF32(np.zeros((nc, 8)))
+        L = app.correction_grad(do.ptr, s, dt, w, dd.ptr)
+        g = dd.numpy()
+        assert L > 0 and np.all(np.isfinite(g))
+        h = 2.0 ** -5
+        n = NG + 8
+        picks = [(cell_index(app, i, j, k), c) for (i, j, k) in
+                 [(NG, NG, NG), (NG + 3, NG + 5, NG + 2), (n, NG + 1, NG + 7), (NG + 7, n, n), (NG - 1, NG, NG + 2),
+                  (1, 1, 1)] for c in range(8)]
+        gmax = float(np.max(np.abs(g)))
+        for q, c in picks:
+            o2 = o.copy()
+            o2[q, c] += h
+            do.set(o2)
+            Lp = app.correction_grad(do.ptr, s, dt, w, dd.ptr)
+            o2[q, c] -= 2 * h
+            do.set(o2)
+            Lm = app.correction_grad(do.ptr, s, dt, w, dd.ptr)
+            fd = (Lp - Lm) / (2 * h)
+            assert abs(fd - g[q, c]) <= 1e-5 * gmax + 1e-6 * abs(fd), (q, c, fd, g[q, c])
+        # deep ghost cells never reach the loss
+        assert np.all(g[cell_index(app, 1, 1, 1)] == 0)
+
+
+@pytest.mark.gpu
+def test_corrected_step_is_the_linearised_change():
+    say("[learned] corrected step minus plain step = linearised correction")
+    a, dt = turb_app()
+    b, _ = turb_app()
+    with a, b:
+        nc = int(np.prod(a.G))
+        rng = np.random.default_rng(8)
+        o = rng.normal(size=(nc, 8))
+        do, dd = DevF32(o), DevF32(np.zeros((nc, 8)))
+        s = 0.05
+        b.set_correction(do.ptr, s)
+        assert a.steps(1) == 0 and b.steps(1) == 0
+        sa, sb = a.get_state(0), b.get_state(0)
+        dG = sb["Gas"] - sa["Gas"]
+        dM = sb["magFlux"] - sa["magFlux"]
+        assert np.max(np.abs(dG[..., 0])) > 1e-6 and np.max(np.abs(dM)) > 0
+        w = [1.0, 1.0, 0.0, 1.0e4]
+        b.correction_target(np.zeros_like(dG), np.zeros_like(dM))
+        L0 = b.correction_grad(do.ptr, s, dt, w, dd.ptr)         # size of the linearised change
+        b.correction_target(dG, dM)
+        L = b.correction_grad(do.ptr, s, dt, w, dd.ptr)
+        say(f"   L0 {L0:.3e}  L {L:.3e}")
+        assert L0 > 0 and L <= 1e-18 * L0
+        b.set_correction(None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [0, 1])
+def test_corrected_steps_keep_divb_and_conserve(precision):
+    say(f"[learned] corrected steps: div.B and conservation, precision {precision}")
+    a, dt = turb_app(precision=precision)
+    with a:
+        g = a.get_grid()
+        A = (slice(NG, NG + 8),) * 3
+        vol = g["volume"][A]
+        s0 = a.get_state(0)
+        nc = int(np.prod(a.G))
+        do = DevF32(np.random.default_rng(9).normal(size=(nc, 8)))
+        a.set_correction(do.ptr, 0.05)
+        assert a.steps(4) == 0
+        a.set_correction(None)
+        s1 = a.get_state(0)
+        assert abs(a.divb()) <= 1e-12 * np.abs(s1["magFlux"]).max()
+        tol = 1e-12 if precision == 0 else 1e-6
+        for v in range(4):
+            t0 = (s0["Gas"][A + (v,)] * vol).sum()
+            t1 = (s1["Gas"][A + (v,)] * vol).sum()
+            assert abs(t1 - t0) <= tol * max(1.0, abs(t0)), (v, t0, t1)
+        # and the correction did change the state
+        assert np.max(np.abs(s1["Gas"] - s0["Gas"])) > 0
+
+
+@pytest.mark.gpu
+def test_trained_correction_lowers_loss_and_rollout_error():
+    if on_emulation():
+        pytest.skip("the network runs only as MFMA code")
+    say("[learned] train 16^3 on 2 seeds, roll out a third")
+    r = learned.experiment(N=16, train_seeds=(0, 1), test_seed=5, steps=12, iters=300, batch=4, log=say)
+    say(f"   {r}")
+    assert r["test_loss_ratio"] < 0.7
+    assert r["gain_mean"] > 1.2 and r["gain_final"] > 1.1

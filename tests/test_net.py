@@ -1,0 +1,130 @@
+"""The learned-correction network's kernels against the CPU reference.
+
+CPU: the reference's own backward equals central finite differences of its
+forward (float64, no BF16), which pins the checker.  GPU: each layer of the
+MFMA forward equals the reference layer applied to the GPU's own BF16 input
+activations; the whole forward and the parameter gradients equal the
+reference's (same BF16 rounding points) to BF16-level tolerances; Adam on the
+GPU fits a small regression.
+"""
+import numpy as np
+import pytest
+
+from oracle import net_ref
+from tests.helpers import say
+
+DIMS = (12, 10, 9)     # ghost-inclusive box of a small block
+
+
+def _rand_problem(seed=0, dims=DIMS):
+    rng = np.random.default_rng(seed)
+    nc = int(np.prod(dims))
+    w = np.concatenate([np.concatenate([rng.normal(0, np.sqrt(2.0 / (27 * ci)), 27 * co * ci), rng.normal(0, 0.1, co)])
+                        for co, ci in zip(net_ref.CO, net_ref.CI)]).astype(np.float32)
+    feat = rng.normal(size=(nc, 8)).astype(np.float32)
+    dout = rng.normal(size=(nc, 8)).astype(np.float32)
+    return w, feat, dout
+
+
+def test_reference_backward_matches_finite_differences():
+    dims = (8, 8, 8)
+    w, feat, dout = _rand_problem(1, dims)
+    ref = net_ref.NetRef(*dims, w, bf=False)
+    ref.forward(feat)
+    g = net_ref.flat_grads(ref.backward(dout))
+    rng = np.random.default_rng(2)
+    w64 = w.astype(np.float64)
+    loss = lambda wv: float((net_ref.NetRef(*dims, wv, bf=False).forward(feat) * dout).sum())
+    for q in rng.choice(w.size, 24, replace=False):
+        h = 1e-5 * max(1.0, abs(w64[q]))
+        wp, wm = w64.copy(), w64.copy()
+        wp[q] += h
+        wm[q] -= h
+        fd = (loss(wp) - loss(wm)) / (2 * h)
+        assert abs(fd - g[q]) <= 1e-6 * (1 + abs(fd)), (q, fd, g[q])
+
+
+@pytest.mark.gpu
+def test_gpu_forward_layers_and_output():
+    from kaiju_amd.net import GamNet
+    w, feat, _ = _rand_problem(3)
+    say(f"[net] forward {DIMS}")
+    with GamNet(*DIMS) as net:
+        net.set_params(w)
+        net.set_features(feat)
+        net.forward()
+        acts = [net.get_act(l) for l in range(3)]
+        out = net.get_output().astype(np.float64)
+    ref = net_ref.NetRef(*DIMS, w)
+    rng = ref.rng
+    assert np.array_equal(acts[0][:, :8], net_ref.bf16(feat))
+    # each hidden layer from the GPU's own BF16 input: equal up to one BF16 rounding step
+    for l in (0, 1):
+        W, b = ref.layers[l]
+        z = net_ref.conv(acts[l][:, :W.shape[2]], W, ref.off, rng) + b
+        want = net_ref.bf16(np.maximum(z, 0))
+        got = acts[l + 1][rng, :W.shape[1]]
+        ulp = np.maximum(np.abs(want), 1e-30) * 2.0 ** -7
+        assert np.all(np.abs(got - want) <= ulp), (l, float(np.abs(got - want).max()))
+    W, b = ref.layers[2]
+    z = net_ref.conv(acts[2], W, ref.off, rng) + b
+    err = np.abs(out[rng] - z).max() / np.abs(z).max()
+    say(f"   output layer from GPU activations: {err:.2e}")
+    assert err <= 1e-5
+    full = ref.forward(feat)
+    errf = np.linalg.norm(out[rng] - full[rng]) / np.linalg.norm(full[rng])
+    say(f"   whole forward vs reference: {errf:.2e}")
+    assert errf <= 2e-2
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_reference():
+    from kaiju_amd.net import GamNet
+    w, feat, dout = _rand_problem(4)
+    say(f"[net] backward {DIMS}")
+    with GamNet(*DIMS) as net:
+        net.set_params(w)
+        net.set_features(feat)
+        net.forward()
+        net.zero_grad()
+        net.backward_host(dout)
+        _, g = net.get_params()
+    ref = net_ref.NetRef(*DIMS, w)
+    ref.forward(feat)
+    gr = net_ref.flat_grads(ref.backward(dout))
+    o = 0
+    for l, (co, ci) in enumerate(zip(net_ref.CO, net_ref.CI)):
+        n = 27 * co * ci + co
+        e = np.linalg.norm(g[o:o + n] - gr[o:o + n]) / np.linalg.norm(gr[o:o + n])
+        say(f"   layer {l} grad rel err {e:.2e}")
+        assert e <= 3e-2, (l, e)
+        o += n
+
+
+@pytest.mark.gpu
+def test_gpu_adam_fits_a_target_field():
+    """Training on the GPU with its own kernels: fit a fixed target output."""
+    from kaiju_amd.net import GamNet, init_params
+    rng = np.random.default_rng(5)
+    nc = int(np.prod(DIMS))
+    feat = rng.normal(size=(nc, 8)).astype(np.float32)
+    target = np.zeros((nc, 8), np.float32)
+    target[:, :5] = 0.5 * np.tanh(feat[:, :5])
+    ref = net_ref.NetRef(*DIMS, init_params(0))
+    rows = ref.rng
+    say("[net] adam regression")
+    with GamNet(*DIMS) as net:
+        net.set_params(init_params(0))
+        net.set_features(feat)
+        losses = []
+        for t in range(1, 151):
+            net.forward()
+            out = net.get_output()
+            d = np.zeros_like(out)
+            d[rows] = 2 * (out[rows] - target[rows]) / rows.size
+            losses.append(float(((out[rows] - target[rows]) ** 2).sum() / rows.size))
+            net.zero_grad()
+            net.backward_host(d)
+            net.adam(3e-3, t)
+        say(f"   loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+        assert losses[-1] < 0.3 * losses[0]
