@@ -18,6 +18,7 @@
 #ifndef GAMERA_HIP_H
 #define GAMERA_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -107,6 +108,14 @@ int  gam_set_background(gam_handle *h, const double *fcB0, const double *edgB0, 
 int  gam_get_background(gam_handle *h, double *fcB0, double *edgB0, double *B0, double *bFlux0,
                         double *dpB0);
 int  gam_background_dipole(gam_handle *h, double M0, const double *xc, const double *bu);
+
+/* --- device buffers (no counterpart in the reference) -------------------- */
+/* Plain device memory for callers without a GPU allocator of their own (a
+ * correction field, a gradient buffer): gam_memcpy copies in any direction
+ * and returns when the copy is done. */
+int  gam_device_alloc(size_t bytes, void **p);
+int  gam_device_free(void *p);
+int  gam_memcpy(void *dst, const void *src, size_t bytes);
 
 /* --- learned coarse-grid correction ------------------------------------- */
 /* No counterpart in the reference.  A correction field o (device FP32, [G][8]:

@@ -2,7 +2,7 @@
  * correction network (no counterpart in the reference).
  *
  * A 3-layer 3x3x3 convolution net on the ghost-inclusive cell grid of a block
- * (8 input features -> 32 -> 32 -> 8 outputs, ReLU between), BF16 weights and
+ * (8 input features -> 32 -> 32 -> 8 outputs, leaky ReLU x > 0 ? x : x/8 between), BF16 weights and
  * activations, FP32 accumulation on MFMA.  Inputs: rho, v, p, B of a state,
  * divided by fixed scales.  Outputs per cell: 5 gas-flux and 3 edge-EMF
  * corrections, handed to the step with gam_set_correction (gamera_hip.h).

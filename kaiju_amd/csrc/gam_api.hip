@@ -596,6 +596,28 @@ extern "C" int gam_background_dipole(gam_handle *h, double M0, const double *xc,
     return 0;
 }
 
+// ----------------------------------------------------------------- device buffers
+extern "C" int gam_device_alloc(size_t bytes, void **p) {
+    if (!p) return -1;
+    *p = nullptr;
+    if (bytes == 0) return -1;
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        *p = nullptr;
+        return -3;
+    }
+    return 0;
+}
+
+extern "C" int gam_device_free(void *p) {
+    if (!p) return 0;
+    return hipFree(p) == hipSuccess ? 0 : -3;
+}
+
+extern "C" int gam_memcpy(void *dst, const void *src, size_t bytes) {
+    if (!dst || !src) return -1;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? 0 : -3;
+}
+
 // ----------------------------------------------------------------- learned correction
 extern "C" int gam_state_device(gam_handle *h, int which, void **Gas, double **magFlux, void **Bxyz) {
     if (!h || which < 0 || which > 1) return -1;

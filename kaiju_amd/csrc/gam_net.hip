@@ -1,5 +1,5 @@
 // gam_net.hip -- the learned coarse-grid correction: a 3-layer 3x3x3 conv net
-// (8 -> 32 -> 32 -> 8 channels, ReLU between) on the ghost-inclusive cell grid,
+// (8 -> 32 -> 32 -> 8 channels, leaky ReLU between) on the ghost-inclusive cell grid,
 // with BF16 weights and activations and FP32 accumulation on gfx950's
 // v_mfma_f32_16x16x32_bf16.  Forward, backward (data and weights) and Adam are
 // hand-written here; libgamera_net.so exports them (include/gamera_net.h).
@@ -51,10 +51,15 @@ __device__ __forceinline__ bf16x8 ld8(const u16 *p) {   // 16-byte aligned
     return *reinterpret_cast<const bf16x8 *>(p);
 }
 
+#define NET_LEAK 0.125f
+
 // ----------------------------------------------------------------- convolution
-// MODE 0: ReLU, write act_out (channels last) and actT_out (channel-major), NCO = 32
+// Hidden activation: leaky ReLU, x > 0 ? x : x/8 (a slope that keeps every
+// unit trainable; 1/8 is exact in BF16 and FP32).
+// MODE 0: activation, write act_out (channels last) and actT_out (channel-major), NCO = 32
 // MODE 1: linear output, write out[p][0..7] FP32 (NCO = 16, channels 8..15 padding)
-// MODE 2: backward data: multiply by ReLU'(mask act), write both BF16 layouts, NCO = 32
+// MODE 2: backward data: multiply by the activation's slope at the forward
+//         output (mask act > 0: 1, else 1/8), write both BF16 layouts, NCO = 32
 // One wave = 16 cells x NCO channels; blockDim 256 = 4 waves.
 template <int NCO, int MODE>
 __global__ void __launch_bounds__(256) k_conv(NetDims d, const u16 *__restrict__ in, const u16 *__restrict__ W,
@@ -89,14 +94,14 @@ __global__ void __launch_bounds__(256) k_conv(NetDims d, const u16 *__restrict__
             float v = acc[t][r];
             if (MODE == 0) {
                 v = v + bias[co];
-                v = v > 0.f ? v : 0.f;
+                v = v > 0.f ? v : NET_LEAK * v;
                 const u16 b = f2bf(v);
                 act_out[(size_t)p * NET_C + co] = b;
                 actT_out[(size_t)co * d.nc + p] = b;
             } else if (MODE == 1) {
                 if (co < 8) out[(size_t)p * 8 + co] = v + bias[co];
             } else {
-                if (bf2f(mask[(size_t)p * NET_C + co]) <= 0.f) v = 0.f;   // ReLU' of the forward output
+                if (bf2f(mask[(size_t)p * NET_C + co]) <= 0.f) v = NET_LEAK * v;   // slope at the forward output
                 const u16 b = f2bf(v);
                 act_out[(size_t)p * NET_C + co] = b;
                 actT_out[(size_t)co * d.nc + p] = b;

@@ -48,6 +48,48 @@ class HipStepError(RuntimeError):
     pass
 
 
+class DeviceArray:
+    """A device buffer of the library's GPU holding a numpy array's bytes
+    (for correction fields and gradient buffers; no torch needed)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        rc = lib().gam_device_alloc(self.nbytes, C.byref(p))
+        if rc != 0 or not p.value:
+            raise HipStepError(f"gam_device_alloc({self.nbytes}) rc={rc}")
+        self.ptr = p.value
+
+    @classmethod
+    def of(cls, a: np.ndarray, dtype=np.float32):
+        d = cls(a.shape, dtype)
+        d.set(a)
+        return d
+
+    def set(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype).reshape(self.shape)
+        if lib().gam_memcpy(self.ptr, a.ctypes.data, self.nbytes) != 0:
+            raise HipStepError("gam_memcpy to device failed")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if lib().gam_memcpy(out.ctypes.data, self.ptr, self.nbytes) != 0:
+            raise HipStepError("gam_memcpy from device failed")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().gam_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 _lib = None
 
 
@@ -108,6 +150,9 @@ def lib():
         "gam_state_device": (C.c_int, [H, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p)]),
         "gam_set_correction": (C.c_int, [H, C.c_void_p, C.c_double]),
+        "gam_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+        "gam_device_free": (C.c_int, [C.c_void_p]),
+        "gam_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
         "gam_correction_target": (C.c_int, [H, _dp, _dp]),
         "gam_correction_grad": (C.c_int, [H, C.c_void_p, C.c_double, C.c_double, _dp, C.c_void_p,
                                           C.POINTER(C.c_double)]),

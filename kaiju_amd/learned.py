@@ -23,8 +23,9 @@ gradient runs back through the network (``gam_net_backward``) into Adam
 
     python -m kaiju_amd.learned [--N 32] [--train-seeds 0,1,2] [--test-seed 7] [--steps 24] [--iters 1500]
 
-prints one JSON line with the plain and corrected rollout errors against the
-downsampled fine run of the held-out seed.
+prints one JSON line with the one-step loss ratio and the plain and corrected
+errors (one step from each sample, and a rollout from sample 0) against the
+downsampled fine run of a held-out seed.
 """
 from __future__ import annotations
 
@@ -394,10 +395,28 @@ def rollout(app: GamApp, tr: Trajectory, steps: int, corr: Optional[Corrector] =
     return errs
 
 
+def one_step_errors(app: GamApp, tr: Trajectory, corr: Optional[Corrector] = None) -> List[Dict[str, float]]:
+    """Per sample n: L1 errors of one coarse step from the downsampled states
+    against sample n+1, with or without the correction."""
+    out = []
+    try:
+        for n in range(len(tr.gas) - 1):
+            load_pair(app, tr, n)
+            app.set_correction(None if corr is None else corr.infer(), 0.0 if corr is None else corr.norm.scale)
+            if app.steps(1) != 0:
+                raise RuntimeError(f"coarse step from sample {n} halted")
+            s = app.get_state(0)
+            out.append(l1_errors(s["Gas"], s["magFlux"], tr.gas[n + 1], tr.mag[n + 1], tr.nG, 1.0 / tr.N))
+    finally:
+        app.set_correction(None)
+    return out
+
+
 def var_norm(tr: Trajectory) -> Dict[str, float]:
-    """Per-variable scale for error sums: mean |x - mean x| of the reference at t_0."""
-    p = primitives(tr.gas[0], cell_b(tr.mag[0], tr.nG, 1.0 / tr.N), tr.nG)
-    return {k: float(np.mean(np.abs(p[k] - p[k].mean()))) + 1e-12 for k in PRIM}
+    """Per-variable scale for error sums: mean |x - mean x| of the reference,
+    averaged over its samples (the density starts uniform)."""
+    P = [primitives(g, cell_b(m, tr.nG, 1.0 / tr.N), tr.nG) for g, m in zip(tr.gas, tr.mag)]
+    return {k: float(np.mean([np.mean(np.abs(p[k] - p[k].mean())) for p in P])) + 1e-12 for k in PRIM}
 
 
 def experiment(N: int = 32, train_seeds=(0, 1, 2), test_seed: int = 7, steps: int = 24, iters: int = 1500,
@@ -427,6 +446,10 @@ def experiment(N: int = 32, train_seeds=(0, 1, 2), test_seed: int = 7, steps: in
             hist = corr.train(trs, tgs, tms, iters, lr=lr, batch=batch, log=log)
             t_train = time.perf_counter() - t0
             L1 = corr.eval_loss(test, tg_test, tm_test)
+            Ltr0 = corr.zero_loss(trs[0], tgs[0], tms[0])
+            Ltr1 = corr.eval_loss(trs[0], tgs[0], tms[0])
+            os_plain = one_step_errors(app, test)
+            os_corr = one_step_errors(app, test, corr)
             plain = rollout(app, test, steps)
             fixed = rollout(app, test, steps, corr)
         finally:
@@ -434,12 +457,17 @@ def experiment(N: int = 32, train_seeds=(0, 1, 2), test_seed: int = 7, steps: in
     finally:
         app.close()
     vn = var_norm(test)
+    op = [l1_total(e, vn) for e in os_plain]
+    oc = [l1_total(e, vn) for e in os_corr]
     ep = [l1_total(e, vn) for e in plain]
     ec = [l1_total(e, vn) for e in fixed]
     return dict(N=N, steps=steps, iters=iters, batch=batch, precision=precision, dt=dt,
                 train_seeds=list(train_seeds), test_seed=test_seed,
                 loss_start=hist[0], loss_end=float(np.mean(hist[-max(1, iters // 20):])),
-                test_loss_zero=L0, test_loss=L1, test_loss_ratio=L1 / L0,
+                test_loss_zero=L0, test_loss=L1, test_loss_ratio=L1 / L0, train_loss_ratio=Ltr1 / Ltr0,
+                one_step_gain=float(np.mean(op) / np.mean(oc)),
+                one_step_gain_per_var={k: float(np.mean([e[k] for e in os_plain]) / np.mean([e[k] for e in os_corr]))
+                                       for k in PRIM},
                 err_plain_final=ep[-1], err_corrected_final=ec[-1],
                 err_plain_mean=float(np.mean(ep)), err_corrected_mean=float(np.mean(ec)),
                 gain_final=ep[-1] / ec[-1], gain_mean=float(np.mean(ep) / np.mean(ec)),

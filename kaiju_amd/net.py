@@ -63,7 +63,7 @@ def num_params() -> int:
 
 
 def init_params(seed: int = 0, out_scale: float = 0.1) -> np.ndarray:
-    """He-normal weights for the ReLU layers, a small last layer, zero biases."""
+    """He-normal weights for the (leaky) ReLU layers, a small last layer, zero biases."""
     rng = np.random.default_rng(seed)
     parts = []
     for l, (co, ci) in enumerate(zip(CO, CI)):

@@ -8,7 +8,8 @@ rounding points documented in include/gamera_net.h.
 
 Float64 arithmetic, with BF16 rounding (round to nearest even) at the points
 where the GPU stores BF16: input features, weights, hidden activations, and the
-upstream gradients of each backward layer.
+upstream gradients of each backward layer.  Hidden activation: leaky ReLU with
+slope 1/8 for negative inputs.
 """
 from __future__ import annotations
 
@@ -17,6 +18,7 @@ import numpy as np
 CO = (32, 32, 8)
 CI = (8, 32, 32)
 TAPS = 27
+LEAK = 0.125
 
 
 def num_params() -> int:
@@ -31,6 +33,11 @@ def unpack(w: np.ndarray):
         out.append((w[o:o + n].reshape(TAPS, co, ci).astype(np.float64), w[o + n:o + n + co].astype(np.float64)))
         o += n + co
     return out
+
+
+def act(z: np.ndarray) -> np.ndarray:
+    """Hidden activation: leaky ReLU, slope 1/8 below zero."""
+    return np.where(z > 0, z, LEAK * z)
 
 
 def bf16(x: np.ndarray) -> np.ndarray:
@@ -82,7 +89,7 @@ class NetRef:
             z = conv(a, W, self.off, rng) + b
             if l < 2:
                 nxt = np.zeros((nc, W.shape[1]))
-                nxt[rng] = self.r(np.maximum(z, 0.0))
+                nxt[rng] = self.r(act(z))
                 a = nxt
                 self.acts.append(a)
             else:
@@ -105,11 +112,11 @@ class NetRef:
             grads[l] = (dW, g[rng].sum(axis=0))
             if l == 0:
                 break
-            # gradient of the layer input: transposed convolution, then ReLU'
+            # gradient of the layer input: transposed convolution, then the activation's slope
             gi = np.zeros((nc, W.shape[2]))
             for t in range(TAPS):
                 gi[rng] += g[rng - off[t]] @ W[t]
-            gi[rng] *= (self.acts[l][rng] > 0)
+            gi[rng] *= np.where(self.acts[l][rng] > 0, 1.0, LEAK)
             g = np.zeros((nc, W.shape[2]))
             g[rng] = self.r(gi[rng])
         return grads

@@ -110,34 +110,7 @@ def on_emulation() -> bool:
     return "emu" in os.path.basename(gamera.LIB_PATH)
 
 
-class DevF32:
-    """An FP32 array the step library reads as device memory: a torch tensor on
-    the GPU, plain host memory on the CPU emulation (where device = host)."""
-
-    def __init__(self, a: np.ndarray):
-        a = np.ascontiguousarray(a, dtype=np.float32).copy()
-        self.shape = a.shape
-        if on_emulation():
-            self.a, self.t = a, None
-            self.ptr = a.ctypes.data
-        else:
-            import torch
-            self.a, self.t = None, torch.from_numpy(a).to("cuda")
-            torch.cuda.synchronize()
-            self.ptr = self.t.data_ptr()
-
-    def set(self, a: np.ndarray):
-        a = np.ascontiguousarray(a, dtype=np.float32).reshape(self.shape)
-        if self.t is None:
-            self.a[...] = a
-        else:
-            import torch
-            self.t.copy_(torch.from_numpy(a.copy()))
-            torch.cuda.synchronize()
-
-    def numpy(self) -> np.ndarray:
-        if self.t is None:
-            return self.a.copy()
-        import torch
-        torch.cuda.synchronize()
-        return self.t.cpu().numpy()
+def DevF32(a: np.ndarray):
+    """An FP32 device copy of a (plain host memory on the CPU emulation)."""
+    from kaiju_amd.gamera import DeviceArray
+    return DeviceArray.of(a, np.float32)

@@ -1,5 +1,5 @@
-"""The C-ABI library loads and exports every entry point include/gamera_hip.h
-declares (no GPU needed: nothing is called)."""
+"""The C-ABI libraries load and export every entry point include/gamera_hip.h
+and include/gamera_net.h declare (no GPU needed: nothing is called)."""
 import ctypes
 import os
 import re
@@ -7,10 +7,12 @@ import re
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "gamera_hip.h")
 LIB = os.path.join(ROOT, "kaiju_amd", "lib", "libgamera_hip.so")
+NET_HEADER = os.path.join(ROOT, "include", "gamera_net.h")
+NET_LIB = os.path.join(ROOT, "kaiju_amd", "lib", "libgamera_net.so")
 
 
-def declared_functions():
-    src = open(HEADER).read()
+def declared_functions(header=HEADER):
+    src = open(header).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(gam_[a-z0-9_]+)\s*\(", src)))
 
@@ -45,3 +47,14 @@ def test_create_rejects_bad_dims_without_touching_the_gpu():
     assert not h.value
     assert L.gam_num_stages() > 0
     assert L.gam_stage_name(0) == b"clock"
+
+
+def test_net_library_exports_every_declared_symbol():
+    assert os.path.exists(NET_LIB), "build first: python -m kaiju_amd.build net"
+    lib = ctypes.CDLL(NET_LIB)
+    names = declared_functions(NET_HEADER)
+    assert "gam_net_forward" in names and "gam_net_backward" in names and "gam_net_adam" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert b"gfx950" in open(NET_LIB, "rb").read()
+    assert lib.gam_net_num_params() == 27 * (32 * 8 + 32 * 32 + 8 * 32) + 32 + 32 + 8

@@ -62,7 +62,7 @@ def test_gpu_forward_layers_and_output():
     for l in (0, 1):
         W, b = ref.layers[l]
         z = net_ref.conv(acts[l][:, :W.shape[2]], W, ref.off, rng) + b
-        want = net_ref.bf16(np.maximum(z, 0))
+        want = net_ref.bf16(net_ref.act(z))
         got = acts[l + 1][rng, :W.shape[1]]
         ulp = np.maximum(np.abs(want), 1e-30) * 2.0 ** -7
         assert np.all(np.abs(got - want) <= ulp), (l, float(np.abs(got - want).max()))
@@ -125,6 +125,12 @@ def test_gpu_adam_fits_a_target_field():
             losses.append(float(((out[rows] - target[rows]) ** 2).sum() / rows.size))
             net.zero_grad()
             net.backward_host(d)
+            if t == 1:
+                w0, g0 = net.get_params()
             net.adam(3e-3, t)
+            if t == 1:   # first bias-corrected step: w - lr g / (|g| + eps)
+                w1, _ = net.get_params()
+                want = w0 - np.float32(3e-3) * g0 / (np.abs(g0) + np.float32(1e-8))
+                assert np.allclose(w1, want, rtol=1e-5, atol=1e-8)
         say(f"   loss {losses[0]:.3e} -> {losses[-1]:.3e}")
         assert losses[-1] < 0.3 * losses[0]
