@@ -136,6 +136,7 @@ int  gam_memcpy(void *dst, const void *src, size_t bytes);
 int  gam_state_device(gam_handle *h, int which, void **Gas, double **magFlux, void **Bxyz);
 int  gam_set_correction(gam_handle *h, const float *o, double scale);
 int  gam_correction_target(gam_handle *h, const double *dGas, const double *dMag);
+int  gam_correction_target_device(gam_handle *h, const double *dGas, const double *dMag);  /* device FP64 */
 int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt, const double *w4,
                          float *dout, double *loss);
 
@@ -147,6 +148,10 @@ int  gam_put_state(gam_handle *h, int which, double time, const double *Gas,
                    const double *magFlux, const double *Bxyz);
 int  gam_get_state(gam_handle *h, int which, double *time, double *Gas,
                    double *magFlux, double *Bxyz);
+/* State/oState from FP64 device arrays (Gas G*5, magFlux F*3); Bxyz is
+ * recomputed from magFlux.  No counterpart in the reference (a host keeping
+ * training samples resident on the GPU). */
+int  gam_put_state_device(gam_handle *h, int which, double time, const double *Gas, const double *magFlux);
 int  gam_set_clock(gam_handle *h, const gam_clock *c);
 int  gam_get_clock(gam_handle *h, gam_clock *c);
 

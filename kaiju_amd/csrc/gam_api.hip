@@ -421,6 +421,28 @@ extern "C" int gam_put_state(gam_handle *h, int which, double time, const double
     return 0;
 }
 
+extern "C" int gam_put_state_device(gam_handle *h, int which, double time, const double *Gas, const double *magFlux) {
+    // FP64 device sources; Bxyz = bFlux2Fld(magFlux) on the device
+    if (!h || which < 0 || which > 1 || !Gas || !magFlux) return -1;
+    void *g, *m, *b;
+    state_ptrs(h, which, &g, &m, &b);
+    const size_t nG5 = 5 * gsz(h);
+    if (h->prec)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cvt<double, float>), dim3((unsigned)((nG5 + 255) / 256)), 256, 0, h->stream,
+                           Gas, (float *)g, nG5);
+    else
+        HIPCHK(h, hipMemcpyAsync(g, Gas, nG5 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(m, magFlux, 3 * fsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (h->prec) launch_b2f<float, double>(h, ST_BPOST, (const double *)m, b);
+    else launch_b2f<double, double>(h, ST_BPOST, (const double *)m, b);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost));
+    if (which == 0) h->clk_host->stime = time; else h->clk_host->otime = time;
+    HIPCHK(h, hipMemcpy(h->clk, h->clk_host, sizeof(DClock), hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int gam_get_state(gam_handle *h, int which, double *time, double *Gas, double *magFlux, double *Bxyz) {
     if (!h || which < 0 || which > 2) return -1;
     void *g, *m, *b;
@@ -638,7 +660,7 @@ extern "C" int gam_set_correction(gam_handle *h, const float *o, double scale) {
     return 0;
 }
 
-extern "C" int gam_correction_target(gam_handle *h, const double *dGas, const double *dMag) {
+static int corr_target(gam_handle *h, const double *dGas, const double *dMag, bool dev) {
     if (!h || !dGas || !dMag) return -1;
     const size_t G = gsz(h), F = fsz(h);
     if (!h->ctG) {
@@ -647,10 +669,23 @@ extern "C" int gam_correction_target(gam_handle *h, const double *dGas, const do
         HIPCHK(h, hipMalloc(&h->crG, 5 * G * sizeof(double)));
         HIPCHK(h, hipMalloc(&h->crM, 3 * F * sizeof(double)));
     }
-    int rc;
-    if ((rc = h2d(h, h->ctG, dGas, 5 * G))) return rc;
-    if ((rc = h2d(h, h->ctM, dMag, 3 * F))) return rc;
+    if (dev) {
+        HIPCHK(h, hipMemcpyAsync(h->ctG, dGas, 5 * G * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemcpyAsync(h->ctM, dMag, 3 * F * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    } else {
+        int rc;
+        if ((rc = h2d(h, h->ctG, dGas, 5 * G))) return rc;
+        if ((rc = h2d(h, h->ctM, dMag, 3 * F))) return rc;
+    }
     return gam_sync(h);
+}
+
+extern "C" int gam_correction_target(gam_handle *h, const double *dGas, const double *dMag) {
+    return corr_target(h, dGas, dMag, false);
+}
+
+extern "C" int gam_correction_target_device(gam_handle *h, const double *dGas, const double *dMag) {
+    return corr_target(h, dGas, dMag, true);
 }
 
 template <typename R>

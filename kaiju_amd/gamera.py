@@ -154,6 +154,8 @@ def lib():
         "gam_device_free": (C.c_int, [C.c_void_p]),
         "gam_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
         "gam_correction_target": (C.c_int, [H, _dp, _dp]),
+        "gam_correction_target_device": (C.c_int, [H, C.c_void_p, C.c_void_p]),
+        "gam_put_state_device": (C.c_int, [H, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),
         "gam_correction_grad": (C.c_int, [H, C.c_void_p, C.c_double, C.c_double, _dp, C.c_void_p,
                                           C.POINTER(C.c_double)]),
     }
@@ -348,6 +350,10 @@ class GamApp:
     def put_state(self, which: int, time: float, Gas=None, magFlux=None, Bxyz=None):
         self._chk(lib().gam_put_state(self.h, which, time, _p(Gas), _p(magFlux), _p(Bxyz)), "gam_put_state")
 
+    def put_state_device(self, which: int, time: float, gas_ptr: int, mag_ptr: int):
+        """State (0) or oState (1) from FP64 device arrays; Bxyz recomputed."""
+        self._chk(lib().gam_put_state_device(self.h, which, time, gas_ptr, mag_ptr), "gam_put_state_device")
+
     def get_state(self, which: int = 0):
         Gas = np.zeros(self.G + (5,), order="F")
         mag = np.zeros(self.F + (3,), order="F")
@@ -416,6 +422,9 @@ class GamApp:
         dMag = np.asfortranarray(dMag, dtype=np.float64)
         assert dGas.shape == self.G + (5,) and dMag.shape == self.F + (3,)
         self._chk(lib().gam_correction_target(self.h, _p(dGas), _p(dMag)), "gam_correction_target")
+
+    def correction_target_device(self, gas_ptr: int, mag_ptr: int):
+        self._chk(lib().gam_correction_target_device(self.h, gas_ptr, mag_ptr), "gam_correction_target_device")
 
     def correction_grad(self, o_ptr: int, scale: float, dt: float, weights, dout_ptr: int) -> float:
         """Loss of the linearised one-step change of o against the target; dL/do into dout_ptr."""

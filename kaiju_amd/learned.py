@@ -15,13 +15,16 @@ downsampled 2x: conserved variables averaged over 2x2x2 fine cells, face
 fluxes summed over 2x2 fine faces (both exact on the Cartesian grid, so the
 coarse magFlux keeps div.B = 0).  The target of the coarse step from sample n
 is the downsampled fine state at n+1 minus the plain coarse step from the
-downsampled states n (and n-1 as the AB2 predictor's old state).  The loss is
-that of the linearised one-step change that o makes (``gam_correction_grad``:
-exact for density, momentum and magFlux, first order for the energy), and its
-gradient runs back through the network (``gam_net_backward``) into Adam
-(``gam_net_adam``); every stage runs on the GPU.
+downsampled states n (and n-1 as the AB2 predictor's old state).  Samples stay
+resident on the GPU.  The loss is that of the linearised one-step change that
+o makes (``gam_correction_grad``: exact for density, momentum and magFlux,
+first order for the energy), and its gradient runs back through the network
+(``gam_net_backward``) into Adam (``gam_net_adam``); every stage runs on the
+GPU.  After the first phase, aggregation rounds add samples on the corrected
+model's own rollout states (target: the downsampled fine state one step
+later), so the network also learns to undo the drift of its rollouts.
 
-    python -m kaiju_amd.learned [--N 32] [--train-seeds 0,1,2] [--test-seed 7] [--steps 24] [--iters 1500]
+    python -m kaiju_amd.learned [--N 32] [--train-seeds 0,1,2,3] [--test-seed 100] [--steps 24] [--iters 1500] [--dagger 0]
 
 prints one JSON line with the one-step loss ratio and the plain and corrected
 errors (one step from each sample, and a rollout from sample 0) against the
@@ -40,7 +43,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from kaiju_amd import configs
-from kaiju_amd.gamera import GamApp
+from kaiju_amd.gamera import DeviceArray, GamApp
 
 GAMMA = 5.0 / 3.0
 PRIM = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
@@ -60,11 +63,11 @@ class TurbIC:
     cosine modes with integer wave numbers |n_x|,|n_y|,|n_z| <= kmax, scaled to
     the rms v_rms and rms(curl A) = b_rms, plus a uniform field b_mean."""
     seed: int
-    kmax: int = 3
-    modes: int = 10
-    v_rms: float = 0.8
-    b_rms: float = 0.5
-    b_mean: Sequence[float] = (0.4, 0.0, 0.0)
+    kmax: int = 4
+    modes: int = 12
+    v_rms: float = 1.0
+    b_rms: float = 1.0
+    b_mean: Sequence[float] = (0.0, 0.0, 0.0)
 
     def _draw(self, rng):
         out = []
@@ -189,7 +192,7 @@ def l1_total(e: Dict[str, float], norm: Dict[str, float]) -> float:
 # ----------------------------------------------------------------- trajectories
 @dataclasses.dataclass
 class Trajectory:
-    """Downsampled fine states at t_n = n * dt (coarse step dt)."""
+    """Downsampled fine states at t_n = n * dt (coarse step dt), host arrays."""
     N: int
     nG: int
     dt: float
@@ -211,13 +214,14 @@ def set_time(app: GamApp, t: float, dt: float, ts: int):
 def fine_trajectory(ic: TurbIC, N: int, steps: int, dt: Optional[float] = None, nG: int = 4) -> Trajectory:
     """Run the solver at (2N)^3 with dt/2 for `steps` coarse intervals and
     downsample every second state."""
+    from kaiju_amd.gamera import gen_corners
     dt = choose_dt(ic, N) if dt is None else dt
     gas, mag = ic.fields(2 * N, nG)
     t0 = time.perf_counter()
     with GamApp(turb_spec(2 * N, 0, nG)) as app:
-        from kaiju_amd.gamera import gen_corners
         app.set_corners(gen_corners(app.spec))
         app.put_state(0, 0.0, gas, mag, None)
+        del gas, mag
         set_time(app, 0.0, 0.5 * dt, 0)
         app.hatch()
         out_g, out_m = [], []
@@ -249,18 +253,80 @@ def load_pair(app: GamApp, tr: Trajectory, n: int):
     set_time(app, t, dt, n)
 
 
-def targets(app: GamApp, tr: Trajectory):
-    """Per sample n < last: (plain coarse step from n) and target = sample n+1 - plain."""
-    tg, tm = [], []
+def plain_step(app: GamApp, gas, mag, ogas, omag, t: float, dt: float, ts: int):
+    """One uncorrected coarse step from (State, oState); the new Gas and magFlux."""
+    app.put_state(0, t, gas, mag, None)
+    app.put_state(1, t - dt, ogas, omag, None)
+    set_time(app, t, dt, ts)
+    app.set_correction(None)
+    if app.steps(1) != 0:
+        raise RuntimeError(f"coarse step at t = {t} halted")
+    s = app.get_state(0)
+    return s["Gas"], s["magFlux"]
+
+
+# ----------------------------------------------------------------- training samples
+def _flat(a: np.ndarray) -> np.ndarray:
+    return np.ravel(a, order="F")      # the library's memory order
+
+
+class Sample:
+    """One training sample resident on the GPU: the state the network sees
+    (Gas, magFlux, FP64) and the target change of its step."""
+
+    def __init__(self, t: float, gas, mag, tg, tm):
+        self.t = t
+        self.gas, self.mag = DeviceArray.of(_flat(gas), np.float64), DeviceArray.of(_flat(mag), np.float64)
+        self.tg, self.tm = DeviceArray.of(_flat(tg), np.float64), DeviceArray.of(_flat(tm), np.float64)
+
+    def free(self):
+        for d in (self.gas, self.mag, self.tg, self.tm):
+            d.free()
+
+
+def trajectory_targets(app: GamApp, tr: Trajectory):
+    """Host targets (Gas, magFlux) of the plain coarse step from each sample n < last."""
+    out = []
     for n in range(len(tr.gas) - 1):
-        load_pair(app, tr, n)
+        m = max(n - 1, 0)
+        g1, m1 = plain_step(app, tr.gas[n], tr.mag[n], tr.gas[m], tr.mag[m], n * tr.dt, tr.dt, n)
+        out.append((tr.gas[n + 1] - g1, tr.mag[n + 1] - m1))
+    return out
+
+
+def trajectory_samples(tr: Trajectory, tgts) -> List[Sample]:
+    return [Sample(n * tr.dt, tr.gas[n], tr.mag[n], tg, tm) for n, (tg, tm) in enumerate(tgts)]
+
+
+def rollout_samples(app: GamApp, tr: Trajectory, corr: "Corrector", steps: int) -> List[Sample]:
+    """Samples on the corrected model's own states: roll out from sample 0 with
+    the correction, then target each visited state's plain step at the
+    downsampled fine state one step later (the correction learns to undo the
+    drift its rollouts accumulate)."""
+    app.set_correction(None)
+    app.put_state(0, 0.0, tr.gas[0], tr.mag[0], None)
+    app.put_state(1, -tr.dt, tr.gas[0], tr.mag[0], None)
+    set_time(app, 0.0, tr.dt, 0)
+    visited = []
+    try:
+        for n in range(steps):
+            s0, s1 = app.get_state(0), app.get_state(1)
+            if not (np.isfinite(s0["Gas"]).all() and np.isfinite(s0["magFlux"]).all()):
+                break                                  # the rollout left the physical states: stop there
+            visited.append((s0["Gas"], s0["magFlux"], s1["Gas"], s1["magFlux"]))
+            app.set_correction(corr.infer(), corr.norm.scale)
+            if app.steps(1) != 0:
+                break
+    finally:
         app.set_correction(None)
-        if app.steps(1) != 0:
-            raise RuntimeError(f"coarse step from sample {n} halted")
-        s = app.get_state(0)
-        tg.append(tr.gas[n + 1] - s["Gas"])
-        tm.append(tr.mag[n + 1] - s["magFlux"])
-    return tg, tm
+    out = []
+    for n, (g, m, og, om) in enumerate(visited):
+        g1, m1 = plain_step(app, g, m, og, om, n * tr.dt, tr.dt, n)
+        tg, tm = tr.gas[n + 1] - g1, tr.mag[n + 1] - m1
+        if not (np.isfinite(tg).all() and np.isfinite(tm).all()):
+            break
+        out.append(Sample(n * tr.dt, g, m, tg, tm))
+    return out
 
 
 # ----------------------------------------------------------------- training
@@ -272,23 +338,32 @@ class Normalisation:
     feat_offsets: tuple                # rho0, p0
 
 
-def normalisation(trs: List[Trajectory], tgs, tms) -> Normalisation:
+def normalisation(trs: List[Trajectory], tgts) -> Normalisation:
     N, nG, dt = trs[0].N, trs[0].nG, trs[0].dt
     dx = 1.0 / N
     a = slice(nG, nG + N)
-    G = np.concatenate([t[a, a, a].reshape(-1, 5) for tg in tgs for t in tg])
-    M = np.concatenate([t[a, a, a].reshape(-1, 3) for tm in tms for t in tm])
-    r = lambda x: float(np.sqrt(np.mean(x * x))) + 1e-300
-    rho_r, mom_r, en_r, mag_r = r(G[:, 0]), r(G[:, 1:4]), r(G[:, 4]), r(M)
+    acc = np.zeros(4)
+    cnt = np.zeros(4)
+    for tl in tgts:
+        for tg, tm in tl:
+            g = tg[a, a, a]
+            acc += [np.sum(g[..., 0] ** 2), np.sum(g[..., 1:4] ** 2), np.sum(g[..., 4] ** 2), np.sum(tm[a, a, a] ** 2)]
+            cnt += [g[..., 0].size, g[..., 1:4].size, g[..., 4].size, tm[a, a, a].size]
+    rho_r, mom_r, en_r, mag_r = np.sqrt(acc / cnt) + 1e-300
     ncell = N ** 3
     w = np.array([1.0 / rho_r ** 2, 1.0 / mom_r ** 2, 1.0 / en_r ** 2, 1.0 / mag_r ** 2]) / ncell
-    scale = max(rho_r, mom_r, en_r, mag_r / dx ** 2) * dx / dt
-    P = [primitives(g, cell_b(m, nG, dx), nG) for tr in trs for g, m in zip(tr.gas, tr.mag)]
-    st = lambda ks: float(np.sqrt(np.mean([np.mean(p[k] ** 2) for p in P for k in ks])))
-    rho0 = float(np.mean([p["rho"].mean() for p in P]))
-    p0 = float(np.mean([p["p"].mean() for p in P]))
-    fs = (float(np.sqrt(np.mean([np.mean((p["rho"] - rho0) ** 2) for p in P]))) + 1e-6, st(("vx", "vy", "vz")),
-          float(np.sqrt(np.mean([np.mean((p["p"] - p0) ** 2) for p in P]))) + 1e-6, st(("bx", "by", "bz")))
+    scale = float(max(rho_r, mom_r, en_r, mag_r / dx ** 2) * dx / dt)
+    stats = {k: [] for k in PRIM}
+    for tr in trs:
+        for g, m in zip(tr.gas, tr.mag):
+            p = primitives(g, cell_b(m, nG, dx), nG)
+            for k in PRIM:
+                stats[k].append((float(p[k].mean()), float(np.mean(p[k] ** 2))))
+    mean = lambda k: float(np.mean([s[0] for s in stats[k]]))
+    msq = lambda ks: float(np.mean([s[1] for k in ks for s in stats[k]]))
+    rho0, p0 = mean("rho"), mean("p")
+    fs = (math.sqrt(max(msq(("rho",)) - rho0 ** 2, 0.0)) + 1e-6, math.sqrt(msq(("vx", "vy", "vz"))),
+          math.sqrt(max(msq(("p",)) - p0 ** 2, 0.0)) + 1e-6, math.sqrt(msq(("bx", "by", "bz"))))
     return Normalisation(scale=scale, weights=w, feat_scales=fs, feat_offsets=(rho0, p0))
 
 
@@ -303,9 +378,11 @@ class Corrector:
         self.net.set_scales(*norm.feat_scales, gamma=GAMMA)
         self.net.set_offsets(*norm.feat_offsets)
         self.precision = app.spec.model.precision
+        self.zero = DeviceArray.of(np.zeros(int(np.prod(app.G)) * 8, np.float32))
 
     def close(self):
         self.net.close()
+        self.zero.free()
 
     def infer(self):
         """Correction field for the app's current State (device, FP32 [G][8])."""
@@ -315,55 +392,40 @@ class Corrector:
         self.net.sync()
         return self.net.output_ptr()
 
-    def sample_grad(self, tr: Trajectory, n: int, tg, tm) -> float:
-        """Loss of sample n at the current parameters; accumulates the gradient."""
-        self.app.put_state(0, n * tr.dt, tr.gas[n], tr.mag[n], None)
-        o = self.infer()
-        self.app.correction_target(tg, tm)
+    def _loss(self, smp: Sample, dt: float, o_ptr=None, backward=False) -> float:
+        self.app.put_state_device(0, smp.t, smp.gas.ptr, smp.mag.ptr)
+        o = self.infer() if o_ptr is None else o_ptr
+        self.app.correction_target_device(smp.tg.ptr, smp.tm.ptr)
         dbuf = self.net.grad_buffer_ptr()
-        loss = self.app.correction_grad(o, self.norm.scale, tr.dt, self.norm.weights, dbuf)
-        self.net.backward(dbuf)
+        loss = self.app.correction_grad(o, self.norm.scale, dt, self.norm.weights, dbuf)
+        if backward and math.isfinite(loss):
+            self.net.backward(dbuf)
         return loss
 
-    def train(self, trs: List[Trajectory], tgs, tms, iters: int, lr: float = 1e-3, batch: int = 4, seed: int = 0,
+    def train(self, pool: List[Sample], dt: float, iters: int, lr: float = 1e-3, batch: int = 4, seed: int = 0,
               log=None) -> List[float]:
         rng = np.random.default_rng(seed)
-        pool = [(q, n) for q, tr in enumerate(trs) for n in range(len(tr.gas) - 1)]
         hist = []
         for it in range(1, iters + 1):
             self.net.zero_grad()
-            tot = 0.0
-            for _ in range(batch):
-                q, n = pool[rng.integers(len(pool))]
-                tot += self.sample_grad(trs[q], n, tgs[q][n], tms[q][n])
-            rate = lr * (0.1 ** (it / iters))          # exponential decay to lr/10
-            self.net.adam(rate, it)
-            hist.append(tot / batch)
+            losses = [self._loss(pool[rng.integers(len(pool))], dt, backward=True) for _ in range(batch)]
+            good = [l for l in losses if math.isfinite(l)]
+            if not good:
+                raise RuntimeError("every sample of a minibatch has a non-finite loss")
+            self.net.adam(lr * (0.1 ** (it / iters)), self.step + it)   # lr decays 10x over a phase
+            hist.append(float(np.mean(good)))
+            if it % 100 == 0 and not np.isfinite(self.net.get_params()[0]).all():
+                raise RuntimeError(f"network parameters became non-finite at iteration {it}")
             if log and (it % max(1, iters // 10) == 0 or it == 1):
                 log(f"[learned] iter {it} loss {hist[-1]:.4f}")
+        self.step += iters
         return hist
 
-    def zero_loss(self, tr: Trajectory, tgs, tms) -> float:
-        """Loss of o = 0 averaged over a trajectory's samples (sum w T^2)."""
-        w = self.norm.weights
-        N, nG = tr.N, tr.nG
-        a = slice(nG, nG + N)
-        out = []
-        for tg, tm in zip(tgs, tms):
-            g = tg[a, a, a]
-            out.append(w[0] * np.sum(g[..., 0] ** 2) + w[1] * np.sum(g[..., 1:4] ** 2) + w[2] * np.sum(g[..., 4] ** 2)
-                       + w[3] * face_sq(tm, N, nG))
-        return float(np.mean(out))
+    step = 0
 
-    def eval_loss(self, tr: Trajectory, tgs, tms) -> float:
-        """Mean loss over a trajectory's samples at the current parameters (no update)."""
-        vals = []
-        for n in range(len(tr.gas) - 1):
-            self.app.put_state(0, n * tr.dt, tr.gas[n], tr.mag[n], None)
-            o = self.infer()
-            self.app.correction_target(tgs[n], tms[n])
-            vals.append(self.app.correction_grad(o, self.norm.scale, tr.dt, self.norm.weights, self.net.grad_buffer_ptr()))
-        return float(np.mean(vals))
+    def mean_loss(self, samples: List[Sample], dt: float, zero: bool = False) -> float:
+        """Mean loss over samples at the current parameters (zero: of o = 0)."""
+        return float(np.mean([self._loss(s, dt, self.zero.ptr if zero else None) for s in samples]))
 
 
 def face_sq(tm: np.ndarray, N: int, nG: int) -> float:
@@ -419,76 +481,96 @@ def var_norm(tr: Trajectory) -> Dict[str, float]:
     return {k: float(np.mean([np.mean(np.abs(p[k] - p[k].mean())) for p in P])) + 1e-12 for k in PRIM}
 
 
-def experiment(N: int = 32, train_seeds=(0, 1, 2), test_seed: int = 7, steps: int = 24, iters: int = 1500,
-               lr: float = 1e-3, batch: int = 4, precision: int = 0, log=None) -> dict:
-    """Train on the train seeds' trajectories, roll out the test seed with and without the correction."""
+def experiment(N: int = 32, train_seeds=(0, 1, 2, 3), test_seed: int = 100, steps: int = 24, iters: int = 1500,
+               lr: float = 1e-3, batch: int = 4, precision: int = 0, dagger: int = 0, log=None,
+               ic: Optional[dict] = None) -> dict:
+    """Train on the train seeds' trajectories (then `dagger` rounds on the
+    corrected model's own rollouts), evaluate on a held-out seed.  `ic`
+    overrides TurbIC's amplitudes and wave numbers."""
     log = log or (lambda *a: None)
-    ics = [TurbIC(seed=s) for s in train_seeds]
-    dt = min(choose_dt(ic, N) for ic in ics + [TurbIC(seed=test_seed)])
+    ic = ic or {}
+    ics = [TurbIC(seed=s, **ic) for s in train_seeds]
+    ic_test = TurbIC(seed=test_seed, **ic)
+    dt = min(choose_dt(x, N) for x in ics + [ic_test])
     t0 = time.perf_counter()
-    trs = [fine_trajectory(ic, N, steps, dt) for ic in ics]
-    test = fine_trajectory(TurbIC(seed=test_seed), N, steps, dt)
+    trs = [fine_trajectory(x, N, steps, dt) for x in ics]
+    test = fine_trajectory(ic_test, N, steps, dt)
     t_fine = time.perf_counter() - t0
     log(f"[learned] fine trajectories {t_fine:.1f} s")
     app = coarse_app(N, precision)
+    pool, test_samples, corr = [], [], None
     try:
-        tgs, tms = [], []
-        for tr in trs:
-            g, m = targets(app, tr)
-            tgs.append(g)
-            tms.append(m)
-        tg_test, tm_test = targets(app, test)
-        norm = normalisation(trs, tgs, tms)
+        tgts = [trajectory_targets(app, tr) for tr in trs]
+        norm = normalisation(trs, tgts)
+        for tr, tl in zip(trs, tgts):
+            pool += trajectory_samples(tr, tl)
+        del tgts
+        test_samples = trajectory_samples(test, trajectory_targets(app, test))
         corr = Corrector(app, norm)
-        try:
-            L0 = corr.zero_loss(test, tg_test, tm_test)
-            t0 = time.perf_counter()
-            hist = corr.train(trs, tgs, tms, iters, lr=lr, batch=batch, log=log)
-            t_train = time.perf_counter() - t0
-            L1 = corr.eval_loss(test, tg_test, tm_test)
-            Ltr0 = corr.zero_loss(trs[0], tgs[0], tms[0])
-            Ltr1 = corr.eval_loss(trs[0], tgs[0], tms[0])
-            os_plain = one_step_errors(app, test)
-            os_corr = one_step_errors(app, test, corr)
-            plain = rollout(app, test, steps)
-            fixed = rollout(app, test, steps, corr)
-        finally:
-            corr.close()
+        L0 = corr.mean_loss(test_samples, dt, zero=True)
+        t0 = time.perf_counter()
+        hist = corr.train(pool, dt, iters, lr=lr, batch=batch, log=log)
+        n_traj = len(pool)
+        cap = 50.0 * corr.mean_loss(pool, dt, zero=True)
+        for r in range(dagger):
+            for tr in trs:
+                for smp in rollout_samples(app, tr, corr, steps):
+                    if corr.mean_loss([smp], dt, zero=True) <= cap:   # drop states far off the data
+                        pool.append(smp)
+                    else:
+                        smp.free()
+            log(f"[learned] aggregation round {r + 1}: {len(pool)} samples")
+            hist += corr.train(pool, dt, max(1, iters // 2), lr=lr, batch=batch, seed=r + 1, log=log)
+        t_train = time.perf_counter() - t0
+        n_pool = len(pool)
+        L1 = corr.mean_loss(test_samples, dt)
+        os_plain = one_step_errors(app, test)
+        os_corr = one_step_errors(app, test, corr)
+        plain = rollout(app, test, steps)
+        fixed = rollout(app, test, steps, corr)
     finally:
+        for smp in pool + test_samples:
+            smp.free()
+        if corr is not None:
+            corr.close()
         app.close()
     vn = var_norm(test)
     op = [l1_total(e, vn) for e in os_plain]
     oc = [l1_total(e, vn) for e in os_corr]
     ep = [l1_total(e, vn) for e in plain]
     ec = [l1_total(e, vn) for e in fixed]
-    return dict(N=N, steps=steps, iters=iters, batch=batch, precision=precision, dt=dt,
+    n_iter = len(hist)
+    return dict(N=N, steps=steps, iters=iters, dagger=dagger, batch=batch, precision=precision, dt=dt,
                 train_seeds=list(train_seeds), test_seed=test_seed,
-                loss_start=hist[0], loss_end=float(np.mean(hist[-max(1, iters // 20):])),
-                test_loss_zero=L0, test_loss=L1, test_loss_ratio=L1 / L0, train_loss_ratio=Ltr1 / Ltr0,
+                loss_start=hist[0], loss_end=float(np.mean(hist[-max(1, n_iter // 20):])),
+                samples_trajectory=n_traj, samples_total=n_pool,
+                test_loss_zero=L0, test_loss=L1, test_loss_ratio=L1 / L0,
                 one_step_gain=float(np.mean(op) / np.mean(oc)),
                 one_step_gain_per_var={k: float(np.mean([e[k] for e in os_plain]) / np.mean([e[k] for e in os_corr]))
                                        for k in PRIM},
                 err_plain_final=ep[-1], err_corrected_final=ec[-1],
                 err_plain_mean=float(np.mean(ep)), err_corrected_mean=float(np.mean(ec)),
                 gain_final=ep[-1] / ec[-1], gain_mean=float(np.mean(ep) / np.mean(ec)),
+                rollout_gain_per_step=[round(a / b, 4) for a, b in zip(ep, ec)],
                 plain_per_var=plain[-1], corrected_per_var=fixed[-1],
-                fine_s=round(t_fine, 2), train_s=round(t_train, 2), ms_per_iter=round(t_train / iters * 1e3, 3))
+                fine_s=round(t_fine, 2), train_s=round(t_train, 2), ms_per_iter=round(t_train / n_iter * 1e3, 3))
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--N", type=int, default=32)
-    ap.add_argument("--train-seeds", default="0,1,2")
-    ap.add_argument("--test-seed", type=int, default=7)
+    ap.add_argument("--train-seeds", default="0,1,2,3")
+    ap.add_argument("--test-seed", type=int, default=100)
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--iters", type=int, default=1500)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--dagger", type=int, default=0)
     ap.add_argument("--fp32", action="store_true")
     a = ap.parse_args(argv)
     log = lambda *m: print(*m, file=sys.stderr, flush=True)
     r = experiment(a.N, tuple(int(s) for s in a.train_seeds.split(",")), a.test_seed, a.steps, a.iters, a.lr,
-                   a.batch, 1 if a.fp32 else 0, log)
+                   a.batch, 1 if a.fp32 else 0, a.dagger, log)
     print(json.dumps(r), flush=True)
 
 

@@ -182,7 +182,8 @@ def test_trained_correction_lowers_held_out_one_step_error():
     if on_emulation():
         pytest.skip("the network runs only as MFMA code")
     say("[learned] train 16^3 on 8 seeds, test on a held-out seed")
-    r = learned.experiment(N=16, train_seeds=tuple(range(8)), test_seed=100, steps=16, iters=800, batch=4, log=say)
+    r = learned.experiment(N=16, train_seeds=tuple(range(8)), test_seed=100, steps=16, iters=800, batch=4, log=say,
+                           ic=dict(kmax=3, v_rms=0.8, b_rms=0.5, b_mean=(0.4, 0.0, 0.0)))
     say(f"   {r}")
     assert r["test_loss_ratio"] < 0.85
     assert r["one_step_gain"] > 1.03
