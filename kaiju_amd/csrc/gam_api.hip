@@ -820,8 +820,8 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     { StageTimer t(h, ST_BHALF); launch_b2f<R, R>(h, ST_BHALF, P<R>(h->hmag), h->hB); }
     if (m.doMHD) {
         { StageTimer t(h, ST_MOM2FACE);
-          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[ST_MOM2FACE],
-                             h->clk, P<R>(h->wk[W_VDV]), P<R>(h->Vf)); }
+          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g,
+                             h->tile[ST_MOM2FACE], h->clk, P<R>(h->wk[W_VDV]), P<R>(h->Vf)); }
         if (m.rmeth == 1) { if (m.doBoris) launch_emfs<1, 1, R>(h); else launch_emfs<1, 0, R>(h); }
         else { if (m.doBoris) launch_emfs<0, 1, R>(h); else launch_emfs<0, 0, R>(h); }
     }
@@ -871,15 +871,27 @@ static void launch_bc(gam_handle *h, bool withB) {
                        h->mag[S], withB ? P<R>(h->B[S]) : (R *)nullptr);
 }
 
+// shell: after a fused step's update only the ghost cells' Bxyz can change
+// (k_b2f_shell); the stand-alone EnforceBCs recomputes every cell.
 template <typename R>
-static int bcs_t(gam_handle *h) {
+static int bcs_t(gam_handle *h, bool shell) {
     const int S = h->s;
     { StageTimer t(h, ST_BC); launch_bc<R>(h, false); }
-    { StageTimer t(h, ST_BPOST); launch_b2f<R, double>(h, ST_BPOST, h->mag[S], h->B[S]); }
+    {
+        StageTimer t(h, ST_BPOST);
+        const GVT<R> &g = view<R>(h);
+        if (shell && !h->mp.do25D) {
+            const long n = b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f_shell<R>), dim3((unsigned)((n + 255) / 256)), 256, 0, h->stream, g, h->clk,
+                               (const double *)h->mag[S], P<R>(h->B[S]));
+        } else {
+            launch_b2f<R, double>(h, ST_BPOST, h->mag[S], h->B[S]);
+        }
+    }
     HIPCHK(h, hipGetLastError());
     return 0;
 }
-static int bcs(gam_handle *h) { return h->prec ? bcs_t<float>(h) : bcs_t<double>(h); }
+static int bcs(gam_handle *h, bool shell = false) { return h->prec ? bcs_t<float>(h, shell) : bcs_t<double>(h, shell); }
 
 extern "C" int gam_advance_mhd(gam_handle *h) {
     if (!h) return -1;
@@ -918,7 +930,7 @@ extern "C" int gam_step(gam_handle *h, int nsteps) {
         int rc = advance(h, 1);
         if (rc) return rc;
         launch_dtfinal(h, 1, 1, update_blocks(h));   // block minima were left in h->partial by k_update
-        if ((rc = bcs(h))) return rc;
+        if ((rc = bcs(h, true))) return rc;
     }
     gam_clock c;
     int rc = gam_get_clock(h, &c);
@@ -1052,7 +1064,7 @@ extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_gr
         for (int k = 0; k < nsteps && !rc; k++) {
             rc = advance(h, 1);
             if (!rc) launch_dtfinal(h, 1, 1, update_blocks(h));
-            if (!rc) rc = bcs(h);
+            if (!rc) rc = bcs(h, true);
         }
         if (!rc && (e = hipEventRecord(g->evs[1 + q], h->stream)) != hipSuccess) rc = -2;
         if (!rc && (e = hipStreamWaitEvent(g->stream, g->evs[1 + q], 0)) != hipSuccess) rc = -2;
@@ -1184,8 +1196,11 @@ extern "C" int gam_stage_bytes(gam_handle *h, int stage, double *bytes) {
         d = F * 6; b = F * (3 + 3 + 3 + 3) + G * (5 + 5 + 1 + 5 + 5 + 5 + 3); break;
     case ST_BHALF:   // hmag (b), xfc, xyzcc, vol (d) -> B (b)
         b = F * 3 + G * 3; d = F * 9 + G * (3 + 1); break;
-    case ST_BPOST:   // mag, xfc, xyzcc, vol (d) -> B (b)
-        b = G * 3; d = F * (3 + 9) + G * (3 + 1); break;
+    case ST_BPOST: {   // mag, xfc, xyzcc, vol (d) -> B (b); the fused 3D step recomputes the ghost shell only
+        const GV &g = h->gv;
+        const double S = h->mp.do25D ? G : (double)b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
+        b = S * 3; d = S * (3 + 9 + 3 + 1); break;
+    }
     case ST_MOM2FACE: b = G * (3 + 1 + 9); break;                                 // VdV, vol -> Vf (9)
     case ST_EMF_I: case ST_EMF_J: case ST_EMF_K: {   // Vf (dT1), rho ; face, bn, bh (dT1, dT2), Te, Teb, edge -> E
         const int e = stage - ST_EMF_I;

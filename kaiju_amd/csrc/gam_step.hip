@@ -140,6 +140,52 @@ __global__ void __launch_bounds__(256) k_b2f(GVT<R> g, Tile t, const DClock *__r
     for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = R(b[x]);
 }
 
+// bFlux2Fld on the ghost cells only.  Used after the periodic BCs of a fused
+// 3D step: those BCs rewrite ghost faces only (an active cell's six faces are
+// owned faces), so the active cells' Bxyz from k_update, computed by the same
+// cell_bxyz from the same faces, is already what a full pass would write.
+// Shell order: the k ghost planes, then the j ghost rows of the active planes,
+// then the i ghosts of the active rows.
+__host__ __device__ __forceinline__ long b2f_shell_count(int Ni, int Nj, int Nk, int Nip, int Njp, int Nkp) {
+    return (long)(Nk - Nkp) * Ni * Nj + (long)Nkp * (Nj - Njp) * Ni + (long)Nkp * Njp * (Ni - Nip);
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_b2f_shell(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ mag,
+                                                   R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ni = g.Ni, Nj = g.Nj, a0 = g.nG;
+    const long nplane = (long)(g.Nk - g.Nkp) * Ni * Nj;
+    const long nrow = (long)g.Nkp * (Nj - g.Njp) * Ni;
+    const long nig = (long)g.Nkp * g.Njp * (Ni - g.Nip);
+    if (idx >= nplane + nrow + nig || clk->halt) return;
+    int i, j, k;
+    if (idx < nplane) {
+        const int p = (int)(idx / ((long)Ni * Nj)), r = (int)(idx % ((long)Ni * Nj));
+        k = p < a0 ? p : p + g.Nkp;
+        j = r / Ni;
+        i = r % Ni;
+    } else if (idx < nplane + nrow) {
+        const long q = idx - nplane;
+        const int row = (int)(q / Ni), gj = Nj - g.Njp;
+        i = (int)(q % Ni);
+        k = a0 + row / gj;
+        const int t = row % gj;
+        j = t < a0 ? t : t + g.Njp;
+    } else {
+        const long q = idx - nplane - nrow;
+        const int gi = Ni - g.Nip;
+        const int t = (int)(q % gi), r = (int)(q / gi);
+        i = t < a0 ? t : t + g.Nip;
+        j = a0 + r % g.Njp;
+        k = a0 + r / g.Njp;
+    }
+    const int c = ci(g, i, j, k);
+    double b[3];
+    cell_bxyz(g, mag, i, j, k, b);
+#pragma unroll
+    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + c] = R(b[x]);
+}
+
 // ----------------------------------------------------------------- E field
 // fields.F90:235-285 Mom2Face at (i,j,k) along dT, c = cell index; the
 // per-cell summand vol*M/max(D,dFloor) comes precomputed in VdV
