@@ -253,7 +253,7 @@ __device__ __forceinline__ void edge_lrs(const GVT<R> &g, const MP &m, const R *
     for (int n = 0; n < 4; n++) nMagB[n] = bn[off + f + (n - 2) * st];
     R Vi = c8(AreaB);
     R l, r;
-    get_lrs<RM>(R(m.pdmb), AreaB, nMagB, MagB, Vi, l, r);
+    get_lrs<RM>(R(m.pdmb), AreaB, nMagB, MagB, R(1) / Vi, l, r);
     bL = l * Vi; bR = r * Vi; fA = Vi;
 }
 
@@ -352,8 +352,9 @@ __device__ __forceinline__ void magkin(const MP &m, R sg, const R *P5, const R *
     R Vac2 = BORIS ? Va2 / (1 + (Va2 / sq(R(m.Ca)))) : Va2;
     R Nx = T[NORMX], Ny = T[NORMY], Nz = T[NORMZ];
     R Vn = Nx * P5[1] + Ny * P5[2] + Nz * P5[3];
-    R lambda = 1 / (2 * P / D + Vac2);
-    R Vn0 = R(0.5) * rerfc(sg * rroot(lambda) * Vn);
+    // lambda = 1/(2P/D + Vac2); sqrt(lambda)*Vn formed as Vn/sqrt(1/lambda) (one division, same value to an ulp)
+    const R xv = Vn / rroot(2 * P / D + Vac2);
+    R Vn0 = R(0.5) * rerfc_e(sg * xv, rexp(-xv * xv));
     VaD = VaD + R(0.5) * rroot(Vac2);
     mF[0] = mF[0] + Vn0 * (dPb * Nx - Bx * Bn);
     mF[1] = mF[1] + Vn0 * (dPb * Ny - By * Bn);
@@ -375,8 +376,10 @@ __device__ __forceinline__ void gaskin(const MP &m, R sg, const R *P5, const R *
     R Vn = T[NORMX] * Vx + T[NORMY] * Vy + T[NORMZ] * Vz;
     R Vt1 = T[TAN1X] * Vx + T[TAN1Y] * Vy + T[TAN1Z] * Vz;
     R Vt2 = T[TAN2X] * Vx + T[TAN2Y] * Vy + T[TAN2Z] * Vz;
-    R Vn0 = R(0.5) * rerfc(sg * rroot(lambda) * Vn);
-    R Vn1 = Vn * Vn0 - R(0.5) * sg * rexp(-lambda * Vn * Vn) / rroot(R(GAM_PI) * lambda);
+    const R e = rexp(-lambda * Vn * Vn);
+    const R sl = rroot(lambda);   // sqrt(pi*lambda) = sqrt(pi)*sqrt(lambda), to an ulp
+    R Vn0 = R(0.5) * rerfc_e(sg * sl * Vn, e);
+    R Vn1 = Vn * Vn0 - R(0.5) * sg * e / (R(GAM_SQRTPI) * sl);
     gF[0] = gF[0] + D * Vn1;
     gF[4] = gF[4] + (E + R(0.5) * P) * Vn1 + R(0.5) * P * Vn0 * Vn;
     R fMn = D * Vn * Vn1 + P * Vn0;
@@ -415,7 +418,7 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, co
     R VolB[8];
 #pragma unroll
     for (int n = 0; n < 8; n++) VolB[n] = g.vol[c + (n - 4) * st];
-    const R Vi = c8(VolB);
+    const R rVi = R(1) / c8(VolB);
     R T[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) T[q] = g.Tf[nf * (q + 9 * DN) + f];
@@ -432,7 +435,7 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, co
             for (int n = 0; n < 8; n++) Q[n] = hB[nc * d + c + (n - 4) * st];
 #pragma unroll
             for (int n = 0; n < 4; n++) nQ[n] = limB[nc * d + c + (n - 2) * st];
-            get_lrs<RM>(pdmb, VolB, nQ, Q, Vi, MagL[d], MagR[d]);
+            get_lrs<RM>(pdmb, VolB, nQ, Q, rVi, MagL[d], MagR[d]);
         }
         Bn = hmag[nf * DN + f] / faB;
     }
@@ -450,7 +453,7 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, co
         for (int n = 0; n < 8; n++) Q[n] = Wh[nc * v + c + (n - 4) * st];
 #pragma unroll
         for (int n = 0; n < 4; n++) nQ[n] = Wn[nc * v + c + (n - 2) * st];
-        get_lrs<RM>(pdmb, VolB, nQ, Q, Vi, PL[v], PR[v]);
+        get_lrs<RM>(pdmb, VolB, nQ, Q, rVi, PL[v], PR[v]);
     }
     const R Ca = R(m.Ca);
     R VaD = R(0.0);
