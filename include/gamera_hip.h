@@ -140,6 +140,16 @@ int  gam_correction_target_device(gam_handle *h, const double *dGas, const doubl
 int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt, const double *w4,
                          float *dout, double *loss);
 
+/* --- boundary conditions ------------------------------------------------ */
+/* Boundary type per face, in the order INI, OUTI, INJ, OUTJ, INK, OUTK (the
+ * order EnforceBCs applies them, step.F90:17-55): 0 periodic (bcs.F90:279-450),
+ * 1 zero gradient (zeroGrad_*, :799-930), 2 zero extension (zeroExten_*,
+ * :454-603), 3 Cartesian reflecting (cartReflect_*, :607-795).  An axis is
+ * periodic on both faces or on neither; 2.5D blocks keep k periodic.  Default:
+ * all periodic.  The reference's BC objects (Grid%externalBCs) map onto these. */
+int  gam_set_bcs(gam_handle *h, const int *type);
+int  gam_get_bcs(gam_handle *h, int *type);
+
 /* --- state ---------------------------------------------------------------- */
 /* which: 0 = State, 1 = oState, 2 = predictor (half-step) state (read only).
  * Gas G*5, magFlux F*3, Bxyz G*3; NULL skips an array, except that a NULL

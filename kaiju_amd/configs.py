@@ -17,6 +17,10 @@ from typing import Optional, Tuple
 CFL_DEFAULT = 0.30000001192092896
 
 
+# the library's boundary types, by code (gam_set_bcs)
+BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect")
+
+
 @dataclasses.dataclass
 class ModelSpec:
     """Model_T knobs (reference base/types/gamtypes.F90:71-146)."""
@@ -57,6 +61,8 @@ class CaseSpec:
     # background field B0 = bu + dipole(x - xc) with moment (0,0,M0): a dict
     # with keys M0, xc, bu (the reference's Model%doBackground + Model%B0)
     background: Optional[dict] = None
+    # boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK (one of BC_TYPES)
+    bcs: Tuple[str, str, str, str, str, str] = ("periodic",) * 6
 
     @property
     def zones(self) -> int:

@@ -52,6 +52,7 @@ struct gam_handle {
     double *bg[5] = {};        // background field fcB0, edgB0, B0, bFlux0, dpB0 (G-sized)
     const float *corr = nullptr; // learned correction field [G][8] (device), or none
     double corr_scale = 0.0;
+    int bct[6] = {0, 0, 0, 0, 0, 0};   // boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK (GAM_BC_*)
     double *ctG = nullptr, *ctM = nullptr, *crG = nullptr, *crM = nullptr;   // correction training buffers
     BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
     double *partial = nullptr;
@@ -618,6 +619,8 @@ extern "C" int gam_background_dipole(gam_handle *h, double M0, const double *xc,
     return 0;
 }
 
+static bool all_periodic(const gam_handle *h);
+
 // ----------------------------------------------------------------- device buffers
 extern "C" int gam_device_alloc(size_t bytes, void **p) {
     if (!p) return -1;
@@ -653,6 +656,7 @@ extern "C" int gam_state_device(gam_handle *h, int which, void **Gas, double **m
 
 extern "C" int gam_set_correction(gam_handle *h, const float *o, double scale) {
     if (!h) return -1;
+    if (o && !all_periodic(h)) return fail(h, -1, "gam_set_correction: periodic boundaries only");
     int rc = gam_sync(h);
     if (rc) return rc;
     h->corr = o;
@@ -862,13 +866,63 @@ static int advance(gam_handle *h, int fuse_dt) {
     return h->prec ? advance_t<float>(h, fuse_dt) : advance_t<double>(h, fuse_dt);
 }
 
-// periodic BCs on State; when withB, Bxyz ghosts are copied too (hatch)
+static bool all_periodic(const gam_handle *h) {
+    for (int f = 0; f < 6; f++) if (h->bct[f] != GAM_BC_PERIODIC) return false;
+    return true;
+}
+
+// BCs on State, in the reference's order (INI, OUTI, INJ, OUTJ, INK, OUTK);
+// when withB, Bxyz ghosts are written too (hatch).  All-periodic: one pass
+// with the composed per-axis maps; otherwise per axis, a periodic pass or the
+// two faces' own BCs one after the other.
 template <typename R>
 static void launch_bc(gam_handle *h, bool withB) {
     const GVT<R> &g = view<R>(h);
     const int S = h->s;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, P<R>(h->Gas[S]),
-                       h->mag[S], withB ? P<R>(h->B[S]) : (R *)nullptr);
+    R *Bw = withB ? P<R>(h->B[S]) : (R *)nullptr;
+    if (all_periodic(h)) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, 7, P<R>(h->Gas[S]),
+                           h->mag[S], Bw);
+        return;
+    }
+    const int ext[3] = {g.Ni, g.Nj, g.Nk};
+    for (int ax = 0; ax < 3; ax++) {
+        if (h->bct[2 * ax] == GAM_BC_PERIODIC) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, 1 << ax,
+                               P<R>(h->Gas[S]), h->mag[S], Bw);
+            continue;
+        }
+        const long n = (long)g.nG * ext[ax == 0 ? 1 : 0] * ext[ax == 2 ? 1 : 2];
+        for (int side = 0; side < 2; side++)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1(n), 256, 0, h->stream, g, h->clk, 2 * ax + side,
+                               h->bct[2 * ax + side], (int)withB, (const double *)h->grid[G_FACE],
+                               (const double *)h->grid[G_TF], P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+    }
+}
+
+extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
+    if (!h || !type) return -1;
+    for (int ax = 0; ax < 3; ax++) {
+        const int a = type[2 * ax], b = type[2 * ax + 1];
+        if (a < 0 || a > GAM_BC_REFLECT || b < 0 || b > GAM_BC_REFLECT)
+            return fail(h, -1, "gam_set_bcs: unknown boundary type");
+        if ((a == GAM_BC_PERIODIC) != (b == GAM_BC_PERIODIC))
+            return fail(h, -1, "gam_set_bcs: an axis is periodic on both faces or on neither");
+    }
+    if (h->mp.do25D && type[4] != GAM_BC_PERIODIC)
+        return fail(h, -1, "gam_set_bcs: 2.5D blocks keep the k boundaries periodic");
+    if (h->corr && !(type[0] == 0 && type[2] == 0 && type[4] == 0))
+        return fail(h, -1, "gam_set_bcs: a learned correction is set (periodic boundaries only)");
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    for (int f = 0; f < 6; f++) h->bct[f] = type[f];
+    return 0;
+}
+
+extern "C" int gam_get_bcs(gam_handle *h, int *type) {
+    if (!h || !type) return -1;
+    for (int f = 0; f < 6; f++) type[f] = h->bct[f];
+    return 0;
 }
 
 // shell: after a fused step's update only the ghost cells' Bxyz can change
@@ -880,7 +934,7 @@ static int bcs_t(gam_handle *h, bool shell) {
     {
         StageTimer t(h, ST_BPOST);
         const GVT<R> &g = view<R>(h);
-        if (shell && !h->mp.do25D) {
+        if (shell && !h->mp.do25D && all_periodic(h)) {
             const long n = b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f_shell<R>), dim3((unsigned)((n + 255) / 256)), 256, 0, h->stream, g, h->clk,
                                (const double *)h->mag[S], P<R>(h->B[S]));
@@ -1198,7 +1252,7 @@ extern "C" int gam_stage_bytes(gam_handle *h, int stage, double *bytes) {
         b = F * 3 + G * 3; d = F * 9 + G * (3 + 1); break;
     case ST_BPOST: {   // mag, xfc, xyzcc, vol (d) -> B (b); the fused 3D step recomputes the ghost shell only
         const GV &g = h->gv;
-        const double S = h->mp.do25D ? G : (double)b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
+        const double S = (h->mp.do25D || !all_periodic(h)) ? G : (double)b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
         b = S * 3; d = S * (3 + 9 + 3 + 1); break;
     }
     case ST_MOM2FACE: b = G * (3 + 1 + 9); break;                                 // VdV, vol -> Vf (9)

@@ -836,8 +836,10 @@ __device__ __forceinline__ int pmap_f(int i, int a0, int n) {   // owned faces a
     return i;
 }
 
+// mask: the periodic axes (bit 0 i, 1 j, 2 k) mapped by this pass; the
+// others are left for their own BCs.
 template <typename R>
-__global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__ clk, int do25D, R *__restrict__ Gas,
+__global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__ clk, int do25D, int mask, R *__restrict__ Gas,
                                             double *__restrict__ mag, R *__restrict__ B) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nf || clk->halt) return;
@@ -846,10 +848,11 @@ __global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__
     const int a0 = g.nG;
     const size_t nc = g.nc, nf = g.nf;
     const int Nk_eff = g.Nkp;
-    auto kmapc = [&](int kk) { return do25D ? ((kk <= g.Nk - 1) ? a0 : kk) : pmap_c(kk, a0, Nk_eff); };
+    const bool mi = mask & 1, mj = mask & 2, mk = mask & 4;
+    auto kmapc = [&](int kk) { return !mk ? kk : do25D ? ((kk <= g.Nk - 1) ? a0 : kk) : pmap_c(kk, a0, Nk_eff); };
     // cells
     if (i < g.Ni && j < g.Nj && k < g.Nk) {
-        int si = pmap_c(i, a0, g.Nip), sj = pmap_c(j, a0, g.Njp), sk = kmapc(k);
+        int si = mi ? pmap_c(i, a0, g.Nip) : i, sj = mj ? pmap_c(j, a0, g.Njp) : j, sk = kmapc(k);
         if (si != i || sj != j || sk != k) {
             int s = ci(g, si, sj, sk), d = ci(g, i, j, k);
 #pragma unroll
@@ -863,16 +866,97 @@ __global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__
     // faces, per component
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        int si = (d == 0) ? pmap_f(i, a0, g.Nip) : pmap_c(i, a0, g.Nip);
-        int sj = (d == 1) ? pmap_f(j, a0, g.Njp) : pmap_c(j, a0, g.Njp);
+        int si = !mi ? i : (d == 0) ? pmap_f(i, a0, g.Nip) : pmap_c(i, a0, g.Nip);
+        int sj = !mj ? j : (d == 1) ? pmap_f(j, a0, g.Njp) : pmap_c(j, a0, g.Njp);
         int sk;
-        if (do25D) sk = (k <= g.Nk - 1) ? a0 : k;
+        if (!mk) sk = k;
+        else if (do25D) sk = (k <= g.Nk - 1) ? a0 : k;
         else sk = (d == 2) ? pmap_f(k, a0, g.Nkp) : pmap_c(k, a0, g.Nkp);
         // the +1 slot of a non-normal axis is never written by the reference's BCs
         if (d != 0 && i == g.Ni) si = i;
         if (d != 1 && j == g.Nj) sj = j;
         if (!do25D && d != 2 && k == g.Nk) sk = k;
         if (si != i || sj != j || sk != k) mag[nf * d + idx] = mag[nf * d + fi(g, si, sj, sk)];
+    }
+}
+
+// One face's non-periodic BC, restating bcs.F90: zeroGrad_{i,o}bc{I,J,K}
+// (:799-930), zeroExten_{i,o}bc{I,J,K} (:454-603), cartReflect_{i,o}bc{I,J,K}
+// (:607-795).  face: 0 INI, 1 OUTI, 2 INJ, 3 OUTJ, 4 INK, 5 OUTK.  One thread
+// per (n, a, b): ghost layer n = 1..nG and the ghost-inclusive cell ranges of
+// the other two axes (the reference's loops).  Every thread writes its own
+// ghost cell and faces and reads only points no thread of the launch writes.
+// withB: also the Bxyz ghosts (hatch; a step's bFlux2Fld recomputes them).
+// face64/Tf64: the FP64 face areas and triads (zeroGrad_obcI's ghost fluxes).
+enum { GAM_BC_PERIODIC = 0, GAM_BC_ZEROGRAD = 1, GAM_BC_ZEROEXTEN = 2, GAM_BC_REFLECT = 3 };
+template <typename R>
+__global__ void __launch_bounds__(256) k_bc_face(GVT<R> g, const DClock *__restrict__ clk, int face, int type, int withB,
+                                                 const double *__restrict__ face64, const double *__restrict__ Tf64,
+                                                 R *__restrict__ Gas, double *__restrict__ mag, R *__restrict__ B) {
+    const int ax = face >> 1, out = face & 1;
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;
+    const int ext[3] = {g.Ni, g.Nj, g.Nk};
+    const int n1 = ext[t1], n2 = ext[t2];
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.nG * n1 * n2 || clk->halt) return;
+    const int a = (int)(idx % n1);
+    const long r = idx / n1;
+    const int n = 1 + (int)(r % g.nG), b = (int)(r / g.nG);
+    const int act[3] = {g.Nip, g.Njp, g.Nkp};
+    const int lo = g.nG, hi = g.nG + act[ax] - 1;
+    const size_t nc = g.nc, nf = g.nf;
+    int p[3], q[3];
+    p[t1] = q[t1] = a;
+    p[t2] = q[t2] = b;
+    const int gi = out ? hi + n : lo - n;
+    p[ax] = gi;
+    q[ax] = type == GAM_BC_REFLECT ? (out ? hi - n + 1 : lo + n - 1) : (out ? hi : lo);
+    const int cg = ci(g, p[0], p[1], p[2]), cs = ci(g, q[0], q[1], q[2]);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        R x = Gas[nc * v + cs];
+        if (type == GAM_BC_REFLECT && v == 1 + ax) x = R(-1.0) * x;
+        Gas[nc * v + cg] = x;
+    }
+    if (withB && (type == GAM_BC_ZEROEXTEN || (type == GAM_BC_REFLECT && !(ax == 0 && out)))) {
+#pragma unroll
+        for (int v = 0; v < 3; v++) B[nc * v + cg] = B[nc * v + cs];
+    }
+    const int fg = fi(g, p[0], p[1], p[2]), fs = fi(g, q[0], q[1], q[2]);
+    int pn[3] = {p[0], p[1], p[2]}, qn[3] = {q[0], q[1], q[2]};
+    if (!out) {
+        if (type == GAM_BC_REFLECT) qn[ax] = lo + n;
+        for (int d = 0; d < 3; d++) {
+            if (d == ax) mag[nf * d + fi(g, pn[0], pn[1], pn[2])] = mag[nf * d + fi(g, qn[0], qn[1], qn[2])];
+            else mag[nf * d + fg] = mag[nf * d + fs];
+        }
+    } else if (type == GAM_BC_ZEROGRAD && ax == 0) {
+        // zeroGrad_obcI: ghost fluxes of the last active cell's Cartesian field
+        double Bc[3];
+        cell_bxyz(g, mag, hi, a, b, Bc);
+        const int f1 = fi(g, gi + 1, a, b);
+        for (int d = 0; d < 3; d++) {
+            const int f = d == 0 ? f1 : fg;
+            const double dotn = Bc[0] * Tf64[nf * (NORMX + 9 * d) + f] + Bc[1] * Tf64[nf * (NORMY + 9 * d) + f] +
+                                Bc[2] * Tf64[nf * (NORMZ + 9 * d) + f];
+            mag[nf * d + f] = face64[nf * d + f] * dotn;
+        }
+    } else {
+        pn[ax] = gi + 1;
+        qn[ax] = type == GAM_BC_REFLECT ? hi - n + 1 : hi + 1;
+        // zeroGrad_obcK reads the j component at ke+1, which the routine's own
+        // n = 1 pass has just set from ke: the value is magFlux(ke, j) for all n
+        const bool zgk = type == GAM_BC_ZEROGRAD && ax == 2;
+        if (zgk) qn[2] = hi;
+        for (int d = 0; d < 3; d++) {
+            if (d == ax) {
+                mag[nf * d + fi(g, pn[0], pn[1], pn[2])] = mag[nf * (zgk ? 1 : d) + fi(g, qn[0], qn[1], qn[2])];
+            } else if (type == GAM_BC_ZEROEXTEN && ax == 0) {
+                mag[nf * d + fg] = (double)B[nc * d + ci(g, hi, a, b)];   // zeroExten_obcI: Bxyz of the last cell
+            } else {
+                mag[nf * d + fg] = mag[nf * d + fs];
+            }
+        }
     }
 }
 

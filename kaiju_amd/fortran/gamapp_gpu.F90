@@ -38,6 +38,9 @@ contains
             write(error_unit, *) 'gam_create failed, rc = ', rc
             error stop 1
         endif
+        if (any(App%Grid%bcType /= 0)) then
+            call gpuCheck(App, gam_set_bcs(App%gpu, int(App%Grid%bcType, c_int)), "gam_set_bcs")
+        endif
     end subroutine CreateGPU
 
     subroutine DestroyGPU(App)

@@ -121,6 +121,13 @@ module gamera_hip_c
             real(c_double), intent(in) :: xc(3), bu(3)
         end function gam_background_dipole
 
+        !> boundary type per face (INI, OUTI, INJ, OUTJ, INK, OUTK)
+        integer(c_int) function gam_set_bcs(h, types) bind(C, name="gam_set_bcs")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), intent(in) :: types(6)
+        end function gam_set_bcs
+
         integer(c_int) function gam_hatch(h) bind(C, name="gam_hatch")
             import :: c_int, c_ptr
             type(c_ptr), value :: h

@@ -58,6 +58,9 @@ module gamtypes_gpu
         real(rp), dimension(:,:,:,:,:), allocatable :: xfc, Tf, Te, Teb
         real(rp), dimension(:,:,:), allocatable :: di, dj, dk, volume
         real(rp), dimension(:,:,:,:), allocatable :: Face, Edge
+        !> the reference's Grid%externalBCs, as one type per face INI, OUTI, INJ,
+        !> OUTJ, INK, OUTK: 0 periodic, 1 zeroGrad, 2 zeroExten, 3 cartReflect
+        integer :: bcType(6) = 0
     end type Grid_T
 
     !> gamtypes.F90:248-264

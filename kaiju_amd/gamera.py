@@ -150,6 +150,8 @@ def lib():
         "gam_state_device": (C.c_int, [H, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p)]),
         "gam_set_correction": (C.c_int, [H, C.c_void_p, C.c_double]),
+        "gam_set_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
+        "gam_get_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
         "gam_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
         "gam_device_free": (C.c_int, [C.c_void_p]),
         "gam_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -253,6 +255,8 @@ class GamApp:
         if rc != 0 or not h.value:
             raise HipStepError(f"gam_create failed rc={rc}")
         self.h = h
+        if any(b != "periodic" for b in spec.bcs):
+            self.set_bcs(spec.bcs)
 
     # ---------------------------------------------------------------- utils
     def _chk(self, rc: int, what: str) -> int:
@@ -404,6 +408,17 @@ class GamApp:
         while st == 0 and self.Model.t < target:
             st = self.stepGamera()
         return st
+
+    # ---------------------------------------------------------------- boundaries
+    def set_bcs(self, bcs):
+        """Boundary type per face (INI, OUTI, INJ, OUTJ, INK, OUTK), names of configs.BC_TYPES."""
+        t = (C.c_int * 6)(*[configs.BC_TYPES.index(b) for b in bcs])
+        self._chk(lib().gam_set_bcs(self.h, t), "gam_set_bcs")
+
+    def get_bcs(self):
+        t = (C.c_int * 6)()
+        self._chk(lib().gam_get_bcs(self.h, t), "gam_get_bcs")
+        return tuple(configs.BC_TYPES[x] for x in t)
 
     # ---------------------------------------------------------------- learned correction
     def state_device(self, which: int = 0):

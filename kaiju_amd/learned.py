@@ -87,21 +87,27 @@ class TurbIC:
         Ni = N + 2 * nG
         xc = (np.arange(Ni) - nG + 0.5) * dx           # cell centres
         xe = (np.arange(Ni + 1) - nG) * dx             # corners
-        X, Y, Z = np.meshgrid(xc, xc, xc, indexing="ij")
-        v = np.zeros((3,) + X.shape)
+
+        def wave(k, ph, x):
+            """exp(i (k.x + ph)) on the tensor grid x^3, as an outer product of 1-D factors."""
+            ex, ey, ez = (np.exp(1j * k[d] * x) for d in range(3))
+            return np.exp(1j * ph) * ex[:, None, None] * ey[None, :, None] * ez[None, None, :]
+
+        v = np.zeros((3, Ni, Ni, Ni))
         for k, a, ph in vmodes:
-            v += a[:, None, None, None] * np.cos(k[0] * X + k[1] * Y + k[2] * Z + ph)[None]
+            c = wave(k, ph, xc).real
+            for d in range(3):
+                v[d] += a[d] * c
         v *= self.v_rms / math.sqrt(np.mean(np.sum(v[:, nG:-nG, nG:-nG, nG:-nG] ** 2, axis=0)))
         # edge integrals of A: L[d][i,j,k] = integral of A_d along the d-edge from corner (i,j,k)
-        Xe, Ye, Ze = np.meshgrid(xe, xe, xe, indexing="ij")
-        L = np.zeros((3,) + Xe.shape)
+        L = np.zeros((3, Ni + 1, Ni + 1, Ni + 1))
         for k, a, ph in amodes:
-            ph0 = k[0] * Xe + k[1] * Ye + k[2] * Ze + ph
+            w = wave(k, ph, xe)
             for d in range(3):
-                if abs(k[d]) > 0:
-                    L[d] += a[d] * (np.sin(ph0 + k[d] * dx) - np.sin(ph0)) / k[d]
+                if abs(k[d]) > 0:   # integral of cos(k.x + ph) over [x_d, x_d + dx]
+                    L[d] += a[d] * (w * ((np.exp(1j * k[d] * dx) - 1.0) / (1j * k[d]))).real
                 else:
-                    L[d] += a[d] * np.cos(ph0) * dx
+                    L[d] += a[d] * w.real * dx
         F = L.shape[1]
         mag = np.zeros((F, F, F, 3), order="F")
         s = lambda a, di, dj, dk: np.roll(a, (-di, -dj, -dk), axis=(0, 1, 2))   # a[i+di, j+dj, k+dk]
@@ -211,7 +217,7 @@ def set_time(app: GamApp, t: float, dt: float, ts: int):
     app.set_clock(c)
 
 
-def fine_trajectory(ic: TurbIC, N: int, steps: int, dt: Optional[float] = None, nG: int = 4) -> Trajectory:
+def fine_trajectory(ic: TurbIC, N: int, steps: int, dt: Optional[float] = None, nG: int = 4, log=None) -> Trajectory:
     """Run the solver at (2N)^3 with dt/2 for `steps` coarse intervals and
     downsample every second state."""
     from kaiju_amd.gamera import gen_corners
@@ -232,8 +238,11 @@ def fine_trajectory(ic: TurbIC, N: int, steps: int, dt: Optional[float] = None, 
                     raise RuntimeError(f"fine run halted with status {st} at coarse step {n}")
             s = app.get_state(0)
             g, m = downsample(s["Gas"], s["magFlux"], nG)
+            del s
             out_g.append(g)
             out_m.append(m)
+            if log and n % 6 == 0:
+                log(f"[learned] seed {ic.seed}: fine sample {n}/{steps}")
     return Trajectory(N=N, nG=nG, dt=dt, gas=out_g, mag=out_m, fine_s=time.perf_counter() - t0)
 
 
@@ -493,14 +502,17 @@ def experiment(N: int = 32, train_seeds=(0, 1, 2, 3), test_seed: int = 100, step
     ic_test = TurbIC(seed=test_seed, **ic)
     dt = min(choose_dt(x, N) for x in ics + [ic_test])
     t0 = time.perf_counter()
-    trs = [fine_trajectory(x, N, steps, dt) for x in ics]
-    test = fine_trajectory(ic_test, N, steps, dt)
+    trs = [fine_trajectory(x, N, steps, dt, log=log) for x in ics]
+    test = fine_trajectory(ic_test, N, steps, dt, log=log)
     t_fine = time.perf_counter() - t0
     log(f"[learned] fine trajectories {t_fine:.1f} s")
     app = coarse_app(N, precision)
     pool, test_samples, corr = [], [], None
     try:
-        tgts = [trajectory_targets(app, tr) for tr in trs]
+        tgts = []
+        for tr in trs:
+            tgts.append(trajectory_targets(app, tr))
+            log(f"[learned] targets of {len(tgts)}/{len(trs)} training trajectories")
         norm = normalisation(trs, tgts)
         for tr, tl in zip(trs, tgts):
             pool += trajectory_samples(tr, tl)

@@ -29,7 +29,8 @@ class OrcModel(C.Structure):
                  "t", "tFin", "dt", "dt0", "limDT0", "limCPR")] + \
                [(n, C.c_int) for n in
                 ("ts", "rmeth", "doLFMLim", "doBoris", "doMHD", "do25D", "doCPR",
-                 "fixedTimestep", "status", "doBackground")]
+                 "fixedTimestep", "status", "doBackground")] + \
+               [("bc", C.c_int * 6)]
 
 
 class OrcGrid(C.Structure):
@@ -185,6 +186,9 @@ class OracleCase:
             setattr(g, k, ptr(a))
         self.m = model_struct(spec.model, spec.do25D)
         self.m.doBackground = int(spec.background is not None)
+        from kaiju_amd.configs import BC_TYPES
+        for f, b in enumerate(getattr(spec, "bcs", ("periodic",) * 6)):
+            self.m.bc[f] = BC_TYPES.index(b)
         self.state = OracleState(G, F)
         self.ostate = OracleState(G, F)
         self.solver = L.orc_solver_new(C.byref(g))

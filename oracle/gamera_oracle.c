@@ -814,10 +814,89 @@ double orc_calc_dt(orc_model *m, const orc_grid *g, const orc_state *s) {
     return dt;
 }
 
-/* ----------------------------------------------------- periodic BCs */
+/* ----------------------------------------------------- non-periodic BCs */
+/* One face's BC, restating bcs.F90: zeroGrad_{i,o}bc{I,J,K} (:799-930),
+ * zeroExten_{i,o}bc{I,J,K} (:454-603), cartReflect_{i,o}bc{I,J,K} (:607-795).
+ * Loops over n = 1..Ng and the ghost-inclusive cell ranges of the other two
+ * axes, as the reference's do loops; the same assignments, including the
+ * reference's choices of source component and of which routines write Bxyz. */
+static void bc_face(const orc_grid *g, orc_state *s, int face, int type) {
+    const size_t nc = g->nc, nf = g->nf;
+    const int nG = g->nG, ax = face / 2, out = face & 1;
+    double *Gas = s->Gas, *B = s->Bxyz, *mag = s->magFlux;
+    const int lo[3] = {g->is, g->js, g->ks}, hi[3] = {g->ie, g->je, g->ke};
+    const int t1 = ax == 0 ? 1 : 0, t2 = ax == 2 ? 1 : 2;      /* the other two axes, in order */
+    const int gl[3] = {g->isg, g->jsg, g->ksg}, gh[3] = {g->ieg, g->jeg, g->keg};
+    for (int b = gl[t2]; b <= gh[t2]; b++)
+        for (int a = gl[t1]; a <= gh[t1]; a++)
+            for (int n = 1; n <= nG; n++) {
+                int p[3], q[3];          /* ghost and source positions along ax */
+                p[t1] = q[t1] = a;
+                p[t2] = q[t2] = b;
+                const int gi = out ? hi[ax] + n : lo[ax] - n;
+                int sc;                  /* source cell along ax */
+                if (type == ORC_BC_REFLECT) sc = out ? hi[ax] - n + 1 : lo[ax] + n - 1;
+                else sc = out ? hi[ax] : lo[ax];
+                p[ax] = gi;
+                q[ax] = sc;
+                const size_t cg = CI(g, p[0], p[1], p[2]), cs = CI(g, q[0], q[1], q[2]);
+                for (int v = 0; v < 5; v++) {
+                    double x = Gas[cs + nc * v];
+                    if (type == ORC_BC_REFLECT && v == 1 + ax) x = -1.0 * x;
+                    Gas[cg + nc * v] = x;
+                }
+                /* Bxyz: zero extension and reflection copy it (not cartReflect_obcI) */
+                if (type == ORC_BC_ZEROEXTEN || (type == ORC_BC_REFLECT && !(ax == 0 && out)))
+                    for (int v = 0; v < 3; v++) B[cg + nc * v] = B[cs + nc * v];
+                const size_t fg = FI(g, p[0], p[1], p[2]), fs = FI(g, q[0], q[1], q[2]);
+                int pn[3] = {p[0], p[1], p[2]};   /* ghost face of the normal component */
+                int qn[3] = {q[0], q[1], q[2]};   /* its source */
+                if (!out) {
+                    /* IN: tangential components at the cell's own face point; the
+                     * normal one too, from lo (zeroGrad/Exten) or lo + n (reflect) */
+                    if (type == ORC_BC_REFLECT) qn[ax] = lo[ax] + n;
+                    for (int d = 0; d < 3; d++) {
+                        if (d == ax) mag[FI(g, pn[0], pn[1], pn[2]) + nf * d] = mag[FI(g, qn[0], qn[1], qn[2]) + nf * d];
+                        else mag[fg + nf * d] = mag[fs + nf * d];
+                    }
+                } else if (type == ORC_BC_ZEROGRAD && ax == 0) {
+                    /* zeroGrad_obcI: face fluxes of the last cell's Cartesian field */
+                    double Bc[3];
+                    cell_bxyz(g, mag, hi[0], a, b, Bc);
+                    const int ig = hi[0] + n;
+                    const size_t fi1 = FI(g, ig + 1, a, b), fi0 = FI(g, ig, a, b);
+                    for (int d = 0; d < 3; d++) {
+                        const size_t f = d == 0 ? fi1 : fi0;
+                        const double *T = g->Tf;
+                        double dotn = Bc[0] * T[f + nf * (NORMX + 9 * d)] + Bc[1] * T[f + nf * (NORMY + 9 * d)] +
+                                      Bc[2] * T[f + nf * (NORMZ + 9 * d)];
+                        mag[f + nf * d] = g->face[f + nf * d] * dotn;
+                    }
+                } else {
+                    /* OUT: the normal component one point further out */
+                    pn[ax] = gi + 1;
+                    qn[ax] = type == ORC_BC_REFLECT ? hi[ax] - n + 1 : hi[ax] + 1;
+                    for (int d = 0; d < 3; d++) {
+                        if (d == ax) {
+                            /* zeroGrad_obcK takes the k face from the j component */
+                            const int dsrc = (type == ORC_BC_ZEROGRAD && ax == 2) ? JD : d;
+                            mag[FI(g, pn[0], pn[1], pn[2]) + nf * d] = mag[FI(g, qn[0], qn[1], qn[2]) + nf * dsrc];
+                        } else if (type == ORC_BC_ZEROEXTEN && ax == 0) {
+                            /* zeroExten_obcI sets the tangential faces to the last cell's Bxyz */
+                            mag[fg + nf * d] = B[CI(g, hi[0], a, b) + nc * d];
+                        } else {
+                            mag[fg + nf * d] = mag[fs + nf * d];
+                        }
+                    }
+                }
+            }
+}
+
+/* ----------------------------------------------------- BCs */
 void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
-    /* bcs.F90:279-450 periodic_{i,o}bc{I,J,K}, applied in the order of
-     * step.F90:27-53 (INI, OUTI, INJ, OUTJ, INK, OUTK) */
+    /* bcs.F90:279-450 periodic_{i,o}bc{I,J,K}, or the face's non-periodic BC
+     * (bc_face), applied in the order of step.F90:27-53 (INI, OUTI, INJ, OUTJ,
+     * INK, OUTK) */
     size_t nc = g->nc, nf = g->nf;
     int nG = g->nG;
     double *Gas = s->Gas, *B = s->Bxyz, *mag = s->magFlux;
@@ -827,14 +906,16 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
         for (int v = 0; v < 3; v++) B[a_ + nc * v] = B[b_ + nc * v]; } while (0)
 #define CPF(di, dj, dk, si, sj, sk, d) do { mag[FI(g, di, dj, dk) + nf * (d)] = mag[FI(g, si, sj, sk) + nf * (d)]; } while (0)
     /* I */
-    for (int k = g->ksg; k <= g->keg + 1; k++)
+    if (m->bc[0] != ORC_BC_PERIODIC) bc_face(g, s, 0, m->bc[0]);
+    else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int n = 1; n <= nG; n++) {
                 int ig = g->is - n, is_ = g->ie - n + 1;
                 if (ISCC(ig, j, k)) CPC(ig, j, k, is_, j, k);
                 for (int d = 0; d < 3; d++) CPF(ig, j, k, is_, j, k, d);
             }
-    for (int k = g->ksg; k <= g->keg + 1; k++)
+    if (m->bc[1] != ORC_BC_PERIODIC) bc_face(g, s, 1, m->bc[1]);
+    else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int n = 1; n <= nG; n++) {
                 int ig = g->ie + n;
@@ -844,14 +925,16 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
                 CPF(ig + 1, j, k, g->is + n, j, k, ID);
             }
     /* J */
-    for (int k = g->ksg; k <= g->keg + 1; k++)
+    if (m->bc[2] != ORC_BC_PERIODIC) bc_face(g, s, 2, m->bc[2]);
+    else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int i = g->isg; i <= g->ieg + 1; i++)
             for (int n = 1; n <= nG; n++) {
                 int jg = g->js - n, js_ = g->je - n + 1;
                 if (ISCC(i, jg, k)) CPC(i, jg, k, i, js_, k);
                 for (int d = 0; d < 3; d++) CPF(i, jg, k, i, js_, k, d);
             }
-    for (int k = g->ksg; k <= g->keg + 1; k++)
+    if (m->bc[3] != ORC_BC_PERIODIC) bc_face(g, s, 3, m->bc[3]);
+    else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int i = g->isg; i <= g->ieg + 1; i++)
             for (int n = 1; n <= nG; n++) {
                 int jg = g->je + n;
@@ -873,14 +956,16 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
             }
         }
     } else {
-        for (int j = g->jsg; j <= g->jeg + 1; j++)
+        if (m->bc[4] != ORC_BC_PERIODIC) bc_face(g, s, 4, m->bc[4]);
+        else for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int i = g->isg; i <= g->ieg + 1; i++)
                 for (int n = 1; n <= nG; n++) {
                     int kg = g->ks - n, ks_ = g->ke - n + 1;
                     if (ISCC(i, j, kg)) CPC(i, j, kg, i, j, ks_);
                     for (int d = 0; d < 3; d++) CPF(i, j, kg, i, j, ks_, d);
                 }
-        for (int j = g->jsg; j <= g->jeg + 1; j++)
+        if (m->bc[5] != ORC_BC_PERIODIC) bc_face(g, s, 5, m->bc[5]);
+        else for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int i = g->isg; i <= g->ieg + 1; i++)
                 for (int n = 1; n <= nG; n++) {
                     int kg = g->ke + n;

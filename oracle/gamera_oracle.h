@@ -28,7 +28,12 @@ typedef struct {
     int ts, rmeth, doLFMLim, doBoris, doMHD, do25D, doCPR, fixedTimestep;
     int status;   /* CalcDT outcome: 0 ok, 1 bad (dt drop), 2 disaster */
     int doBackground;   /* background field B0 split (gamtypes.F90:88) */
+    int bc[6];          /* boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK: ORC_BC_* */
 } orc_model;
+
+/* boundary conditions (bcs.F90): periodic, zero gradient, zero extension,
+ * Cartesian reflecting */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3 };
 
 /* Grid_T subset (reference gamtypes.F90:163-245) */
 typedef struct {

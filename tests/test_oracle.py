@@ -370,6 +370,115 @@ def test_periodic_bcs_index_encoding():
                     assert s.magFlux[i, j, k, d] == code(si, sj, sk, d), (i, j, k, d)
 
 
+@pytest.mark.parametrize("ax", [0, 1, 2])
+@pytest.mark.parametrize("typ", ["zerograd", "zeroexten", "reflect"])
+def test_nonperiodic_bcs_index_encoding(ax, typ):
+    """Every slot coded by its (i,j,k,v); one axis gets the BC under test on both
+    faces, the others stay periodic.  The ghosts of that axis (at active
+    transverse positions) must hold what bcs.F90's routine for it assigns:
+    the source cell / face, the negated normal momentum of the reflecting wall,
+    which routines leave Bxyz alone, zeroExten_obcI's tangential faces from
+    Bxyz, zeroGrad_obcK's k faces from the j component (read after the routine's
+    own first pass rewrote it), and zeroGrad_obcI's
+    ghost fluxes from the last cell's field, which on a Cartesian grid is the
+    mean of that cell's two faces (closed form)."""
+    spec = configs.orszag_tang(16, 8)
+    spec.N = (10, 9, 8)
+    bcs = ["periodic"] * 6
+    bcs[2 * ax] = bcs[2 * ax + 1] = typ
+    spec.bcs = tuple(bcs)
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.gen_corners()
+    oc.metric()
+    g = oc.g
+    Ni, Nj, Nk = oc.G
+    I, J, K = np.meshgrid(np.arange(Ni + 1), np.arange(Nj + 1), np.arange(Nk + 1), indexing="ij")
+    code = lambda i, j, k, v: v * 1e6 + k * 1e4 + j * 1e2 + i
+    s = oc.state
+    for v in range(5):
+        s.Gas[..., v] = code(I[:-1, :-1, :-1], J[:-1, :-1, :-1], K[:-1, :-1, :-1], v)
+    for v in range(3):
+        s.Bxyz[..., v] = code(I[:-1, :-1, :-1], J[:-1, :-1, :-1], K[:-1, :-1, :-1], 10 + v)
+    for d in range(3):
+        s.magFlux[..., d] = code(I, J, K, 20 + d)
+    gas0, b0, mag0 = s.Gas.copy(), s.Bxyz.copy(), s.magFlux.copy()
+    L.orc_enforce_bcs(C.byref(oc.m), C.byref(g), C.byref(s.s))
+    nG = spec.nG
+    lo, hi = nG, nG + spec.N[ax] - 1
+    act = [range(nG, nG + n) for n in spec.N]
+    t1, t2 = (1, 2) if ax == 0 else ((0, 2) if ax == 1 else (0, 1))
+
+    def at(a, b, x):   # position with coordinate x along ax, a along t1, b along t2
+        p = [0, 0, 0]
+        p[ax], p[t1], p[t2] = x, a, b
+        return tuple(p)
+
+    for a in act[t1]:
+        for b in act[t2]:
+            for n in range(1, nG + 1):
+                for out in (0, 1):
+                    gi = hi + n if out else lo - n
+                    if typ == "reflect":
+                        sc = hi - n + 1 if out else lo + n - 1
+                    else:
+                        sc = hi if out else lo
+                    for v in range(5):
+                        want = gas0[at(a, b, sc) + (v,)] * (-1.0 if (typ == "reflect" and v == 1 + ax) else 1.0)
+                        assert s.Gas[at(a, b, gi) + (v,)] == want
+                    writes_b = typ == "zeroexten" or (typ == "reflect" and not (ax == 0 and out))
+                    for v in range(3):
+                        src = b0[at(a, b, sc) + (v,)] if writes_b else b0[at(a, b, gi) + (v,)]
+                        assert s.Bxyz[at(a, b, gi) + (v,)] == src
+                    for d in range(3):
+                        if d == ax:
+                            if out:
+                                gpos, spos = at(a, b, gi + 1), at(a, b, hi - n + 1 if typ == "reflect" else hi + 1)
+                            else:
+                                gpos, spos = at(a, b, gi), at(a, b, lo + n if typ == "reflect" else lo)
+                        else:
+                            gpos, spos = at(a, b, gi), at(a, b, sc)
+                        got = s.magFlux[gpos + (d,)]
+                        if out and typ == "zerograd" and ax == 0:
+                            # face * (n . B(ie)) with B(ie) the mean of ie's two faces over their area
+                            e = [0, 0, 0]
+                            e[d] = 1
+                            c0 = at(a, b, hi)
+                            c1 = (c0[0] + e[0], c0[1] + e[1], c0[2] + e[2])
+                            want = 0.5 * (mag0[c0 + (d,)] + mag0[c1 + (d,)])
+                            assert abs(got - want) <= 1e-9 * abs(want), (d, got, want)
+                        elif out and typ == "zerograd" and ax == 2 and d == 2:
+                            # zeroGrad_obcK reads the j component at ke+1, which its own n = 1
+                            # pass has just set from ke: every k ghost face gets magFlux(ke, j)
+                            assert got == mag0[at(a, b, hi) + (1,)]
+                        elif out and typ == "zeroexten" and ax == 0 and d != 0:
+                            assert got == b0[at(a, b, hi) + (d,)]
+                        else:
+                            assert got == mag0[spos + (d,)], (a, b, n, out, d)
+
+
+def test_reflecting_walls_stop_the_mass_flux():
+    """With reflecting i and j walls the ghost cells mirror the active ones, so
+    the kinetic fluxes through the walls cancel exactly and the total mass of
+    an Orszag-Tang state (non-uniform at the walls) is conserved to rounding."""
+    spec = configs.orszag_tang(24, 4, steps=10)
+    spec.bcs = ("reflect", "reflect", "reflect", "reflect", "periodic", "periodic")
+    oc = oracle.OracleCase(spec)
+    nG = spec.nG
+    A = tuple(slice(nG, nG + n) for n in spec.N)
+    vol = oc.grid["volume"][A]
+    m0 = (oc.state.Gas[A + (0,)] * vol).sum()
+    assert oc.step(spec.steps) == 0
+    m1 = (oc.state.Gas[A + (0,)] * vol).sum()
+    assert abs(m1 - m0) <= 1e-13 * m0, (m0, m1)
+    # the same state with outflow (zero-gradient) walls does lose mass
+    spec2 = configs.orszag_tang(24, 4, steps=10)
+    spec2.bcs = ("zerograd", "zerograd", "zerograd", "zerograd", "periodic", "periodic")
+    oc2 = oracle.OracleCase(spec2)
+    assert oc2.step(spec2.steps) == 0
+    m2 = (oc2.state.Gas[A + (0,)] * vol).sum()
+    assert abs(m2 - m0) > 1e-8 * m0
+
+
 # ----------------------------------------------------------- invariants of the step
 @pytest.mark.parametrize("name", ["ot", "bw"])
 def test_divb_and_conservation(name):
