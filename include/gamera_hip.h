@@ -144,7 +144,8 @@ int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt,
 /* Boundary type per face, in the order INI, OUTI, INJ, OUTJ, INK, OUTK (the
  * order EnforceBCs applies them, step.F90:17-55): 0 periodic (bcs.F90:279-450),
  * 1 zero gradient (zeroGrad_*, :799-930), 2 zero extension (zeroExten_*,
- * :454-603), 3 Cartesian reflecting (cartReflect_*, :607-795).  An axis is
+ * :454-603), 3 Cartesian reflecting (cartReflect_*, :607-795), 4 spherical
+ * radial (SphIn on INI, SphOut on OUTI, :934-982; i faces only).  An axis is
  * periodic on both faces or on neither; 2.5D blocks keep k periodic.  Default:
  * all periodic.  The reference's BC objects (Grid%externalBCs) map onto these. */
 int  gam_set_bcs(gam_handle *h, const int *type);

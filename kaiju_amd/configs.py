@@ -18,7 +18,7 @@ CFL_DEFAULT = 0.30000001192092896
 
 
 # the library's boundary types, by code (gam_set_bcs)
-BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect")
+BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere")
 
 
 @dataclasses.dataclass

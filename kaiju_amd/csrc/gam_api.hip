@@ -904,8 +904,10 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     if (!h || !type) return -1;
     for (int ax = 0; ax < 3; ax++) {
         const int a = type[2 * ax], b = type[2 * ax + 1];
-        if (a < 0 || a > GAM_BC_REFLECT || b < 0 || b > GAM_BC_REFLECT)
+        if (a < 0 || a > GAM_BC_SPHERE || b < 0 || b > GAM_BC_SPHERE)
             return fail(h, -1, "gam_set_bcs: unknown boundary type");
+        if (ax > 0 && (a == GAM_BC_SPHERE || b == GAM_BC_SPHERE))
+            return fail(h, -1, "gam_set_bcs: the spherical BCs are i-face BCs");
         if ((a == GAM_BC_PERIODIC) != (b == GAM_BC_PERIODIC))
             return fail(h, -1, "gam_set_bcs: an axis is periodic on both faces or on neither");
     }

@@ -888,7 +888,9 @@ __global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__
 // ghost cell and faces and reads only points no thread of the launch writes.
 // withB: also the Bxyz ghosts (hatch; a step's bFlux2Fld recomputes them).
 // face64/Tf64: the FP64 face areas and triads (zeroGrad_obcI's ghost fluxes).
-enum { GAM_BC_PERIODIC = 0, GAM_BC_ZEROGRAD = 1, GAM_BC_ZEROEXTEN = 2, GAM_BC_REFLECT = 3 };
+// GAM_BC_SPHERE: SphIn (INI, the same assignments as zeroExten_ibcI) and SphOut
+// (OUTI: every ghost face and cell from the last cell ie), bcs.F90:934-982.
+enum { GAM_BC_PERIODIC = 0, GAM_BC_ZEROGRAD = 1, GAM_BC_ZEROEXTEN = 2, GAM_BC_REFLECT = 3, GAM_BC_SPHERE = 4 };
 template <typename R>
 __global__ void __launch_bounds__(256) k_bc_face(GVT<R> g, const DClock *__restrict__ clk, int face, int type, int withB,
                                                  const double *__restrict__ face64, const double *__restrict__ Tf64,
@@ -918,7 +920,7 @@ __global__ void __launch_bounds__(256) k_bc_face(GVT<R> g, const DClock *__restr
         if (type == GAM_BC_REFLECT && v == 1 + ax) x = R(-1.0) * x;
         Gas[nc * v + cg] = x;
     }
-    if (withB && (type == GAM_BC_ZEROEXTEN || (type == GAM_BC_REFLECT && !(ax == 0 && out)))) {
+    if (withB && (type == GAM_BC_ZEROEXTEN || type == GAM_BC_SPHERE || (type == GAM_BC_REFLECT && !(ax == 0 && out)))) {
 #pragma unroll
         for (int v = 0; v < 3; v++) B[nc * v + cg] = B[nc * v + cs];
     }
@@ -943,7 +945,7 @@ __global__ void __launch_bounds__(256) k_bc_face(GVT<R> g, const DClock *__restr
         }
     } else {
         pn[ax] = gi + 1;
-        qn[ax] = type == GAM_BC_REFLECT ? hi - n + 1 : hi + 1;
+        qn[ax] = type == GAM_BC_REFLECT ? hi - n + 1 : (type == GAM_BC_SPHERE ? hi : hi + 1);
         // zeroGrad_obcK reads the j component at ke+1, which the routine's own
         // n = 1 pass has just set from ke: the value is magFlux(ke, j) for all n
         const bool zgk = type == GAM_BC_ZEROGRAD && ax == 2;

@@ -59,7 +59,7 @@ module gamtypes_gpu
         real(rp), dimension(:,:,:), allocatable :: di, dj, dk, volume
         real(rp), dimension(:,:,:,:), allocatable :: Face, Edge
         !> the reference's Grid%externalBCs, as one type per face INI, OUTI, INJ,
-        !> OUTJ, INK, OUTK: 0 periodic, 1 zeroGrad, 2 zeroExten, 3 cartReflect
+        !> OUTJ, INK, OUTK: 0 periodic, 1 zeroGrad, 2 zeroExten, 3 cartReflect, 4 SphIn/SphOut
         integer :: bcType(6) = 0
     end type Grid_T
 

@@ -816,7 +816,9 @@ double orc_calc_dt(orc_model *m, const orc_grid *g, const orc_state *s) {
 
 /* ----------------------------------------------------- non-periodic BCs */
 /* One face's BC, restating bcs.F90: zeroGrad_{i,o}bc{I,J,K} (:799-930),
- * zeroExten_{i,o}bc{I,J,K} (:454-603), cartReflect_{i,o}bc{I,J,K} (:607-795).
+ * zeroExten_{i,o}bc{I,J,K} (:454-603), cartReflect_{i,o}bc{I,J,K} (:607-795),
+ * SphIn/SphOut (:934-982; i faces only).  SphIn assigns exactly what
+ * zeroExten_ibcI does; SphOut takes every ghost face from the last cell ie.
  * Loops over n = 1..Ng and the ghost-inclusive cell ranges of the other two
  * axes, as the reference's do loops; the same assignments, including the
  * reference's choices of source component and of which routines write Bxyz. */
@@ -845,8 +847,8 @@ static void bc_face(const orc_grid *g, orc_state *s, int face, int type) {
                     if (type == ORC_BC_REFLECT && v == 1 + ax) x = -1.0 * x;
                     Gas[cg + nc * v] = x;
                 }
-                /* Bxyz: zero extension and reflection copy it (not cartReflect_obcI) */
-                if (type == ORC_BC_ZEROEXTEN || (type == ORC_BC_REFLECT && !(ax == 0 && out)))
+                /* Bxyz: zero extension, sphere and reflection copy it (not cartReflect_obcI) */
+                if (type == ORC_BC_ZEROEXTEN || type == ORC_BC_SPHERE || (type == ORC_BC_REFLECT && !(ax == 0 && out)))
                     for (int v = 0; v < 3; v++) B[cg + nc * v] = B[cs + nc * v];
                 const size_t fg = FI(g, p[0], p[1], p[2]), fs = FI(g, q[0], q[1], q[2]);
                 int pn[3] = {p[0], p[1], p[2]};   /* ghost face of the normal component */
@@ -875,7 +877,7 @@ static void bc_face(const orc_grid *g, orc_state *s, int face, int type) {
                 } else {
                     /* OUT: the normal component one point further out */
                     pn[ax] = gi + 1;
-                    qn[ax] = type == ORC_BC_REFLECT ? hi[ax] - n + 1 : hi[ax] + 1;
+                    qn[ax] = type == ORC_BC_REFLECT ? hi[ax] - n + 1 : (type == ORC_BC_SPHERE ? hi[ax] : hi[ax] + 1);
                     for (int d = 0; d < 3; d++) {
                         if (d == ax) {
                             /* zeroGrad_obcK takes the k face from the j component */

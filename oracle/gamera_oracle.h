@@ -32,8 +32,8 @@ typedef struct {
 } orc_model;
 
 /* boundary conditions (bcs.F90): periodic, zero gradient, zero extension,
- * Cartesian reflecting */
-enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3 };
+ * Cartesian reflecting, spherical radial (SphIn on INI, SphOut on OUTI) */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3, ORC_BC_SPHERE = 4 };
 
 /* Grid_T subset (reference gamtypes.F90:163-245) */
 typedef struct {

@@ -370,8 +370,8 @@ def test_periodic_bcs_index_encoding():
                     assert s.magFlux[i, j, k, d] == code(si, sj, sk, d), (i, j, k, d)
 
 
-@pytest.mark.parametrize("ax", [0, 1, 2])
-@pytest.mark.parametrize("typ", ["zerograd", "zeroexten", "reflect"])
+@pytest.mark.parametrize("ax,typ", [(a, t) for a in (0, 1, 2) for t in ("zerograd", "zeroexten", "reflect")] +
+                         [(0, "sphere")])
 def test_nonperiodic_bcs_index_encoding(ax, typ):
     """Every slot coded by its (i,j,k,v); one axis gets the BC under test on both
     faces, the others stay periodic.  The ghosts of that axis (at active
@@ -425,14 +425,15 @@ def test_nonperiodic_bcs_index_encoding(ax, typ):
                     for v in range(5):
                         want = gas0[at(a, b, sc) + (v,)] * (-1.0 if (typ == "reflect" and v == 1 + ax) else 1.0)
                         assert s.Gas[at(a, b, gi) + (v,)] == want
-                    writes_b = typ == "zeroexten" or (typ == "reflect" and not (ax == 0 and out))
+                    writes_b = typ in ("zeroexten", "sphere") or (typ == "reflect" and not (ax == 0 and out))
                     for v in range(3):
                         src = b0[at(a, b, sc) + (v,)] if writes_b else b0[at(a, b, gi) + (v,)]
                         assert s.Bxyz[at(a, b, gi) + (v,)] == src
                     for d in range(3):
                         if d == ax:
                             if out:
-                                gpos, spos = at(a, b, gi + 1), at(a, b, hi - n + 1 if typ == "reflect" else hi + 1)
+                                src = hi - n + 1 if typ == "reflect" else (hi if typ == "sphere" else hi + 1)
+                                gpos, spos = at(a, b, gi + 1), at(a, b, src)
                             else:
                                 gpos, spos = at(a, b, gi), at(a, b, lo + n if typ == "reflect" else lo)
                         else:
