@@ -105,6 +105,36 @@ def fp32_leg(n: int, steps: int, warmup: int, s64: dict, ms64: float):
                 stage_avg_ms={k: round(v["ms"] / v["launches"], 4) for k, v in tim.items() if v["launches"] > 0})
 
 
+def msph_leg():
+    """The magnetosphere-like configuration (192x192x256 spherical-polar wedge,
+    dipole background, FP32 bulk, Boris): its 500 steps timed from the hatched
+    state (setup excluded)."""
+    from kaiju_amd import configs
+    from kaiju_amd.gamera import GamApp
+    spec = configs.magnetosphere()
+    t0 = time.perf_counter()
+    app = GamApp(spec)
+    app.InitModel()
+    app.sync()
+    setup = time.perf_counter() - t0
+    divb0 = app.divb()
+    t0 = time.perf_counter()
+    st = app.steps(spec.steps)
+    app.sync()
+    wall = time.perf_counter() - t0
+    divb = app.divb()
+    t_end, dt = app.Model.t, app.Model.dt
+    app.close()
+    log(f"[bench] msph {spec.N}: setup {setup:.1f} s, {spec.steps} steps in {wall:.3f} s, status {st}")
+    return dict(workload="magnetosphere-like block: stretched spherical-polar wedge 192x192x256, r 2-30 R_E, "
+                         "Earth dipole background, solar wind, SphIn/SphOut + zero-gradient walls, Boris 1000 km/s, "
+                         "FP32 bulk (FP64 CT/energy/dt)",
+                value=round(spec.steps * spec.zones / wall, 1), unit="zone-updates/s",
+                ms_per_step=round(wall / spec.steps * 1e3, 4), steps=spec.steps, status=st,
+                t_end_code_units=t_end, dt_last=dt, divB_total_before=divb0, divB_total_after=divb,
+                setup_s=round(setup, 2))
+
+
 def roofline(tim: dict, ms_per_step: float, zups: float):
     # dominant kernel = largest share of the timed region
     stages = {k: v for k, v in tim.items() if v["launches"] > 0}
@@ -183,6 +213,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle stepping")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
+    ap.add_argument("--no-msph", action="store_true", help="skip the magnetosphere-like leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -202,6 +233,13 @@ def main():
         del s64
     else:
         app.close()
+    msph = None
+    if not args.no_msph:
+        try:
+            msph = msph_leg()
+        except Exception as e:   # reported beside the headline, never in place of it
+            log(f"[bench] msph leg failed: {e!r}")
+            msph = {"value": None, "error": repr(e)}
     cpu = None
     if not args.no_cpu:
         try:
@@ -227,6 +265,7 @@ def main():
         "roofline": rf,
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
+        "magnetosphere": msph,
     }
     print(json.dumps(out), flush=True)
 

@@ -108,6 +108,25 @@ def blast_wave(n: int = 256, steps: int = 100, warp_amp: float = 0.05, B0: float
                     model=m, steps=steps)
 
 
+def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in: float = 2.0, r_out: float = 30.0,
+                  theta0: float = 0.15, Ca_kms: float = 1000.0, **mkw) -> CaseSpec:
+    """Magnetosphere-like block (kaiju_amd/msph.py): stretched spherical-polar
+    wedge r in [2, 30] R_E around the x axis, Earth dipole -31000 nT R_E^3 as the
+    background field, solar wind rho = 5 cm^-3, v = (-400, 0, 0) km/s (at rest
+    inside ~12 R_E at the start), T = 1e5 K, IMF Bz = -5 nT, Boris speed cap
+    1000 km/s, FP32 bulk."""
+    from kaiju_amd import msph
+    rho, v, p, b = msph.wind_state()
+    m = ModelSpec(gamma=5.0 / 3.0, CFL=0.3, doBoris=True, Ca=Ca_kms / 100.0, precision=precision, **mkw)
+    nr, nt, nphi = N
+    return CaseSpec(name=f"msph{nr}x{nt}x{nphi}", N=(nr, nt, nphi), bounds=(r_in, r_out, theta0, math.pi - theta0,
+                                                                         0.0, 2 * math.pi),
+                    grid="sph", ic="msph", ic_params=dict(r_in=r_in, r_out=r_out, theta0=theta0, rho=rho, v=v, p=p, b=b),
+                    model=m, steps=steps, background=dict(M0=msph.dipole_moment(), xc=(0.0, 0.0, 0.0),
+                                                          bu=(0.0, 0.0, 0.0)),
+                    bcs=("sphere", "sphere", "zerograd", "zerograd", "periodic", "periodic"))
+
+
 # The two workloads named by the benchmark configuration list.
 def config_c1() -> CaseSpec:
     return orszag_tang(256, 4, 200)

@@ -218,6 +218,10 @@ def gen_corners(spec: configs.CaseSpec) -> np.ndarray:
         xyz[..., 0] += s * Lx
         xyz[..., 1] += s * Ly
         xyz[..., 2] += s * Lz
+    elif spec.grid == "sph":
+        from kaiju_amd import msph
+        p = spec.ic_params
+        return msph.corners(spec.N, nG, p["r_in"], p["r_out"], p["theta0"])
     elif spec.grid != "cart":
         raise ValueError(f"grid {spec.grid} is generated by the caller; use set_corners()")
     return xyz
@@ -311,6 +315,10 @@ class GamApp:
         elif sp.ic == "bw":
             self._chk(L.gam_ic_blast_wave(self.h, p["x0"], p["y0"], p["z0"], p["D0"], p["P0"], p["pRat"],
                                           p["dRat"], p["rC"], p["B0"]), "ic_bw")
+        elif sp.ic == "msph":   # uniform wind on the spherical grid (host-built, exact face fluxes)
+            from kaiju_amd import msph
+            gas, mag = msph.uniform_state(gen_corners(sp), p["rho"], p["v"], p["p"], p["b"], sp.model.gamma)
+            self.put_state(0, 0.0, gas, mag, None)
         else:
             raise ValueError(sp.ic)
 

@@ -136,3 +136,62 @@ def test_bench_sample_matches_oracle():
         oc.step(5)
         e = compare_state(app, oc, 0)
     assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-11, e
+
+
+# ----------------------------------------------------------------- magnetosphere-like block
+def _msph_oracle(N, steps, precision=0):
+    from kaiju_amd import msph
+    spec = configs.magnetosphere(N, steps=steps, precision=precision)
+    p = spec.ic_params
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.grid["xyz"][...] = msph.corners(spec.N, spec.nG, p["r_in"], p["r_out"], p["theta0"])
+    oc.metric()
+    oc.add_b0()
+    gas, mag = msph.uniform_state(oc.grid["xyz"], p["rho"], p["v"], p["p"], p["b"], spec.model.gamma)
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    oc.hatch_finish()
+    return spec, oc
+
+
+def test_magnetosphere_wedge_matches_oracle():
+    """Spherical-polar wedge, dipole background, solar wind, SphIn/SphOut and
+    zero-gradient walls, Boris: the GPU's metric from the same corners, and 4
+    steps from the oracle's hatched state, against the oracle."""
+    say("[case] msph 16x12x16 vs oracle")
+    spec, oc = _msph_oracle((16, 12, 16), steps=4)
+    with GamApp(spec) as a:   # the GPU's own metric from the same corners
+        a.set_corners(oc.grid["xyz"])
+        g = a.get_grid()
+        for k in ("volume", "face", "Tf", "edge", "Teb", "xfc", "xyzcc"):
+            e = np.abs(g[k] - oc.grid[k]).max() / max(np.abs(oc.grid[k]).max(), 1e-300)
+            assert e <= 1e-13, (k, e)
+    with app_from_oracle(oc) as app:
+        divb0 = app.divb()
+        assert app.steps(1) == 0
+        oc.step(1)
+        e1 = compare_state(app, oc, 0)
+        assert app.steps(spec.steps - 1) == 0
+        oc.step(spec.steps - 1)
+        e = compare_state(app, oc, 0)
+        say(f"   after 1: {e1}  after {spec.steps}: {e}")
+        assert max(e1["Gas"], e1["magFlux"], e1["Bxyz"]) <= 1e-12, e1
+        assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-10, e
+        scale = float(np.abs(oc.state.magFlux).max()) * spec.zones
+        assert abs(app.divb() - divb0) <= 1e-13 * scale
+
+
+def test_config_magnetosphere_full_size():
+    """The magnetosphere-like configuration at its size (192x192x256, FP32 bulk,
+    500 steps): no CalcDT failure, div.B unchanged to rounding, a finite state."""
+    say("[case] msph 192x192x256, 500 steps")
+    spec = configs.magnetosphere()
+    with GamApp(spec) as app:
+        app.InitModel()
+        divb0 = app.divb()
+        assert app.steps(spec.steps) == 0
+        s = app.get_state(0)
+        assert np.isfinite(s["Gas"]).all() and np.isfinite(s["magFlux"]).all()
+        scale = float(np.abs(s["magFlux"]).max()) * spec.zones
+        assert abs(app.divb() - divb0) <= 1e-12 * scale
+        say(f"   t = {app.Model.t:.4e}, dt = {app.Model.dt:.3e}")
