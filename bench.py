@@ -105,6 +105,73 @@ def fp32_leg(n: int, steps: int, warmup: int, s64: dict, ms64: float):
                 stage_avg_ms={k: round(v["ms"] / v["launches"], 4) for k, v in tim.items() if v["launches"] > 0})
 
 
+def _fixed_run(n: int, precision: int, dt: float, nsteps: int):
+    """Blast wave at n^3 stepped nsteps times with a fixed dt; (state, grid volume, wall s)."""
+    import dataclasses
+    from kaiju_amd import configs
+    from kaiju_amd.gamera import GamApp
+    spec = configs.blast_wave(n, steps=nsteps, precision=precision)
+    spec = dataclasses.replace(spec, model=dataclasses.replace(spec.model, fixedTimestep=True))
+    with GamApp(spec) as app:
+        app.InitModel()
+        c = app.get_clock()
+        c.dt, c.oStateTime = dt, c.stateTime - dt
+        app.set_clock(c)
+        app.sync()
+        t0 = time.perf_counter()
+        st = app.steps(nsteps)
+        app.sync()
+        wall = time.perf_counter() - t0
+        if st:
+            raise RuntimeError(f"fixed-dt run {n}^3 status {st}")
+        s = app.get_state(0)
+        vol = app.get_grid()["volume"]
+    return s, vol, wall
+
+
+def equal_error_leg(n: int = 128, steps: int = 20):
+    """FP64 and FP32-bulk at n^3 against the FP64 solution at (2n)^3, same
+    physical time (fixed dt, dt/2 on the fine grid): L1 error per conserved
+    variable of each coarse run against the fine run downsampled by
+    volume-weighted 2x2x2 averages (the warp maps the fine corners of even index
+    onto the coarse corners)."""
+    import numpy as np
+    from kaiju_amd import configs
+    from kaiju_amd.gamera import GamApp
+    with GamApp(configs.blast_wave(n)) as app:   # CFL dt of the initial state
+        app.InitModel()
+        dt = 0.8 * app.Model.dt
+    s64, vol, w64 = _fixed_run(n, 0, dt, steps)
+    s32, _, w32 = _fixed_run(n, 1, dt, steps)
+    sf, volf, wf = _fixed_run(2 * n, 0, 0.5 * dt, 2 * steps)
+    nG = 4
+    A = slice(nG, nG + 2 * n)
+    U = sf["Gas"][A, A, A] * volf[A, A, A][..., None]
+    V = volf[A, A, A]
+    sh = (n, 2, n, 2, n, 2)
+    Vc = V.reshape(sh).sum(axis=(1, 3, 5))
+    ref = np.stack([U[..., v].reshape(sh).sum(axis=(1, 3, 5)) / Vc for v in range(5)], axis=-1)
+    del U, sf
+    a = slice(nG, nG + n)
+    names = ("rho", "momx", "momy", "momz", "energy")
+
+    def l1(s):
+        g = s["Gas"][a, a, a]
+        out = {}
+        for v, name in enumerate(names):
+            grp = (1, 2, 3) if 1 <= v <= 3 else (v,)
+            den = sum(float(np.abs(ref[..., q]).mean()) for q in grp) / len(grp)
+            out[name] = float(f"{float(np.abs(g[..., v] - ref[..., v]).mean()) / den:.4e}")
+        return out
+    e64, e32 = l1(s64), l1(s32)
+    ratio = max(e32[k] / e64[k] for k in names)
+    log(f"[bench] equal-error check {n}^3 vs {2 * n}^3: FP64 {e64}  FP32 {e32}  worst ratio {ratio:.4f}")
+    return dict(workload=f"blast wave, warped {n}^3 (FP64 and FP32 bulk) against FP64 at {2 * n}^3, {steps} steps "
+                         f"of fixed dt = {dt:.4e} (fine: {2 * steps} of dt/2)",
+                l1_vs_fine_fp64=e64, l1_vs_fine_fp32=e32, worst_error_ratio_fp32_over_fp64=round(ratio, 5),
+                coarse_ms_per_step_fp64=round(w64 / steps * 1e3, 4), coarse_ms_per_step_fp32=round(w32 / steps * 1e3, 4))
+
+
 def msph_leg():
     """The magnetosphere-like configuration (192x192x256 spherical-polar wedge,
     dipole background, FP32 bulk, Boris): its 500 steps timed from the hatched
@@ -233,6 +300,15 @@ def main():
         del s64
     else:
         app.close()
+    eq = None
+    if not args.no_fp32:
+        try:
+            eq = equal_error_leg()
+        except Exception as e:
+            log(f"[bench] equal-error leg failed: {e!r}")
+            eq = {"error": repr(e)}
+        if fp32 is not None:
+            fp32["equal_error"] = eq
     msph = None
     if not args.no_msph:
         try:
