@@ -64,7 +64,8 @@ typedef struct {
     int32_t ts;         /* Model%ts                                  */
     int32_t status;     /* last CalcDT outcome: 0 ok, 1 bad, 2 disaster */
     int32_t dt0_after;  /* capture dt0 at the first step with ts >= dt0_after (0: never) */
-    int32_t pad;
+    int32_t ncpr;       /* CPR repairs done (params doCPR: a bad, not disastrous CalcDT
+                           runs StateCPR, step.F90:140-142, and the step goes on) */
 } gam_clock;
 
 /* --- lifetime ------------------------------------------------------------ */

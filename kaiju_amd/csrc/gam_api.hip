@@ -53,6 +53,7 @@ struct gam_handle {
     const float *corr = nullptr; // learned correction field [G][8] (device), or none
     double corr_scale = 0.0;
     int bct[6] = {0, 0, 0, 0, 0, 0};   // boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK (GAM_BC_*)
+    int docpr = 0;                     // params doCPR: StateCPR after a bad CalcDT
     double *ctG = nullptr, *ctM = nullptr, *crG = nullptr, *crM = nullptr;   // correction training buffers
     BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
     double *partial = nullptr;
@@ -236,6 +237,7 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     DClock c = {};
     c.tFin = par->tFin; c.limDT0 = par->limDT0; c.limCPR = par->limCPR;
     c.fixedTimestep = par->fixedTimestep; c.doCPR = par->doCPR;
+    h->docpr = par->doCPR;
     *h->clk_host = c;
     if (hipMemcpy(h->clk, &c, sizeof c, hipMemcpyHostToDevice) != hipSuccess) return cleanup(-2);
     // geometry defaults (init.F90:846-850,973-976,1098)
@@ -464,7 +466,7 @@ extern "C" int gam_set_clock(gam_handle *h, const gam_clock *c) {
     HIPCHK(h, hipMemcpy(h->clk_host, h->clk, sizeof(DClock), hipMemcpyDeviceToHost));
     DClock &d = *h->clk_host;
     d.t = c->t; d.dt = c->dt; d.dt0 = c->dt0; d.stime = c->stateTime; d.otime = c->oStateTime;
-    d.ts = c->ts; d.status = c->status; d.dt0_after = c->dt0_after;
+    d.ts = c->ts; d.status = c->status; d.dt0_after = c->dt0_after; d.ncpr = c->ncpr; d.cpr_pending = 0;
     d.halt = 0;
     HIPCHK(h, hipMemcpy(h->clk, h->clk_host, sizeof(DClock), hipMemcpyHostToDevice));
     return 0;
@@ -477,7 +479,7 @@ extern "C" int gam_get_clock(gam_handle *h, gam_clock *c) {
     if (rc) return rc;
     const DClock &d = *h->clk_host;
     c->t = d.t; c->dt = d.dt; c->dt0 = d.dt0; c->stateTime = d.stime; c->oStateTime = d.otime;
-    c->ts = d.ts; c->status = d.status; c->dt0_after = d.dt0_after; c->pad = 0;
+    c->ts = d.ts; c->status = d.status; c->dt0_after = d.dt0_after; c->ncpr = d.ncpr;
     return 0;
 }
 
@@ -756,6 +758,24 @@ static void launch_dtfinal(gam_handle *h, int advance, int calc, int np) {
     hipLaunchKernelGGL(k_dtfinal, 1, 1024, 0, h->stream, h->clk, h->partial, np, advance, calc);
 }
 
+// StateCPR on State when the CalcDT just done asked for it (doCPR only; the
+// kernel returns at once unless the device clock says so)
+template <typename R>
+static void launch_cpr_t(gam_handle *h) {
+    const GVT<R> &g = view<R>(h);
+    const long n = (long)g.Nip * g.Njp * g.Nkp;
+    if (h->mp.doBoris)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cpr<1, R>), grid1(n), 256, 0, h->stream, g, h->mp, h->bgv, h->clk,
+                           P<R>(h->Gas[h->s]), P<R>(h->B[h->s]));
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cpr<0, R>), grid1(n), 256, 0, h->stream, g, h->mp, h->bgv, h->clk,
+                           P<R>(h->Gas[h->s]), P<R>(h->B[h->s]));
+}
+static void launch_cpr(gam_handle *h) {
+    if (!h->docpr) return;
+    if (h->prec) launch_cpr_t<float>(h); else launch_cpr_t<double>(h);
+}
+
 template <int RM, int BO, int MH, int BG, typename R>
 static void launch_flux_bg(gam_handle *h, int d) {
     const GVT<R> &g = view<R>(h);
@@ -961,6 +981,7 @@ extern "C" int gam_calc_dt(gam_handle *h, double *dt, int *status) {
     int rc = celldt(h, h->Gas[h->s], h->B[h->s]);
     if (rc) return rc;
     launch_dtfinal(h, 0, 1, h->npart);
+    launch_cpr(h);
     gam_clock c;
     rc = gam_get_clock(h, &c);
     if (rc) return rc;
@@ -986,6 +1007,7 @@ extern "C" int gam_step(gam_handle *h, int nsteps) {
         int rc = advance(h, 1);
         if (rc) return rc;
         launch_dtfinal(h, 1, 1, update_blocks(h));   // block minima were left in h->partial by k_update
+        launch_cpr(h);
         if ((rc = bcs(h, true))) return rc;
     }
     gam_clock c;
@@ -1120,6 +1142,7 @@ extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_gr
         for (int k = 0; k < nsteps && !rc; k++) {
             rc = advance(h, 1);
             if (!rc) launch_dtfinal(h, 1, 1, update_blocks(h));
+            if (!rc) launch_cpr(h);
             if (!rc) rc = bcs(h, true);
         }
         if (!rc && (e = hipEventRecord(g->evs[1 + q], h->stream)) != hipSuccess) rc = -2;

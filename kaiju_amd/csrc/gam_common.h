@@ -36,6 +36,7 @@ struct DClock {
     double t, dt, dt0, tFin, stime, otime, pdt, odt;
     double limDT0, limCPR;
     int ts, status, dt0_after, fixedTimestep, doCPR, halt;
+    int cpr_pending, ncpr;   // CalcDT asked for a CPR repair this step; repairs so far
 };
 
 // Per-cell / per-face quantities the stencil kernels read, computed once per

@@ -770,6 +770,7 @@ __global__ void __launch_bounds__(256) k_celldt(GVT<R> g, MP m, BGV bg, const DC
 // when advance != 0, then CalcDT's checks and clamps (step.F90:103-166).
 __global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int np, int advance, int calc) {
     if (clk->halt) return;
+    if (threadIdx.x == 0) clk->cpr_pending = 0;
 #ifdef GAM_EMU
     if (threadIdx.x != 0) return;
     double dtMin = GAM_HUGE;
@@ -803,6 +804,11 @@ __global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int n
         if (clk->doCPR && clk->dt0 > GAM_TINY && dtMin / clk->dt0 <= clk->limCPR) bad = 1;
         if (dtMin <= GAM_TINY) disaster = 1;
         status = disaster ? 2 : (bad ? 1 : 0);
+        if (clk->doCPR && bad && !disaster) {   // step.F90:140-142: StateCPR (k_cpr) and go on
+            status = 0;
+            clk->cpr_pending = 1;
+            clk->ncpr = clk->ncpr + 1;
+        }
     }
     if ((clk->t + dt) > clk->tFin) dt = fmax(clk->tFin - clk->t, GAM_TINY);
     if (clk->dt0 > GAM_TINY && dt > (10.0 * clk->dt0)) dt = 10.0 * clk->dt0;
@@ -811,6 +817,53 @@ __global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int n
     if (status) clk->halt = 1;   // the reference stops here (step.F90:138-154); the host decides
     // console-output dt0 capture (output.F90:57-59) at the configured cadence
     if (clk->dt0_after > 0 && clk->ts >= clk->dt0_after && clk->dt0 < GAM_TINY) clk->dt0 = clk->dt;
+}
+
+// step.F90:279-302 StateCPR after a bad (not disastrous) CalcDT with doCPR:
+// CellCPR (step.F90:305-430, single fluid) on every active cell whose CFL step
+// is at most limCPR of dt0: the fastest of the flow and sound speeds is slowed
+// by alpha = 0.9 (velocity) or alpha^2 (pressure); a floored cell loses its
+// velocity; the Alfven branch is off in the reference.  FP64 on the stored state.
+template <int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_cpr(GVT<R> g, MP m, BGV bg, const DClock *__restrict__ clk, R *__restrict__ Gas,
+                                             const R *__restrict__ B) {
+    if (!clk->cpr_pending || clk->halt) return;
+    const int n = g.Nip * g.Njp * g.Nkp;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = g.nG + q % g.Nip, r = q / g.Nip, j = g.nG + r % g.Njp, k = g.nG + r / g.Njp;
+    const int c = ci(g, i, j, k);
+    const size_t nc = g.nc;
+    double U[5], Bt[3];
+#pragma unroll
+    for (int v = 0; v < 5; v++) U[v] = (double)Gas[nc * v + c];
+#pragma unroll
+    for (int x = 0; x < 3; x++) Bt[x] = (double)B[nc * x + c] + (bg.on ? bg.B0[nc * x + c] : 0.0);
+    if (!(cell_dt<BORIS>(g, m, U, Bt, c) / clk->dt0 <= clk->limCPR)) return;
+    const double alpha = 0.9;
+    double pW[5];
+    cell_c2p(m, U, pW);
+    const double Vf = sqrt(sq(pW[1]) + sq(pW[2]) + sq(pW[3]));
+    const double rhoF = fmax(U[0], m.dFloor);   // gamutils.F90:324-341 CellPress2Cs
+    const double vx = U[1] / rhoF, vy = U[2] / rhoF, vz = U[3] / rhoF;
+    const double KinE = 0.5 * rhoF * (sq(vx) + sq(vy) + sq(vz));
+    const double P = fmax((m.gamma - 1) * (U[4] - KinE), m.pFloor);
+    const double Cs = sqrt(m.gamma * P / rhoF);
+    double Va = 0.0;
+    if (m.doMHD) {
+        Va = sqrt(sq(Bt[0]) + sq(Bt[1]) + sq(Bt[2])) / sqrt(U[0]);
+        if (m.doBoris) Va = m.Ca * Va / sqrt(m.Ca * m.Ca + Va * Va);
+    }
+    const bool doVf = Vf >= Cs && Vf >= Va;
+    const bool doCs = !doVf && Cs >= Va && Cs >= Vf;
+    const bool floored = (pW[0] < m.dFloor + GAM_TINY) || (pW[4] < m.pFloor + GAM_TINY);
+    if (doVf) { pW[1] = alpha * pW[1]; pW[2] = alpha * pW[2]; pW[3] = alpha * pW[3]; }
+    if (doCs) pW[4] = alpha * alpha * pW[4];
+    if (floored) pW[1] = pW[2] = pW[3] = 0.0;
+    double Un[5];
+    cell_p2c(m, pW, Un);
+#pragma unroll
+    for (int v = 0; v < 5; v++) Gas[nc * v + c] = R(Un[v]);
 }
 
 // ----------------------------------------------------------------- BCs

@@ -65,7 +65,7 @@ contains
         type(gam_clock_t) :: c
         c%t = App%Model%t; c%dt = App%Model%dt; c%dt0 = App%Model%dt0; c%ts = App%Model%ts
         c%stateTime = App%State%time; c%oStateTime = App%oState%time
-        c%status = 0; c%dt0_after = 0; c%pad = 0
+        c%status = 0; c%dt0_after = 0; c%ncpr = 0
         call gpuCheck(App, gam_set_clock(App%gpu, c), "gam_set_clock")
     end subroutine PushClock
 

@@ -16,7 +16,7 @@ module gamera_hip_c
 
     type, bind(C) :: gam_clock_t
         real(c_double) :: t, dt, dt0, stateTime, oStateTime
-        integer(c_int32_t) :: ts, status, dt0_after, pad
+        integer(c_int32_t) :: ts, status, dt0_after, ncpr
     end type gam_clock_t
 
     interface

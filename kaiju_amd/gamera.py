@@ -41,7 +41,7 @@ class GamParams(C.Structure):
 
 class GamClock(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("t", "dt", "dt0", "stateTime", "oStateTime")] + \
-               [(n, C.c_int32) for n in ("ts", "status", "dt0_after", "pad")]
+               [(n, C.c_int32) for n in ("ts", "status", "dt0_after", "ncpr")]
 
 
 class HipStepError(RuntimeError):

@@ -30,7 +30,7 @@ class OrcModel(C.Structure):
                [(n, C.c_int) for n in
                 ("ts", "rmeth", "doLFMLim", "doBoris", "doMHD", "do25D", "doCPR",
                  "fixedTimestep", "status", "doBackground")] + \
-               [("bc", C.c_int * 6)]
+               [("bc", C.c_int * 6), ("ncpr", C.c_int)]
 
 
 class OrcGrid(C.Structure):

@@ -786,8 +786,57 @@ double orc_cell_dt(const orc_model *m, const orc_grid *g, const orc_state *s, in
     return m->CFL * dl / vCFL;
 }
 
-double orc_calc_dt(orc_model *m, const orc_grid *g, const orc_state *s) {
-    /* step.F90:58-168 CalcDT; failure paths return a status instead of stopping */
+/* step.F90:305-430 CellCPR (single fluid): slow the fastest of the flow and
+ * sound speeds of one cell by alpha = 0.9 (velocity) or alpha^2 (pressure);
+ * a floored cell loses its velocity.  The Alfven branch is disabled in the
+ * reference (doVa = .false.). */
+static void cell_cpr(const orc_model *m, const orc_grid *g, orc_state *s, int i, int j, int k) {
+    const double alpha = 0.9;
+    const size_t nc = g->nc, c = CI(g, i, j, k);
+    double U[5], pW[5];
+    for (int v = 0; v < 5; v++) U[v] = s->Gas[c + nc * v];
+    orc_cell_c2p(m, U, pW);
+    const double Vf = sqrt(sq(pW[1]) + sq(pW[2]) + sq(pW[3]));
+    /* gamutils.F90:324-341 CellPress2Cs */
+    const double rhoF = fmax(U[DEN], m->dFloor);
+    const double Vx = U[MOMX] / rhoF, Vy = U[MOMY] / rhoF, Vz = U[MOMZ] / rhoF;
+    const double KinE = 0.5 * rhoF * (sq(Vx) + sq(Vy) + sq(Vz));
+    const double P = fmax((m->gamma - 1) * (U[ENERGY] - KinE), m->pFloor);
+    const double Cs = sqrt(m->gamma * P / rhoF);
+    double Va = 0.0;
+    if (m->doMHD) {
+        double B[3];
+        for (int d = 0; d < 3; d++) B[d] = s->Bxyz[c + nc * d] + (m->doBackground ? g->B0[c + nc * d] : 0.0);
+        const double MagB = sqrt(sq(B[0]) + sq(B[1]) + sq(B[2]));
+        Va = MagB / sqrt(U[DEN]);
+        if (m->doBoris) Va = m->Ca * Va / sqrt(m->Ca * m->Ca + Va * Va);
+    }
+    int doVf = 0, doCs = 0;
+    if (Vf >= Cs && Vf >= Va) doVf = 1;
+    else if (Cs >= Va && Cs >= Vf) doCs = 1;
+    const int isFloored = (pW[DEN] < m->dFloor + TINY) || (pW[4] < m->pFloor + TINY);
+    double pWS[5], pConS[5];
+    orc_cell_c2p(m, U, pWS);
+    if (doVf) for (int d = 1; d <= 3; d++) pWS[d] = alpha * pWS[d];
+    if (doCs && Cs >= Va && Cs >= Vf) pWS[4] = alpha * alpha * pWS[4];   /* the bulk channel's Cs is Cs */
+    if (isFloored) pWS[1] = pWS[2] = pWS[3] = 0.0;
+    orc_cell_p2c(m, pWS, pConS);
+    for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = pConS[v];
+}
+
+/* step.F90:279-302 StateCPR: CellCPR on every active cell whose CFL step is at
+ * most limCPR of dt0 */
+static void state_cpr(const orc_model *m, const orc_grid *g, orc_state *s) {
+    for (int k = g->ks; k <= g->ke; k++)
+        for (int j = g->js; j <= g->je; j++)
+            for (int i = g->is; i <= g->ie; i++)
+                if (orc_cell_dt(m, g, s, i, j, k) / m->dt0 <= m->limCPR) cell_cpr(m, g, s, i, j, k);
+}
+
+double orc_calc_dt(orc_model *m, const orc_grid *g, orc_state *s) {
+    /* step.F90:58-168 CalcDT; failure paths return a status instead of
+     * stopping, except that with doCPR a bad (not disastrous) step is repaired
+     * by StateCPR and the run goes on (:140-142) */
     m->status = 0;
     if (m->fixedTimestep) return m->dt;
     double dtOld = m->dt;
@@ -808,6 +857,11 @@ double orc_calc_dt(orc_model *m, const orc_grid *g, const orc_state *s) {
         if (m->doCPR && m->dt0 > TINY && dtMin / m->dt0 <= m->limCPR) bad = 1;
         if (dtMin <= TINY) disaster = 1;
         m->status = disaster ? 2 : (bad ? 1 : 0);
+        if (m->doCPR && bad && !disaster) {
+            state_cpr(m, g, s);
+            m->ncpr = m->ncpr + 1;
+            m->status = 0;
+        }
     }
     if ((m->t + dt) > m->tFin) dt = fmax(m->tFin - m->t, TINY);
     if (m->dt0 > TINY && dt > (10.0 * m->dt0)) dt = 10.0 * m->dt0;

@@ -29,6 +29,7 @@ typedef struct {
     int status;   /* CalcDT outcome: 0 ok, 1 bad (dt drop), 2 disaster */
     int doBackground;   /* background field B0 split (gamtypes.F90:88) */
     int bc[6];          /* boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK: ORC_BC_* */
+    int ncpr;           /* CalcDT CPR repairs done (step.F90:140-142) */
 } orc_model;
 
 /* boundary conditions (bcs.F90): periodic, zero gradient, zero extension,
@@ -98,7 +99,7 @@ double     *orc_solver_array(orc_solver *sv, int which);
 void   orc_hatch_finish(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o);
 void   orc_advance_mhd(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o,
                        orc_solver *sv, double dt);
-double orc_calc_dt(orc_model *m, const orc_grid *g, const orc_state *s);
+double orc_calc_dt(orc_model *m, const orc_grid *g, orc_state *s);
 void   orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s);
 void   orc_bflux2fld(const orc_grid *g, const double *magFlux, double *Bxyz);
 void   orc_step(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o, orc_solver *sv);

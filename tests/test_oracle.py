@@ -480,6 +480,44 @@ def test_reflecting_walls_stop_the_mass_flux():
     assert abs(m2 - m0) > 1e-8 * m0
 
 
+def test_cpr_scales_the_fastest_speed_of_slow_cells():
+    """StateCPR (step.F90:279-430): after a bad CalcDT with doCPR the run goes
+    on, and each repaired cell differs from the unrepaired step only by its
+    pressure times 0.81 (sound speed the fastest) or its velocity times 0.9
+    (flow speed the fastest), with its density unchanged."""
+    kw = dict(steps=5, B0=0.5)
+    a = oracle.OracleCase(configs.blast_wave(16, doCPR=True, limCPR=1.05, **kw))
+    b = oracle.OracleCase(configs.blast_wave(16, **kw))
+    for n in range(5):
+        ncpr = a.m.ncpr
+        b.state.copy_from(a.state)
+        b.ostate.copy_from(a.ostate)
+        for f in ("t", "dt", "dt0", "ts"):
+            setattr(b.m, f, getattr(a.m, f))
+        assert a.step(1) == 0 and b.step(1) == 0
+        if a.m.ncpr == ncpr:
+            continue
+        nG = 4
+        A = (slice(nG, nG + 16),) * 3
+        ga, gb = a.state.Gas[A], b.state.Gas[A]
+        changed = np.any(ga != gb, axis=-1)
+        assert changed.any()
+        assert np.array_equal(ga[..., 0], gb[..., 0])
+        g = 5.0 / 3.0
+
+        def prim(U):
+            v = U[..., 1:4] / U[..., :1]
+            p = (g - 1) * (U[..., 4] - 0.5 * U[..., 0] * np.sum(v * v, axis=-1))
+            return v, p
+        va, pa = prim(ga[changed])
+        vb, pb = prim(gb[changed])
+        is_cs = np.isclose(pa, 0.81 * pb, rtol=1e-11) & np.all(np.isclose(va, vb, rtol=1e-11, atol=1e-14), axis=-1)
+        is_vf = np.isclose(pa, pb, rtol=1e-11) & np.all(np.isclose(va, 0.9 * vb, rtol=1e-11, atol=1e-14), axis=-1)
+        assert np.all(is_cs | is_vf)
+        return
+    pytest.fail("no CPR repair happened")
+
+
 # ----------------------------------------------------------- invariants of the step
 @pytest.mark.parametrize("name", ["ot", "bw"])
 def test_divb_and_conservation(name):
