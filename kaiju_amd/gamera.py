@@ -374,6 +374,46 @@ class GamApp:
         self._chk(lib().gam_get_state(self.h, which, C.byref(t), _p(Gas), _p(mag), _p(B)), "gam_get_state")
         return dict(time=t.value, Gas=Gas, magFlux=mag, Bxyz=B)
 
+    # ---------------------------------------------------------------- restart
+    CLOCK_FIELDS = ("t", "dt", "dt0", "stateTime", "oStateTime", "ts", "status", "dt0_after", "ncpr")
+
+    def save_restart(self, path: str):
+        """Write State, oState and the clock to an .npz file: the content of the
+        reference's restart files (gioH5.F90:650-872: both states, ts, dt, dt0),
+        enough for a bit-exact resume of this block."""
+        c = self.get_clock()
+        arrays = {}
+        for which, tag in ((0, "State"), (1, "oState")):
+            s = self.get_state(which)
+            for k in ("Gas", "magFlux", "Bxyz"):
+                arrays[f"{tag}_{k}"] = s[k]
+            arrays[f"{tag}_time"] = np.float64(s["time"])
+        for f in self.CLOCK_FIELDS:
+            arrays[f"clock_{f}"] = np.array(getattr(c, f))
+        arrays["dims"] = np.array(self.spec.N + (self.nG,))
+        arrays["precision"] = np.array(self.spec.model.precision)
+        np.savez(path, **arrays)
+
+    def load_restart(self, path: str, corners: Optional[np.ndarray] = None):
+        """Resume from save_restart's file (same block shape and precision): the
+        grid from the spec's corners (or the given ones) and its background
+        field, as Hatch builds them, then both states and the clock."""
+        with np.load(path, allow_pickle=False) as z:
+            if tuple(z["dims"]) != self.spec.N + (self.nG,) or int(z["precision"]) != self.spec.model.precision:
+                raise ValueError(f"{path}: restart of another block shape or precision")
+            self.set_corners(gen_corners(self.spec) if corners is None else corners)
+            if self.spec.background is not None:
+                b = self.spec.background
+                self.background_dipole(b.get("M0", 0.0), b.get("xc", (0.0, 0.0, 0.0)), b.get("bu", (0.0, 0.0, 0.0)))
+            for which, tag in ((0, "State"), (1, "oState")):
+                self.put_state(which, float(z[f"{tag}_time"]), np.asfortranarray(z[f"{tag}_Gas"]),
+                               np.asfortranarray(z[f"{tag}_magFlux"]), np.asfortranarray(z[f"{tag}_Bxyz"]))
+            c = self.get_clock()
+            for f in self.CLOCK_FIELDS:
+                setattr(c, f, type(getattr(c, f))(z[f"clock_{f}"].item()))
+            self.set_clock(c)
+        self._sync_model()
+
     def get_clock(self) -> GamClock:
         c = GamClock()
         self._chk(lib().gam_get_clock(self.h, C.byref(c)), "gam_get_clock")
