@@ -146,11 +146,28 @@ int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt,
  * order EnforceBCs applies them, step.F90:17-55): 0 periodic (bcs.F90:279-450),
  * 1 zero gradient (zeroGrad_*, :799-930), 2 zero extension (zeroExten_*,
  * :454-603), 3 Cartesian reflecting (cartReflect_*, :607-795), 4 spherical
- * radial (SphIn on INI, SphOut on OUTI, :934-982; i faces only).  An axis is
- * periodic on both faces or on neither; 2.5D blocks keep k periodic.  Default:
- * all periodic.  The reference's BC objects (Grid%externalBCs) map onto these. */
+ * radial (SphIn on INI, SphOut on OUTI, :934-982; i faces only), 5 LFM axis
+ * (lfmIn on INJ, lfmOut on OUTJ, :1020-1106; both j faces, k periodic over an
+ * even Nkp >= 16: the ghosts across the axis from the cells half a turn away).
+ * An axis is periodic on both faces or on neither; 2.5D blocks keep k periodic.
+ * Default: all periodic.  The reference's BC objects (Grid%externalBCs) map
+ * onto these. */
 int  gam_set_bcs(gam_handle *h, const int *type);
 int  gam_get_bcs(gam_handle *h, int *type);
+
+/* --- ring average about the LFM axis (Model%doRing, ring/gid = "lfm") ----- */
+/* Replaces InitRings/RingGridFix (ringutils.F90:37-390) and turns on, inside
+ * every step, RingFlux (stress.F90:137-139), PoleE (mhdgroup.F90:138-142),
+ * RingFlux2FieldFix in bFlux2Fld (gridutils.F90:298-300) and RingAverage after
+ * the update (mhdgroup.F90:69-74, ringav.F90:25-73; PPM ring reconstruction,
+ * one Fourier mode removed from the ring fluxes, fast loop cleaning, no random
+ * shift).  Ring r = 1..nring of each pole (cell rows js+r-1 and je-r+1) is
+ * averaged in nch[r-1] chunks along k.  Call before gam_set_corners, which then
+ * applies RingGridFix to the geometry it derives (gam_set_grid takes geometry
+ * as given).  Needs an even Nkp in 16..512, Njp >= 2*nring + 4, chunk counts
+ * dividing Nkp and at most Nkp/2.  nring = 0 turns it off. */
+int  gam_set_ring(gam_handle *h, int nring, const int *nch);
+int  gam_get_ring(gam_handle *h, int *nring, int *nch);
 
 /* --- state ---------------------------------------------------------------- */
 /* which: 0 = State, 1 = oState, 2 = predictor (half-step) state (read only).

@@ -18,7 +18,18 @@ CFL_DEFAULT = 0.30000001192092896
 
 
 # the library's boundary types, by code (gam_set_bcs)
-BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere")
+BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere", "lfm")
+
+
+def ring_chunks(Np: int) -> Tuple[int, ...]:
+    """Default chunks per ring of an LFM grid with Np cells around the axis
+    (reference ringutils.F90:132-195, the non-aggressive set)."""
+    table = {8: (4,), 16: (8,), 32: (8, 16), 64: (8, 16, 32, 32), 128: (8, 16, 32, 32, 64, 64, 64, 64),
+             256: (8, 16, 32, 32, 64, 64, 64, 64, 128, 128, 128, 128),
+             512: (8, 16, 32, 32, 64, 64, 64, 64, 128, 128, 128, 128, 256, 256, 256, 256)}
+    if Np not in table:
+        raise ValueError(f"no default ring chunking for Np = {Np}")
+    return table[Np]
 
 
 @dataclasses.dataclass
@@ -63,6 +74,8 @@ class CaseSpec:
     background: Optional[dict] = None
     # boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK (one of BC_TYPES)
     bcs: Tuple[str, str, str, str, str, str] = ("periodic",) * 6
+    # ring average about the LFM axis (Model%doRing): chunks per ring, or None
+    ring: Optional[Tuple[int, ...]] = None
 
     @property
     def zones(self) -> int:
@@ -109,22 +122,33 @@ def blast_wave(n: int = 256, steps: int = 100, warp_amp: float = 0.05, B0: float
 
 
 def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in: float = 2.0, r_out: float = 30.0,
-                  theta0: float = 0.15, Ca_kms: float = 1000.0, **mkw) -> CaseSpec:
+                  theta0: Optional[float] = None, Ca_kms: float = 1000.0, axis: bool = True, **mkw) -> CaseSpec:
     """Magnetosphere-like block (kaiju_amd/msph.py): stretched spherical-polar
-    wedge r in [2, 30] R_E around the x axis, Earth dipole -31000 nT R_E^3 as the
+    grid r in [2, 30] R_E around the x axis, Earth dipole -31000 nT R_E^3 as the
     background field, solar wind rho = 5 cm^-3, v = (-400, 0, 0) km/s (at rest
     inside ~12 R_E at the start), T = 1e5 K, IMF Bz = -5 nT, Boris speed cap
-    1000 km/s, FP32 bulk."""
+    1000 km/s, FP32 bulk.
+
+    axis = True (the reference's LFM-grid layout): theta spans [theta0, pi -
+    theta0] with theta0 = 1e-3 by default, the j faces take the LFM axis BCs and
+    the rings next to the axis are averaged with the default chunking for Np.
+    axis = False: a wedge over [theta0, pi - theta0] (default theta0 = 0.15)
+    with zero-gradient theta walls."""
     from kaiju_amd import msph
     rho, v, p, b = msph.wind_state()
     m = ModelSpec(gamma=5.0 / 3.0, CFL=0.3, doBoris=True, Ca=Ca_kms / 100.0, precision=precision, **mkw)
     nr, nt, nphi = N
-    return CaseSpec(name=f"msph{nr}x{nt}x{nphi}", N=(nr, nt, nphi), bounds=(r_in, r_out, theta0, math.pi - theta0,
-                                                                         0.0, 2 * math.pi),
-                    grid="sph", ic="msph", ic_params=dict(r_in=r_in, r_out=r_out, theta0=theta0, rho=rho, v=v, p=p, b=b),
+    if theta0 is None:
+        theta0 = msph.AXIS_THETA0 if axis else 0.15
+    wall = "lfm" if axis else "zerograd"
+    return CaseSpec(name=f"msph{nr}x{nt}x{nphi}" + ("" if axis else "w"), N=(nr, nt, nphi),
+                    bounds=(r_in, r_out, theta0, math.pi - theta0, 0.0, 2 * math.pi),
+                    grid="sph", ic="msph",
+                    ic_params=dict(r_in=r_in, r_out=r_out, theta0=theta0, rho=rho, v=v, p=p, b=b, axis=axis),
                     model=m, steps=steps, background=dict(M0=msph.dipole_moment(), xc=(0.0, 0.0, 0.0),
                                                           bu=(0.0, 0.0, 0.0)),
-                    bcs=("sphere", "sphere", "zerograd", "zerograd", "periodic", "periodic"))
+                    bcs=("sphere", "sphere", wall, wall, "periodic", "periodic"),
+                    ring=ring_chunks(nphi) if axis else None)
 
 
 # The two workloads named by the benchmark configuration list.

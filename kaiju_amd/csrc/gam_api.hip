@@ -14,15 +14,16 @@
 // The kernels are part of this translation unit.
 #include "gam_step.hip"
 #include "gam_setup.hip"
+#include "gam_ring.hip"
 
 // ----------------------------------------------------------------- stages
 enum Stage {
     ST_CLOCK = 0, ST_PREDICT, ST_BHALF, ST_MOM2FACE, ST_EMF_I, ST_EMF_J, ST_EMF_K, ST_FLUX_I, ST_FLUX_J,
-    ST_FLUX_K, ST_CT, ST_UPDATE, ST_DTFINAL, ST_BC, ST_BPOST, ST_CELLDT, ST_N
+    ST_FLUX_K, ST_CT, ST_UPDATE, ST_DTFINAL, ST_BC, ST_BPOST, ST_CELLDT, ST_RING, ST_N
 };
 static const char *kStageNames[ST_N] = {
     "clock", "predictor", "bflux_half", "mom2face", "emf_i", "emf_j", "emf_k", "flux_i", "flux_j",
-    "flux_k", "ct_update", "update", "dt_final", "bcs", "bflux_post", "celldt"};
+    "flux_k", "ct_update", "update", "dt_final", "bcs", "bflux_post", "celldt", "ring"};
 
 enum { G_XYZ = 0, G_XYZCC, G_XFC, G_FACE, G_TF, G_EDGE, G_TE, G_TEB, G_VOL, G_DI, G_DJ, G_DK, G_N };
 // geometry the bulk kernels read, kept as an FP32 copy in precision 1
@@ -54,6 +55,10 @@ struct gam_handle {
     double corr_scale = 0.0;
     int bct[6] = {0, 0, 0, 0, 0, 0};   // boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK (GAM_BC_*)
     int docpr = 0;                     // params doCPR: StateCPR after a bad CalcDT
+    RingP ring = {};                   // ring average about the LFM axis (gam_set_ring)
+    double *ringSE = nullptr;          // its smoothing EMF (G x 3, zero off the ring rows)
+    double *ringTab = nullptr;         // cos, sin of the ring angles phi_n
+    int fused_np = 0;                  // block minima the fused step's CalcDT reduces
     double *ctG = nullptr, *ctM = nullptr, *crG = nullptr, *crM = nullptr;   // correction training buffers
     BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
     double *partial = nullptr;
@@ -274,7 +279,7 @@ extern "C" int gam_destroy(gam_handle *h) {
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
     for (int q = 0; q < 9; q++) if (h->grid32[q]) (void)hipFree(h->grid32[q]);
     for (int q = 0; q < 5; q++) if (h->bg[q]) (void)hipFree(h->bg[q]);
-    for (double *b : {h->ctG, h->ctM, h->crG, h->crM}) if (b) (void)hipFree(b);
+    for (double *b : {h->ctG, h->ctM, h->crG, h->crM, h->ringSE, h->ringTab}) if (b) (void)hipFree(b);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
     void *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, h->clk,
                       h->wk[0], h->wk[1], h->wk[2], h->wk[3], h->wk[4]};
@@ -375,6 +380,10 @@ extern "C" int gam_set_corners(gam_handle *h, const double *xyz) {
         hipLaunchKernelGGL(k_faces, grid1(n), 256, 0, h->stream, w, d);
     }
     hipLaunchKernelGGL(k_volume, grid1(g.nc), 256, 0, h->stream, w);
+    if (h->ring.on) {   // RingGridFix (init.F90:968), before the edge systems read the face normals
+        hipLaunchKernelGGL(k_ring_dk, grid1((long)g.Ni * g.Nk * h->ring.nring * 2), 256, 0, h->stream, w, h->ring);
+        hipLaunchKernelGGL(k_ring_ghost_geom, grid1((long)(g.Ni + 1) * (g.Nk + 1) * g.nG * 2), 256, 0, h->stream, w);
+    }
     for (int d = 0; d < 3; d++) {
         long nx = (d == 1) ? g.Ni - 1 : (d == 0 ? g.Ni : g.Ni + 1);
         long ny = (d == 0) ? g.Nj + 1 : (d == 1 ? g.Nj : g.Nj - 1);
@@ -828,6 +837,28 @@ static void launch_b2f(gam_handle *h, int st, const TM *mag, void *B) {
     const GVT<R> &g = view<R>(h);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[st],
                        h->clk, mag, P<R>(B));
+    if (h->ring.on)   // RingFlux2FieldFix (gridutils.F90:298-300)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g,
+                           h->clk, P<R>(B));
+}
+
+// RingAverage (ringav.F90:25-73) on State, after the update: smoothing EMF,
+// its E2Flux, CleanLoops, bFlux2Fld, SmoothGas
+template <typename R>
+static void ring_average(gam_handle *h) {
+    const GVT<R> &g = view<R>(h);
+    const GV &g64 = h->gv;
+    const int S = h->s;
+    const RingP &rp = h->ring;
+    StageTimer t(h, ST_RING);
+    hipLaunchKernelGGL(k_ring_smoothe, dim3((unsigned)((g.Nip + 1) * rp.nring * 2)), 256, 0, h->stream, g64, rp, h->clk,
+                       h->ringTab, h->mag[S], h->ringSE);
+    hipLaunchKernelGGL(k_ring_ct, grid1((long)(g.Nip + 1) * (g.Nkp + 1) * 2 * (rp.nring + 2)), 256, 0, h->stream, g64, rp,
+                       h->clk, h->ringSE, h->mag[S]);
+    hipLaunchKernelGGL(k_ring_loops, grid1((long)(g.Nip + 1) * 2), 256, 0, h->stream, g64, h->clk, h->mag[S]);
+    launch_b2f<R, double>(h, ST_BPOST, h->mag[S], h->B[S]);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_gas<R>), dim3((unsigned)(g.Nip * rp.nring * 2)), 256, 0, h->stream, g, h->mp,
+                       rp, h->clk, P<R>(h->Gas[S]));
 }
 
 // AdvanceMHD + UpdateStateData time update; the dt min of the fused path is
@@ -855,6 +886,12 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     } else {
         if (m.rmeth == 1) launch_fluxes<1, 0, 0, R>(h); else launch_fluxes<0, 0, 0, R>(h);
     }
+    if (h->ring.on)   // RingFlux (stress.F90:137-139)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_flux<R>), grid1((long)g.Ni * g.Nk * 2 + (long)g.Nip * g.Njp), 256, 0,
+                           h->stream, g, h->clk, m.doMHD, P<R>(h->Flx));
+    if (h->ring.on && m.doMHD)   // PoleE (mhdgroup.F90:138-142)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pole_e<R>), grid1((long)(g.Nip + 1) * (g.Njp + 1)), 256, 0, h->stream, g,
+                           h->clk, P<R>(h->E));
     if (h->corr)   // learned correction of fluxes and EMFs, before CT and the update
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_apply_corr<R>), grid1(g.nc), 256, 0, h->stream, g, h->clk, m.do25D, h->corr,
                            h->corr_scale, P<R>(h->Flx), P<R>(h->E));
@@ -878,6 +915,15 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     }
     // CopyState State->oState is the buffer swap
     h->s = O;
+    h->fused_np = update_blocks(h);
+    if (h->ring.on) {   // mhdgroup.F90:69-74; CalcDT then sees the averaged rings
+        ring_average<R>(h);
+        if (fuse_dt) {
+            int rc = celldt(h, h->Gas[h->s], h->B[h->s]);
+            if (rc) return rc;
+            h->fused_np = h->npart;
+        }
+    }
     if (!fuse_dt) launch_dtfinal(h, /*advance*/ 1, /*calc*/ 0, 0);
     HIPCHK(h, hipGetLastError());
     return 0;
@@ -912,6 +958,12 @@ static void launch_bc(gam_handle *h, bool withB) {
                                P<R>(h->Gas[S]), h->mag[S], Bw);
             continue;
         }
+        if (h->bct[2 * ax] == GAM_BC_LFM) {   // lfmIn then lfmOut (bcs.F90:1020-1106)
+            for (int side = 0; side < 2; side++)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_lfm<R>), grid1((long)(g.Ni + 1) * (g.Nk + 1) * g.nG), 256, 0,
+                                   h->stream, g, h->clk, side, P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+            continue;
+        }
         const long n = (long)g.nG * ext[ax == 0 ? 1 : 0] * ext[ax == 2 ? 1 : 2];
         for (int side = 0; side < 2; side++)
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1(n), 256, 0, h->stream, g, h->clk, 2 * ax + side,
@@ -924,8 +976,10 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     if (!h || !type) return -1;
     for (int ax = 0; ax < 3; ax++) {
         const int a = type[2 * ax], b = type[2 * ax + 1];
-        if (a < 0 || a > GAM_BC_SPHERE || b < 0 || b > GAM_BC_SPHERE)
+        if (a < 0 || a > GAM_BC_LFM || b < 0 || b > GAM_BC_LFM)
             return fail(h, -1, "gam_set_bcs: unknown boundary type");
+        if ((a == GAM_BC_LFM || b == GAM_BC_LFM) && (ax != 1 || a != b))
+            return fail(h, -1, "gam_set_bcs: the LFM axis BCs are the two j faces' BC");
         if (ax > 0 && (a == GAM_BC_SPHERE || b == GAM_BC_SPHERE))
             return fail(h, -1, "gam_set_bcs: the spherical BCs are i-face BCs");
         if ((a == GAM_BC_PERIODIC) != (b == GAM_BC_PERIODIC))
@@ -933,6 +987,8 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     }
     if (h->mp.do25D && type[4] != GAM_BC_PERIODIC)
         return fail(h, -1, "gam_set_bcs: 2.5D blocks keep the k boundaries periodic");
+    if (type[2] == GAM_BC_LFM && (type[4] != GAM_BC_PERIODIC || h->gv.Nkp < 16 || (h->gv.Nkp & 1)))
+        return fail(h, -1, "gam_set_bcs: the LFM axis BCs need k periodic over an even Nkp >= 16");
     if (h->corr && !(type[0] == 0 && type[2] == 0 && type[4] == 0))
         return fail(h, -1, "gam_set_bcs: a learned correction is set (periodic boundaries only)");
     int rc = gam_sync(h);
@@ -944,6 +1000,48 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
 extern "C" int gam_get_bcs(gam_handle *h, int *type) {
     if (!h || !type) return -1;
     for (int f = 0; f < 6; f++) type[f] = h->bct[f];
+    return 0;
+}
+
+extern "C" int gam_set_ring(gam_handle *h, int nring, const int *nch) {
+    if (!h) return -1;
+    const GV &g = h->gv;
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    if (nring == 0) { h->ring.on = 0; return 0; }
+    if (nring < 0 || nring > GAM_RING_MAX || !nch) return fail(h, -1, "gam_set_ring: 1 to %d rings", GAM_RING_MAX);
+    if (h->mp.do25D || g.Nkp < 16 || g.Nkp > GAM_RING_NPMAX || (g.Nkp & 1))
+        return fail(h, -1, "gam_set_ring: Nkp must be even, 16 .. %d", GAM_RING_NPMAX);
+    if (g.Njp < 2 * nring + 4) return fail(h, -1, "gam_set_ring: Njp must be >= 2*nring + 4");
+    for (int r = 0; r < nring; r++)
+        if (nch[r] < 1 || nch[r] > g.Nkp / 2 || g.Nkp % nch[r])
+            return fail(h, -1, "gam_set_ring: ring %d: chunks must divide Nkp, at most Nkp/2", r + 1);
+    if (h->corr) return fail(h, -1, "gam_set_ring: a learned correction is set (periodic boundaries only)");
+    const size_t nse = 3 * gsz(h);
+    if (!h->ringSE) {
+        HIPCHK(h, hipMalloc(&h->ringSE, nse * sizeof(double)));
+        HIPCHK(h, hipMalloc(&h->ringTab, 2 * (size_t)GAM_RING_NPMAX * sizeof(double)));
+    }
+    HIPCHK(h, hipMemset(h->ringSE, 0, nse * sizeof(double)));   // never written off the ring rows
+    // ringav.F90:447-451: phi_n = dp*n - dp/2, n = 1..Np, dp = 2 pi/Np
+    std::vector<double> tab(2 * (size_t)g.Nkp);
+    const double dp = 2.0 * M_PI / g.Nkp;
+    for (int n = 1; n <= g.Nkp; n++) {
+        const double phi = dp * n - 0.5 * dp;
+        tab[n - 1] = cos(1.0 * phi);
+        tab[g.Nkp + n - 1] = sin(1.0 * phi);
+    }
+    HIPCHK(h, hipMemcpy(h->ringTab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    h->ring.on = 1;
+    h->ring.nring = nring;
+    for (int r = 0; r < GAM_RING_MAX; r++) h->ring.nch[r] = r < nring ? nch[r] : 0;
+    return 0;
+}
+
+extern "C" int gam_get_ring(gam_handle *h, int *nring, int *nch) {
+    if (!h || !nring) return -1;
+    *nring = h->ring.on ? h->ring.nring : 0;
+    if (nch) for (int r = 0; r < *nring; r++) nch[r] = h->ring.nch[r];
     return 0;
 }
 
@@ -1006,7 +1104,7 @@ extern "C" int gam_step(gam_handle *h, int nsteps) {
     for (int n = 0; n < nsteps; n++) {
         int rc = advance(h, 1);
         if (rc) return rc;
-        launch_dtfinal(h, 1, 1, update_blocks(h));   // block minima were left in h->partial by k_update
+        launch_dtfinal(h, 1, 1, h->fused_np);   // block minima left in h->partial by k_update (or CellDT)
         launch_cpr(h);
         if ((rc = bcs(h, true))) return rc;
     }
@@ -1141,7 +1239,7 @@ extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_gr
         if ((e = hipStreamWaitEvent(h->stream, g->evs[0], 0)) != hipSuccess) { rc = -2; break; }
         for (int k = 0; k < nsteps && !rc; k++) {
             rc = advance(h, 1);
-            if (!rc) launch_dtfinal(h, 1, 1, update_blocks(h));
+            if (!rc) launch_dtfinal(h, 1, 1, h->fused_np);
             if (!rc) launch_cpr(h);
             if (!rc) rc = bcs(h, true);
         }

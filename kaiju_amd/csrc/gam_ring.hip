@@ -1,0 +1,458 @@
+// gam_ring.hip -- the LFM-grid axis: through-the-axis ghost geometry and BCs,
+// the axis fixes of the fluxes and EMFs, and the ring average (reference
+// src/gamera/ringutils.F90, ringav.F90, ringrecon.F90, bcs.F90:1020-1106,
+// gamdebug.F90:91-175).  Included by gam_api.hip.
+//
+// Grid: j runs from one axis (the j = js faces) to the other (j = je+1), k goes
+// once around the axis in Np = Nkp cells.  A ghost cell j = js-n sits across
+// the axis from the active cell js+n-1 half a turn away in k; the face and
+// edge maps below follow the reference's index rules.  Ring r of a pole is
+// the cell row js+r-1 (or je-r+1), averaged in nch[r-1] chunks along k.
+//
+// Indices here are 0-based and ghost-inclusive (active cells nG .. nG+N-1);
+// the k numbering of the reference's WrapK is 1-based from ks.
+
+#define GAM_RING_MAX 32
+#define GAM_RING_NPMAX 512
+
+// Block-cooperative ring kernels stride over a ring with the block's threads
+// between __syncthreads() phases; the CPU emulation (threads of a block run
+// one after another) does every phase on thread 0 instead.
+#ifdef GAM_EMU
+#define RING_T0 0
+#define RING_NT 1
+#define RING_ONE_THREAD() do { if (threadIdx.x != 0) return; } while (0)
+#else
+#define RING_T0 ((int)threadIdx.x)
+#define RING_NT ((int)blockDim.x)
+#define RING_ONE_THREAD() do { } while (0)
+#endif
+
+struct RingP {
+    int on, nring;
+    int nch[GAM_RING_MAX];
+};
+
+// ringutils.F90:631-639 WrapK on a ghost-inclusive k
+__host__ __device__ __forceinline__ int ring_wrapk(int k, int nG, int Np) {
+    int kr = k - nG + 1 + Np / 2;
+    if (kr > Np) kr = kr - Np;
+    return kr + nG - 1;
+}
+
+// ringutils.F90:483-520 lfmIJKcc
+__host__ __device__ __forceinline__ void lfm_cc(int nG, int Njp, int Nkp, int j, int k, int &jp, int &kp) {
+    const int js = nG, je = nG + Njp - 1, ks = nG, ke = nG + Nkp - 1;
+    int kk = k < ks ? k + Nkp : (k > ke ? k - Nkp : k), jj = j;
+    if (j < js) { jj = js + (js - j) - 1; kk = ring_wrapk(kk, nG, Nkp); }
+    if (j > je) { jj = je - (j - je) + 1; kk = ring_wrapk(kk, nG, Nkp); }
+    jp = jj; kp = kk;
+}
+
+// ringutils.F90:523-577 lfmIJKfc, face direction d
+__host__ __device__ __forceinline__ void lfm_fc(int nG, int Njp, int Nkp, int d, int j, int k, int &jp, int &kp) {
+    if (d == 0) { lfm_cc(nG, Njp, Nkp, j, k, jp, kp); return; }
+    const int js = nG, je = nG + Njp - 1, ks = nG, ke = nG + Nkp - 1;
+    int kk = k, jj = j;
+    if (k < ks) kk = k + Nkp;
+    if (d == 1 && k > ke) kk = k - Nkp;
+    if (d == 2 && k > ke + 1) kk = k - Nkp;
+    if (d == 1) {
+        if (j < js) { jj = js + (js - j); kk = ring_wrapk(k, nG, Nkp); }
+        if (j > je + 1) { jj = je + 1 - (j - je) + 1; kk = ring_wrapk(k, nG, Nkp); }
+    } else {
+        if (j < js) { jj = js + (js - j) - 1; kk = ring_wrapk(k, nG, Nkp); }
+        if (j > je) { jj = je - (j - je) + 1; kk = ring_wrapk(k, nG, Nkp); }
+    }
+    jp = jj; kp = kk;
+}
+
+// ----------------------------------------------------------------- grid (RingGridFix)
+// ringutils.F90:262-280: the CFL length along k of ring r's cells times the
+// chunk width Np/nch[r-1] (all i and k, both poles)
+__global__ void __launch_bounds__(256) k_ring_dk(GVw g, RingP rp) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)g.Ni * g.Nk * rp.nring * 2;
+    if (idx >= n) return;
+    const int i = (int)(idx % g.Ni);
+    long r = idx / g.Ni;
+    const int k = (int)(r % g.Nk);
+    r /= g.Nk;
+    const int ring = (int)(r % rp.nring), pole = (int)(r / rp.nring);
+    const int j = pole == 0 ? g.nG + ring : g.nG + g.Njp - 1 - ring;
+    const int dJ = g.Nkp / rp.nch[ring];
+    const int c = ciw(g, i, j, k);
+    g.dk[c] = dJ * g.dk[c];
+}
+
+// ringutils.F90:282-389: ghost volume, di/dj/dk, face areas and face normals
+// across the axis (J and K normals flip); thread = (i, k, ghost layer n, pole)
+// over i <= Ni, k <= Nk.  Sources are active rows, which no thread writes.
+__global__ void __launch_bounds__(256) k_ring_ghost_geom(GVw g) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n = (long)(g.Ni + 1) * (g.Nk + 1) * g.nG * 2;
+    if (idx >= n) return;
+    const int i = (int)(idx % (g.Ni + 1));
+    long r = idx / (g.Ni + 1);
+    const int k = (int)(r % (g.Nk + 1));
+    r /= (g.Nk + 1);
+    const int nl = 1 + (int)(r % g.nG), pole = (int)(r / g.nG);
+    const int js = g.nG, je = g.nG + g.Njp - 1;
+    const int jg = pole == 0 ? js - nl : je + nl;
+    int jp, kp;
+    if (k < g.Nk && i < g.Ni) {
+        lfm_cc(g.nG, g.Njp, g.Nkp, jg, k, jp, kp);
+        const int c = ciw(g, i, jg, k), cs = ciw(g, i, jp, kp);
+        g.vol[c] = g.vol[cs];
+        g.di[c] = g.di[cs]; g.dj[c] = g.dj[cs]; g.dk[c] = g.dk[cs];
+    }
+    const size_t nf = g.nf;
+    for (int d = 0; d < 3; d++) {
+        const int jj = (d == 1 && pole == 1) ? jg + 1 : jg;   // past je+1 the first J ghost face is je+2
+        lfm_fc(g.nG, g.Njp, g.Nkp, d, jj, k, jp, kp);
+        const int f = fiw(g, i, jj, k), fs = fiw(g, i, jp, kp);
+        for (int x = 0; x < 3; x++) {
+            const double t = g.Tf[nf * (NORMX + x + 9 * d) + fs];
+            g.Tf[nf * (NORMX + x + 9 * d) + f] = d == 0 ? t : -t;
+        }
+        g.face[nf * d + f] = g.face[nf * d + fs];
+    }
+}
+
+// ----------------------------------------------------------------- state fixes
+// bcs.F90:1020-1106 lfmIn (out = 0) / lfmOut (out = 1); thread = (i, k, n) over
+// i <= Ni, k <= Nk.  Cells and Bxyz from the conjugate cell, I faces as they
+// are, J and K faces with the sign flipped.
+template <typename R>
+__global__ void __launch_bounds__(256) k_bc_lfm(GVT<R> g, const DClock *__restrict__ clk, int out, R *__restrict__ Gas,
+                                                double *__restrict__ mag, R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(g.Ni + 1) * (g.Nk + 1) * g.nG || clk->halt) return;
+    const int i = (int)(idx % (g.Ni + 1));
+    const long r = idx / (g.Ni + 1);
+    const int k = (int)(r % (g.Nk + 1)), nl = 1 + (int)(r / (g.Nk + 1));
+    const int jg = out ? g.nG + g.Njp - 1 + nl : g.nG - nl;
+    const size_t nc = g.nc, nf = g.nf;
+    int jp, kp;
+    if (k < g.Nk && i < g.Ni) {
+        lfm_cc(g.nG, g.Njp, g.Nkp, jg, k, jp, kp);
+        const int c = ci(g, i, jg, k), cs = ci(g, i, jp, kp);
+#pragma unroll
+        for (int v = 0; v < 5; v++) Gas[nc * v + c] = Gas[nc * v + cs];
+#pragma unroll
+        for (int v = 0; v < 3; v++) B[nc * v + c] = B[nc * v + cs];
+    }
+    lfm_fc(g.nG, g.Njp, g.Nkp, 0, jg, k, jp, kp);
+    mag[fi(g, i, jg, k)] = mag[fi(g, i, jp, kp)];
+    const int jf = out ? jg + 1 : jg;
+    lfm_fc(g.nG, g.Njp, g.Nkp, 1, jf, k, jp, kp);
+    mag[nf + fi(g, i, jf, k)] = -mag[nf + fi(g, i, jp, kp)];
+    lfm_fc(g.nG, g.Njp, g.Nkp, 2, jg, k, jp, kp);
+    mag[2 * nf + fi(g, i, jg, k)] = -mag[2 * nf + fi(g, i, jp, kp)];
+}
+
+// ringutils.F90:393-444 RingFlux2FieldFix (after every bFlux2Fld): Bxyz of the
+// j ghost cells from their conjugates; thread = (i, k, n, pole)
+template <typename R>
+__global__ void __launch_bounds__(256) k_ring_bfix(GVT<R> g, const DClock *__restrict__ clk, R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.Ni * g.Nk * g.nG * 2 || clk->halt) return;
+    const int i = (int)(idx % g.Ni);
+    long r = idx / g.Ni;
+    const int k = (int)(r % g.Nk);
+    r /= g.Nk;
+    const int nl = 1 + (int)(r % g.nG), pole = (int)(r / g.nG);
+    const int jg = pole == 0 ? g.nG - nl : g.nG + g.Njp - 1 + nl;
+    int jp, kp;
+    lfm_cc(g.nG, g.Njp, g.Nkp, jg, k, jp, kp);
+    const size_t nc = g.nc;
+    const int c = ci(g, i, jg, k), cs = ci(g, i, jp, kp);
+#pragma unroll
+    for (int x = 0; x < 3; x++) B[nc * x + c] = B[nc * x + cs];
+}
+
+// ringutils.F90:664-679 RingFlux: no gas or magnetic flux through the axis
+// faces (J faces js and je+1, all i and k); with MHD, gamdebug.F90:91-124
+// FixGasFluxLFM: one k flux on the periodic seam (faces ks and ke+1 averaged).
+// Flux slabs Flx[(8*d + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
+template <typename R>
+__global__ void __launch_bounds__(256) k_ring_flux(GVT<R> g, const DClock *__restrict__ clk, int mhd, R *__restrict__ Flx) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nA = (long)g.Ni * g.Nk * 2, nB = (long)g.Nip * g.Njp;
+    if (idx >= nA + nB || clk->halt) return;
+    const size_t nc = g.nc;
+    const int nv = mhd ? 8 : 5;
+    if (idx < nA) {
+        const int i = (int)(idx % g.Ni);
+        const long r = idx / g.Ni;
+        const int k = (int)(r % g.Nk), pole = (int)(r / g.Nk);
+        const int j = pole == 0 ? g.nG : g.nG + g.Njp;
+        const int c = ci(g, i, j, k);
+        for (int v = 0; v < nv; v++) Flx[nc * (8 + v) + c] = R(0);
+        return;
+    }
+    if (!mhd) return;
+    const long q = idx - nA;
+    const int i = g.nG + (int)(q % g.Nip), j = g.nG + (int)(q / g.Nip);
+    const int a = ci(g, i, j, g.nG), b = ci(g, i, j, g.nG + g.Nkp);
+    for (int v = 0; v < 8; v++) {
+        const double m = 0.5 * ((double)Flx[nc * (16 + v) + a] + (double)Flx[nc * (16 + v) + b]);
+        Flx[nc * (16 + v) + a] = R(m);
+        Flx[nc * (16 + v) + b] = R(m);
+    }
+}
+
+// ringav.F90:402-431 PoleE ("lfm") on the axis rows js and je+1: no k EMF, the
+// i EMF is its ring mean; then gamdebug.F90:149-175 FixEFieldLFM: one i and j
+// EMF on the periodic seam (ks and ke+1 averaged).  Thread = (i, j) over
+// i in is..ie+1, j in js..je+1; the PoleE of (i, axis row) is this thread's.
+template <typename R>
+__global__ void __launch_bounds__(256) k_pole_e(GVT<R> g, const DClock *__restrict__ clk, R *__restrict__ E) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(g.Nip + 1) * (g.Njp + 1) || clk->halt) return;
+    const int i = g.nG + (int)(idx % (g.Nip + 1)), j = g.nG + (int)(idx / (g.Nip + 1));
+    const size_t nc = g.nc;
+    R *Ei = E, *Ej = E + nc, *Ek = E + 2 * nc;
+    const int ks = g.nG, ke = g.nG + g.Nkp - 1;
+    if (j == g.nG || j == g.nG + g.Njp) {
+        double sum = 0.0;
+        for (int k = ks; k <= ke; k++) sum = sum + (double)Ei[ci(g, i, j, k)];
+        const R avg = R(sum / g.Nkp);
+        for (int k = 0; k < g.Nk; k++) {
+            Ek[ci(g, i, j, k)] = R(0);
+            Ei[ci(g, i, j, k)] = avg;
+        }
+    }
+    const int a = ci(g, i, j, ks), b = ci(g, i, j, ke + 1);
+    const R ei = R(0.5 * ((double)Ei[a] + (double)Ei[b])), ej = R(0.5 * ((double)Ej[a] + (double)Ej[b]));
+    Ei[a] = ei; Ei[b] = ei; Ej[a] = ej; Ej[b] = ej;
+}
+
+// ----------------------------------------------------------------- ring average
+// math.F90:192-197 minmod and ringrecon.F90 PPM_IntLR
+__device__ __forceinline__ double ring_minmod(double x, double y) {
+    return 0.5 * (copysign(1.0, x) + copysign(1.0, y)) * fmin(fabs(x), fabs(y));
+}
+__device__ __forceinline__ void ring_ppm(double fm2, double fm1, double f, double fp1, double fp2, double &vm, double &vp) {
+    const double dvm2 = fm1 - fm2, dvm1 = f - fm1, dvp1 = fp1 - f, dvp2 = fp2 - fp1;
+    double dvc = 0.5 * (dvm2 + dvm1), SM = 2.0 * ring_minmod(dvm2, dvm1);
+    const double Sm1 = ring_minmod(dvc, SM);
+    dvc = 0.5 * (dvm1 + dvp1); SM = 2.0 * ring_minmod(dvm1, dvp1);
+    const double Sp1 = ring_minmod(dvc, SM);
+    dvc = 0.5 * (dvp2 + dvp1); SM = 2.0 * ring_minmod(dvp2, dvp1);
+    const double Sp2 = ring_minmod(dvc, SM);
+    const double p = 0.5 * (f + fp1) - (Sp2 - Sp1) / 6.0;
+    const double q = 0.5 * (f + fm1) - (Sp1 - Sm1) / 6.0;
+    double ap = p - f, am = q - f;
+    if (ap * am >= 0.0) { ap = 0.0; am = 0.0; }
+    else {
+        if (fabs(ap) >= 2.0 * fabs(am)) ap = -2.0 * am;
+        if (fabs(am) >= 2.0 * fabs(ap)) am = -2.0 * ap;
+    }
+    vp = f + ap;
+    vm = f + am;
+}
+
+// ringrecon.F90:33-87 ReconstructRing for cell n of a ring whose Nc chunk
+// means are mean[] (periodic in the chunk index): the cell mean of its
+// chunk's PPM parabola
+__device__ __forceinline__ double ring_recon_cell(const double *mean, int Nc, int Np, int n) {
+    const int dJ = Np / Nc, c = n / dJ, m = n - c * dJ + 1;
+    auto ch = [&](int q) { return mean[((q % Nc) + Nc) % Nc]; };
+    double fL, fR;
+    const double f = ch(c);
+    ring_ppm(ch(c - 2), ch(c - 1), f, ch(c + 1), ch(c + 2), fL, fR);
+    const double a = 3 * (fL + fR - 2 * f), b = 2 * (3 * f - fR - 2 * fL), cc = fL;
+    return (a / 3.0) * (3 * m * m - 3 * m + 1) / (dJ * dJ) + 0.5 * b * (2 * m - 1) / dJ + cc;
+}
+
+// chunk means of q (sequential sum within a chunk, as the reference's sum())
+__device__ void ring_chunk_means(const double *q, double *mean, int Nc, int Np) {
+    const int dC = Np / Nc;
+    for (int c = RING_T0; c < Nc; c += RING_NT) {
+        double s = 0.0;
+        for (int m = 0; m < dC; m++) s = s + q[c * dC + m];
+        mean[c] = s / dC;
+    }
+    __syncthreads();
+}
+
+// ringav.F90:206-329 CalcSmoothE: one block per (slice i in is..ie+1, ring,
+// pole).  The ring's J (then I) face fluxes lose their m = 0, 1 Fourier modes
+// (CleanRing, NFTMAG = 1), are chunk-averaged and PPM-reconstructed, get the
+// modes back (DirtyRing); the flux change becomes an EMF whose curl makes it,
+// summed along each chunk.  SE (FP64, G-sized, zero outside the ring rows) is
+// written on the ring rows; the PoleE and FixEFieldLFM the reference applies
+// to SE next leave it unchanged (the axis rows stay zero, the seam values are
+// written equal), so they are not launched.  tab: cos, sin of phi_n.
+__global__ void __launch_bounds__(256) k_ring_smoothe(GV g, RingP rp, const DClock *__restrict__ clk,
+                                                      const double *__restrict__ tab, const double *__restrict__ mag,
+                                                      double *__restrict__ SE) {
+    __shared__ double bF[2][GAM_RING_NPMAX];
+    __shared__ double rF[GAM_RING_NPMAX], dEr[2][GAM_RING_NPMAX];
+    __shared__ double mean[GAM_RING_NPMAX];
+    __shared__ double cF[4];
+    if (clk->halt) return;
+    RING_ONE_THREAD();
+    const int Np = g.Nkp, nS = g.nG + (int)(blockIdx.x % (g.Nip + 1));
+    const int rest = (int)(blockIdx.x / (g.Nip + 1));
+    const int nR = 1 + rest % rp.nring, pole = rest / rp.nring;
+    const int NumCh = rp.nch[nR - 1], dC = Np / NumCh;
+    const int js = g.nG, je = g.nG + g.Njp - 1, ks = g.nG;
+    const size_t nf = g.nf, nc = g.nc;
+    const double *cs = tab, *sn = tab + Np;
+    for (int n = RING_T0; n < Np; n += RING_NT) {   // PullRingI
+        const int k = ks + n;
+        if (pole == 0) {
+            bF[0][n] = mag[fi(g, nS, js + nR - 1, k)];
+            bF[1][n] = mag[nf + fi(g, nS, js + nR, k)];
+        } else {
+            bF[0][n] = mag[fi(g, nS, je - nR + 1, k)];
+            bF[1][n] = mag[nf + fi(g, nS, je - nR + 1, k)];
+        }
+        dEr[0][n] = 0.0; dEr[1][n] = 0.0;
+    }
+    __syncthreads();
+    const int fD[2] = {1, 0}, eD[2] = {0, 1};
+    const double eS[2] = {1.0, -1.0};
+    for (int d = 0; d < 2; d++) {
+        const double *B = bF[fD[d]];
+        for (int t = RING_T0; t < 4; t += RING_NT) {   // CleanRing coefficients, each a sequential sum over the ring
+            double s = 0.0;
+            for (int n = 0; n < Np; n++) {
+                const double w = t == 0 ? 1.0 : (t == 1 ? 0.0 : (t == 2 ? cs[n] : sn[n]));
+                s = s + B[n] * w;
+            }
+            const double aScl = 2.0 / Np;
+            cF[t] = (t < 2 ? 0.5 * aScl : 1.0 * aScl) * s;
+        }
+        __syncthreads();
+        for (int n = RING_T0; n < Np; n += RING_NT) {
+            double q = B[n] - cF[0] * 1.0 - cF[1] * 0.0;
+            rF[n] = q - cF[2] * cs[n] - cF[3] * sn[n];
+        }
+        __syncthreads();
+        ring_chunk_means(rF, mean, NumCh, Np);
+        for (int n = RING_T0; n < Np; n += RING_NT) {   // reconstruct, DirtyRing, the flux change
+            double q = ring_recon_cell(mean, NumCh, Np, n) + cF[0] * 1.0 + cF[1] * 0.0;
+            q = q + cF[2] * cs[n] + cF[3] * sn[n];
+            rF[n] = q - B[n];
+        }
+        __syncthreads();
+        for (int c = RING_T0; c < NumCh; c += RING_NT) {
+            const int lRs = c * dC;
+            double cum = -rF[lRs];
+            for (int m = 2; m <= dC; m++) {
+                dEr[eD[d]][lRs + m - 1] = eS[d] * cum;
+                cum = cum - rF[lRs + m - 1];
+            }
+        }
+        __syncthreads();
+    }
+    const int jI = pole ? je - nR + 1 : js + nR, jJ = pole ? je - nR + 1 : js + nR - 1;
+    for (int n = RING_T0; n < Np; n += RING_NT) {
+        SE[ci(g, nS, jI, ks + n)] = dEr[0][n];
+        SE[nc + ci(g, nS, jJ, ks + n)] = dEr[1][n];
+    }
+    if (RING_T0 == 0) {   // the k = ke+1 edges repeat k = ks
+        SE[ci(g, nS, jI, ks + Np)] = dEr[0][0];
+        SE[nc + ci(g, nS, jJ, ks + Np)] = dEr[1][0];
+    }
+}
+
+// fields.F90:18-51 E2Flux with dt = 1 and E = SE, in place, on the faces whose
+// curl can see a ring row: j in [js, js+nring+1] and [je-nring, je+1] (the
+// library keeps these bands apart); thread = (i, k, band row) over i in
+// is..ie+1, k in ks..ke+1
+__global__ void __launch_bounds__(256) k_ring_ct(GV g, RingP rp, const DClock *__restrict__ clk, const double *__restrict__ SE,
+                                                 double *__restrict__ mag) {
+    const int nb = rp.nring + 2;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(g.Nip + 1) * (g.Nkp + 1) * 2 * nb || clk->halt) return;
+    const int i = g.nG + (int)(idx % (g.Nip + 1));
+    long r = idx / (g.Nip + 1);
+    const int k = g.nG + (int)(r % (g.Nkp + 1));
+    const int row = (int)(r / (g.Nkp + 1));
+    const int j = row < nb ? g.nG + row : g.nG + g.Njp - (row - nb);
+    const size_t nc = g.nc, nf = g.nf;
+    const double *Ei = SE, *Ej = SE + nc, *Ek = SE + 2 * nc;
+    const int c = ci(g, i, j, k);
+    const double dfi = Ek[ci(g, i, j + 1, k)] - Ek[c] - Ej[ci(g, i, j, k + 1)] + Ej[c];
+    const double dfj = Ei[ci(g, i, j, k + 1)] - Ei[c] - Ek[ci(g, i + 1, j, k)] + Ek[c];
+    const double dfk = Ej[ci(g, i + 1, j, k)] - Ej[c] - Ei[ci(g, i, j + 1, k)] + Ei[c];
+    const int f = fi(g, i, j, k);
+    mag[f] = mag[f] - 1.0 * dfi;
+    mag[nf + f] = mag[nf + f] - 1.0 * dfj;
+    mag[2 * nf + f] = mag[2 * nf + f] - 1.0 * dfk;
+}
+
+// ringav.F90:349-388 CleanLoops (doFastLoop): the ring mean of the K fluxes of
+// the cells next to the axis (rows js and je) is removed; thread = (slice i in
+// is..ie+1, pole)
+__global__ void __launch_bounds__(256) k_ring_loops(GV g, const DClock *__restrict__ clk, double *__restrict__ mag) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(g.Nip + 1) * 2 || clk->halt) return;
+    const int i = g.nG + (int)(idx % (g.Nip + 1)), pole = (int)(idx / (g.Nip + 1));
+    const int j = pole == 0 ? g.nG : g.nG + g.Njp - 1;
+    const size_t nf = g.nf;
+    const int ks = g.nG, ke = g.nG + g.Nkp - 1;
+    double s = 0.0;
+    for (int k = ks; k <= ke; k++) s = s + mag[2 * nf + fi(g, i, j, k)];
+    const double avg = s / g.Nkp;
+    for (int k = ks; k <= ke + 1; k++) {
+        const int f = 2 * nf + fi(g, i, j, k);
+        mag[f] = mag[f] - 1.0 * avg;
+    }
+}
+
+// ringav.F90:76-200 SmoothGas (single fluid, doMassRA = F, no shift, no
+// velocity cleaning): one block per (slice i in is..ie, ring, pole).  Ring
+// variables rho, momentum, internal energy (Gas2Ring) are chunk-averaged and
+// PPM-reconstructed, then turned back into conserved variables (Ring2Gas).
+template <typename R>
+__global__ void __launch_bounds__(256) k_ring_gas(GVT<R> g, MP mp, RingP rp, const DClock *__restrict__ clk,
+                                                  R *__restrict__ Gas) {
+    __shared__ double w[5][GAM_RING_NPMAX];
+    __shared__ double mean[5][GAM_RING_NPMAX / 2];
+    if (clk->halt) return;
+    RING_ONE_THREAD();
+    const int Np = g.Nkp, nS = g.nG + (int)(blockIdx.x % g.Nip);
+    const int rest = (int)(blockIdx.x / g.Nip);
+    const int nR = 1 + rest % rp.nring, pole = rest / rp.nring;
+    const int NumCh = rp.nch[nR - 1];
+    const int j = pole ? g.nG + g.Njp - 1 - (nR - 1) : g.nG + nR - 1;
+    const size_t nc = g.nc;
+    const double gm1 = mp.gamma - 1;
+    for (int n = RING_T0; n < Np; n += RING_NT) {   // PullRingCC + Gas2Ring
+        const int c = ci(g, nS, j, g.nG + n);
+        const double D = fmax((double)Gas[c], mp.dFloor);
+        const double Mx = Gas[nc + c], My = Gas[2 * nc + c], Mz = Gas[3 * nc + c], E = Gas[4 * nc + c];
+        const double KinE = 0.5 * (Mx * Mx + My * My + Mz * Mz) / D;
+        const double P = fmax(gm1 * (E - KinE), mp.pFloor);
+        w[0][n] = D; w[1][n] = Mx; w[2][n] = My; w[3][n] = Mz;
+        w[4][n] = P / gm1;
+    }
+    __syncthreads();
+    const int dC = Np / NumCh;
+    for (int t = RING_T0; t < 5 * NumCh; t += RING_NT) {   // chunk means
+        const int v = t / NumCh, c = t - v * NumCh;
+        double s = 0.0;
+        for (int m = 0; m < dC; m++) s = s + w[v][c * dC + m];
+        mean[v][c] = s / dC;
+    }
+    __syncthreads();
+    for (int n = RING_T0; n < Np; n += RING_NT) {
+        double q[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) q[v] = ring_recon_cell(mean[v], NumCh, Np, n);
+        // Ring2Gas + PushRingCC
+        const double D = fmax(q[0], mp.dFloor);
+        double P = gm1 * q[4];
+        P = fmax(P, mp.pFloor);
+        const double KinE = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / D;
+        const int cidx = ci(g, nS, j, g.nG + n);
+        Gas[cidx] = R(D);
+        Gas[nc + cidx] = R(q[1]); Gas[2 * nc + cidx] = R(q[2]); Gas[3 * nc + cidx] = R(q[3]);
+        Gas[4 * nc + cidx] = R(KinE + P / gm1);
+    }
+}

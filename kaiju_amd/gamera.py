@@ -152,6 +152,8 @@ def lib():
         "gam_set_correction": (C.c_int, [H, C.c_void_p, C.c_double]),
         "gam_set_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
         "gam_get_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
+        "gam_set_ring": (C.c_int, [H, C.c_int, C.POINTER(C.c_int)]),
+        "gam_get_ring": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "gam_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
         "gam_device_free": (C.c_int, [C.c_void_p]),
         "gam_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -221,7 +223,7 @@ def gen_corners(spec: configs.CaseSpec) -> np.ndarray:
     elif spec.grid == "sph":
         from kaiju_amd import msph
         p = spec.ic_params
-        return msph.corners(spec.N, nG, p["r_in"], p["r_out"], p["theta0"])
+        return msph.corners(spec.N, nG, p["r_in"], p["r_out"], p["theta0"], p.get("axis", False))
     elif spec.grid != "cart":
         raise ValueError(f"grid {spec.grid} is generated by the caller; use set_corners()")
     return xyz
@@ -261,6 +263,8 @@ class GamApp:
         self.h = h
         if any(b != "periodic" for b in spec.bcs):
             self.set_bcs(spec.bcs)
+        if spec.ring:
+            self.set_ring(spec.ring)
 
     # ---------------------------------------------------------------- utils
     def _chk(self, rc: int, what: str) -> int:
@@ -467,6 +471,18 @@ class GamApp:
         t = (C.c_int * 6)()
         self._chk(lib().gam_get_bcs(self.h, t), "gam_get_bcs")
         return tuple(configs.BC_TYPES[x] for x in t)
+
+    def set_ring(self, chunks):
+        """Ring average about the LFM axis with chunks[r] chunks in ring r+1
+        (empty: off); before set_corners, which applies RingGridFix."""
+        n = len(chunks)
+        t = (C.c_int * max(n, 1))(*chunks)
+        self._chk(lib().gam_set_ring(self.h, n, t), "gam_set_ring")
+
+    def get_ring(self):
+        n, t = C.c_int(), (C.c_int * 32)()
+        self._chk(lib().gam_get_ring(self.h, C.byref(n), t), "gam_get_ring")
+        return tuple(t[:n.value])
 
     # ---------------------------------------------------------------- learned correction
     def state_device(self, which: int = 0):

@@ -16,10 +16,10 @@ m_p cm^-3), so time R_E / (100 km/s) = 63.71 s, pressure m_p n v^2 =
 The initial wind is brought to rest inside r ~ 12 R_E (a tanh taper), so the
 run starts with the dipole unperturbed near the inner boundary.
 
-Boundaries: SphIn / SphOut on the radial faces (bcs.F90:934-982), zero
-gradient on the two theta faces (this wedge stops short of the axis; the
-reference's pole BCs and ring averaging are not built), periodic in phi.  The
-background field B0 is the dipole (gam_background_dipole); the evolved field
+Boundaries: SphIn / SphOut on the radial faces (bcs.F90:934-982); on the
+theta faces either the LFM axis BCs with ring averaging (the grid reaches the
+axis up to a hole of half-angle theta0 = 1e-3) or, for a wedge, zero gradient;
+periodic in phi.  The background field B0 is the dipole (gam_background_dipole); the evolved field
 starts as the uniform IMF, whose face fluxes are the discrete curl of the
 exact edge integrals of A = (B x r)/2 (div.B = 0 to rounding).
 """
@@ -54,13 +54,28 @@ def dipole_moment(m_nT_RE3=-31000.0) -> float:
     return m_nT_RE3 / B_UNIT
 
 
-def corners(N, nG=4, r_in=2.0, r_out=30.0, theta0=0.15):
-    """Corner array (F,3) of the stretched spherical-polar wedge, ghosts included."""
+AXIS_THETA0 = 1.0e-3   # polar hole of the axis grid (see corners)
+
+
+def corners(N, nG=4, r_in=2.0, r_out=30.0, theta0=0.15, axis=False):
+    """Corner array (F,3) of the stretched spherical-polar grid, ghosts included.
+
+    axis = False: a wedge, theta linear in j over [theta0, pi - theta0] and on
+    into the ghosts.  axis = True: the LFM-grid layout, the ghost rows past
+    each pole mirrored through the axis (theta(-j) = -theta(j), theta(Nt + j) =
+    2 pi - theta(Nt - j)), so a ghost cell lies on its conjugate cell half a
+    turn away in phi.  theta0 > 0 keeps a hole of that half-angle around the
+    axis: on faces lying exactly on it the reference's metric divides zero by
+    zero (face centre = integral / area)."""
     Nr, Nt, Np = N
     ir = np.arange(-nG, Nr + nG + 1)
     r = r_in * (r_out / r_in) ** (ir / Nr)
     it = np.arange(-nG, Nt + nG + 1)
     th = theta0 + (math.pi - 2 * theta0) * it / Nt
+    if axis:
+        lo, hi = it < 0, it > Nt
+        th[lo] = -(theta0 + (math.pi - 2 * theta0) * (-it[lo]) / Nt)
+        th[hi] = 2 * math.pi - (theta0 + (math.pi - 2 * theta0) * (2 * Nt - it[hi]) / Nt)
     ip = np.arange(-nG, Np + nG + 1)
     ph = 2 * math.pi * ip / Np
     R = r[:, None, None]

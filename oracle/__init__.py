@@ -40,7 +40,8 @@ class OrcGrid(C.Structure):
                  "isDT", "ieDT", "jsDT", "jeDT", "ksDT", "keDT")] + \
                [("nc", C.c_size_t), ("nf", C.c_size_t)] + \
                [(n, _dp) for n in ("xyz", "xyzcc", "xfc", "face", "Tf", "edge", "Te", "Teb",
-                                   "volume", "di", "dj", "dk", "fcB0", "edgB0", "B0", "bFlux0", "dpB0")]
+                                   "volume", "di", "dj", "dk", "fcB0", "edgB0", "B0", "bFlux0", "dpB0")] + \
+               [("ring", C.c_int), ("nring", C.c_int), ("nch", C.c_int * 32)]
 
 
 class OrcState(C.Structure):
@@ -99,6 +100,10 @@ def lib():
             "orc_cell_dt": (C.c_double, [M, G, S, C.c_int, C.c_int, C.c_int]),
             "orc_magkinflux_b0": (None, [M, C.c_int, _dp, _dp, C.c_double, _dp, _dp, C.c_double, _dp, _dp, _dp]),
             "orc_add_b0": (None, [M, G, C.c_double, _dp, _dp]),
+            "orc_ring_average": (None, [M, G, S, _dp]),
+            "orc_reconstruct_ring": (None, [_dp, C.c_int, C.c_int]),
+            "orc_ppm_intlr": (None, [C.c_double] * 5 + [_dp, _dp]),
+            "orc_lfm_map": (None, [G, C.c_int, C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -166,6 +171,11 @@ class OracleCase:
         g.Nip, g.Njp, g.Nkp = spec.N
         g.nG = spec.nG
         L.orc_grid_derive(C.byref(g))
+        ring = getattr(spec, "ring", None)
+        if ring:
+            g.ring, g.nring = 1, len(ring)
+            for r, n in enumerate(ring):
+                g.nch[r] = n
         self.g = g
         G = (g.Ni, g.Nj, g.Nk)
         F = (g.Ni + 1, g.Nj + 1, g.Nk + 1)

@@ -274,8 +274,402 @@ static inline void cell_bxyz(const orc_grid *g, const double *mag, int i, int j,
     }
 }
 
+static void e2flux(const orc_grid *g, double *mag, const double *E, double dt);
+
+/* ----------------------------------------------------- LFM pole and ring average */
+/* ringutils.F90:631-639 WrapK: half a turn around the axis, k numbered 1..Np */
+static inline int wrapk(int k, int Np) {
+    int kp = k + Np / 2;
+    if (kp > Np) kp = kp - Np;
+    return kp;
+}
+
+/* ringutils.F90:483-520 lfmIJKcc: the cell across the axis from (i,j,k) */
+static void lfm_cc(const orc_grid *g, int i, int j, int k, int *ip, int *jp, int *kp) {
+    const int Np = g->Nkp;
+    int kk = k < g->ks ? k + Np : (k > g->ke ? k - Np : k), jj = j;
+    if (j < g->js) { jj = g->js + (g->js - j) - 1; kk = wrapk(kk, Np); }
+    if (j > g->je) { jj = g->je - (j - g->je) + 1; kk = wrapk(kk, Np); }
+    *ip = i; *jp = jj; *kp = kk;
+}
+
+/* ringutils.F90:523-577 lfmIJKfc: the face of direction d across the axis */
+static void lfm_fc(const orc_grid *g, int d, int i, int j, int k, int *ip, int *jp, int *kp) {
+    if (d == ID) { lfm_cc(g, i, j, k, ip, jp, kp); return; }
+    const int Np = g->Nkp;
+    int kk = k, jj = j;
+    if (k < g->ks) kk = k + Np;
+    if (d == JD && k > g->ke) kk = k - Np;
+    if (d == KD && k > g->ke + 1) kk = k - Np;
+    if (d == JD) {
+        if (j < g->js) { jj = g->js + (g->js - j); kk = wrapk(k, Np); }
+        if (j > g->je + 1) { jj = g->je + 1 - (j - g->je) + 1; kk = wrapk(k, Np); }
+    } else {
+        if (j < g->js) { jj = g->js + (g->js - j) - 1; kk = wrapk(k, Np); }
+        if (j > g->je) { jj = g->je - (j - g->je) + 1; kk = wrapk(k, Np); }
+    }
+    *ip = i; *jp = jj; *kp = kk;
+}
+
+/* what: 0 cell, 1..3 the I/J/K face */
+void orc_lfm_map(const orc_grid *g, int what, int i, int j, int k, int *ip, int *jp, int *kp) {
+    if (what == 0) lfm_cc(g, i, j, k, ip, jp, kp);
+    else lfm_fc(g, what - 1, i, j, k, ip, jp, kp);
+}
+
+/* ringutils.F90:242-390 RingGridFix ("lfm"): CFL length along k of the ring
+ * cells times the chunk width, and the ghost geometry across the axis from the
+ * conjugate cells and faces (the J and K face normals flip) */
+static void ring_grid_fix(const orc_grid *g) {
+    const int Np = g->Nkp;
+    const size_t nf = g->nf;
+    for (int r = 1; r <= g->nring; r++) {
+        const int dJ = Np / g->nch[r - 1];
+        const int rows[2] = {g->js + r - 1, g->je - r + 1};
+        for (int p = 0; p < 2; p++)
+            for (int k = g->ksg; k <= g->keg; k++)
+                for (int i = g->isg; i <= g->ieg; i++) {
+                    size_t c = CI(g, i, rows[p], k);
+                    g->dk[c] = dJ * g->dk[c];
+                }
+    }
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int i = g->isg; i <= g->ieg + 1; i++)
+            for (int n = 1; n <= g->nG; n++)
+                for (int p = 0; p < 2; p++) {
+                    const int jg = p == 0 ? g->js - n : g->je + n;
+                    const int jgJ = p == 0 ? jg : jg + 1;   /* first J ghost face past je+1 is je+2 */
+                    int ip, jp, kp;
+                    if (k <= g->keg && i <= g->ieg) {
+                        lfm_cc(g, i, jg, k, &ip, &jp, &kp);
+                        size_t c = CI(g, i, jg, k), cs = CI(g, ip, jp, kp);
+                        g->volume[c] = g->volume[cs];
+                        g->di[c] = g->di[cs]; g->dj[c] = g->dj[cs]; g->dk[c] = g->dk[cs];
+                    }
+                    for (int d = 0; d < 3; d++) {
+                        const int jj = d == JD ? jgJ : jg;
+                        lfm_fc(g, d, i, jj, k, &ip, &jp, &kp);
+                        size_t f = FI(g, i, jj, k), fs = FI(g, ip, jp, kp);
+                        for (int x = 0; x < 3; x++) {
+                            double t = g->Tf[fs + nf * (NORMX + x + 9 * d)];
+                            g->Tf[f + nf * (NORMX + x + 9 * d)] = d == ID ? t : -t;
+                        }
+                        g->face[f + nf * d] = g->face[fs + nf * d];
+                    }
+                }
+}
+
+/* ringutils.F90:393-444 RingFlux2FieldFix: Bxyz of the j ghost cells from their
+ * conjugate cells */
+static void ring_bfix(const orc_grid *g, double *Bxyz) {
+    const size_t nc = g->nc;
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int i = g->isg; i <= g->ieg; i++)
+            for (int n = 1; n <= g->nG; n++)
+                for (int p = 0; p < 2; p++) {
+                    const int jg = p == 0 ? g->js - n : g->je + n;
+                    int ip, jp, kp;
+                    lfm_cc(g, i, jg, k, &ip, &jp, &kp);
+                    size_t c = CI(g, i, jg, k), cs = CI(g, ip, jp, kp);
+                    for (int x = 0; x < 3; x++) Bxyz[c + nc * x] = Bxyz[cs + nc * x];
+                }
+}
+
+/* bcs.F90:1020-1106 lfmIn (out = 0) / lfmOut (out = 1) */
+static void bc_lfm(const orc_grid *g, orc_state *s, int out) {
+    const size_t nc = g->nc, nf = g->nf;
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int i = g->isg; i <= g->ieg + 1; i++)
+            for (int n = 1; n <= g->nG; n++) {
+                const int jg = out ? g->je + n : g->js - n;
+                int ip, jp, kp;
+                if (k <= g->keg && i <= g->ieg) {   /* isCellCenterG */
+                    lfm_cc(g, i, jg, k, &ip, &jp, &kp);
+                    size_t c = CI(g, i, jg, k), cs = CI(g, ip, jp, kp);
+                    for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = s->Gas[cs + nc * v];
+                    for (int v = 0; v < 3; v++) s->Bxyz[c + nc * v] = s->Bxyz[cs + nc * v];
+                }
+                lfm_fc(g, ID, i, jg, k, &ip, &jp, &kp);
+                s->magFlux[FI(g, i, jg, k)] = s->magFlux[FI(g, ip, jp, kp)];
+                const int jf = out ? jg + 1 : jg;
+                lfm_fc(g, JD, i, jf, k, &ip, &jp, &kp);
+                s->magFlux[FI(g, i, jf, k) + nf] = -s->magFlux[FI(g, ip, jp, kp) + nf];
+                lfm_fc(g, KD, i, jg, k, &ip, &jp, &kp);
+                s->magFlux[FI(g, i, jg, k) + 2 * nf] = -s->magFlux[FI(g, ip, jp, kp) + 2 * nf];
+            }
+}
+
+/* ringutils.F90:664-679 RingFlux: no flux through the axis faces; with MHD,
+ * gamdebug.F90:91-124 FixGasFluxLFM: one k flux on the periodic seam */
+static void ring_flux(const orc_model *m, const orc_grid *g, double *gFlx, double *mFlx) {
+    const size_t nc = g->nc;
+    const int rows[2] = {g->js, g->je + 1};
+    for (int p = 0; p < 2; p++)
+        for (int k = g->ksg; k <= g->keg; k++)
+            for (int i = g->isg; i <= g->ieg; i++) {
+                size_t c = CI(g, i, rows[p], k);
+                for (int v = 0; v < 5; v++) gFlx[c + nc * (v + 5 * JD)] = 0.0;
+                if (m->doMHD) for (int v = 0; v < 3; v++) mFlx[c + nc * (v + 3 * JD)] = 0.0;
+            }
+    if (!m->doMHD) return;
+    for (int j = g->js; j <= g->je; j++)
+        for (int i = g->is; i <= g->ie; i++) {
+            size_t a = CI(g, i, j, g->ks), b = CI(g, i, j, g->ke + 1);
+            for (int v = 0; v < 5; v++) {
+                double gF = 0.5 * (gFlx[a + nc * (v + 5 * KD)] + gFlx[b + nc * (v + 5 * KD)]);
+                gFlx[a + nc * (v + 5 * KD)] = gF; gFlx[b + nc * (v + 5 * KD)] = gF;
+            }
+            for (int v = 0; v < 3; v++) {
+                double mF = 0.5 * (mFlx[a + nc * (v + 3 * KD)] + mFlx[b + nc * (v + 3 * KD)]);
+                mFlx[a + nc * (v + 3 * KD)] = mF; mFlx[b + nc * (v + 3 * KD)] = mF;
+            }
+        }
+}
+
+/* ringav.F90:402-431 PoleE ("lfm"): no k EMF on the axis, the i EMF on the
+ * axis is its ring mean; then gamdebug.F90:149-175 FixEFieldLFM: one i and j
+ * EMF on the periodic seam */
+static void pole_e(const orc_grid *g, double *E) {
+    const size_t nc = g->nc;
+    double *Ei = E, *Ej = E + nc, *Ek = E + 2 * nc;
+    const int Np = g->Nkp;
+    const int rows[2] = {g->js, g->je + 1};
+    for (int nS = g->is; nS <= g->ie + 1; nS++)
+        for (int p = 0; p < 2; p++) {
+            const int j = rows[p];
+            for (int k = g->ksg; k <= g->keg; k++) Ek[CI(g, nS, j, k)] = 0.0;
+            double sum = 0.0;
+            for (int k = g->ks; k <= g->ke; k++) sum = sum + Ei[CI(g, nS, j, k)];
+            const double avg = sum / Np;
+            for (int k = g->ksg; k <= g->keg; k++) Ei[CI(g, nS, j, k)] = avg;
+        }
+    for (int j = g->js; j <= g->je + 1; j++)
+        for (int i = g->is; i <= g->ie + 1; i++) {
+            size_t a = CI(g, i, j, g->ks), b = CI(g, i, j, g->ke + 1);
+            double ei = 0.5 * (Ei[a] + Ei[b]), ej = 0.5 * (Ej[a] + Ej[b]);
+            Ei[a] = ei; Ei[b] = ei; Ej[a] = ej; Ej[b] = ej;
+        }
+}
+
+/* ringrecon.F90 PPM_IntLR and math.F90:192-197 minmod */
+static inline double minmod(double x, double y) {
+    return 0.5 * (copysign(1.0, x) + copysign(1.0, y)) * fmin(fabs(x), fabs(y));
+}
+void orc_ppm_intlr(double fm2, double fm1, double f, double fp1, double fp2, double *vm, double *vp) {
+    double dvm2 = fm1 - fm2, dvm1 = f - fm1, dvp1 = fp1 - f, dvp2 = fp2 - fp1;
+    double dvc = 0.5 * (dvm2 + dvm1), SM = 2.0 * minmod(dvm2, dvm1);
+    double Sm1 = minmod(dvc, SM);
+    dvc = 0.5 * (dvm1 + dvp1); SM = 2.0 * minmod(dvm1, dvp1);
+    double Sp1 = minmod(dvc, SM);
+    dvc = 0.5 * (dvp2 + dvp1); SM = 2.0 * minmod(dvp2, dvp1);
+    double Sp2 = minmod(dvc, SM);
+    double p = 0.5 * (f + fp1) - (Sp2 - Sp1) / 6.0;
+    double q = 0.5 * (f + fm1) - (Sp1 - Sm1) / 6.0;
+    double ap = p - f, am = q - f;
+    if (ap * am >= 0.0) { ap = 0.0; am = 0.0; }
+    else {
+        if (fabs(ap) >= 2.0 * fabs(am)) ap = -2.0 * am;
+        if (fabs(am) >= 2.0 * fabs(ap)) am = -2.0 * ap;
+    }
+    *vp = f + ap;
+    *vm = f + am;
+}
+
+/* ringrecon.F90:33-87 ReconstructRing (PPM interval values, the default
+ * ring/rngrec): rW holds Nc chunk means, each repeated over its Np/Nc cells;
+ * each chunk becomes the cell means of its parabola */
+void orc_reconstruct_ring(double *rW, int Nc, int Np) {
+    const int dJ = Np / Nc;
+    double chW[1024 + 4];               /* Nc <= Np <= 1024 */
+    double *ch = chW + 2;                 /* ch[n-1] = chW(n), n = -1 .. Nc+2 */
+    for (int n = 1; n <= Nc; n++) ch[n - 1] = rW[(n - 1) * dJ];
+    for (int n = 1; n <= 2; n++) { ch[1 - n - 1] = ch[Nc - n]; ch[Nc + n - 1] = ch[n - 1]; }
+    for (int n = 1; n <= Nc; n++) {
+        const int mS = (n - 1) * dJ;
+        double fL, fR;
+        const double f = ch[n - 1];
+        orc_ppm_intlr(ch[n - 3], ch[n - 2], f, ch[n], ch[n + 1], &fL, &fR);
+        const double a = 3 * (fL + fR - 2 * f), b = 2 * (3 * f - fR - 2 * fL), c = fL;
+        for (int m = 1; m <= dJ; m++)
+            rW[mS + m - 1] = (a / 3.0) * (3 * m * m - 3 * m + 1) / (dJ * dJ) + 0.5 * b * (2 * m - 1) / dJ + c;
+    }
+}
+
+/* ringav.F90:433-491 CleanRing / DirtyRing with NFTMAG = 1: the m = 0 and m = 1
+ * Fourier modes around the ring, at phi_n = dp*n - dp/2 */
+static void clean_ring(double *Q, int Np, double cF[2][2], const double *cs, const double *sn) {
+    const double aScl = 2.0 / Np;
+    for (int m = 0; m < 2; m++) cF[m][0] = cF[m][1] = 0.0;
+    for (int n = 0; n < Np; n++) {
+        cF[0][0] = cF[0][0] + Q[n] * 1.0;
+        cF[0][1] = cF[0][1] + Q[n] * 0.0;
+        cF[1][0] = cF[1][0] + Q[n] * cs[n];
+        cF[1][1] = cF[1][1] + Q[n] * sn[n];
+    }
+    cF[0][0] = 0.5 * aScl * cF[0][0]; cF[0][1] = 0.5 * aScl * cF[0][1];
+    cF[1][0] = 1.0 * aScl * cF[1][0]; cF[1][1] = 1.0 * aScl * cF[1][1];
+    for (int n = 0; n < Np; n++) {
+        Q[n] = Q[n] - cF[0][0] * 1.0 - cF[0][1] * 0.0;
+        Q[n] = Q[n] - cF[1][0] * cs[n] - cF[1][1] * sn[n];
+    }
+}
+static void dirty_ring(double *Q, int Np, double cF[2][2], const double *cs, const double *sn) {
+    for (int n = 0; n < Np; n++) {
+        Q[n] = Q[n] + cF[0][0] * 1.0 + cF[0][1] * 0.0;
+        Q[n] = Q[n] + cF[1][0] * cs[n] + cF[1][1] * sn[n];
+    }
+}
+
+/* ringav.F90:206-329 CalcSmoothE for one pole (epole 0: +x pole at js, 1: -x
+ * pole at je), into SE (zeroed by the caller) */
+static void calc_smooth_e(const orc_grid *g, const double *mag, double *SE, int epole,
+                          const double *cs, const double *sn) {
+    const int Np = g->Nkp;
+    const size_t nc = g->nc, nf = g->nf;
+    const int fD[2] = {JD, ID}, eD[2] = {ID, JD};
+    const double eS[2] = {1.0, -1.0};
+    double *bF = malloc(sizeof(double) * 3 * Np), *rF = malloc(sizeof(double) * Np);
+    double *dF = malloc(sizeof(double) * Np), *dEr = malloc(sizeof(double) * 3 * Np);
+    for (int nS = g->is; nS <= g->ie + 1; nS++)
+        for (int nR = 1; nR <= g->nring; nR++) {
+            const int NumCh = g->nch[nR - 1], dC = Np / NumCh;
+            for (int n = 0; n < Np; n++) {          /* PullRingI */
+                const int k = g->ks + n;
+                if (!epole) {
+                    bF[n + Np * ID] = mag[FI(g, nS, g->js + nR - 1, k)];
+                    bF[n + Np * JD] = mag[FI(g, nS, g->js + nR, k) + nf];
+                    bF[n + Np * KD] = mag[FI(g, nS, g->js + nR - 1, k) + 2 * nf];
+                } else {
+                    for (int d = 0; d < 3; d++) bF[n + Np * d] = mag[FI(g, nS, g->je - nR + 1, k) + nf * d];
+                }
+            }
+            for (int n = 0; n < 3 * Np; n++) dEr[n] = 0.0;
+            for (int d = 0; d < 2; d++) {
+                double cF[2][2];
+                for (int n = 0; n < Np; n++) rF[n] = bF[n + Np * fD[d]];
+                clean_ring(rF, Np, cF, cs, sn);
+                for (int c = 0; c < NumCh; c++) {      /* chunk means */
+                    double sum = 0.0;
+                    for (int m = 0; m < dC; m++) sum = sum + rF[c * dC + m];
+                    const double avg = sum / dC;
+                    for (int m = 0; m < dC; m++) rF[c * dC + m] = avg;
+                }
+                orc_reconstruct_ring(rF, NumCh, Np);
+                dirty_ring(rF, Np, cF, cs, sn);
+                for (int n = 0; n < Np; n++) dF[n] = rF[n] - bF[n + Np * fD[d]];
+                for (int c = 0; c < NumCh; c++) {      /* EMF whose curl gives the flux change */
+                    const int lRs = c * dC;
+                    double cum = -dF[lRs];
+                    for (int m = 2; m <= dC; m++) {
+                        dEr[lRs + m - 1 + Np * eD[d]] = eS[d] * cum;
+                        cum = cum - dF[lRs + m - 1];
+                    }
+                }
+            }
+            const int jI = epole ? g->je - nR + 1 : g->js + nR, jJ = epole ? g->je - nR + 1 : g->js + nR - 1;
+            for (int n = 0; n < Np; n++) {
+                const int k = g->ks + n;
+                SE[CI(g, nS, jI, k)] = dEr[n + Np * ID];
+                SE[CI(g, nS, jJ, k) + nc] = dEr[n + Np * JD];
+                SE[CI(g, nS, jJ, k) + 2 * nc] = dEr[n + Np * KD];
+            }
+            SE[CI(g, nS, jI, g->ke + 1)] = SE[CI(g, nS, jI, g->ks)];
+            SE[CI(g, nS, jJ, g->ke + 1) + nc] = SE[CI(g, nS, jJ, g->ks) + nc];
+        }
+    free(bF); free(rF); free(dF); free(dEr);
+}
+
+/* ringav.F90:349-388 CleanLoops (doFastLoop): the ring mean of the k fluxes
+ * next to the axis is removed */
+static void clean_loops(const orc_grid *g, double *mag) {
+    const size_t nf = g->nf;
+    const int Np = g->Nkp;
+    const int rows[2] = {g->js, g->je};
+    for (int nS = g->is; nS <= g->ie + 1; nS++)
+        for (int p = 0; p < 2; p++) {
+            double sum = 0.0;
+            for (int k = g->ks; k <= g->ke; k++) sum = sum + mag[FI(g, nS, rows[p], k) + 2 * nf];
+            const double avg = sum / Np;
+            for (int k = g->ks; k <= g->ke + 1; k++) {
+                size_t f = FI(g, nS, rows[p], k) + 2 * nf;
+                mag[f] = mag[f] - 1.0 * avg;
+            }
+        }
+}
+
+/* ringav.F90:76-200 SmoothGas for one pole (doMassRA = F, no shift, no velocity
+ * cleaning): chunk means of rho, momentum and internal energy, PPM within
+ * each chunk */
+static void smooth_gas(const orc_model *m, const orc_grid *g, orc_state *s, int epole) {
+    const int Np = g->Nkp;
+    const size_t nc = g->nc;
+    const double gam = m->gamma;
+    double *rW = malloc(sizeof(double) * 5 * Np);
+    for (int nS = g->is; nS <= g->ie; nS++)
+        for (int nR = 1; nR <= g->nring; nR++) {
+            const int NumCh = g->nch[nR - 1], dC = Np / NumCh;
+            const int j = epole ? g->je - nR + 1 : g->js + nR - 1;
+            for (int n = 0; n < Np; n++) {
+                size_t c = CI(g, nS, j, g->ks + n);
+                double *w = rW;
+                for (int v = 0; v < 5; v++) w[n + Np * v] = s->Gas[c + nc * v];
+                /* Gas2Ring */
+                double D = fmax(w[n], m->dFloor), Mx = w[n + Np], My = w[n + 2 * Np], Mz = w[n + 3 * Np];
+                double E = w[n + 4 * Np];
+                double KinE = 0.5 * (Mx * Mx + My * My + Mz * Mz) / D;
+                double P = fmax((gam - 1) * (E - KinE), m->pFloor);
+                w[n] = D;
+                w[n + 4 * Np] = P / (gam - 1);
+            }
+            for (int v = 0; v < 5; v++) {
+                double *q = rW + Np * v;
+                for (int c = 0; c < NumCh; c++) {
+                    double sum = 0.0;
+                    for (int mm = 0; mm < dC; mm++) sum = sum + q[c * dC + mm];
+                    const double avg = sum / dC;
+                    for (int mm = 0; mm < dC; mm++) q[c * dC + mm] = avg;
+                }
+                orc_reconstruct_ring(q, NumCh, Np);
+            }
+            for (int n = 0; n < Np; n++) {
+                /* Ring2Gas */
+                double D = fmax(rW[n], m->dFloor), Mx = rW[n + Np], My = rW[n + 2 * Np], Mz = rW[n + 3 * Np];
+                double P = (gam - 1) * rW[n + 4 * Np];
+                P = fmax(P, m->pFloor);
+                double KinE = 0.5 * (Mx * Mx + My * My + Mz * Mz) / D;
+                size_t c = CI(g, nS, j, g->ks + n);
+                s->Gas[c] = D;
+                s->Gas[c + nc] = Mx; s->Gas[c + 2 * nc] = My; s->Gas[c + 3 * nc] = Mz;
+                s->Gas[c + 4 * nc] = KinE + P / (gam - 1);
+            }
+        }
+    free(rW);
+}
+
+/* ringav.F90:25-73 RingAverage ("lfm", both poles) */
+void orc_ring_average(const orc_model *m, const orc_grid *g, orc_state *s, double *SmoothE) {
+    const int Np = g->Nkp;
+    double *cs = malloc(sizeof(double) * Np), *sn = malloc(sizeof(double) * Np);
+    const double dp = 2.0 * PI / Np;
+    for (int n = 1; n <= Np; n++) {
+        const double phi = dp * n - 0.5 * dp;
+        cs[n - 1] = cos(1.0 * phi);
+        sn[n - 1] = sin(1.0 * phi);
+    }
+    memset(SmoothE, 0, sizeof(double) * 3 * g->nc);
+    calc_smooth_e(g, s->magFlux, SmoothE, 0, cs, sn);
+    calc_smooth_e(g, s->magFlux, SmoothE, 1, cs, sn);
+    pole_e(g, SmoothE);
+    e2flux(g, s->magFlux, SmoothE, 1.0);
+    clean_loops(g, s->magFlux);
+    orc_bflux2fld(g, s->magFlux, s->Bxyz);
+    smooth_gas(m, g, s, 0);
+    smooth_gas(m, g, s, 1);
+    free(cs); free(sn);
+}
+
 void orc_bflux2fld(const orc_grid *g, const double *mag, double *Bxyz) {
-    /* gridutils.F90:281-301 bFlux2Fld (no ring fix) */
+    /* gridutils.F90:281-301 bFlux2Fld, with RingFlux2FieldFix on a ring grid */
 #pragma omp parallel for collapse(2)
     for (int k = g->ksg; k <= g->keg; k++)
         for (int j = g->jsg; j <= g->jeg; j++)
@@ -285,6 +679,7 @@ void orc_bflux2fld(const orc_grid *g, const double *mag, double *Bxyz) {
                 size_t c = CI(g, i, j, k);
                 Bxyz[c] = B[0]; Bxyz[c + g->nc] = B[1]; Bxyz[c + 2 * g->nc] = B[2];
             }
+    if (g->ring) ring_bfix(g, Bxyz);
 }
 
 /* ----------------------------------------------------- predictor */
@@ -560,6 +955,7 @@ void orc_calc_stress(const orc_model *m, const orc_grid *g, const orc_state *n,
                     if (m->doMHD) for (int v = 0; v < 3; v++) mFlx[c + nc * (v + 3 * d)] = mO[v];
                 }
     }
+    if (g->ring) ring_flux(m, g, gFlx, mFlx);   /* stress.F90:137-139 */
     /* Flux2Deltas stress.F90:152-194 */
 #pragma omp parallel for collapse(2)
     for (int k = g->ks; k <= g->ke; k++)
@@ -686,6 +1082,7 @@ static void apply_update(orc_model *m, const orc_grid *g, orc_state *s, orc_stat
                 for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = s->Gas[c + nc * v] + m->dt * dGasH[c + nc * v];
             }
     if (!m->doMHD) return;
+    if (g->ring) pole_e(g, (double *)E);   /* mhdgroup.F90:138-142 */
     e2flux(g, s->magFlux, E, dt);
     orc_bflux2fld(g, s->magFlux, s->Bxyz);
     int boris = m->doBoris;
@@ -709,7 +1106,7 @@ static void apply_update(orc_model *m, const orc_grid *g, orc_state *s, orc_stat
 /* ----------------------------------------------------- solver object */
 struct orc_solver {
     size_t nc, nf;
-    double *gFlx, *mFlx, *Vf, *dGasH, *dGasM;
+    double *gFlx, *mFlx, *Vf, *dGasH, *dGasM, *SmoothE;
     orc_state hf;
 };
 
@@ -722,6 +1119,7 @@ orc_solver *orc_solver_new(const orc_grid *g) {
     sv->Vf = calloc(3 * g->nc, sizeof(double));
     sv->dGasH = calloc(5 * g->nc, sizeof(double));
     sv->dGasM = calloc(3 * g->nc, sizeof(double));
+    sv->SmoothE = calloc(3 * g->nc, sizeof(double));
     sv->hf.Gas = calloc(5 * g->nc, sizeof(double));
     sv->hf.Bxyz = calloc(3 * g->nc, sizeof(double));
     sv->hf.Efld = calloc(3 * g->nc, sizeof(double));
@@ -731,7 +1129,7 @@ orc_solver *orc_solver_new(const orc_grid *g) {
 
 void orc_solver_free(orc_solver *sv) {
     if (!sv) return;
-    free(sv->gFlx); free(sv->mFlx); free(sv->Vf); free(sv->dGasH); free(sv->dGasM);
+    free(sv->gFlx); free(sv->mFlx); free(sv->Vf); free(sv->dGasH); free(sv->dGasM); free(sv->SmoothE);
     free(sv->hf.Gas); free(sv->hf.Bxyz); free(sv->hf.Efld); free(sv->hf.magFlux);
     free(sv);
 }
@@ -746,6 +1144,7 @@ double *orc_solver_array(orc_solver *sv, int which) {
     case 5: return sv->mFlx;
     case 6: return sv->dGasH;
     case 7: return sv->dGasM;
+    case 8: return sv->SmoothE;
     default: return NULL;
     }
 }
@@ -753,12 +1152,13 @@ double *orc_solver_array(orc_solver *sv, int which) {
 /* ----------------------------------------------------- advance */
 void orc_advance_mhd(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o,
                      orc_solver *sv, double dt) {
-    /* mhdgroup.F90:16-76 AdvanceMHD (no gravity, no ring average) */
+    /* mhdgroup.F90:16-76 AdvanceMHD (no gravity) */
     if (fabs(o->time - s->time) < TINY) o->time = s->time - m->dt;
     orc_predictor(m, g, s, o, &sv->hf, (double)0.5f * dt);
     if (m->doMHD) orc_calc_efield(m, g, s, &sv->hf, sv->Vf, s->Efld);
     orc_calc_stress(m, g, s, &sv->hf, sv->gFlx, sv->mFlx, sv->dGasH, sv->dGasM);
     apply_update(m, g, s, o, dt, sv->dGasH, sv->dGasM, s->Efld);
+    if (g->ring) orc_ring_average(m, g, s, sv->SmoothE);   /* mhdgroup.F90:69-74 */
 }
 
 double orc_cell_dt(const orc_model *m, const orc_grid *g, const orc_state *s, int i, int j, int k) {
@@ -877,6 +1277,7 @@ double orc_calc_dt(orc_model *m, const orc_grid *g, orc_state *s) {
  * axes, as the reference's do loops; the same assignments, including the
  * reference's choices of source component and of which routines write Bxyz. */
 static void bc_face(const orc_grid *g, orc_state *s, int face, int type) {
+    if (type == ORC_BC_LFM) { bc_lfm(g, s, face & 1); return; }
     const size_t nc = g->nc, nf = g->nf;
     const int nG = g->nG, ax = face / 2, out = face & 1;
     double *Gas = s->Gas, *B = s->Bxyz, *mag = s->magFlux;
@@ -1220,7 +1621,8 @@ static inline void cell_coords(const orc_grid *g, int i, int j, int k, double C[
 }
 
 void orc_corners2grid(const orc_grid *g) {
-    /* init.F90:801-1080 Corners2Grid (doQuadFT = F, no ring fix) + ebGeom 1083-1145 */
+    /* init.F90:801-1080 Corners2Grid (doQuadFT = F) with RingGridFix (init.F90:968)
+     * on a ring grid, + ebGeom 1083-1145 */
     init_weights();
     size_t nf = g->nf, nc = g->nc;
     for (size_t f = 0; f < 3 * nf; f++) { g->face[f] = 1.0; g->edge[f] = 1.0; }
@@ -1299,6 +1701,7 @@ void orc_corners2grid(const orc_grid *g) {
                 fA = fmax(g->face[FI(g, i, j, k + 1) + 2 * nf], g->face[FI(g, i, j, k) + 2 * nf]);
                 g->dk[c] = g->volume[c] / fA;
             }
+    if (g->ring) ring_grid_fix(g);
     /* edge triads init.F90:972-1048 */
     for (int d = 0; d < 3; d++) {
         int is = g->isg, ie = g->ieg + 1, js = g->jsg, je = g->jeg + 1, ks = g->ksg, ke = g->keg + 1;

@@ -33,8 +33,11 @@ typedef struct {
 } orc_model;
 
 /* boundary conditions (bcs.F90): periodic, zero gradient, zero extension,
- * Cartesian reflecting, spherical radial (SphIn on INI, SphOut on OUTI) */
-enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3, ORC_BC_SPHERE = 4 };
+ * Cartesian reflecting, spherical radial (SphIn on INI, SphOut on OUTI),
+ * through-the-axis LFM pole (lfmIn on INJ, lfmOut on OUTJ) */
+enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3, ORC_BC_SPHERE = 4,
+       ORC_BC_LFM = 5 };
+#define ORC_MAX_RINGS 32
 
 /* Grid_T subset (reference gamtypes.F90:163-245) */
 typedef struct {
@@ -61,6 +64,12 @@ typedef struct {
     double *B0;      /* G x 3 cell-averaged B0 */
     double *bFlux0;  /* G x 3 B0 flux through each face */
     double *dpB0;    /* G x 3 -div(B0 Maxwell stress)/volume (active cells) */
+    /* ring average about the LFM-grid axis (Model%doRing, gid "lfm"; set before
+     * orc_corners2grid): the j = js and j = je+1 faces lie on the axis, k goes
+     * once around it (Nkp = Np cells), ring r (1-based) of each pole is split
+     * into nch[r-1] chunks */
+    int ring, nring;
+    int nch[ORC_MAX_RINGS];
 } orc_grid;
 
 /* State_T subset (reference gamtypes.F90:248-264) */
@@ -115,6 +124,12 @@ void   orc_calc_efield(const orc_model *m, const orc_grid *g, const orc_state *n
 void   orc_calc_stress(const orc_model *m, const orc_grid *g, const orc_state *n,
                        const orc_state *hf, double *gFlx, double *mFlx,
                        double *dGasH, double *dGasM);
+
+/* ring-average pieces exposed for known-answer tests (ringav.F90, ringrecon.F90) */
+void   orc_ring_average(const orc_model *m, const orc_grid *g, orc_state *s, double *SmoothE);
+void   orc_reconstruct_ring(double *rW, int Nc, int Np);
+void   orc_ppm_intlr(double fm2, double fm1, double f, double fp1, double fp2, double *vm, double *vp);
+void   orc_lfm_map(const orc_grid *g, int what, int i, int j, int k, int *ip, int *jp, int *kp);
 
 /* cell-level helpers exposed for known-answer tests */
 void   orc_cell_c2p(const orc_model *m, const double *con, double *prim);

@@ -45,3 +45,13 @@ def test_correction_kernels_on_cpu_emulation_and_in_bounds():
     env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
     out = _run(lib, env, "correction or corrected", 900, "test_learned.py")
     assert " passed" in out
+
+
+def test_ring_kernels_on_cpu_emulation_and_in_bounds():
+    lib = build_emu.build(asan=False)
+    out = _run(lib, {}, "axis", 600, "test_ring.py")
+    assert " passed" in out
+    lib = build_emu.build(asan=True)
+    env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
+    out = _run(lib, env, "axis", 900, "test_ring.py")
+    assert " passed" in out

@@ -139,12 +139,12 @@ def test_bench_sample_matches_oracle():
 
 
 # ----------------------------------------------------------------- magnetosphere-like block
-def _msph_oracle(N, steps, precision=0):
+def _msph_oracle(N, steps, precision=0, axis=False):
     from kaiju_amd import msph
-    spec = configs.magnetosphere(N, steps=steps, precision=precision)
+    spec = configs.magnetosphere(N, steps=steps, precision=precision, axis=axis)
     p = spec.ic_params
     oc = oracle.OracleCase(spec, setup=False)
-    oc.grid["xyz"][...] = msph.corners(spec.N, spec.nG, p["r_in"], p["r_out"], p["theta0"])
+    oc.grid["xyz"][...] = msph.corners(spec.N, spec.nG, p["r_in"], p["r_out"], p["theta0"], axis)
     oc.metric()
     oc.add_b0()
     gas, mag = msph.uniform_state(oc.grid["xyz"], p["rho"], p["v"], p["p"], p["b"], spec.model.gamma)

@@ -1,0 +1,323 @@
+"""The LFM-grid axis: through-the-axis BCs and ghost geometry, and the ring
+average (reference ringutils.F90, ringav.F90, ringrecon.F90, bcs.F90:1020-1106).
+
+CPU, pinning the oracle:
+* the conjugate maps (lfmIJKcc / lfmIJKfc) against geometry: on a grid whose
+  ghost rows are mirrored through the axis, every ghost cell and face has the
+  same corners as the cell or face the map names, and the ghost face normals
+  computed from the corners equal the conjugate's (I faces) or its negative
+  (J and K faces), which is what the BC's and RingGridFix's signs assume;
+* PPM interval values (textbook properties: linear data reproduced, a local
+  extremum flattened, a steep step limited) and ReconstructRing (each chunk's
+  cell means add up to the chunk mean, a constant ring is unchanged);
+* the ring average: a uniform field (only m = 0, 1 Fourier modes on every
+  ring) and a uniform gas are left unchanged; for a random state the ring
+  totals of mass and momentum, and div.B of every cell, are unchanged.
+
+GPU (and the CPU emulation of the kernels): the axis block 16x12x32 (two
+rings, chunks 8 and 16) against the oracle: the metric with RingGridFix from
+the same corners, 1 step <= 1e-12, 4 steps <= 1e-10, div.B unchanged; the
+fused step equals the separate C-ABI calls bit for bit; FP32 bulk stays
+within 2e-4 of FP64; the full-size magnetosphere configuration with its 12
+rings runs 500 steps.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from kaiju_amd import configs, msph
+from tests.helpers import app_from_oracle, compare_state, say
+
+NG = 4
+
+
+def _oracle_case(N, axis=True, theta0=None, ring=True):
+    spec = configs.magnetosphere(N, steps=4, precision=0, axis=axis, theta0=theta0)
+    if not ring:
+        spec.ring = None
+    p = spec.ic_params
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.grid["xyz"][...] = msph.corners(spec.N, spec.nG, p["r_in"], p["r_out"], p["theta0"], axis)
+    return spec, oc
+
+
+def _map(oc, what, i, j, k):
+    ip, jp, kp = C.c_int(), C.c_int(), C.c_int()
+    oracle.lib().orc_lfm_map(C.byref(oc.g), what, i, j, k, C.byref(ip), C.byref(jp), C.byref(kp))
+    return ip.value, jp.value, kp.value
+
+
+def _ghost_positions(oc):
+    """(what, i, j, k) of the ghost cells and faces the axis BCs write that have
+    all their corners in the corner array, in the reference's 1-based numbering
+    (isg = 1 - nG)."""
+    g = oc.g
+    out = []
+    for k in range(g.ksg, g.keg + 2):
+        for i in range(g.isg, g.ieg + 2, 3):
+            for n in range(1, g.nG + 1):
+                for jg, jJ in ((g.js - n, g.js - n), (g.je + n, g.je + n + 1)):
+                    if k <= g.keg and i <= g.ieg:
+                        out += [(0, i, jg, k), (2, i, jJ, k)]
+                    if k <= g.keg:
+                        out.append((1, i, jg, k))
+                    if i <= g.ieg:
+                        out.append((3, i, jg, k))
+    return out
+
+
+def _corners(xyz, what, i, j, k, g):
+    a, b, c = i - g.isg, j - g.jsg, k - g.ksg
+    if what == 0:
+        pts = [xyz[a + x, b + y, c + z] for x in (0, 1) for y in (0, 1) for z in (0, 1)]
+    elif what == 1:
+        pts = [xyz[a, b + y, c + z] for y in (0, 1) for z in (0, 1)]
+    elif what == 2:
+        pts = [xyz[a + x, b, c + z] for x in (0, 1) for z in (0, 1)]
+    else:
+        pts = [xyz[a + x, b + y, c] for x in (0, 1) for y in (0, 1)]
+    return np.array(pts)
+
+
+def _same_points(A, B, tol):
+    d = np.sqrt(((A[:, None, :] - B[None, :, :]) ** 2).sum(axis=-1))
+    return d.min(axis=1).max() <= tol and d.min(axis=0).max() <= tol
+
+
+def test_conjugate_maps_match_the_geometry():
+    """Exact-axis corners (theta0 = 0, ghost rows mirrored): a ghost cell or face
+    and the one the map names have the same corners.  The ghost cells' own
+    corner frame is left-handed (i and k run as at the conjugate, j runs back
+    toward the axis), so the normals computed from the corners (no RingGridFix)
+    are the conjugate's negated for I and K faces and equal for J faces."""
+    spec, oc = _oracle_case((16, 8, 16), theta0=0.0, ring=False)
+    g, xyz = oc.g, oc.grid["xyz"]
+    oc.metric()
+    Tf = oc.grid["Tf"]
+    pos = _ghost_positions(oc)
+    assert len(pos) > 1000
+    worst = 0.0
+    for what, i, j, k in pos:
+        ip, jp, kp = _map(oc, what, i, j, k)
+        assert g.jsg <= jp <= g.jeg + 1 and g.ks <= kp <= g.ke + 1 and jp != j
+        assert _same_points(_corners(xyz, what, i, j, k, g), _corners(xyz, what, ip, jp, kp, g), 1e-12), (what, i, j, k)
+        if what > 0:
+            d = what - 1
+            a, b = (i - g.isg, j - g.jsg, k - g.ksg), (ip - g.isg, jp - g.jsg, kp - g.ksg)
+            n_g, n_c = Tf[a + (slice(0, 3), d)], Tf[b + (slice(0, 3), d)]
+            if np.all(np.isfinite(n_c)):
+                sign = 1.0 if d == 1 else -1.0
+                worst = max(worst, float(np.abs(n_g - sign * n_c).max()))
+    assert worst <= 1e-9, worst
+
+
+def test_axis_bc_flux_signs_match_the_fixed_normals():
+    """RingGridFix gives the ghost faces the conjugate's normal for I faces and
+    its negative for J and K faces; the axis BCs copy the fluxes with the same
+    signs.  For a uniform field every ghost flux then has the sign of
+    B . n(ghost) * face (a sign slip in either one breaks this)."""
+    spec, oc = _oracle_case((12, 12, 32))
+    oc.metric()
+    p = spec.ic_params
+    B = (0.3, -0.2, 0.5)
+    gas, mag = msph.uniform_state(oc.grid["xyz"], p["rho"], (0.0, 0.0, 0.0), p["p"], B, spec.model.gamma,
+                                  r_taper=0.0)
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    oc.hatch_finish()
+    g, Tf, face = oc.g, oc.grid["Tf"], oc.grid["face"]
+    checked = 0
+    for what, i, j, k in _ghost_positions(oc):
+        if what == 0 or not g.is_ <= i <= g.ie:   # the i ghosts hold the radial BCs' values
+            continue
+        d = what - 1
+        a = (i - g.isg, j - g.jsg, k - g.ksg)
+        bn = float(np.dot(B, Tf[a + (slice(0, 3), d)])) * face[a + (d,)]
+        phi = oc.state.magFlux[a + (d,)]
+        if abs(bn) > 0.05 * np.abs(B).max() * face[a + (d,)]:
+            assert phi * bn > 0 and abs(phi / bn - 1) < 0.05, (what, i, j, k, phi, bn)
+            checked += 1
+    assert checked > 1000
+
+
+def test_ppm_interval_values():
+    L = oracle.lib()
+    vm, vp = C.c_double(), C.c_double()
+
+    def ppm(*f):
+        L.orc_ppm_intlr(*f, C.byref(vm), C.byref(vp))
+        return vm.value, vp.value
+
+    assert ppm(0.0, 1.0, 2.0, 3.0, 4.0) == (1.5, 2.5)           # linear data: the interface values
+    assert ppm(1.0, 1.0, 2.0, 1.0, 1.0) == (2.0, 2.0)           # local maximum: flattened
+    m, p = ppm(0.0, 0.0, 0.0, 1.0, 1.0)                          # a step: the values stay within the data
+    assert 0.0 <= m <= 1.0 and 0.0 <= p <= 1.0 and m == 0.0
+    # the parabola through (vm, f, vp) stays monotone: |f - vm| and |vp - f| at most 2x each other
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        f = rng.normal(size=5)
+        m, p = ppm(*f)
+        lo, hi = min(f[1], f[2], f[3]), max(f[1], f[2], f[3])
+        assert lo - 1e-15 <= m <= hi + 1e-15 and lo - 1e-15 <= p <= hi + 1e-15
+
+
+def test_reconstruct_ring_conserves_each_chunk():
+    L = oracle.lib()
+    rng = np.random.default_rng(2)
+    for Np, Nc in ((32, 8), (32, 16), (256, 8), (256, 128), (16, 1)):
+        dJ = Np // Nc
+        means = rng.normal(size=Nc) + 3.0
+        q = np.repeat(means, dJ).astype(np.float64)
+        L.orc_reconstruct_ring(q.ctypes.data_as(C.POINTER(C.c_double)), Nc, Np)
+        got = q.reshape(Nc, dJ).mean(axis=1)
+        assert np.abs(got - means).max() <= 1e-14 * np.abs(means).max(), (Np, Nc)
+        c = np.full(Np, 2.5)
+        L.orc_reconstruct_ring(c.ctypes.data_as(C.POINTER(C.c_double)), Nc, Np)
+        assert np.array_equal(c, np.full(Np, 2.5))
+
+
+def _ring_average(oc):
+    L = oracle.lib()
+    se = oc.solver_array(8, oc.G + (3,))
+    L.orc_ring_average(C.byref(oc.m), C.byref(oc.g), C.byref(oc.state.s), se.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+def _cell_div(mag):
+    return (mag[1:, :-1, :-1, 0] - mag[:-1, :-1, :-1, 0] + mag[:-1, 1:, :-1, 1] - mag[:-1, :-1, :-1, 1]
+            + mag[:-1, :-1, 1:, 2] - mag[:-1, :-1, :-1, 2])
+
+
+def test_ring_average_keeps_a_uniform_field_and_gas():
+    """Every ring's face fluxes of a uniform B are m = 0, 1 Fourier modes, which
+    CleanRing removes and DirtyRing restores: nothing is left to average."""
+    spec, oc = _oracle_case((12, 12, 32))
+    oc.metric()
+    p = spec.ic_params
+    gas, mag = msph.uniform_state(oc.grid["xyz"], p["rho"], (0.0, 0.0, 0.0), p["p"], (0.3, -0.2, 0.5),
+                                  spec.model.gamma, r_taper=0.0)
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    _ring_average(oc)
+    A = slice(NG, -NG)
+    e_mag = np.abs(oc.state.magFlux - mag)[A, A, A].max() / np.abs(mag[A, A, A]).max()
+    e_gas = np.abs(oc.state.Gas - gas)[A, A, A].max() / np.abs(gas[A, A, A]).max()
+    say(f"   uniform: magFlux {e_mag:.2e}  Gas {e_gas:.2e}")
+    assert e_mag <= 1e-13 and e_gas <= 1e-14
+
+
+def test_ring_average_conserves_ring_totals_and_divb():
+    spec, oc = _oracle_case((12, 12, 32))
+    oc.metric()
+    rng = np.random.default_rng(3)
+    G, F = oc.G, oc.F
+    gas = np.zeros(G + (5,), order="F")
+    gas[..., 0] = 1.0 + 0.5 * rng.random(G)
+    gas[..., 1:4] = rng.normal(size=G + (3,)) * 0.3
+    gas[..., 4] = 2.0 + rng.random(G)
+    # a divergence-free random field: the discrete curl of random edge values
+    Lx = rng.normal(size=F + (3,))
+    mag = np.zeros(F + (3,), order="F")
+    s = lambda a, di, dj, dk: np.roll(a, (-di, -dj, -dk), axis=(0, 1, 2))
+    mag[..., 0] = Lx[..., 1] + s(Lx[..., 2], 0, 1, 0) - s(Lx[..., 1], 0, 0, 1) - Lx[..., 2]
+    mag[..., 1] = s(Lx[..., 0], 0, 0, 1) - Lx[..., 0] - s(Lx[..., 2], 1, 0, 0) + Lx[..., 2]
+    mag[..., 2] = s(Lx[..., 1], 1, 0, 0) - Lx[..., 1] - s(Lx[..., 0], 0, 1, 0) + Lx[..., 0]
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    div0 = _cell_div(oc.state.magFlux.copy())
+    _ring_average(oc)
+    g = oc.g
+    js, je = g.js - g.jsg, g.je - g.jsg
+    rows = [js + r for r in range(len(spec.ring))] + [je - r for r in range(len(spec.ring))]
+    A = slice(NG, NG + spec.N[0])
+    K = slice(NG, NG + spec.N[2])
+    for v in range(4):
+        t0 = gas[A, rows][:, :, K, v].sum(axis=2)
+        t1 = oc.state.Gas[A, rows][:, :, K, v].sum(axis=2)
+        assert np.abs(t1 - t0).max() <= 1e-13 * np.abs(gas[..., v]).max() * spec.N[2], v
+    assert not np.allclose(oc.state.Gas[A, rows][:, :, K, 0], gas[A, rows][:, :, K, 0])   # it did average
+    div1 = _cell_div(oc.state.magFlux)
+    act = (slice(NG, NG + spec.N[0]), slice(NG, NG + spec.N[1]), K)
+    assert np.abs(div1 - div0)[act].max() <= 1e-12 * np.abs(mag).max()
+    assert not np.allclose(oc.state.magFlux, mag)
+
+
+# ----------------------------------------------------------------- GPU
+def _hatched(N, precision=0):
+    spec, oc = _oracle_case(N)
+    if precision:
+        spec.model.precision = precision
+    oc.metric()
+    oc.add_b0()
+    p = spec.ic_params
+    gas, mag = msph.uniform_state(oc.grid["xyz"], p["rho"], p["v"], p["p"], p["b"], spec.model.gamma)
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    oc.hatch_finish()
+    return spec, oc
+
+
+@pytest.mark.gpu
+def test_axis_block_matches_oracle():
+    """Axis block 16x12x32 (rings of 8 and 16 chunks), dipole background, Boris,
+    SphIn/SphOut, LFM axis BCs: the GPU metric with RingGridFix from the same
+    corners, then 4 steps from the oracle's hatched state."""
+    from kaiju_amd.gamera import GamApp
+    say("[ring] msph axis 16x12x32 vs oracle")
+    spec, oc = _hatched((16, 12, 32))
+    with GamApp(spec) as a:
+        assert a.get_ring() == (8, 16) and a.get_bcs()[2:4] == ("lfm", "lfm")
+        a.set_corners(oc.grid["xyz"])
+        gg = a.get_grid()
+        for k in ("volume", "face", "Tf", "edge", "Te", "Teb", "xfc", "xyzcc", "di", "dj", "dk"):
+            e = np.abs(gg[k] - oc.grid[k]).max() / max(np.abs(oc.grid[k]).max(), 1e-300)
+            assert e <= 1e-13, (k, e)
+    with app_from_oracle(oc) as app:
+        divb0 = app.divb()
+        assert app.steps(1) == 0
+        oc.step(1)
+        e1 = compare_state(app, oc, 0)
+        assert app.steps(spec.steps - 1) == 0
+        oc.step(spec.steps - 1)
+        e = compare_state(app, oc, 0)
+        say(f"   after 1: {e1}  after {spec.steps}: {e}")
+        assert max(e1["Gas"], e1["magFlux"], e1["Bxyz"]) <= 1e-12, e1
+        assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-10, e
+        assert abs(app.Model.dt - oc.m.dt) <= 1e-12 * oc.m.dt
+        scale = float(np.abs(oc.state.magFlux).max()) * spec.zones
+        assert abs(app.divb() - divb0) <= 1e-13 * scale
+
+
+@pytest.mark.gpu
+def test_axis_block_fused_step_equals_separate_calls():
+    """gam_step (ring average, CellDT after it, BCs, ring Bxyz fix) equals
+    gam_advance_mhd + gam_calc_dt + gam_enforce_bcs, bit for bit."""
+    say("[ring] fused step vs separate calls")
+    spec, oc = _hatched((16, 12, 32))
+    with app_from_oracle(oc) as a, app_from_oracle(oc) as b:
+        for _ in range(3):
+            a.stepGamera(fused=True)
+            b.stepGamera(fused=False)
+        sa, sb = a.get_state(0), b.get_state(0)
+        for k in ("Gas", "magFlux", "Bxyz"):
+            assert np.array_equal(sa[k], sb[k]), k
+        assert a.Model.dt == b.Model.dt and a.Model.t == b.Model.t
+
+
+@pytest.mark.gpu
+def test_axis_block_fp32_bulk():
+    """FP32 bulk on the axis block: 4 steps within 2e-4 of the FP64 oracle, div.B
+    unchanged."""
+    say("[ring] msph axis 16x12x32 FP32 bulk")
+    spec, oc = _hatched((16, 12, 32), precision=1)
+    with app_from_oracle(oc) as app:
+        divb0 = app.divb()
+        assert app.steps(spec.steps) == 0
+        oc.step(spec.steps)
+        e = compare_state(app, oc, 0)
+        say(f"   {e}")
+        assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 2e-4, e
+        scale = float(np.abs(oc.state.magFlux).max()) * spec.zones
+        assert abs(app.divb() - divb0) <= 1e-12 * scale
