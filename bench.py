@@ -173,9 +173,10 @@ def equal_error_leg(n: int = 128, steps: int = 20):
 
 
 def msph_leg():
-    """The magnetosphere-like configuration (192x192x256 spherical-polar wedge,
-    dipole background, FP32 bulk, Boris): its 500 steps timed from the hatched
-    state (setup excluded)."""
+    """The magnetosphere-like configuration (192x192x256 spherical-polar grid to
+    the axis, LFM axis BCs with 12 averaged rings, dipole background, FP32 bulk,
+    Boris): its 500 steps timed from the hatched state (setup excluded), then 20
+    more with stage timers for the per-stage split."""
     from kaiju_amd import configs
     from kaiju_amd.gamera import GamApp
     spec = configs.magnetosphere()
@@ -191,15 +192,20 @@ def msph_leg():
     wall = time.perf_counter() - t0
     divb = app.divb()
     t_end, dt = app.Model.t, app.Model.dt
+    app.set_timing(True)
+    app.steps(20)
+    app.sync()
+    tim = app.timings()
     app.close()
     log(f"[bench] msph {spec.N}: setup {setup:.1f} s, {spec.steps} steps in {wall:.3f} s, status {st}")
-    return dict(workload="magnetosphere-like block: stretched spherical-polar wedge 192x192x256, r 2-30 R_E, "
-                         "Earth dipole background, solar wind, SphIn/SphOut + zero-gradient walls, Boris 1000 km/s, "
-                         "FP32 bulk (FP64 CT/energy/dt)",
+    return dict(workload="magnetosphere-like block: stretched spherical-polar 192x192x256 to the x axis (hole 1e-3 rad), "
+                         "r 2-30 R_E, Earth dipole background, solar wind, SphIn/SphOut, LFM axis BCs, ring average "
+                         "(12 rings), Boris 1000 km/s, FP32 bulk (FP64 CT/energy/dt)",
                 value=round(spec.steps * spec.zones / wall, 1), unit="zone-updates/s",
                 ms_per_step=round(wall / spec.steps * 1e3, 4), steps=spec.steps, status=st,
                 t_end_code_units=t_end, dt_last=dt, divB_total_before=divb0, divB_total_after=divb,
-                setup_s=round(setup, 2))
+                setup_s=round(setup, 2),
+                stage_avg_ms={k: round(v["ms"] / v["launches"], 4) for k, v in tim.items() if v["launches"] > 0})
 
 
 def roofline(tim: dict, ms_per_step: float, zups: float):

@@ -761,6 +761,24 @@ static int update_blocks(const gam_handle *h) {
     return tile_blocks(h->tile[ST_UPDATE], h->gv.Ni, h->gv.Nj, h->gv.Nk);
 }
 
+// blocks of k_celldt_rows (DT box cells of the ring band), 0 without rings
+static int ring_dt_blocks(const gam_handle *h) {
+    if (!h->ring.on) return 0;
+    const long n = (long)(h->gv.Nip + 2) * (h->gv.Nkp + 2) * 2 * (h->ring.nring + 3);
+    return (int)((n + 255) / 256);
+}
+
+// room for n block minima in h->partial
+static int ensure_partial(gam_handle *h, int n) {
+    if (n <= h->pcap) return 0;
+    double *p = nullptr;
+    HIPCHK(h, hipMalloc(&p, (size_t)n * sizeof(double)));
+    HIPCHK(h, hipFree(h->partial));
+    h->partial = p;
+    h->pcap = n;
+    return 0;
+}
+
 // np = number of block minima in h->partial (calc != 0)
 static void launch_dtfinal(gam_handle *h, int advance, int calc, int np) {
     StageTimer t(h, ST_DTFINAL);
@@ -855,8 +873,12 @@ static void ring_average(gam_handle *h) {
                        h->ringTab, h->mag[S], h->ringSE);
     hipLaunchKernelGGL(k_ring_ct, grid1((long)(g.Nip + 1) * (g.Nkp + 1) * 2 * (rp.nring + 2)), 256, 0, h->stream, g64, rp,
                        h->clk, h->ringSE, h->mag[S]);
-    hipLaunchKernelGGL(k_ring_loops, grid1((long)(g.Nip + 1) * 2), 256, 0, h->stream, g64, h->clk, h->mag[S]);
-    launch_b2f<R, double>(h, ST_BPOST, h->mag[S], h->B[S]);
+    hipLaunchKernelGGL(k_ring_loops, dim3((unsigned)((g.Nip + 1) * 2)), 256, 0, h->stream, g64, h->clk, h->mag[S]);
+    const int nb = rp.nring + 3;   // bFlux2Fld where faces changed, then RingFlux2FieldFix
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f_rows<R>), grid1((long)g.Ni * g.Nk * 2 * nb), 256, 0, h->stream, g, h->clk, nb,
+                       (const double *)h->mag[S], P<R>(h->B[S]));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g, h->clk,
+                       P<R>(h->B[S]));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_gas<R>), dim3((unsigned)(g.Nip * rp.nring * 2)), 256, 0, h->stream, g, h->mp,
                        rp, h->clk, P<R>(h->Gas[S]));
 }
@@ -889,9 +911,12 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     if (h->ring.on)   // RingFlux (stress.F90:137-139)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_flux<R>), grid1((long)g.Ni * g.Nk * 2 + (long)g.Nip * g.Njp), 256, 0,
                            h->stream, g, h->clk, m.doMHD, P<R>(h->Flx));
-    if (h->ring.on && m.doMHD)   // PoleE (mhdgroup.F90:138-142)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pole_e<R>), grid1((long)(g.Nip + 1) * (g.Njp + 1)), 256, 0, h->stream, g,
+    if (h->ring.on && m.doMHD) {   // PoleE + FixEFieldLFM (mhdgroup.F90:138-142)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pole_e<R>), dim3((unsigned)((g.Nip + 1) * 2)), 256, 0, h->stream, g, h->clk,
+                           P<R>(h->E));
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fix_e<R>), grid1((long)(g.Nip + 1) * (g.Njp + 1)), 256, 0, h->stream, g,
                            h->clk, P<R>(h->E));
+    }
     if (h->corr)   // learned correction of fluxes and EMFs, before CT and the update
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_apply_corr<R>), grid1(g.nc), 256, 0, h->stream, g, h->clk, m.do25D, h->corr,
                            h->corr_scale, P<R>(h->Flx), P<R>(h->E));
@@ -906,11 +931,12 @@ static int advance_t(gam_handle *h, int fuse_dt) {
         const Tile tu = h->tile[ST_UPDATE];
         const R *nGas = P<R>(h->Gas[S]), *nB = P<R>(h->B[S]), *Flx = P<R>(h->Flx);
         R *oGas = P<R>(h->Gas[O]), *oB = P<R>(h->B[O]);
+        const int skip = h->ring.on ? h->ring.nring + 3 : 0;
         if (m.doMHD) {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
+            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
         } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
         }
     }
     // CopyState State->oState is the buffer swap
@@ -918,10 +944,17 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     h->fused_np = update_blocks(h);
     if (h->ring.on) {   // mhdgroup.F90:69-74; CalcDT then sees the averaged rings
         ring_average<R>(h);
-        if (fuse_dt) {
-            int rc = celldt(h, h->Gas[h->s], h->B[h->s]);
-            if (rc) return rc;
-            h->fused_np = h->npart;
+        if (fuse_dt) {   // the ring band's CFL minima after k_update's
+            const int nb = h->ring.nring + 3;
+            StageTimer t(h, ST_CELLDT);
+            const dim3 gr(ring_dt_blocks(h));
+            const R *Gs = P<R>(h->Gas[h->s]), *Bs = P<R>(h->B[h->s]);
+            double *pt = h->partial + h->fused_np;
+            if (h->mp.doBoris)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt_rows<1, R>), gr, 256, 0, h->stream, g, m, h->bgv, h->clk, nb, Gs, Bs, pt);
+            else
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_celldt_rows<0, R>), gr, 256, 0, h->stream, g, m, h->bgv, h->clk, nb, Gs, Bs, pt);
+            h->fused_np += ring_dt_blocks(h);
         }
     }
     if (!fuse_dt) launch_dtfinal(h, /*advance*/ 1, /*calc*/ 0, 0);
@@ -1035,7 +1068,7 @@ extern "C" int gam_set_ring(gam_handle *h, int nring, const int *nch) {
     h->ring.on = 1;
     h->ring.nring = nring;
     for (int r = 0; r < GAM_RING_MAX; r++) h->ring.nch[r] = r < nring ? nch[r] : 0;
-    return 0;
+    return ensure_partial(h, update_blocks(h) + ring_dt_blocks(h));
 }
 
 extern "C" int gam_get_ring(gam_handle *h, int *nring, int *nch) {
@@ -1054,10 +1087,13 @@ static int bcs_t(gam_handle *h, bool shell) {
     {
         StageTimer t(h, ST_BPOST);
         const GVT<R> &g = view<R>(h);
-        if (shell && !h->mp.do25D && all_periodic(h)) {
+        if (shell && !h->mp.do25D) {
             const long n = b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f_shell<R>), dim3((unsigned)((n + 255) / 256)), 256, 0, h->stream, g, h->clk,
                                (const double *)h->mag[S], P<R>(h->B[S]));
+            if (h->ring.on)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g,
+                                   h->clk, P<R>(h->B[S]));
         } else {
             launch_b2f<R, double>(h, ST_BPOST, h->mag[S], h->B[S]);
         }
@@ -1333,15 +1369,9 @@ extern "C" int gam_set_tile(gam_handle *h, int stage, int ti, int tj, int tk, in
     int rc = gam_sync(h);   // launches already queued keep the tile they were given
     if (rc) return rc;
     const Tile t = {ti, tj, tk, fast};
-    if (stage == ST_UPDATE) {   // one block minimum per update block
-        const int nb = tile_blocks(t, h->gv.Ni, h->gv.Nj, h->gv.Nk);
-        if (nb > h->pcap) {
-            double *p = nullptr;
-            HIPCHK(h, hipMalloc(&p, (size_t)nb * sizeof(double)));
-            HIPCHK(h, hipFree(h->partial));
-            h->partial = p;
-            h->pcap = nb;
-        }
+    if (stage == ST_UPDATE) {   // one block minimum per update block, then the ring band's
+        rc = ensure_partial(h, tile_blocks(t, h->gv.Ni, h->gv.Nj, h->gv.Nk) + ring_dt_blocks(h));
+        if (rc) return rc;
     }
     h->tile[stage] = t;
     return 0;
@@ -1375,7 +1405,7 @@ extern "C" int gam_stage_bytes(gam_handle *h, int stage, double *bytes) {
         b = F * 3 + G * 3; d = F * 9 + G * (3 + 1); break;
     case ST_BPOST: {   // mag, xfc, xyzcc, vol (d) -> B (b); the fused 3D step recomputes the ghost shell only
         const GV &g = h->gv;
-        const double S = (h->mp.do25D || !all_periodic(h)) ? G : (double)b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
+        const double S = h->mp.do25D ? G : (double)b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
         b = S * 3; d = S * (3 + 9 + 3 + 1); break;
     }
     case ST_MOM2FACE: b = G * (3 + 1 + 9); break;                                 // VdV, vol -> Vf (9)

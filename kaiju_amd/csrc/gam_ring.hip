@@ -203,27 +203,45 @@ __global__ void __launch_bounds__(256) k_ring_flux(GVT<R> g, const DClock *__res
 }
 
 // ringav.F90:402-431 PoleE ("lfm") on the axis rows js and je+1: no k EMF, the
-// i EMF is its ring mean; then gamdebug.F90:149-175 FixEFieldLFM: one i and j
-// EMF on the periodic seam (ks and ke+1 averaged).  Thread = (i, j) over
-// i in is..ie+1, j in js..je+1; the PoleE of (i, axis row) is this thread's.
+// i EMF is its ring mean (summed in k order); one block per (slice i in
+// is..ie+1, axis row), the column staged in LDS.
 template <typename R>
 __global__ void __launch_bounds__(256) k_pole_e(GVT<R> g, const DClock *__restrict__ clk, R *__restrict__ E) {
+    __shared__ double col[GAM_RING_NPMAX];
+    __shared__ double mean;
+    if (clk->halt) return;
+    RING_ONE_THREAD();
+    const int i = g.nG + (int)(blockIdx.x % (g.Nip + 1)), pole = (int)(blockIdx.x / (g.Nip + 1));
+    const int j = pole ? g.nG + g.Njp : g.nG, ks = g.nG, Np = g.Nkp;
+    const size_t nc = g.nc;
+    R *Ei = E, *Ek = E + 2 * nc;
+    for (int n = RING_T0; n < Np; n += RING_NT) col[n] = (double)Ei[ci(g, i, j, ks + n)];
+    __syncthreads();
+    if (RING_T0 == 0) {
+        double sum = 0.0;
+#pragma unroll 8
+        for (int n = 0; n < Np; n++) sum = sum + col[n];
+        mean = sum / Np;
+    }
+    __syncthreads();
+    const R avg = R(mean);
+    for (int k = RING_T0; k < g.Nk; k += RING_NT) {
+        Ek[ci(g, i, j, k)] = R(0);
+        Ei[ci(g, i, j, k)] = avg;
+    }
+}
+
+// gamdebug.F90:149-175 FixEFieldLFM (after PoleE): one i and one j EMF on the
+// periodic seam, ks and ke+1 averaged; thread = (i, j) over i in is..ie+1,
+// j in js..je+1
+template <typename R>
+__global__ void __launch_bounds__(256) k_fix_e(GVT<R> g, const DClock *__restrict__ clk, R *__restrict__ E) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)(g.Nip + 1) * (g.Njp + 1) || clk->halt) return;
     const int i = g.nG + (int)(idx % (g.Nip + 1)), j = g.nG + (int)(idx / (g.Nip + 1));
     const size_t nc = g.nc;
-    R *Ei = E, *Ej = E + nc, *Ek = E + 2 * nc;
-    const int ks = g.nG, ke = g.nG + g.Nkp - 1;
-    if (j == g.nG || j == g.nG + g.Njp) {
-        double sum = 0.0;
-        for (int k = ks; k <= ke; k++) sum = sum + (double)Ei[ci(g, i, j, k)];
-        const R avg = R(sum / g.Nkp);
-        for (int k = 0; k < g.Nk; k++) {
-            Ek[ci(g, i, j, k)] = R(0);
-            Ei[ci(g, i, j, k)] = avg;
-        }
-    }
-    const int a = ci(g, i, j, ks), b = ci(g, i, j, ke + 1);
+    R *Ei = E, *Ej = E + nc;
+    const int a = ci(g, i, j, g.nG), b = ci(g, i, j, g.nG + g.Nkp);
     const R ei = R(0.5 * ((double)Ei[a] + (double)Ei[b])), ej = R(0.5 * ((double)Ej[a] + (double)Ej[b]));
     Ei[a] = ei; Ei[b] = ei; Ej[a] = ej; Ej[b] = ej;
 }
@@ -288,7 +306,7 @@ __device__ void ring_chunk_means(const double *q, double *mean, int Nc, int Np) 
 __global__ void __launch_bounds__(256) k_ring_smoothe(GV g, RingP rp, const DClock *__restrict__ clk,
                                                       const double *__restrict__ tab, const double *__restrict__ mag,
                                                       double *__restrict__ SE) {
-    __shared__ double bF[2][GAM_RING_NPMAX];
+    __shared__ double bF[2][GAM_RING_NPMAX], cs[GAM_RING_NPMAX], sn[GAM_RING_NPMAX];
     __shared__ double rF[GAM_RING_NPMAX], dEr[2][GAM_RING_NPMAX];
     __shared__ double mean[GAM_RING_NPMAX];
     __shared__ double cF[4];
@@ -300,7 +318,6 @@ __global__ void __launch_bounds__(256) k_ring_smoothe(GV g, RingP rp, const DClo
     const int NumCh = rp.nch[nR - 1], dC = Np / NumCh;
     const int js = g.nG, je = g.nG + g.Njp - 1, ks = g.nG;
     const size_t nf = g.nf, nc = g.nc;
-    const double *cs = tab, *sn = tab + Np;
     for (int n = RING_T0; n < Np; n += RING_NT) {   // PullRingI
         const int k = ks + n;
         if (pole == 0) {
@@ -311,6 +328,7 @@ __global__ void __launch_bounds__(256) k_ring_smoothe(GV g, RingP rp, const DClo
             bF[1][n] = mag[nf + fi(g, nS, je - nR + 1, k)];
         }
         dEr[0][n] = 0.0; dEr[1][n] = 0.0;
+        cs[n] = tab[n]; sn[n] = tab[Np + n];
     }
     __syncthreads();
     const int fD[2] = {1, 0}, eD[2] = {0, 1};
@@ -318,10 +336,15 @@ __global__ void __launch_bounds__(256) k_ring_smoothe(GV g, RingP rp, const DClo
     for (int d = 0; d < 2; d++) {
         const double *B = bF[fD[d]];
         for (int t = RING_T0; t < 4; t += RING_NT) {   // CleanRing coefficients, each a sequential sum over the ring
+            const double *W = t == 2 ? cs : sn;
             double s = 0.0;
-            for (int n = 0; n < Np; n++) {
-                const double w = t == 0 ? 1.0 : (t == 1 ? 0.0 : (t == 2 ? cs[n] : sn[n]));
-                s = s + B[n] * w;
+            if (t < 2) {
+                const double w = t == 0 ? 1.0 : 0.0;
+#pragma unroll 8
+                for (int n = 0; n < Np; n++) s = s + B[n] * w;
+            } else {
+#pragma unroll 8
+                for (int n = 0; n < Np; n++) s = s + B[n] * W[n];
             }
             const double aScl = 2.0 / Np;
             cF[t] = (t < 2 ? 0.5 * aScl : 1.0 * aScl) * s;
@@ -387,21 +410,29 @@ __global__ void __launch_bounds__(256) k_ring_ct(GV g, RingP rp, const DClock *_
 }
 
 // ringav.F90:349-388 CleanLoops (doFastLoop): the ring mean of the K fluxes of
-// the cells next to the axis (rows js and je) is removed; thread = (slice i in
-// is..ie+1, pole)
+// the cells next to the axis (rows js and je) is removed; one block per (slice
+// i in is..ie+1, pole), the column staged in LDS
 __global__ void __launch_bounds__(256) k_ring_loops(GV g, const DClock *__restrict__ clk, double *__restrict__ mag) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (long)(g.Nip + 1) * 2 || clk->halt) return;
-    const int i = g.nG + (int)(idx % (g.Nip + 1)), pole = (int)(idx / (g.Nip + 1));
-    const int j = pole == 0 ? g.nG : g.nG + g.Njp - 1;
+    __shared__ double col[GAM_RING_NPMAX];
+    __shared__ double mean;
+    if (clk->halt) return;
+    RING_ONE_THREAD();
+    const int i = g.nG + (int)(blockIdx.x % (g.Nip + 1)), pole = (int)(blockIdx.x / (g.Nip + 1));
+    const int j = pole == 0 ? g.nG : g.nG + g.Njp - 1, ks = g.nG, Np = g.Nkp;
     const size_t nf = g.nf;
-    const int ks = g.nG, ke = g.nG + g.Nkp - 1;
-    double s = 0.0;
-    for (int k = ks; k <= ke; k++) s = s + mag[2 * nf + fi(g, i, j, k)];
-    const double avg = s / g.Nkp;
-    for (int k = ks; k <= ke + 1; k++) {
-        const int f = 2 * nf + fi(g, i, j, k);
-        mag[f] = mag[f] - 1.0 * avg;
+    for (int n = RING_T0; n < Np; n += RING_NT) col[n] = mag[2 * nf + fi(g, i, j, ks + n)];
+    __syncthreads();
+    if (RING_T0 == 0) {
+        double s = 0.0;
+#pragma unroll 8
+        for (int n = 0; n < Np; n++) s = s + col[n];
+        mean = s / Np;
+    }
+    __syncthreads();
+    const double avg = mean;
+    for (int n = RING_T0; n <= Np; n += RING_NT) {
+        const int f = 2 * nf + fi(g, i, j, ks + n);
+        mag[f] = (n < Np ? col[n] : mag[f]) - 1.0 * avg;
     }
 }
 
@@ -455,4 +486,58 @@ __global__ void __launch_bounds__(256) k_ring_gas(GVT<R> g, MP mp, RingP rp, con
         Gas[nc + cidx] = R(q[1]); Gas[2 * nc + cidx] = R(q[2]); Gas[3 * nc + cidx] = R(q[3]);
         Gas[4 * nc + cidx] = R(KinE + P / gm1);
     }
+}
+
+// The ring band: cell rows j in [js-1, js+nring+1] and [je-nring-1, je+1]
+// (nb = nring + 3 rows each), the cells whose faces or Gas the ring average
+// can change, with the axis ghost row.  Row r of 2*nb -> j.
+__host__ __device__ __forceinline__ int ring_band_row(int nG, int Njp, int nb, int r) {
+    return r < nb ? nG - 1 + r : nG + Njp - (r - nb);
+}
+
+// bFlux2Fld on the ring band (every other cell's faces are unchanged since
+// k_update wrote its Bxyz); thread = (i, k, band row) over all i, k
+template <typename R>
+__global__ void __launch_bounds__(256) k_b2f_rows(GVT<R> g, const DClock *__restrict__ clk, int nb,
+                                                  const double *__restrict__ mag, R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.Ni * g.Nk * 2 * nb || clk->halt) return;
+    const int i = (int)(idx % g.Ni);
+    const long r = idx / g.Ni;
+    const int k = (int)(r % g.Nk), j = ring_band_row(g.nG, g.Njp, nb, (int)(r / g.Nk));
+    double b[3];
+    cell_bxyz(g, mag, i, j, k, b);
+    const int c = ci(g, i, j, k);
+#pragma unroll
+    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + c] = R(b[x]);
+}
+
+// CellDT block minima over the DT box cells of the ring band, after the ring
+// average (k_update leaves these rows out of its minima); thread = (i, k, band
+// row) over i in is-1..ie+1, k in ks-1..ke+1
+template <int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_celldt_rows(GVT<R> g, MP m, BGV bg, const DClock *__restrict__ clk, int nb,
+                                                     const R *__restrict__ Gas, const R *__restrict__ B,
+                                                     double *__restrict__ partial) {
+    if (clk->halt) return;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double dtl = GAM_HUGE;
+    if (idx < (long)(g.Nip + 2) * (g.Nkp + 2) * 2 * nb) {
+        const int i = g.nG - 1 + (int)(idx % (g.Nip + 2));
+        const long r = idx / (g.Nip + 2);
+        const int k = g.nG - 1 + (int)(r % (g.Nkp + 2)), j = ring_band_row(g.nG, g.Njp, nb, (int)(r / (g.Nkp + 2)));
+        const int c = ci(g, i, j, k);
+        const size_t nc = g.nc;
+        double U[5], Bv[3];
+#pragma unroll
+        for (int v = 0; v < 5; v++) U[v] = (double)Gas[nc * v + c];
+#pragma unroll
+        for (int x = 0; x < 3; x++) Bv[x] = (double)B[nc * x + c];
+        if (bg.on) {
+#pragma unroll
+            for (int x = 0; x < 3; x++) Bv[x] = Bv[x] + bg.B0[nc * x + c];
+        }
+        dtl = cell_dt<BORIS>(g, m, U, Bv, c);
+    }
+    block_min_store(dtl, partial);
 }

@@ -140,10 +140,12 @@ __global__ void __launch_bounds__(256) k_b2f(GVT<R> g, Tile t, const DClock *__r
     for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = R(b[x]);
 }
 
-// bFlux2Fld on the ghost cells only.  Used after the periodic BCs of a fused
-// 3D step: those BCs rewrite ghost faces only (an active cell's six faces are
-// owned faces), so the active cells' Bxyz from k_update, computed by the same
-// cell_bxyz from the same faces, is already what a full pass would write.
+// bFlux2Fld on the ghost cells only.  Used after the BCs of a fused 3D step:
+// every BC type rewrites ghost faces only (an active cell's six faces are
+// owned faces; a normal component's first ghost face is past the owned upper
+// face), so the active cells' Bxyz from k_update (or, on the ring rows, from
+// the ring average), computed by the same cell_bxyz from the same faces, is
+// already what a full pass would write.
 // Shell order: the k ghost planes, then the j ghost rows of the active planes,
 // then the i ghosts of the active rows.
 __host__ __device__ __forceinline__ long b2f_shell_count(int Ni, int Nj, int Nk, int Nip, int Njp, int Nkp) {
@@ -669,7 +671,7 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
                                                 const R *__restrict__ nGas, const R *__restrict__ nB,
                                                 const R *__restrict__ Flx, const double *__restrict__ magNew,
                                                 R *__restrict__ oGas, R *__restrict__ oB,
-                                                double *__restrict__ partial) {
+                                                double *__restrict__ partial, int skip_nb) {
     if (clk->halt) return;
     double dtl = GAM_HUGE;
     int i, j, k;
@@ -722,7 +724,9 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
         for (int v = 0; v < 5; v++) oGas[nc * v + idx] = R(U[v]);
 #pragma unroll
         for (int x = 0; x < 3; x++) oB[nc * x + idx] = R(Bn[x]);
-        if (in_dt_box(g, i, j, k)) {
+        // skip_nb > 0: the ring rows' CFL bound is taken after the ring average (k_celldt_rows)
+        const bool band = skip_nb > 0 && (j < g.nG - 1 + skip_nb || j > g.nG + g.Njp - skip_nb);
+        if (in_dt_box(g, i, j, k) && !band) {
             if (sizeof(R) < sizeof(double)) {   // CFL bound of the stored state
 #pragma unroll
                 for (int v = 0; v < 5; v++) U[v] = (double)R(U[v]);

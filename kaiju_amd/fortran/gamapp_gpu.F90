@@ -41,6 +41,10 @@ contains
         if (any(App%Grid%bcType /= 0)) then
             call gpuCheck(App, gam_set_bcs(App%gpu, int(App%Grid%bcType, c_int)), "gam_set_bcs")
         endif
+        if (App%Model%doRing) then   !> InitRings + RingGridFix (init.F90:562-567,968)
+            call gpuCheck(App, gam_set_ring(App%gpu, int(App%Model%Ring%NumR, c_int), &
+                                            int(App%Model%Ring%Nch, c_int)), "gam_set_ring")
+        endif
     end subroutine CreateGPU
 
     subroutine DestroyGPU(App)

@@ -128,6 +128,14 @@ module gamera_hip_c
             integer(c_int), intent(in) :: types(6)
         end function gam_set_bcs
 
+        !> ring average about the LFM axis: nch(r) chunks in ring r (before gam_set_corners)
+        integer(c_int) function gam_set_ring(h, nring, nch) bind(C, name="gam_set_ring")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), value :: nring
+            integer(c_int), intent(in) :: nch(*)
+        end function gam_set_ring
+
         integer(c_int) function gam_hatch(h) bind(C, name="gam_hatch")
             import :: c_int, c_ptr
             type(c_ptr), value :: h

@@ -19,6 +19,12 @@ module gamtypes_gpu
         enumerator :: IDIR = 1, JDIR, KDIR
     endenum
 
+    !> gamtypes.F90:30-50 Ring_T, the LFM-grid ring average ("lfm" only)
+    type :: Ring_T
+        integer :: NumR = 0            !> rings per pole
+        integer :: Nch(32) = 0         !> chunks in ring 1..NumR
+    end type Ring_T
+
     !> gamtypes.F90:71-146 (the step's knobs; XML/IO members left out)
     type :: Model_T
         character(len=64) :: RunID = "Sim"
@@ -45,6 +51,9 @@ module gamtypes_gpu
         !> FP32-bulk step (FP64 kept for magFlux/CT, the cell update and dt);
         !> not a reference knob, the reference is FP64 throughout (kdefs.F90:14)
         logical :: doFP32Bulk = .false.
+        !> ring average about the LFM axis (Model%doRing, ring/gid = "lfm")
+        logical :: doRing = .false.
+        type(Ring_T) :: Ring
     end type Model_T
 
     !> gamtypes.F90:163-245 (sizes, index ranges and the geometry arrays)
@@ -59,7 +68,8 @@ module gamtypes_gpu
         real(rp), dimension(:,:,:), allocatable :: di, dj, dk, volume
         real(rp), dimension(:,:,:,:), allocatable :: Face, Edge
         !> the reference's Grid%externalBCs, as one type per face INI, OUTI, INJ,
-        !> OUTJ, INK, OUTK: 0 periodic, 1 zeroGrad, 2 zeroExten, 3 cartReflect, 4 SphIn/SphOut
+        !> OUTJ, INK, OUTK: 0 periodic, 1 zeroGrad, 2 zeroExten, 3 cartReflect, 4 SphIn/SphOut,
+        !> 5 lfmIn/lfmOut (j faces)
         integer :: bcType(6) = 0
     end type Grid_T
 
