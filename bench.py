@@ -286,12 +286,17 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of oracle stepping")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
-    ap.add_argument("--no-msph", action="store_true", help="skip the magnetosphere-like leg")
+    ap.add_argument("--no-msph", action="store_true", help="skip the magnetosphere leg")
+    ap.add_argument("--msph-only", action="store_true",
+                    help="run only the magnetosphere leg and print its JSON (profiling; not the bench line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.msph_only:
+        print(json.dumps(msph_leg()))
+        return
 
     app, spec, wall, tim, divb = gpu_run(args.n, args.steps, args.warmup)
     zones = spec.zones
