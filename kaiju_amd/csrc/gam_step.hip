@@ -27,6 +27,16 @@
 #include "gam_common.h"
 
 // ----------------------------------------------------------------- helpers
+// Streaming (non-temporal) store, used where it measured faster: the Mom2Face
+// and update outputs, which later stages read back from HBM anyway.
+template <typename T> __device__ __forceinline__ void st_nt(T *p, T v) {
+#ifdef GAM_EMU
+    *p = v;
+#else
+    __builtin_nontemporal_store(v, p);
+#endif
+}
+
 template <typename R> __device__ __forceinline__ void decode_f(const GVT<R> &g, int idx, int &i, int &j, int &k) {
     int nx = g.Ni + 1, ny = g.Nj + 1;
     i = idx % nx; int r = idx / nx; j = r % ny; k = r / ny;
@@ -222,17 +232,17 @@ __global__ void __launch_bounds__(256) k_vf(GVT<R> g, Tile t, const DClock *__re
     // dT1 = I (used by K edges): i in is..ie+1   (fields.F90:161-175)
     if (i >= a0 && i <= a0 + g.Nip) {
         mom2face(g, VdV, idx, 0, v);
-        Vf[nc * 0 + idx] = v[0]; Vf[nc * 1 + idx] = v[1]; Vf[nc * 2 + idx] = v[2];
+        st_nt(&Vf[nc * 0 + idx], v[0]); st_nt(&Vf[nc * 1 + idx], v[1]); st_nt(&Vf[nc * 2 + idx], v[2]);
     }
     // dT1 = J (used by I edges): j in js..je+1   (fields.F90:119-133)
     if (j >= a0 && j <= a0 + g.Njp) {
         mom2face(g, VdV, idx, 1, v);
-        Vf[nc * 3 + idx] = v[0]; Vf[nc * 4 + idx] = v[1]; Vf[nc * 5 + idx] = v[2];
+        st_nt(&Vf[nc * 3 + idx], v[0]); st_nt(&Vf[nc * 4 + idx], v[1]); st_nt(&Vf[nc * 5 + idx], v[2]);
     }
     // dT1 = K (used by J edges): k in ks..ke+1   (fields.F90:140-154)
     if (k >= a0 && k <= a0 + g.Nkp) {
         mom2face(g, VdV, idx, 2, v);
-        Vf[nc * 6 + idx] = v[0]; Vf[nc * 7 + idx] = v[1]; Vf[nc * 8 + idx] = v[2];
+        st_nt(&Vf[nc * 6 + idx], v[0]); st_nt(&Vf[nc * 7 + idx], v[1]); st_nt(&Vf[nc * 8 + idx], v[2]);
     }
 }
 
@@ -721,9 +731,9 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
             }
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) oGas[nc * v + idx] = R(U[v]);
+        for (int v = 0; v < 5; v++) st_nt(&oGas[nc * v + idx], R(U[v]));
 #pragma unroll
-        for (int x = 0; x < 3; x++) oB[nc * x + idx] = R(Bn[x]);
+        for (int x = 0; x < 3; x++) st_nt(&oB[nc * x + idx], R(Bn[x]));
         // skip_nb > 0: the ring rows' CFL bound is taken after the ring average (k_celldt_rows)
         const bool band = skip_nb > 0 && (j < g.nG - 1 + skip_nb || j > g.nG + g.Njp - skip_nb);
         if (in_dt_box(g, i, j, k) && !band) {
