@@ -148,12 +148,18 @@ int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt,
  * :454-603), 3 Cartesian reflecting (cartReflect_*, :607-795), 4 spherical
  * radial (SphIn on INI, SphOut on OUTI, :934-982; i faces only), 5 LFM axis
  * (lfmIn on INJ, lfmOut on OUTJ, :1020-1106; both j faces, k periodic over an
- * even Nkp >= 16: the ghosts across the axis from the cells half a turn away).
+ * even Nkp >= 16: the ghosts across the axis from the cells half a turn away),
+ * 6 solar wind (WindBC on OUTI, apps/wind.F90:206-374, with the constant wind of
+ * gam_set_wind; k periodic; the inner i face keeps its own non-periodic BC). 
  * An axis is periodic on both faces or on neither; 2.5D blocks keep k periodic.
  * Default: all periodic.  The reference's BC objects (Grid%externalBCs) map
  * onto these. */
 int  gam_set_bcs(gam_handle *h, const int *type);
 int  gam_get_bcs(gam_handle *h, int *type);
+/* The solar wind (density, pressure, velocity, field; code units) of the
+ * GAM_BC_WIND boundary, constant in time: the reference's WindBC reading a
+ * constant time series (GetWindAt, apps/wind.F90:160-177).  Before gam_set_bcs. */
+int  gam_set_wind(gam_handle *h, double rho, double p, const double *v, const double *b);
 
 /* --- ring average about the LFM axis (Model%doRing, ring/gid = "lfm") ----- */
 /* Replaces InitRings/RingGridFix (ringutils.F90:37-390) and turns on, inside

@@ -59,6 +59,7 @@ struct gam_handle {
     double *ringSE = nullptr;          // its smoothing EMF (G x 3, zero off the ring rows)
     double *ringTab = nullptr;         // cos, sin of the ring angles phi_n
     int fused_np = 0;                  // block minima the fused step's CalcDT reduces
+    WindP wind = {};                   // the solar wind of a GAM_BC_WIND outer i face (gam_set_wind)
     double *ctG = nullptr, *ctM = nullptr, *crG = nullptr, *crM = nullptr;   // correction training buffers
     BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
     double *partial = nullptr;
@@ -991,6 +992,21 @@ static void launch_bc(gam_handle *h, bool withB) {
                                P<R>(h->Gas[S]), h->mag[S], Bw);
             continue;
         }
+        if (ax == 0 && h->bct[1] == GAM_BC_WIND) {   // the inner face's BC, then WindBC (apps/wind.F90:206-374)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1((long)g.nG * g.Nj * g.Nk), 256, 0, h->stream, g, h->clk, 0,
+                               h->bct[0], (int)withB, (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF],
+                               P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_wind<R>), grid1((long)(g.Nj + 1) * (g.Nk + 1) * g.nG), 256, 0, h->stream, g,
+                               h->mp, h->wind, h->clk, (const double *)h->grid[G_XYZCC], (const double *)h->grid[G_XFC],
+                               (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF], P<R>(h->Gas[S]), h->mag[S],
+                               P<R>(h->B[S]));
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wind_bxyz<R>), grid1((long)g.Nj * g.Nk * g.nG), 256, 0, h->stream, g, h->clk,
+                               h->ring.on, (const double *)h->mag[S], P<R>(h->B[S]));
+            if (h->ring.on)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wind_bfix<R>), grid1((long)g.nG * g.Nk * g.nG * 2), 256, 0, h->stream, g,
+                                   h->clk, P<R>(h->B[S]));
+            continue;
+        }
         if (h->bct[2 * ax] == GAM_BC_LFM) {   // lfmIn then lfmOut (bcs.F90:1020-1106)
             for (int side = 0; side < 2; side++)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_lfm<R>), grid1((long)(g.Ni + 1) * (g.Nk + 1) * g.nG), 256, 0,
@@ -1009,8 +1025,10 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     if (!h || !type) return -1;
     for (int ax = 0; ax < 3; ax++) {
         const int a = type[2 * ax], b = type[2 * ax + 1];
-        if (a < 0 || a > GAM_BC_LFM || b < 0 || b > GAM_BC_LFM)
+        if (a < 0 || a > GAM_BC_WIND || b < 0 || b > GAM_BC_WIND)
             return fail(h, -1, "gam_set_bcs: unknown boundary type");
+        if (a == GAM_BC_WIND || (b == GAM_BC_WIND && ax != 0))
+            return fail(h, -1, "gam_set_bcs: the solar-wind BC is the outer i face's BC");
         if ((a == GAM_BC_LFM || b == GAM_BC_LFM) && (ax != 1 || a != b))
             return fail(h, -1, "gam_set_bcs: the LFM axis BCs are the two j faces' BC");
         if (ax > 0 && (a == GAM_BC_SPHERE || b == GAM_BC_SPHERE))
@@ -1022,6 +1040,8 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
         return fail(h, -1, "gam_set_bcs: 2.5D blocks keep the k boundaries periodic");
     if (type[2] == GAM_BC_LFM && (type[4] != GAM_BC_PERIODIC || h->gv.Nkp < 16 || (h->gv.Nkp & 1)))
         return fail(h, -1, "gam_set_bcs: the LFM axis BCs need k periodic over an even Nkp >= 16");
+    if (type[1] == GAM_BC_WIND && (!h->wind.on || type[4] != GAM_BC_PERIODIC || type[0] == GAM_BC_PERIODIC))
+        return fail(h, -1, "gam_set_bcs: the solar-wind BC needs gam_set_wind first, k periodic and a non-periodic i axis");
     if (h->corr && !(type[0] == 0 && type[2] == 0 && type[4] == 0))
         return fail(h, -1, "gam_set_bcs: a learned correction is set (periodic boundaries only)");
     int rc = gam_sync(h);
@@ -1033,6 +1053,18 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
 extern "C" int gam_get_bcs(gam_handle *h, int *type) {
     if (!h || !type) return -1;
     for (int f = 0; f < 6; f++) type[f] = h->bct[f];
+    return 0;
+}
+
+extern "C" int gam_set_wind(gam_handle *h, double rho, double p, const double *v, const double *b) {
+    if (!h || !v || !b) return -1;
+    if (!(rho > 0) || !(p > 0) || !(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0))
+        return fail(h, -1, "gam_set_wind: density, pressure and speed must be positive");
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    h->wind.rho = rho; h->wind.p = p;
+    for (int x = 0; x < 3; x++) { h->wind.v[x] = v[x]; h->wind.b[x] = b[x]; }
+    h->wind.on = 1;
     return 0;
 }
 

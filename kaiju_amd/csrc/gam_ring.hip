@@ -541,3 +541,140 @@ __global__ void __launch_bounds__(256) k_celldt_rows(GVT<R> g, MP m, BGV bg, con
     }
     block_min_store(dtl, partial);
 }
+
+// ----------------------------------------------------------------- solar-wind BC
+// apps/wind.F90:206-374 WindBC on the outer i face, single fluid, with a wind
+// constant in time (the reference interpolates a time series; a constant one
+// gives this).  jmap: the ghost Bxyz maps j across the axis (ring grids).
+struct WindP {
+    double rho, p, v[3], b[3];
+    int on;
+};
+
+// math.F90:97-111 RampDown
+__device__ __forceinline__ double wind_ramp_down(double r, double rC, double lC) {
+    if (r <= rC) return 1.0;
+    if (r >= rC + lC) return 0.0;
+    const double rScl = (r - rC) / lC;
+    const double eArg = -1.0 / (1.0 - rScl * rScl);
+    return exp(eArg + 1.0);
+}
+
+// apps/wind.F90:180-203 wgtWind
+__device__ __forceinline__ double wind_wgt(const WindP &w, const double *xyz) {
+    double nH[3], vH[3];
+    normvec(xyz, nH);
+    normvec(w.v, vH);
+    const double d = -vH[0] * nH[0] + -vH[1] * nH[1] + -vH[2] * nH[2];
+    const double dTh = acos(d) * 180.0 / GAM_PI;
+    return wind_ramp_down(dTh, 90.0, 90.0);
+}
+
+// gamutils.F90:324-360 CellPress2Cs + Con2Fast
+__device__ __forceinline__ double wind_con2fast(const MP &m, const double *U, const double *B) {
+    const double rho = fmax(U[0], m.dFloor);
+    const double Vx = U[1] / rho, Vy = U[2] / rho, Vz = U[3] / rho;
+    const double KinE = 0.5 * rho * (sq(Vx) + sq(Vy) + sq(Vz));
+    const double P = fmax((m.gamma - 1) * (U[4] - KinE), m.pFloor);
+    const double Cs = sqrt(m.gamma * P / rho);
+    const double MagB = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    double Va = MagB / sqrt(U[0]);
+    if (m.doBoris) Va = m.Ca * Va / sqrt(sq(m.Ca) + sq(Va));
+    return sqrt(sq(Cs) + sq(Va));
+}
+
+// Ghost cells (weight at the cell centre: the wind, or the last cell's state
+// pushed outward at >= MinMach = 1 fast speed) and ghost face fluxes (the wind
+// field's flux mixed with the last cell's); thread = (j, k, n) over j <= Nj,
+// k <= Nk.  Faces that bound no cell are left alone.  FP64 arithmetic.
+template <typename R>
+__global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const DClock *__restrict__ clk,
+                                                 const double *__restrict__ xyzcc, const double *__restrict__ xfc,
+                                                 const double *__restrict__ face64, const double *__restrict__ Tf64,
+                                                 R *__restrict__ Gas, double *__restrict__ mag, const R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(g.Nj + 1) * (g.Nk + 1) * g.nG || clk->halt) return;
+    const int j = (int)(idx % (g.Nj + 1));
+    const long r = idx / (g.Nj + 1);
+    const int k = (int)(r % (g.Nk + 1)), nl = 1 + (int)(r / (g.Nk + 1));
+    const int ip = g.nG + g.Nip - 1, ig = ip + nl;
+    const size_t nc = g.nc, nf = g.nf;
+    const double MinMach = 1.0;
+    if (k < g.Nk && j < g.Nj) {
+        const int c = ci(g, ig, j, k), cp = ci(g, ip, j, k);
+        const double xcc[3] = {xyzcc[c], xyzcc[nc + c], xyzcc[2 * nc + c]};
+        const double wSW = wind_wgt(w, xcc), wMHD = 1.0 - wSW;
+        double gSW[5] = {0, 0, 0, 0, 0}, gIN[5] = {0, 0, 0, 0, 0};
+        if (wSW > GAM_TINY) {
+            gSW[0] = w.rho; gSW[4] = w.p;
+            for (int x = 0; x < 3; x++) gSW[1 + x] = w.v[x];
+        }
+        if (wMHD > GAM_TINY) {
+            double U[5], Bin[3];
+            for (int v = 0; v < 5; v++) U[v] = (double)Gas[nc * v + cp];
+            for (int x = 0; x < 3; x++) Bin[x] = (double)B[nc * x + cp];
+            cell_c2p(m, U, gIN);
+            const double Cx = wind_con2fast(m, U, Bin);
+            const int fn = fi(g, ip + 1, j, k);
+            const double nH[3] = {Tf64[nf * NORMX + fn], Tf64[nf * NORMY + fn], Tf64[nf * NORMZ + fn]};
+            double V[3] = {gIN[1], gIN[2], gIN[3]};
+            const double vn = V[0] * nH[0] + V[1] * nH[1] + V[2] * nH[2];
+            if (vn < MinMach * Cx)
+                for (int x = 0; x < 3; x++) V[x] = V[x] - vn * nH[x] + MinMach * Cx * nH[x];
+            for (int x = 0; x < 3; x++) gIN[1 + x] = V[x];
+        }
+        double gW[5], U[5];
+        for (int v = 0; v < 5; v++) gW[v] = wSW * gSW[v] + wMHD * gIN[v];
+        cell_p2c(m, gW, U);
+        for (int v = 0; v < 5; v++) Gas[nc * v + c] = R(U[v]);
+    }
+    for (int d = 0; d < 3; d++) {
+        if ((d != 1 && j >= g.Nj) || (d != 2 && k >= g.Nk)) continue;
+        const int f = fi(g, d == 0 ? ig + 1 : ig, j, k), fin = fi(g, d == 0 ? ip + 1 : ip, j, k);
+        const double x[3] = {xfc[nf * (0 + 3 * d) + f], xfc[nf * (1 + 3 * d) + f], xfc[nf * (2 + 3 * d) + f]};
+        const double wSW = wind_wgt(w, x);
+        const double bn = Tf64[nf * (NORMX + 9 * d) + f] * w.b[0] + Tf64[nf * (NORMY + 9 * d) + f] * w.b[1] +
+                          Tf64[nf * (NORMZ + 9 * d) + f] * w.b[2];
+        const double swFlx = face64[nf * d + f] * bn;
+        mag[nf * d + f] = wSW * swFlx + (1 - wSW) * mag[nf * d + fin];
+    }
+}
+
+// WindBC's ghost Bxyz: bFlux2Fld of the cell the map names (k periodic; j
+// across the axis when jmap); thread = (j, k, n) over the ghost cells i > ie
+template <typename R>
+__global__ void __launch_bounds__(256) k_wind_bxyz(GVT<R> g, const DClock *__restrict__ clk, int jmap,
+                                                   const double *__restrict__ mag, R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.Nj * g.Nk * g.nG || clk->halt) return;
+    const int j = (int)(idx % g.Nj);
+    const long r = idx / g.Nj;
+    const int k = (int)(r % g.Nk), ig = g.nG + g.Nip - 1 + 1 + (int)(r / g.Nk);
+    int jp = j, kp;
+    if (jmap) lfm_cc(g.nG, g.Njp, g.Nkp, j, k, jp, kp);
+    else kp = k < g.nG ? k + g.Nkp : (k > g.nG + g.Nkp - 1 ? k - g.Nkp : k);
+    double b[3];
+    cell_bxyz(g, mag, ig, jp, kp, b);
+    const int c = ci(g, ig, j, k);
+#pragma unroll
+    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + c] = R(b[x]);
+}
+
+// then, on ring grids, the j ghosts of those i ghost cells from their
+// conjugates; thread = (i > ie, k, n, pole)
+template <typename R>
+__global__ void __launch_bounds__(256) k_wind_bfix(GVT<R> g, const DClock *__restrict__ clk, R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.nG * g.Nk * g.nG * 2 || clk->halt) return;
+    const int i = g.nG + g.Nip + (int)(idx % g.nG);
+    long r = idx / g.nG;
+    const int k = (int)(r % g.Nk);
+    r /= g.Nk;
+    const int nl = 1 + (int)(r % g.nG), pole = (int)(r / g.nG);
+    const int jg = pole == 0 ? g.nG - nl : g.nG + g.Njp - 1 + nl;
+    int jp, kp;
+    lfm_cc(g.nG, g.Njp, g.Nkp, jg, k, jp, kp);
+    const int c = ci(g, i, jg, k), cs = ci(g, i, jp, kp);
+#pragma unroll
+    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + c] = B[(size_t)g.nc * x + cs];
+}

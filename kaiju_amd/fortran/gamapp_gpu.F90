@@ -38,6 +38,11 @@ contains
             write(error_unit, *) 'gam_create failed, rc = ', rc
             error stop 1
         endif
+        if (any(App%Grid%bcType == 6)) then   !> WindBC (apps/wind.F90) with a constant wind
+            call gpuCheck(App, gam_set_wind(App%gpu, real(App%Grid%wind(1), c_double), real(App%Grid%wind(2), c_double), &
+                                            real(App%Grid%wind(3:5), c_double), real(App%Grid%wind(6:8), c_double)), &
+                          "gam_set_wind")
+        endif
         if (any(App%Grid%bcType /= 0)) then
             call gpuCheck(App, gam_set_bcs(App%gpu, int(App%Grid%bcType, c_int)), "gam_set_bcs")
         endif

@@ -128,6 +128,14 @@ module gamera_hip_c
             integer(c_int), intent(in) :: types(6)
         end function gam_set_bcs
 
+        !> the constant solar wind of a WindBC outer i face (before gam_set_bcs)
+        integer(c_int) function gam_set_wind(h, rho, p, v, b) bind(C, name="gam_set_wind")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            real(c_double), value :: rho, p
+            real(c_double), intent(in) :: v(3), b(3)
+        end function gam_set_wind
+
         !> ring average about the LFM axis: nch(r) chunks in ring r (before gam_set_corners)
         integer(c_int) function gam_set_ring(h, nring, nch) bind(C, name="gam_set_ring")
             import :: c_int, c_ptr

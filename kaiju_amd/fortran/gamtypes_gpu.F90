@@ -69,8 +69,10 @@ module gamtypes_gpu
         real(rp), dimension(:,:,:,:), allocatable :: Face, Edge
         !> the reference's Grid%externalBCs, as one type per face INI, OUTI, INJ,
         !> OUTJ, INK, OUTK: 0 periodic, 1 zeroGrad, 2 zeroExten, 3 cartReflect, 4 SphIn/SphOut,
-        !> 5 lfmIn/lfmOut (j faces)
+        !> 5 lfmIn/lfmOut (j faces), 6 WindBC (outer i face)
         integer :: bcType(6) = 0
+        !> the WindBC's solar wind, constant in time: rho, P, V(3), B(3) (code units)
+        real(rp) :: wind(8) = 0.0_rp
     end type Grid_T
 
     !> gamtypes.F90:248-264

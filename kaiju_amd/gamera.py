@@ -153,6 +153,7 @@ def lib():
         "gam_set_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
         "gam_get_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
         "gam_set_ring": (C.c_int, [H, C.c_int, C.POINTER(C.c_int)]),
+        "gam_set_wind": (C.c_int, [H, C.c_double, C.c_double, _dp, _dp]),
         "gam_get_ring": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "gam_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
         "gam_device_free": (C.c_int, [C.c_void_p]),
@@ -261,6 +262,8 @@ class GamApp:
         if rc != 0 or not h.value:
             raise HipStepError(f"gam_create failed rc={rc}")
         self.h = h
+        if spec.wind is not None:
+            self.set_wind(*spec.wind)
         if any(b != "periodic" for b in spec.bcs):
             self.set_bcs(spec.bcs)
         if spec.ring:
@@ -471,6 +474,11 @@ class GamApp:
         t = (C.c_int * 6)()
         self._chk(lib().gam_get_bcs(self.h, t), "gam_get_bcs")
         return tuple(configs.BC_TYPES[x] for x in t)
+
+    def set_wind(self, rho, p, v, b):
+        """The constant solar wind of a "wind" outer i boundary (before set_bcs)."""
+        vv, bb = (C.c_double * 3)(*v), (C.c_double * 3)(*b)
+        self._chk(lib().gam_set_wind(self.h, rho, p, vv, bb), "gam_set_wind")
 
     def set_ring(self, chunks):
         """Ring average about the LFM axis with chunks[r] chunks in ring r+1

@@ -30,7 +30,8 @@ class OrcModel(C.Structure):
                [(n, C.c_int) for n in
                 ("ts", "rmeth", "doLFMLim", "doBoris", "doMHD", "do25D", "doCPR",
                  "fixedTimestep", "status", "doBackground")] + \
-               [("bc", C.c_int * 6), ("ncpr", C.c_int)]
+               [("bc", C.c_int * 6), ("ncpr", C.c_int)] + \
+               [("wind_rho", C.c_double), ("wind_p", C.c_double), ("wind_v", C.c_double * 3), ("wind_b", C.c_double * 3)]
 
 
 class OrcGrid(C.Structure):
@@ -104,6 +105,7 @@ def lib():
             "orc_reconstruct_ring": (None, [_dp, C.c_int, C.c_int]),
             "orc_ppm_intlr": (None, [C.c_double] * 5 + [_dp, _dp]),
             "orc_lfm_map": (None, [G, C.c_int, C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3),
+            "orc_wgt_wind": (C.c_double, [M, _dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -199,6 +201,12 @@ class OracleCase:
         from kaiju_amd.configs import BC_TYPES
         for f, b in enumerate(getattr(spec, "bcs", ("periodic",) * 6)):
             self.m.bc[f] = BC_TYPES.index(b)
+        wind = getattr(spec, "wind", None)
+        if wind is not None:
+            rho, p, v, b = wind
+            self.m.wind_rho, self.m.wind_p = rho, p
+            for x in range(3):
+                self.m.wind_v[x], self.m.wind_b[x] = v[x], b[x]
         self.state = OracleState(G, F)
         self.ostate = OracleState(G, F)
         self.solver = L.orc_solver_new(C.byref(g))

@@ -1276,8 +1276,11 @@ double orc_calc_dt(orc_model *m, const orc_grid *g, orc_state *s) {
  * Loops over n = 1..Ng and the ghost-inclusive cell ranges of the other two
  * axes, as the reference's do loops; the same assignments, including the
  * reference's choices of source component and of which routines write Bxyz. */
-static void bc_face(const orc_grid *g, orc_state *s, int face, int type) {
+static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s);
+
+static void bc_face(const orc_model *m, const orc_grid *g, orc_state *s, int face, int type) {
     if (type == ORC_BC_LFM) { bc_lfm(g, s, face & 1); return; }
+    if (type == ORC_BC_WIND) { bc_wind(m, g, s); return; }
     const size_t nc = g->nc, nf = g->nf;
     const int nG = g->nG, ax = face / 2, out = face & 1;
     double *Gas = s->Gas, *B = s->Bxyz, *mag = s->magFlux;
@@ -1349,6 +1352,124 @@ static void bc_face(const orc_grid *g, orc_state *s, int face, int type) {
             }
 }
 
+
+/* ----------------------------------------------------- solar-wind BC */
+/* math.F90:97-111 RampDown */
+static double ramp_down(double r, double rC, double lC) {
+    if (r <= rC) return 1.0;
+    if (r >= rC + lC) return 0.0;
+    const double rScl = (r - rC) / lC;
+    const double eArg = -1.0 / (1.0 - rScl * rScl);
+    return exp(eArg + 1.0);
+}
+
+/* apps/wind.F90:180-203 wgtWind: 1 where the radial direction faces the wind
+ * (within 90 degrees of -v), ramping to 0 on the downstream axis */
+double orc_wgt_wind(const orc_model *m, const double *xyz) {
+    double nH[3], vH[3];
+    normvec(xyz, nH);
+    normvec(m->wind_v, vH);
+    const double d = -vH[0] * nH[0] + -vH[1] * nH[1] + -vH[2] * nH[2];
+    const double dTh = acos(d) * 180.0 / PI;
+    return ramp_down(dTh, 90.0, 90.0);
+}
+
+/* gamutils.F90:324-360 CellPress2Cs + Con2Fast */
+static double con2fast(const orc_model *m, const double *U, const double *B) {
+    const double rho = fmax(U[0], m->dFloor);
+    const double Vx = U[1] / rho, Vy = U[2] / rho, Vz = U[3] / rho;
+    const double KinE = 0.5 * rho * (sq(Vx) + sq(Vy) + sq(Vz));
+    const double P = fmax((m->gamma - 1) * (U[4] - KinE), m->pFloor);
+    const double Cs = sqrt(m->gamma * P / rho);
+    const double MagB = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+    double Va = MagB / sqrt(U[0]);
+    if (m->doBoris) Va = m->Ca * Va / sqrt(sq(m->Ca) + sq(Va));
+    return sqrt(sq(Cs) + sq(Va));
+}
+
+/* apps/wind.F90:206-374 WindBC (outer i face, single fluid, constant wind):
+ * ghost cells mix the wind (weight wgtWind at the cell centre) with the last
+ * cell's state forced outward at >= MinMach = 1 fast speed; ghost face fluxes
+ * mix the wind field's flux with the last cell's; ghost Bxyz from the faces
+ * of the cell the ring map names (ring grids: across the axis, then the j
+ * ghosts' Bxyz from their conjugates).  Faces that bound no cell (an I face
+ * past jeg or keg, a J face past keg, a K face past jeg) are left alone. */
+static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s) {
+    const size_t nc = g->nc, nf = g->nf;
+    const double MinMach = 1.0;
+    const double *Bw = m->wind_b;
+    const int ip = g->ie;
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int n = 1; n <= g->nG; n++) {
+                const int ig = g->ie + n;
+                if (k <= g->keg && j <= g->jeg && ig <= g->ieg) {
+                    const size_t c = CI(g, ig, j, k), cp = CI(g, ip, j, k);
+                    double xcc[3] = {g->xyzcc[c], g->xyzcc[c + nc], g->xyzcc[c + 2 * nc]};
+                    const double wSW = orc_wgt_wind(m, xcc), wMHD = 1.0 - wSW;
+                    double gSW[5] = {0, 0, 0, 0, 0}, gIN[5] = {0, 0, 0, 0, 0};
+                    if (wSW > TINY) {
+                        gSW[0] = m->wind_rho; gSW[4] = m->wind_p;
+                        for (int x = 0; x < 3; x++) gSW[1 + x] = m->wind_v[x];
+                    }
+                    if (wMHD > TINY) {
+                        double U[5], Bin[3];
+                        for (int v = 0; v < 5; v++) U[v] = s->Gas[cp + nc * v];
+                        for (int x = 0; x < 3; x++) Bin[x] = s->Bxyz[cp + nc * x];
+                        orc_cell_c2p(m, U, gIN);
+                        const double Cx = con2fast(m, U, Bin);
+                        const size_t fn = FI(g, ip + 1, j, k);
+                        const double nH[3] = {g->Tf[fn + nf * (NORMX + 9 * ID)], g->Tf[fn + nf * (NORMY + 9 * ID)],
+                                              g->Tf[fn + nf * (NORMZ + 9 * ID)]};
+                        double V[3] = {gIN[1], gIN[2], gIN[3]};
+                        const double vn = V[0] * nH[0] + V[1] * nH[1] + V[2] * nH[2];
+                        if (vn < MinMach * Cx)
+                            for (int x = 0; x < 3; x++) V[x] = V[x] - vn * nH[x] + MinMach * Cx * nH[x];
+                        for (int x = 0; x < 3; x++) gIN[1 + x] = V[x];
+                    }
+                    double gW[5], U[5];
+                    for (int v = 0; v < 5; v++) gW[v] = wSW * gSW[v] + wMHD * gIN[v];
+                    orc_cell_p2c(m, gW, U);
+                    for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = U[v];
+                }
+                for (int d = 0; d < 3; d++) {
+                    const int fi_ = d == ID ? ig + 1 : ig;
+                    if ((d != JD && j > g->jeg) || (d != KD && k > g->keg)) continue;
+                    const size_t f = FI(g, fi_, j, k), fin = FI(g, d == ID ? ip + 1 : ip, j, k);
+                    const double x[3] = {g->xfc[f + nf * (0 + 3 * d)], g->xfc[f + nf * (1 + 3 * d)], g->xfc[f + nf * (2 + 3 * d)]};
+                    const double wSW = orc_wgt_wind(m, x);
+                    const double bn = g->Tf[f + nf * (NORMX + 9 * d)] * Bw[0] + g->Tf[f + nf * (NORMY + 9 * d)] * Bw[1] +
+                                      g->Tf[f + nf * (NORMZ + 9 * d)] * Bw[2];
+                    const double swFlx = g->face[f + nf * d] * bn;
+                    const double inFlx = s->magFlux[fin + nf * d];
+                    s->magFlux[f + nf * d] = wSW * swFlx + (1 - wSW) * inFlx;
+                }
+            }
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int j = g->jsg; j <= g->jeg; j++)
+            for (int n = 1; n <= g->nG; n++) {
+                const int ig = g->ie + n;
+                int ip_, jp = j, kp = k;
+                if (g->ring) lfm_cc(g, ig, j, k, &ip_, &jp, &kp);
+                else kp = k < g->ks ? k + g->Nkp : (k > g->ke ? k - g->Nkp : k);
+                double B[3];
+                cell_bxyz(g, s->magFlux, ig, jp, kp, B);
+                const size_t c = CI(g, ig, j, k);
+                for (int x = 0; x < 3; x++) s->Bxyz[c + nc * x] = B[x];
+            }
+    if (!g->ring) return;
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int i = g->ie + 1; i <= g->ieg; i++)
+            for (int n = 1; n <= g->nG; n++)
+                for (int p = 0; p < 2; p++) {
+                    const int jg = p == 0 ? g->js - n : g->je + n;
+                    int ip_, jp, kp;
+                    lfm_cc(g, i, jg, k, &ip_, &jp, &kp);
+                    const size_t c = CI(g, i, jg, k), cs = CI(g, i, jp, kp);
+                    for (int x = 0; x < 3; x++) s->Bxyz[c + nc * x] = s->Bxyz[cs + nc * x];
+                }
+}
+
 /* ----------------------------------------------------- BCs */
 void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
     /* bcs.F90:279-450 periodic_{i,o}bc{I,J,K}, or the face's non-periodic BC
@@ -1363,7 +1484,7 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
         for (int v = 0; v < 3; v++) B[a_ + nc * v] = B[b_ + nc * v]; } while (0)
 #define CPF(di, dj, dk, si, sj, sk, d) do { mag[FI(g, di, dj, dk) + nf * (d)] = mag[FI(g, si, sj, sk) + nf * (d)]; } while (0)
     /* I */
-    if (m->bc[0] != ORC_BC_PERIODIC) bc_face(g, s, 0, m->bc[0]);
+    if (m->bc[0] != ORC_BC_PERIODIC) bc_face(m, g, s, 0, m->bc[0]);
     else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int n = 1; n <= nG; n++) {
@@ -1371,7 +1492,7 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
                 if (ISCC(ig, j, k)) CPC(ig, j, k, is_, j, k);
                 for (int d = 0; d < 3; d++) CPF(ig, j, k, is_, j, k, d);
             }
-    if (m->bc[1] != ORC_BC_PERIODIC) bc_face(g, s, 1, m->bc[1]);
+    if (m->bc[1] != ORC_BC_PERIODIC) bc_face(m, g, s, 1, m->bc[1]);
     else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int n = 1; n <= nG; n++) {
@@ -1382,7 +1503,7 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
                 CPF(ig + 1, j, k, g->is + n, j, k, ID);
             }
     /* J */
-    if (m->bc[2] != ORC_BC_PERIODIC) bc_face(g, s, 2, m->bc[2]);
+    if (m->bc[2] != ORC_BC_PERIODIC) bc_face(m, g, s, 2, m->bc[2]);
     else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int i = g->isg; i <= g->ieg + 1; i++)
             for (int n = 1; n <= nG; n++) {
@@ -1390,7 +1511,7 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
                 if (ISCC(i, jg, k)) CPC(i, jg, k, i, js_, k);
                 for (int d = 0; d < 3; d++) CPF(i, jg, k, i, js_, k, d);
             }
-    if (m->bc[3] != ORC_BC_PERIODIC) bc_face(g, s, 3, m->bc[3]);
+    if (m->bc[3] != ORC_BC_PERIODIC) bc_face(m, g, s, 3, m->bc[3]);
     else for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int i = g->isg; i <= g->ieg + 1; i++)
             for (int n = 1; n <= nG; n++) {
@@ -1413,7 +1534,7 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
             }
         }
     } else {
-        if (m->bc[4] != ORC_BC_PERIODIC) bc_face(g, s, 4, m->bc[4]);
+        if (m->bc[4] != ORC_BC_PERIODIC) bc_face(m, g, s, 4, m->bc[4]);
         else for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int i = g->isg; i <= g->ieg + 1; i++)
                 for (int n = 1; n <= nG; n++) {
@@ -1421,7 +1542,7 @@ void orc_enforce_bcs(const orc_model *m, const orc_grid *g, orc_state *s) {
                     if (ISCC(i, j, kg)) CPC(i, j, kg, i, j, ks_);
                     for (int d = 0; d < 3; d++) CPF(i, j, kg, i, j, ks_, d);
                 }
-        if (m->bc[5] != ORC_BC_PERIODIC) bc_face(g, s, 5, m->bc[5]);
+        if (m->bc[5] != ORC_BC_PERIODIC) bc_face(m, g, s, 5, m->bc[5]);
         else for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int i = g->isg; i <= g->ieg + 1; i++)
                 for (int n = 1; n <= nG; n++) {

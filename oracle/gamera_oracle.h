@@ -30,13 +30,17 @@ typedef struct {
     int doBackground;   /* background field B0 split (gamtypes.F90:88) */
     int bc[6];          /* boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK: ORC_BC_* */
     int ncpr;           /* CalcDT CPR repairs done (step.F90:140-142) */
+    /* solar wind of the WindBC (apps/wind.F90), constant in time: density,
+     * pressure, velocity, field (code units) */
+    double wind_rho, wind_p, wind_v[3], wind_b[3];
 } orc_model;
 
 /* boundary conditions (bcs.F90): periodic, zero gradient, zero extension,
  * Cartesian reflecting, spherical radial (SphIn on INI, SphOut on OUTI),
- * through-the-axis LFM pole (lfmIn on INJ, lfmOut on OUTJ) */
+ * through-the-axis LFM pole (lfmIn on INJ, lfmOut on OUTJ), solar wind (WindBC on
+ * OUTI, apps/wind.F90:206-374, with a wind constant in time) */
 enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3, ORC_BC_SPHERE = 4,
-       ORC_BC_LFM = 5 };
+       ORC_BC_LFM = 5, ORC_BC_WIND = 6 };
 #define ORC_MAX_RINGS 32
 
 /* Grid_T subset (reference gamtypes.F90:163-245) */
@@ -130,6 +134,7 @@ void   orc_ring_average(const orc_model *m, const orc_grid *g, orc_state *s, dou
 void   orc_reconstruct_ring(double *rW, int Nc, int Np);
 void   orc_ppm_intlr(double fm2, double fm1, double f, double fp1, double fp2, double *vm, double *vp);
 void   orc_lfm_map(const orc_grid *g, int what, int i, int j, int k, int *ip, int *jp, int *kp);
+double orc_wgt_wind(const orc_model *m, const double *xyz);
 
 /* cell-level helpers exposed for known-answer tests */
 void   orc_cell_c2p(const orc_model *m, const double *con, double *prim);

@@ -321,3 +321,72 @@ def test_axis_block_fp32_bulk():
         assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 2e-4, e
         scale = float(np.abs(oc.state.magFlux).max()) * spec.zones
         assert abs(app.divb() - divb0) <= 1e-12 * scale
+
+
+# ----------------------------------------------------------------- solar-wind BC (CPU)
+def test_wind_weight_known_answers():
+    """wgtWind (apps/wind.F90:180-203) with RampDown (math.F90:97-111): 1 within
+    90 degrees of the upstream direction, exp(1 - 1/(1 - s^2)) beyond, with
+    s = (angle - 90)/90, 0 on the downstream axis."""
+    spec, oc = _oracle_case((12, 12, 32))
+    L = oracle.lib()
+
+    def w(x):
+        return L.orc_wgt_wind(C.byref(oc.m), (C.c_double * 3)(*x))
+
+    assert oc.m.wind_v[0] < 0 and oc.m.wind_v[1] == 0 and oc.m.wind_v[2] == 0   # wind along -x
+    assert w((20.0, 3.0, -1.0)) == 1.0
+    assert w((0.0, 25.0, 0.0)) == 1.0                                  # 90 degrees
+    a = math.radians(135.0)
+    assert math.isclose(w((25 * math.cos(a), 25 * math.sin(a), 0.0)), math.exp(1 - 1 / 0.75), rel_tol=1e-12)
+    assert w((-30.0, 0.0, 0.0)) == 0.0                                 # 180 degrees
+
+
+def test_wind_bc_drives_the_wind_in_and_lets_the_flow_out():
+    """After the BCs: upstream ghost cells hold the wind itself, upstream ghost
+    faces its field's flux; far downstream the ghosts carry the last cell's
+    state leaving at least at its fast speed (checked on the outflow part of
+    the mixed ghosts, weight < 1/2)."""
+    spec, oc = _oracle_case((12, 12, 32))
+    oc.metric()
+    p = spec.ic_params
+    gas, mag = msph.uniform_state(oc.grid["xyz"], p["rho"], p["v"], p["p"], p["b"], spec.model.gamma)
+    gas[..., 1:4] *= -0.5     # an inflowing interior: the outflow rule has work to do
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    oc.hatch_finish()
+    g, m = oc.g, oc.m
+    gam = spec.model.gamma
+    rho, pw, v, b = spec.wind
+    xcc, Tf, face = oc.grid["xyzcc"], oc.grid["Tf"], oc.grid["face"]
+    W = np.array([rho, *v, pw])
+    Uw = np.array([rho, *(rho * np.array(v)), pw / (gam - 1) + 0.5 * rho * np.dot(v, v)])
+    ie = g.ie - g.isg
+    n_up = n_down = 0
+    L = oracle.lib()
+    for jj in range(NG, NG + spec.N[1]):
+        for kk in range(NG, NG + spec.N[2]):
+            for n in range(1, NG + 1):
+                x = xcc[ie + n, jj, kk]
+                wgt = L.orc_wgt_wind(C.byref(m), (C.c_double * 3)(*x))
+                U = oc.state.Gas[ie + n, jj, kk]
+                if wgt == 1.0:
+                    assert np.allclose(U, Uw, rtol=1e-14, atol=0), (jj, kk, n)
+                    n_up += 1
+                elif wgt < 0.5:
+                    # primitives of the ghost = wgt * wind + (1 - wgt) * outflow state
+                    Wg = np.array([U[0], *(U[1:4] / U[0]), (gam - 1) * (U[4] - 0.5 * np.dot(U[1:4], U[1:4]) / U[0])])
+                    Win = (Wg - wgt * W) / (1.0 - wgt)
+                    D = oc.state.Gas[ie, jj, kk]
+                    P = (gam - 1) * (D[4] - 0.5 * np.dot(D[1:4], D[1:4]) / D[0])
+                    cs = math.sqrt(gam * P / D[0])
+                    nh = Tf[ie + 1, jj, kk, 0:3, 0]
+                    assert np.dot(Win[1:4], nh) >= cs * (1 - 1e-9), (jj, kk, n)   # leaves at >= the fast speed
+                    assert math.isclose(Win[0], D[0], rel_tol=1e-12)                # the last cell's density
+                    n_down += 1
+            # upstream I face of the first ghost layer: the wind field's flux
+            xf = oc.grid["xfc"][ie + 2, jj, kk, :, 0]
+            if L.orc_wgt_wind(C.byref(m), (C.c_double * 3)(*xf)) == 1.0:
+                want = face[ie + 2, jj, kk, 0] * np.dot(Tf[ie + 2, jj, kk, 0:3, 0], b)
+                assert math.isclose(oc.state.magFlux[ie + 2, jj, kk, 0], want, rel_tol=1e-13, abs_tol=1e-18)
+    assert n_up > 100 and n_down > 10, (n_up, n_down)
