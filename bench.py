@@ -234,7 +234,11 @@ def roofline(tim: dict, ms_per_step: float, zups: float):
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 avg_launch_ms=round(avg_ms, 4), algo_bytes_per_launch=d["bytes"],
                 step_algo_GBs=round(step_gbs, 1), step_frac=round(step_gbs / HBM_PEAK_GBS, 4),
-                stage_avg_ms={k: round(v["ms"] / v["launches"], 4) for k, v in stages.items()})
+                stage_avg_ms={k: round(v["ms"] / v["launches"], 4) for k, v in stages.items()},
+                # every stage's algorithmic bytes per launch over its average launch time, as a
+                # fraction of the HBM peak (the same measure as frac, for the other kernels)
+                stage_hbm_frac={k: round(v["bytes"] / (v["ms"] / v["launches"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                                for k, v in stages.items() if v["bytes"] > 0 and v["ms"] > 0})
 
 
 def cpu_baseline(n: int, budget_s: float):

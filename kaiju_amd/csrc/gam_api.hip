@@ -236,7 +236,7 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     }
     default_tiles(h);
     h->npart = (int)((G + 255) / 256);
-    h->pcap = std::max(h->npart, tile_blocks(h->tile[ST_UPDATE], g.Ni, g.Nj, g.Nk));
+    h->pcap = std::max(h->npart, tile_blocks(h->tile[ST_UPDATE], g.Ni, g.Nj, cdiv(g.Nk, UPDATE_KC)));
     if (hipMalloc(&h->partial, (size_t)h->pcap * sizeof(double)) != hipSuccess) return cleanup(-3);
     if (hipMalloc(&h->clk, sizeof(DClock)) != hipSuccess) return cleanup(-3);
     if (hipHostMalloc(&h->clk_host, sizeof(DClock), hipHostMallocDefault) != hipSuccess) return cleanup(-3);
@@ -758,8 +758,9 @@ static int celldt(gam_handle *h, const void *Gas, const void *B) {
     return h->prec ? launch_celldt<float>(h, Gas, B) : launch_celldt<double>(h, Gas, B);
 }
 
+// k_update's launch box: (i, j, column of UPDATE_KC cells in k)
 static int update_blocks(const gam_handle *h) {
-    return tile_blocks(h->tile[ST_UPDATE], h->gv.Ni, h->gv.Nj, h->gv.Nk);
+    return tile_blocks(h->tile[ST_UPDATE], h->gv.Ni, h->gv.Nj, cdiv(h->gv.Nk, UPDATE_KC));
 }
 
 // blocks of k_celldt_rows (DT box cells of the ring band), 0 without rings
@@ -854,7 +855,7 @@ static void launch_fluxes(gam_handle *h) {
 template <typename R, typename TM>
 static void launch_b2f(gam_handle *h, int st, const TM *mag, void *B) {
     const GVT<R> &g = view<R>(h);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[st],
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, cdiv(g.Nk, UPDATE_KC)), 256, 0, h->stream, g, h->tile[st],
                        h->clk, mag, P<R>(B));
     if (h->ring.on)   // RingFlux2FieldFix (gridutils.F90:298-300)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g,
@@ -1402,7 +1403,7 @@ extern "C" int gam_set_tile(gam_handle *h, int stage, int ti, int tj, int tk, in
     if (rc) return rc;
     const Tile t = {ti, tj, tk, fast};
     if (stage == ST_UPDATE) {   // one block minimum per update block, then the ring band's
-        rc = ensure_partial(h, tile_blocks(t, h->gv.Ni, h->gv.Nj, h->gv.Nk) + ring_dt_blocks(h));
+        rc = ensure_partial(h, tile_blocks(t, h->gv.Ni, h->gv.Nj, cdiv(h->gv.Nk, UPDATE_KC)) + ring_dt_blocks(h));
         if (rc) return rc;
     }
     h->tile[stage] = t;
