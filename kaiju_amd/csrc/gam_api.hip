@@ -236,7 +236,7 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     }
     default_tiles(h);
     h->npart = (int)((G + 255) / 256);
-    h->pcap = std::max(h->npart, tile_blocks(h->tile[ST_UPDATE], g.Ni, g.Nj, cdiv(g.Nk, UPDATE_KC)));
+    h->pcap = std::max(h->npart, tile_blocks(h->tile[ST_UPDATE], g.Ni, g.Nj, g.Nk));
     if (hipMalloc(&h->partial, (size_t)h->pcap * sizeof(double)) != hipSuccess) return cleanup(-3);
     if (hipMalloc(&h->clk, sizeof(DClock)) != hipSuccess) return cleanup(-3);
     if (hipHostMalloc(&h->clk_host, sizeof(DClock), hipHostMallocDefault) != hipSuccess) return cleanup(-3);
@@ -758,9 +758,10 @@ static int celldt(gam_handle *h, const void *Gas, const void *B) {
     return h->prec ? launch_celldt<float>(h, Gas, B) : launch_celldt<double>(h, Gas, B);
 }
 
-// k_update's launch box: (i, j, column of UPDATE_KC cells in k)
+// k_update's launch box: (i, j, column of UpdateKC cells in k)
+static int update_kc(const gam_handle *h) { return h->prec ? 1 : UPDATE_KC; }
 static int update_blocks(const gam_handle *h) {
-    return tile_blocks(h->tile[ST_UPDATE], h->gv.Ni, h->gv.Nj, cdiv(h->gv.Nk, UPDATE_KC));
+    return tile_blocks(h->tile[ST_UPDATE], h->gv.Ni, h->gv.Nj, cdiv(h->gv.Nk, update_kc(h)));
 }
 
 // blocks of k_celldt_rows (DT box cells of the ring band), 0 without rings
@@ -855,7 +856,7 @@ static void launch_fluxes(gam_handle *h) {
 template <typename R, typename TM>
 static void launch_b2f(gam_handle *h, int st, const TM *mag, void *B) {
     const GVT<R> &g = view<R>(h);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, cdiv(g.Nk, UPDATE_KC)), 256, 0, h->stream, g, h->tile[st],
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[st],
                        h->clk, mag, P<R>(B));
     if (h->ring.on)   // RingFlux2FieldFix (gridutils.F90:298-300)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g,
@@ -934,11 +935,21 @@ static int advance_t(gam_handle *h, int fuse_dt) {
         const R *nGas = P<R>(h->Gas[S]), *nB = P<R>(h->B[S]), *Flx = P<R>(h->Flx);
         R *oGas = P<R>(h->Gas[O]), *oB = P<R>(h->B[O]);
         const int skip = h->ring.on ? h->ring.nring + 3 : 0;
-        if (m.doMHD) {
+        // FP64: the k-column kernel; FP32 bulk: one cell per thread
+        if constexpr (sizeof(R) == sizeof(double)) {
+            if (m.doMHD) {
+                if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update_col<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+                else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update_col<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+            } else {
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update_col<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+            }
+        } else {
+          if (m.doMHD) {
             if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
             else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
         } else {
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+          }
         }
     }
     // CopyState State->oState is the buffer swap
@@ -1403,7 +1414,7 @@ extern "C" int gam_set_tile(gam_handle *h, int stage, int ti, int tj, int tk, in
     if (rc) return rc;
     const Tile t = {ti, tj, tk, fast};
     if (stage == ST_UPDATE) {   // one block minimum per update block, then the ring band's
-        rc = ensure_partial(h, tile_blocks(t, h->gv.Ni, h->gv.Nj, cdiv(h->gv.Nk, UPDATE_KC)) + ring_dt_blocks(h));
+        rc = ensure_partial(h, tile_blocks(t, h->gv.Ni, h->gv.Nj, cdiv(h->gv.Nk, update_kc(h))) + ring_dt_blocks(h));
         if (rc) return rc;
     }
     h->tile[stage] = t;

@@ -104,7 +104,7 @@ __device__ __forceinline__ void cell_bxyz_kc(const GVT<R> &g, const TM *mag, int
     kc.fK = fKp;
 }
 
-// cells per k_update thread along k (a column walked bottom up)
+// cells per k_update_col thread along k (a column walked bottom up)
 #define UPDATE_KC 4
 
 // ----------------------------------------------------------------- clock
@@ -172,24 +172,17 @@ __global__ void __launch_bounds__(256) k_predict(GVT<R> g, MP m, const DClock *_
     for (int x = 0; x < 3; x++) w.VdV[(size_t)nc * x + c] = vol * pQ[1 + x] / D;
 }
 
-// bFlux2Fld over all cells, FP64 arithmetic, stored in the bulk type; each
-// thread walks UPDATE_KC cells up a k column (each K face read once)
+// bFlux2Fld over all cells, FP64 arithmetic, stored in the bulk type
 template <typename R, typename TM>
 __global__ void __launch_bounds__(256) k_b2f(GVT<R> g, Tile t, const DClock *__restrict__ clk, const TM *__restrict__ mag,
                                              R *__restrict__ B) {
-    int i, j, kb;
-    if (!tile_pos(t, g.Ni, g.Nj, cdiv(g.Nk, UPDATE_KC), i, j, kb) || clk->halt) return;
-    KCarry kc;
-#pragma unroll 1
-    for (int kk = 0; kk < UPDATE_KC; kk++) {
-        const int k = kb * UPDATE_KC + kk;
-        if (k >= g.Nk) break;
-        const int idx = ci(g, i, j, k);
-        double b[3];
-        cell_bxyz_kc(g, mag, i, j, k, b, kc, kk > 0);
+    int i, j, k;
+    if (!tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
+    const int idx = ci(g, i, j, k);
+    double b[3];
+    cell_bxyz(g, mag, i, j, k, b);
 #pragma unroll
-        for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = R(b[x]);
-    }
+    for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = R(b[x]);
 }
 
 // bFlux2Fld on the ghost cells only.  Used after the BCs of a fused 3D step:
@@ -717,8 +710,7 @@ template <typename R> __device__ __forceinline__ bool in_dt_box(const GVT<R> &g,
 
 // Flux2Deltas + applyUpdate (Reynolds, Bxyz of the new fluxes, Maxwell/Boris)
 // + CellDT min, in FP64 from the bulk-typed inputs.  Writes the new Gas/Bxyz
-// into the O buffers; ghosts copy Gas n.  Each thread walks UPDATE_KC cells up
-// a k column, reading each K face's fluxes, magFlux and centre once.
+// into the O buffers; ghosts copy Gas n.
 template <int BORIS, int MHD, typename R>
 __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                 const R *__restrict__ nGas, const R *__restrict__ nB,
@@ -727,8 +719,89 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
                                                 double *__restrict__ partial, int skip_nb) {
     if (clk->halt) return;
     double dtl = GAM_HUGE;
+    int i, j, k;
+    if (tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k)) {
+        const int idx = ci(g, i, j, k);
+        const size_t nc = g.nc;
+        double U[5], Bn[3] = {0, 0, 0};
+#pragma unroll
+        for (int v = 0; v < 5; v++) U[v] = (double)nGas[nc * v + idx];
+        cell_bxyz(g, magNew, i, j, k, Bn);
+        const int a0 = g.nG;
+        bool active = i >= a0 && i < a0 + g.Nip && j >= a0 && j < a0 + g.Njp && k >= a0 && k < a0 + g.Nkp;
+        if (active) {
+            const double dt = clk->dt;
+            const int ci1 = idx + 1, cj1 = idx + g.Ni, ck1 = idx + g.Ni * g.Nj;
+            const double dV = g.dvol[idx];
+            auto F = [&](int slab, int c) { return (double)Flx[nc * slab + c]; };
+            double dGH[5], dGM[3] = {0, 0, 0};
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                dGH[v] = (F(0 + v, idx) - F(0 + v, ci1) + F(8 + v, idx) - F(8 + v, cj1)
+                        + F(16 + v, idx) - F(16 + v, ck1)) / dV;
+            }
+            if (MHD) {
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                    dGM[v] = (F(5 + v, idx) - F(5 + v, ci1) + F(13 + v, idx) - F(13 + v, cj1)
+                            + F(21 + v, idx) - F(21 + v, ck1)) / dV;
+                    if (bg.on) dGM[v] = dGM[v] + bg.dpB0[nc * v + idx];   // stress.F90:185-188
+                }
+            }
+            double U0[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++) { U0[v] = U[v]; U[v] = U[v] + dt * dGH[v]; }   // CellReynolds, Model%dt
+            if (MHD) {
+                if (BORIS) {
+                    double oBv[3] = {(double)nB[idx], (double)nB[nc + idx], (double)nB[2 * nc + idx]};
+                    double Bt[3] = {Bn[0], Bn[1], Bn[2]};
+                    if (bg.on) {   // mhdgroup.F90:180-183 total fields
+#pragma unroll
+                        for (int x = 0; x < 3; x++) { Bt[x] = Bt[x] + bg.B0[nc * x + idx]; oBv[x] = oBv[x] + bg.B0[nc * x + idx]; }
+                    }
+                    cell_boris(m, U, U0, Bt, oBv, dGH, dGM, m.Ca, dt);
+                } else {
+                    cell_maxwell(m, U, dGM, dt);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++) st_nt(&oGas[nc * v + idx], R(U[v]));
+#pragma unroll
+        for (int x = 0; x < 3; x++) st_nt(&oB[nc * x + idx], R(Bn[x]));
+        // skip_nb > 0: the ring rows' CFL bound is taken after the ring average (k_celldt_rows)
+        const bool band = skip_nb > 0 && (j < g.nG - 1 + skip_nb || j > g.nG + g.Njp - skip_nb);
+        if (in_dt_box(g, i, j, k) && !band) {
+            if (sizeof(R) < sizeof(double)) {   // CFL bound of the stored state
+#pragma unroll
+                for (int v = 0; v < 5; v++) U[v] = (double)R(U[v]);
+#pragma unroll
+                for (int x = 0; x < 3; x++) Bn[x] = (double)R(Bn[x]);
+            }
+            if (bg.on) {   // gridutils.F90:82-86
+#pragma unroll
+                for (int x = 0; x < 3; x++) Bn[x] = Bn[x] + bg.B0[nc * x + idx];
+            }
+            dtl = cell_dt<BORIS>(g, m, U, Bn, idx);
+        }
+    }
+    block_min_store(dtl, partial);
+}
+
+// k_update walking UPDATE_KC cells up a k column per thread, reading each K
+// face's fluxes, magFlux and centre once (the FP64 step's update: less HBM
+// traffic there; in FP32 bulk it measured slower and k_update is used).
+template <int BORIS, int MHD, typename R>
+__global__ void __launch_bounds__(256) k_update_col(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                                const R *__restrict__ nGas, const R *__restrict__ nB,
+                                                const R *__restrict__ Flx, const double *__restrict__ magNew,
+                                                R *__restrict__ oGas, R *__restrict__ oB,
+                                                double *__restrict__ partial, int skip_nb) {
+    if (clk->halt) return;
+    double dtl = GAM_HUGE;
+    constexpr int KC = UPDATE_KC;
     int i, j, kb;
-    if (tile_pos(t, g.Ni, g.Nj, cdiv(g.Nk, UPDATE_KC), i, j, kb)) {
+    if (tile_pos(t, g.Ni, g.Nj, cdiv(g.Nk, KC), i, j, kb)) {
         const size_t nc = g.nc;
         const int a0 = g.nG;
         const bool act_ij = i >= a0 && i < a0 + g.Nip && j >= a0 && j < a0 + g.Njp;
@@ -737,8 +810,8 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
         double cF[8];          // K fluxes of the lower K face, carried from the cell below
         bool haveF = false;
 #pragma unroll 1
-        for (int kk = 0; kk < UPDATE_KC; kk++) {
-            const int k = kb * UPDATE_KC + kk;
+        for (int kk = 0; kk < KC; kk++) {
+            const int k = kb * KC + kk;
             if (k >= g.Nk) break;
             const int idx = ci(g, i, j, k);
             double U[5], Bn[3] = {0, 0, 0};
