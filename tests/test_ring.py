@@ -390,3 +390,46 @@ def test_wind_bc_drives_the_wind_in_and_lets_the_flow_out():
                 want = face[ie + 2, jj, kk, 0] * np.dot(Tf[ie + 2, jj, kk, 0:3, 0], b)
                 assert math.isclose(oc.state.magFlux[ie + 2, jj, kk, 0], want, rel_tol=1e-13, abs_tol=1e-18)
     assert n_up > 100 and n_down > 10, (n_up, n_down)
+
+
+def test_ring_and_wind_argument_checks():
+    """The library's checks on the axis/ring/wind settings (gam_set_ring,
+    gam_set_wind, gam_set_bcs), run on the CPU emulation build of the same
+    sources: bad settings fail with a message, good ones are kept."""
+    import dataclasses
+    from kaiju_amd import gamera
+    from tests.emu import build_emu
+    saved = gamera.LIB_PATH
+    gamera.use_library(build_emu.build(asan=False))
+    try:
+        spec = configs.magnetosphere((16, 12, 32), steps=1, precision=0)
+        bare = dataclasses.replace(spec, bcs=("periodic",) * 6, ring=None, wind=None)
+        with gamera.GamApp(bare) as app:
+            # 3 does not divide Nkp = 32; 32 > Nkp/2; five rings need Njp >= 14
+            for chunks in ((3,), (8, 32), (8, 8, 8, 8, 8)):
+                with pytest.raises(gamera.HipStepError):
+                    app.set_ring(chunks)
+            app.set_ring(())                      # off
+            assert app.get_ring() == ()
+            app.set_ring((8, 16))
+            assert app.get_ring() == (8, 16)
+            with pytest.raises(gamera.HipStepError):   # axis BCs on the i faces
+                app.set_bcs(("lfm", "lfm", "periodic", "periodic", "periodic", "periodic"))
+            with pytest.raises(gamera.HipStepError):   # wind before gam_set_wind
+                app.set_bcs(("sphere", "wind", "lfm", "lfm", "periodic", "periodic"))
+            with pytest.raises(gamera.HipStepError):
+                app.set_wind(-1.0, 1.0, (-4.0, 0.0, 0.0), (0.0, 0.0, -1.0))
+            app.set_wind(*spec.wind)
+            with pytest.raises(gamera.HipStepError):   # the wind BC belongs to the outer i face
+                app.set_bcs(("wind", "sphere", "lfm", "lfm", "periodic", "periodic"))
+            app.set_bcs(spec.bcs)
+            assert app.get_bcs() == spec.bcs
+        short = dataclasses.replace(spec, N=(16, 12, 12), ring=None, wind=None,
+                                    bcs=("sphere", "sphere", "periodic", "periodic", "periodic", "periodic"))
+        with gamera.GamApp(short) as app:
+            with pytest.raises(gamera.HipStepError):   # Nkp < 16
+                app.set_ring((4,))
+            with pytest.raises(gamera.HipStepError):
+                app.set_bcs(("sphere", "sphere", "lfm", "lfm", "periodic", "periodic"))
+    finally:
+        gamera.use_library(saved)
