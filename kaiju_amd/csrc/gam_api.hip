@@ -900,7 +900,7 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     { StageTimer t(h, ST_BHALF); launch_b2f<R, R>(h, ST_BHALF, P<R>(h->hmag), h->hB); }
     if (m.doMHD) {
         { StageTimer t(h, ST_MOM2FACE);
-          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g,
+          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, cdiv(g.Nk, VF_KC)), 256, 0, h->stream, g,
                              h->tile[ST_MOM2FACE], h->clk, P<R>(h->wk[W_VDV]), P<R>(h->Vf)); }
         if (m.rmeth == 1) { if (m.doBoris) launch_emfs<1, 1, R>(h); else launch_emfs<1, 0, R>(h); }
         else { if (m.doBoris) launch_emfs<0, 1, R>(h); else launch_emfs<0, 0, R>(h); }

@@ -254,30 +254,54 @@ __device__ __forceinline__ void mom2face(const GVT<R> &g, const R *__restrict__ 
     Vf[2] = c8(Vz) / dV;
 }
 
-// Vf arrays: VfA[dT1][xyz] stored as slab (3*dT1 + xyz)
+// Vf arrays: VfA[dT1][xyz] stored as slab (3*dT1 + xyz).  Each thread walks
+// VF_KC cells up a k column; the k-direction stencil (8 points along k) is a
+// window kept in registers, shifted by one point per cell.
+#define VF_KC 8
 template <typename R>
 __global__ void __launch_bounds__(256) k_vf(GVT<R> g, Tile t, const DClock *__restrict__ clk, const R *__restrict__ VdV,
                                             R *__restrict__ Vf) {
-    int i, j, k;
-    if (!tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
-    const int idx = ci(g, i, j, k);
+    int i, j, kb;
+    if (!tile_pos(t, g.Ni, g.Nj, cdiv(g.Nk, VF_KC), i, j, kb) || clk->halt) return;
     const int a0 = g.nG;
     const size_t nc = g.nc;
-    R v[3];
-    // dT1 = I (used by K edges): i in is..ie+1   (fields.F90:161-175)
-    if (i >= a0 && i <= a0 + g.Nip) {
-        mom2face(g, VdV, idx, 0, v);
-        st_nt(&Vf[nc * 0 + idx], v[0]); st_nt(&Vf[nc * 1 + idx], v[1]); st_nt(&Vf[nc * 2 + idx], v[2]);
-    }
-    // dT1 = J (used by I edges): j in js..je+1   (fields.F90:119-133)
-    if (j >= a0 && j <= a0 + g.Njp) {
-        mom2face(g, VdV, idx, 1, v);
-        st_nt(&Vf[nc * 3 + idx], v[0]); st_nt(&Vf[nc * 4 + idx], v[1]); st_nt(&Vf[nc * 5 + idx], v[2]);
-    }
-    // dT1 = K (used by J edges): k in ks..ke+1   (fields.F90:140-154)
-    if (k >= a0 && k <= a0 + g.Nkp) {
-        mom2face(g, VdV, idx, 2, v);
-        st_nt(&Vf[nc * 6 + idx], v[0]); st_nt(&Vf[nc * 7 + idx], v[1]); st_nt(&Vf[nc * 8 + idx], v[2]);
+    const int sk = cstride(g, 2);
+    R wVol[8], wX[8], wY[8], wZ[8];   // k window of the K-face Mom2Face: points k-4 .. k+3
+    bool haveW = false;
+#pragma unroll 1
+    for (int kk = 0; kk < VF_KC; kk++) {
+        const int k = kb * VF_KC + kk;
+        if (k >= g.Nk) break;
+        const int idx = ci(g, i, j, k);
+        R v[3];
+        // dT1 = I (used by K edges): i in is..ie+1   (fields.F90:161-175)
+        if (i >= a0 && i <= a0 + g.Nip) {
+            mom2face(g, VdV, idx, 0, v);
+            st_nt(&Vf[nc * 0 + idx], v[0]); st_nt(&Vf[nc * 1 + idx], v[1]); st_nt(&Vf[nc * 2 + idx], v[2]);
+        }
+        // dT1 = J (used by I edges): j in js..je+1   (fields.F90:119-133)
+        if (j >= a0 && j <= a0 + g.Njp) {
+            mom2face(g, VdV, idx, 1, v);
+            st_nt(&Vf[nc * 3 + idx], v[0]); st_nt(&Vf[nc * 4 + idx], v[1]); st_nt(&Vf[nc * 5 + idx], v[2]);
+        }
+        // dT1 = K (used by J edges): k in ks..ke+1   (fields.F90:140-154)
+        if (k >= a0 && k <= a0 + g.Nkp) {
+            if (!haveW) {
+#pragma unroll
+                for (int n = 0; n < 8; n++) {
+                    const int cc = idx + (n - 4) * sk;
+                    wVol[n] = g.vol[cc]; wX[n] = VdV[cc]; wY[n] = VdV[nc + cc]; wZ[n] = VdV[2 * nc + cc];
+                }
+                haveW = true;
+            } else {
+#pragma unroll
+                for (int n = 0; n < 7; n++) { wVol[n] = wVol[n + 1]; wX[n] = wX[n + 1]; wY[n] = wY[n + 1]; wZ[n] = wZ[n + 1]; }
+                const int cc = idx + 3 * sk;
+                wVol[7] = g.vol[cc]; wX[7] = VdV[cc]; wY[7] = VdV[nc + cc]; wZ[7] = VdV[2 * nc + cc];
+            }
+            const R dV = rmax(c8(wVol), R(GAM_TINY));
+            st_nt(&Vf[nc * 6 + idx], c8(wX) / dV); st_nt(&Vf[nc * 7 + idx], c8(wY) / dV); st_nt(&Vf[nc * 8 + idx], c8(wZ) / dV);
+        }
     }
 }
 
