@@ -834,8 +834,12 @@ static void launch_emf(gam_handle *h) {
     const int st = ST_EMF_I + ED;
     const WKT<R> w = wkt<R>(h);
     StageTimer t(h, st);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<ED, RM, BO, R>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
-                       256, 0, h->stream, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
+    if (ED == 1)   // J edges: k columns with the k stencil in registers
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf_jcol<RM, BO, R>), tgrid(h, st, g.Nip + 1, g.Njp, cdiv(g.Nkp + 1, EMF_KC)),
+                           256, 0, h->stream, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<ED, RM, BO, R>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
+                           256, 0, h->stream, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
 }
 
 template <int RM, int BO, typename R>

@@ -328,11 +328,24 @@ __device__ __forceinline__ void edge_lrs(const GVT<R> &g, const MP &m, const R *
     bL = l * Vi; bR = r * Vi; fA = Vi;
 }
 
+// The same from stencils already loaded (AreaB, MagB: 8 points; nMagB: 4)
+template <int RM, typename R>
+__device__ __forceinline__ void edge_lrs_w(const MP &m, const R *AreaB, const R *MagB, const R *nMagB,
+                                           R &bL, R &bR, R &fA) {
+    R Vi = c8(AreaB);
+    R l, r;
+    get_lrs<RM>(R(m.pdmb), AreaB, nMagB, MagB, R(1) / Vi, l, r);
+    bL = l * Vi; bR = r * Vi; fA = Vi;
+}
+
 // One edge EMF (fields.F90:196-221) for edge direction eD at corner (i,j,k)
+// wA/wB/wN: the dT2 faces' stencils along dT1 (face, bh: 8 points; bn: 4)
+// already loaded (a k-column walker's register window), or null.
 template <int RM, int BORIS, typename R>
 __device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, const BGV &bg, double dt64, const R *__restrict__ Vf,
                                       const R *__restrict__ hGas, const R *__restrict__ bn,
-                                      const R *__restrict__ bh, int eD, int i, int j, int k) {
+                                      const R *__restrict__ bh, int eD, int i, int j, int k,
+                                      const R *wA = nullptr, const R *wB = nullptr, const R *wN = nullptr) {
     const R dt = R(dt64);
     const int dT1 = (eD + 1) % 3, dT2 = (eD + 2) % 3;
     const int f = fi(g, i, j, k), c = ci(g, i, j, k);
@@ -362,7 +375,8 @@ __device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, const BGV &b
     // GetCornerB fields.F90:365-424
     R bT1L, bT1R, fA1, bT2L, bT2R, fA2;
     edge_lrs<RM>(g, m, bn, bh, f, dT1, dT2, bT1L, bT1R, fA1);
-    edge_lrs<RM>(g, m, bn, bh, f, dT2, dT1, bT2L, bT2R, fA2);
+    if (wA) edge_lrs_w<RM>(m, wA, wB, wN, bT2L, bT2R, fA2);
+    else edge_lrs<RM>(g, m, bn, bh, f, dT2, dT1, bT2L, bT2R, fA2);
     const R *Teb = g.Teb + nf * 4 * eD;
     R xni = Teb[nf * XNQI + f], yni = Teb[nf * YNQI + f];
     R xnj = Teb[nf * XNQJ + f], ynj = Teb[nf * YNQJ + f];
@@ -402,6 +416,44 @@ __global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, BGV bg, con
     if (!tile_pos(t, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2), a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
     E[(size_t)g.nc * ED + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, ED, i, j, k);
+}
+
+// J edges walking EMF_KC edges up a k column: the I faces' stencils along k
+// (the one k-direction stencil of a J edge) are a register window shifted by
+// one point per edge.  Box [is,ie+1]x[js,je]x[ks,ke+1].
+#define EMF_KC 8
+template <int RM, int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_emf_jcol(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                                  const R *__restrict__ Vf, const R *__restrict__ hGas,
+                                                  const R *__restrict__ bn, const R *__restrict__ bh, R *__restrict__ E) {
+    const int nz = g.Nkp + 1;
+    int a, b, cb;
+    if (!tile_pos(t, g.Nip + 1, g.Njp, cdiv(nz, EMF_KC), a, b, cb) || clk->halt) return;
+    const int i = a + g.nG, j = b + g.nG;
+    const int st = fstride(g, 2);
+    const R *fI = g.face, *bhI = bh, *bnI = bn;   // I-face slabs (direction 0)
+    R wA[8], wB[8], wN[4];
+#pragma unroll 1
+    for (int kk = 0; kk < EMF_KC; kk++) {
+        const int cz = cb * EMF_KC + kk;
+        if (cz >= nz) break;
+        const int k = cz + g.nG;
+        const int f = fi(g, i, j, k);
+        if (kk == 0) {
+#pragma unroll
+            for (int n = 0; n < 8; n++) { wA[n] = fI[f + (n - 4) * st]; wB[n] = bhI[f + (n - 4) * st]; }
+#pragma unroll
+            for (int n = 0; n < 4; n++) wN[n] = bnI[f + (n - 2) * st];
+        } else {
+#pragma unroll
+            for (int n = 0; n < 7; n++) { wA[n] = wA[n + 1]; wB[n] = wB[n + 1]; }
+#pragma unroll
+            for (int n = 0; n < 3; n++) wN[n] = wN[n + 1];
+            wA[7] = fI[f + 3 * st]; wB[7] = bhI[f + 3 * st]; wN[3] = bnI[f + st];
+        }
+        E[(size_t)g.nc * 1 + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, 1, i, j, k,
+                                                                   wA, wB, wN);
+    }
 }
 
 // ----------------------------------------------------------------- fluxes
