@@ -60,6 +60,7 @@ struct gam_handle {
     double *ringTab = nullptr;         // cos, sin of the ring angles phi_n
     int fused_np = 0;                  // block minima the fused step's CalcDT reduces
     WindP wind = {};                   // the solar wind of a GAM_BC_WIND outer i face (gam_set_wind)
+    bool corners_set = false;          // geometry derived by gam_set_corners (RingGridFix applied then)
     double *ctG = nullptr, *ctM = nullptr, *crG = nullptr, *crM = nullptr;   // correction training buffers
     BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
     double *partial = nullptr;
@@ -399,6 +400,7 @@ extern "C" int gam_set_corners(gam_handle *h, const double *xyz) {
     }
     HIPCHK(h, hipGetLastError());
     if ((rc = sync_grid32(h))) return rc;
+    h->corners_set = true;
     return gam_sync(h);
 }
 
@@ -1089,6 +1091,8 @@ extern "C" int gam_set_ring(gam_handle *h, int nring, const int *nch) {
     const GV &g = h->gv;
     int rc = gam_sync(h);
     if (rc) return rc;
+    if (h->corners_set && (nring != (h->ring.on ? h->ring.nring : 0)))
+        return fail(h, -1, "gam_set_ring: call it before gam_set_corners (RingGridFix is part of the geometry)");
     if (nring == 0) { h->ring.on = 0; return 0; }
     if (nring < 0 || nring > GAM_RING_MAX || !nch) return fail(h, -1, "gam_set_ring: 1 to %d rings", GAM_RING_MAX);
     if (h->mp.do25D || g.Nkp < 16 || g.Nkp > GAM_RING_NPMAX || (g.Nkp & 1))

@@ -424,6 +424,9 @@ def test_ring_and_wind_argument_checks():
                 app.set_bcs(("wind", "sphere", "lfm", "lfm", "periodic", "periodic"))
             app.set_bcs(spec.bcs)
             assert app.get_bcs() == spec.bcs
+            app.set_corners(gamera.gen_corners(spec))
+            with pytest.raises(gamera.HipStepError):   # RingGridFix belongs to the geometry built before
+                app.set_ring((8,))
         short = dataclasses.replace(spec, N=(16, 12, 12), ring=None, wind=None,
                                     bcs=("sphere", "sphere", "periodic", "periodic", "periodic", "periodic"))
         with gamera.GamApp(short) as app:
