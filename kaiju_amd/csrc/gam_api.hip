@@ -58,6 +58,8 @@ struct gam_handle {
     RingP ring = {};                   // ring average about the LFM axis (gam_set_ring)
     double *ringSE = nullptr;          // its smoothing EMF (G x 3, zero off the ring rows)
     double *ringTab = nullptr;         // cos, sin of the ring angles phi_n
+    double *ringMean = nullptr;        // SmoothGas chunk means [pole][ring][var][Nkp/2][Nip]
+    size_t ringMean_n = 0;             // its capacity in doubles
     int fused_np = 0;                  // block minima the fused step's CalcDT reduces
     WindP wind = {};                   // the solar wind of a GAM_BC_WIND outer i face (gam_set_wind)
     bool corners_set = false;          // geometry derived by gam_set_corners (RingGridFix applied then)
@@ -281,7 +283,7 @@ extern "C" int gam_destroy(gam_handle *h) {
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
     for (int q = 0; q < 9; q++) if (h->grid32[q]) (void)hipFree(h->grid32[q]);
     for (int q = 0; q < 5; q++) if (h->bg[q]) (void)hipFree(h->bg[q]);
-    for (double *b : {h->ctG, h->ctM, h->crG, h->crM, h->ringSE, h->ringTab}) if (b) (void)hipFree(b);
+    for (double *b : {h->ctG, h->ctM, h->crG, h->crM, h->ringSE, h->ringTab, h->ringMean}) if (b) (void)hipFree(b);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
     void *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, h->clk,
                       h->wk[0], h->wk[1], h->wk[2], h->wk[3], h->wk[4]};
@@ -888,8 +890,10 @@ static void ring_average(gam_handle *h) {
                        (const double *)h->mag[S], P<R>(h->B[S]));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g, h->clk,
                        P<R>(h->B[S]));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_gas<R>), dim3((unsigned)(g.Nip * rp.nring * 2)), 256, 0, h->stream, g, h->mp,
-                       rp, h->clk, P<R>(h->Gas[S]));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_gas_mean<R>), grid1((long)g.Nip * (g.Nkp / 2) * rp.nring * 2), 256, 0,
+                       h->stream, g, h->mp, rp, h->clk, (const R *)P<R>(h->Gas[S]), h->ringMean);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_gas_set<R>), grid1((long)g.Nip * g.Nkp * rp.nring * 2), 256, 0, h->stream, g,
+                       h->mp, rp, h->clk, (const double *)h->ringMean, P<R>(h->Gas[S]));
 }
 
 // AdvanceMHD + UpdateStateData time update; the dt min of the fused path is
@@ -999,14 +1003,15 @@ static void launch_bc(gam_handle *h, bool withB) {
     const int S = h->s;
     R *Bw = withB ? P<R>(h->B[S]) : (R *)nullptr;
     if (all_periodic(h)) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, 7, P<R>(h->Gas[S]),
-                           h->mag[S], Bw);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(bc_shell_count(bc_shell(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp, g.nG, 7, h->mp.do25D))),
+                           256, 0, h->stream, g, h->clk, h->mp.do25D, 7, P<R>(h->Gas[S]), h->mag[S], Bw);
         return;
     }
     const int ext[3] = {g.Ni, g.Nj, g.Nk};
     for (int ax = 0; ax < 3; ax++) {
         if (h->bct[2 * ax] == GAM_BC_PERIODIC) {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mp.do25D, 1 << ax,
+            const long nsh = bc_shell_count(bc_shell(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp, g.nG, 1 << ax, h->mp.do25D));
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(nsh), 256, 0, h->stream, g, h->clk, h->mp.do25D, 1 << ax,
                                P<R>(h->Gas[S]), h->mag[S], Bw);
             continue;
         }
@@ -1108,6 +1113,14 @@ extern "C" int gam_set_ring(gam_handle *h, int nring, const int *nch) {
         HIPCHK(h, hipMalloc(&h->ringTab, 2 * (size_t)GAM_RING_NPMAX * sizeof(double)));
     }
     HIPCHK(h, hipMemset(h->ringSE, 0, nse * sizeof(double)));   // never written off the ring rows
+    const size_t nmean = (size_t)2 * nring * 5 * (g.Nkp / 2) * g.Nip;
+    if (nmean > h->ringMean_n) {
+        if (h->ringMean) HIPCHK(h, hipFree(h->ringMean));
+        h->ringMean = nullptr;
+        h->ringMean_n = 0;
+        HIPCHK(h, hipMalloc(&h->ringMean, nmean * sizeof(double)));
+        h->ringMean_n = nmean;
+    }
     // ringav.F90:447-451: phi_n = dp*n - dp/2, n = 1..Np, dp = 2 pi/Np
     std::vector<double> tab(2 * (size_t)g.Nkp);
     const double dp = 2.0 * M_PI / g.Nkp;

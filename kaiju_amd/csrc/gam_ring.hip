@@ -272,11 +272,11 @@ __device__ __forceinline__ void ring_ppm(double fm2, double fm1, double f, doubl
 }
 
 // ringrecon.F90:33-87 ReconstructRing for cell n of a ring whose Nc chunk
-// means are mean[] (periodic in the chunk index): the cell mean of its
-// chunk's PPM parabola
-__device__ __forceinline__ double ring_recon_cell(const double *mean, int Nc, int Np, int n) {
+// means are mean[0], mean[stride], ... (periodic in the chunk index): the
+// cell mean of its chunk's PPM parabola
+__device__ __forceinline__ double ring_recon_cell(const double *mean, int Nc, int Np, int n, int stride = 1) {
     const int dJ = Np / Nc, c = n / dJ, m = n - c * dJ + 1;
-    auto ch = [&](int q) { return mean[((q % Nc) + Nc) % Nc]; };
+    auto ch = [&](int q) { return mean[(size_t)(((q % Nc) + Nc) % Nc) * stride]; };
     double fL, fR;
     const double f = ch(c);
     ring_ppm(ch(c - 2), ch(c - 1), f, ch(c + 1), ch(c + 2), fL, fR);
@@ -437,55 +437,74 @@ __global__ void __launch_bounds__(256) k_ring_loops(GV g, const DClock *__restri
 }
 
 // ringav.F90:76-200 SmoothGas (single fluid, doMassRA = F, no shift, no
-// velocity cleaning): one block per (slice i in is..ie, ring, pole).  Ring
-// variables rho, momentum, internal energy (Gas2Ring) are chunk-averaged and
-// PPM-reconstructed, then turned back into conserved variables (Ring2Gas).
+// velocity cleaning), in two passes.  First the chunk means of the ring
+// variables rho, momentum, internal energy (Gas2Ring); thread = (i in is..ie,
+// chunk, ring, pole), i fastest so that a wave reads whole cache lines, the
+// chunk's cells summed in order of k.  mean[((pole*nring + ring)*5 + v)*(Np/2)
+// + chunk][i - is] (G-independent scratch sized by gam_set_ring).
 template <typename R>
-__global__ void __launch_bounds__(256) k_ring_gas(GVT<R> g, MP mp, RingP rp, const DClock *__restrict__ clk,
-                                                  R *__restrict__ Gas) {
-    __shared__ double w[5][GAM_RING_NPMAX];
-    __shared__ double mean[5][GAM_RING_NPMAX / 2];
-    if (clk->halt) return;
-    RING_ONE_THREAD();
-    const int Np = g.Nkp, nS = g.nG + (int)(blockIdx.x % g.Nip);
-    const int rest = (int)(blockIdx.x / g.Nip);
-    const int nR = 1 + rest % rp.nring, pole = rest / rp.nring;
+__global__ void __launch_bounds__(256) k_ring_gas_mean(GVT<R> g, MP mp, RingP rp, const DClock *__restrict__ clk,
+                                                       const R *__restrict__ Gas, double *__restrict__ mean) {
+    const int Np = g.Nkp, hc = Np / 2;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.Nip * hc * rp.nring * 2 || clk->halt) return;
+    const int i = (int)(idx % g.Nip);
+    long r = idx / g.Nip;
+    const int c = (int)(r % hc);
+    r /= hc;
+    const int nR = 1 + (int)(r % rp.nring), pole = (int)(r / rp.nring);
     const int NumCh = rp.nch[nR - 1];
+    if (c >= NumCh) return;
     const int j = pole ? g.nG + g.Njp - 1 - (nR - 1) : g.nG + nR - 1;
+    const int dC = Np / NumCh;
     const size_t nc = g.nc;
     const double gm1 = mp.gamma - 1;
-    for (int n = RING_T0; n < Np; n += RING_NT) {   // PullRingCC + Gas2Ring
-        const int c = ci(g, nS, j, g.nG + n);
-        const double D = fmax((double)Gas[c], mp.dFloor);
-        const double Mx = Gas[nc + c], My = Gas[2 * nc + c], Mz = Gas[3 * nc + c], E = Gas[4 * nc + c];
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int m = 0; m < dC; m++) {   // PullRingCC + Gas2Ring
+        const int cc = ci(g, g.nG + i, j, g.nG + c * dC + m);
+        const double D = fmax((double)Gas[cc], mp.dFloor);
+        const double Mx = Gas[nc + cc], My = Gas[2 * nc + cc], Mz = Gas[3 * nc + cc], E = Gas[4 * nc + cc];
         const double KinE = 0.5 * (Mx * Mx + My * My + Mz * Mz) / D;
         const double P = fmax(gm1 * (E - KinE), mp.pFloor);
-        w[0][n] = D; w[1][n] = Mx; w[2][n] = My; w[3][n] = Mz;
-        w[4][n] = P / gm1;
+        s[0] = s[0] + D; s[1] = s[1] + Mx; s[2] = s[2] + My; s[3] = s[3] + Mz;
+        s[4] = s[4] + P / gm1;
     }
-    __syncthreads();
-    const int dC = Np / NumCh;
-    for (int t = RING_T0; t < 5 * NumCh; t += RING_NT) {   // chunk means
-        const int v = t / NumCh, c = t - v * NumCh;
-        double s = 0.0;
-        for (int m = 0; m < dC; m++) s = s + w[v][c * dC + m];
-        mean[v][c] = s / dC;
-    }
-    __syncthreads();
-    for (int n = RING_T0; n < Np; n += RING_NT) {
-        double q[5];
+    const size_t sv = (size_t)hc * g.Nip;
+    double *o = mean + (size_t)((pole * rp.nring + nR - 1) * 5) * sv + (size_t)c * g.Nip + i;
 #pragma unroll
-        for (int v = 0; v < 5; v++) q[v] = ring_recon_cell(mean[v], NumCh, Np, n);
-        // Ring2Gas + PushRingCC
-        const double D = fmax(q[0], mp.dFloor);
-        double P = gm1 * q[4];
-        P = fmax(P, mp.pFloor);
-        const double KinE = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / D;
-        const int cidx = ci(g, nS, j, g.nG + n);
-        Gas[cidx] = R(D);
-        Gas[nc + cidx] = R(q[1]); Gas[2 * nc + cidx] = R(q[2]); Gas[3 * nc + cidx] = R(q[3]);
-        Gas[4 * nc + cidx] = R(KinE + P / gm1);
-    }
+    for (int v = 0; v < 5; v++) o[v * sv] = s[v] / dC;
+}
+
+// Then each ring cell from the PPM reconstruction of its chunk's neighbourhood
+// of means, turned back into conserved variables (Ring2Gas + PushRingCC);
+// thread = (i in is..ie, k, ring, pole)
+template <typename R>
+__global__ void __launch_bounds__(256) k_ring_gas_set(GVT<R> g, MP mp, RingP rp, const DClock *__restrict__ clk,
+                                                      const double *__restrict__ mean, R *__restrict__ Gas) {
+    const int Np = g.Nkp, hc = Np / 2;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.Nip * Np * rp.nring * 2 || clk->halt) return;
+    const int i = (int)(idx % g.Nip);
+    long r = idx / g.Nip;
+    const int n = (int)(r % Np);
+    r /= Np;
+    const int nR = 1 + (int)(r % rp.nring), pole = (int)(r / rp.nring);
+    const int NumCh = rp.nch[nR - 1];
+    const int j = pole ? g.nG + g.Njp - 1 - (nR - 1) : g.nG + nR - 1;
+    const size_t nc = g.nc, sv = (size_t)hc * g.Nip;
+    const double *mb = mean + (size_t)((pole * rp.nring + nR - 1) * 5) * sv + i;
+    const double gm1 = mp.gamma - 1;
+    double q[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) q[v] = ring_recon_cell(mb + v * sv, NumCh, Np, n, g.Nip);
+    const double D = fmax(q[0], mp.dFloor);
+    double P = gm1 * q[4];
+    P = fmax(P, mp.pFloor);
+    const double KinE = 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / D;
+    const int cidx = ci(g, g.nG + i, j, g.nG + n);
+    Gas[cidx] = R(D);
+    Gas[nc + cidx] = R(q[1]); Gas[2 * nc + cidx] = R(q[2]); Gas[3 * nc + cidx] = R(q[3]);
+    Gas[4 * nc + cidx] = R(KinE + P / gm1);
 }
 
 // The ring band: cell rows j in [js-1, js+nring+1] and [je-nring-1, je+1]
@@ -585,7 +604,7 @@ __device__ __forceinline__ double wind_con2fast(const MP &m, const double *U, co
 
 // Ghost cells (weight at the cell centre: the wind, or the last cell's state
 // pushed outward at >= MinMach = 1 fast speed) and ghost face fluxes (the wind
-// field's flux mixed with the last cell's); thread = (j, k, n) over j <= Nj,
+// field's flux mixed with the last cell's); thread = (n, j, k) over j <= Nj,
 // k <= Nk.  Faces that bound no cell are left alone.  FP64 arithmetic.
 template <typename R>
 __global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const DClock *__restrict__ clk,
@@ -594,9 +613,9 @@ __global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const 
                                                  R *__restrict__ Gas, double *__restrict__ mag, const R *__restrict__ B) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)(g.Nj + 1) * (g.Nk + 1) * g.nG || clk->halt) return;
-    const int j = (int)(idx % (g.Nj + 1));
-    const long r = idx / (g.Nj + 1);
-    const int k = (int)(r % (g.Nk + 1)), nl = 1 + (int)(r / (g.Nk + 1));
+    const int nl = 1 + (int)(idx % g.nG);   // ghost layer fastest: neighbouring lanes share cache lines
+    const long r = idx / g.nG;
+    const int j = (int)(r % (g.Nj + 1)), k = (int)(r / (g.Nj + 1));
     const int ip = g.nG + g.Nip - 1, ig = ip + nl;
     const size_t nc = g.nc, nf = g.nf;
     const double MinMach = 1.0;
@@ -641,15 +660,15 @@ __global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const 
 }
 
 // WindBC's ghost Bxyz: bFlux2Fld of the cell the map names (k periodic; j
-// across the axis when jmap); thread = (j, k, n) over the ghost cells i > ie
+// across the axis when jmap); thread = (i, j, k) over the ghost cells i > ie
 template <typename R>
 __global__ void __launch_bounds__(256) k_wind_bxyz(GVT<R> g, const DClock *__restrict__ clk, int jmap,
                                                    const double *__restrict__ mag, R *__restrict__ B) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)g.Nj * g.Nk * g.nG || clk->halt) return;
-    const int j = (int)(idx % g.Nj);
-    const long r = idx / g.Nj;
-    const int k = (int)(r % g.Nk), ig = g.nG + g.Nip - 1 + 1 + (int)(r / g.Nk);
+    const int ig = g.nG + g.Nip + (int)(idx % g.nG);
+    const long r = idx / g.nG;
+    const int j = (int)(r % g.Nj), k = (int)(r / g.Nj);
     int jp = j, kp;
     if (jmap) lfm_cc(g.nG, g.Njp, g.Nkp, j, k, jp, kp);
     else kp = k < g.nG ? k + g.Nkp : (k > g.nG + g.Nkp - 1 ? k - g.Nkp : k);

@@ -1115,15 +1115,64 @@ __device__ __forceinline__ int pmap_f(int i, int a0, int n) {   // owned faces a
     return i;
 }
 
+// Both maps are the identity on the owned cells a0..a0+n-1 of a mapped axis
+// (2.5D k: on ks alone) and on every index of an axis not mapped, so only the
+// points of the (Ni+1)(Nj+1)(Nk+1) face-point box outside that inner box can
+// change anything: the pass runs over this shell.  Shell order: the k planes
+// outside, then the j rows outside of the inner planes, then the i points
+// outside of the inner rows (i fastest in each part).
+struct BcShell {
+    int lo[3], in[3], np[3];   // per axis: first inner point, inner count, point count
+};
+__host__ __device__ __forceinline__ BcShell bc_shell(int Ni, int Nj, int Nk, int Nip, int Njp, int Nkp, int a0, int mask,
+                                                     int do25D) {
+    BcShell s;
+    const int N[3] = {Ni, Nj, Nk}, Np[3] = {Nip, Njp, Nkp};
+    for (int x = 0; x < 3; x++) {
+        s.np[x] = N[x] + 1;
+        const bool m = (mask >> x) & 1;
+        s.lo[x] = m ? a0 : 0;
+        s.in[x] = m ? (x == 2 && do25D ? 1 : Np[x]) : N[x] + 1;
+    }
+    return s;
+}
+__host__ __device__ __forceinline__ long bc_shell_count(const BcShell &s) {
+    return (long)s.np[0] * s.np[1] * s.np[2] - (long)s.in[0] * s.in[1] * s.in[2];
+}
+
 // mask: the periodic axes (bit 0 i, 1 j, 2 k) mapped by this pass; the
 // others are left for their own BCs.
 template <typename R>
 __global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__ clk, int do25D, int mask, R *__restrict__ Gas,
                                             double *__restrict__ mag, R *__restrict__ B) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.nf || clk->halt) return;
+    const BcShell sh = bc_shell(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp, g.nG, mask, do25D);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nplane = (long)(sh.np[2] - sh.in[2]) * sh.np[0] * sh.np[1];
+    const long nrow = (long)sh.in[2] * (sh.np[1] - sh.in[1]) * sh.np[0];
+    if (t >= bc_shell_count(sh) || clk->halt) return;
     int i, j, k;
-    decode_f(g, idx, i, j, k);
+    if (t < nplane) {
+        const long pl = (long)sh.np[0] * sh.np[1];
+        const int p = (int)(t / pl), r = (int)(t % pl);
+        k = p < sh.lo[2] ? p : p + sh.in[2];
+        j = r / sh.np[0];
+        i = r % sh.np[0];
+    } else if (t < nplane + nrow) {
+        const long q = t - nplane;
+        const int gj = sh.np[1] - sh.in[1], row = (int)(q / sh.np[0]);
+        i = (int)(q % sh.np[0]);
+        k = sh.lo[2] + row / gj;
+        const int r = row % gj;
+        j = r < sh.lo[1] ? r : r + sh.in[1];
+    } else {
+        const long q = t - nplane - nrow;
+        const int gi = sh.np[0] - sh.in[0];
+        const int r = (int)(q % gi), rr = (int)(q / gi);
+        i = r < sh.lo[0] ? r : r + sh.in[0];
+        j = sh.lo[1] + rr % sh.in[1];
+        k = sh.lo[2] + rr / sh.in[1];
+    }
+    const int idx = i + sh.np[0] * (j + sh.np[1] * k);
     const int a0 = g.nG;
     const size_t nc = g.nc, nf = g.nf;
     const int Nk_eff = g.Nkp;
