@@ -58,7 +58,7 @@ struct gam_handle {
     RingP ring = {};                   // ring average about the LFM axis (gam_set_ring)
     double *ringSE = nullptr;          // its smoothing EMF (G x 3, zero off the ring rows)
     double *ringTab = nullptr;         // cos, sin of the ring angles phi_n
-    double *ringMean = nullptr;        // SmoothGas chunk means [pole][ring][var][Nkp/2][Nip]
+    double *ringMean = nullptr;        // ring-average scratch (chunk means, Fourier coefficients)
     size_t ringMean_n = 0;             // its capacity in doubles
     int fused_np = 0;                  // block minima the fused step's CalcDT reduces
     WindP wind = {};                   // the solar wind of a GAM_BC_WIND outer i face (gam_set_wind)
@@ -880,8 +880,13 @@ static void ring_average(gam_handle *h) {
     const int S = h->s;
     const RingP &rp = h->ring;
     StageTimer t(h, ST_RING);
-    hipLaunchKernelGGL(k_ring_smoothe, dim3((unsigned)((g.Nip + 1) * rp.nring * 2)), 256, 0, h->stream, g64, rp, h->clk,
-                       h->ringTab, h->mag[S], h->ringSE);
+    const long nsr = (long)(g.Nip + 1) * 2 * rp.nring * 2;   // (slice, d, ring, pole)
+    hipLaunchKernelGGL(k_ring_se_coef, grid1(nsr), 256, 0, h->stream, g64, rp, h->clk, (const double *)h->ringTab,
+                       (const double *)h->mag[S], h->ringMean);
+    hipLaunchKernelGGL(k_ring_se_mean, grid1(nsr * (g.Nkp / 2)), 256, 0, h->stream, g64, rp, h->clk,
+                       (const double *)h->ringTab, (const double *)h->mag[S], h->ringMean);
+    hipLaunchKernelGGL(k_ring_se_emf, grid1(nsr * (g.Nkp / 2)), 256, 0, h->stream, g64, rp, h->clk,
+                       (const double *)h->ringTab, (const double *)h->mag[S], (const double *)h->ringMean, h->ringSE);
     hipLaunchKernelGGL(k_ring_ct, grid1((long)(g.Nip + 1) * (g.Nkp + 1) * 2 * (rp.nring + 2)), 256, 0, h->stream, g64, rp,
                        h->clk, h->ringSE, h->mag[S]);
     hipLaunchKernelGGL(k_ring_loops, dim3((unsigned)((g.Nip + 1) * 2)), 256, 0, h->stream, g64, h->clk, h->mag[S]);
@@ -1113,7 +1118,9 @@ extern "C" int gam_set_ring(gam_handle *h, int nring, const int *nch) {
         HIPCHK(h, hipMalloc(&h->ringTab, 2 * (size_t)GAM_RING_NPMAX * sizeof(double)));
     }
     HIPCHK(h, hipMemset(h->ringSE, 0, nse * sizeof(double)));   // never written off the ring rows
-    const size_t nmean = (size_t)2 * nring * 5 * (g.Nkp / 2) * g.Nip;
+    // ring scratch: SmoothGas chunk means, or CalcSmoothE chunk means + Fourier coefficients
+    const size_t hc = g.Nkp / 2;
+    const size_t nmean = std::max((size_t)2 * nring * 5 * hc * g.Nip, (size_t)2 * nring * 2 * (hc + 4) * (g.Nip + 1));
     if (nmean > h->ringMean_n) {
         if (h->ringMean) HIPCHK(h, hipFree(h->ringMean));
         h->ringMean = nullptr;

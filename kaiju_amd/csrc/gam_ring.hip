@@ -274,113 +274,184 @@ __device__ __forceinline__ void ring_ppm(double fm2, double fm1, double f, doubl
 // ringrecon.F90:33-87 ReconstructRing for cell n of a ring whose Nc chunk
 // means are mean[0], mean[stride], ... (periodic in the chunk index): the
 // cell mean of its chunk's PPM parabola
-__device__ __forceinline__ double ring_recon_cell(const double *mean, int Nc, int Np, int n, int stride = 1) {
-    const int dJ = Np / Nc, c = n / dJ, m = n - c * dJ + 1;
+struct RingParab {
+    double a, b, cc;
+};
+__device__ __forceinline__ RingParab ring_recon_chunk(const double *mean, int Nc, int c, int stride) {
     auto ch = [&](int q) { return mean[(size_t)(((q % Nc) + Nc) % Nc) * stride]; };
     double fL, fR;
     const double f = ch(c);
     ring_ppm(ch(c - 2), ch(c - 1), f, ch(c + 1), ch(c + 2), fL, fR);
-    const double a = 3 * (fL + fR - 2 * f), b = 2 * (3 * f - fR - 2 * fL), cc = fL;
-    return (a / 3.0) * (3 * m * m - 3 * m + 1) / (dJ * dJ) + 0.5 * b * (2 * m - 1) / dJ + cc;
+    return RingParab{3 * (fL + fR - 2 * f), 2 * (3 * f - fR - 2 * fL), fL};
+}
+// cell m = 1..dJ of the chunk
+__device__ __forceinline__ double ring_recon_eval(const RingParab &p, int m, int dJ) {
+    return (p.a / 3.0) * (3 * m * m - 3 * m + 1) / (dJ * dJ) + 0.5 * p.b * (2 * m - 1) / dJ + p.cc;
+}
+__device__ __forceinline__ double ring_recon_cell(const double *mean, int Nc, int Np, int n, int stride = 1) {
+    const int dJ = Np / Nc, c = n / dJ, m = n - c * dJ + 1;
+    return ring_recon_eval(ring_recon_chunk(mean, Nc, c, stride), m, dJ);
 }
 
-// chunk means of q (sequential sum within a chunk, as the reference's sum())
-__device__ void ring_chunk_means(const double *q, double *mean, int Nc, int Np) {
-    const int dC = Np / Nc;
-    for (int c = RING_T0; c < Nc; c += RING_NT) {
-        double s = 0.0;
-        for (int m = 0; m < dC; m++) s = s + q[c * dC + m];
-        mean[c] = s / dC;
-    }
-    __syncthreads();
+// ringav.F90:206-329 CalcSmoothE, in three passes over threads with the slice
+// i in is..ie+1 fastest (a wave reads whole cache lines of the face fluxes).
+// Per (slice, ring, pole) the reference pulls the ring's J face fluxes (d = 0,
+// their EMF goes to the I edges) and I face fluxes (d = 1, to the J edges),
+// removes their m = 0, 1 Fourier modes (CleanRing, NFTMAG = 1), chunk-averages
+// and PPM-reconstructs the rest, adds the modes back (DirtyRing) and turns
+// the flux change into an EMF summed along each chunk.  SE (FP64, G-sized,
+// zero outside the ring rows) gets that EMF on the ring rows; the PoleE and
+// FixEFieldLFM the reference applies to SE next leave it unchanged (the axis
+// rows stay zero, the seam values are written equal), so they are not
+// launched.  tab: cos, sin of phi_n.  Scratch (gam_set_ring sizes it):
+// chunk means [pole][ring][d][Np/2][Nip+1], then the Fourier coefficients
+// [pole][ring][d][4][Nip+1].
+struct SmoothEIdx {
+    int s, d, j;   // slice, flux kind, row of the fluxes and of their EMF
+};
+__device__ __forceinline__ SmoothEIdx smoothe_idx(const GV &g, int s, int d, int nR, int pole) {
+    const int js = g.nG, je = g.nG + g.Njp - 1;
+    // J faces -> I edges on row js+nR (je-nR+1); I faces -> J edges on js+nR-1 (je-nR+1)
+    const int j = pole ? je - nR + 1 : (d == 0 ? js + nR : js + nR - 1);
+    return SmoothEIdx{s, d, j};
+}
+__device__ __forceinline__ double smoothe_flux(const GV &g, const SmoothEIdx &x, const double *__restrict__ mag, int n) {
+    const size_t f = fi(g, g.nG + x.s, x.j, g.nG + n);
+    return x.d == 0 ? mag[g.nf + f] : mag[f];
 }
 
-// ringav.F90:206-329 CalcSmoothE: one block per (slice i in is..ie+1, ring,
-// pole).  The ring's J (then I) face fluxes lose their m = 0, 1 Fourier modes
-// (CleanRing, NFTMAG = 1), are chunk-averaged and PPM-reconstructed, get the
-// modes back (DirtyRing); the flux change becomes an EMF whose curl makes it,
-// summed along each chunk.  SE (FP64, G-sized, zero outside the ring rows) is
-// written on the ring rows; the PoleE and FixEFieldLFM the reference applies
-// to SE next leave it unchanged (the axis rows stay zero, the seam values are
-// written equal), so they are not launched.  tab: cos, sin of phi_n.
-__global__ void __launch_bounds__(256) k_ring_smoothe(GV g, RingP rp, const DClock *__restrict__ clk,
-                                                      const double *__restrict__ tab, const double *__restrict__ mag,
-                                                      double *__restrict__ SE) {
-    __shared__ double bF[2][GAM_RING_NPMAX], cs[GAM_RING_NPMAX], sn[GAM_RING_NPMAX];
-    __shared__ double rF[GAM_RING_NPMAX], dEr[2][GAM_RING_NPMAX];
-    __shared__ double mean[GAM_RING_NPMAX];
-    __shared__ double cF[4];
-    if (clk->halt) return;
-    RING_ONE_THREAD();
-    const int Np = g.Nkp, nS = g.nG + (int)(blockIdx.x % (g.Nip + 1));
-    const int rest = (int)(blockIdx.x / (g.Nip + 1));
-    const int nR = 1 + rest % rp.nring, pole = rest / rp.nring;
-    const int NumCh = rp.nch[nR - 1], dC = Np / NumCh;
-    const int js = g.nG, je = g.nG + g.Njp - 1, ks = g.nG;
-    const size_t nf = g.nf, nc = g.nc;
-    for (int n = RING_T0; n < Np; n += RING_NT) {   // PullRingI
-        const int k = ks + n;
-        if (pole == 0) {
-            bF[0][n] = mag[fi(g, nS, js + nR - 1, k)];
-            bF[1][n] = mag[nf + fi(g, nS, js + nR, k)];
-        } else {
-            bF[0][n] = mag[fi(g, nS, je - nR + 1, k)];
-            bF[1][n] = mag[nf + fi(g, nS, je - nR + 1, k)];
+// CleanRing coefficients, each a sequential sum over the ring; thread = (slice,
+// d, ring, pole)
+__global__ void __launch_bounds__(256) k_ring_se_coef(GV g, RingP rp, const DClock *__restrict__ clk,
+                                                     const double *__restrict__ tab, const double *__restrict__ mag,
+                                                     double *__restrict__ scr) {
+    const int Np = g.Nkp, ns = g.Nip + 1;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ns * 2 * rp.nring * 2 || clk->halt) return;
+    const int s = (int)(idx % ns);
+    long r = idx / ns;
+    const int d = (int)(r % 2);
+    r /= 2;
+    const int nR = 1 + (int)(r % rp.nring), pole = (int)(r / rp.nring);
+    const SmoothEIdx x = smoothe_idx(g, s, d, nR, pole);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    constexpr int NB = 16;   // loads issued ahead of the (sequential) sums
+    for (int n0 = 0; n0 < Np; n0 += NB) {
+        double B[NB], C[NB], S[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int n = n0 + u < Np ? n0 + u : Np - 1;
+            B[u] = smoothe_flux(g, x, mag, n);
+            C[u] = tab[n];
+            S[u] = tab[Np + n];
         }
-        dEr[0][n] = 0.0; dEr[1][n] = 0.0;
-        cs[n] = tab[n]; sn[n] = tab[Np + n];
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            if (n0 + u < Np) {
+                s0 = s0 + B[u] * 1.0;
+                s1 = s1 + B[u] * 0.0;
+                s2 = s2 + B[u] * C[u];
+                s3 = s3 + B[u] * S[u];
+            }
+        }
     }
-    __syncthreads();
-    const int fD[2] = {1, 0}, eD[2] = {0, 1};
-    const double eS[2] = {1.0, -1.0};
-    for (int d = 0; d < 2; d++) {
-        const double *B = bF[fD[d]];
-        for (int t = RING_T0; t < 4; t += RING_NT) {   // CleanRing coefficients, each a sequential sum over the ring
-            const double *W = t == 2 ? cs : sn;
-            double s = 0.0;
-            if (t < 2) {
-                const double w = t == 0 ? 1.0 : 0.0;
-#pragma unroll 8
-                for (int n = 0; n < Np; n++) s = s + B[n] * w;
+    const double aScl = 2.0 / Np;
+    const size_t hc = Np / 2;
+    double *cf = scr + (size_t)2 * rp.nring * 2 * hc * ns + ((size_t)((pole * rp.nring + nR - 1) * 2 + d) * 4) * ns + s;
+    cf[0] = 0.5 * aScl * s0;
+    cf[ns] = 0.5 * aScl * s1;
+    cf[2 * (size_t)ns] = 1.0 * aScl * s2;
+    cf[3 * (size_t)ns] = 1.0 * aScl * s3;
+}
+
+// the cleaned flux of cell n
+__device__ __forceinline__ double smoothe_clean(const double *cf, size_t ns, double B, double c, double sn) {
+    const double q = B - cf[0] * 1.0 - cf[ns] * 0.0;
+    return q - cf[2 * ns] * c - cf[3 * ns] * sn;
+}
+
+// chunk means of the cleaned fluxes (each summed in order of k); thread =
+// (slice, chunk, d, ring, pole)
+__global__ void __launch_bounds__(256) k_ring_se_mean(GV g, RingP rp, const DClock *__restrict__ clk,
+                                                     const double *__restrict__ tab, const double *__restrict__ mag,
+                                                     double *__restrict__ scr) {
+    const int Np = g.Nkp, ns = g.Nip + 1, hc = Np / 2;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ns * hc * 2 * rp.nring * 2 || clk->halt) return;
+    const int s = (int)(idx % ns);
+    long r = idx / ns;
+    const int c = (int)(r % hc);
+    r /= hc;
+    const int d = (int)(r % 2);
+    r /= 2;
+    const int nR = 1 + (int)(r % rp.nring), pole = (int)(r / rp.nring);
+    const int NumCh = rp.nch[nR - 1];
+    if (c >= NumCh) return;
+    const int dC = Np / NumCh;
+    const SmoothEIdx x = smoothe_idx(g, s, d, nR, pole);
+    const int pr = (pole * rp.nring + nR - 1) * 2 + d;
+    const double *cf = scr + (size_t)2 * rp.nring * 2 * hc * ns + (size_t)pr * 4 * ns + s;
+    double sum = 0.0;
+    for (int m = 0; m < dC; m++) {
+        const int n = c * dC + m;
+        sum = sum + smoothe_clean(cf, ns, smoothe_flux(g, x, mag, n), tab[n], tab[Np + n]);
+    }
+    scr[((size_t)pr * hc + c) * ns + s] = sum / dC;
+}
+
+// reconstruct, DirtyRing, the flux change, and its EMF summed along the chunk
+// (zero at the chunk's first edge) into SE; thread = (slice, chunk, d, ring,
+// pole).  The k = ke+1 edges repeat k = ks (a chunk's first edge: zero).
+__global__ void __launch_bounds__(256) k_ring_se_emf(GV g, RingP rp, const DClock *__restrict__ clk,
+                                                    const double *__restrict__ tab, const double *__restrict__ mag,
+                                                    const double *__restrict__ scr, double *__restrict__ SE) {
+    const int Np = g.Nkp, ns = g.Nip + 1, hc = Np / 2;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)ns * hc * 2 * rp.nring * 2 || clk->halt) return;
+    const int s = (int)(idx % ns);
+    long r = idx / ns;
+    const int c = (int)(r % hc);
+    r /= hc;
+    const int d = (int)(r % 2);
+    r /= 2;
+    const int nR = 1 + (int)(r % rp.nring), pole = (int)(r / rp.nring);
+    const int NumCh = rp.nch[nR - 1];
+    if (c >= NumCh) return;
+    const int dC = Np / NumCh;
+    const SmoothEIdx x = smoothe_idx(g, s, d, nR, pole);
+    const int pr = (pole * rp.nring + nR - 1) * 2 + d;
+    const double *cf = scr + (size_t)2 * rp.nring * 2 * hc * ns + (size_t)pr * 4 * ns + s;
+    const double *mean = scr + (size_t)pr * hc * ns + s;
+    const double eS = d == 0 ? 1.0 : -1.0;
+    double *E = SE + (d == 0 ? 0 : g.nc);
+    const int lRs = c * dC;
+    const RingParab pb = ring_recon_chunk(mean, NumCh, c, ns);
+    const double c0 = cf[0], c1 = cf[ns], c2 = cf[2 * (size_t)ns], c3 = cf[3 * (size_t)ns];
+    double cum = 0.0;
+    constexpr int NB = 8;   // loads issued ahead of the (sequential) chunk sum
+    for (int m0 = 1; m0 <= dC; m0 += NB) {
+        double B[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++) B[u] = smoothe_flux(g, x, mag, lRs + (m0 + u <= dC ? m0 + u : dC) - 1);
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+            const int m = m0 + u;
+            if (m > dC) break;
+            const int n = lRs + m - 1;
+            double q = ring_recon_eval(pb, m, dC) + c0 * 1.0 + c1 * 0.0;
+            q = q + c2 * tab[n] + c3 * tab[Np + n];
+            const double dF = q - B[u];
+            const int e = ci(g, g.nG + s, x.j, g.nG + n);
+            if (m == 1) {
+                E[e] = 0.0;
+                cum = -dF;
             } else {
-#pragma unroll 8
-                for (int n = 0; n < Np; n++) s = s + B[n] * W[n];
-            }
-            const double aScl = 2.0 / Np;
-            cF[t] = (t < 2 ? 0.5 * aScl : 1.0 * aScl) * s;
-        }
-        __syncthreads();
-        for (int n = RING_T0; n < Np; n += RING_NT) {
-            double q = B[n] - cF[0] * 1.0 - cF[1] * 0.0;
-            rF[n] = q - cF[2] * cs[n] - cF[3] * sn[n];
-        }
-        __syncthreads();
-        ring_chunk_means(rF, mean, NumCh, Np);
-        for (int n = RING_T0; n < Np; n += RING_NT) {   // reconstruct, DirtyRing, the flux change
-            double q = ring_recon_cell(mean, NumCh, Np, n) + cF[0] * 1.0 + cF[1] * 0.0;
-            q = q + cF[2] * cs[n] + cF[3] * sn[n];
-            rF[n] = q - B[n];
-        }
-        __syncthreads();
-        for (int c = RING_T0; c < NumCh; c += RING_NT) {
-            const int lRs = c * dC;
-            double cum = -rF[lRs];
-            for (int m = 2; m <= dC; m++) {
-                dEr[eD[d]][lRs + m - 1] = eS[d] * cum;
-                cum = cum - rF[lRs + m - 1];
+                E[e] = eS * cum;
+                cum = cum - dF;
             }
         }
-        __syncthreads();
     }
-    const int jI = pole ? je - nR + 1 : js + nR, jJ = pole ? je - nR + 1 : js + nR - 1;
-    for (int n = RING_T0; n < Np; n += RING_NT) {
-        SE[ci(g, nS, jI, ks + n)] = dEr[0][n];
-        SE[nc + ci(g, nS, jJ, ks + n)] = dEr[1][n];
-    }
-    if (RING_T0 == 0) {   // the k = ke+1 edges repeat k = ks
-        SE[ci(g, nS, jI, ks + Np)] = dEr[0][0];
-        SE[nc + ci(g, nS, jJ, ks + Np)] = dEr[1][0];
-    }
+    if (c == 0) E[ci(g, g.nG + s, x.j, g.nG + Np)] = 0.0;
 }
 
 // fields.F90:18-51 E2Flux with dt = 1 and E = SE, in place, on the faces whose

@@ -1230,9 +1230,18 @@ __global__ void __launch_bounds__(256) k_bc_face(GVT<R> g, const DClock *__restr
     const int n1 = ext[t1], n2 = ext[t2];
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)g.nG * n1 * n2 || clk->halt) return;
-    const int a = (int)(idx % n1);
-    const long r = idx / n1;
-    const int n = 1 + (int)(r % g.nG), b = (int)(r / g.nG);
+    int a, n, b;
+    if (ax == 0) {   // i ghosts: the layer fastest, so that neighbouring lanes share cache lines
+        n = 1 + (int)(idx % g.nG);
+        const long r = idx / g.nG;
+        a = (int)(r % n1);
+        b = (int)(r / n1);
+    } else {
+        a = (int)(idx % n1);
+        const long r = idx / n1;
+        n = 1 + (int)(r % g.nG);
+        b = (int)(r / g.nG);
+    }
     const int act[3] = {g.Nip, g.Njp, g.Nkp};
     const int lo = g.nG, hi = g.nG + act[ax] - 1;
     const size_t nc = g.nc, nf = g.nf;

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 CSRC = os.path.join(ROOT, "kaiju_amd", "csrc")
 OUT = os.path.join(HERE, "_build")
-SRCS = [os.path.join(CSRC, f) for f in ("gam_api.hip", "gam_step.hip", "gam_setup.hip", "gam_common.h")] + \
+# every source the step library's translation unit can include
+SRCS = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".h"))) + \
        [os.path.join(HERE, "gam_emu.h"), os.path.join(ROOT, "include", "gamera_hip.h")]
 
 
