@@ -125,7 +125,7 @@ def blast_wave(n: int = 256, steps: int = 100, warp_amp: float = 0.05, B0: float
 
 def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in: float = 2.0, r_out: float = 30.0,
                   theta0: Optional[float] = None, Ca_kms: float = 1000.0, axis: bool = True,
-                  wind: Optional[bool] = None, **mkw) -> CaseSpec:
+                  wind: Optional[bool] = None, chunks: Optional[Tuple[int, ...]] = None, **mkw) -> CaseSpec:
     """Magnetosphere-like block (kaiju_amd/msph.py): stretched spherical-polar
     grid r in [2, 30] R_E around the x axis, Earth dipole -31000 nT R_E^3 as the
     background field, solar wind rho = 5 cm^-3, v = (-400, 0, 0) km/s (at rest
@@ -134,7 +134,8 @@ def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in:
 
     axis = True (the reference's LFM-grid layout): theta spans [theta0, pi -
     theta0] with theta0 = 1e-3 by default, the j faces take the LFM axis BCs and
-    the rings next to the axis are averaged with the default chunking for Np.
+    the rings next to the axis are averaged with the default chunking for Np
+    (or chunks: the chunks of each ring, innermost first).
     axis = False: a wedge over [theta0, pi - theta0] (default theta0 = 0.15)
     with zero-gradient theta walls.  wind (default: axis): the outer radial face
     drives the wind in (WindBC with this constant wind) instead of SphOut."""
@@ -154,7 +155,8 @@ def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in:
                     model=m, steps=steps, background=dict(M0=msph.dipole_moment(), xc=(0.0, 0.0, 0.0),
                                                           bu=(0.0, 0.0, 0.0)),
                     bcs=("sphere", "wind" if wind else "sphere", wall, wall, "periodic", "periodic"),
-                    ring=ring_chunks(nphi) if axis else None, wind=(rho, p, v, b) if wind else None)
+                    ring=(tuple(chunks) if chunks else ring_chunks(nphi)) if axis else None,
+                    wind=(rho, p, v, b) if wind else None)
 
 
 # The two workloads named by the benchmark configuration list.

@@ -34,8 +34,8 @@ from tests.helpers import app_from_oracle, compare_state, say
 NG = 4
 
 
-def _oracle_case(N, axis=True, theta0=None, ring=True):
-    spec = configs.magnetosphere(N, steps=4, precision=0, axis=axis, theta0=theta0)
+def _oracle_case(N, axis=True, theta0=None, ring=True, chunks=None):
+    spec = configs.magnetosphere(N, steps=4, precision=0, axis=axis, theta0=theta0, chunks=chunks)
     if not ring:
         spec.ring = None
     p = spec.ic_params
@@ -245,8 +245,8 @@ def test_ring_average_conserves_ring_totals_and_divb():
 
 
 # ----------------------------------------------------------------- GPU
-def _hatched(N, precision=0):
-    spec, oc = _oracle_case(N)
+def _hatched(N, precision=0, chunks=None):
+    spec, oc = _oracle_case(N, chunks=chunks)
     if precision:
         spec.model.precision = precision
     oc.metric()
@@ -257,6 +257,23 @@ def _hatched(N, precision=0):
     oc.state.magFlux[...] = mag
     oc.hatch_finish()
     return spec, oc
+
+
+@pytest.mark.gpu
+def test_axis_block_uneven_chunks_matches_oracle():
+    """Axis block 12x12x24 with three rings of 4, 8 and 6 chunks (chunk widths
+    6, 3 and 4 cells, and a ring length no multiple of 16): 4 steps from the
+    oracle's hatched state."""
+    say("[ring] msph axis 12x12x24 chunks (4, 8, 6) vs oracle")
+    spec, oc = _hatched((12, 12, 24), chunks=(4, 8, 6))
+    with app_from_oracle(oc) as app:
+        assert app.get_ring() == (4, 8, 6)
+        assert app.steps(spec.steps) == 0
+        oc.step(spec.steps)
+        e = compare_state(app, oc, 0)
+        say(f"   after {spec.steps}: {e}")
+        assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-10, e
+        assert abs(app.Model.dt - oc.m.dt) <= 1e-12 * oc.m.dt
 
 
 @pytest.mark.gpu
