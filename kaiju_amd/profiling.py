@@ -10,6 +10,9 @@ rocprofv3 CSV output.
       per-stage HBM bytes per launch from two --pmc runs (FETCH_SIZE, WRITE_SIZE,
       KiB per dispatch), each scaled by what the same counter reads for the
       calibration copy; writes <out.json> (read by bench.py) and prints markdown.
+  python -m kaiju_amd.profiling counters <counter_collection.csv> "<command>"
+      per-stage mean per launch of every counter of one --pmc run, with the
+      VALU-issue time SQ_INSTS_VALU implies when it holds SQ_INSTS_VALU.
 """
 from __future__ import annotations
 
@@ -111,6 +114,47 @@ def traffic(fetch_csv, write_csv, cf_csv, cw_csv, gib, out_json, command) -> str
     return "\n".join(lines) + "\n"
 
 
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its SIMD for
+# 4 cycles at the 2.4 GHz peak engine clock (FP64 included: 78.6 TFLOP/s of
+# FMA = 1024 SIMDs x 64 lanes x 2 flops x 2.4 GHz / 4)
+N_SIMD = 1024
+CLOCK_HZ = 2.4e9
+
+
+def counters(csv_path: str, command: str) -> str:
+    rows = list(csv.DictReader(open(csv_path)))
+    names = sorted({r["Counter_Name"] for r in rows})
+    per = defaultdict(lambda: defaultdict(float))   # dispatch -> counter -> value
+    kern = {}
+    for r in rows:
+        d = r["Dispatch_Id"]
+        per[d][r["Counter_Name"]] += float(r["Counter_Value"])
+        kern[d] = r["Kernel_Name"]
+    acc = defaultdict(lambda: [0, defaultdict(float)])
+    for d, vals in per.items():
+        st = stage_of(kern[d])
+        if not st:
+            continue
+        acc[st][0] += 1
+        for c, v in vals.items():
+            acc[st][1][c] += v
+    valu = "SQ_INSTS_VALU" in names
+    head = "| stage | launches | " + " | ".join(names) + (" | VALU issue (ms) |" if valu else " |")
+    lines = ["# rocprofv3 counters per launch", "", "Command:", "", "```", command, "```", ""]
+    if valu:
+        lines += [f"VALU issue time = SQ_INSTS_VALU x 4 cycles / ({N_SIMD} SIMDs x {CLOCK_HZ / 1e9:.1f} GHz): "
+                  "the time the step's vector instructions alone would take with every SIMD issuing "
+                  "one each 4 cycles.", ""]
+    lines += [head, "|---|---:|" + "---:|" * (len(names) + (1 if valu else 0))]
+    for st in sorted(acc):
+        n, tot = acc[st]
+        cells = [f"{tot[c] / n:.4g}" for c in names]
+        if valu:
+            cells.append(f"{tot['SQ_INSTS_VALU'] / n * 4 / (N_SIMD * CLOCK_HZ) * 1e3:.3f}")
+        lines.append(f"| {st} | {n} | " + " | ".join(cells) + " |")
+    return "\n".join(lines) + "\n"
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "copy":
@@ -119,6 +163,8 @@ if __name__ == "__main__":
         print(trace_summary(sys.argv[2], sys.argv[3]), end="")
     elif cmd == "traffic":
         print(traffic(*sys.argv[2:9]), end="")
+    elif cmd == "counters":
+        print(counters(sys.argv[2], sys.argv[3]), end="")
     else:
         print(__doc__)
         sys.exit(2)
