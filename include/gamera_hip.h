@@ -171,7 +171,10 @@ int  gam_set_wind(gam_handle *h, double rho, double p, const double *v, const do
  * averaged in nch[r-1] chunks along k.  Call before gam_set_corners, which then
  * applies RingGridFix to the geometry it derives (gam_set_grid takes geometry
  * as given).  Needs an even Nkp in 16..512, Njp >= 2*nring + 4, chunk counts
- * dividing Nkp and at most Nkp/2.  nring = 0 turns it off. */
+ * dividing Nkp and at most Nkp/2.  nring = 0 turns it off.  Allocates device
+ * memory held until gam_destroy: the smoothing EMF (3 G doubles) and a ring
+ * scratch of about 5 nring Nkp Nip doubles (grown, never shrunk, when called
+ * again with more rings). */
 int  gam_set_ring(gam_handle *h, int nring, const int *nch);
 int  gam_get_ring(gam_handle *h, int *nring, int *nch);
 
