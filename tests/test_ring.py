@@ -208,6 +208,67 @@ def test_ring_average_keeps_a_uniform_field_and_gas():
     assert e_mag <= 1e-13 and e_gas <= 1e-14
 
 
+def _random_state(oc, rng):
+    """Random gas and a divergence-free random field (the discrete curl of
+    random edge values) on the oracle case's arrays."""
+    G, F = oc.G, oc.F
+    gas = np.zeros(G + (5,), order="F")
+    gas[..., 0] = 1.0 + 0.5 * rng.random(G)
+    gas[..., 1:4] = rng.normal(size=G + (3,)) * 0.3
+    gas[..., 4] = 2.0 + rng.random(G)
+    Lx = rng.normal(size=F + (3,))
+    mag = np.zeros(F + (3,), order="F")
+    s = lambda a, di, dj, dk: np.roll(a, (-di, -dj, -dk), axis=(0, 1, 2))
+    mag[..., 0] = Lx[..., 1] + s(Lx[..., 2], 0, 1, 0) - s(Lx[..., 1], 0, 0, 1) - Lx[..., 2]
+    mag[..., 1] = s(Lx[..., 0], 0, 0, 1) - Lx[..., 0] - s(Lx[..., 2], 1, 0, 0) + Lx[..., 2]
+    mag[..., 2] = s(Lx[..., 1], 1, 0, 0) - Lx[..., 1] - s(Lx[..., 0], 0, 1, 0) + Lx[..., 0]
+    return gas, mag
+
+
+def test_ring_average_sets_ring_fluxes_to_their_smoothed_values():
+    """After the ring average, each ring's I face fluxes (its own row) and J
+    face fluxes (the row above it, away from the axis) hold the smoothed ring:
+    the m = 0 and m = 1 Fourier modes over phi_n = dp (n - 1/2) taken out, the
+    rest replaced by the PPM reconstruction of its chunk means, the modes put
+    back.  Computed here with numpy sums and the separately pinned
+    orc_reconstruct_ring, so a wrong sign, row or chunk sum of the smoothing
+    EMF or of its curl fails."""
+    spec, oc = _oracle_case((12, 12, 32))
+    oc.metric()
+    gas, mag = _random_state(oc, np.random.default_rng(5))
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    _ring_average(oc)
+    L = oracle.lib()
+    g = oc.g
+    js, je = g.js - g.jsg, g.je - g.jsg
+    Nip, Np = spec.N[0], spec.N[2]
+    dp = 2 * np.pi / Np
+    phi = dp * np.arange(1, Np + 1) - 0.5 * dp
+    I = slice(NG, NG + Nip + 1)
+    K = slice(NG, NG + Np)
+    for r, Nc in enumerate(spec.ring, start=1):
+        for pole in (0, 1):
+            rows = ((0, js + r - 1), (1, js + r)) if pole == 0 else ((0, je - r + 1), (1, je - r + 1))
+            for comp, j in rows:
+                B = mag[I, j, K, comp]
+                a0 = B.mean(axis=1, keepdims=True)
+                a1 = 2.0 / Np * (B * np.cos(phi)).sum(axis=1, keepdims=True)
+                b1 = 2.0 / Np * (B * np.sin(phi)).sum(axis=1, keepdims=True)
+                modes = a0 + a1 * np.cos(phi) + b1 * np.sin(phi)
+                means = (B - modes).reshape(Nip + 1, Nc, Np // Nc).mean(axis=2)
+                want = np.empty_like(B)
+                for i in range(Nip + 1):
+                    q = np.ascontiguousarray(np.repeat(means[i], Np // Nc))
+                    L.orc_reconstruct_ring(q.ctypes.data_as(C.POINTER(C.c_double)), Nc, Np)
+                    want[i] = q
+                want += modes
+                got = oc.state.magFlux[I, j, K, comp]
+                err = np.abs(got - want).max() / np.abs(B).max()
+                assert err <= 1e-12, (r, pole, comp, err)
+                assert np.abs(got - B).max() > 1e-3 * np.abs(B).max()   # it did smooth
+
+
 def test_ring_average_conserves_ring_totals_and_divb():
     spec, oc = _oracle_case((12, 12, 32))
     oc.metric()
