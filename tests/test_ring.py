@@ -272,19 +272,7 @@ def test_ring_average_sets_ring_fluxes_to_their_smoothed_values():
 def test_ring_average_conserves_ring_totals_and_divb():
     spec, oc = _oracle_case((12, 12, 32))
     oc.metric()
-    rng = np.random.default_rng(3)
-    G, F = oc.G, oc.F
-    gas = np.zeros(G + (5,), order="F")
-    gas[..., 0] = 1.0 + 0.5 * rng.random(G)
-    gas[..., 1:4] = rng.normal(size=G + (3,)) * 0.3
-    gas[..., 4] = 2.0 + rng.random(G)
-    # a divergence-free random field: the discrete curl of random edge values
-    Lx = rng.normal(size=F + (3,))
-    mag = np.zeros(F + (3,), order="F")
-    s = lambda a, di, dj, dk: np.roll(a, (-di, -dj, -dk), axis=(0, 1, 2))
-    mag[..., 0] = Lx[..., 1] + s(Lx[..., 2], 0, 1, 0) - s(Lx[..., 1], 0, 0, 1) - Lx[..., 2]
-    mag[..., 1] = s(Lx[..., 0], 0, 0, 1) - Lx[..., 0] - s(Lx[..., 2], 1, 0, 0) + Lx[..., 2]
-    mag[..., 2] = s(Lx[..., 1], 1, 0, 0) - Lx[..., 1] - s(Lx[..., 0], 0, 1, 0) + Lx[..., 0]
+    gas, mag = _random_state(oc, np.random.default_rng(3))
     oc.state.Gas[...] = gas
     oc.state.magFlux[...] = mag
     div0 = _cell_div(oc.state.magFlux.copy())
