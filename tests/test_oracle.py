@@ -285,6 +285,92 @@ def test_bflux2fld_uniform_field_on_affine_cells():
     assert np.allclose(Bx[:-1, :-1, :-1], B, rtol=0, atol=1e-13)
 
 
+def test_calc_efield_uniform_state_is_minus_v_cross_b():
+    """CalcElecField (fields.F90:70-232) of a uniform state on affine cells: on
+    every active edge the EMF times the edge length is -(v x B) . t times the
+    length, t the unit edge direction taken from the corners (the diffusive
+    term vanishes with the jump of a uniform B).  Pins the composed
+    Mom2Face / GetCornerV / GetCornerD / GetCornerB / EdgeLRs chain and the
+    sign and frame of the ideal term."""
+    A = np.array([[1.1, 0.3, -0.1], [0.2, 0.9, 0.25], [-0.15, 0.1, 1.05]])
+    oc = _affine_case(A)
+    g = oc.grid
+    rho, v, p, B = 1.3, np.array([0.4, -0.7, 0.25]), 0.8, np.array([0.3, -1.2, 0.7])
+    gam = oc.spec.model.gamma
+    for st in (oc.state, oc.ostate):
+        st.Gas[..., 0] = rho
+        st.Gas[..., 1:4] = rho * v
+        st.Gas[..., 4] = 0.5 * rho * v @ v + p / (gam - 1)
+        for d in range(3):
+            st.magFlux[..., d] = g["face"][..., d] * np.einsum("...x,x->...", g["Tf"][..., 0:3, d], B)
+        st.Bxyz[...] = B
+    oc.m.dt = 1.0e-3
+    Vf = np.zeros(oc.G + (3,), order="F")
+    E = np.zeros(oc.G + (3,), order="F")
+    L.orc_calc_efield(C.byref(oc.m), C.byref(oc.g), C.byref(oc.state.s), C.byref(oc.ostate.s),
+                      oracle.ptr(Vf), oracle.ptr(E))
+    xyz = g["xyz"]
+    vxb = np.cross(v, B)
+    n = 0
+    for eD in range(3):
+        e = np.zeros(3, dtype=int)
+        e[eD] = 1
+        nG = oc.spec.nG
+        act = [slice(nG, nG + oc.spec.N[a] + (a != eD)) for a in range(3)]
+        x0 = xyz[act[0], act[1], act[2]]
+        x1 = xyz[act[0].start + e[0]:act[0].stop + e[0], act[1].start + e[1]:act[1].stop + e[1],
+                 act[2].start + e[2]:act[2].stop + e[2]]
+        seg = x1 - x0   # edge vector: length times unit direction
+        want = -(seg @ vxb)
+        got = E[act[0], act[1], act[2], eD]
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max(), eD
+        n += got.size
+    assert n > 0
+
+
+def test_calc_stress_uniform_state_is_the_exact_flux():
+    """Fluxes (stress.F90:204-429) of a uniform state on affine cells: every
+    interface carries area x the exact Euler flux (rho vn, rho v vn + p n,
+    (E + p) vn) and area x the Maxwell stress (|B|^2/2 n - B (B.n)), n the
+    face's unit normal; the divergence of these fluxes vanishes cell by cell.
+    Pins the composed stencil loads, reconstructions, kinetic fluxes, HOGS
+    terms (zero jumps) and the face scaling of each direction."""
+    A = np.array([[1.1, 0.3, -0.1], [0.2, 0.9, 0.25], [-0.15, 0.1, 1.05]])
+    oc = _affine_case(A)
+    g = oc.grid
+    rho, v, p, B = 1.3, np.array([0.4, -0.7, 0.25]), 0.8, np.array([0.3, -1.2, 0.7])
+    gam = oc.spec.model.gamma
+    En = 0.5 * rho * v @ v + p / (gam - 1)
+    for st in (oc.state, oc.ostate):
+        st.Gas[..., 0] = rho
+        st.Gas[..., 1:4] = rho * v
+        st.Gas[..., 4] = En
+        for d in range(3):
+            st.magFlux[..., d] = g["face"][..., d] * np.einsum("...x,x->...", g["Tf"][..., 0:3, d], B)
+        st.Bxyz[...] = B
+    oc.m.dt = 1.0e-3
+    gF = np.zeros(oc.G + (5, 3), order="F")
+    mF = np.zeros(oc.G + (3, 3), order="F")
+    dH = np.zeros(oc.G + (5,), order="F")
+    dM = np.zeros(oc.G + (3,), order="F")
+    L.orc_calc_stress(C.byref(oc.m), C.byref(oc.g), C.byref(oc.state.s), C.byref(oc.ostate.s),
+                      oracle.ptr(gF), oracle.ptr(mF), oracle.ptr(dH), oracle.ptr(dM))
+    nG = oc.spec.nG
+    scale = max(rho * np.abs(v).max() ** 2 + p, B @ B) * g["face"].max()
+    for d in range(3):
+        act = tuple(slice(nG, nG + oc.spec.N[a] + (a == d)) for a in range(3))
+        n = g["Tf"][act + (slice(0, 3), d)]
+        area = g["face"][act + (d,)][..., None]
+        vn = (n @ v)[..., None]
+        want_g = np.concatenate([rho * vn, rho * v * vn + p * n, (En + p) * vn], axis=-1) * area
+        want_m = (0.5 * (B @ B) * n - B * (n @ B)[..., None]) * area
+        assert np.abs(gF[act + (slice(None), d)] - want_g).max() <= 1e-13 * scale, d
+        assert np.abs(mF[act + (slice(None), d)] - want_m).max() <= 1e-13 * scale, d
+    cells = tuple(slice(nG, nG + oc.spec.N[a]) for a in range(3))
+    assert np.abs(dH[cells]).max() <= 1e-12 * scale / g["volume"].min()
+    assert np.abs(dM[cells]).max() <= 1e-12 * scale / g["volume"].min()
+
+
 # ----------------------------------------------------------- time step pieces
 def test_celldt_uniform_state_closed_form():
     spec = configs.orszag_tang(16, 4)
