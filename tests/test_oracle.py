@@ -6,7 +6,9 @@ Each check comes from outside the oracle's own formulas:
   * closed forms: reconstruction weights reproduce polynomials exactly, the
     kinetic flux splitting sums to the exact Euler / Maxwell flux when the two
     sides agree, CellDT of a uniform state, AB2 exact on linear data, the
-    metric of affine cells;
+    metric of affine cells, and a uniform state's EMFs (-v x B along each
+    edge) and interface fluxes (exact Euler flux and Maxwell stress) through
+    the composed CalcElecField and Fluxes on sheared cells;
   * invariants: div.B preserved to rounding by constrained transport, mass and
     momentum conservation on a periodic box, point symmetry of Orszag-Tang
     (reference tests/cases/testCases.pf:184-225);
@@ -14,6 +16,10 @@ Each check comes from outside the oracle's own formulas:
   * the 1D advection accuracy bar of tests/cases/testCases.pf:115-146.
 A plausible slip (a dropped term, a wrong sign or index, a transposed stencil)
 fails at least one of them.
+
+Parity partially pinned: the diffusive EMF term Vd0 vDiff Jd and its vDiff cap
+(fields.F90:204-221) are zero for a uniform field and are checked only through
+the whole-step invariants above.
 """
 import ctypes as C
 import math
