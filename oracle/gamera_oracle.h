@@ -11,9 +11,8 @@
  *   face arrays  (isg:ieg+1, jsg:jeg+1, ksg:keg+1, ncomp)   "F"
  * with isg = 1-nG, ieg = Nip+nG (reference base/types/gamtypes.F90:163-245).
  *
- * Parity partially pinned: CalcElecField's diffusive term Vd0 vDiff Jd and its
- * vDiff cap (checked only through whole-step invariants); every other part has
- * an independent check in tests/ (listed in tests/test_oracle.py).
+ * Every part has an independent check in tests/ (listed in the docstring of
+ * tests/test_oracle.py).
  */
 #ifndef GAMERA_ORACLE_H
 #define GAMERA_ORACLE_H

@@ -9,6 +9,8 @@ Each check comes from outside the oracle's own formulas:
     metric of affine cells, and a uniform state's EMFs (-v x B along each
     edge) and interface fluxes (exact Euler flux and Maxwell stress) through
     the composed CalcElecField and Fluxes on sheared cells;
+  * physics that fixes a sign: the diffusive EMF of a field step lies on the
+    step's edges with the sign of the current, capped at CFL dx/dt;
   * invariants: div.B preserved to rounding by constrained transport, mass and
     momentum conservation on a periodic box, point symmetry of Orszag-Tang
     (reference tests/cases/testCases.pf:184-225);
@@ -17,9 +19,6 @@ Each check comes from outside the oracle's own formulas:
 A plausible slip (a dropped term, a wrong sign or index, a transposed stencil)
 fails at least one of them.
 
-Parity partially pinned: the diffusive EMF term Vd0 vDiff Jd and its vDiff cap
-(fields.F90:204-221) are zero for a uniform field and are checked only through
-the whole-step invariants above.
 """
 import ctypes as C
 import math
@@ -332,6 +331,57 @@ def test_calc_efield_uniform_state_is_minus_v_cross_b():
         assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max(), eD
         n += got.size
     assert n > 0
+
+
+@pytest.mark.parametrize("capped", [False, True])
+@pytest.mark.parametrize("comp", ["x", "y"])
+def test_calc_efield_diffusive_term_of_a_field_step(comp, capped):
+    """The diffusive EMF Vd0 vDiff Jd (fields.F90:204-221) at rest on a Cartesian
+    grid, Bx stepping from a to b across the row j0 of y (or By across the
+    column i0 of x; div B = 0): the current J_z = -(b - a)/dy (or +(b - a)/dx)
+    sits on the z edges of that row (column), so only those edges carry an
+    EMF, with the sign of J (a positive resistivity: the curl of E then lowers
+    the upper plateau and raises the lower one), magnitude Vd0 vDiff (b - a)
+    dz, vDiff the Alfven speed of the edge's mean field |a + b|/2 / sqrt(rho),
+    or CFL dz/dt when that is smaller (the cap).  Every other edge, where both
+    sides reconstruct the same plateau, carries none."""
+    oc = _affine_case(np.eye(3))
+    g = oc.grid
+    nG, N = oc.spec.nG, oc.spec.N
+    h = 1.0 / N[0]
+    rho, p, a, b = 1.7, 0.9, 0.4, 1.3
+    s0 = nG + N[1] // 2
+    gam = oc.spec.model.gamma
+    lo = (slice(None), slice(0, s0)) if comp == "x" else (slice(0, s0),)
+    hi = (slice(None), slice(s0, None)) if comp == "x" else (slice(s0, None),)
+    d = 0 if comp == "x" else 1
+    for st in (oc.state, oc.ostate):
+        st.Gas[...] = 0.0
+        st.Gas[..., 0] = rho
+        st.Gas[..., 4] = p / (gam - 1)
+        st.magFlux[...] = 0.0
+        st.magFlux[lo + (Ellipsis, d)] = a * g["face"][lo + (Ellipsis, d)]
+        st.magFlux[hi + (Ellipsis, d)] = b * g["face"][hi + (Ellipsis, d)]
+        st.Bxyz[...] = 0.0
+    vA = 0.5 * abs(a + b) / math.sqrt(rho)
+    m = oc.m
+    m.dt = 10.0 * m.CFL * h / vA if capped else 1.0e-3
+    vDiff = min(vA, m.CFL * h / m.dt)
+    assert (vDiff < vA) == capped
+    Vf = np.zeros(oc.G + (3,), order="F")
+    E = np.zeros(oc.G + (3,), order="F")
+    L.orc_calc_efield(C.byref(m), C.byref(oc.g), C.byref(oc.state.s), C.byref(oc.ostate.s),
+                      oracle.ptr(Vf), oracle.ptr(E))
+    act = [tuple(slice(nG, nG + N[x] + (x != eD)) for x in range(3)) for eD in range(3)]
+    want = np.zeros(oc.G + (3,), order="F")
+    if comp == "x":
+        want[nG:nG + N[0] + 1, s0, nG:nG + N[2], 2] = -m.Vd0 * vDiff * (b - a) * h
+    else:
+        want[s0, nG:nG + N[1] + 1, nG:nG + N[2], 2] = m.Vd0 * vDiff * (b - a) * h
+    for eD in range(3):
+        got = E[act[eD] + (eD,)]
+        assert np.abs(got - want[act[eD] + (eD,)]).max() <= 1e-14, eD
+    assert np.abs(E[act[2] + (2,)]).max() > 0.1 * abs(b - a) * h * m.Vd0 * vDiff
 
 
 def test_calc_stress_uniform_state_is_the_exact_flux():
