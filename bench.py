@@ -241,10 +241,36 @@ def roofline(tim: dict, ms_per_step: float, zups: float):
                                 for k, v in stages.items() if v["bytes"] > 0 and v["ms"] > 0})
 
 
+def host_cpu() -> dict:
+    """lscpu's model, sockets, cores and threads per core, and the logical CPUs
+    this process may run on."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+
+    def num(key):
+        try:
+            return int(info[key])
+        except (KeyError, ValueError):
+            return None
+    sockets, cps, tpc = num("Socket(s)"), num("Core(s) per socket"), num("Thread(s) per core")
+    return dict(model=info.get("Model name") or platform.processor() or platform.machine(),
+                sockets=sockets, cores_per_socket=cps, threads_per_core=tpc,
+                physical_cores=(sockets * cps) if sockets and cps else None,
+                logical_cpus_visible=len(os.sched_getaffinity(0)))
+
+
 def cpu_baseline(n: int, budget_s: float):
     """The oracle (CPU restatement, OpenMP) timed on this host's cores on a
     bounded sample: the same blast-wave problem at n^3, inputs built on the GPU
-    and copied to the oracle, timed over whole steps."""
+    and copied to the oracle, timed over whole steps.  One OpenMP thread per
+    core of the process's CPU share (OMP_NUM_THREADS), bound close to cores."""
     import oracle
     from kaiju_amd import configs
     from kaiju_amd.gamera import GamApp
@@ -270,19 +296,14 @@ def cpu_baseline(n: int, budget_s: float):
     app.steps(nsteps + 1)
     err = compare_state(app, oc, 0)
     app.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    cores = oracle.omp_threads()
     zups = nsteps * spec.zones / wall
-    cpu = platform.processor() or platform.machine()
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                cpu = line.split(":", 1)[1].strip()
-                break
-    except OSError:
-        pass
+    hc = host_cpu()
     return dict(value=round(zups, 1), unit="zone-updates/s", cores=cores, kind="port",
-                sample=f"oracle (C/OpenMP restatement) blast wave warped {n}^3, {nsteps} steps, {wall:.1f} s on {cores} threads of {cpu}",
-                gpu_vs_oracle_max_rel_err=max(err["Gas"], err["magFlux"], err["Bxyz"]))
+                sample=f"oracle (C/OpenMP restatement) blast wave warped {n}^3, {nsteps} steps, {wall:.1f} s on "
+                       f"{cores} OpenMP threads (OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')}, "
+                       f"OMP_PLACES={os.environ.get('OMP_PLACES')}) of {hc['model']}",
+                host=hc, gpu_vs_oracle_max_rel_err=max(err["Gas"], err["magFlux"], err["Bxyz"]))
 
 
 def main():
@@ -299,6 +320,10 @@ def main():
     ap.add_argument("--msph-only", action="store_true",
                     help="run only the magnetosphere leg and print its JSON (profiling; not the bench line)")
     args = ap.parse_args()
+    # the CPU baseline's OpenMP threads: one per core, packed (read when the
+    # oracle library first starts its runtime, after the GPU legs)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))

@@ -260,7 +260,14 @@ int  gam_get_tile(gam_handle *h, int stage, int *tile4);
  * dt); gam_graph_sync waits and returns the largest CalcDT status (a block
  * whose status is nonzero stops where gam_step would have stopped).  Between
  * gam_graph_launch and gam_graph_sync the handles belong to the graph: make no
- * other call on them.  Stage timers must be off while capturing. */
+ * other call on them.  Stage timers must be off while capturing.  The graph
+ * holds each handle's buffers and launch parameters as they were at capture:
+ * until gam_graph_destroy, gam_set_tile, gam_set_ring, gam_set_bcs,
+ * gam_set_wind, gam_set_correction, gam_set_background,
+ * gam_background_dipole and gam_destroy return -1 on a held handle, and
+ * gam_graph_launch returns -1 if a handle that has not halted was stepped an
+ * odd number of times outside the graph (its State and oState buffers are then
+ * swapped against the capture). */
 typedef struct gam_graph gam_graph;
 int  gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_graph **out);
 int  gam_graph_launch(gam_graph *g, int replays);

@@ -77,7 +77,18 @@ struct gam_handle {
     std::vector<hipEvent_t> evpool;
     double ms[ST_N] = {};
     int64_t launches[ST_N] = {};
+    int graph_refs = 0;        // captured graphs holding this handle's buffers and launch parameters
 };
+
+// Calls that would change what a captured graph launches (kernels, their
+// by-value parameters, or buffers it baked in) are refused while one holds h.
+static int refuse_if_captured(gam_handle *h, const char *what) {
+    if (h && h->graph_refs > 0) {
+        h->err = std::string(what) + ": a captured graph holds this handle (gam_graph_destroy it first)";
+        return -1;
+    }
+    return 0;
+}
 
 static int fail(gam_handle *h, int code, const char *fmt, ...) {
     char buf[512];
@@ -276,6 +287,7 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
 
 extern "C" int gam_destroy(gam_handle *h) {
     if (!h) return 0;
+    if (refuse_if_captured(h, "gam_destroy")) return -1;
     // teardown: failures here cannot be acted on, the handle is gone either way
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     collect_timers(h);
@@ -585,6 +597,7 @@ static int bg_alloc(gam_handle *h) {
 extern "C" int gam_set_background(gam_handle *h, const double *fcB0, const double *edgB0, const double *B0,
                                   const double *bFlux0, const double *dpB0) {
     if (!h || !fcB0 || !edgB0 || !B0 || !bFlux0 || !dpB0) return -1;
+    if (refuse_if_captured(h, "gam_set_background")) return -1;
     int rc = bg_alloc(h);
     if (rc) return rc;
     const double *src[5] = {fcB0, edgB0, B0, bFlux0, dpB0};
@@ -609,6 +622,7 @@ extern "C" int gam_get_background(gam_handle *h, double *fcB0, double *edgB0, do
 extern "C" int gam_background_dipole(gam_handle *h, double M0, const double *xc, const double *bu) {
     // background.F90:20-167 AddB0 on the GPU for B0 = bu + dipole(x - xc)
     if (!h || !xc || !bu) return -1;
+    if (refuse_if_captured(h, "gam_background_dipole")) return -1;
     int rc = bg_alloc(h);
     if (rc) return rc;
     const GV &g = h->gv;
@@ -672,6 +686,7 @@ extern "C" int gam_state_device(gam_handle *h, int which, void **Gas, double **m
 
 extern "C" int gam_set_correction(gam_handle *h, const float *o, double scale) {
     if (!h) return -1;
+    if (refuse_if_captured(h, "gam_set_correction")) return -1;
     if (o && !all_periodic(h)) return fail(h, -1, "gam_set_correction: periodic boundaries only");
     int rc = gam_sync(h);
     if (rc) return rc;
@@ -1051,6 +1066,7 @@ static void launch_bc(gam_handle *h, bool withB) {
 
 extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     if (!h || !type) return -1;
+    if (refuse_if_captured(h, "gam_set_bcs")) return -1;
     for (int ax = 0; ax < 3; ax++) {
         const int a = type[2 * ax], b = type[2 * ax + 1];
         if (a < 0 || a > GAM_BC_WIND || b < 0 || b > GAM_BC_WIND)
@@ -1086,6 +1102,7 @@ extern "C" int gam_get_bcs(gam_handle *h, int *type) {
 
 extern "C" int gam_set_wind(gam_handle *h, double rho, double p, const double *v, const double *b) {
     if (!h || !v || !b) return -1;
+    if (refuse_if_captured(h, "gam_set_wind")) return -1;
     if (!(rho > 0) || !(p > 0) || !(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] > 0))
         return fail(h, -1, "gam_set_wind: density, pressure and speed must be positive");
     int rc = gam_sync(h);
@@ -1098,11 +1115,16 @@ extern "C" int gam_set_wind(gam_handle *h, double rho, double p, const double *v
 
 extern "C" int gam_set_ring(gam_handle *h, int nring, const int *nch) {
     if (!h) return -1;
+    if (refuse_if_captured(h, "gam_set_ring")) return -1;
     const GV &g = h->gv;
     int rc = gam_sync(h);
     if (rc) return rc;
-    if (h->corners_set && (nring != (h->ring.on ? h->ring.nring : 0)))
-        return fail(h, -1, "gam_set_ring: call it before gam_set_corners (RingGridFix is part of the geometry)");
+    if (h->corners_set) {   // RingGridFix scaled the geometry by these rings and chunks
+        bool same = nring == (h->ring.on ? h->ring.nring : 0);
+        for (int r = 0; same && r < nring; r++) same = nch && nch[r] == h->ring.nch[r];
+        if (!same)
+            return fail(h, -1, "gam_set_ring: call it before gam_set_corners (RingGridFix is part of the geometry)");
+    }
     if (nring == 0) { h->ring.on = 0; return 0; }
     if (nring < 0 || nring > GAM_RING_MAX || !nch) return fail(h, -1, "gam_set_ring: 1 to %d rings", GAM_RING_MAX);
     if (h->mp.do25D || g.Nkp < 16 || g.Nkp > GAM_RING_NPMAX || (g.Nkp & 1))
@@ -1291,6 +1313,8 @@ extern "C" int gam_get_work(gam_handle *h, int which, double *out) {
 struct gam_graph {
     std::vector<gam_handle *> hs;
     std::vector<int> ts0, s0;
+    std::vector<int> s_cap;              // State index of each handle when captured
+    std::vector<double *> partial_cap;   // its block-minima buffer baked into the graph
     int nsteps = 0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -1307,6 +1331,8 @@ static int gfail(gam_graph *g, int code, const char *msg, hipError_t e) {
 extern "C" int gam_graph_destroy(gam_graph *g) {
     if (!g) return 0;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->exec)
+        for (auto h : g->hs) h->graph_refs--;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     for (auto e : g->evs) (void)hipEventDestroy(e);
@@ -1369,6 +1395,11 @@ extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_gr
     }
     g->ts0.assign(n, 0);
     g->s0.assign(n, 0);
+    for (int q = 0; q < n; q++) {   // nsteps is even: every handle is back at its capture-time State index
+        g->s_cap.push_back(hs[q]->s);
+        g->partial_cap.push_back(hs[q]->partial);
+        hs[q]->graph_refs++;
+    }
     *out = g;
     return 0;
 }
@@ -1378,11 +1409,19 @@ extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_gr
 extern "C" int gam_graph_launch(gam_graph *g, int replays) {
     if (!g || replays < 0) return -1;
     for (size_t q = 0; q < g->hs.size(); q++) {
+        gam_handle *h = g->hs[q];
         gam_clock c;
-        int rc = gam_get_clock(g->hs[q], &c);   // also waits for the handle's own stream
+        int rc = gam_get_clock(h, &c);   // also waits for the handle's own stream
         if (rc) return rc;
+        // the graph's nodes hold the buffers of the capture-time State index; a
+        // halted handle's kernels return at once, so only a running one must match
+        if (h->partial != g->partial_cap[q] || (!h->clk_host->halt && h->s != g->s_cap[q])) {
+            g->err = "gam_graph_launch: handle " + std::to_string(q) +
+                     " was stepped an odd number of times (or re-tiled) since the capture";
+            return -1;
+        }
         g->ts0[q] = c.ts;
-        g->s0[q] = g->hs[q]->s;
+        g->s0[q] = h->s;
     }
     hipError_t e;
     for (int r = 0; r < replays; r++)
@@ -1403,7 +1442,7 @@ extern "C" int gam_graph_sync(gam_graph *g) {
         int rc = gam_get_clock(h, &c);
         if (rc) return rc;
         const int done = c.ts - g->ts0[q];
-        h->s = (g->s0[q] + done) & 1 ? 1 - g->s0[q] : g->s0[q];
+        h->s = (g->s0[q] + done) & 1;   // each step swaps State and oState
         g->ts0[q] = c.ts;
         g->s0[q] = h->s;
         worst = std::max(worst, c.status);
@@ -1434,6 +1473,7 @@ extern "C" int gam_get_timing(gam_handle *h, int stage, double *ms_total, int64_
 
 extern "C" int gam_set_tile(gam_handle *h, int stage, int ti, int tj, int tk, int fast) {
     if (!h) return -1;
+    if (refuse_if_captured(h, "gam_set_tile")) return -1;
     if (stage < 0 || stage >= ST_N || !tiled_stage(stage))
         return fail(h, -1, "gam_set_tile: stage %d takes no tile", stage);
     if (ti < 1 || tj < 1 || tk < 1 || ti * tj * tk != 256 || fast < 0 || fast > (TILE_LINEAR | TILE_NOXCD))

@@ -278,7 +278,9 @@ class GamApp:
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
-            lib().gam_destroy(self.h)
+            rc = lib().gam_destroy(self.h)
+            if rc < 0:   # a captured graph still holds the handle
+                raise HipStepError(f"gam_destroy: rc={rc} {lib().gam_last_error(self.h).decode()}")
             self.h = None
 
     def __del__(self):

@@ -106,6 +106,7 @@ def lib():
             "orc_ppm_intlr": (None, [C.c_double] * 5 + [_dp, _dp]),
             "orc_lfm_map": (None, [G, C.c_int, C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3),
             "orc_wgt_wind": (C.c_double, [M, _dp]),
+            "orc_omp_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -113,6 +114,11 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def omp_threads() -> int:
+    """OpenMP threads of the oracle's parallel loops."""
+    return int(lib().orc_omp_threads())
 
 
 def ptr(a: np.ndarray):

@@ -15,8 +15,12 @@
 #include "gamera_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* OpenMP threads the parallel loops below run on (the CPU baseline's core count) */
+int orc_omp_threads(void) { return omp_get_max_threads(); }
 
 #define TINY 1.0e-12                       /* base/kdefs.F90:20 */
 #define HUGEV 1.0e+12

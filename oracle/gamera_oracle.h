@@ -122,6 +122,7 @@ void   orc_step(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o, orc
 int    orc_run_steps(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o,
                      orc_solver *sv, int nsteps, int dt0_after);
 double orc_divb(const orc_grid *g, const orc_state *s);
+int orc_omp_threads(void);
 
 /* stage entry points used by stage-level parity tests */
 void   orc_predictor(const orc_model *m, const orc_grid *g, const orc_state *s,

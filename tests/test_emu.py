@@ -55,3 +55,16 @@ def test_ring_kernels_on_cpu_emulation_and_in_bounds():
     env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
     out = _run(lib, env, "axis", 900, "test_ring.py")
     assert " passed" in out
+
+
+def test_graph_replays_on_cpu_emulation_and_in_bounds():
+    """Captured-graph replays: eager pre-steps, a halt then a second launch,
+    and the setters a held handle refuses (the emulation records and replays
+    the captured launches)."""
+    lib = build_emu.build(asan=False)
+    out = _run(lib, {}, "graph", 600, "test_ensemble.py")
+    assert " passed" in out
+    lib = build_emu.build(asan=True)
+    env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
+    out = _run(lib, env, "graph_after or graph_run_after", 900, "test_ensemble.py")
+    assert " passed" in out

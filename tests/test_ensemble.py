@@ -128,3 +128,79 @@ def test_autotune_restores_state_and_changes_no_result():
     assert autotune.tune(c, cache=cache) == best
     for x in (a, b, c):
         x.close()
+
+
+@pytest.mark.gpu
+def test_graph_after_an_odd_eager_prestep():
+    """A member stepped once eagerly before the capture (State in the second
+    buffer set) replays in step with its twin, and a later odd eager step on it
+    makes the next launch refuse instead of stepping swapped buffers."""
+    from kaiju_amd.gamera import HipStepError
+    specs = small_members(1)[:2]
+    say("[ensemble] capture after an eager step")
+    twins = eager_twins(specs)
+    ens = Ensemble.__new__(Ensemble)
+    ens.specs, ens.nsteps, ens.apps, ens.tiles, ens.graph = specs, 2, eager_twins(specs), {}, None
+    try:
+        for a, t in zip(ens.apps, twins):
+            assert a.steps(1) == 0 and t.steps(1) == 0
+        ens._capture()
+        for rep in range(2):
+            assert ens.run(2) == 0
+            for t in twins:
+                assert t.steps(2) == 0
+            for a, t in zip(ens.apps, twins):
+                same(a, t)
+        assert ens.apps[0].steps(1) == 0
+        with pytest.raises(HipStepError):
+            ens.launch(1)
+        assert ens.apps[0].steps(1) == 0   # even again: the graph runs
+        assert ens.run(2) == 0
+        assert twins[0].steps(4) == 0 and twins[1].steps(2) == 0
+        for a, t in zip(ens.apps, twins):
+            same(a, t)
+    finally:
+        ens.close()
+        for t in twins:
+            t.close()
+
+
+@pytest.mark.gpu
+def test_graph_run_after_a_halt():
+    """A member that halts after an odd number of steps keeps its State through
+    a second launch (its kernels return at once), the other member goes on."""
+    specs = small_members(1)[:2]
+    say("[ensemble] second launch after a halt")
+    twins = eager_twins(specs)
+    with Ensemble(specs, tune=False, nsteps=2) as ens:
+        for x in (ens.apps[0], twins[0]):
+            c = x.get_clock()
+            c.dt0 = c.dt * 1e6
+            x.set_clock(c)
+        assert ens.run(2) == 1
+        assert twins[0].steps(2) == 1 and twins[1].steps(2) == 0
+        for a, t in zip(ens.apps, twins):
+            same(a, t)
+        assert ens.run(2) == 1
+        assert twins[1].steps(2) == 0
+        assert ens.apps[0].Model.ts == 1 and ens.apps[1].Model.ts == 4
+        for a, t in zip(ens.apps, twins):
+            same(a, t)
+    for t in twins:
+        t.close()
+
+
+@pytest.mark.gpu
+def test_graph_refuses_changes_to_held_handles():
+    from kaiju_amd.gamera import HipStepError
+    specs = small_members(1)[:1]
+    with Ensemble(specs, tune=False, nsteps=2) as ens:
+        a = ens.apps[0]
+        with pytest.raises(HipStepError):
+            a.set_tile("flux_j", 32, 8, 1, 0)
+        with pytest.raises(HipStepError):
+            a.set_bcs(("zerograd",) * 6)
+        assert ens.run(2) == 0
+    b = eager_twins(specs)[0]   # released: the same calls work
+    b.set_tile("flux_j", 32, 8, 1, 0)
+    b.close()

@@ -863,3 +863,222 @@ def test_zero_background_reproduces_the_plain_step():
     assert a.step(3) == 0 and b.step(3) == 0
     for k in ("Gas", "magFlux", "Bxyz"):
         assert np.array_equal(getattr(a.state, k), getattr(b.state, k)), k
+
+
+# ----------------------------------------------------------- Boris correction, alpha > TINY
+def _boris(m, U0, B2, B0, dGH, dGM, Ca, dt):
+    """CellBoris on the Reynolds-updated state U0 + dt dGH (mhdgroup.F90:113-126)."""
+    U2 = _arr(list(U0 + dt * dGH))
+    L.orc_cell_boris(C.byref(m), U2, _arr(list(U0)), _arr(list(B2)), _arr(list(B0)), _arr(list(dGH)),
+                     _arr(list(dGM)), Ca, dt)
+    return np.array(U2[:])
+
+
+def _bhat(B2, B0):
+    B1 = 0.5 * (B2 + B0)
+    return B1 / np.linalg.norm(B1), B1
+
+
+@pytest.mark.parametrize("alpha", [3e-3, 0.5, 7.0, 2e3])
+def test_boris_parallel_momentum_moves_by_the_gas_force_alone(alpha):
+    """mhdgroup.F90:375-379: along b1 = B(n+1/2)/|B| the momentum changes by the
+    parallel gas force only, whatever the magnetic force and Ca (the Boris
+    split slows only the perpendicular response)."""
+    rng = np.random.default_rng(11)
+    m = _model(gamma=5.0 / 3.0)
+    for _ in range(6):
+        U0 = np.array([1.3, 0.2, -0.1, 0.3, 2.5])
+        B2, B0 = rng.normal(size=3), rng.normal(size=3)
+        dGH, dGM = rng.normal(size=5), 5 * rng.normal(size=3)
+        dt = 1e-2
+        b1, B1 = _bhat(B2, B0)
+        D2 = U0[0] + dt * dGH[0]
+        Ca = math.sqrt(B1 @ B1 / (D2 * alpha))        # alpha = |B1|^2 / (D2 Ca^2)
+        U = _boris(m, U0, B2, B0, dGH, dGM, Ca, dt)
+        dM = U[1:4] - U0[1:4]
+        assert abs(dM @ b1 - dt * dGH[1:4] @ b1) <= 1e-13 * (1 + np.abs(dt * dGM).max()), (alpha, dM @ b1)
+        assert abs(U[0] - D2) <= 1e-15 * D2
+        # the perpendicular response does see the magnetic force (the branch ran)
+        perp = lambda x: x - (x @ b1) * b1
+        assert np.linalg.norm(perp(dM) - perp(dt * dGH[1:4])) > 1e-6
+
+
+def test_boris_heavy_field_keeps_perpendicular_velocity():
+    """alpha >> 1 (Ca far below the Alfven speed): the field's inertia pins the
+    velocity across b1 at its old value, V2_perp = V0_perp + O(1/alpha), while
+    along b1 the gas force acts in full (mhdgroup.F90:375-379 with
+    pCon -> 0, alpha pCon -> 1)."""
+    rng = np.random.default_rng(12)
+    m = _model(gamma=5.0 / 3.0)
+    for _ in range(6):
+        U0 = np.array([0.9, 0.4, 0.7, -0.2, 3.0])
+        B2, B0 = rng.normal(size=3), rng.normal(size=3)
+        dGH, dGM = rng.normal(size=5), 5 * rng.normal(size=3)
+        dt = 1e-2
+        b1, B1 = _bhat(B2, B0)
+        D2 = U0[0] + dt * dGH[0]
+        alpha = 1e8
+        U = _boris(m, U0, B2, B0, dGH, dGM, math.sqrt(B1 @ B1 / (D2 * alpha)), dt)
+        perp = lambda x: x - (x @ b1) * b1
+        V0, V2 = U0[1:4] / U0[0], U[1:4] / U[0]
+        assert np.linalg.norm(perp(V2) - perp(V0)) <= 1e-7, np.linalg.norm(perp(V2) - perp(V0))
+        assert abs((V2 - V0) @ b1) > 1e-4          # not frozen along the field
+
+
+def test_boris_light_field_is_maxwell_without_parallel_magnetic_force():
+    """alpha just above TINY (mhdgroup.F90:375-379 with pCon -> 1, alpha pCon -> 0):
+    CellBoris equals CellMaxwell driven by the magnetic force with its
+    component along b1 removed, to O(alpha)."""
+    rng = np.random.default_rng(13)
+    m = _model(gamma=5.0 / 3.0)
+    for _ in range(6):
+        U0 = np.array([1.1, -0.3, 0.2, 0.5, 2.2])
+        B2, B0 = rng.normal(size=3), rng.normal(size=3)
+        dGH, dGM = rng.normal(size=5), 5 * rng.normal(size=3)
+        dt = 1e-2
+        b1, B1 = _bhat(B2, B0)
+        D2 = U0[0] + dt * dGH[0]
+        alpha = 1e-9
+        U = _boris(m, U0, B2, B0, dGH, dGM, math.sqrt(B1 @ B1 / (D2 * alpha)), dt)
+        dGMp = dGM - (dGM @ b1) * b1
+        W = _arr(list(U0 + dt * dGH))
+        L.orc_cell_maxwell(C.byref(m), W, _arr(list(dGMp)), dt)
+        assert np.allclose(U, np.array(W[:]), rtol=1e-7, atol=1e-9), (U, W[:])
+        # and differs from the full Maxwell update by the parallel force
+        X = _arr(list(U0 + dt * dGH))
+        L.orc_cell_maxwell(C.byref(m), X, _arr(list(dGM)), dt)
+        assert np.abs(U[1:4] - np.array(X[1:4]) + dt * (dGM @ b1) * b1).max() <= 1e-7
+
+
+# ----------------------------------------------------------- HOGS with jumps, the Boris nuke
+def _cart_case(N=(16, 8, 8)):
+    return _affine_case(np.eye(3), N)
+
+
+def _calc_stress(oc):
+    gF = np.zeros(oc.G + (5, 3), order="F")
+    mF = np.zeros(oc.G + (3, 3), order="F")
+    dH = np.zeros(oc.G + (5,), order="F")
+    dM = np.zeros(oc.G + (3,), order="F")
+    L.orc_calc_stress(C.byref(oc.m), C.byref(oc.g), C.byref(oc.state.s), C.byref(oc.ostate.s),
+                      oracle.ptr(gF), oracle.ptr(mF), oracle.ptr(dH), oracle.ptr(dM))
+    return gF, mF
+
+
+def _set_state(oc, rho, v, p, B):
+    """rho, p: arrays over the cells (G); v, B: 3-vectors or (G,3) arrays; the
+    face fluxes are the cell field's flux on the Cartesian grid."""
+    g = oc.grid
+    gam = oc.spec.model.gamma
+    v = np.broadcast_to(v, oc.G + (3,))
+    B = np.broadcast_to(B, oc.G + (3,))
+    for st in (oc.state, oc.ostate):
+        st.Gas[..., 0] = rho
+        st.Gas[..., 1:4] = rho[..., None] * v
+        st.Gas[..., 4] = 0.5 * rho * (v * v).sum(-1) + p / (gam - 1)
+        st.Bxyz[...] = B
+        st.magFlux[...] = 0.0
+        for d in range(3):
+            st.magFlux[:-1, :-1, :-1, d] = g["face"][:-1, :-1, :-1, d] * B[..., d]
+
+
+@pytest.mark.parametrize("ab", [(1.0, 4.0), (3.0, 0.5)])
+def test_hogs_at_a_stationary_contact(ab):
+    """A density jump at rest in pressure balance with a uniform field through
+    the interface (stress.F90:371-396).  The 7UP + PDM reconstruction is first
+    order at the jump (L = left cell, R = right cell), so the interface mass
+    flux has the closed form
+        F_rho = (rho_L / sqrt(lam_L) - rho_R / sqrt(lam_R)) / (2 sqrt(pi))
+                - cHogH VaD (rho_R - rho_L),   VaD = (|B|/sqrt(rho_L) + |B|/sqrt(rho_R)) / 2,
+    lam = rho / (2 p): both terms carry mass down the jump, the second with the
+    Alfven speed of the two sides; the normal momentum flux is p and the
+    energy flux (E + p/2) (Vn1_L + Vn1_R)."""
+    a, b = ab
+    oc = _cart_case()
+    nG, N = oc.spec.nG, oc.spec.N
+    i0 = nG + N[0] // 2          # the jump is the lower face of cell i0
+    ii = np.arange(oc.G[0])[:, None, None] * np.ones(oc.G)
+    rho = np.where(ii < i0, a, b)
+    p0, Bx = 0.6, 1.7
+    _set_state(oc, rho, np.zeros(3), np.full(oc.G, p0), np.array([Bx, 0.0, 0.0]))
+    m = oc.m
+    gF, mF = _calc_stress(oc)
+    gam = oc.spec.model.gamma
+    p = (gam - 1) * (p0 / (gam - 1))          # block_c2p's pressure of the stored energy
+    lamL, lamR = a / (2 * p), b / (2 * p)
+    sp = math.sqrt(math.pi)
+    VaD = 0.5 * (Bx / math.sqrt(a) + Bx / math.sqrt(b))
+    want_rho = (a / math.sqrt(lamL) - b / math.sqrt(lamR)) / (2 * sp) - m.cHogH * VaD * (b - a)
+    Vn1 = 0.5 / (sp * math.sqrt(lamL)) - 0.5 / (sp * math.sqrt(lamR))
+    want_E = (p / (gam - 1) + 0.5 * p) * Vn1
+    h2 = oc.grid["face"][i0, nG, nG, 0]
+    sl = (i0, slice(nG, nG + N[1]), slice(nG, nG + N[2]))
+    assert np.abs(gF[sl + (0, 0)] / h2 - want_rho).max() <= 1e-14 * (a + b), (gF[sl + (0, 0)][0, 0] / h2, want_rho)
+    assert np.abs(gF[sl + (1, 0)] / h2 - p).max() <= 1e-14
+    assert np.abs(gF[sl + (2, 0)]).max() <= 1e-15 and np.abs(gF[sl + (3, 0)]).max() <= 1e-15
+    assert np.abs(gF[sl + (4, 0)] / h2 - want_E).max() <= 1e-14 * p
+    assert np.abs(mF[sl + (0, 0)] / h2 + 0.5 * Bx * Bx).max() <= 1e-14 * Bx * Bx
+    # mass runs from the dense side to the light side
+    assert np.sign(gF[sl + (0, 0)]).max() == np.sign(a - b) == np.sign(gF[sl + (0, 0)]).min()
+
+
+def test_boris_magnetic_hogs_at_a_shear_layer():
+    """A tangential velocity jump (vy: u_L -> u_R across an x interface) with
+    uniform density, pressure and field, Boris on: the magnetic HOGS term of
+    stress.F90:335-351 adds -cHogM VaD |B|^2 (u_R - u_L)/Ca^2 to the y
+    component of the Maxwell flux -Bx By, VaD = sqrt(Va^2 / (1 + Va^2/Ca^2))."""
+    oc = _cart_case()
+    nG, N = oc.spec.nG, oc.spec.N
+    i0 = nG + N[0] // 2
+    uL, uR = -0.4, 0.9
+    ii = np.arange(oc.G[0])[:, None, None] * np.ones(oc.G)
+    v = np.zeros(oc.G + (3,))
+    v[..., 1] = np.where(ii < i0, uL, uR)
+    rho, p0, B = 1.4, 0.8, np.array([0.9, -0.6, 0.3])
+    _set_state(oc, np.full(oc.G, rho), v, np.full(oc.G, p0), B)
+    m = oc.m
+    m.doBoris, m.Ca = 1, 1.1
+    gF, mF = _calc_stress(oc)
+    Va2 = B @ B / rho
+    VaD = math.sqrt(Va2 / (1 + Va2 / m.Ca ** 2))
+    want_y = -B[0] * B[1] - m.cHogM * VaD * (B @ B) * (uR - uL) / m.Ca ** 2
+    want_z = -B[0] * B[2]
+    h2 = oc.grid["face"][i0, nG, nG, 0]
+    sl = (i0, slice(nG, nG + N[1]), slice(nG, nG + N[2]))
+    assert np.abs(mF[sl + (1, 0)] / h2 - want_y).max() <= 1e-13, (mF[sl + (1, 0)][0, 0] / h2, want_y)
+    assert np.abs(mF[sl + (2, 0)] / h2 - want_z).max() <= 1e-13
+    # away from the layer (uniform stencils) the term vanishes
+    far = (nG + 1, slice(nG, nG + N[1]), slice(nG, nG + N[2]))
+    assert np.abs(mF[far + (1, 0)] / h2 + B[0] * B[1]).max() <= 1e-13
+
+
+@pytest.mark.parametrize("ratio,fires", [(1.6, True), (1.4, False)])
+def test_boris_nuke_adds_ca_times_the_cell_jump(ratio, fires):
+    """stress.F90:399-414: where the interface speed sqrt((|vL|^2 + |vR|^2)/2)
+    is at least 1.5 Ca, the gas flux gains -Ca (U_i - U_{i-1}) of the half-step
+    conserved state.  Field-free with a uniform velocity u along x (its
+    reconstruction is exact: vL = vR = u), so Boris changes nothing else: the
+    flux with Boris minus the flux without is the nuke term, on every x
+    interface when u >= 1.5 Ca and nowhere when u < 1.5 Ca.  The density is
+    quadratic in x, so each interface's cell jump is its own."""
+    oc = _cart_case()
+    nG, N = oc.spec.nG, oc.spec.N
+    u, p0 = 2.0, 0.5
+    ii = np.arange(oc.G[0])[:, None, None] * np.ones(oc.G)
+    rho = 1.0 + 0.05 * ii + 0.01 * ii ** 2
+    _set_state(oc, rho, np.array([u, 0.0, 0.0]), np.full(oc.G, p0), np.zeros(3))
+    g0, _ = _calc_stress(oc)
+    oc.m.doBoris, oc.m.Ca = 1, u / ratio
+    g1, _ = _calc_stress(oc)
+    U = oc.ostate.Gas
+    act = (slice(nG, nG + N[0] + 1), slice(nG, nG + N[1]), slice(nG, nG + N[2]))
+    h2 = oc.grid["face"][nG, nG, nG, 0]
+    dCon = U[nG:nG + N[0] + 1, nG:nG + N[1], nG:nG + N[2], :] - U[nG - 1:nG + N[0], nG:nG + N[1], nG:nG + N[2], :]
+    diff = (g1[act + (slice(None), 0)] - g0[act + (slice(None), 0)]) / h2
+    want = -oc.m.Ca * dCon if fires else np.zeros_like(dCon)
+    assert np.abs(diff - want).max() <= 1e-13, np.abs(diff - want).max()
+    if fires:
+        assert np.abs(diff[..., 0]).min() > 0.04
+    # y and z interfaces see no jump: unchanged either way
+    for d in (1, 2):
+        assert np.abs(g1[..., d] - g0[..., d]).max() <= 1e-13
