@@ -73,7 +73,8 @@ def build_net(verbose: bool = False) -> str:
     return _build_so(["gam_net.hip"], NET_LIB, verbose)
 
 
-FORTRAN_MODULES = ["gamera_hip_c.F90", "gamtypes_gpu.F90", "gamapp_gpu.F90", "gamprob_gpu.F90"]
+FORTRAN_MODULES = ["gamera_hip_c.F90", "basetypes_gpu.F90", "gamtypes_gpu.F90", "gamapp_gpu.F90",
+                   "gamtypessub_gpu.F90", "gamprob_gpu.F90"]
 
 
 def build_fortran(verbose: bool = False, lib: str = HIP_LIB, exe: str = None) -> str:

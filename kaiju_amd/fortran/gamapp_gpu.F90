@@ -1,8 +1,8 @@
 !> Per-step app routines over the GPU step, in the reference's order:
 !> stepGamera = UpdateStateData (AdvanceMHD, ts += 1, t += dt) -> CalcDT ->
-!> EnforceBCs + bFlux2Fld (reference src/gamera/gamapp.F90:73-116), and
-!> gamAdvanceModel (reference src/gamera/gamtypessub.F90:78-94), the call the
-!> Voltron coupler and gamera.x make.
+!> EnforceBCs + bFlux2Fld (reference src/gamera/gamapp.F90:73-116); the
+!> gamApp_T bindings over them (AdvanceModel, the call the Voltron coupler and
+!> gamera.x make) are in the submodule gamtypessub_gpu.
 module gamapp_gpu
     use, intrinsic :: iso_c_binding
     use, intrinsic :: iso_fortran_env, only: output_unit, error_unit, int64
@@ -123,18 +123,6 @@ contains
         endif
         call gpuCheck(App, gam_enforce_bcs(App%gpu), "gam_enforce_bcs")
     end subroutine stepGamera
-
-    !> gamtypessub.F90:78-94
-    subroutine gamAdvanceModel(App, dt)
-        type(gamApp_T), intent(inout) :: App
-        real(rp), intent(in) :: dt
-        real(rp) :: targetSimT
-        targetSimT = App%Model%t + dt
-        call stepGamera(App)
-        do while (App%Model%t < targetSimT .and. App%Model%status == 0)
-            call stepGamera(App)
-        enddo
-    end subroutine gamAdvanceModel
 
     !> output.F90:47-85: dt0 capture at the first output after ts > 0, kZCs
     subroutine consoleOutput(App)

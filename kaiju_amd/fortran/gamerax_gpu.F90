@@ -1,13 +1,16 @@
 !> gamera.x-style driver on the GPU step (reference src/drivers/gamerax.F90:30-62):
 !>   gamerax_gpu <ot|bw> <Nip> <Njp> <Nkp> <nsteps>
-!> Each loop iteration advances the model by its current dt through
-!> gamAdvanceModel, then writes console output (dt0 capture, kZCs).  The last
+!> Each loop iteration advances the model by its current dt through the
+!> type-bound AdvanceModel, called on a class(BaseApp_T) dummy as the Voltron
+!> coupler calls it (reference voltron/modelInterfaces/gamCplTypessub.F90:129,146),
+!> then writes console output (dt0 capture, kZCs).  The last
 !> lines are the run's summary: time, step count, dt, totals of the State
 !> arrays over the active cells and div.B.
 program gamerax_gpu
     use, intrinsic :: iso_c_binding
     use, intrinsic :: iso_fortran_env, only: output_unit
     use gamera_hip_c
+    use basetypes_gpu
     use gamtypes_gpu
     use gamapp_gpu
     use gamprob_gpu
@@ -30,9 +33,9 @@ program gamerax_gpu
     write(output_unit, '(a,a,3(1x,i0),a,es24.16)') 'hatched ', trim(problem), Nip, Njp, Nkp, ' dt = ', gApp%Model%dt
 
     do n = 1, nsteps
-        call gamAdvanceModel(gApp, gApp%Model%dt)
+        call coupledAdvance(gApp, gApp%Model%dt)
         if (gApp%Model%status /= 0) exit
-        call consoleOutput(gApp)
+        call gApp%WriteConsoleOutput()
     enddo
 
     call PullState(gApp)
@@ -51,5 +54,15 @@ program gamerax_gpu
     end associate
     write(output_unit, '(a,es24.16)') 'RESULT divb ', divb
     write(output_unit, '(a,es12.4)') 'RESULT kzcs ', gApp%Model%kzcsMHD
-    call DestroyGPU(gApp)
+    call gApp%Cleanup()
+
+contains
+
+    !> a coupler's view of the model: any BaseApp_T, advanced through its binding
+    subroutine coupledAdvance(App, dt)
+        class(BaseApp_T), intent(inout) :: App
+        real(rp), intent(in) :: dt
+        call App%AdvanceModel(dt)
+    end subroutine coupledAdvance
+
 end program gamerax_gpu

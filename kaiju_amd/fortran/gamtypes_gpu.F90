@@ -6,9 +6,8 @@
 module gamtypes_gpu
     use, intrinsic :: iso_c_binding
     use gamera_hip_c
+    use basetypes_gpu
     implicit none
-
-    integer, parameter :: rp = c_double
     integer, parameter :: NVAR = 5, NDIM = 3, BLK = 0
     real(rp), parameter :: TINY = 1.0D-12, HUGE = 1.0D+12
     real(rp), parameter :: PI = 4.0D0*atan(1.0D0)
@@ -84,13 +83,35 @@ module gamtypes_gpu
         real(rp), dimension(:,:,:,:), allocatable :: Bxyz
     end type State_T
 
-    !> gamtypes.F90:294-314; the solver work arrays live on the GPU
-    type :: gamApp_T
+    !> gamtypes.F90:294-314; the solver work arrays live on the GPU.  The
+    !> bindings' bodies are in the submodule gamtypessub_gpu (they call the step
+    !> routines of gamapp_gpu, which uses this module).
+    type, extends(BaseApp_T) :: gamApp_T
         type(Model_T) :: Model
         type(Grid_T)  :: Grid
         type(State_T) :: State, oState, ooState
         type(c_ptr)   :: gpu = c_null_ptr
+    contains
+        procedure :: AdvanceModel => gamAdvanceModel
+        procedure :: WriteConsoleOutput => gamWriteConsoleOutput
+        procedure :: Cleanup => gamCleanup
     end type gamApp_T
+
+    interface
+        !> gamtypessub.F90:78-94: stepGamera at least once, then until t >= t0 + dt
+        module subroutine gamAdvanceModel(App, dt)
+            class(gamApp_T), intent(inout) :: App
+            real(rp), intent(in) :: dt
+        end subroutine gamAdvanceModel
+        !> output.F90:47-85 consoleOutput
+        module subroutine gamWriteConsoleOutput(App)
+            class(gamApp_T), intent(inout) :: App
+        end subroutine gamWriteConsoleOutput
+        !> releases the GPU step (gamtypessub.F90 gamCleanup has nothing to free)
+        module subroutine gamCleanup(App)
+            class(gamApp_T), intent(inout) :: App
+        end subroutine gamCleanup
+    end interface
 
 contains
 

@@ -107,6 +107,7 @@ def lib():
             "orc_lfm_map": (None, [G, C.c_int, C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3),
             "orc_wgt_wind": (C.c_double, [M, _dp]),
             "orc_omp_threads": (C.c_int, []),
+            "orc_nuke_count": (C.c_long, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

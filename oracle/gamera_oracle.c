@@ -22,6 +22,15 @@
 /* OpenMP threads the parallel loops below run on (the CPU baseline's core count) */
 int orc_omp_threads(void) { return omp_get_max_threads(); }
 
+/* interfaces where the Boris nuke fired since the last reset (a test checks
+ * that a case reaches the branch) */
+static long nuke_fired = 0;
+long orc_nuke_count(int reset) {
+    long n = nuke_fired;
+    if (reset) nuke_fired = 0;
+    return n;
+}
+
 #define TINY 1.0e-12                       /* base/kdefs.F90:20 */
 #define HUGEV 1.0e+12
 static const double PI = 3.14159265358979323846264338327950288;
@@ -929,6 +938,8 @@ static void fluxes_iface(const orc_model *m, const orc_grid *g, const orc_state 
         double vL = nrm3(&PL[1]), vR = nrm3(&PR[1]);
         double Vn = sqrt(sq(vL) + sq(vR)) / (double)sqrtf(2.0f);
         if (Vn >= 1.5 * m->Ca) {
+#pragma omp atomic
+            nuke_fired++;
             for (int v = 0; v < 5; v++) {
                 double dCon = ConB[4][v] - ConB[3][v];
                 gF[v] = gF[v] - m->Ca * dCon;

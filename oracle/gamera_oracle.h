@@ -123,6 +123,7 @@ int    orc_run_steps(orc_model *m, const orc_grid *g, orc_state *s, orc_state *o
                      orc_solver *sv, int nsteps, int dt0_after);
 double orc_divb(const orc_grid *g, const orc_state *s);
 int orc_omp_threads(void);
+long orc_nuke_count(int reset);
 
 /* stage entry points used by stage-level parity tests */
 void   orc_predictor(const orc_model *m, const orc_grid *g, const orc_state *s,

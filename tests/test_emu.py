@@ -68,3 +68,10 @@ def test_graph_replays_on_cpu_emulation_and_in_bounds():
     env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
     out = _run(lib, env, "graph_after or graph_run_after", 900, "test_ensemble.py")
     assert " passed" in out
+
+
+def test_calcdt_failure_branches_on_cpu_emulation():
+    """CalcDT's 10x-drop and dt <= TINY branches against the oracle."""
+    lib = build_emu.build(asan=False)
+    out = _run(lib, {}, "drop or disaster", 300, "test_calcdt.py")
+    assert " passed" in out

@@ -1082,3 +1082,21 @@ def test_boris_nuke_adds_ca_times_the_cell_jump(ratio, fires):
     # y and z interfaces see no jump: unchanged either way
     for d in (1, 2):
         assert np.abs(g1[..., d] - g0[..., d]).max() <= 1e-13
+
+
+def test_nuke_case_reaches_the_branch():
+    """The parity case built for the Boris nuke does fire it on the oracle (in
+    its first step and over the 8 steps the parity tests run), so the GPU
+    comparison of that case covers stress.F90:399-414."""
+    from tests.helpers import small_cases
+    spec = [s for s in small_cases() if s.name == "ot32x32x4_boris_nuke"][0]
+    oc = oracle.OracleCase(spec)
+    L.orc_nuke_count(1)
+    assert oc.step(1) == 0
+    first = L.orc_nuke_count(0)
+    assert oc.step(spec.steps - 1) == 0
+    total = L.orc_nuke_count(1)
+    ni, nj, nk = spec.N
+    interfaces = (ni + 1) * nj * nk + ni * (nj + 1) * nk + ni * nj * (nk + 1)
+    assert 0.05 * interfaces < first < interfaces, (first, interfaces)
+    assert total > first
