@@ -3,11 +3,13 @@
 * ``kaiju_amd/lib/libgamera_hip.so`` -- the HIP step for gfx950 (hipcc).
 * ``kaiju_amd/lib/libgamera_net.so`` -- the learned-correction network
   (BF16 MFMA convolutions, backward and Adam) for gfx950.
+* ``kaiju_amd/lib/libgamera_io.so`` -- the reference's HDF5 grid and restart
+  files on the host (g++ against the HDF5 C library; built when HDF5 is found).
 * ``kaiju_amd/bin/gamerax_gpu`` and ``kaiju_amd/lib/libgamhost.so`` -- the
   Fortran host (amdflang) that keeps the reference's Model/Grid/State types and
   calls the C-ABI.
 
-Usage: ``python -m kaiju_amd.build [hip|fortran|all]``.
+Usage: ``python -m kaiju_amd.build [hip|net|io|fortran|all]``.
 """
 from __future__ import annotations
 
@@ -24,6 +26,9 @@ LIBDIR = os.path.join(PKG, "lib")
 BINDIR = os.path.join(PKG, "bin")
 HIP_LIB = os.path.join(LIBDIR, "libgamera_hip.so")
 NET_LIB = os.path.join(LIBDIR, "libgamera_net.so")
+IO_LIB = os.path.join(LIBDIR, "libgamera_io.so")
+# HDF5 1.10 C library and headers of the image
+HDF5_PREFIXES = [os.environ.get("HDF5_DIR", ""), "/opt/conda", "/usr"]
 
 # one translation unit: gam_api.hip includes gam_step.hip and gam_setup.hip
 HIP_SOURCES = ["gam_api.hip"]
@@ -73,6 +78,34 @@ def build_net(verbose: bool = False) -> str:
     return _build_so(["gam_net.hip"], NET_LIB, verbose)
 
 
+def hdf5_prefix():
+    """Directory holding include/hdf5.h and lib/libhdf5_hl.so, or None."""
+    for p in HDF5_PREFIXES:
+        if p and os.path.exists(os.path.join(p, "include", "hdf5.h")) and \
+                os.path.exists(os.path.join(p, "lib", "libhdf5_hl.so")):
+            return p
+    return None
+
+
+def build_io(verbose: bool = False) -> str:
+    """libgamera_io.so: host C++ over the HDF5 C API (no GPU code)."""
+    pre = hdf5_prefix()
+    if pre is None:
+        raise RuntimeError("HDF5 C library not found (set HDF5_DIR)")
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = IO_LIB + ".tmp"
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(pre, "include"), os.path.join(CSRC, "gam_io.cpp"), "-o", tmp,
+           "-L", os.path.join(pre, "lib"), "-lhdf5_hl", "-lhdf5", "-Wl,-rpath," + os.path.join(pre, "lib"),
+           "-Wl,-soname,libgamera_io.so"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 FORTRAN_MODULES = ["gamera_hip_c.F90", "basetypes_gpu.F90", "gamtypes_gpu.F90", "gamapp_gpu.F90",
                    "gamtypessub_gpu.F90", "gamprob_gpu.F90"]
 
@@ -109,6 +142,8 @@ def build_fortran(verbose: bool = False, lib: str = HIP_LIB, exe: str = None) ->
 def build_all(verbose: bool = False):
     build_hip(verbose)
     build_net(verbose)
+    if hdf5_prefix() is not None:
+        build_io(verbose)
     if os.path.isdir(FSRC) and all(os.path.exists(os.path.join(FSRC, m)) for m in FORTRAN_MODULES):
         build_fortran(verbose)
 
@@ -119,6 +154,8 @@ if __name__ == "__main__":
         build_hip(True)
     elif what == "net":
         build_net(True)
+    elif what == "io":
+        build_io(True)
     elif what == "fortran":
         build_fortran(True)
     else:

@@ -241,6 +241,8 @@ class Model:
         self.ts = 0
         self.tFin = spec.model.tFin
         self.status = 0
+        self.nOut = 0    # Model%IO counters a restart file carries
+        self.nRes = 0
 
 
 class GamApp:
@@ -300,12 +302,14 @@ class GamApp:
         xyz = np.asfortranarray(xyz, dtype=np.float64)
         assert xyz.shape == self.F + (3,)
         self._chk(lib().gam_set_corners(self.h, _p(xyz)), "gam_set_corners")
+        self._xyz = xyz
 
     def set_grid(self, grid: dict):
         names = ("xyz", "xyzcc", "xfc", "face", "Tf", "edge", "Te", "Teb", "volume", "di", "dj", "dk")
         arrs = [np.asfortranarray(grid[n], dtype=np.float64) for n in names]
         self._keep = arrs
         self._chk(lib().gam_set_grid(self.h, *[_p(a) for a in arrs]), "gam_set_grid")
+        self._xyz = arrs[0]
 
     def get_grid(self) -> dict:
         G, F = self.G, self.F
@@ -422,6 +426,47 @@ class GamApp:
                 setattr(c, f, type(getattr(c, f))(z[f"clock_{f}"].item()))
             self.set_clock(c)
         self._sync_model()
+
+    def write_restart_h5(self, path: str):
+        """writeH5Res (gioH5.F90:650-704): the reference's restart file, with
+        State, oState, the corners and the nOut/nRes/ts/dt/dt0 attributes."""
+        from kaiju_amd import gamio
+        c = self.get_clock()
+        xyz = getattr(self, "_xyz", None)
+        if xyz is None:
+            xyz = self.get_grid()["xyz"]
+        ioc = gamio.IOClock(nOut=self.Model.nOut, nRes=self.Model.nRes, ts=c.ts, t=c.stateTime, dt=c.dt,
+                            dt0=c.dt0, ot=c.oStateTime)
+        gamio.write_restart(path, self.spec.N, self.nG, ioc, xyz, self.get_state(0), self.get_state(1),
+                            doMHD=bool(self._params.doMHD))
+
+    def read_restart_h5(self, path: str):
+        """readH5Restart (gioH5.F90:707-872) plus the grid read from the same
+        file (readH5Grid, gioH5.F90:36-124, as a restarting run does): the
+        corners, the spec's background field, State and oState (Bxyz derived
+        from magFlux), and the clock: t = State%time, oState%time, dt, dt0, ts;
+        nRes one past the file's."""
+        from kaiju_amd import gamio
+        N = gamio.grid_dims(path, self.nG)
+        if tuple(N) != tuple(self.spec.N):
+            raise ValueError(f"{path}: restart of a {N} block, this block is {self.spec.N}")
+        self.set_corners(gamio.read_grid(path, self.nG, N))
+        if self.spec.background is not None:
+            b = self.spec.background
+            self.background_dipole(b.get("M0", 0.0), b.get("xc", (0.0, 0.0, 0.0)), b.get("bu", (0.0, 0.0, 0.0)))
+        mhd = bool(self._params.doMHD)
+        ioc, st, ost = gamio.read_restart(path, N, self.nG, doMHD=mhd)
+        self.put_state(0, ioc.t, st["Gas"], st["magFlux"] if mhd else None, None)
+        self.put_state(1, ioc.ot, ost["Gas"], ost["magFlux"] if mhd else None, None)
+        c = self.get_clock()
+        c.t, c.dt, c.dt0, c.ts = ioc.t, ioc.dt, ioc.dt0, ioc.ts
+        c.stateTime, c.oStateTime = ioc.t, ioc.ot
+        c.status, c.ncpr = 0, 0
+        c.dt0_after = self.spec.dt0_after
+        self.set_clock(c)
+        self.Model.nOut, self.Model.nRes = ioc.nOut, ioc.nRes
+        self._sync_model()
+        return ioc
 
     def get_clock(self) -> GamClock:
         c = GamClock()

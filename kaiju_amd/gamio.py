@@ -1,0 +1,143 @@
+"""The reference's HDF5 files for a block: the X/Y/Z corner grid and the
+RunID.gam.Res.#####.h5 restart, through libgamera_io.so (include/gamera_io.h).
+
+* ``grid_dims(path)``, ``read_grid(path)``, ``write_grid(path, xyz)`` --
+  readH5Grid (reference src/gamera/gioH5.F90:36-124): corners with ghost
+  corners, active cells = corner count - 2 nG - 1.
+* ``restart_name(RunID, nRes)`` -- output.F90:155.
+* ``write_restart`` / ``read_restart`` -- writeH5Res / readH5Restart
+  (gioH5.F90:650-872).
+
+Arrays are Fortran-ordered FP64 with ghosts, as ``GamApp.get_state`` returns
+them.  Host code only; no GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+IO_LIB = os.path.join(_HERE, "lib", "libgamera_io.so")
+_lib = None
+_dp = C.POINTER(C.c_double)
+
+
+class GamIOError(RuntimeError):
+    pass
+
+
+class IOClock(C.Structure):
+    _fields_ = [("nOut", C.c_int), ("nRes", C.c_int), ("ts", C.c_int),
+                ("t", C.c_double), ("dt", C.c_double), ("dt0", C.c_double), ("ot", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(IO_LIB):
+        raise GamIOError(f"{IO_LIB} is missing; run python -m kaiju_amd.build io")
+    L = C.CDLL(IO_LIB)
+    sig = {
+        "gam_io_grid_dims": [C.c_char_p, C.c_int, C.POINTER(C.c_int)],
+        "gam_io_read_grid": [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
+        "gam_io_write_grid": [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp],
+        "gam_io_write_restart": [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(IOClock), _dp,
+                                 _dp, _dp, _dp, _dp, _dp, _dp],
+        "gam_io_read_restart": [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(IOClock),
+                                _dp, _dp, _dp, _dp],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.restype = C.c_int
+        f.argtypes = args
+    L.gam_io_last_error.restype = C.c_char_p
+    L.gam_io_last_error.argtypes = []
+    _lib = L
+    return L
+
+
+def _chk(rc: int, what: str):
+    if rc != 0:
+        raise GamIOError(f"{what}: {lib().gam_io_last_error().decode()}")
+
+
+def _fa(a: Optional[np.ndarray], shape) -> Optional[np.ndarray]:
+    if a is None:
+        return None
+    a = np.asfortranarray(a, dtype=np.float64)
+    if a.shape != tuple(shape):
+        raise ValueError(f"array of shape {a.shape}, expected {tuple(shape)}")
+    return a
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _shapes(N, nG):
+    G = tuple(n + 2 * nG for n in N)
+    F = tuple(n + 1 for n in G)
+    return G, F
+
+
+def restart_name(RunID: str, nRes: int) -> str:
+    """output.F90:155: RunID.gam.Res.#####.h5"""
+    return f"{RunID}.gam.Res.{nRes:05d}.h5"
+
+
+def grid_dims(path: str, nG: int = 4) -> Tuple[int, int, int]:
+    d = (C.c_int * 3)()
+    _chk(lib().gam_io_grid_dims(path.encode(), nG, d), "gam_io_grid_dims")
+    return tuple(d)
+
+
+def read_grid(path: str, nG: int = 4, N: Optional[Tuple[int, int, int]] = None) -> np.ndarray:
+    """Corners (Ni+1, Nj+1, Nk+1, 3) of the grid file (or restart) at path."""
+    N = grid_dims(path, nG) if N is None else tuple(N)
+    if min(N) < 1:
+        raise GamIOError(f"{path}: grid of {N} active cells")
+    _, F = _shapes(N, nG)
+    xyz = np.zeros(F + (3,), order="F")
+    _chk(lib().gam_io_read_grid(path.encode(), nG, *N, _p(xyz)), "gam_io_read_grid")
+    return xyz
+
+
+def write_grid(path: str, xyz: np.ndarray, nG: int = 4):
+    N = tuple(s - 1 - 2 * nG for s in xyz.shape[:3])
+    _, F = _shapes(N, nG)
+    xyz = _fa(xyz, F + (3,))
+    _chk(lib().gam_io_write_grid(path.encode(), nG, *N, _p(xyz)), "gam_io_write_grid")
+
+
+def write_restart(path: str, N, nG: int, clock: IOClock, xyz, state: dict, ostate: dict, doMHD: bool = True):
+    """writeH5Res: clock attributes, corners, State and oState (dicts with
+    Gas, magFlux, Bxyz as GamApp.get_state returns)."""
+    G, F = _shapes(N, nG)
+    xyz = _fa(xyz, F + (3,))
+    arrs = [_fa(state["Gas"], G + (5,)), _fa(state.get("magFlux"), F + (3,)), _fa(state.get("Bxyz"), G + (3,)),
+            _fa(ostate["Gas"], G + (5,)), _fa(ostate.get("magFlux"), F + (3,)), _fa(ostate.get("Bxyz"), G + (3,))]
+    _chk(lib().gam_io_write_restart(path.encode(), nG, *N, int(doMHD), C.byref(clock), _p(xyz),
+                                    *[_p(a) for a in arrs]), "gam_io_write_restart")
+
+
+def read_restart(path: str, N, nG: int = 4, doMHD: bool = True, fill: Optional[dict] = None):
+    """readH5Restart: (clock, State dict, oState dict).  ``fill`` supplies the
+    arrays a ghost-less (older) file leaves outside the active cells."""
+    G, F = _shapes(N, nG)
+    fill = fill or {}
+
+    def arr(k, shape):
+        a = fill.get(k)
+        return np.zeros(shape, order="F") if a is None else np.array(a, dtype=np.float64, order="F", copy=True)
+
+    gas, ogas = arr("Gas", G + (5,)), arr("oGas", G + (5,))
+    mag = arr("magFlux", F + (3,)) if doMHD else None
+    omag = arr("omagFlux", F + (3,)) if doMHD else None
+    c = IOClock()
+    _chk(lib().gam_io_read_restart(path.encode(), nG, *N, int(doMHD), C.byref(c), _p(gas), _p(mag), _p(ogas),
+                                   _p(omag)), "gam_io_read_restart")
+    return c, dict(time=c.t, Gas=gas, magFlux=mag), dict(time=c.ot, Gas=ogas, magFlux=omag)

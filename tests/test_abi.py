@@ -9,6 +9,8 @@ HEADER = os.path.join(ROOT, "include", "gamera_hip.h")
 LIB = os.path.join(ROOT, "kaiju_amd", "lib", "libgamera_hip.so")
 NET_HEADER = os.path.join(ROOT, "include", "gamera_net.h")
 NET_LIB = os.path.join(ROOT, "kaiju_amd", "lib", "libgamera_net.so")
+IO_HEADER = os.path.join(ROOT, "include", "gamera_io.h")
+IO_LIB = os.path.join(ROOT, "kaiju_amd", "lib", "libgamera_io.so")
 
 
 def declared_functions(header=HEADER):
@@ -58,3 +60,16 @@ def test_net_library_exports_every_declared_symbol():
     assert not missing, missing
     assert b"gfx950" in open(NET_LIB, "rb").read()
     assert lib.gam_net_num_params() == 27 * (32 * 8 + 32 * 32 + 8 * 32) + 32 + 32 + 8
+
+
+def test_io_library_exports_every_declared_symbol():
+    from kaiju_amd import build
+    if build.hdf5_prefix() is None:
+        import pytest
+        pytest.skip("no HDF5 C library in this image")
+    assert os.path.exists(IO_LIB), "build first: python -m kaiju_amd.build io"
+    lib = ctypes.CDLL(IO_LIB)
+    names = declared_functions(IO_HEADER)
+    assert "gam_io_read_restart" in names and "gam_io_read_grid" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing

@@ -75,3 +75,10 @@ def test_calcdt_failure_branches_on_cpu_emulation():
     lib = build_emu.build(asan=False)
     out = _run(lib, {}, "drop or disaster", 300, "test_calcdt.py")
     assert " passed" in out
+
+
+def test_restart_files_on_cpu_emulation():
+    """Bit-exact resume through the .npz and the reference's HDF5 restart."""
+    lib = build_emu.build(asan=False)
+    out = _run(lib, {}, "restart or grid_file", 600, "test_restart.py")
+    assert " passed" in out

@@ -1,0 +1,252 @@
+"""The reference's HDF5 grid and restart files (kaiju_amd/gamio.py over
+libgamera_io.so).  The layout is checked with HDF5's own h5dump, independently
+of the reader: dataset names, C-order dimensions (the reference's Fortran
+shapes reversed), types, the element order, and the attribute types; files the
+library cannot write (species, ghost-less older restarts) are made here with
+the HDF5 C library directly.  No GPU."""
+import ctypes as C
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from kaiju_amd import build, gamio
+
+HDF5 = build.hdf5_prefix()
+pytestmark = pytest.mark.skipif(HDF5 is None, reason="no HDF5 C library in this image")
+
+N = (6, 5, 3)
+nG = 4
+G = tuple(n + 2 * nG for n in N)
+F = tuple(n + 1 for n in G)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    if not os.path.exists(gamio.IO_LIB) or os.path.getmtime(gamio.IO_LIB) < os.path.getmtime(
+            os.path.join(build.CSRC, "gam_io.cpp")):
+        build.build_io()
+
+
+def h5dump(*args) -> str:
+    exe = shutil.which("h5dump") or os.path.join(HDF5, "bin", "h5dump")
+    return subprocess.run([exe, *args], capture_output=True, text=True, check=True).stdout
+
+
+def _rand(shape, seed):
+    return np.asfortranarray(np.random.default_rng(seed).standard_normal(shape))
+
+
+def _state(seed):
+    return dict(Gas=_rand(G + (5,), seed), magFlux=_rand(F + (3,), seed + 1), Bxyz=_rand(G + (3,), seed + 2))
+
+
+def _clock(**kw):
+    c = gamio.IOClock(nOut=7, nRes=3, ts=1234, t=0.75, dt=1.25e-3, dt0=2.5e-3, ot=0.75 - 1.25e-3)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _dataset_header(dump: str, name: str) -> str:
+    m = re.search(r'DATASET "%s" \{(.*?)\n   \}' % re.escape(name), dump, re.S)
+    assert m, f"no dataset {name}"
+    return m.group(1)
+
+
+def _dims(header: str):
+    m = re.search(r"DATASPACE\s+SIMPLE \{ \( ([0-9, ]+) \)", header)
+    return tuple(int(x) for x in m.group(1).split(","))
+
+
+def test_grid_file_layout_and_round_trip(tmp_path):
+    xyz = _rand(F + (3,), 1)
+    p = str(tmp_path / "grid.h5")
+    gamio.write_grid(p, xyz, nG)
+    dump = h5dump("-H", p)
+    for v in "XYZ":
+        h = _dataset_header(dump, v)
+        assert "H5T_IEEE_F64LE" in h
+        assert _dims(h) == (F[2], F[1], F[0])   # Fortran (Ni+1, Nj+1, Nk+1) seen from C
+    # element order: C index [k][j][i] of Y is Fortran Y(i, j, k)
+    i, j, k = 3, 1, 2
+    out = h5dump("-d", "Y", "-s", f"{k},{j},{i}", "-c", "1,1,1", "-m", "%.17g", p)
+    val = float(re.search(r"\(%d,%d,%d\): ([-0-9.e+]+)" % (k, j, i), out).group(1))
+    assert val == xyz[i, j, k, 1]
+    assert gamio.grid_dims(p, nG) == N
+    back = gamio.read_grid(p, nG)
+    assert np.array_equal(back, xyz)
+
+
+def test_grid_of_another_size_is_refused(tmp_path):
+    p = str(tmp_path / "grid.h5")
+    gamio.write_grid(p, _rand(F + (3,), 2), nG)
+    with pytest.raises(gamio.GamIOError, match="active cells"):
+        gamio.read_grid(p, nG, (N[0] + 1, N[1], N[2]))
+
+
+def test_restart_file_layout(tmp_path):
+    p = str(tmp_path / gamio.restart_name("blast", 3))
+    assert os.path.basename(p) == "blast.gam.Res.00003.h5"   # output.F90:155
+    gamio.write_restart(p, N, nG, _clock(), _rand(F + (3,), 3), _state(10), _state(20))
+    dump = h5dump("-H", "-A", p)
+    # attributes on the root: integers nOut, nRes, ts; reals dt, dt0, t, ot
+    for name, typ in (("nOut", "H5T_STD_I32LE"), ("nRes", "H5T_STD_I32LE"), ("ts", "H5T_STD_I32LE"),
+                      ("dt", "H5T_IEEE_F64LE"), ("dt0", "H5T_IEEE_F64LE"), ("t", "H5T_IEEE_F64LE"),
+                      ("ot", "H5T_IEEE_F64LE")):
+        m = re.search(r'ATTRIBUTE "%s" \{\s*DATATYPE\s+(\S+)\s*DATASPACE\s+SCALAR' % name, dump)
+        assert m and m.group(1) == typ, name
+    for stamp in ("GITHASH", "GITBRANCH", "COMPILER", "COMPILEROPTS", "DATETIME"):
+        assert f'ATTRIBUTE "{stamp}"' in dump
+    shapes = {"X": (F[2], F[1], F[0]), "Gas": (1, 5, G[2], G[1], G[0]), "oGas": (1, 5, G[2], G[1], G[0]),
+              "magFlux": (3, F[2], F[1], F[0]), "omagFlux": (3, F[2], F[1], F[0]),
+              "Bxyz": (3, G[2], G[1], G[0]), "oBxyz": (3, G[2], G[1], G[0])}
+    for name, shp in shapes.items():
+        h = _dataset_header(dump, name)
+        assert _dims(h) == shp, name
+        assert "H5T_IEEE_F64LE" in h
+        assert re.search(r'ATTRIBUTE "Units".*?"CODE"', h, re.S), name
+        assert re.search(r'ATTRIBUTE "Description".*?"DESCRIPTION"', h, re.S), name
+    # Gas(i, j, k, v, 0) sits at C index [0][v][k][j][i]
+    i, j, k, v = 5, 2, 7, 4
+    st = _state(10)
+    out = h5dump("-d", "Gas", "-s", f"0,{v},{k},{j},{i}", "-c", "1,1,1,1,1", "-m", "%.17g", p)
+    val = float(re.search(r"\(0,%d,%d,%d,%d\): ([-0-9.e+]+)" % (v, k, j, i), out).group(1))
+    assert val == st["Gas"][i, j, k, v]
+
+
+def test_restart_round_trip_and_clock(tmp_path):
+    p = str(tmp_path / "r.h5")
+    xyz, s, o = _rand(F + (3,), 4), _state(30), _state(40)
+    gamio.write_restart(p, N, nG, _clock(), xyz, s, o)
+    c, st, ost = gamio.read_restart(p, N, nG)
+    for k in ("Gas", "magFlux"):
+        assert np.array_equal(st[k], s[k]), k
+        assert np.array_equal(ost[k], o[k]), k
+    assert (c.nOut, c.ts, c.t, c.dt, c.dt0, c.ot) == (7, 1234, 0.75, 1.25e-3, 2.5e-3, 0.75 - 1.25e-3)
+    assert c.nRes == 4                     # gioH5.F90:812 nRes + 1
+    assert np.array_equal(gamio.read_grid(p, nG), xyz)   # a restart holds the grid (gioH5.F90:669-671)
+
+
+@pytest.mark.parametrize("dt0, expect", [(5e-12, 0.0), (1e-11, 1e-11), (0.0, 0.0), (3e-4, 3e-4)])
+def test_small_dt0_reads_as_zero(tmp_path, dt0, expect):
+    """dt0 below 10*TINY is written (gioH5.F90:663-667) and read (:838-845) as 0."""
+    p = str(tmp_path / "r.h5")
+    gamio.write_restart(p, N, nG, _clock(dt0=dt0), _rand(F + (3,), 5), _state(1), _state(2))
+    dump = h5dump("-a", "dt0", "-m", "%.17g", p)
+    written = float(re.search(r"\(0\): ([-0-9.e+]+)", dump).group(1))
+    assert written == expect
+    c, _, _ = gamio.read_restart(p, N, nG)
+    assert c.dt0 == expect
+
+
+def test_hydro_restart_has_no_field(tmp_path):
+    p = str(tmp_path / "r.h5")
+    s, o = _state(3), _state(4)
+    gamio.write_restart(p, N, nG, _clock(), _rand(F + (3,), 6), s, o, doMHD=False)
+    dump = h5dump("-H", p)
+    assert 'DATASET "Gas"' in dump and 'DATASET "magFlux"' not in dump and 'DATASET "oBxyz"' not in dump
+    c, st, ost = gamio.read_restart(p, N, nG, doMHD=False)
+    assert np.array_equal(st["Gas"], s["Gas"]) and st["magFlux"] is None
+
+
+# ------------------------------------------------- files written with the HDF5 C library directly
+class _H5:
+    """The few HDF5 C calls needed to write files the library itself never
+    writes (species, ghost-less states, missing pieces)."""
+
+    def __init__(self):
+        self.L = C.CDLL(os.path.join(HDF5, "lib", "libhdf5.so"), mode=C.RTLD_GLOBAL)
+        self.HL = C.CDLL(os.path.join(HDF5, "lib", "libhdf5_hl.so"))
+        self.L.H5open()
+        self.double = C.c_int64.in_dll(self.L, "H5T_NATIVE_DOUBLE_g").value
+        self.int = C.c_int64.in_dll(self.L, "H5T_NATIVE_INT_g").value
+        self.L.H5Fcreate.restype = C.c_int64
+        self.L.H5Fcreate.argtypes = [C.c_char_p, C.c_uint, C.c_int64, C.c_int64]
+        self.L.H5Fclose.argtypes = [C.c_int64]
+        self.L.H5Screate.restype = C.c_int64
+        self.L.H5Sclose.argtypes = [C.c_int64]
+        self.L.H5Acreate2.restype = C.c_int64
+        self.L.H5Acreate2.argtypes = [C.c_int64, C.c_char_p, C.c_int64, C.c_int64, C.c_int64, C.c_int64]
+        self.L.H5Awrite.argtypes = [C.c_int64, C.c_int64, C.c_void_p]
+        self.L.H5Aclose.argtypes = [C.c_int64]
+        self.HL.H5LTmake_dataset_double.argtypes = [C.c_int64, C.c_char_p, C.c_int, C.POINTER(C.c_uint64),
+                                                    C.c_void_p]
+
+    def write(self, path, datasets: dict, attrs: dict):
+        f = self.L.H5Fcreate(path.encode(), 2, 0, 0)   # H5F_ACC_TRUNC, default property lists
+        assert f >= 0
+        for name, (cdims, arr) in datasets.items():
+            arr = np.ascontiguousarray(arr.ravel(order="F"))
+            d = (C.c_uint64 * len(cdims))(*cdims)
+            assert self.HL.H5LTmake_dataset_double(f, name.encode(), len(cdims), d, arr.ctypes.data) >= 0
+        for name, val in attrs.items():
+            s = self.L.H5Screate(0)   # H5S_SCALAR
+            typ, buf = (self.int, C.c_int(val)) if isinstance(val, int) else (self.double, C.c_double(val))
+            a = self.L.H5Acreate2(f, name.encode(), typ, s, 0, 0)
+            assert a >= 0 and self.L.H5Awrite(a, typ, C.byref(buf)) >= 0
+            self.L.H5Aclose(a)
+            self.L.H5Sclose(s)
+        self.L.H5Fclose(f)
+
+
+ATTRS = dict(nOut=1, nRes=2, ts=30, t=0.5, dt=1e-3, ot=0.499)
+
+
+def test_ghostless_restart_fills_the_active_cells(tmp_path):
+    """An older restart without ghosts (gioH5.F90:790-798): Gas Nip wide fills
+    is:ie, js:je, ks:ke only; magFlux is:ie+1, ...; the rest stays as passed."""
+    h5 = _H5()
+    Na = N
+    Fa = tuple(n + 1 for n in Na)
+    g = _rand(Na + (5,), 7)
+    og = _rand(Na + (5,), 8)
+    m = _rand(Fa + (3,), 9)
+    om = _rand(Fa + (3,), 10)
+    p = str(tmp_path / "old.h5")
+    h5.write(p, {"Gas": ((1, 5, Na[2], Na[1], Na[0]), g), "oGas": ((1, 5, Na[2], Na[1], Na[0]), og),
+                 "magFlux": ((3, Fa[2], Fa[1], Fa[0]), m), "omagFlux": ((3, Fa[2], Fa[1], Fa[0]), om)}, ATTRS)
+    fill = dict(Gas=np.full(G + (5,), -7.0, order="F"), oGas=np.full(G + (5,), -8.0, order="F"),
+                magFlux=np.full(F + (3,), -9.0, order="F"), omagFlux=np.full(F + (3,), -10.0, order="F"))
+    c, st, ost = gamio.read_restart(p, N, nG, fill=fill)
+    a = slice(nG, None)
+    assert np.array_equal(st["Gas"][nG:nG + N[0], nG:nG + N[1], nG:nG + N[2]], g)
+    assert np.array_equal(ost["Gas"][nG:nG + N[0], nG:nG + N[1], nG:nG + N[2]], og)
+    assert np.array_equal(st["magFlux"][nG:nG + Fa[0], nG:nG + Fa[1], nG:nG + Fa[2]], m)
+    assert np.array_equal(ost["magFlux"][nG:nG + Fa[0], nG:nG + Fa[1], nG:nG + Fa[2]], om)
+    mask = np.ones(G, bool)
+    mask[nG:nG + N[0], nG:nG + N[1], nG:nG + N[2]] = False
+    assert np.all(st["Gas"][mask] == -7.0) and np.all(ost["Gas"][mask] == -8.0)
+    assert st["magFlux"][0, 0, 0, 0] == -9.0 and st["magFlux"][a][-1, -1, -1, 2] == -9.0
+    assert c.dt0 == 0.0 and c.nRes == 3 and c.ts == 30   # no dt0 in the file: 0 (gioH5.F90:846-848)
+
+
+def test_restart_refusals(tmp_path):
+    h5 = _H5()
+    gshape = (1, 5, G[2], G[1], G[0])
+    mshape = (3, F[2], F[1], F[0])
+    # no oState: "too old" (gioH5.F90:733-736)
+    p = str(tmp_path / "noo.h5")
+    h5.write(p, {"Gas": (gshape, _rand(G + (5,), 1)), "magFlux": (mshape, _rand(F + (3,), 2))}, ATTRS)
+    with pytest.raises(gamio.GamIOError, match="too old"):
+        gamio.read_restart(p, N, nG)
+    # species: Gas's last Fortran dimension 2 (gioH5.F90:781-786)
+    p = str(tmp_path / "spc.h5")
+    two = (2, 5, G[2], G[1], G[0])
+    h5.write(p, {"Gas": (two, _rand(G + (5, 2), 3)), "oGas": (two, _rand(G + (5, 2), 4)),
+                 "magFlux": (mshape, _rand(F + (3,), 5)), "omagFlux": (mshape, _rand(F + (3,), 6))}, ATTRS)
+    with pytest.raises(gamio.GamIOError, match="species"):
+        gamio.read_restart(p, N, nG)
+    # another block size (gioH5.F90:793-796)
+    p = str(tmp_path / "size.h5")
+    gamio.write_restart(p, N, nG, _clock(), _rand(F + (3,), 7), _state(5), _state(6))
+    with pytest.raises(gamio.GamIOError, match="another size"):
+        gamio.read_restart(p, (N[0] + 1, N[1], N[2]), nG)
+    # a grid file is no restart
+    p = str(tmp_path / "grid.h5")
+    gamio.write_grid(p, _rand(F + (3,), 8), nG)
+    with pytest.raises(gamio.GamIOError, match="too old"):
+        gamio.read_restart(p, N, nG)
