@@ -196,16 +196,14 @@ __global__ void __launch_bounds__(256) k_b2f(GVT<R> g, Tile t, const DClock *__r
 __host__ __device__ __forceinline__ long b2f_shell_count(int Ni, int Nj, int Nk, int Nip, int Njp, int Nkp) {
     return (long)(Nk - Nkp) * Ni * Nj + (long)Nkp * (Nj - Njp) * Ni + (long)Nkp * Njp * (Ni - Nip);
 }
+// (i, j, k) of ghost cell idx in that order; false past the shell
 template <typename R>
-__global__ void __launch_bounds__(256) k_b2f_shell(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ mag,
-                                                   R *__restrict__ B) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ bool shell_cell(const GVT<R> &g, long idx, int &i, int &j, int &k) {
     const int Ni = g.Ni, Nj = g.Nj, a0 = g.nG;
     const long nplane = (long)(g.Nk - g.Nkp) * Ni * Nj;
     const long nrow = (long)g.Nkp * (Nj - g.Njp) * Ni;
     const long nig = (long)g.Nkp * g.Njp * (Ni - g.Nip);
-    if (idx >= nplane + nrow + nig || clk->halt) return;
-    int i, j, k;
+    if (idx >= nplane + nrow + nig) return false;
     if (idx < nplane) {
         const int p = (int)(idx / ((long)Ni * Nj)), r = (int)(idx % ((long)Ni * Nj));
         k = p < a0 ? p : p + g.Nkp;
@@ -226,6 +224,15 @@ __global__ void __launch_bounds__(256) k_b2f_shell(GVT<R> g, const DClock *__res
         j = a0 + r % g.Njp;
         k = a0 + r / g.Njp;
     }
+    return true;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_b2f_shell(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ mag,
+                                                   R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (clk->halt || !shell_cell(g, idx, i, j, k)) return;
     const int c = ci(g, i, j, k);
     double b[3];
     cell_bxyz(g, mag, i, j, k, b);
@@ -518,23 +525,17 @@ __device__ __forceinline__ void gaskin(const MP &m, R sg, const R *P5, const R *
     dW[4] = dW[4] + sg * E;
 }
 
-// Interface fluxes in direction DN (stress.F90:204-429) for the interface box
-// DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
-// Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.  BG: the
-// background-field terms are compiled in (a separate instantiation keeps the
-// plain kernel's register count).
+// One interface of direction DN (stress.F90:204-429) at its cell c and face
+// point f: out[0..4] the gas fluxes, out[5..7] the magnetic ones (MHD), each
+// times the face area.  BG: the background-field terms are compiled in (a
+// separate instantiation keeps the plain kernel's register count).
 template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
-__global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
-                                              const R *__restrict__ Wn, const R *__restrict__ nB,
-                                              const R *__restrict__ Wh, const R *__restrict__ hB,
-                                              const R *__restrict__ hmag, const R *__restrict__ hGas,
-                                              R *__restrict__ Flx) {
-    const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
-    int a, b, cz;
-    if (!tile_pos(t, nx, ny, nz, a, b, cz) || clk->halt) return;
-    const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
+__device__ __forceinline__ void face_flux(const GVT<R> &g, const MP &m, const BGV &bg,
+                                          const R *__restrict__ Wn, const R *__restrict__ nB,
+                                          const R *__restrict__ Wh, const R *__restrict__ hB,
+                                          const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                          int c, int f, R *out) {
     const size_t nc = g.nc, nf = g.nf;
-    const int c = ci(g, i, j, k), f = fi(g, i, j, k);
     const int st = cstride(g, DN);
     const R pdmb = R(m.pdmb);
 
@@ -593,7 +594,7 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, co
             }
         }
 #pragma unroll
-        for (int d = 0; d < 3; d++) Flx[nc * (8 * DN + 5 + d) + c] = faB * mF[d];
+        for (int d = 0; d < 3; d++) out[5 + d] = faB * mF[d];
     }
     R gF[5] = {0, 0, 0, 0, 0}, dW[5] = {0, 0, 0, 0, 0};
     gaskin(m, R(-1.0), PL, T, gF, dW);
@@ -613,7 +614,28 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, co
         }
     }
 #pragma unroll
-    for (int v = 0; v < 5; v++) Flx[nc * (8 * DN + v) + c] = faB * gF[v];
+    for (int v = 0; v < 5; v++) out[v] = faB * gF[v];
+}
+
+// Interface fluxes in direction DN for the interface box
+// DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
+// Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+__global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                              const R *__restrict__ Wn, const R *__restrict__ nB,
+                                              const R *__restrict__ Wh, const R *__restrict__ hB,
+                                              const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                              R *__restrict__ Flx) {
+    const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
+    int a, b, cz;
+    if (!tile_pos(t, nx, ny, nz, a, b, cz) || clk->halt) return;
+    const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
+    const size_t nc = g.nc;
+    const int c = ci(g, i, j, k);
+    R o[8];
+    face_flux<DN, RM, BORIS, MHD, BG>(g, m, bg, Wn, nB, Wh, hB, hmag, hGas, c, fi(g, i, j, k), o);
+#pragma unroll
+    for (int v = 0; v < (MHD ? 8 : 5); v++) Flx[nc * (8 * DN + v) + c] = o[v];
 }
 
 // ----------------------------------------------------------------- CT update
@@ -784,6 +806,46 @@ template <typename R> __device__ __forceinline__ bool in_dt_box(const GVT<R> &g,
     return i >= a0 && i <= g.nG + g.Nip && j >= a0 && j <= g.nG + g.Njp && k >= a0 && k <= g.nG + g.Nkp;
 }
 
+// applyUpdate of one active cell (mhdgroup.F90:79-199) from its deltas: the
+// Reynolds update of U (State n in, new state out), then Maxwell or Boris with
+// Bn, the cell field of the new face fluxes (total fields with a background).
+template <int BORIS, int MHD, typename R>
+__device__ __forceinline__ void cell_apply(const MP &m, const BGV &bg, size_t nc, int idx, double dt, const R *__restrict__ nB,
+                                           double *U, const double *Bn, const double *dGH, const double *dGM) {
+    double U0[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) { U0[v] = U[v]; U[v] = U[v] + dt * dGH[v]; }   // CellReynolds, Model%dt
+    if (MHD) {
+        if (BORIS) {
+            double oBv[3] = {(double)nB[idx], (double)nB[nc + idx], (double)nB[2 * nc + idx]};
+            double Bt[3] = {Bn[0], Bn[1], Bn[2]};
+            if (bg.on) {   // mhdgroup.F90:180-183 total fields
+#pragma unroll
+                for (int x = 0; x < 3; x++) { Bt[x] = Bt[x] + bg.B0[nc * x + idx]; oBv[x] = oBv[x] + bg.B0[nc * x + idx]; }
+            }
+            cell_boris(m, U, U0, Bt, oBv, dGH, dGM, m.Ca, dt);
+        } else {
+            cell_maxwell(m, U, dGM, dt);
+        }
+    }
+}
+
+// CellDT of the stored new state (rounded to the bulk type first) with the
+// total field (gridutils.F90:82-86)
+template <int BORIS, typename R>
+__device__ __forceinline__ double cell_dt_stored(const GVT<R> &g, const MP &m, const BGV &bg, int idx, const double *Us,
+                                                 const double *Bs) {
+    double U[5], Bn[3];
+#pragma unroll
+    for (int v = 0; v < 5; v++) U[v] = (double)R(Us[v]);
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        Bn[x] = (double)R(Bs[x]);
+        if (bg.on) Bn[x] = Bn[x] + bg.B0[(size_t)g.nc * x + idx];
+    }
+    return cell_dt<BORIS>(g, m, U, Bn, idx);
+}
+
 // Flux2Deltas + applyUpdate (Reynolds, Bxyz of the new fluxes, Maxwell/Boris)
 // + CellDT min, in FP64 from the bulk-typed inputs.  Writes the new Gas/Bxyz
 // into the O buffers; ghosts copy Gas n.
@@ -824,22 +886,7 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
                     if (bg.on) dGM[v] = dGM[v] + bg.dpB0[nc * v + idx];   // stress.F90:185-188
                 }
             }
-            double U0[5];
-#pragma unroll
-            for (int v = 0; v < 5; v++) { U0[v] = U[v]; U[v] = U[v] + dt * dGH[v]; }   // CellReynolds, Model%dt
-            if (MHD) {
-                if (BORIS) {
-                    double oBv[3] = {(double)nB[idx], (double)nB[nc + idx], (double)nB[2 * nc + idx]};
-                    double Bt[3] = {Bn[0], Bn[1], Bn[2]};
-                    if (bg.on) {   // mhdgroup.F90:180-183 total fields
-#pragma unroll
-                        for (int x = 0; x < 3; x++) { Bt[x] = Bt[x] + bg.B0[nc * x + idx]; oBv[x] = oBv[x] + bg.B0[nc * x + idx]; }
-                    }
-                    cell_boris(m, U, U0, Bt, oBv, dGH, dGM, m.Ca, dt);
-                } else {
-                    cell_maxwell(m, U, dGM, dt);
-                }
-            }
+            cell_apply<BORIS, MHD>(m, bg, nc, idx, dt, nB, U, Bn, dGH, dGM);
         }
 #pragma unroll
         for (int v = 0; v < 5; v++) st_nt(&oGas[nc * v + idx], R(U[v]));
@@ -847,19 +894,7 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
         for (int x = 0; x < 3; x++) st_nt(&oB[nc * x + idx], R(Bn[x]));
         // skip_nb > 0: the ring rows' CFL bound is taken after the ring average (k_celldt_rows)
         const bool band = skip_nb > 0 && (j < g.nG - 1 + skip_nb || j > g.nG + g.Njp - skip_nb);
-        if (in_dt_box(g, i, j, k) && !band) {
-            if (sizeof(R) < sizeof(double)) {   // CFL bound of the stored state
-#pragma unroll
-                for (int v = 0; v < 5; v++) U[v] = (double)R(U[v]);
-#pragma unroll
-                for (int x = 0; x < 3; x++) Bn[x] = (double)R(Bn[x]);
-            }
-            if (bg.on) {   // gridutils.F90:82-86
-#pragma unroll
-                for (int x = 0; x < 3; x++) Bn[x] = Bn[x] + bg.B0[nc * x + idx];
-            }
-            dtl = cell_dt<BORIS>(g, m, U, Bn, idx);
-        }
+        if (in_dt_box(g, i, j, k) && !band) dtl = cell_dt_stored<BORIS>(g, m, bg, idx, U, Bn);   // CFL bound of the stored state
     }
     block_min_store(dtl, partial);
 }
@@ -922,22 +957,7 @@ __global__ void __launch_bounds__(256) k_update_col(GVT<R> g, Tile t, MP m, BGV 
                     }
                 }
                 haveF = true;
-                double U0[5];
-#pragma unroll
-                for (int v = 0; v < 5; v++) { U0[v] = U[v]; U[v] = U[v] + dt * dGH[v]; }   // CellReynolds, Model%dt
-                if (MHD) {
-                    if (BORIS) {
-                        double oBv[3] = {(double)nB[idx], (double)nB[nc + idx], (double)nB[2 * nc + idx]};
-                        double Bt[3] = {Bn[0], Bn[1], Bn[2]};
-                        if (bg.on) {   // mhdgroup.F90:180-183 total fields
-#pragma unroll
-                            for (int x = 0; x < 3; x++) { Bt[x] = Bt[x] + bg.B0[nc * x + idx]; oBv[x] = oBv[x] + bg.B0[nc * x + idx]; }
-                        }
-                        cell_boris(m, U, U0, Bt, oBv, dGH, dGM, m.Ca, dt);
-                    } else {
-                        cell_maxwell(m, U, dGM, dt);
-                    }
-                }
+                cell_apply<BORIS, MHD>(m, bg, nc, idx, dt, nB, U, Bn, dGH, dGM);
             } else {
                 haveF = false;
             }
@@ -947,19 +967,7 @@ __global__ void __launch_bounds__(256) k_update_col(GVT<R> g, Tile t, MP m, BGV 
             for (int x = 0; x < 3; x++) st_nt(&oB[nc * x + idx], R(Bn[x]));
             // skip_nb > 0: the ring rows' CFL bound is taken after the ring average (k_celldt_rows)
             const bool band = skip_nb > 0 && (j < g.nG - 1 + skip_nb || j > g.nG + g.Njp - skip_nb);
-            if (in_dt_box(g, i, j, k) && !band) {
-                if (sizeof(R) < sizeof(double)) {   // CFL bound of the stored state
-#pragma unroll
-                    for (int v = 0; v < 5; v++) U[v] = (double)R(U[v]);
-#pragma unroll
-                    for (int x = 0; x < 3; x++) Bn[x] = (double)R(Bn[x]);
-                }
-                if (bg.on) {   // gridutils.F90:82-86
-#pragma unroll
-                    for (int x = 0; x < 3; x++) Bn[x] = Bn[x] + bg.B0[nc * x + idx];
-                }
-                dtl = fmin(dtl, cell_dt<BORIS>(g, m, U, Bn, idx));
-            }
+            if (in_dt_box(g, i, j, k) && !band) dtl = fmin(dtl, cell_dt_stored<BORIS>(g, m, bg, idx, U, Bn));   // CFL bound of the stored state
         }
     }
     block_min_store(dtl, partial);
