@@ -216,6 +216,17 @@ int  gam_enforce_bcs(gam_handle *h);
 int  gam_step(gam_handle *h, int nsteps);
 /* DivB total (gridutils.F90:717-770). */
 int  gam_divb(gam_handle *h, double *tot);
+/* The same total, bit for bit, from an entry point that allocates nothing: no
+ * device or host allocation on any call, for callers that cannot allocate
+ * (tight monitoring loops, harnesses that forbid it).  No counterpart in the
+ * reference.  Its block sums live in a file-scope device array of
+ * GAM_NOALLOC_BLOCKS + 1 doubles (1 MiB) that each GPU holds until the process
+ * ends; blocks with more than GAM_NOALLOC_BLOCKS * 256 active cells are refused
+ * (returns -1, gam_last_error says why).  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (handles on different streams,
+ * host threads) race and must be serialised by the caller. */
+#define GAM_NOALLOC_BLOCKS 131072
+int  gam_divb_noalloc(gam_handle *h, double *tot);
 
 /* --- diagnostics ---------------------------------------------------------- */
 /* Copy a solver work array of the last step to the host, for stage-level

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,8 @@ struct gam_handle {
     double ms[ST_N] = {};
     int64_t launches[ST_N] = {};
     int graph_refs = 0;        // captured graphs holding this handle's buffers and launch parameters
+    int device = 0;            // the GPU current at gam_create (its stream's device)
+    double *scratch = nullptr; // that GPU's copy of the file-scope scratch array (gam_divb_noalloc)
 };
 
 // Calls that would change what a captured graph launches (kernels, their
@@ -195,6 +198,31 @@ static int sync_grid32(gam_handle *h) {
     return 0;
 }
 
+// ----------------------------------------------------------------- scratch
+// Device memory of gam_divb_noalloc, the entry point that allocates nothing:
+// a file-scope array of GAM_NOALLOC_BLOCKS block sums plus their total.  Every
+// GPU the library runs on holds its own copy until the process ends.  In host
+// code the name `scratch` is no device address: each GPU's copy is reached
+// through the address hipGetSymbolAddress returns with that GPU current, which
+// gam_create looks up once per device and keeps in g_scratch_dev.
+static __device__ __align__(256) double scratch[GAM_NOALLOC_BLOCKS + 1];
+#define GAM_MAX_DEVICES 64
+static double *g_scratch_dev[GAM_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_lookup(gam_handle *h) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (h->device < 0 || h->device >= GAM_MAX_DEVICES) return fail(h, -1, "device %d: at most %d GPUs", h->device, GAM_MAX_DEVICES);
+    if (!g_scratch_dev[h->device]) {
+        void *p = nullptr;
+        HIPCHK(h, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(h, -2, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_dev[h->device] = static_cast<double *>(p);
+    }
+    h->scratch = g_scratch_dev[h->device];
+    return 0;
+}
+
 // ----------------------------------------------------------------- lifetime
 extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handle **out) {
     if (!dims || !par || !out) return -1;
@@ -222,6 +250,8 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     m.rmeth = par->rmeth; m.doLFMLim = par->doLFMLim; m.doBoris = par->doBoris; m.doMHD = par->doMHD;
     m.do25D = par->do25D;
     auto cleanup = [&](int code) { gam_destroy(h); return code; };
+    if (hipGetDevice(&h->device) != hipSuccess) return cleanup(-2);
+    if (scratch_lookup(h)) return cleanup(-2);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(-2);
     const size_t G = gsz(h), F = fsz(h), es = h->es;
     const size_t gcomp[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
@@ -1290,6 +1320,22 @@ extern "C" int gam_divb(gam_handle *h, double *tot) {
     for (double v : hp) s += v;
     *tot = s;
     return 0;
+}
+
+extern "C" int gam_divb_noalloc(gam_handle *h, double *tot) {
+    if (!h || !tot) return -1;
+    const GV &g = h->gv;
+    const long n = (long)g.Nip * g.Njp * g.Nkp;
+    const long nb = (n + 255) / 256;
+    if (nb > GAM_NOALLOC_BLOCKS)
+        return fail(h, -1, "gam_divb_noalloc: %ld active cells need %ld block sums, more than the %d of its scratch array "
+                    "(at most %ld cells; use gam_divb)", n, nb, GAM_NOALLOC_BLOCKS, (long)GAM_NOALLOC_BLOCKS * 256);
+    double *sc = h->scratch;   // this GPU's copy of `scratch`
+    hipLaunchKernelGGL(k_divb, dim3((unsigned)nb), 256, 0, h->stream, g, (const double *)h->mag[h->s], sc);
+    hipLaunchKernelGGL(k_sum_ordered, 1, 256, 0, h->stream, (const double *)sc, (int)nb, sc + GAM_NOALLOC_BLOCKS);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(tot, sc + GAM_NOALLOC_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    return gam_sync(h);
 }
 
 extern "C" int gam_get_work(gam_handle *h, int which, double *out) {

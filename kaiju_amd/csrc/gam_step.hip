@@ -1342,6 +1342,32 @@ __global__ void k_divb(GV g, const double *__restrict__ mag, double *__restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// Sum of np values in index order (the order of gam_divb's host loop, so the
+// totals agree bit for bit): each round stages 256 values in LDS and thread 0
+// adds them in turn.  One block.
+__global__ void __launch_bounds__(256) k_sum_ordered(const double *__restrict__ part, int np, double *__restrict__ out) {
+#ifdef GAM_EMU
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int q = 0; q < np; q++) s += part[q];
+    *out = s;
+#else
+    __shared__ double v[256];
+    double s = 0.0;
+    for (int q0 = 0; q0 < np; q0 += 256) {
+        const int q = q0 + (int)threadIdx.x;
+        v[threadIdx.x] = q < np ? part[q] : 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = np - q0 < 256 ? np - q0 : 256;
+            for (int r = 0; r < m; r++) s += v[r];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = s;
+#endif
+}
+
 // ----------------------------------------------------------------- learned correction
 // A correction field o[c][8] (5 gas-flux and 3 edge-EMF channels per cell,
 // FP32) adds scale*face_d(c)*o[c][v] to the gas flux v of every active

@@ -131,6 +131,7 @@ def lib():
         "gam_enforce_bcs": (C.c_int, [H]),
         "gam_step": (C.c_int, [H, C.c_int]),
         "gam_divb": (C.c_int, [H, C.POINTER(C.c_double)]),
+        "gam_divb_noalloc": (C.c_int, [H, C.POINTER(C.c_double)]),
         "gam_get_work": (C.c_int, [H, C.c_int, _dp]),
         "gam_set_timing": (C.c_int, [H, C.c_int]),
         "gam_get_timing": (C.c_int, [H, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
@@ -579,6 +580,12 @@ class GamApp:
     def divb(self) -> float:
         v = C.c_double()
         self._chk(lib().gam_divb(self.h, C.byref(v)), "gam_divb")
+        return v.value
+
+    def divb_noalloc(self) -> float:
+        """The same total from the entry point that allocates nothing."""
+        v = C.c_double()
+        self._chk(lib().gam_divb_noalloc(self.h, C.byref(v)), "gam_divb_noalloc")
         return v.value
 
     def sync(self):

@@ -31,6 +31,7 @@ using std::fabs;
 #define __shared__ static
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define __align__(n) alignas(n)
 
 struct dim3 {
     unsigned x, y, z;
@@ -65,6 +66,9 @@ inline hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t) { std::m
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t);
 inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { *s = (void *)0x1; return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventCreate(hipEvent_t *e) { *e = (void *)0x1; return hipSuccess; }
