@@ -56,17 +56,24 @@ class Ensemble:
     """The members' apps and the graph that steps them."""
 
     def __init__(self, specs: List[configs.CaseSpec], tune: bool = True, cache: Optional[dict] = None,
-                 nsteps: int = 2):
+                 nsteps: int = 2, apps: Optional[List[GamApp]] = None):
+        """apps: the members' apps, already set up (the ensemble then owns
+        them); default: one per spec, grid and initial state from InitModel."""
         if nsteps < 2 or nsteps % 2:
             raise ValueError("nsteps must be even")
+        if apps is not None and len(apps) != len(specs):
+            raise ValueError("one app per spec")
         self.specs = specs
         self.nsteps = nsteps
         self.apps = []
         t0 = time.perf_counter()
-        for s in specs:
-            a = GamApp(s)
-            a.InitModel()
-            self.apps.append(a)
+        if apps is not None:
+            self.apps = list(apps)
+        else:
+            for s in specs:
+                a = GamApp(s)
+                a.InitModel()
+                self.apps.append(a)
         self.setup_s = time.perf_counter() - t0
         self.tiles = {}
         t0 = time.perf_counter()

@@ -312,13 +312,14 @@ class GamApp:
         self._chk(lib().gam_set_grid(self.h, *[_p(a) for a in arrs]), "gam_set_grid")
         self._xyz = arrs[0]
 
-    def get_grid(self) -> dict:
+    def get_grid(self, only: Optional[tuple] = None) -> dict:
+        """The geometry arrays (only: the names to fetch, default all)."""
         G, F = self.G, self.F
         shapes = dict(xyz=F + (3,), xyzcc=G + (3,), xfc=F + (3, 3), face=F + (3,), Tf=F + (9, 3),
                       edge=F + (3,), Te=F + (9, 3), Teb=F + (4, 3), volume=G, di=G, dj=G, dk=G)
-        out = {k: np.zeros(s, order="F") for k, s in shapes.items()}
         names = ("xyz", "xyzcc", "xfc", "face", "Tf", "edge", "Te", "Teb", "volume", "di", "dj", "dk")
-        self._chk(lib().gam_get_grid(self.h, *[_p(out[n]) for n in names]), "gam_get_grid")
+        out = {k: np.zeros(s, order="F") for k, s in shapes.items() if only is None or k in only}
+        self._chk(lib().gam_get_grid(self.h, *[_p(out.get(n)) for n in names]), "gam_get_grid")
         return out
 
     def init_ic(self):

@@ -4,16 +4,20 @@ CPU: the seeded member draw.  GPU (and CPU emulation): replaying the captured
 graph gives, at every replay, bit-for-bit the state, clock and status of the
 same blocks stepped eagerly with gam_step; a block that halts on a CalcDT
 status inside the graph stops exactly where gam_step stops; the autotuner
-restores the block it tuned and changes no result.
+restores the block it tuned and changes no result.  GPU: the ensemble workload
+itself (32 seeded blocks, edges 64-160, FP32 bulk) steps under the graph with
+every block conserving and keeping div.B, and a seeded draw of oracle-sized
+FP64 blocks under the graph agrees with the oracle block by block.
 """
 import dataclasses
 
 import numpy as np
 import pytest
 
+import oracle
 from kaiju_amd import autotune, configs
 from kaiju_amd.ensemble import Ensemble, draw_members
-from tests.helpers import say
+from tests.helpers import app_from_oracle, compare_state, say
 
 
 def test_members_are_seeded_and_in_range():
@@ -204,3 +208,58 @@ def test_graph_refuses_changes_to_held_handles():
     b = eager_twins(specs)[0]   # released: the same calls work
     b.set_tile("flux_j", 32, 8, 1, 0)
     b.close()
+
+
+def _totals(s: dict, vol: np.ndarray, nG: int):
+    """Mass, momentum and the momentum scale sum |rho v| dV over the active cells."""
+    A = tuple(slice(nG, -nG) for _ in range(3))
+    g, v = s["Gas"][A], vol[A]
+    mass = float((g[..., 0] * v).sum())
+    mom = [float((g[..., 1 + d] * v).sum()) for d in range(3)]
+    scale = float(sum((np.abs(g[..., 1 + d]) * v).sum() for d in range(3)))
+    return mass, mom, scale
+
+
+@pytest.mark.gpu
+def test_ensemble_workload_32_seeded_blocks():
+    """The ensemble configuration at its own size: 32 blocks drawn from seed
+    31337 (edges 64-160 per axis, half Orszag-Tang, half blast waves with drawn
+    field and pressure ratio, FP32 bulk), 4 steps under one captured graph.
+    Every block steps without a CalcDT halt, stays finite, keeps its mass and
+    momentum (periodic blocks: FP32 storage rounding only) and its div.B."""
+    specs = draw_members(32, 31337)
+    say(f"[ensemble] C4 workload: {len(specs)} blocks, {sum(s.zones for s in specs)} zones")
+    with Ensemble(specs, tune=False, nsteps=2) as ens:
+        before = []
+        for a in ens.apps:
+            vol = a.get_grid(only=("volume",))["volume"]
+            before.append((_totals(a.get_state(0), vol, a.spec.nG), a.divb(), vol))
+        assert ens.run(4) == 0
+        for a, ((m0, p0, sc0), d0, vol) in zip(ens.apps, before):
+            s = a.get_state(0)
+            assert a.Model.ts == 4, a.spec.name
+            assert np.isfinite(s["Gas"]).all() and np.isfinite(s["magFlux"]).all(), a.spec.name
+            m1, p1, sc1 = _totals(s, vol, a.spec.nG)   # a blast wave starts at rest: scale after the steps
+            assert abs(m1 - m0) <= 1e-6 * m0, (a.spec.name, m0, m1)
+            assert max(abs(x - y) for x, y in zip(p0, p1)) <= 1e-6 * max(sc0, sc1), (a.spec.name, p0, p1)
+            assert abs(a.divb() - d0) <= 1e-12 * np.abs(s["magFlux"]).max() * a.spec.zones, a.spec.name
+    say("   32 blocks: 4 steps, status 0, mass, momentum and div.B kept")
+
+
+@pytest.mark.gpu
+def test_seeded_small_members_under_the_graph_match_the_oracle():
+    """A seeded draw of oracle-sized blocks (edges 8-20, both kinds, FP64), each
+    starting from the oracle's own hatched block, stepped twice under one
+    captured graph: every block agrees with the oracle's two steps of it."""
+    specs = draw_members(6, 4242, lo=8, hi=20, precision=0)
+    say(f"[ensemble] seeded small members vs the oracle: {[(s.N, s.ic) for s in specs]}")
+    ocs = [oracle.OracleCase(s) for s in specs]
+    apps = [app_from_oracle(oc) for oc in ocs]
+    with Ensemble(specs, tune=False, nsteps=2, apps=apps) as ens:
+        assert ens.run(2) == 0
+        for a, oc in zip(ens.apps, ocs):
+            oc.step(2)
+            e = compare_state(a, oc, 0)
+            say(f"   {a.spec.name}: {e}")
+            assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-11, (a.spec.name, e)
+            assert a.Model.ts == oc.m.ts

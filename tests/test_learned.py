@@ -12,7 +12,9 @@ div.B at rounding level and conserve mass and momentum.
 
 GPU only (the network is MFMA code): a network trained on the GPU on the
 solver's downsampled fine trajectories lowers the one-step loss and the
-one-step error of a held-out seed.
+one-step error of a held-out seed; and at the learned configuration's own size
+(128^3 from a 256^3 trajectory) a corrected run conserves, keeps div.B and
+stays finite.
 """
 import math
 
@@ -177,8 +179,10 @@ def test_corrected_steps_keep_divb_and_conserve(precision):
 def test_trained_correction_lowers_held_out_one_step_error():
     """Trained on the GPU on 8 seeds' downsampled fine trajectories (16^3
     coarse), the correction lowers the one-step loss and the one-step L1 error
-    of a held-out seed; its rollout stays bounded (no blow-up).  A rollout gain
-    is not claimed: the training is one-step (teacher forced)."""
+    of a held-out seed, and its 16-step rollout ends no worse than the plain
+    solver's (measured 1.04x better at the end; the 1% margin covers the
+    run-to-run spread of the training, whose weight gradients are summed with
+    FP32 atomics in no fixed order)."""
     if on_emulation():
         pytest.skip("the network runs only as MFMA code")
     say("[learned] train 16^3 on 8 seeds, test on a held-out seed")
@@ -187,4 +191,58 @@ def test_trained_correction_lowers_held_out_one_step_error():
     say(f"   {r}")
     assert r["test_loss_ratio"] < 0.85
     assert r["one_step_gain"] > 1.03
-    assert np.isfinite(r["err_corrected_final"]) and r["err_corrected_final"] < 1.5 * r["err_plain_final"]
+    assert np.isfinite(r["err_corrected_final"]) and r["err_corrected_final"] <= 1.01 * r["err_plain_final"]
+
+
+@pytest.mark.gpu
+def test_learned_workload_corrected_run_at_128():
+    """The learned configuration at its own size: decaying turbulence (Fourier
+    modes k <= 4 in v and A, rms dv = dB = 1, rho = p = 1) at 128^3, a network
+    trained on the GPU on the solver's own 256^3 trajectory downsampled 2x
+    (a short training: 3 samples, 40 minibatches), then 8 coarse steps with the
+    network's correction applied at every step.  The run keeps div.B at
+    rounding, conserves mass and momentum (the correction enters as flux and
+    EMF changes) and stays finite without a CalcDT halt."""
+    if on_emulation():
+        pytest.skip("the network runs only as MFMA code")
+    N = 128
+    ic = learned.TurbIC(seed=7, kmax=4, v_rms=1.0, b_rms=1.0)
+    say(f"[learned] {N}^3 corrected run: fine {2 * N}^3 trajectory")
+    tr = learned.fine_trajectory(ic, N, steps=3)
+    app = learned.coarse_app(N)
+    samples, corr = [], None
+    try:
+        tgts = learned.trajectory_targets(app, tr)
+        norm = learned.normalisation([tr], [tgts])
+        samples = learned.trajectory_samples(tr, tgts)
+        del tgts
+        corr = learned.Corrector(app, norm, seed=0)
+        hist = corr.train(samples, tr.dt, 40, batch=4)
+        say(f"   training loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+        assert all(np.isfinite(hist))
+        learned.load_pair(app, tr, 0)
+        a = slice(NG, NG + N)
+        s0 = app.get_state(0)
+        divb0 = app.divb()
+        tot0 = s0["Gas"][a, a, a, :4].sum(axis=(0, 1, 2))
+        scale = np.abs(s0["Gas"][a, a, a, 1:4]).sum()
+        for n in range(8):
+            app.set_correction(corr.infer(), corr.norm.scale)
+            assert app.steps(1) == 0, n
+        app.set_correction(None)
+        s1 = app.get_state(0)
+        assert np.isfinite(s1["Gas"]).all() and np.isfinite(s1["magFlux"]).all()
+        assert np.max(np.abs(s1["Gas"] - s0["Gas"])) > 0
+        tot1 = s1["Gas"][a, a, a, :4].sum(axis=(0, 1, 2))
+        say(f"   mass {tot0[0]:.15e} -> {tot1[0]:.15e}, momentum {tot0[1:]} -> {tot1[1:]}, "
+            f"div.B {divb0:.3e} -> {app.divb():.3e}")
+        assert abs(tot1[0] - tot0[0]) <= 1e-12 * tot0[0]
+        assert np.max(np.abs(tot1[1:] - tot0[1:])) <= 1e-12 * scale
+        assert abs(app.divb() - divb0) <= 1e-12 * np.abs(s1["magFlux"]).max() * N ** 3
+    finally:
+        for smp in samples:
+            smp.free()
+        if corr is not None:
+            corr.close()
+        app.set_correction(None)
+        app.close()
