@@ -6,11 +6,16 @@
 //
 // Layouts (nc = cells of the ghost-inclusive box, p = linear cell index, i fastest):
 //   activations  act[p][32]   BF16, channels last (A operand of the convolutions)
-//                actT[c][p]   BF16, channel-major copy (operand of the weight gradients)
-//   weights      W[l][tap][co][ci] BF16 (ci = 32 padded), tap = dz*9 + dy*3 + dx
+//                actT[c][p]   BF16, channel-major copy (operand of the weight gradients),
+//                             rows of ncs >= nc elements (a multiple of 8, with a gap),
+//                             NET_PAD elements of zeros before row 0 and after row 31
+//   weights      W[l][tap][co][ci] BF16 (ci = 32 padded), tap = dz*9 + dy*3 + dx;
+//                the first layer (8 input channels) packs 4 taps per K = 32:
+//                W0[g][co][4*8] with taps 4g..4g+3 (zero past tap 26)
 //   output       out[p][8]    FP32 (the correction field, channels last)
 // A convolution is an implicit GEMM: M = 16 consecutive cells per MFMA, N = 16
-// output channels per tile, K = 32 input channels of one tap per MFMA.  Output
+// output channels per tile, K = 32 input channels of one tap per MFMA (of four
+// taps in the first layer).  Output
 // cells cover [lo, lo + 16*ntile) with lo = Ni*Nj + Ni + 1, so every tap's
 // neighbour p + off stays inside the array; cells near the box faces read
 // wrapped neighbours and are never used (the correction is read on the active
@@ -30,6 +35,9 @@ typedef unsigned short u16;
 
 #define NET_C 32      // padded channel count of every activation
 #define NET_TAPS 27
+#define NET_G0 7      // K steps of the first layer: ceil(27 taps / 4)
+#define NET_PAD 64    // zero elements before and after a channel-major array
+#define NET_MT 4      // 16-cell tiles per wave in the convolutions
 
 // round-to-nearest-even FP32 -> BF16 (finite inputs)
 __host__ __device__ inline u16 f2bf(float f) {
@@ -45,7 +53,7 @@ __host__ __device__ inline float bf2f(u16 b) {
     return f;
 }
 
-struct NetDims { int Ni, Nj, Nk, nc, lo, ntile; int off[NET_TAPS]; };
+struct NetDims { int Ni, Nj, Nk, nc, ncs, lo, ntile; int off[NET_TAPS]; };
 
 __device__ __forceinline__ bf16x8 ld8(const u16 *p) {   // 16-byte aligned
     return *reinterpret_cast<const bf16x8 *>(p);
@@ -60,106 +68,160 @@ __device__ __forceinline__ bf16x8 ld8(const u16 *p) {   // 16-byte aligned
 // MODE 1: linear output, write out[p][0..7] FP32 (NCO = 16, channels 8..15 padding)
 // MODE 2: backward data: multiply by the activation's slope at the forward
 //         output (mask act > 0: 1, else 1/8), write both BF16 layouts, NCO = 32
-// One wave = 16 cells x NCO channels; blockDim 256 = 4 waves.
-template <int NCO, int MODE>
+// TPK taps per K step: 1 (K = 32 channels of one tap) or 4 (the first layer:
+// K = 8 channels of 4 taps, lane group kg reading tap 4s + kg).
+// One wave = NET_MT tiles of 16 cells x NCO channels (each K step's weights
+// loaded once for all its tiles); blockDim 256 = 4 waves.
+template <int NCO, int MODE, int TPK>
 __global__ void __launch_bounds__(256) k_conv(NetDims d, const u16 *__restrict__ in, const u16 *__restrict__ W,
                                               const float *__restrict__ bias, const u16 *__restrict__ mask,
                                               u16 *__restrict__ act_out, u16 *__restrict__ actT_out,
                                               float *__restrict__ out) {
     const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (tile >= d.ntile) return;                       // whole waves exit together
-    const int p0 = d.lo + 16 * tile;
+    const int tile0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * NET_MT;   // this wave's NET_MT consecutive tiles
+    if (tile0 >= d.ntile) return;                      // whole waves exit together
     const int row = lane & 15, kg = lane >> 4;         // A row (cell), K group of 8 channels
-    f32x4 acc[NCO / 16];
+    f32x4 acc[NET_MT][NCO / 16];
 #pragma unroll
-    for (int t = 0; t < NCO / 16; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int tap = 0; tap < NET_TAPS; tap++) {
-        const int q = p0 + row + d.off[tap];
-        const bf16x8 a = ld8(in + (size_t)q * NET_C + 8 * kg);
+    for (int m = 0; m < NET_MT; m++)
 #pragma unroll
-        for (int t = 0; t < NCO / 16; t++) {
-            const int co = 16 * t + row;
-            const bf16x8 b = ld8(W + ((size_t)tap * NCO + co) * NET_C + 8 * kg);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+        for (int t = 0; t < NCO / 16; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    constexpr int NSTEP = (NET_TAPS + TPK - 1) / TPK;
+    for (int st = 0; st < NSTEP; st++) {
+        const int tap = TPK == 1 ? st : TPK * st + kg;
+        bf16x8 b[NCO / 16];                            // this K step's weights, used by every tile of the wave
+#pragma unroll
+        for (int t = 0; t < NCO / 16; t++) b[t] = ld8(W + ((size_t)st * NCO + 16 * t + row) * NET_C + 8 * kg);
+        const bool live = TPK == 1 || tap < NET_TAPS;  // K slots past the last tap: weights are zero there too
+        const int q0 = d.lo + 16 * tile0 + row + (live ? d.off[tap] : 0);
+#pragma unroll
+        for (int m = 0; m < NET_MT; m++) {
+            if (tile0 + m >= d.ntile) break;           // uniform in the wave
+            const bf16x8 a = live ? ld8(in + (size_t)(q0 + 16 * m) * NET_C + (TPK == 1 ? 8 * kg : 0)) : bf16x8{};
+#pragma unroll
+            for (int t = 0; t < NCO / 16; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[m][t], 0, 0, 0);
         }
     }
     // D layout: lane holds rows 4*kg + r (cells), column row (channel)
 #pragma unroll
-    for (int t = 0; t < NCO / 16; t++) {
-        const int co = 16 * t + row;
+    for (int m = 0; m < NET_MT; m++) {
+        if (tile0 + m >= d.ntile) break;
+        const int p0 = d.lo + 16 * (tile0 + m);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int p = p0 + 4 * kg + r;
-            float v = acc[t][r];
-            if (MODE == 0) {
-                v = v + bias[co];
-                v = v > 0.f ? v : NET_LEAK * v;
-                const u16 b = f2bf(v);
-                act_out[(size_t)p * NET_C + co] = b;
-                actT_out[(size_t)co * d.nc + p] = b;
-            } else if (MODE == 1) {
-                if (co < 8) out[(size_t)p * 8 + co] = v + bias[co];
-            } else {
-                if (bf2f(mask[(size_t)p * NET_C + co]) <= 0.f) v = NET_LEAK * v;   // slope at the forward output
-                const u16 b = f2bf(v);
-                act_out[(size_t)p * NET_C + co] = b;
-                actT_out[(size_t)co * d.nc + p] = b;
+        for (int t = 0; t < NCO / 16; t++) {
+            const int co = 16 * t + row;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int p = p0 + 4 * kg + r;
+                float v = acc[m][t][r];
+                if (MODE == 0) {
+                    v = v + bias[co];
+                    v = v > 0.f ? v : NET_LEAK * v;
+                    const u16 bb = f2bf(v);
+                    act_out[(size_t)p * NET_C + co] = bb;
+                    actT_out[(size_t)co * d.ncs + p] = bb;
+                } else if (MODE == 1) {
+                    if (co < 8) out[(size_t)p * 8 + co] = v + bias[co];
+                } else {
+                    if (bf2f(mask[(size_t)p * NET_C + co]) <= 0.f) v = NET_LEAK * v;   // slope at the forward output
+                    const u16 bb = f2bf(v);
+                    act_out[(size_t)p * NET_C + co] = bb;
+                    actT_out[(size_t)co * d.ncs + p] = bb;
+                }
             }
         }
     }
 }
 
-// Weight gradient of one layer: dW[tap][co][ci] += sum_p g[p][co] * a[p + off][ci]
-// over this block's chunk of cells (FP32 atomics into the unpadded parameter
-// layout [tap][nco][nci]).  gT is the channel-major gradient (NCO rows), aT the
-// channel-major input activation (32 rows).  One wave per (tap, chunk); K = 32
-// cells per MFMA, 8 per lane group.
-template <int NCO>
-__global__ void __launch_bounds__(64) k_wgrad(NetDims d, const u16 *__restrict__ gT, const u16 *__restrict__ aT,
-                                              int chunk, int nco, int nci, float *__restrict__ dW) {
-    const int lane = threadIdx.x;
-    const int tap = blockIdx.y;
-    const int c0 = d.lo + blockIdx.x * chunk;          // chunk is a multiple of 32
-    const int cend = min(c0 + chunk, d.lo + 16 * d.ntile);
-    const int row = lane & 15, kg = lane >> 4;
-    const int off = d.off[tap];
-    f32x4 acc[NCO / 16][2];
+// 8 BF16 at element offset S (0..7) of the 16 aligned ones in w[0..7] (dwords)
+template <int S> __device__ __forceinline__ bf16x8 take8(const uint32_t *w) {
+    uint32_t r[4];
 #pragma unroll
-    for (int m = 0; m < NCO / 16; m++) { acc[m][0] = f32x4{0.f, 0.f, 0.f, 0.f}; acc[m][1] = f32x4{0.f, 0.f, 0.f, 0.f}; }
+    for (int j = 0; j < 4; j++)
+        r[j] = (S % 2 == 0) ? w[j + S / 2] : __builtin_amdgcn_alignbyte(w[j + S / 2 + 1], w[j + S / 2], 2);
+    bf16x8 v;
+    memcpy(&v, r, 16);
+    return v;
+}
+__device__ __forceinline__ void ld16x2(const u16 *p, uint32_t *w) {   // 32 bytes from a 16-byte aligned address
+    const uint4 a = *reinterpret_cast<const uint4 *>(p), b = *reinterpret_cast<const uint4 *>(p + 8);
+    w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+}
+
+// Weight gradient of one layer, dW[tap][co][ci] += sum_p g[p][co] * a[p + off][ci]:
+// per tap a GEMM with M = output channels, N = input channels and K = cells.
+// Block = 4 waves on 4 consecutive chunks of cells of one tap (blocks ordered
+// tap fastest, so the 27 taps of a chunk run together and share its gradient
+// rows in L2).  Per 32 cells a wave loads its A tiles (gradient, channel-major,
+// 16 aligned bytes per lane: chunks start at multiples of 8 cells) and its B
+// tiles (activation at the tap-shifted cells: 32 aligned bytes per lane, then
+// the 8 wanted values taken at the wave-uniform misalignment S of the tap
+// offset).  Cells outside the computed range carry a zero gradient (never
+// written, zero from allocation), the arrays' zero padding covers the reads
+// past either end.  The 4 waves' sums meet in LDS; one FP32 atomic per weight
+// per block goes to the unpadded parameter layout [tap][nco][nci].
+template <int NCO, int NT, int S>
+__device__ __forceinline__ void wgrad_chunk(const NetDims &d, const u16 *__restrict__ gT, const u16 *__restrict__ aT,
+                                            int c0, int cend, int off, f32x4 (&acc)[NCO / 16][NT]) {
+    const int lane = threadIdx.x & 63, row = lane & 15, kg = lane >> 4;
     for (int c = c0; c < cend; c += 32) {
-        const int pk = c + 8 * kg;                     // 8 cells of this lane group
-        bf16x8 bv[2];
+        const int pk = c + 8 * kg;                             // 8 cells of this lane group
+        bf16x8 bv[NT];
 #pragma unroll
-        for (int n = 0; n < 2; n++) {                  // B[k = cell][n = ci]: a at the shifted cell, unaligned
-            const u16 *src = aT + (size_t)(16 * n + row) * d.nc + pk + off;
-            u16 tmp[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) tmp[e] = (pk + e < cend) ? src[e] : (u16)0;
-            memcpy(&bv[n], tmp, 16);
+        for (int n = 0; n < NT; n++) {                         // B[k = cell][n = ci] at cell pk + off
+            uint32_t w[8];
+            ld16x2(aT + (size_t)(16 * n + row) * d.ncs + pk + off - S, w);
+            bv[n] = take8<S>(w);
         }
 #pragma unroll
-        for (int m = 0; m < NCO / 16; m++) {           // A[m = co][k = cell]
-            const u16 *src = gT + (size_t)(16 * m + row) * d.nc + pk;
-            u16 tmp[8];
+        for (int m = 0; m < NCO / 16; m++) {                   // A[m = co][k = cell]
+            const bf16x8 av = ld8(gT + (size_t)(16 * m + row) * d.ncs + pk);
 #pragma unroll
-            for (int e = 0; e < 8; e++) tmp[e] = (pk + e < cend) ? src[e] : (u16)0;
-            bf16x8 av;
-            memcpy(&av, tmp, 16);
+            for (int n = 0; n < NT; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[n], acc[m][n], 0, 0, 0);
+        }
+    }
+}
+
+template <int NCO, int NT>
+__global__ void __launch_bounds__(256) k_wgrad(NetDims d, const u16 *__restrict__ gT, const u16 *__restrict__ aT,
+                                               int chunk, int nchunk, int nco, int nci, float *__restrict__ dW) {
+    __shared__ float red[4][NCO][16 * NT];
+    const int tap = blockIdx.x % NET_TAPS;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = lane & 15, kg = lane >> 4;
+    const int q = (blockIdx.x / NET_TAPS) * 4 + wave;          // this wave's chunk
+    const int A0 = d.lo & ~7, A1 = d.lo + 16 * d.ntile;
+    const int off = d.off[tap];
+    f32x4 acc[NCO / 16][NT];
 #pragma unroll
-            for (int n = 0; n < 2; n++) acc[m][n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv[n], acc[m][n], 0, 0, 0);
+    for (int m = 0; m < NCO / 16; m++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    if (q < nchunk) {
+        const int c0 = A0 + q * chunk, cend = min(c0 + chunk, A1);
+        switch (off & 7) {   // the misalignment S of cell pk + off, pk a multiple of 8
+        case 0: wgrad_chunk<NCO, NT, 0>(d, gT, aT, c0, cend, off, acc); break;
+        case 1: wgrad_chunk<NCO, NT, 1>(d, gT, aT, c0, cend, off, acc); break;
+        case 2: wgrad_chunk<NCO, NT, 2>(d, gT, aT, c0, cend, off, acc); break;
+        case 3: wgrad_chunk<NCO, NT, 3>(d, gT, aT, c0, cend, off, acc); break;
+        case 4: wgrad_chunk<NCO, NT, 4>(d, gT, aT, c0, cend, off, acc); break;
+        case 5: wgrad_chunk<NCO, NT, 5>(d, gT, aT, c0, cend, off, acc); break;
+        case 6: wgrad_chunk<NCO, NT, 6>(d, gT, aT, c0, cend, off, acc); break;
+        default: wgrad_chunk<NCO, NT, 7>(d, gT, aT, c0, cend, off, acc); break;
         }
     }
     // D[i = co][j = ci]: lane holds co = 16m + 4kg + r, ci = 16n + row
 #pragma unroll
     for (int m = 0; m < NCO / 16; m++)
 #pragma unroll
-        for (int n = 0; n < 2; n++)
+        for (int n = 0; n < NT; n++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int co = 16 * m + 4 * kg + r, ci = 16 * n + row;
-                if (co < nco && ci < nci) atomicAdd(&dW[((size_t)tap * nco + co) * nci + ci], acc[m][n][r]);
-            }
+            for (int r = 0; r < 4; r++) red[wave][16 * m + 4 * kg + r][16 * n + row] = acc[m][n][r];
+    __syncthreads();
+    for (int e = threadIdx.x; e < NCO * 16 * NT; e += 256) {
+        const int co = e / (16 * NT), ci = e % (16 * NT);
+        if (co < nco && ci < nci)
+            atomicAdd(&dW[((size_t)tap * nco + co) * nci + ci], red[0][co][ci] + red[1][co][ci] + red[2][co][ci] + red[3][co][ci]);
+    }
 }
 
 // bias gradient: db[co] += sum over the tile range of g[p][co]
@@ -169,7 +231,7 @@ __global__ void __launch_bounds__(256) k_bgrad(NetDims d, const u16 *__restrict_
     const int co = blockIdx.y;
     const int n = 16 * d.ntile;
     float s = 0.f;
-    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) s += bf2f(gT[(size_t)co * d.nc + d.lo + q]);
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < n; q += gridDim.x * 256) s += bf2f(gT[(size_t)co * d.ncs + d.lo + q]);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -189,13 +251,13 @@ __global__ void __launch_bounds__(256) k_dout(NetDims d, const float *__restrict
     for (int c = 0; c < NET_C; c++) {
         const u16 b = (c < 8 && in) ? f2bf(dout[(size_t)p * 8 + c]) : (u16)0;
         g[(size_t)p * NET_C + c] = b;
-        if (c < 16) gT[(size_t)c * nc + p] = b;
+        if (c < 16) gT[(size_t)c * d.ncs + p] = b;
     }
 }
 
 // state -> input features ((rho - rho0)/s_rho, v/s_v, (p - p0)/s_p, B/s_B), BF16, both layouts
 template <typename R>
-__global__ void __launch_bounds__(256) k_features(int nc, const R *__restrict__ Gas, const R *__restrict__ B, float gamma,
+__global__ void __launch_bounds__(256) k_features(int nc, int ncs, const R *__restrict__ Gas, const R *__restrict__ B, float gamma,
                                                   float4 sc, float2 off, u16 *__restrict__ act, u16 *__restrict__ actT) {
     int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nc) return;
@@ -210,7 +272,7 @@ __global__ void __launch_bounds__(256) k_features(int nc, const R *__restrict__ 
     for (int c = 0; c < NET_C; c++) {
         const u16 b = c < 8 ? f2bf(f[c]) : (u16)0;
         act[(size_t)p * NET_C + c] = b;
-        actT[(size_t)c * n + p] = b;
+        actT[(size_t)c * ncs + p] = b;
     }
 }
 
@@ -243,6 +305,15 @@ __global__ void __launch_bounds__(256) k_pack(const float *__restrict__ w, int o
     if (ci < NET_C && co < NET_C) Wb[((size_t)mt * NET_C + ci) * NET_C + co] = f2bf(val);
 }
 
+// the first layer's weights, 4 taps per K step: W0[g][co][8q + ci] = w[tap = 4g + q][co][ci]
+__global__ void __launch_bounds__(256) k_pack0(const float *__restrict__ w, u16 *__restrict__ W0) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= NET_G0 * NET_C * NET_C) return;
+    const int k = q % NET_C, co = (q / NET_C) % NET_C, g = q / (NET_C * NET_C);
+    const int tap = 4 * g + k / 8, ci = k % 8;
+    W0[q] = tap < NET_TAPS ? f2bf(w[(tap * NET_C + co) * 8 + ci]) : (u16)0;
+}
+
 // ----------------------------------------------------------------- host side
 struct gam_net {
     NetDims d;
@@ -252,6 +323,7 @@ struct gam_net {
     u16 *Wf[3] = {}, *Wb[3] = {};
     u16 *act[4] = {}, *actT[4] = {};     // 0 features, 1..2 hidden, 3 scratch gradient
     u16 *gr[2] = {}, *grT[2] = {};       // gradient ping-pong
+    u16 *actT0[4] = {}, *grT0[2] = {};   // allocations of the channel-major arrays (NET_PAD before row 0)
     float *out = nullptr;
     float *dbuf = nullptr;               // upstream gradient uploaded from the host
     float4 scales = {1.f, 1.f, 1.f, 1.f};
@@ -279,8 +351,8 @@ extern "C" int gam_net_destroy(gam_net *n) {
     void *bufs[] = {n->w, n->g, n->m, n->v, n->out, n->dbuf};
     for (void *b : bufs) if (b) (void)hipFree(b);
     for (int l = 0; l < 3; l++) { (void)hipFree(n->Wf[l]); (void)hipFree(n->Wb[l]); }
-    for (int q = 0; q < 4; q++) { (void)hipFree(n->act[q]); (void)hipFree(n->actT[q]); }
-    for (int q = 0; q < 2; q++) { (void)hipFree(n->gr[q]); (void)hipFree(n->grT[q]); }
+    for (int q = 0; q < 4; q++) { (void)hipFree(n->act[q]); (void)hipFree(n->actT0[q]); }
+    for (int q = 0; q < 2; q++) { (void)hipFree(n->gr[q]); (void)hipFree(n->grT0[q]); }
     if (n->stream) (void)hipStreamDestroy(n->stream);
     delete n;
     return 0;
@@ -290,10 +362,11 @@ extern "C" int gam_net_create(int Ni, int Nj, int Nk, gam_net **out) {
     if (!out || Ni < 8 || Nj < 8 || Nk < 8) return -1;   // the 3-layer stencil needs 3 cells around each used cell
     *out = nullptr;
     long nc = (long)Ni * Nj * Nk;
-    if (nc * NET_C > 0x7fffffffL) return -1;
+    if ((nc + 24) * NET_C + 2 * NET_PAD > 0x7fffffffL) return -1;
     gam_net *n = new gam_net();
     NetDims &d = n->d;
     d.Ni = Ni; d.Nj = Nj; d.Nk = Nk; d.nc = (int)nc;
+    d.ncs = (int)(((nc + 7) / 8) * 8 + 16);   // channel-major row stride: 16-byte aligned rows, a gap between them
     const int O = Ni * Nj + Ni + 1;
     d.lo = O;
     d.ntile = (int)((nc - 2L * O) / 16);
@@ -313,14 +386,16 @@ extern "C" int gam_net_create(int Ni, int Nj, int Nk, gam_net **out) {
         if (hipMalloc(&n->Wb[l], (size_t)NET_TAPS * NET_C * NET_C * 2) != hipSuccess) return cleanup(-3);
         if (hipMemset(n->Wb[l], 0, (size_t)NET_TAPS * NET_C * NET_C * 2) != hipSuccess) return cleanup(-2);
     }
-    const size_t ab = (size_t)nc * NET_C * 2;
+    const size_t ab = (size_t)nc * NET_C * 2, abT = ((size_t)d.ncs * NET_C + 2 * NET_PAD) * 2;
     for (int q = 0; q < 4; q++) {
-        if (hipMalloc(&n->act[q], ab) != hipSuccess || hipMalloc(&n->actT[q], ab) != hipSuccess) return cleanup(-3);
-        if (hipMemset(n->act[q], 0, ab) != hipSuccess || hipMemset(n->actT[q], 0, ab) != hipSuccess) return cleanup(-2);
+        if (hipMalloc(&n->act[q], ab) != hipSuccess || hipMalloc(&n->actT0[q], abT) != hipSuccess) return cleanup(-3);
+        if (hipMemset(n->act[q], 0, ab) != hipSuccess || hipMemset(n->actT0[q], 0, abT) != hipSuccess) return cleanup(-2);
+        n->actT[q] = n->actT0[q] + NET_PAD;
     }
     for (int q = 0; q < 2; q++) {
-        if (hipMalloc(&n->gr[q], ab) != hipSuccess || hipMalloc(&n->grT[q], ab) != hipSuccess) return cleanup(-3);
-        if (hipMemset(n->gr[q], 0, ab) != hipSuccess || hipMemset(n->grT[q], 0, ab) != hipSuccess) return cleanup(-2);
+        if (hipMalloc(&n->gr[q], ab) != hipSuccess || hipMalloc(&n->grT0[q], abT) != hipSuccess) return cleanup(-3);
+        if (hipMemset(n->gr[q], 0, ab) != hipSuccess || hipMemset(n->grT0[q], 0, abT) != hipSuccess) return cleanup(-2);
+        n->grT[q] = n->grT0[q] + NET_PAD;
     }
     if (hipMalloc(&n->out, (size_t)nc * 8 * sizeof(float)) != hipSuccess) return cleanup(-3);
     if (hipMalloc(&n->dbuf, (size_t)nc * 8 * sizeof(float)) != hipSuccess) return cleanup(-3);
@@ -335,7 +410,9 @@ extern "C" const float *gam_net_output(gam_net *n) { return n ? n->out : nullptr
 extern "C" float *gam_net_grad_buffer(gam_net *n) { return n ? n->dbuf : nullptr; }
 
 static int pack(gam_net *n) {
-    for (int l = 0; l < 3; l++) {
+    hipLaunchKernelGGL(k_pack0, dim3((NET_G0 * NET_C * NET_C + 255) / 256), 256, 0, n->stream, (const float *)n->w + woff(0),
+                       n->Wf[0]);
+    for (int l = 1; l < 3; l++) {
         const int total = NET_TAPS * kCoPad[l] * NET_C;
         hipLaunchKernelGGL(k_pack, dim3((total + 255) / 256), 256, 0, n->stream, (const float *)n->w, woff(l), kCo[l], kCi[l],
                            kCoPad[l], n->Wf[l], n->Wb[l]);
@@ -377,9 +454,9 @@ extern "C" int gam_net_set_offsets(gam_net *n, float rho0, float p0) {
 extern "C" int gam_net_features(gam_net *n, const void *Gas, const void *B, int precision) {
     if (!n || !Gas || !B) return -1;
     const int nb = (n->d.nc + 255) / 256;
-    if (precision) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_features<float>), dim3(nb), 256, 0, n->stream, n->d.nc, (const float *)Gas,
+    if (precision) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_features<float>), dim3(nb), 256, 0, n->stream, n->d.nc, n->d.ncs, (const float *)Gas,
                                       (const float *)B, n->gamma, n->scales, n->offs, n->act[0], n->actT[0]);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_features<double>), dim3(nb), 256, 0, n->stream, n->d.nc, (const double *)Gas,
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_features<double>), dim3(nb), 256, 0, n->stream, n->d.nc, n->d.ncs, (const double *)Gas,
                             (const double *)B, n->gamma, n->scales, n->offs, n->act[0], n->actT[0]);
     NCHK(n, hipGetLastError());
     return 0;
@@ -388,26 +465,26 @@ extern "C" int gam_net_features(gam_net *n, const void *Gas, const void *B, int 
 extern "C" int gam_net_set_features(gam_net *n, const float *feat) {
     // host features [nc][8] FP32 (tests): rounded to BF16 like k_features
     if (!n || !feat) return -1;
-    const size_t nc = n->d.nc;
-    std::vector<u16> a(nc * NET_C, 0), aT(nc * NET_C, 0);
+    const size_t nc = n->d.nc, ncs = n->d.ncs;
+    std::vector<u16> a(nc * NET_C, 0), aT(ncs * NET_C, 0);
     for (size_t p = 0; p < nc; p++)
-        for (int c = 0; c < 8; c++) { u16 b = f2bf(feat[p * 8 + c]); a[p * NET_C + c] = b; aT[c * nc + p] = b; }
+        for (int c = 0; c < 8; c++) { u16 b = f2bf(feat[p * 8 + c]); a[p * NET_C + c] = b; aT[c * ncs + p] = b; }
     NCHK(n, hipMemcpyAsync(n->act[0], a.data(), a.size() * 2, hipMemcpyHostToDevice, n->stream));
     NCHK(n, hipMemcpyAsync(n->actT[0], aT.data(), aT.size() * 2, hipMemcpyHostToDevice, n->stream));
     NCHK(n, hipStreamSynchronize(n->stream));
     return 0;
 }
 
-static dim3 conv_grid(const gam_net *n) { return dim3((unsigned)((n->d.ntile + 3) / 4)); }
+static dim3 conv_grid(const gam_net *n) { return dim3((unsigned)((n->d.ntile + 4 * NET_MT - 1) / (4 * NET_MT))); }
 
 extern "C" int gam_net_forward(gam_net *n) {
     if (!n) return -1;
     const NetDims &d = n->d;
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 0>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[0], (const u16 *)n->Wf[0],
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 0, 4>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[0], (const u16 *)n->Wf[0],
                        (const float *)(n->w + boff(0)), (const u16 *)nullptr, n->act[1], n->actT[1], (float *)nullptr);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 0>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[1], (const u16 *)n->Wf[1],
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 0, 1>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[1], (const u16 *)n->Wf[1],
                        (const float *)(n->w + boff(1)), (const u16 *)nullptr, n->act[2], n->actT[2], (float *)nullptr);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<16, 1>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[2], (const u16 *)n->Wf[2],
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<16, 1, 1>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[2], (const u16 *)n->Wf[2],
                        (const float *)(n->w + boff(2)), (const u16 *)nullptr, (u16 *)nullptr, (u16 *)nullptr, n->out);
     NCHK(n, hipGetLastError());
     return 0;
@@ -418,24 +495,25 @@ extern "C" int gam_net_forward(gam_net *n) {
 extern "C" int gam_net_backward(gam_net *n, const float *dout) {
     if (!n || !dout) return -1;
     const NetDims &d = n->d;
-    const int chunk = 2048;
-    const dim3 wg((unsigned)((16 * d.ntile + chunk - 1) / chunk), NET_TAPS);
+    const int chunk = 8192;   // cells per wave, a multiple of 32
+    const int nchunk = (d.lo + 16 * d.ntile - (d.lo & ~7) + chunk - 1) / chunk;
+    const dim3 wg((unsigned)(NET_TAPS * ((nchunk + 3) / 4)));
     hipLaunchKernelGGL(k_dout, dim3((d.nc + 255) / 256), 256, 0, n->stream, d, dout, n->gr[0], n->grT[0]);
     // layer 2 (linear, 8 outputs padded to 16)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<16>), wg, 64, 0, n->stream, d, (const u16 *)n->grT[0], (const u16 *)n->actT[2], chunk,
-                       kCo[2], kCi[2], n->g + woff(2));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<16, 2>), wg, 256, 0, n->stream, d, (const u16 *)n->grT[0], (const u16 *)n->actT[2], chunk,
+                       nchunk, kCo[2], kCi[2], n->g + woff(2));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgrad<16>), dim3(64, 8), 256, 0, n->stream, d, (const u16 *)n->grT[0], n->g + boff(2));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 2>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->gr[0], (const u16 *)n->Wb[2],
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 2, 1>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->gr[0], (const u16 *)n->Wb[2],
                        (const float *)nullptr, (const u16 *)n->act[2], n->gr[1], n->grT[1], (float *)nullptr);
     // layer 1
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<32>), wg, 64, 0, n->stream, d, (const u16 *)n->grT[1], (const u16 *)n->actT[1], chunk,
-                       kCo[1], kCi[1], n->g + woff(1));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<32, 2>), wg, 256, 0, n->stream, d, (const u16 *)n->grT[1], (const u16 *)n->actT[1], chunk,
+                       nchunk, kCo[1], kCi[1], n->g + woff(1));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgrad<32>), dim3(64, 32), 256, 0, n->stream, d, (const u16 *)n->grT[1], n->g + boff(1));
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 2>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->gr[1], (const u16 *)n->Wb[1],
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<32, 2, 1>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->gr[1], (const u16 *)n->Wb[1],
                        (const float *)nullptr, (const u16 *)n->act[1], n->gr[0], n->grT[0], (float *)nullptr);
     // layer 0 (input gradient not needed)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<32>), wg, 64, 0, n->stream, d, (const u16 *)n->grT[0], (const u16 *)n->actT[0], chunk,
-                       kCo[0], kCi[0], n->g + woff(0));
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wgrad<32, 1>), wg, 256, 0, n->stream, d, (const u16 *)n->grT[0], (const u16 *)n->actT[0], chunk,
+                       nchunk, kCo[0], kCi[0], n->g + woff(0));
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bgrad<32>), dim3(64, 32), 256, 0, n->stream, d, (const u16 *)n->grT[0], n->g + boff(0));
     NCHK(n, hipGetLastError());
     return 0;
