@@ -13,6 +13,10 @@ rocprofv3 CSV output.
   python -m kaiju_amd.profiling counters <counter_collection.csv> "<command>"
       per-stage mean per launch of every counter of one --pmc run, with the
       VALU-issue time SQ_INSTS_VALU implies when it holds SQ_INSTS_VALU.
+  python -m kaiju_amd.profiling mfma <kernel_stats.csv> <N> "<command>"
+      the learned-correction network's kernels of a --kernel-trace --stats run
+      on an N^3 block: algorithmic FLOPs per launch (real channel counts), the
+      FLOP rate of the mean launch and its fraction of the dense BF16 peak.
 """
 from __future__ import annotations
 
@@ -155,6 +159,49 @@ def counters(csv_path: str, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+# MI355X dense BF16 MFMA peak (no sparsity)
+BF16_PEAK_TFLOPS = 2500.0
+# per network kernel: multiply-adds per computed cell and per tap (real channels:
+# 8 -> 32 -> 32 -> 8); the backward-data launches alternate between the output
+# layer's (8 -> 32) and the hidden layer's (32 -> 32) and are averaged
+NET_MACS = {"k_conv<32, 0, 4>": 32 * 8, "k_conv<32, 0, 1>": 32 * 32, "k_conv<16, 1, 1>": 8 * 32,
+            "k_conv<32, 2, 1>": (8 * 32 + 32 * 32) / 2, "k_wgrad<32, 2>": 32 * 32, "k_wgrad<16, 2>": 8 * 32,
+            "k_wgrad<32, 1>": 32 * 8}
+NET_ROLE = {"k_conv<32, 0, 4>": "forward, layer 1 (8 -> 32, 4 taps per K)", "k_conv<32, 0, 1>": "forward, layer 2 (32 -> 32)",
+            "k_conv<16, 1, 1>": "forward, layer 3 (32 -> 8)", "k_conv<32, 2, 1>": "backward data, layers 3 and 2",
+            "k_wgrad<32, 2>": "weight gradient, layer 2", "k_wgrad<16, 2>": "weight gradient, layer 3",
+            "k_wgrad<32, 1>": "weight gradient, layer 1"}
+
+
+def mfma(stats_csv: str, N: int, command: str) -> str:
+    """Network kernels' FLOP rates on an N^3 block (ghost-inclusive box N + 8)."""
+    Ni = N + 8
+    nc = Ni ** 3
+    O = Ni * Ni + Ni + 1
+    cells = 16 * ((nc - 2 * O) // 16)   # the cells every convolution computes
+    lines = ["# Learned-correction network: MFMA kernels", "", "Command:", "", "```", command, "```", "",
+             f"Block {N}^3 (box {Ni}^3, {cells} cells computed per convolution).  Algorithmic FLOPs per launch = "
+             "2 x cells x 27 taps x (output x input channels, real counts 8 -> 32 -> 32 -> 8); rate = FLOPs / mean "
+             f"launch time; fraction of the {BF16_PEAK_TFLOPS:.0f} TFLOP/s dense BF16 MFMA peak.", "",
+             "| kernel | role | calls | mean (us) | GFLOP/launch | TFLOP/s | of peak |", "|---|---|---:|---:|---:|---:|---:|"]
+    tot_f = tot_t = 0.0
+    for r in csv.DictReader(open(stats_csv)):
+        short = r["Name"].replace("void ", "").split("(")[0]
+        if short not in NET_MACS:
+            continue
+        t = float(r["AverageNs"]) * 1e-9
+        f = 2.0 * cells * 27 * NET_MACS[short]
+        n = int(r["Calls"])
+        tot_f += f * n
+        tot_t += t * n
+        lines.append(f"| `{short}` | {NET_ROLE[short]} | {n} | {t * 1e6:.1f} | {f / 1e9:.1f} | {f / t / 1e12:.1f} | "
+                     f"{f / t / 1e12 / BF16_PEAK_TFLOPS * 100:.2f}% |")
+    if tot_t > 0:
+        lines += ["", f"All network launches: {tot_f / 1e12:.2f} TFLOP in {tot_t * 1e3:.1f} ms = "
+                      f"{tot_f / tot_t / 1e12:.1f} TFLOP/s, {tot_f / tot_t / 1e12 / BF16_PEAK_TFLOPS * 100:.2f}% of peak."]
+    return "\n".join(lines) + "\n"
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1] if len(sys.argv) > 1 else ""
     if cmd == "copy":
@@ -165,6 +212,8 @@ if __name__ == "__main__":
         print(traffic(*sys.argv[2:9]), end="")
     elif cmd == "counters":
         print(counters(sys.argv[2], sys.argv[3]), end="")
+    elif cmd == "mfma":
+        print(mfma(sys.argv[2], int(sys.argv[3]), sys.argv[4]), end="")
     else:
         print(__doc__)
         sys.exit(2)
