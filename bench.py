@@ -11,8 +11,11 @@ One step = one stepGamera: predictor, E field, stresses, CT/update, CalcDT,
 periodic BCs.  The headline value is the plain FP64 step; the same workload in
 FP32-bulk form (FP64 kept for magFlux/CT, the cell update and dt) is reported
 beside it under "fp32_bulk" with its L1 error per variable against the FP64
-run after the same steps.  value = steps * 256^3 / wall time of the timed steps (the
-reference's kZCs definition x 1e3, reference src/gamera/output.F90:64-85).
+run after the same steps, and the learned-coarse form (128^3 turbulence with the
+trained correction, judged against the 256^3 FP64 solution) under
+"learned_coarse" with its speed at equal L1 error.  value = steps * 256^3 /
+wall time of the timed steps (the reference's kZCs definition x 1e3, reference
+src/gamera/output.F90:64-85).
 
 Prints ONE JSON line.  ``--gpus N`` with N > 1 exits: this build runs one GPU.
 """
@@ -204,8 +207,10 @@ def msph_leg():
     app.close()
     log(f"[bench] msph {spec.N}: setup {setup:.1f} s, {spec.steps} steps in {wall:.3f} s, status {st}")
     return dict(workload="magnetosphere-like block: stretched spherical-polar 192x192x256 to the x axis (hole 1e-3 rad), "
-                         "r 2-30 R_E, Earth dipole background, solar wind, SphIn/SphOut, LFM axis BCs, ring average "
-                         "(12 rings), Boris 1000 km/s, FP32 bulk (FP64 CT/energy/dt)",
+                         "r 2-30 R_E, Earth dipole background, constant solar wind on the outer face (WindBC; the wind "
+                         "starts tapered to rest inside ~12 R_E), inner radial face SphIn (zero-gradient copy of the "
+                         "first active cell, standing in for the coupled run's fixed ionospheric boundary), LFM axis "
+                         "BCs, ring average (12 rings), Boris 1000 km/s, FP32 bulk (FP64 CT/energy/dt)",
                 value=round(spec.steps * spec.zones / wall, 1), unit="zone-updates/s",
                 ms_per_step=round(wall / spec.steps * 1e3, 4), steps=spec.steps, status=st,
                 t_end_code_units=t_end, dt_last=dt, divB_total_before=divb0, divB_total_after=divb,
@@ -317,6 +322,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-bulk leg")
     ap.add_argument("--no-msph", action="store_true", help="skip the magnetosphere leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-coarse leg")
     ap.add_argument("--msph-only", action="store_true",
                     help="run only the magnetosphere leg and print its JSON (profiling; not the bench line)")
     args = ap.parse_args()
@@ -361,6 +367,14 @@ def main():
         except Exception as e:   # reported beside the headline, never in place of it
             log(f"[bench] msph leg failed: {e!r}")
             msph = {"value": None, "error": repr(e)}
+    learned_leg = None
+    if not args.no_learned:
+        try:
+            from kaiju_amd import learned
+            learned_leg = learned.equal_error_leg(log=log)
+        except Exception as e:   # reported beside the headline, never in place of it
+            log(f"[bench] learned leg failed: {e!r}")
+            learned_leg = {"value": None, "error": repr(e)}
     cpu = None
     if not args.no_cpu:
         try:
@@ -387,6 +401,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32_bulk": fp32,
         "magnetosphere": msph,
+        "learned_coarse": learned_leg,
     }
     print(json.dumps(out), flush=True)
 

@@ -568,6 +568,96 @@ def experiment(N: int = 32, train_seeds=(0, 1, 2, 3), test_seed: int = 100, step
                 fine_s=round(t_fine, 2), train_s=round(t_train, 2), ms_per_iter=round(t_train / n_iter * 1e3, 3))
 
 
+def _timed_steps(app: GamApp, tr: Trajectory, n: int, corr: Optional[Corrector] = None) -> float:
+    """Mean wall time of n coarse steps from sample 0 (with the correction
+    inferred and applied before each step when corr is given)."""
+    load_pair(app, tr, 0)
+    app.sync()
+    t0 = time.perf_counter()
+    try:
+        for _ in range(n):
+            if corr is not None:
+                app.set_correction(corr.infer(), corr.norm.scale)
+            if app.steps(1) != 0:
+                raise RuntimeError("timed coarse step halted")
+        app.sync()
+    finally:
+        app.set_correction(None)
+    return (time.perf_counter() - t0) / n
+
+
+def equal_error_leg(N: int = 128, train_seeds=(0, 1), test_seed: int = 100, steps: int = 8, train_steps: int = 4,
+                    iters: int = 150, batch: int = 4, log=None) -> dict:
+    """The learned-coarse form at equal L1 error.  A network is trained on the
+    GPU on the solver's (2N)^3 FP64 trajectories of the training seeds,
+    downsampled 2x; on a held-out seed, N^3 FP64 runs with and without the
+    correction are compared with the (2N)^3 solution over `steps` coarse steps.
+    The plain scheme's convergence order p comes from the same comparison at
+    (N/2)^3 (reference downsampled 4x, steps of 2 dt), so the plain resolution
+    whose error equals the corrected run's is N_eq = N (E_plain/E_corr)^(1/p),
+    and the plain FP64 cost at that error is the measured N^3 step time times
+    (N_eq/N)^4 (zones x steps for the same physical time).  Errors: sum over
+    rho, v, p, B of the active-cell L1 error relative to each variable's mean
+    fluctuation, at the last step."""
+    log = log or (lambda *a: None)
+    ics = [TurbIC(seed=s) for s in train_seeds]
+    ic_test = TurbIC(seed=test_seed)
+    dt = min(choose_dt(x, N) for x in ics + [ic_test])
+    t0 = time.perf_counter()
+    trs = [fine_trajectory(x, N, train_steps, dt) for x in ics]
+    test = fine_trajectory(ic_test, N, steps, dt)
+    t_fine = time.perf_counter() - t0
+    log(f"[learned] fine {2 * N}^3 trajectories: {t_fine:.1f} s")
+    app = coarse_app(N)
+    pool, corr = [], None
+    try:
+        tgts = [trajectory_targets(app, tr) for tr in trs]
+        norm = normalisation(trs, tgts)
+        for tr, tl in zip(trs, tgts):
+            pool += trajectory_samples(tr, tl)
+        del tgts
+        corr = Corrector(app, norm)
+        t0 = time.perf_counter()
+        corr.train(pool, dt, iters, batch=batch)
+        app.sync()
+        t_train = time.perf_counter() - t0
+        log(f"[learned] training {iters} minibatches of {batch}: {t_train:.1f} s")
+        vn = var_norm(test)
+        e_plain = l1_total(rollout(app, test, steps)[-1], vn)
+        e_corr = l1_total(rollout(app, test, steps, corr)[-1], vn)
+        t_plain = _timed_steps(app, test, 20)
+        t_corr = _timed_steps(app, test, 20, corr)
+    finally:
+        for smp in pool:
+            smp.free()
+        if corr is not None:
+            corr.close()
+        app.close()
+    # plain at (N/2)^3, steps of 2 dt, against the reference downsampled once more
+    half = [downsample(g, m, test.nG) for g, m in zip(test.gas[::2], test.mag[::2])]
+    tr2 = Trajectory(N=N // 2, nG=test.nG, dt=2 * dt, gas=[g for g, _ in half], mag=[m for _, m in half])
+    with coarse_app(N // 2) as app2:
+        e_half = l1_total(rollout(app2, tr2, steps // 2)[-1], vn)
+    p = math.log2(e_half / e_plain) if e_half > e_plain > 0 else float("nan")
+    n_eq = N * (e_plain / e_corr) ** (1.0 / p) if p > 0 and e_corr > 0 else float("nan")
+    cost_eq = t_plain * (n_eq / N) ** 4
+    zones = N ** 3
+    log(f"[learned] L1 at {N}^3: plain {e_plain:.4f}, corrected {e_corr:.4f}; at {N // 2}^3 plain {e_half:.4f}; "
+        f"order {p:.2f}, N_eq {n_eq:.1f}; step {t_plain * 1e3:.3f} ms plain, {t_corr * 1e3:.3f} ms corrected")
+    return dict(workload=f"decaying MHD turbulence (Fourier modes k <= 4 in v and A, rms 1, rho = p = 1), {N}^3 FP64 "
+                         f"corrected by the 3-layer BF16 conv net, trained on {len(train_seeds)} seeds' {2 * N}^3 FP64 "
+                         f"trajectories ({train_steps} coarse steps each, {iters} minibatches of {batch}), judged on "
+                         f"held-out seed {test_seed} against its {2 * N}^3 FP64 solution over {steps} steps",
+                value=round(zones / t_corr, 1), unit="zone-updates/s",
+                ms_per_step_corrected=round(t_corr * 1e3, 4), ms_per_step_plain=round(t_plain * 1e3, 4),
+                l1_vs_fine_plain=round(e_plain, 5), l1_vs_fine_corrected=round(e_corr, 5),
+                l1_vs_fine_plain_half=round(e_half, 5), plain_order=round(p, 3),
+                plain_cells_at_equal_error=round(n_eq, 2),
+                plain_fp64_ms_at_equal_error=round(cost_eq * 1e3, 4),
+                speedup_at_equal_error=round(cost_eq / t_corr, 4),
+                fine_s=round(t_fine, 1), train_s=round(t_train, 1))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--N", type=int, default=32)
