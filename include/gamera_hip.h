@@ -150,7 +150,9 @@ int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt,
  * (lfmIn on INJ, lfmOut on OUTJ, :1020-1106; both j faces, k periodic over an
  * even Nkp >= 16: the ghosts across the axis from the cells half a turn away),
  * 6 solar wind (WindBC on OUTI, apps/wind.F90:206-374, with the constant wind of
- * gam_set_wind; k periodic; the inner i face keeps its own non-periodic BC). 
+ * gam_set_wind; k periodic; the inner i face keeps its own non-periodic BC),
+ * 7 cylindrical pole (cylpole on INI, bcs.F90:987-1017: j goes once around the
+ * pole, periodic over an even Njp >= 8; ghosts from the cells half a turn away). 
  * An axis is periodic on both faces or on neither; 2.5D blocks keep k periodic.
  * Default: all periodic.  The reference's BC objects (Grid%externalBCs) map
  * onto these. */

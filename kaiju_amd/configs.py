@@ -18,7 +18,7 @@ CFL_DEFAULT = 0.30000001192092896
 
 
 # the library's boundary types, by code (gam_set_bcs)
-BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere", "lfm", "wind")
+BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere", "lfm", "wind", "cylpole")
 
 
 def ring_chunks(Np: int) -> Tuple[int, ...]:
@@ -61,7 +61,7 @@ class CaseSpec:
     name: str
     N: Tuple[int, int, int]
     bounds: Tuple[float, float, float, float, float, float]
-    grid: str = "cart"          # "cart" | "warp3" | "refwarp"
+    grid: str = "cart"          # "cart" | "warp3" | "refwarp" | "cyl" | "sph"
     warp_amp: float = 0.0
     ic: str = "ot"              # "ot" | "bw"
     ic_params: dict = dataclasses.field(default_factory=dict)
@@ -121,6 +121,21 @@ def blast_wave(n: int = 256, steps: int = 100, warp_amp: float = 0.05, B0: float
                     ic_params=dict(x0=0.5, y0=0.5, z0=0.5, D0=1.0, P0=pout,
                                    pRat=pin / pout, dRat=1.0, rC=0.1, B0=B0),
                     model=m, steps=steps)
+
+
+def cylinder_blast(N=(16, 16, 8), steps: int = 8, r_in: float = 0.5, r_out: float = 1.5, **mkw) -> CaseSpec:
+    """Blast wave in a cylindrical shell (R in [r_in, r_out], phi once around,
+    z in [0, 1]; the reference's doCyl grid, init.F90:712-716) with the
+    through-pole BC on the inner radial face (cylpole, bcs.F90:987-1017; the
+    reference's ringBW pairs it with zero gradient outside, prob.F90:145-162)
+    and periodic phi and z.  The shell keeps every cell off the pole, so the
+    metric stays finite; the inner ghosts take the cells half a turn away."""
+    m = ModelSpec(gamma=5.0 / 3.0, CFL=0.3, **mkw)
+    nr, nph, nz = N
+    return CaseSpec(name=f"cylbw{nr}x{nph}x{nz}", N=(nr, nph, nz), bounds=(r_in, r_out, 0.0, 1.0, 0.0, 1.0),
+                    grid="cyl", ic="bw",
+                    ic_params=dict(x0=1.0, y0=0.0, z0=0.5, D0=1.0, P0=0.1, pRat=50.0, dRat=1.0, rC=0.2, B0=2.0),
+                    model=m, steps=steps, bcs=("cylpole", "zerograd", "periodic", "periodic", "periodic", "periodic"))
 
 
 def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in: float = 2.0, r_out: float = 30.0,

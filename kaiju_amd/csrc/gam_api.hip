@@ -1087,10 +1087,16 @@ static void launch_bc(gam_handle *h, bool withB) {
             continue;
         }
         const long n = (long)g.nG * ext[ax == 0 ? 1 : 0] * ext[ax == 2 ? 1 : 2];
-        for (int side = 0; side < 2; side++)
+        for (int side = 0; side < 2; side++) {
+            if (h->bct[2 * ax + side] == GAM_BC_CYLPOLE) {   // cylpole, the inner i face (bcs.F90:987-1017)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_cylpole<R>), grid1(n), 256, 0, h->stream, g, h->clk, (int)withB,
+                                   P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+                continue;
+            }
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1(n), 256, 0, h->stream, g, h->clk, 2 * ax + side,
                                h->bct[2 * ax + side], (int)withB, (const double *)h->grid[G_FACE],
                                (const double *)h->grid[G_TF], P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+        }
     }
 }
 
@@ -1099,8 +1105,10 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     if (refuse_if_captured(h, "gam_set_bcs")) return -1;
     for (int ax = 0; ax < 3; ax++) {
         const int a = type[2 * ax], b = type[2 * ax + 1];
-        if (a < 0 || a > GAM_BC_WIND || b < 0 || b > GAM_BC_WIND)
+        if (a < 0 || a > GAM_BC_CYLPOLE || b < 0 || b > GAM_BC_CYLPOLE)
             return fail(h, -1, "gam_set_bcs: unknown boundary type");
+        if ((a == GAM_BC_CYLPOLE && ax != 0) || b == GAM_BC_CYLPOLE)
+            return fail(h, -1, "gam_set_bcs: the cylindrical pole BC is the inner i face's BC");
         if (a == GAM_BC_WIND || (b == GAM_BC_WIND && ax != 0))
             return fail(h, -1, "gam_set_bcs: the solar-wind BC is the outer i face's BC");
         if ((a == GAM_BC_LFM || b == GAM_BC_LFM) && (ax != 1 || a != b))
@@ -1114,6 +1122,8 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
         return fail(h, -1, "gam_set_bcs: 2.5D blocks keep the k boundaries periodic");
     if (type[2] == GAM_BC_LFM && (type[4] != GAM_BC_PERIODIC || h->gv.Nkp < 16 || (h->gv.Nkp & 1)))
         return fail(h, -1, "gam_set_bcs: the LFM axis BCs need k periodic over an even Nkp >= 16");
+    if (type[0] == GAM_BC_CYLPOLE && (type[2] != GAM_BC_PERIODIC || h->gv.Njp < 8 || (h->gv.Njp & 1) || type[1] == GAM_BC_WIND))
+        return fail(h, -1, "gam_set_bcs: the cylindrical pole BC needs j periodic over an even Njp >= 8 (no wind)");
     if (type[1] == GAM_BC_WIND && (!h->wind.on || type[4] != GAM_BC_PERIODIC || type[0] == GAM_BC_PERIODIC))
         return fail(h, -1, "gam_set_bcs: the solar-wind BC needs gam_set_wind first, k periodic and a non-periodic i axis");
     if (h->corr && !(type[0] == 0 && type[2] == 0 && type[4] == 0))

@@ -1227,7 +1227,7 @@ __global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__
 // GAM_BC_SPHERE: SphIn (INI, the same assignments as zeroExten_ibcI) and SphOut
 // (OUTI: every ghost face and cell from the last cell ie), bcs.F90:934-982.
 enum { GAM_BC_PERIODIC = 0, GAM_BC_ZEROGRAD = 1, GAM_BC_ZEROEXTEN = 2, GAM_BC_REFLECT = 3, GAM_BC_SPHERE = 4, GAM_BC_LFM = 5,
-       GAM_BC_WIND = 6 };
+       GAM_BC_WIND = 6, GAM_BC_CYLPOLE = 7 };
 template <typename R>
 __global__ void __launch_bounds__(256) k_bc_face(GVT<R> g, const DClock *__restrict__ clk, int face, int type, int withB,
                                                  const double *__restrict__ face64, const double *__restrict__ Tf64,
@@ -1316,6 +1316,39 @@ __global__ void __launch_bounds__(256) k_cvt(const TI *__restrict__ in, TO *__re
 }
 
 // ----------------------------------------------------------------- diagnostics
+// bcs.F90:987-1017 cylpole, the inner i face of a cylindrical grid whose j
+// index goes once around the pole (Np = Njp cells, even): ghost layer n of
+// column j takes the first cells half a turn away, jp = j + Np/2 wrapped past
+// Np, the I and J face fluxes negated (their normals turn over across the
+// pole), the K flux and the cell values as they are.  Every (j, k) of the cell
+// box, as the reference loops jsg..jeg and ksg..keg (the J faces at jeg+1 and
+// the K faces at keg+1 are left alone); sources are owned cells, so one pass.
+template <typename R>
+__global__ void __launch_bounds__(256) k_bc_cylpole(GVT<R> g, const DClock *__restrict__ clk, int withB,
+                                                    R *__restrict__ Gas, double *__restrict__ mag, R *__restrict__ B) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)g.nG * g.Nj * g.Nk || clk->halt) return;
+    const int nl = 1 + (int)(idx % g.nG);   // ghost layer fastest
+    const long r = idx / g.nG;
+    const int j = (int)(r % g.Nj), k = (int)(r / g.Nj);
+    const int Np = g.Njp;
+    int jp = j + Np / 2;                    // the reference's jp = j + Np2, 1-based active j
+    if (jp - g.nG + 1 > Np) jp -= Np;
+    const int ig = g.nG - nl, is = g.nG + nl - 1;
+    const size_t nc = g.nc, nf = g.nf;
+    const int c = ci(g, ig, j, k), cs = ci(g, is, jp, k);
+#pragma unroll
+    for (int v = 0; v < 5; v++) Gas[nc * v + c] = Gas[nc * v + cs];
+    if (withB) {
+#pragma unroll
+        for (int x = 0; x < 3; x++) B[nc * x + c] = B[nc * x + cs];
+    }
+    const int f = fi(g, ig, j, k);
+    mag[f] = -1.0 * mag[fi(g, is + 1, jp, k)];
+    mag[nf + f] = -1.0 * mag[nf + fi(g, is, jp, k)];
+    mag[2 * nf + f] = mag[2 * nf + fi(g, is, jp, k)];
+}
+
 __global__ void k_divb(GV g, const double *__restrict__ mag, double *__restrict__ partial) {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;

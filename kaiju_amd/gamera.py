@@ -222,6 +222,11 @@ def gen_corners(spec: configs.CaseSpec) -> np.ndarray:
         xyz[..., 0] += s * Lx
         xyz[..., 1] += s * Ly
         xyz[..., 2] += s * Lz
+    elif spec.grid == "cyl":   # init.F90:712-716: (x1, x2, x3) = (R, phi / 2 pi, z)
+        R, ph = X[:, None], 2 * math.pi * Y[None, :]
+        xyz[..., 0] = (R * np.cos(ph))[..., None]
+        xyz[..., 1] = (R * np.sin(ph))[..., None]
+        xyz[..., 2] = Z[None, None, :]
     elif spec.grid == "sph":
         from kaiju_amd import msph
         p = spec.ic_params

@@ -70,6 +70,7 @@ def lib():
             "orc_grid_derive": (None, [G]),
             "orc_gen_corners_cart": (None, [G, _dp, C.c_int]),
             "orc_gen_corners_warp3": (None, [G, _dp, C.c_double]),
+            "orc_gen_corners_cyl": (None, [G, _dp]),
             "orc_gen_corners_refwarp": (None, [G, _dp, C.c_double, C.c_double, C.c_int, C.c_int]),
             "orc_corners2grid": (None, [G]),
             "orc_ic_ot": (None, [M, G, S, C.c_double, C.c_double, C.c_double, C.c_double]),
@@ -238,6 +239,8 @@ class OracleCase:
             L.orc_gen_corners_warp3(C.byref(self.g), bds, sp.warp_amp)
         elif sp.grid == "refwarp":
             L.orc_gen_corners_refwarp(C.byref(self.g), bds, sp.warp_amp, 1.0, 1, int(sp.do25D))
+        elif sp.grid == "cyl":
+            L.orc_gen_corners_cyl(C.byref(self.g), bds)
         else:
             raise ValueError(sp.grid)
 

@@ -1293,8 +1293,28 @@ double orc_calc_dt(orc_model *m, const orc_grid *g, orc_state *s) {
  * reference's choices of source component and of which routines write Bxyz. */
 static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s);
 
+/* bcs.F90:987-1017 cylpole: inner i face of a cylindrical grid, Np = Njp cells
+ * around the pole */
+static void bc_cylpole(const orc_grid *g, orc_state *s) {
+    const size_t nc = g->nc, nf = g->nf;
+    const int Np = g->Njp, Np2 = Np / 2;
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int j = g->jsg; j <= g->jeg; j++)
+            for (int n = 1; n <= g->nG; n++) {
+                int jp = j + Np2;
+                if (jp > Np) jp = jp - Np;
+                const size_t c = CI(g, g->is - n, j, k), cs = CI(g, g->is + n - 1, jp, k);
+                for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = s->Gas[cs + nc * v];
+                s->magFlux[FI(g, g->is - n, j, k)] = -1.0 * s->magFlux[FI(g, g->is + n, jp, k)];
+                s->magFlux[FI(g, g->is - n, j, k) + nf] = -1.0 * s->magFlux[FI(g, g->is + n - 1, jp, k) + nf];
+                s->magFlux[FI(g, g->is - n, j, k) + 2 * nf] = s->magFlux[FI(g, g->is + n - 1, jp, k) + 2 * nf];
+                for (int v = 0; v < 3; v++) s->Bxyz[c + nc * v] = s->Bxyz[cs + nc * v];
+            }
+}
+
 static void bc_face(const orc_model *m, const orc_grid *g, orc_state *s, int face, int type) {
     if (type == ORC_BC_LFM) { bc_lfm(g, s, face & 1); return; }
+    if (type == ORC_BC_CYLPOLE) { bc_cylpole(g, s); return; }
     if (type == ORC_BC_WIND) { bc_wind(m, g, s); return; }
     const size_t nc = g->nc, nf = g->nf;
     const int nG = g->nG, ax = face / 2, out = face & 1;
@@ -1641,6 +1661,21 @@ void orc_gen_corners_cart(const orc_grid *g, const double bds[6], int do25D) {
                 g->xyz[f] = bds[0] + (i - 1) * dx;
                 g->xyz[f + nf] = bds[2] + (j - 1) * dy;
                 g->xyz[f + 2 * nf] = bds[4] + (k - 1) * dz;
+            }
+}
+
+void orc_gen_corners_cyl(const orc_grid *g, const double bds[6]) {
+    /* init.F90:669-731 genGridXML with doCyl: (x1, x2, x3) = (R, phi/2pi, z) */
+    size_t nf = g->nf;
+    double dx = (bds[1] - bds[0]) / g->Nip, dy = (bds[3] - bds[2]) / g->Njp, dz = (bds[5] - bds[4]) / g->Nkp;
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int i = g->isg; i <= g->ieg + 1; i++) {
+                size_t f = FI(g, i, j, k);
+                const double x1 = bds[0] + (i - 1) * dx, x2 = bds[2] + (j - 1) * dy, x3 = bds[4] + (k - 1) * dz;
+                g->xyz[f] = x1 * cos(2 * PI * x2);
+                g->xyz[f + nf] = x1 * sin(2 * PI * x2);
+                g->xyz[f + 2 * nf] = x3;
             }
 }
 

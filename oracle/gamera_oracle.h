@@ -43,7 +43,7 @@ typedef struct {
  * through-the-axis LFM pole (lfmIn on INJ, lfmOut on OUTJ), solar wind (WindBC on
  * OUTI, apps/wind.F90:206-374, with a wind constant in time) */
 enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3, ORC_BC_SPHERE = 4,
-       ORC_BC_LFM = 5, ORC_BC_WIND = 6 };
+       ORC_BC_LFM = 5, ORC_BC_WIND = 6, ORC_BC_CYLPOLE = 7 };
 #define ORC_MAX_RINGS 32
 
 /* Grid_T subset (reference gamtypes.F90:163-245) */
@@ -94,6 +94,7 @@ void   orc_grid_derive(orc_grid *g);
 
 /* setup producers */
 void   orc_gen_corners_cart(const orc_grid *g, const double bds[6], int do25D);
+void   orc_gen_corners_cyl(const orc_grid *g, const double bds[6]);
 void   orc_gen_corners_warp3(const orc_grid *g, const double bds[6], double amp);
 void   orc_gen_corners_refwarp(const orc_grid *g, const double bds[6], double w0, double R, int Nw, int do25D);
 void   orc_corners2grid(const orc_grid *g);

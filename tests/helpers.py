@@ -61,14 +61,16 @@ def small_cases():
     # Boris with Ca below the flow speed: |v| reaches 1.5 Ca (the nuke of
     # stress.F90:399-414 fires) and alpha = |B|^2/(rho Ca^2) ~ 1
     nuke = configs.orszag_tang(32, 4, steps=8, doBoris=True, Ca=0.6)
+    # cylindrical shell with the through-pole inner BC (cylpole) and zero gradient outside
+    cyl = configs.cylinder_blast((12, 16, 8), steps=6)
     for s, n in ((ot, "ot32x32x4"), (ot8, "ot32x32x4_8cent"), (ot25, "ot32x32x1_25d"), (bw, "bw16_warp"),
                  (bwb, "bw16_warp_boris"), (hyd, "bw16_hydro"), (nolfm, "ot24x24x4_nolfmlim"),
                  (thin, "bw24x2x3_thin"), (dip, "bw16_warp_dipole"), (dipb, "bw16_warp_dipole_boris"),
                  (bcA, "ot24_bc_exten_reflect_grad"), (bcB, "ot24_bc_grad_exten_reflect_boris"),
                  (bcC, "ot24_bc_reflect_grad_exten"), (bcS, "ot24_bc_sphere"), (cpr, "bw16_warp_cpr"),
-                 (nuke, "ot32x32x4_boris_nuke")):
+                 (nuke, "ot32x32x4_boris_nuke"), (cyl, "cyl12x16x8_cylpole")):
         s.name = n
-    return [ot, ot8, ot25, bw, bwb, hyd, nolfm, thin, dip, dipb, bcA, bcB, bcC, bcS, cpr, nuke]
+    return [ot, ot8, ot25, bw, bwb, hyd, nolfm, thin, dip, dipb, bcA, bcB, bcC, bcS, cpr, nuke, cyl]
 
 
 # B0 = (0.5, -1, 0.25) + dipole of moment (0,0,8) at (-0.7, 0.45, 0.6): |B0| ~ 1-20 in the unit cube

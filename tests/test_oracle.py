@@ -1100,3 +1100,63 @@ def test_nuke_case_reaches_the_branch():
     interfaces = (ni + 1) * nj * nk + ni * (nj + 1) * nk + ni * nj * (nk + 1)
     assert 0.05 * interfaces < first < interfaces, (first, interfaces)
     assert total > first
+
+
+def test_cylpole_ghosts_are_the_cells_across_the_pole():
+    """cylpole (bcs.F90:987-1017) against the geometry it stands for.  On a
+    cylinder whose radius starts at the axis, the reference's cylindrical
+    corners put inner ghost layer n (R < 0) on the cells half a turn away.  The
+    test finds, from the corners alone, the physical cell and faces each ghost
+    is (same eight or four points), and checks: a state that is a function of
+    position (cell values of the cell centroid) reappears in the ghosts, and a
+    uniform field's ghost face fluxes are B . n A with the partner face's area
+    and the ghost normals RingGridFix gives a cylinder's ghosts (I and J
+    normals reversed, K kept: ringutils.F90:305-307), to rounding.  A wrong
+    partner column, the I flux taken one face off, or a sign off on any
+    component breaks it."""
+    spec = configs.cylinder_blast((6, 16, 4), r_in=0.0, r_out=1.0)
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.gen_corners()    # corners only: the metric divides zero by zero on the axis faces
+    xyz = oc.grid["xyz"]
+    Ni, Nj, Nk = oc.G
+    nG = spec.nG
+    Np = spec.N[1]
+    cen = sum(xyz[a:a + Ni, b:b + Nj, c:c + Nk] for a in (0, 1) for b in (0, 1) for c in (0, 1)) / 8.0
+    x, y, z = cen[..., 0], cen[..., 1], cen[..., 2]
+    fields = [2.0 + np.sin(x) * np.cos(2 * y), x * x - 0.5 * y, 0.3 * y + z, np.cos(3 * z) + x * y,
+              4.0 + np.exp(-(x * x + y * y))]
+    B0 = np.array([0.7, -1.3, 0.4])
+
+    def sl(i0, j0, k0):
+        return xyz[i0:i0 + Ni, j0:j0 + Nj, k0:k0 + Nk]
+    # each face's four corners, in the order its two tangents turn: I (j, k), J (k, i), K (i, j)
+    quads = {0: (sl(0, 0, 0), sl(0, 1, 0), sl(0, 1, 1), sl(0, 0, 1)),
+             1: (sl(0, 0, 0), sl(0, 0, 1), sl(1, 0, 1), sl(1, 0, 0)),
+             2: (sl(0, 0, 0), sl(1, 0, 0), sl(1, 1, 0), sl(0, 1, 0))}
+    area = {d: 0.5 * np.cross(q[2] - q[0], q[3] - q[1]) for d, q in quads.items()}   # vector area, corner-oriented
+    pts = lambda d, i, j, k: np.sort(np.round(np.array([q[i, j, k] for q in quads[d]]), 12), axis=0)
+    s = oc.state
+    for v in range(5):
+        s.Gas[..., v] = fields[v]
+    for d in range(3):
+        s.magFlux[:Ni, :Nj, :Nk, d] = area[d] @ B0
+    s.Gas[:nG] = 99.0
+    s.magFlux[:nG] = 99.0
+    L.orc_enforce_bcs(C.byref(oc.m), C.byref(oc.g), C.byref(s.s))
+    sign = {0: -1.0, 1: -1.0, 2: 1.0}   # RingGridFix's ghost normals for a cylinder
+    scale = max(np.abs(f).max() for f in fields)
+    fscale = np.abs(area[0] @ B0).max()
+    for n in range(1, nG + 1):
+        ig = nG - n
+        for j in range(nG, nG + Np):
+            half = (j + Np // 2 - nG) % Np + nG          # the column half a turn away
+            for k in range(nG, Nk - nG):
+                # the cell's partner: the active cell with the same corners
+                pc = nG + n - 1
+                assert np.allclose(cen[ig, j, k], cen[pc, half, k], atol=1e-12)
+                assert np.abs(s.Gas[ig, j, k] - [f[pc, half, k] for f in fields]).max() <= 1e-12 * scale
+                for d in range(3):
+                    pf = pc + 1 if d == 0 else pc            # the poleward I face is one further out
+                    assert np.array_equal(pts(d, ig, j, k), pts(d, pf, half, k)), (d, n, j, k)
+                    want = sign[d] * (area[d][pf, half, k] @ B0)
+                    assert abs(s.magFlux[ig, j, k, d] - want) <= 1e-12 * fscale, (d, n, j, k)
