@@ -149,8 +149,9 @@ int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt,
  * radial (SphIn on INI, SphOut on OUTI, :934-982; i faces only), 5 LFM axis
  * (lfmIn on INJ, lfmOut on OUTJ, :1020-1106; both j faces, k periodic over an
  * even Nkp >= 16: the ghosts across the axis from the cells half a turn away),
- * 6 solar wind (WindBC on OUTI, apps/wind.F90:206-374, with the constant wind of
- * gam_set_wind; k periodic; the inner i face keeps its own non-periodic BC),
+ * 6 solar wind (WindBC on OUTI, apps/wind.F90:206-374, with the wind of
+ * gam_set_wind or gam_set_wind_series; k periodic; the inner i face keeps its
+ * own non-periodic BC),
  * 7 cylindrical pole (cylpole on INI, bcs.F90:987-1017: j goes once around the
  * pole, periodic over an even Njp >= 8; ghosts from the cells half a turn away). 
  * An axis is periodic on both faces or on neither; 2.5D blocks keep k periodic.
@@ -162,6 +163,24 @@ int  gam_get_bcs(gam_handle *h, int *type);
  * GAM_BC_WIND boundary, constant in time: the reference's WindBC reading a
  * constant time series (GetWindAt, apps/wind.F90:160-177).  Before gam_set_bcs. */
 int  gam_set_wind(gam_handle *h, double rho, double p, const double *v, const double *b);
+/* The solar wind of the GAM_BC_WIND boundary as a time series, replacing
+ * readWind (apps/wind.F90:617-760) after its unit conversion: nt >= 2 samples
+ * at times t (code units, increasing, uniform to 1e-6 as readWind demands) of
+ * density rho, pressure p, velocity v and field b (v, b, xyzBS: (nt, 3), each
+ * component's series contiguous, as readWind's Q(:,VELX:VELZ), B and xyzBS
+ * hold them); b's x component is replaced by
+ * Bx0 + ByC By + BzC Bz (the tilted front, wind.F90:718-721); xyzBS, the
+ * series' marker position (the bow shock, xBS/yBS/zBS), may be NULL (zero).
+ * The boundary then takes, at the clock's time t, each point's wind at t - DelT
+ * (GetWindAt, :160-177), DelT the travel time of the front (RefreshWind,
+ * :116-158) from the marker, from the sample whose interval holds that time
+ * (discrete walls, InterpWgts :819-868), with a null field at or before the
+ * first sample.  Refuses a wind at or above the Boris speed (doBoris), as
+ * readWind does.  Before gam_set_bcs; gam_set_wind switches back to a
+ * constant wind. */
+int  gam_set_wind_series(gam_handle *h, int nt, const double *t, const double *rho, const double *p,
+                         const double *v, const double *b, const double *xyzBS, double ByC, double BzC,
+                         double Bx0);
 
 /* --- ring average about the LFM axis (Model%doRing, ring/gid = "lfm") ----- */
 /* Replaces InitRings/RingGridFix (ringutils.F90:37-390) and turns on, inside
@@ -276,7 +295,7 @@ int  gam_get_tile(gam_handle *h, int stage, int *tile4);
  * other call on them.  Stage timers must be off while capturing.  The graph
  * holds each handle's buffers and launch parameters as they were at capture:
  * until gam_graph_destroy, gam_set_tile, gam_set_ring, gam_set_bcs,
- * gam_set_wind, gam_set_correction, gam_set_background,
+ * gam_set_wind, gam_set_wind_series, gam_set_correction, gam_set_background,
  * gam_background_dipole and gam_destroy return -1 on a held handle, and
  * gam_graph_launch returns -1 if a handle that has not halted was stepped an
  * odd number of times outside the graph (its State and oState buffers are then

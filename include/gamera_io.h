@@ -10,6 +10,8 @@
  *                 datasets X, Y, Z, Gas, magFlux, Bxyz, oGas, omagFlux, oBxyz
  *                 (src/gamera/gioH5.F90:650-704 writeH5Res, :707-872
  *                 readH5Restart)
+ *   bcwind:       the solar-wind time series (src/gamera/apps/wind.F90:617-760
+ *                 readWind), below
  *
  * Arrays are Fortran order (i fastest) with the ghost cells, the layout of
  * gam_put_state/gam_get_state: xyz (Ni+1, Nj+1, Nk+1, 3), Gas (Ni, Nj, Nk, 5),
@@ -67,6 +69,30 @@ int gam_io_write_restart(const char *path, int nG, int Nip, int Njp, int Nkp, in
  * magFlux. */
 int gam_io_read_restart(const char *path, int nG, int Nip, int Njp, int Nkp, int doMHD, gam_io_clock *c,
                         double *Gas, double *magFlux, double *oGas, double *omagFlux);
+
+/* bcwind: the solar-wind time series readWind ingests (reference
+ * src/gamera/apps/wind.F90:617-760), in the file's units (t [s], D [#/cc],
+ * V [m/s], Temp [K], B [nT]): datasets T, D, Vx, Vy, Vz, Temp, Bx, By, Bz of
+ * nt samples each; the front coefficients ByC, BzC, Bx0 (Bx0 in nT), each a
+ * root attribute or a dataset, absent ones 0; the marker xBS, yBS, zBS
+ * (datasets, in planet radii), absent: the origin.  V, B and xyzBS are (nt, 3)
+ * Fortran-ordered (each component's series contiguous).  A file without Temp
+ * is refused, as readWind refuses it (:641-646). */
+typedef struct {
+    double ByC, BzC, Bx0;   /* Bx0 in nT, as the file holds it */
+    int hasCoef[3];         /* ByC, BzC, Bx0 present */
+    int hasBS;              /* xBS present */
+} gam_io_wind_info;
+
+/* Samples in the file (the length of T). */
+int gam_io_wind_len(const char *path, int *nt);
+/* The file's series; nt must be gam_io_wind_len's. */
+int gam_io_read_wind(const char *path, int nt, double *T, double *D, double *V, double *Temp, double *B,
+                     double *xyzBS, gam_io_wind_info *info);
+/* A bcwind file of those series; coef (ByC, BzC, Bx0) and xyzBS may be NULL
+ * (not written).  The coefficients become root attributes. */
+int gam_io_write_wind(const char *path, int nt, const double *T, const double *D, const double *V,
+                      const double *Temp, const double *B, const double *xyzBS, const double *coef);
 
 const char *gam_io_last_error(void);
 

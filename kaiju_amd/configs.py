@@ -13,6 +13,8 @@ import dataclasses
 import math
 from typing import Optional, Tuple
 
+import numpy as np
+
 # init.F90:32 MaxCFL = 0.3 is a default-real literal, widened to double.
 CFL_DEFAULT = 0.30000001192092896
 
@@ -78,6 +80,8 @@ class CaseSpec:
     ring: Optional[Tuple[int, ...]] = None
     # constant solar wind (rho, p, v[3], b[3]) of a "wind" outer i boundary
     wind: Optional[tuple] = None
+    # or a time series of it (takes precedence over wind)
+    wind_series: Optional["WindSeries"] = None
 
     @property
     def zones(self) -> int:
@@ -86,6 +90,33 @@ class CaseSpec:
     @property
     def do25D(self) -> bool:
         return self.N[2] == 1
+
+
+@dataclasses.dataclass
+class WindSeries:
+    """A solar-wind time series in code units, as readWind
+    (apps/wind.F90:617-760) holds it after its unit conversion: sample times t
+    (increasing, uniformly spaced), density rho, pressure p, velocity v and
+    field b (nt x 3; b's x component is replaced by Bx0 + ByC By + BzC Bz),
+    the marker position xyzBS (nt x 3, the bow shock; None: the origin) and
+    the front coefficients ByC, BzC, Bx0."""
+    t: np.ndarray
+    rho: np.ndarray
+    p: np.ndarray
+    v: np.ndarray
+    b: np.ndarray
+    xyzBS: Optional[np.ndarray] = None
+    ByC: float = 0.0
+    BzC: float = 0.0
+    Bx0: float = 0.0
+
+    def arrays(self):
+        """FP64 arrays (t, rho, p, v, b, xyzBS or None), the (nt, 3) ones in
+        Fortran order (each component's series contiguous)."""
+        f = lambda a, shp: np.asfortranarray(np.asarray(a, dtype=np.float64).reshape(shp))
+        nt = len(self.t)
+        return (f(self.t, (nt,)), f(self.rho, (nt,)), f(self.p, (nt,)), f(self.v, (nt, 3)), f(self.b, (nt, 3)),
+                None if self.xyzBS is None else f(self.xyzBS, (nt, 3)))
 
 
 def orszag_tang(n=256, nk: int = 4, steps: int = 200, **mkw) -> CaseSpec:

@@ -62,7 +62,8 @@ struct gam_handle {
     double *ringMean = nullptr;        // ring-average scratch (chunk means, Fourier coefficients)
     size_t ringMean_n = 0;             // its capacity in doubles
     int fused_np = 0;                  // block minima the fused step's CalcDT reduces
-    WindP wind = {};                   // the solar wind of a GAM_BC_WIND outer i face (gam_set_wind)
+    WindP wind = {};                   // the solar wind of a GAM_BC_WIND outer i face (gam_set_wind*)
+    double *windSer = nullptr;         // its time series (nt x WIND_COLS), wind.ser points here
     bool corners_set = false;          // geometry derived by gam_set_corners (RingGridFix applied then)
     double *ctG = nullptr, *ctM = nullptr, *crG = nullptr, *crM = nullptr;   // correction training buffers
     BGV bgv = {};              // device view of bg; bgv.on = 0 until a background is set
@@ -325,7 +326,7 @@ extern "C" int gam_destroy(gam_handle *h) {
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
     for (int q = 0; q < 9; q++) if (h->grid32[q]) (void)hipFree(h->grid32[q]);
     for (int q = 0; q < 5; q++) if (h->bg[q]) (void)hipFree(h->bg[q]);
-    for (double *b : {h->ctG, h->ctM, h->crG, h->crM, h->ringSE, h->ringTab, h->ringMean}) if (b) (void)hipFree(b);
+    for (double *b : {h->ctG, h->ctM, h->crG, h->crM, h->ringSE, h->ringTab, h->ringMean, h->windSer}) if (b) (void)hipFree(b);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }
     void *bufs[13] = {h->hGas, h->hmag, h->hB, h->Vf, h->E, h->Flx, h->partial, h->clk,
                       h->wk[0], h->wk[1], h->wk[2], h->wk[3], h->wk[4]};
@@ -1147,9 +1148,62 @@ extern "C" int gam_set_wind(gam_handle *h, double rho, double p, const double *v
         return fail(h, -1, "gam_set_wind: density, pressure and speed must be positive");
     int rc = gam_sync(h);
     if (rc) return rc;
+    if (h->windSer) HIPCHK(h, hipFree(h->windSer));
+    h->windSer = nullptr;
+    h->wind = WindP{};
     h->wind.rho = rho; h->wind.p = p;
     for (int x = 0; x < 3; x++) { h->wind.v[x] = v[x]; h->wind.b[x] = b[x]; }
     h->wind.on = 1;
+    return 0;
+}
+
+// readWind (wind.F90:617-760) after its unit conversion, and InitWind's
+// discrete series (wind.F90:69-111): the checks it makes (uniform spacing to
+// 1e-6, the wind slower than the Boris speed), Bx replaced by
+// Bx0 + ByC By + BzC Bz, the bow-shock marker xyzBS zero when not given.
+extern "C" int gam_set_wind_series(gam_handle *h, int nt, const double *t, const double *rho, const double *p,
+                                   const double *v, const double *b, const double *xyzBS, double ByC, double BzC,
+                                   double Bx0) {
+    if (!h || !t || !rho || !p || !v || !b) return -1;
+    if (refuse_if_captured(h, "gam_set_wind_series")) return -1;
+    if (nt < 2) return fail(h, -1, "gam_set_wind_series: the series needs at least two samples");
+    double dtMin = t[1] - t[0], dtMax = dtMin, tMin = t[0], tMax = t[0], vMax = 0.0;
+    for (int n = 0; n < nt; n++) {
+        if (n > 0) { dtMin = fmin(dtMin, t[n] - t[n - 1]); dtMax = fmax(dtMax, t[n] - t[n - 1]); }
+        tMin = fmin(tMin, t[n]); tMax = fmax(tMax, t[n]);
+        const double vx = v[n], vy = v[nt + n], vz = v[2 * nt + n];
+        vMax = fmax(vMax, sqrt(vx * vx + vy * vy + vz * vz));
+        if (!(rho[n] > 0) || !(p[n] > 0) || !(vx * vx + vy * vy + vz * vz > 0))
+            return fail(h, -1, "gam_set_wind_series: density, pressure and speed must be positive");
+    }
+    if (!(dtMin > 0)) return fail(h, -1, "gam_set_wind_series: the sample times must increase");
+    if (fabs(dtMin - dtMax) > 1.0e-6 * sqrt(dtMin * dtMin + dtMax * dtMax))
+        return fail(h, -1, "gam_set_wind_series: the series requires uniform spacing");
+    if (h->mp.doBoris && vMax >= h->mp.Ca)
+        return fail(h, -1, "gam_set_wind_series: the solar wind is faster than the Boris speed of light");
+    std::vector<double> tab((size_t)nt * WIND_COLS);
+    for (int n = 0; n < nt; n++) {
+        double *r = &tab[(size_t)WIND_COLS * n];
+        r[0] = t[n]; r[1] = rho[n]; r[2] = p[n];
+        for (int x = 0; x < 3; x++) {
+            const size_t q = (size_t)nt * x + n;
+            r[3 + x] = v[q]; r[6 + x] = b[q]; r[9 + x] = xyzBS ? xyzBS[q] : 0.0;
+        }
+        r[6] = Bx0 + ByC * r[7] + BzC * r[8];
+    }
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    if (h->windSer) HIPCHK(h, hipFree(h->windSer));
+    h->windSer = nullptr;
+    h->wind = WindP{};   // off until the new table is in place
+    HIPCHK(h, hipMalloc(&h->windSer, tab.size() * sizeof(double)));
+    HIPCHK(h, hipMemcpy(h->windSer, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    WindP w = {};
+    w.rho = rho[0]; w.p = p[0];
+    for (int x = 0; x < 3; x++) { w.v[x] = tab[3 + x]; w.b[x] = tab[6 + x]; }
+    w.on = 1; w.nt = nt; w.ser = h->windSer;
+    w.tMin = tMin; w.tMax = tMax; w.dtW = dtMin; w.ByC = ByC; w.BzC = BzC;
+    h->wind = w;
     return 0;
 }
 

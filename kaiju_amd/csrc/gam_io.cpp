@@ -323,3 +323,114 @@ extern "C" int gam_io_read_restart(const char *path, int nG, int Nip, int Njp, i
     *c = in;
     return 0;
 }
+
+// ------------------------------------------------------------ solar-wind file
+namespace {
+
+const char *const kWindSeries[9] = {"T", "D", "Vx", "Vy", "Vz", "Temp", "Bx", "By", "Bz"};
+const char *const kWindCoef[3] = {"ByC", "BzC", "Bx0"};
+const char *const kWindBS[3] = {"xBS", "yBS", "zBS"};
+
+// elements of a dataset (any rank)
+int dataset_len(hid_t file, const char *name, hsize_t *n) {
+    std::vector<hsize_t> d;
+    if (dataset_dims(file, name, d)) return -1;
+    *n = 1;
+    for (hsize_t x : d) *n *= x;
+    return 0;
+}
+
+// ioExist (base/ioH5.F90:348-399): a dataset or a root attribute of that name
+bool io_exist(hid_t file, const char *name) { return has_dataset(file, name) || has_attr(file, name); }
+
+// a scalar by ReadVars' rule (base/ioH5.F90:449-484): the root attribute,
+// then the dataset (its first element) when there is one
+int read_scalar(hid_t file, const char *name, double *out, int *found) {
+    *found = 0;
+    if (has_attr(file, name)) {
+        if (get_scalar_attr(file, name, H5T_NATIVE_DOUBLE, out)) return -1;
+        *found = 1;
+    }
+    if (has_dataset(file, name)) {
+        hsize_t n = 0;
+        if (dataset_len(file, name, &n)) return -1;
+        std::vector<double> tmp(n ? n : 1);
+        if (n == 0 || read_dataset(file, name, tmp.data())) return fail("cannot read %s", name);
+        *out = tmp[0];
+        *found = 1;
+    }
+    return 0;
+}
+
+int read_series(hid_t file, const char *name, int nt, double *out) {
+    hsize_t n = 0;
+    if (dataset_len(file, name, &n)) return -1;
+    if (n != (hsize_t)nt) return fail("wind dataset %s has %d samples, T has %d", name, (int)n, nt);
+    return read_dataset(file, name, out);
+}
+
+}  // namespace
+
+extern "C" int gam_io_wind_len(const char *path, int *nt) {
+    if (!path || !nt) return fail("gam_io_wind_len: bad argument");
+    QuietH5 q;
+    File f;
+    if (open_ro(path, f)) return -1;
+    hsize_t n = 0;
+    if (dataset_len(f.id, "T", &n)) return -1;
+    *nt = (int)n;
+    return 0;
+}
+
+extern "C" int gam_io_read_wind(const char *path, int nt, double *T, double *D, double *V, double *Temp, double *B,
+                                double *xyzBS, gam_io_wind_info *info) {
+    if (!path || nt < 1 || !T || !D || !V || !Temp || !B || !xyzBS || !info) return fail("gam_io_read_wind: bad argument");
+    QuietH5 q;
+    File f;
+    if (open_ro(path, f)) return -1;
+    const hid_t r = f.id;
+    if (!io_exist(r, "Temp"))   // wind.F90:641-646
+        return fail("%s: no Temp variable (solar wind temperature rather than thermal pressure is required)", path);
+    double *dst[9] = {T, D, V, V + nt, V + 2 * (size_t)nt, Temp, B, B + nt, B + 2 * (size_t)nt};
+    for (int v = 0; v < 9; v++)
+        if (read_series(r, kWindSeries[v], nt, dst[v])) return -1;
+    gam_io_wind_info in{};
+    double *coef[3] = {&in.ByC, &in.BzC, &in.Bx0};
+    for (int c = 0; c < 3; c++) {   // wind.F90:704-716, absent ones 0
+        *coef[c] = 0.0;
+        if (read_scalar(r, kWindCoef[c], coef[c], &in.hasCoef[c])) return -1;
+        if (!in.hasCoef[c]) *coef[c] = 0.0;
+    }
+    in.hasBS = io_exist(r, "xBS") ? 1 : 0;   // wind.F90:723-733, else the origin
+    for (int x = 0; x < 3; x++) {
+        double *o = xyzBS + (size_t)nt * x;
+        if (in.hasBS) {
+            if (read_series(r, kWindBS[x], nt, o)) return -1;
+        } else {
+            for (int n = 0; n < nt; n++) o[n] = 0.0;
+        }
+    }
+    *info = in;
+    return 0;
+}
+
+extern "C" int gam_io_write_wind(const char *path, int nt, const double *T, const double *D, const double *V,
+                                 const double *Temp, const double *B, const double *xyzBS, const double *coef) {
+    if (!path || nt < 1 || !T || !D || !V || !Temp || !B) return fail("gam_io_write_wind: bad argument");
+    QuietH5 q;
+    File f;
+    if (create(path, f)) return -1;
+    const hid_t r = f.id;
+    const std::vector<hsize_t> d{(hsize_t)nt};
+    const double *src[9] = {T, D, V, V + nt, V + 2 * (size_t)nt, Temp, B, B + nt, B + 2 * (size_t)nt};
+    for (int v = 0; v < 9; v++)
+        if (write_dataset(r, kWindSeries[v], d, src[v])) return -1;
+    if (coef)
+        for (int c = 0; c < 3; c++)
+            if (set_scalar_attr(r, kWindCoef[c], H5T_NATIVE_DOUBLE, coef[c])) return -1;
+    if (xyzBS)
+        for (int x = 0; x < 3; x++)
+            if (write_dataset(r, kWindBS[x], d, xyzBS + (size_t)nt * x)) return -1;
+    if (H5Fflush(r, H5F_SCOPE_GLOBAL) < 0) return fail("cannot flush %s", path);
+    return 0;
+}

@@ -633,13 +633,73 @@ __global__ void __launch_bounds__(256) k_celldt_rows(GVT<R> g, MP m, BGV bg, con
 }
 
 // ----------------------------------------------------------------- solar-wind BC
-// apps/wind.F90:206-374 WindBC on the outer i face, single fluid, with a wind
-// constant in time (the reference interpolates a time series; a constant one
-// gives this).  jmap: the ghost Bxyz maps j across the axis (ring grids).
+// apps/wind.F90:206-374 WindBC on the outer i face, single fluid.  The wind is
+// constant in time, or a time series as readWind (wind.F90:617-760) leaves it:
+// nt rows of WIND_COLS doubles [t, rho, p, vx, vy, vz, bx, by, bz, xBS, yBS,
+// zBS] in code units, uniform in t, bx already Bx0 + ByC by + BzC bz.
+// jmap: the ghost Bxyz maps j across the axis (ring grids).
+#define WIND_COLS 12
 struct WindP {
-    double rho, p, v[3], b[3];
-    int on;
+    double rho, p, v[3], b[3];   // the constant wind (nt == 0)
+    int on, nt;
+    const double *ser;           // the series, nt x WIND_COLS (device memory)
+    double tMin, tMax, dtW, ByC, BzC;
 };
+
+// InterpWgts (wind.F90:819-868) with discrete walls (doWindInterp false): the
+// row, 0-based, whose values hold at time t
+__device__ __forceinline__ int wind_row(const WindP &w, double t) {
+    if (t >= w.tMax) return w.nt - 1;
+    if (t <= w.tMin) return 0;
+    const int i0 = (int)floor((t - w.tMin) / w.dtW);
+    return i0 + 2 <= w.nt ? i0 : w.nt - 2;
+}
+
+// InterpWind (wind.F90:765-806) with discrete walls: q = rho, p, v[3], B[3] at
+// time t; at or before the first sample the field is null (no spin-up sequence)
+__device__ __forceinline__ void wind_get(const WindP &w, double t, double *q) {
+    if (w.nt == 0) {
+        q[0] = w.rho; q[1] = w.p;
+        for (int x = 0; x < 3; x++) { q[2 + x] = w.v[x]; q[5 + x] = w.b[x]; }
+        return;
+    }
+    const double *r = w.ser + (size_t)WIND_COLS * wind_row(w, t);
+    for (int x = 0; x < 8; x++) q[x] = r[1 + x];
+    if (t <= w.tMin) q[5] = q[6] = q[7] = 0.0;
+}
+
+// RefreshWind + PlaceWind (wind.F90:116-158, 808-817) at time t: the series'
+// marker xyzW and the front velocity, |v(t)| against the flow, along x or
+// tilted by the front coefficients ByC, BzC
+struct WindFront {
+    double xyzW[3], vFr[3];
+};
+__device__ __forceinline__ void wind_front(const WindP &w, double t, WindFront &fr) {
+    const double *r = w.ser + (size_t)WIND_COLS * wind_row(w, t);
+    for (int x = 0; x < 3; x++) fr.xyzW[x] = r[9 + x];
+    double q[8];
+    wind_get(w, t, q);
+    const double vSpd = -sqrt(q[2] * q[2] + q[3] * q[3] + q[4] * q[4]);
+    const double bcc = sqrt(w.ByC * w.ByC + w.BzC * w.BzC), btt = sqrt(1 + bcc * bcc);
+    if (bcc > GAM_TINY) {
+        const double CosBp = w.ByC / bcc, SinBp = w.BzC / bcc, CosF = 1.0 / btt, SinF = bcc / btt;
+        fr.vFr[0] = vSpd * (CosF * CosF);
+        fr.vFr[1] = -(vSpd * CosF * SinF * CosBp);
+        fr.vFr[2] = -(vSpd * CosF * SinF * SinBp);
+    } else {
+        fr.vFr[0] = vSpd; fr.vFr[1] = 0.0; fr.vFr[2] = 0.0;
+    }
+}
+
+// GetWindAt (wind.F90:160-176): the wind at point xyz and time t, taken at
+// t - DelT, DelT the front's travel time from the marker to xyz
+__device__ __forceinline__ void wind_at(const WindP &w, const WindFront &fr, const double *xyz, double t, double *q) {
+    if (w.nt == 0) { wind_get(w, t, q); return; }
+    const double vMag = sqrt(dot3(fr.vFr, fr.vFr));
+    const double d[3] = {xyz[0] - fr.xyzW[0], xyz[1] - fr.xyzW[1], xyz[2] - fr.xyzW[2]};
+    const double DelT = dot3(d, fr.vFr) / (vMag * vMag);
+    wind_get(w, t - DelT, q);
+}
 
 // math.F90:97-111 RampDown
 __device__ __forceinline__ double wind_ramp_down(double r, double rC, double lC) {
@@ -650,11 +710,11 @@ __device__ __forceinline__ double wind_ramp_down(double r, double rC, double lC)
     return exp(eArg + 1.0);
 }
 
-// apps/wind.F90:180-203 wgtWind
-__device__ __forceinline__ double wind_wgt(const WindP &w, const double *xyz) {
+// apps/wind.F90:180-203 wgtWind, given the wind velocity v there
+__device__ __forceinline__ double wind_wgt(const double *v, const double *xyz) {
     double nH[3], vH[3];
     normvec(xyz, nH);
-    normvec(w.v, vH);
+    normvec(v, vH);
     const double d = -vH[0] * nH[0] + -vH[1] * nH[1] + -vH[2] * nH[2];
     const double dTh = acos(d) * 180.0 / GAM_PI;
     return wind_ramp_down(dTh, 90.0, 90.0);
@@ -673,9 +733,10 @@ __device__ __forceinline__ double wind_con2fast(const MP &m, const double *U, co
     return sqrt(sq(Cs) + sq(Va));
 }
 
-// Ghost cells (weight at the cell centre: the wind, or the last cell's state
-// pushed outward at >= MinMach = 1 fast speed) and ghost face fluxes (the wind
-// field's flux mixed with the last cell's); thread = (n, j, k) over j <= Nj,
+// Ghost cells (weight at the cell centre: the wind there (GetWindAt), or the
+// last cell's state pushed outward at >= MinMach = 1 fast speed) and ghost face
+// fluxes (the flux of the wind's field at the face centre mixed with the last
+// cell's); the wind is taken at the clock's time; thread = (n, j, k) over j <= Nj,
 // k <= Nk.  Faces that bound no cell are left alone.  FP64 arithmetic.
 template <typename R>
 __global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const DClock *__restrict__ clk,
@@ -690,14 +751,19 @@ __global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const 
     const int ip = g.nG + g.Nip - 1, ig = ip + nl;
     const size_t nc = g.nc, nf = g.nf;
     const double MinMach = 1.0;
+    const double t = clk->t;
+    WindFront fr = {};
+    if (w.nt > 0) wind_front(w, t, fr);   // RefreshWind at Model%t
     if (k < g.Nk && j < g.Nj) {
         const int c = ci(g, ig, j, k), cp = ci(g, ip, j, k);
         const double xcc[3] = {xyzcc[c], xyzcc[nc + c], xyzcc[2 * nc + c]};
-        const double wSW = wind_wgt(w, xcc), wMHD = 1.0 - wSW;
+        double q[8];
+        wind_at(w, fr, xcc, t, q);
+        const double wSW = wind_wgt(q + 2, xcc), wMHD = 1.0 - wSW;
         double gSW[5] = {0, 0, 0, 0, 0}, gIN[5] = {0, 0, 0, 0, 0};
         if (wSW > GAM_TINY) {
-            gSW[0] = w.rho; gSW[4] = w.p;
-            for (int x = 0; x < 3; x++) gSW[1 + x] = w.v[x];
+            gSW[0] = q[0]; gSW[4] = q[1];
+            for (int x = 0; x < 3; x++) gSW[1 + x] = q[2 + x];
         }
         if (wMHD > GAM_TINY) {
             double U[5], Bin[3];
@@ -722,9 +788,11 @@ __global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const 
         if ((d != 1 && j >= g.Nj) || (d != 2 && k >= g.Nk)) continue;
         const int f = fi(g, d == 0 ? ig + 1 : ig, j, k), fin = fi(g, d == 0 ? ip + 1 : ip, j, k);
         const double x[3] = {xfc[nf * (0 + 3 * d) + f], xfc[nf * (1 + 3 * d) + f], xfc[nf * (2 + 3 * d) + f]};
-        const double wSW = wind_wgt(w, x);
-        const double bn = Tf64[nf * (NORMX + 9 * d) + f] * w.b[0] + Tf64[nf * (NORMY + 9 * d) + f] * w.b[1] +
-                          Tf64[nf * (NORMZ + 9 * d) + f] * w.b[2];
+        double q[8];
+        wind_at(w, fr, x, t, q);
+        const double wSW = wind_wgt(q + 2, x);
+        const double bn = Tf64[nf * (NORMX + 9 * d) + f] * q[5] + Tf64[nf * (NORMY + 9 * d) + f] * q[6] +
+                          Tf64[nf * (NORMZ + 9 * d) + f] * q[7];
         const double swFlx = face64[nf * d + f] * bn;
         mag[nf * d + f] = wSW * swFlx + (1 - wSW) * mag[nf * d + fin];
     }

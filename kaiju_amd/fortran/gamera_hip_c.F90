@@ -136,6 +136,17 @@ module gamera_hip_c
             real(c_double), intent(in) :: v(3), b(3)
         end function gam_set_wind
 
+        !> the solar wind as readWind's time series in code units: tW(nt), Q(:,DEN),
+        !> Q(:,PRESSURE), Q(:,VELX:VELZ), B(nt,3), xyzBS(nt,3) and ByC, BzC, Bx0
+        integer(c_int) function gam_set_wind_series(h, nt, t, rho, p, v, b, xyzBS, ByC, BzC, Bx0) &
+                bind(C, name="gam_set_wind_series")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: h
+            integer(c_int), value :: nt
+            real(c_double), intent(in) :: t(nt), rho(nt), p(nt), v(nt, 3), b(nt, 3), xyzBS(nt, 3)
+            real(c_double), value :: ByC, BzC, Bx0
+        end function gam_set_wind_series
+
         !> ring average about the LFM axis: nch(r) chunks in ring r (before gam_set_corners)
         integer(c_int) function gam_set_ring(h, nring, nch) bind(C, name="gam_set_ring")
             import :: c_int, c_ptr

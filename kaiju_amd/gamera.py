@@ -155,6 +155,8 @@ def lib():
         "gam_get_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
         "gam_set_ring": (C.c_int, [H, C.c_int, C.POINTER(C.c_int)]),
         "gam_set_wind": (C.c_int, [H, C.c_double, C.c_double, _dp, _dp]),
+        "gam_set_wind_series": (C.c_int, [H, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                          C.c_double]),
         "gam_get_ring": (C.c_int, [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "gam_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
         "gam_device_free": (C.c_int, [C.c_void_p]),
@@ -270,7 +272,9 @@ class GamApp:
         if rc != 0 or not h.value:
             raise HipStepError(f"gam_create failed rc={rc}")
         self.h = h
-        if spec.wind is not None:
+        if spec.wind_series is not None:
+            self.set_wind_series(spec.wind_series)
+        elif spec.wind is not None:
             self.set_wind(*spec.wind)
         if any(b != "periodic" for b in spec.bcs):
             self.set_bcs(spec.bcs)
@@ -533,6 +537,16 @@ class GamApp:
         """The constant solar wind of a "wind" outer i boundary (before set_bcs)."""
         vv, bb = (C.c_double * 3)(*v), (C.c_double * 3)(*b)
         self._chk(lib().gam_set_wind(self.h, rho, p, vv, bb), "gam_set_wind")
+
+    def set_wind_series(self, ws):
+        """The solar wind of a "wind" outer i boundary as a time series (a
+        configs.WindSeries in code units; before set_bcs): each boundary point
+        takes the sample holding at the clock's time less the front's travel
+        time from the series' marker (GetWindAt, apps/wind.F90:160-177)."""
+        t, rho, p, v, b, xyzBS = ws.arrays()
+        d = lambda a: None if a is None else a.ctypes.data_as(_dp)
+        self._chk(lib().gam_set_wind_series(self.h, len(t), d(t), d(rho), d(p), d(v), d(b), d(xyzBS),
+                                            float(ws.ByC), float(ws.BzC), float(ws.Bx0)), "gam_set_wind_series")
 
     def set_ring(self, chunks):
         """Ring average about the LFM axis with chunks[r] chunks in ring r+1

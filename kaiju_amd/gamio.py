@@ -8,6 +8,9 @@ RunID.gam.Res.#####.h5 restart, through libgamera_io.so (include/gamera_io.h).
 * ``write_restart`` / ``read_restart`` -- writeH5Res / readH5Restart
   (gioH5.F90:650-872).
 
+* ``read_wind`` / ``write_wind`` -- the bcwind solar-wind time series that
+  readWind ingests (apps/wind.F90:617-760), in the file's units.
+
 Arrays are Fortran-ordered FP64 with ghosts, as ``GamApp.get_state`` returns
 them.  Host code only; no GPU.
 """
@@ -27,6 +30,11 @@ _dp = C.POINTER(C.c_double)
 
 class GamIOError(RuntimeError):
     pass
+
+
+class WindInfo(C.Structure):
+    _fields_ = [("ByC", C.c_double), ("BzC", C.c_double), ("Bx0", C.c_double), ("hasCoef", C.c_int * 3),
+                ("hasBS", C.c_int)]
 
 
 class IOClock(C.Structure):
@@ -49,6 +57,9 @@ def lib():
                                  _dp, _dp, _dp, _dp, _dp, _dp],
         "gam_io_read_restart": [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(IOClock),
                                 _dp, _dp, _dp, _dp],
+        "gam_io_wind_len": [C.c_char_p, C.POINTER(C.c_int)],
+        "gam_io_read_wind": [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, C.POINTER(WindInfo)],
+        "gam_io_write_wind": [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -141,3 +152,32 @@ def read_restart(path: str, N, nG: int = 4, doMHD: bool = True, fill: Optional[d
     _chk(lib().gam_io_read_restart(path.encode(), nG, *N, int(doMHD), C.byref(c), _p(gas), _p(mag), _p(ogas),
                                    _p(omag)), "gam_io_read_restart")
     return c, dict(time=c.t, Gas=gas, magFlux=mag), dict(time=c.ot, Gas=ogas, magFlux=omag)
+
+
+def read_wind(path: str) -> dict:
+    """The bcwind file at path in its own units: T [s], D [#/cc], V (nt, 3)
+    [m/s], Temp [K], B (nt, 3) [nT], xyzBS (nt, 3) (zero when absent), and
+    ByC, BzC, Bx0 [nT] (0 when absent) with has_coef, has_bs flags."""
+    n = C.c_int()
+    _chk(lib().gam_io_wind_len(path.encode(), C.byref(n)), "gam_io_wind_len")
+    nt = n.value
+    T, D, Temp = np.zeros(nt), np.zeros(nt), np.zeros(nt)
+    V, B, BS = (np.zeros((nt, 3), order="F") for _ in range(3))
+    info = WindInfo()
+    _chk(lib().gam_io_read_wind(path.encode(), nt, _p(T), _p(D), _p(V), _p(Temp), _p(B), _p(BS), C.byref(info)),
+         "gam_io_read_wind")
+    return dict(T=T, D=D, V=V, Temp=Temp, B=B, xyzBS=BS, ByC=info.ByC, BzC=info.BzC, Bx0=info.Bx0,
+                has_coef=tuple(bool(x) for x in info.hasCoef), has_bs=bool(info.hasBS))
+
+
+def write_wind(path: str, T, D, V, Temp, B, xyzBS=None, coef=None):
+    """A bcwind file (datasets T, D, Vx..Vz, Temp, Bx..Bz, optional xBS..zBS;
+    optional root attributes ByC, BzC, Bx0)."""
+    nt = len(T)
+    f1 = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(nt)
+    T, D, Temp = f1(T), f1(D), f1(Temp)
+    V, B = _fa(np.asarray(V), (nt, 3)), _fa(np.asarray(B), (nt, 3))
+    BS = None if xyzBS is None else _fa(np.asarray(xyzBS), (nt, 3))
+    cf = None if coef is None else (C.c_double * 3)(*coef)
+    _chk(lib().gam_io_write_wind(path.encode(), nt, _p(T), _p(D), _p(V), _p(Temp), _p(B), _p(BS), cf),
+         "gam_io_write_wind")

@@ -49,6 +49,25 @@ def wind_state(n_cc=5.0, v_kms=(-400.0, 0.0, 0.0), T_K=1.0e5, b_nT=(0.0, 0.0, -5
     return rho, v, p, b
 
 
+KBLTZ_CGS = 1.380649e-16     # erg/K
+ERGCC2NPA = 1.0e8
+
+
+def wind_series_from_file(path: str, gT0: float = T_UNIT, gv0: float = V0, gP0_nPa: float = P_UNIT * 1e9,
+                          gB0_nT: float = B_UNIT):
+    """The bcwind file at path as a configs.WindSeries in code units, converted
+    as readWind does (apps/wind.F90:667-721): t / gT0, density as is, V / gv0,
+    pressure D kB Temp (erg/cc -> nPa) / gP0, B / gB0, Bx0 / gB0, ByC and BzC
+    unscaled, xyzBS as is (planet radii).  The defaults are this module's code
+    units."""
+    from kaiju_amd import configs, gamio
+    w = gamio.read_wind(path)
+    D = w["D"]
+    return configs.WindSeries(t=(1 / gT0) * w["T"], rho=(1 / 1.0) * D, p=(1 / gP0_nPa) * D * KBLTZ_CGS * w["Temp"] * ERGCC2NPA,
+                              v=(1 / gv0) * w["V"], b=(1 / gB0_nT) * w["B"], xyzBS=w["xyzBS"],
+                              ByC=w["ByC"], BzC=w["BzC"], Bx0=w["Bx0"] * (1 / gB0_nT))
+
+
 def dipole_moment(m_nT_RE3=-31000.0) -> float:
     """Dipole moment along z in code units (B = M (3 z r_hat - z_hat)/r^3 style)."""
     return m_nT_RE3 / B_UNIT

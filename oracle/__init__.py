@@ -31,7 +31,9 @@ class OrcModel(C.Structure):
                 ("ts", "rmeth", "doLFMLim", "doBoris", "doMHD", "do25D", "doCPR",
                  "fixedTimestep", "status", "doBackground")] + \
                [("bc", C.c_int * 6), ("ncpr", C.c_int)] + \
-               [("wind_rho", C.c_double), ("wind_p", C.c_double), ("wind_v", C.c_double * 3), ("wind_b", C.c_double * 3)]
+               [("wind_rho", C.c_double), ("wind_p", C.c_double), ("wind_v", C.c_double * 3), ("wind_b", C.c_double * 3),
+                ("wind_nt", C.c_int), ("wind_ser", C.POINTER(C.c_double))] + \
+               [(n, C.c_double) for n in ("wind_tMin", "wind_tMax", "wind_dtW", "wind_ByC", "wind_BzC", "wind_Bx0")]
 
 
 class OrcGrid(C.Structure):
@@ -107,6 +109,7 @@ def lib():
             "orc_ppm_intlr": (None, [C.c_double] * 5 + [_dp, _dp]),
             "orc_lfm_map": (None, [G, C.c_int, C.c_int, C.c_int, C.c_int] + [C.POINTER(C.c_int)] * 3),
             "orc_wgt_wind": (C.c_double, [M, _dp]),
+            "orc_wind_at": (None, [M, _dp, C.c_double, _dp]),
             "orc_omp_threads": (C.c_int, []),
             "orc_nuke_count": (C.c_long, [C.c_int]),
         }
@@ -215,11 +218,34 @@ class OracleCase:
             self.m.wind_rho, self.m.wind_p = rho, p
             for x in range(3):
                 self.m.wind_v[x], self.m.wind_b[x] = v[x], b[x]
+        if getattr(spec, "wind_series", None) is not None:
+            self.set_wind_series(spec.wind_series)
         self.state = OracleState(G, F)
         self.ostate = OracleState(G, F)
         self.solver = L.orc_solver_new(C.byref(g))
         if setup:
             self.setup()
+
+    def set_wind_series(self, ws):
+        """The wind as a time series (a configs.WindSeries): readWind's table
+        (apps/wind.F90:667-702) with dtW = the smallest sample spacing."""
+        t, rho, p, v, b, xyzBS = ws.arrays()
+        nt = len(t)
+        tab = np.zeros((nt, 12))
+        tab[:, 0], tab[:, 1], tab[:, 2], tab[:, 3:6], tab[:, 6:9] = t, rho, p, v, b
+        if xyzBS is not None:
+            tab[:, 9:12] = xyzBS
+        self._wind_tab = np.ascontiguousarray(tab).ravel()    # row n at [12 n, 12 n + 12)
+        m = self.m
+        m.wind_nt, m.wind_ser = nt, ptr(self._wind_tab)
+        m.wind_tMin, m.wind_tMax, m.wind_dtW = float(t.min()), float(t.max()), float(np.diff(t).min())
+        m.wind_ByC, m.wind_BzC, m.wind_Bx0 = ws.ByC, ws.BzC, ws.Bx0
+
+    def wind_at(self, xyz, t):
+        """GetWindAt after RefreshWind at t: (rho, p, v[3], B[3]) at point xyz."""
+        q = (C.c_double * 8)()
+        lib().orc_wind_at(C.byref(self.m), (C.c_double * 3)(*xyz), t, q)
+        return np.array(q[:])
 
     def __del__(self):
         try:

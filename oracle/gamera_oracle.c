@@ -1398,15 +1398,108 @@ static double ramp_down(double r, double rC, double lC) {
     return exp(eArg + 1.0);
 }
 
+/* apps/wind.F90:819-868 InterpWgts: rows i0, i1 (0-based) and weights of the
+ * samples around time t; doWindInterp = .false. (wind.F90:18) then sets the
+ * weights to the discrete walls 1, 0 */
+static void wind_interp_wgts(const orc_model *m, double t, double *w0, double *w1, int *i0, int *i1) {
+    const int NumT = m->wind_nt;
+    if (t >= m->wind_tMax) {
+        *i0 = NumT - 1; *i1 = *i0; *w0 = 1.0; *w1 = 0.0;
+    } else if (t <= m->wind_tMin) {
+        *i0 = 0; *i1 = *i0; *w0 = 1.0; *w1 = 0.0;
+    } else {
+        /* 1-based i0 = floor((t - tMin)/dtW) + 1, i1 = i0 + 1 */
+        int a = (int)floor((t - m->wind_tMin) / m->wind_dtW) + 1, b = a + 1;
+        if (b <= NumT) {
+            const double tb = m->wind_ser[(size_t)ORC_WIND_COLS * (b - 1)], ta = m->wind_ser[(size_t)ORC_WIND_COLS * (a - 1)];
+            const double dT = tb - ta;
+            *w0 = (tb - t) / dT;
+            *w1 = (t - ta) / dT;
+        } else {
+            b = NumT; a = b - 1; *w0 = 0.0; *w1 = 1.0;
+        }
+        *i0 = a - 1; *i1 = b - 1;
+    }
+    *w0 = 1.0; *w1 = 0.0;   /* discrete walls */
+}
+
+/* apps/wind.F90:765-806 InterpWind (the getWind of a discrete series): q =
+ * rho, p, v[3], B[3] at time t, Bx from the front coefficients, a null field
+ * at or before tMin (doSeq off); the constant wind when there is no series */
+static void wind_get(const orc_model *m, double t, double *q) {
+    if (m->wind_nt < 2) {
+        q[0] = m->wind_rho; q[1] = m->wind_p;
+        for (int x = 0; x < 3; x++) { q[2 + x] = m->wind_v[x]; q[5 + x] = m->wind_b[x]; }
+        return;
+    }
+    double w0, w1;
+    int i0, i1;
+    wind_interp_wgts(m, t, &w0, &w1, &i0, &i1);
+    const double *r0 = m->wind_ser + (size_t)ORC_WIND_COLS * i0, *r1 = m->wind_ser + (size_t)ORC_WIND_COLS * i1;
+    for (int x = 0; x < 8; x++) q[x] = w0 * r0[1 + x] + w1 * r1[1 + x];
+    q[5] = m->wind_Bx0 + m->wind_ByC * q[6] + m->wind_BzC * q[7];
+    if (t <= m->wind_tMin) q[5] = q[6] = q[7] = 0.0;
+}
+
+/* apps/wind.F90:116-158 RefreshWind with PlaceWind (:808-817): the marker
+ * xyzW at time t and the front velocity vFr (speed |v(t)| against the flow,
+ * tilted by ByC, BzC) */
+static void wind_refresh(const orc_model *m, double t, double *xyzW, double *vFr) {
+    double w0, w1, q[8];
+    int i0, i1;
+    wind_interp_wgts(m, t, &w0, &w1, &i0, &i1);
+    for (int x = 0; x < 3; x++)
+        xyzW[x] = w0 * m->wind_ser[(size_t)ORC_WIND_COLS * i0 + 9 + x] + w1 * m->wind_ser[(size_t)ORC_WIND_COLS * i1 + 9 + x];
+    wind_get(m, t, q);
+    const double vSpd = -sqrt(dot3(q + 2, q + 2));
+    const double ByC = m->wind_ByC, BzC = m->wind_BzC;
+    const double bcc = sqrt(ByC * ByC + BzC * BzC), btt = sqrt(1 + bcc * bcc);
+    if (bcc > TINY) {
+        const double CosBp = ByC / bcc, SinBp = BzC / bcc;
+        const double CosF = 1.0 / btt, SinF = bcc / btt;
+        vFr[0] = vSpd * (CosF * CosF);
+        vFr[1] = -(vSpd * CosF * SinF * CosBp);
+        vFr[2] = -(vSpd * CosF * SinF * SinBp);
+    } else {
+        vFr[0] = vSpd; vFr[1] = 0.0; vFr[2] = 0.0;
+    }
+}
+
+/* apps/wind.F90:160-177 GetWindAt: the wind at xyz at time t, read at t - DelT
+ * with DelT = (xyz - xyzW).vFr / |vFr|^2 */
+static void wind_get_at(const orc_model *m, const double *xyzW, const double *vFr, const double *xyz, double t, double *q) {
+    if (m->wind_nt < 2) { wind_get(m, t, q); return; }
+    const double vMag = sqrt(dot3(vFr, vFr));
+    const double d[3] = {xyz[0] - xyzW[0], xyz[1] - xyzW[1], xyz[2] - xyzW[2]};
+    const double DelT = dot3(d, vFr) / (vMag * vMag);
+    wind_get(m, t - DelT, q);
+}
+
+/* GetWindAt at the time t after RefreshWind at t (what WindBC does with
+ * t = Model%t) */
+void orc_wind_at(const orc_model *m, const double *xyz, double t, double *q) {
+    double xyzW[3] = {0, 0, 0}, vFr[3] = {-1.0, 0, 0};
+    if (m->wind_nt >= 2) wind_refresh(m, t, xyzW, vFr);
+    wind_get_at(m, xyzW, vFr, xyz, t, q);
+}
+
 /* apps/wind.F90:180-203 wgtWind: 1 where the radial direction faces the wind
- * (within 90 degrees of -v), ramping to 0 on the downstream axis */
-double orc_wgt_wind(const orc_model *m, const double *xyz) {
+ * (within 90 degrees of -v, v the wind there), ramping to 0 on the downstream
+ * axis */
+static double wgt_wind_v(const double *v, const double *xyz) {
     double nH[3], vH[3];
     normvec(xyz, nH);
-    normvec(m->wind_v, vH);
+    normvec(v, vH);
     const double d = -vH[0] * nH[0] + -vH[1] * nH[1] + -vH[2] * nH[2];
     const double dTh = acos(d) * 180.0 / PI;
     return ramp_down(dTh, 90.0, 90.0);
+}
+
+/* wgtWind at xyz at the model's time */
+double orc_wgt_wind(const orc_model *m, const double *xyz) {
+    double q[8];
+    orc_wind_at(m, xyz, m->t, q);
+    return wgt_wind_v(q + 2, xyz);
 }
 
 /* gamutils.F90:324-360 CellPress2Cs + Con2Fast */
@@ -1422,7 +1515,8 @@ static double con2fast(const orc_model *m, const double *U, const double *B) {
     return sqrt(sq(Cs) + sq(Va));
 }
 
-/* apps/wind.F90:206-374 WindBC (outer i face, single fluid, constant wind):
+/* apps/wind.F90:206-374 WindBC (outer i face, single fluid, the wind at each
+ * point from GetWindAt after RefreshWind at Model%t):
  * ghost cells mix the wind (weight wgtWind at the cell centre) with the last
  * cell's state forced outward at >= MinMach = 1 fast speed; ghost face fluxes
  * mix the wind field's flux with the last cell's; ghost Bxyz from the faces
@@ -1432,8 +1526,10 @@ static double con2fast(const orc_model *m, const double *U, const double *B) {
 static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s) {
     const size_t nc = g->nc, nf = g->nf;
     const double MinMach = 1.0;
-    const double *Bw = m->wind_b;
     const int ip = g->ie;
+    const double t = m->t;
+    double xyzW[3] = {0, 0, 0}, vFr[3] = {-1.0, 0, 0};   /* InitWind's defaults */
+    if (m->wind_nt >= 2) wind_refresh(m, t, xyzW, vFr);
     for (int k = g->ksg; k <= g->keg + 1; k++)
         for (int j = g->jsg; j <= g->jeg + 1; j++)
             for (int n = 1; n <= g->nG; n++) {
@@ -1441,11 +1537,14 @@ static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s) {
                 if (k <= g->keg && j <= g->jeg && ig <= g->ieg) {
                     const size_t c = CI(g, ig, j, k), cp = CI(g, ip, j, k);
                     double xcc[3] = {g->xyzcc[c], g->xyzcc[c + nc], g->xyzcc[c + 2 * nc]};
-                    const double wSW = orc_wgt_wind(m, xcc), wMHD = 1.0 - wSW;
+                    double q[8];
+                    wind_get_at(m, xyzW, vFr, xcc, t, q);
+                    const double wSW = wgt_wind_v(q + 2, xcc), wMHD = 1.0 - wSW;
                     double gSW[5] = {0, 0, 0, 0, 0}, gIN[5] = {0, 0, 0, 0, 0};
                     if (wSW > TINY) {
-                        gSW[0] = m->wind_rho; gSW[4] = m->wind_p;
-                        for (int x = 0; x < 3; x++) gSW[1 + x] = m->wind_v[x];
+                        wind_get_at(m, xyzW, vFr, xcc, t, q);
+                        gSW[0] = q[0]; gSW[4] = q[1];
+                        for (int x = 0; x < 3; x++) gSW[1 + x] = q[2 + x];
                     }
                     if (wMHD > TINY) {
                         double U[5], Bin[3];
@@ -1472,7 +1571,11 @@ static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s) {
                     if ((d != JD && j > g->jeg) || (d != KD && k > g->keg)) continue;
                     const size_t f = FI(g, fi_, j, k), fin = FI(g, d == ID ? ip + 1 : ip, j, k);
                     const double x[3] = {g->xfc[f + nf * (0 + 3 * d)], g->xfc[f + nf * (1 + 3 * d)], g->xfc[f + nf * (2 + 3 * d)]};
-                    const double wSW = orc_wgt_wind(m, x);
+                    double q[8];
+                    wind_get_at(m, xyzW, vFr, x, t, q);                   /* wgtWind's GetWindAt */
+                    const double wSW = wgt_wind_v(q + 2, x);
+                    wind_get_at(m, xyzW, vFr, x, t, q);                   /* WindMagFlux's */
+                    const double *Bw = q + 5;
                     const double bn = g->Tf[f + nf * (NORMX + 9 * d)] * Bw[0] + g->Tf[f + nf * (NORMY + 9 * d)] * Bw[1] +
                                       g->Tf[f + nf * (NORMZ + 9 * d)] * Bw[2];
                     const double swFlx = g->face[f + nf * d] * bn;

@@ -36,7 +36,15 @@ typedef struct {
     /* solar wind of the WindBC (apps/wind.F90), constant in time: density,
      * pressure, velocity, field (code units) */
     double wind_rho, wind_p, wind_v[3], wind_b[3];
+    /* or a time series (readWind, apps/wind.F90:617-760, after its unit
+     * conversion) when wind_nt >= 2: wind_nt rows of ORC_WIND_COLS doubles
+     * [t, rho, p, vx, vy, vz, Bx, By, Bz, xBS, yBS, zBS]; tMin, tMax, dtW
+     * and the front coefficients ByC, BzC, Bx0 */
+    int wind_nt;
+    const double *wind_ser;
+    double wind_tMin, wind_tMax, wind_dtW, wind_ByC, wind_BzC, wind_Bx0;
 } orc_model;
+#define ORC_WIND_COLS 12
 
 /* boundary conditions (bcs.F90): periodic, zero gradient, zero extension,
  * Cartesian reflecting, spherical radial (SphIn on INI, SphOut on OUTI),
@@ -141,6 +149,7 @@ void   orc_reconstruct_ring(double *rW, int Nc, int Np);
 void   orc_ppm_intlr(double fm2, double fm1, double f, double fp1, double fp2, double *vm, double *vp);
 void   orc_lfm_map(const orc_grid *g, int what, int i, int j, int k, int *ip, int *jp, int *kp);
 double orc_wgt_wind(const orc_model *m, const double *xyz);
+void   orc_wind_at(const orc_model *m, const double *xyz, double t, double *q);
 
 /* cell-level helpers exposed for known-answer tests */
 void   orc_cell_c2p(const orc_model *m, const double *con, double *prim);

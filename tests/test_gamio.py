@@ -250,3 +250,90 @@ def test_restart_refusals(tmp_path):
     gamio.write_grid(p, _rand(F + (3,), 8), nG)
     with pytest.raises(gamio.GamIOError, match="too old"):
         gamio.read_restart(p, N, nG)
+
+
+# ------------------------------------------------- bcwind (readWind's solar-wind series)
+def _wind_arrays(nt=7):
+    r = np.random.default_rng(5)
+    T = 60.0 * np.arange(nt) - 120.0
+    V = np.asfortranarray(np.stack([-4.0e5 + 1e4 * r.standard_normal(nt), 1e3 * r.standard_normal(nt),
+                                    1e3 * r.standard_normal(nt)], axis=1))
+    B = np.asfortranarray(r.standard_normal((nt, 3)) * 5.0)
+    BS = np.asfortranarray(np.stack([12.0 + 0.1 * np.arange(nt), np.zeros(nt), np.ones(nt)], axis=1))
+    return T, 5.0 + r.random(nt), V, 1.0e5 * (1 + r.random(nt)), B, BS
+
+
+def test_wind_file_layout_and_round_trip(tmp_path):
+    """Datasets T, D, Vx, Vy, Vz, Temp, Bx, By, Bz (and xBS, yBS, zBS) of nt
+    doubles, the front coefficients root scalar attributes; read back bit for
+    bit (apps/wind.F90:648-733 reads these names)."""
+    T, D, V, Temp, B, BS = _wind_arrays()
+    p = str(tmp_path / "bcwind.h5")
+    gamio.write_wind(p, T, D, V, Temp, B, xyzBS=BS, coef=(0.25, -0.5, 1.5))
+    dump = h5dump("-H", "-A", p)
+    for name in ("T", "D", "Vx", "Vy", "Vz", "Temp", "Bx", "By", "Bz", "xBS", "yBS", "zBS"):
+        h = _dataset_header(dump, name)
+        assert _dims(h) == (len(T),) and "H5T_IEEE_F64LE" in h, name
+    for name in ("ByC", "BzC", "Bx0"):
+        m = re.search(r'ATTRIBUTE "%s" \{\s*DATATYPE\s+(\S+)\s*DATASPACE\s+SCALAR' % name, dump)
+        assert m and m.group(1) == "H5T_IEEE_F64LE", name
+    out = h5dump("-d", "Vy", "-s", "3", "-c", "1", "-m", "%.17g", p)
+    assert float(re.search(r"\(3\): ([-0-9.e+]+)", out).group(1)) == V[3, 1]
+    w = gamio.read_wind(p)
+    for k, a in (("T", T), ("D", D), ("V", V), ("Temp", Temp), ("B", B), ("xyzBS", BS)):
+        assert np.array_equal(w[k], a), k
+    assert (w["ByC"], w["BzC"], w["Bx0"]) == (0.25, -0.5, 1.5)
+    assert w["has_coef"] == (True, True, True) and w["has_bs"]
+
+
+def test_wind_file_optional_parts_and_refusals(tmp_path):
+    """ReadVars' rules (base/ioH5.F90:449-484): a coefficient may be a dataset
+    or a root attribute, an absent one is 0 (wind.F90:711-716); no xBS: the
+    marker at the origin; no Temp (an older file with pressure): refused
+    (wind.F90:641-646)."""
+    h5 = _H5()
+    T, D, V, Temp, B, BS = _wind_arrays(5)
+    ds = {"T": ((5,), T), "D": ((5,), D), "Vx": ((5,), V[:, 0]), "Vy": ((5,), V[:, 1]), "Vz": ((5,), V[:, 2]),
+          "Temp": ((5,), Temp), "Bx": ((5,), B[:, 0]), "By": ((5,), B[:, 1]), "Bz": ((5,), B[:, 2]),
+          "ByC": ((1,), np.array([0.75]))}
+    p = str(tmp_path / "w1.h5")
+    h5.write(p, ds, {"BzC": -0.125})
+    w = gamio.read_wind(p)
+    assert (w["ByC"], w["BzC"], w["Bx0"]) == (0.75, -0.125, 0.0)
+    assert w["has_coef"] == (True, True, False) and not w["has_bs"]
+    assert np.array_equal(w["xyzBS"], np.zeros((5, 3))) and np.array_equal(w["V"], V)
+    p2 = str(tmp_path / "w2.h5")
+    h5.write(p2, {k: v for k, v in ds.items() if k != "Temp"} | {"P": ((5,), Temp)}, {})
+    with pytest.raises(gamio.GamIOError, match="Temp"):
+        gamio.read_wind(p2)
+    p3 = str(tmp_path / "w3.h5")
+    h5.write(p3, ds | {"Bz": ((4,), B[:4, 2])}, {})
+    with pytest.raises(gamio.GamIOError, match="Bz"):
+        gamio.read_wind(p3)
+
+
+def test_wind_file_units():
+    """readWind's conversion (wind.F90:667-721) to the magnetosphere's code
+    units against the SI form of the same quantities: p = n kB T with n in
+    m^-3 and kB in J/K over the pressure unit (msph.wind_state), v / 100 km/s,
+    B / B unit, t / (R_E / 100 km/s); Bx replaced by Bx0 + ByC By + BzC Bz
+    happens later, in the boundary (the series keeps the file's Bx)."""
+    import math
+    import tempfile
+    from kaiju_amd import msph
+    nt = 4
+    T = np.array([0.0, 60.0, 120.0, 180.0])
+    V = np.asfortranarray(np.tile([-4.0e5, 2.0e4, 0.0], (nt, 1)))
+    B = np.asfortranarray(np.tile([1.0, 2.0, -5.0], (nt, 1)))
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "bcwind.h5")
+        gamio.write_wind(p, T, np.full(nt, 5.0), V, np.full(nt, 1.0e5), B, coef=(0.5, 0.0, 3.0))
+        ws = msph.wind_series_from_file(p)
+    rho, v, pres, b = msph.wind_state(n_cc=5.0, v_kms=(-400.0, 20.0, 0.0), T_K=1.0e5, b_nT=(1.0, 2.0, -5.0))
+    assert np.allclose(ws.t, T / msph.T_UNIT, rtol=1e-15, atol=0)
+    assert math.isclose(ws.t[1], 60.0 * msph.V0 / msph.RE, rel_tol=1e-15)
+    assert np.array_equal(ws.rho, np.full(nt, rho))
+    assert np.allclose(ws.p, pres, rtol=1e-14, atol=0)
+    assert np.allclose(ws.v, np.tile(v, (nt, 1)), rtol=1e-15, atol=0)
+    assert np.allclose(ws.b, np.tile(b, (nt, 1)), rtol=1e-15, atol=0)
+    assert (ws.ByC, ws.BzC) == (0.5, 0.0) and math.isclose(ws.Bx0, 3.0 / msph.B_UNIT, rel_tol=1e-15)

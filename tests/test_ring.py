@@ -486,6 +486,19 @@ def test_ring_and_wind_argument_checks():
             with pytest.raises(gamera.HipStepError):
                 app.set_wind(-1.0, 1.0, (-4.0, 0.0, 0.0), (0.0, 0.0, -1.0))
             app.set_wind(*spec.wind)
+            # a series readWind would refuse: uneven spacing, faster than the
+            # Boris speed Ca = 10; and one sample, decreasing times
+            ws = _wind_series(nt=6)
+            bad = [dataclasses.replace(ws, t=ws.t + np.array([0, 0, 0, 1e-3, 1e-3, 1e-3])),
+                   dataclasses.replace(ws, v=ws.v * 3.0),
+                   dataclasses.replace(ws, t=ws.t[:1], rho=ws.rho[:1], p=ws.p[:1], v=ws.v[:1], b=ws.b[:1],
+                                       xyzBS=ws.xyzBS[:1]),
+                   dataclasses.replace(ws, t=ws.t[::-1].copy())]
+            for b in bad:
+                with pytest.raises(gamera.HipStepError):
+                    app.set_wind_series(b)
+            app.set_wind_series(dataclasses.replace(ws, t=ws.t + np.array([0, 0, 0, 1e-7, 1e-7, 1e-7])))
+            app.set_wind(*spec.wind)
             with pytest.raises(gamera.HipStepError):   # the wind BC belongs to the outer i face
                 app.set_bcs(("wind", "sphere", "lfm", "lfm", "periodic", "periodic"))
             app.set_bcs(spec.bcs)
@@ -502,3 +515,147 @@ def test_ring_and_wind_argument_checks():
                 app.set_bcs(("sphere", "sphere", "lfm", "lfm", "periodic", "periodic"))
     finally:
         gamera.use_library(saved)
+
+
+# ----------------------------------------------------------------- solar-wind time series
+def _wind_series(nt=41, tMin=-10.0, tMax=4.0, ByC=0.3, BzC=-0.2, Bx0=0.1, marker=True):
+    """A varying wind in code units: every sample differs, the field's x
+    component is a placeholder readWind replaces, the marker drifts."""
+    rho0, v0, p0, b0 = msph.wind_state()
+    n = np.arange(nt)
+    t = tMin + (tMax - tMin) * n / (nt - 1)
+    v = np.stack([v0[0] - 0.8 * np.sin(0.4 * n), 0.3 * np.cos(0.9 * n), -0.2 * np.sin(1.3 * n)], axis=1)
+    b = np.stack([np.full(nt, 9.9), 0.6 * np.sin(0.5 * n), b0[2] + 0.4 * np.cos(0.8 * n)], axis=1)
+    xyzBS = np.stack([12.0 + 0.05 * n, np.full(nt, 0.5), -0.3 + 0.01 * n], axis=1) if marker else None
+    return configs.WindSeries(t=t, rho=rho0 * (1 + 0.4 * np.sin(0.7 * n)), p=p0 * (1 + 0.5 * np.cos(0.3 * n)),
+                              v=v, b=b, xyzBS=xyzBS, ByC=ByC, BzC=BzC, Bx0=Bx0)
+
+
+def _row_values(ws, r):
+    """(rho, p, v, B) of sample r as InterpWind returns it: Bx from the front
+    coefficients (wind.F90:718-721, 788-789)."""
+    b = np.array(ws.b[r], dtype=float)
+    b[0] = ws.Bx0 + ws.ByC * b[1] + ws.BzC * b[2]
+    return np.concatenate([[ws.rho[r], ws.p[r]], ws.v[r], b])
+
+
+def test_wind_series_lookup_known_answers():
+    """InterpWgts with discrete walls (wind.F90:819-868, doWindInterp false) and
+    InterpWind (:765-806): at the marker (no lag) the wind at time t is sample
+    floor((t - tMin)/dtW), the first sample with a null field at or before
+    tMin, the last at or after tMax; a last interval shorter than dtW (spacing
+    uniform to 1e-6) maps the times past tMin + (nt-1) dtW to the
+    next-to-last sample (the i1 > NumT branch)."""
+    spec, oc = _oracle_case((12, 12, 32))
+    ws = _wind_series(nt=9, tMin=0.0, tMax=8.0, ByC=0.0, BzC=0.0, Bx0=0.0, marker=False)
+    oc.set_wind_series(ws)
+    at = lambda t: oc.wind_at((0.0, 0.0, 0.0), t)          # marker at the origin: DelT = 0
+    for t, r in ((0.5, 0), (1.0 + 1e-9, 1), (3.999, 3), (4.0, 4), (7.5, 7)):
+        assert np.array_equal(at(t), _row_values(ws, r)), t
+    for t in (0.0, -3.0):
+        want = _row_values(ws, 0)
+        want[5:] = 0.0
+        assert np.array_equal(at(t), want), t
+    for t in (8.0, 100.0):
+        assert np.array_equal(at(t), _row_values(ws, 8)), t
+    # last spacing 1e-7 short: dtW = 1 - 1e-7, so t in [tMin + 3 dtW, tMax) reads row 2
+    ws2 = configs.WindSeries(t=np.array([0.0, 1.0, 2.0, 3.0 - 1e-7]), rho=np.array([1.0, 2.0, 3.0, 4.0]),
+                             p=np.ones(4), v=np.tile([-4.0, 0.0, 0.0], (4, 1)), b=np.zeros((4, 3)))
+    oc.set_wind_series(ws2)
+    t = 3.0 - 1.5e-7
+    assert t / (1 - 1e-7) >= 3.0 and t < 3.0 - 1e-7
+    assert at(t)[0] == 3.0                                   # row 2, not the last row
+    assert at(2.5)[0] == 3.0 and at(3.0)[0] == 4.0
+
+
+def test_wind_series_front_lag_and_field_known_answers():
+    """RefreshWind/GetWindAt (wind.F90:116-177) against the planar-front closed
+    form: with n = (1, -ByC, -BzC) the front velocity is -|v| n / |n|^2, so the
+    lag of point x is DelT = -((x - xyzW).n)/|v|: points on one plane
+    x.n = const read the same sample, a point moved by s along n reads the
+    wind s/|v| later, and the field of every sample satisfies B.n = Bx0."""
+    spec, oc = _oracle_case((12, 12, 32))
+    ws = _wind_series()
+    oc.set_wind_series(ws)
+    rng = np.random.default_rng(7)
+    nvec = np.array([1.0, -ws.ByC, -ws.BzC])
+    t = 1.3
+    dtW = float(np.diff(ws.t).min())
+    rW = int(np.floor((t - ws.t[0]) / dtW))
+    xW = ws.xyzBS[rW]                                         # PlaceWind at Model%t
+    speed = float(np.linalg.norm(ws.v[rW]))                   # RefreshWind's |V0| at Model%t
+    checked = 0
+    for _ in range(60):
+        x = xW + rng.uniform(-20, 20, 3)
+        lag = -float((x - xW) @ nvec) / speed
+        tt = t - lag
+        if not (ws.t[0] < tt < ws.t[-1]):
+            continue
+        u = (tt - ws.t[0]) / dtW
+        if abs(u - round(u)) < 1e-6:                          # too close to a wall to call
+            continue
+        want = _row_values(ws, int(np.floor(u)))
+        got = oc.wind_at(x, t)
+        assert np.allclose(got, want, rtol=1e-14, atol=0), (x, tt)
+        # the whole plane through x reads the same sample
+        e1 = np.cross(nvec, [0.0, 0.0, 1.0]); e1 /= np.linalg.norm(e1)
+        e2 = np.cross(nvec, e1); e2 /= np.linalg.norm(e2)
+        y = x + rng.uniform(-15, 15) * e1 + rng.uniform(-15, 15) * e2
+        assert np.array_equal(oc.wind_at(y, t), got) or abs(((y - xW) @ nvec) - ((x - xW) @ nvec)) > 1e-9
+        assert abs(got[5:] @ nvec - ws.Bx0) <= 1e-14 * (1 + np.abs(got[5:]).max())
+        checked += 1
+    assert checked >= 20, checked
+
+
+def test_constant_series_is_the_constant_wind():
+    """A series of identical samples that started long ago gives the ghost
+    cells and faces of the constant-wind BC bit for bit (the constant wind is
+    what the BC reads at every point then)."""
+    import dataclasses
+    spec, oc_c = _hatched((12, 12, 32))
+    rho, pw, v, b = spec.wind
+    nt = 5
+    ws = configs.WindSeries(t=np.linspace(-1.0e4, -1.0e4 + 4.0, nt), rho=np.full(nt, rho), p=np.full(nt, pw),
+                            v=np.tile(v, (nt, 1)), b=np.tile(b, (nt, 1)), ByC=0.0, BzC=0.0, Bx0=b[0])
+    spec_s = dataclasses.replace(spec, wind_series=ws)
+    spec_s, oc_s = _hatched_spec(spec_s)
+    for k in ("Gas", "magFlux", "Bxyz"):
+        assert np.array_equal(getattr(oc_s.state, k), getattr(oc_c.state, k)), k
+
+
+def _hatched_spec(spec):
+    p = spec.ic_params
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.grid["xyz"][...] = msph.corners(spec.N, spec.nG, p["r_in"], p["r_out"], p["theta0"], p["axis"])
+    oc.metric()
+    oc.add_b0()
+    gas, mag = msph.uniform_state(oc.grid["xyz"], p["rho"], p["v"], p["p"], p["b"], spec.model.gamma)
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = mag
+    oc.hatch_finish()
+    return spec, oc
+
+
+@pytest.mark.gpu
+def test_axis_block_wind_series_matches_oracle():
+    """The axis block 16x12x32 with a varying solar-wind series on its outer
+    face (tilted front ByC = 0.3, BzC = -0.2, Bx0 = 0.1, drifting marker,
+    samples every 0.35 from t = -10 to 4): at t = 0 the shell's lag spans the
+    whole series, null field before it included; hatch and 4 steps against the
+    oracle."""
+    import dataclasses
+    say("[wind] msph axis 16x12x32 wind series vs oracle")
+    spec = configs.magnetosphere((16, 12, 32), steps=4, precision=0)
+    spec = dataclasses.replace(spec, wind_series=_wind_series())
+    spec, oc = _hatched_spec(spec)
+    with app_from_oracle(oc) as app:
+        assert app.steps(1) == 0
+        oc.step(1)
+        e1 = compare_state(app, oc, 0)
+        assert app.steps(spec.steps - 1) == 0
+        oc.step(spec.steps - 1)
+        e = compare_state(app, oc, 0)
+        say(f"   after 1: {e1}  after {spec.steps}: {e}")
+        assert max(e1["Gas"], e1["magFlux"], e1["Bxyz"]) <= 1e-12, e1
+        assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-10, e
+        assert abs(app.Model.dt - oc.m.dt) <= 1e-12 * oc.m.dt
