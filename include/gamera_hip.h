@@ -306,6 +306,20 @@ int  gam_graph_launch(gam_graph *g, int replays);
 int  gam_graph_sync(gam_graph *g);
 int  gam_graph_destroy(gam_graph *g);
 const char *gam_graph_error(const gam_graph *g);
+/* The same steps with one launch per kernel for all the handles (no
+ * counterpart in the reference: its ensemble of independent runs is one
+ * process per run).  Each handle's launches are recorded instead of run; the
+ * launches of one kernel by several handles become one launch whose blocks are
+ * split among them (each handle's blocks start at a multiple of 8, keeping
+ * their XCD placement), every block running the kernel with its own handle's
+ * arguments, read from a device table the graph owns.  Handles whose launch
+ * sequences differ (2.5D and 3D blocks) are merged in order: a launch packs
+ * every handle that runs that kernel next.  Periodic MHD handles without ring
+ * average or learned correction; others are refused (-1, message on the
+ * handle).  gam_graph_launch/sync/destroy as for gam_graph_create. */
+int  gam_graph_create_packed(gam_handle *const *hs, int n, int nsteps, gam_graph **out);
+/* Nodes (kernel launches and copies) of the captured graph. */
+int  gam_graph_nodes(gam_graph *g, int *nodes);
 
 #ifdef __cplusplus
 }

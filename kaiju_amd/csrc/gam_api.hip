@@ -8,8 +8,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 // The kernels are part of this translation unit.
@@ -82,6 +85,7 @@ struct gam_handle {
     int graph_refs = 0;        // captured graphs holding this handle's buffers and launch parameters
     int device = 0;            // the GPU current at gam_create (its stream's device)
     double *scratch = nullptr; // that GPU's copy of the file-scope scratch array (gam_divb_noalloc)
+    std::vector<std::unique_ptr<struct PackRec>> *rec = nullptr;   // set: launches are recorded, not run
 };
 
 // Calls that would change what a captured graph launches (kernels, their
@@ -160,6 +164,132 @@ struct StageTimer {
 };
 
 static inline dim3 grid1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// ----------------------------------------------------------------- packed launches
+// One launch of a kernel for several handles (the blocks of an ensemble): the
+// blocks of member m are blocks first[m] .. first[m] + count[m] - 1 of the
+// packed grid, first[m] a multiple of 8 so that each member's blocks keep
+// their XCD classes; each runs the kernel's body (b_*) with the member's own
+// arguments, read from a device table.  Pointers read from the table are
+// re-typed as global memory, so the bodies' loads stay global loads as they
+// are when the pointers arrive as kernel arguments.
+template <typename... A> struct PkArgs;
+template <> struct PkArgs<> {};
+template <typename A0, typename... A> struct PkArgs<A0, A...> {
+    A0 a;
+    PkArgs<A...> r;
+};
+template <size_t I, typename P> __host__ __device__ constexpr const auto &pk_get(const P &p) {
+    if constexpr (I == 0) return p.a;
+    else return pk_get<I - 1>(p.r);
+}
+template <typename... A> struct PkMake;
+template <> struct PkMake<> {
+    static PkArgs<> make() { return {}; }
+};
+template <typename A0, typename... A> struct PkMake<A0, A...> {
+    template <typename B0, typename... B> static PkArgs<A0, A...> make(B0 &&b0, B &&...b) {
+        return {static_cast<A0>(b0), PkMake<A...>::make(std::forward<B>(b)...)};
+    }
+};
+// the argument list of a body void b_*(unsigned vb, A...)
+template <typename F> struct PkSig;
+template <typename... A> struct PkSig<void (*)(unsigned, A...)> {
+    using Args = PkArgs<A...>;
+    using Make = PkMake<A...>;
+    static constexpr size_t N = sizeof...(A);
+};
+
+template <typename T> __device__ __forceinline__ T pk_glob(const T &v) { return v; }
+#ifdef GAM_EMU
+template <typename T> __device__ __forceinline__ T *pk_glob(T *const &p) { return p; }
+#else
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wincompatible-pointer-types-discards-qualifiers"
+template <typename T> __device__ __forceinline__ T *pk_glob(T *const &p) {
+    typedef __attribute__((address_space(1))) T GT;
+    return (T *)*(GT *const *)&p;
+}
+#pragma clang diagnostic pop
+#endif
+template <typename R> __device__ __forceinline__ GVT<R> pk_glob(const GVT<R> &g) {
+    GVT<R> o = g;
+    o.xyz = pk_glob(g.xyz); o.xyzcc = pk_glob(g.xyzcc); o.xfc = pk_glob(g.xfc); o.dvol = pk_glob(g.dvol);
+    o.face = pk_glob(g.face); o.Tf = pk_glob(g.Tf); o.edge = pk_glob(g.edge); o.Te = pk_glob(g.Te);
+    o.Teb = pk_glob(g.Teb); o.vol = pk_glob(g.vol); o.di = pk_glob(g.di); o.dj = pk_glob(g.dj); o.dk = pk_glob(g.dk);
+    return o;
+}
+template <typename R> __device__ __forceinline__ WKT<R> pk_glob(const WKT<R> &w) {
+    WKT<R> o;
+    o.Wh = pk_glob(w.Wh); o.Wn = pk_glob(w.Wn); o.VdV = pk_glob(w.VdV); o.bh = pk_glob(w.bh); o.bn = pk_glob(w.bn);
+    return o;
+}
+__device__ __forceinline__ BGV pk_glob(const BGV &b) {
+    BGV o = b;
+    o.fcB0 = pk_glob(b.fcB0); o.edgB0 = pk_glob(b.edgB0); o.B0 = pk_glob(b.B0); o.bFlux0 = pk_glob(b.bFlux0);
+    o.dpB0 = pk_glob(b.dpB0);
+    return o;
+}
+
+template <auto Body, typename P, size_t... I>
+__device__ __forceinline__ void pk_call(const unsigned vb, const P &p, std::index_sequence<I...>) {
+    Body(vb, pk_glob(pk_get<I>(p))...);
+}
+
+template <int NT, auto Body>
+__global__ void __launch_bounds__(NT) k_packed(const typename PkSig<decltype(Body)>::Args *__restrict__ args,
+                                               const unsigned *__restrict__ first, const unsigned *__restrict__ count,
+                                               int n) {
+    int lo = 0, hi = n - 1;   // the last member whose first block is at or below this one
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= blockIdx.x) lo = mid;
+        else hi = mid - 1;
+    }
+    const unsigned vb = blockIdx.x - first[lo];
+    if (vb >= count[lo]) return;
+    pk_call<Body>(vb, args[lo], std::make_index_sequence<PkSig<decltype(Body)>::N>{});
+}
+
+// one recorded launch of a handle (gam_handle::rec set): grid, block size and
+// the body's arguments
+struct PackRec {
+    virtual ~PackRec() {}
+    dim3 grid;
+    int nt = 256;
+    virtual const void *key() const = 0;   // the packed kernel's instantiation
+    virtual size_t arg_bytes() const = 0;
+    virtual void put_args(void *dst) const = 0;
+    virtual void launch(const void *args, const unsigned *first, const unsigned *count, int n, unsigned total,
+                        hipStream_t s) const = 0;
+};
+template <int NT, auto Body> struct PackRecT : PackRec {
+    using Args = typename PkSig<decltype(Body)>::Args;
+    Args args;
+    PackRecT(dim3 g, const Args &a) : args(a) { grid = g; nt = NT; }
+    const void *key() const override { return reinterpret_cast<const void *>(&k_packed<NT, Body>); }
+    size_t arg_bytes() const override { return sizeof(Args); }
+    void put_args(void *dst) const override { std::memcpy(dst, &args, sizeof(Args)); }
+    void launch(const void *a, const unsigned *first, const unsigned *count, int n, unsigned total,
+                hipStream_t s) const override {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_packed<NT, Body>), dim3(total), dim3(NT), 0, s, (const Args *)a, first,
+                           count, n);
+    }
+};
+
+// launches kernel KERN, or records it while h->rec is set (packed graphs);
+// KERN and BODY (the kernel's b_* body) are parenthesized
+template <auto Body, int NT, typename... B>
+static void pk_record(std::vector<std::unique_ptr<PackRec>> *rec, dim3 grid, B &&...b) {
+    using Sig = PkSig<decltype(Body)>;
+    static_assert(sizeof...(B) == Sig::N, "argument count");
+    rec->push_back(std::make_unique<PackRecT<NT, Body>>(grid, Sig::Make::make(std::forward<B>(b)...)));
+}
+#define PLAUNCH(h, KERN, BODY, grid, nt, ...)                                              \
+    do {                                                                                   \
+        if ((h)->rec) pk_record<BODY, nt>((h)->rec, (grid), __VA_ARGS__);                  \
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME KERN, (grid), (nt), 0, (h)->stream, __VA_ARGS__); \
+    } while (0)
 
 // stages whose kernels take a Tile, and the default shapes (256 threads each)
 static bool tiled_stage(int st) {
@@ -835,7 +965,7 @@ static int ensure_partial(gam_handle *h, int n) {
 // np = number of block minima in h->partial (calc != 0)
 static void launch_dtfinal(gam_handle *h, int advance, int calc, int np) {
     StageTimer t(h, ST_DTFINAL);
-    hipLaunchKernelGGL(k_dtfinal, 1, 1024, 0, h->stream, h->clk, h->partial, np, advance, calc);
+    PLAUNCH(h, (k_dtfinal), (b_dtfinal), dim3(1), 1024, h->clk, h->partial, np, advance, calc);
 }
 
 // StateCPR on State when the CalcDT just done asked for it (doCPR only; the
@@ -845,10 +975,10 @@ static void launch_cpr_t(gam_handle *h) {
     const GVT<R> &g = view<R>(h);
     const long n = (long)g.Nip * g.Njp * g.Nkp;
     if (h->mp.doBoris)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cpr<1, R>), grid1(n), 256, 0, h->stream, g, h->mp, h->bgv, h->clk,
+        PLAUNCH(h, (k_cpr<1, R>), (b_cpr<1, R>), grid1(n), 256, g, h->mp, h->bgv, h->clk,
                            P<R>(h->Gas[h->s]), P<R>(h->B[h->s]));
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cpr<0, R>), grid1(n), 256, 0, h->stream, g, h->mp, h->bgv, h->clk,
+        PLAUNCH(h, (k_cpr<0, R>), (b_cpr<0, R>), grid1(n), 256, g, h->mp, h->bgv, h->clk,
                            P<R>(h->Gas[h->s]), P<R>(h->B[h->s]));
 }
 static void launch_cpr(gam_handle *h) {
@@ -867,9 +997,9 @@ static void launch_flux_bg(gam_handle *h, int d) {
     const R *nB = P<R>(h->B[S]), *hB = P<R>(h->hB), *hmag = P<R>(h->hmag), *hGas = P<R>(h->hGas);
     R *Flx = P<R>(h->Flx);
     StageTimer t(h, ST_FLUX_I + d);
-    if (d == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<0, RM, BO, MH, BG, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-    else if (d == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<1, RM, BO, MH, BG, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_flux<2, RM, BO, MH, BG, R>), gr, 256, 0, h->stream, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    if (d == 0) PLAUNCH(h, (k_flux<0, RM, BO, MH, BG, R>), (b_flux<0, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else if (d == 1) PLAUNCH(h, (k_flux<1, RM, BO, MH, BG, R>), (b_flux<1, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else PLAUNCH(h, (k_flux<2, RM, BO, MH, BG, R>), (b_flux<2, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
 }
 
 template <int RM, int BO, int MH, typename R>
@@ -885,11 +1015,11 @@ static void launch_emf(gam_handle *h) {
     const WKT<R> w = wkt<R>(h);
     StageTimer t(h, st);
     if (ED == 1)   // J edges: k columns with the k stencil in registers
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf_jcol<RM, BO, R>), tgrid(h, st, g.Nip + 1, g.Njp, cdiv(g.Nkp + 1, EMF_KC)),
-                           256, 0, h->stream, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
+        PLAUNCH(h, (k_emf_jcol<RM, BO, R>), (b_emf_jcol<RM, BO, R>), tgrid(h, st, g.Nip + 1, g.Njp, cdiv(g.Nkp + 1, EMF_KC)),
+                256, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
     else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_emf<ED, RM, BO, R>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
-                           256, 0, h->stream, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
+        PLAUNCH(h, (k_emf<ED, RM, BO, R>), (b_emf<ED, RM, BO, R>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
+                256, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
 }
 
 template <int RM, int BO, typename R>
@@ -910,8 +1040,8 @@ static void launch_fluxes(gam_handle *h) {
 template <typename R, typename TM>
 static void launch_b2f(gam_handle *h, int st, const TM *mag, void *B) {
     const GVT<R> &g = view<R>(h);
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, g.Nk), 256, 0, h->stream, g, h->tile[st],
-                       h->clk, mag, P<R>(B));
+    PLAUNCH(h, (k_b2f<R, TM>), (b_b2f<R, TM>), tgrid(h, st, g.Ni, g.Nj, g.Nk), 256, g, h->tile[st],
+            h->clk, mag, P<R>(B));
     if (h->ring.on)   // RingFlux2FieldFix (gridutils.F90:298-300)
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g,
                            h->clk, P<R>(B));
@@ -954,15 +1084,15 @@ static int advance_t(gam_handle *h, int fuse_dt) {
     const GVT<R> &g = view<R>(h);
     const MP &m = h->mp;
     const int S = h->s, O = 1 - h->s;
-    { StageTimer t(h, ST_CLOCK); hipLaunchKernelGGL(k_clock_begin, 1, 64, 0, h->stream, h->clk); }
+    { StageTimer t(h, ST_CLOCK); PLAUNCH(h, (k_clock_begin), (b_clock_begin), dim3(1), 64, h->clk); }
     { StageTimer t(h, ST_PREDICT);
-      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_predict<R>), grid1(g.nf), 256, 0, h->stream, g, m, h->clk, P<R>(h->Gas[S]), P<R>(h->Gas[O]),
-                         h->mag[S], h->mag[O], P<R>(h->hGas), P<R>(h->hmag), wkt<R>(h)); }
+      PLAUNCH(h, (k_predict<R>), (b_predict<R>), grid1(g.nf), 256, g, m, h->clk, P<R>(h->Gas[S]), P<R>(h->Gas[O]),
+              h->mag[S], h->mag[O], P<R>(h->hGas), P<R>(h->hmag), wkt<R>(h)); }
     { StageTimer t(h, ST_BHALF); launch_b2f<R, R>(h, ST_BHALF, P<R>(h->hmag), h->hB); }
     if (m.doMHD) {
         { StageTimer t(h, ST_MOM2FACE);
-          hipLaunchKernelGGL(HIP_KERNEL_NAME(k_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, cdiv(g.Nk, VF_KC)), 256, 0, h->stream, g,
-                             h->tile[ST_MOM2FACE], h->clk, P<R>(h->wk[W_VDV]), P<R>(h->Vf)); }
+          PLAUNCH(h, (k_vf<R>), (b_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, cdiv(g.Nk, VF_KC)), 256, g,
+                  h->tile[ST_MOM2FACE], h->clk, P<R>(h->wk[W_VDV]), P<R>(h->Vf)); }
         if (m.rmeth == 1) { if (m.doBoris) launch_emfs<1, 1, R>(h); else launch_emfs<1, 0, R>(h); }
         else { if (m.doBoris) launch_emfs<0, 1, R>(h); else launch_emfs<0, 0, R>(h); }
     }
@@ -986,7 +1116,7 @@ static int advance_t(gam_handle *h, int fuse_dt) {
                            h->corr_scale, P<R>(h->Flx), P<R>(h->E));
     {
         StageTimer t(h, ST_CT);
-        if (m.doMHD) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ct<R>), grid1(g.nf), 256, 0, h->stream, g, h->clk, h->mag[S], P<R>(h->E), h->mag[O]);
+        if (m.doMHD) PLAUNCH(h, (k_ct<R>), (b_ct<R>), grid1(g.nf), 256, g, h->clk, h->mag[S], P<R>(h->E), h->mag[O]);
         else HIPCHK(h, hipMemcpyAsync(h->mag[O], h->mag[S], 3 * fsz(h) * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     }
     {
@@ -999,17 +1129,17 @@ static int advance_t(gam_handle *h, int fuse_dt) {
         // FP64: the k-column kernel; FP32 bulk: one cell per thread
         if constexpr (sizeof(R) == sizeof(double)) {
             if (m.doMHD) {
-                if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update_col<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
-                else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update_col<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+                if (m.doBoris) PLAUNCH(h, (k_update_col<1, 1, R>), (b_update_col<1, 1, R>), gr, 256, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+                else PLAUNCH(h, (k_update_col<0, 1, R>), (b_update_col<0, 1, R>), gr, 256, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
             } else {
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update_col<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+                PLAUNCH(h, (k_update_col<0, 0, R>), (b_update_col<0, 0, R>), gr, 256, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
             }
         } else {
           if (m.doMHD) {
-            if (m.doBoris) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<1, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
-            else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 1, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+            if (m.doBoris) PLAUNCH(h, (k_update<1, 1, R>), (b_update<1, 1, R>), gr, 256, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+            else PLAUNCH(h, (k_update<0, 1, R>), (b_update<0, 1, R>), gr, 256, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
         } else {
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_update<0, 0, R>), gr, 256, 0, h->stream, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
+            PLAUNCH(h, (k_update<0, 0, R>), (b_update<0, 0, R>), gr, 256, g, tu, m, h->bgv, h->clk, nGas, nB, Flx, h->mag[O], oGas, oB, h->partial, skip);
           }
         }
     }
@@ -1054,8 +1184,8 @@ static void launch_bc(gam_handle *h, bool withB) {
     const int S = h->s;
     R *Bw = withB ? P<R>(h->B[S]) : (R *)nullptr;
     if (all_periodic(h)) {
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc<R>), grid1(bc_shell_count(bc_shell(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp, g.nG, 7, h->mp.do25D))),
-                           256, 0, h->stream, g, h->clk, h->mp.do25D, 7, P<R>(h->Gas[S]), h->mag[S], Bw);
+        PLAUNCH(h, (k_bc<R>), (b_bc<R>), grid1(bc_shell_count(bc_shell(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp, g.nG, 7, h->mp.do25D))),
+                256, g, h->clk, h->mp.do25D, 7, P<R>(h->Gas[S]), h->mag[S], Bw);
         return;
     }
     const int ext[3] = {g.Ni, g.Nj, g.Nk};
@@ -1277,8 +1407,8 @@ static int bcs_t(gam_handle *h, bool shell) {
         const GVT<R> &g = view<R>(h);
         if (shell && !h->mp.do25D) {
             const long n = b2f_shell_count(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp);
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_b2f_shell<R>), dim3((unsigned)((n + 255) / 256)), 256, 0, h->stream, g, h->clk,
-                               (const double *)h->mag[S], P<R>(h->B[S]));
+            PLAUNCH(h, (k_b2f_shell<R>), (b_b2f_shell<R>), dim3((unsigned)((n + 255) / 256)), 256, g, h->clk,
+                    (const double *)h->mag[S], P<R>(h->B[S]));
             if (h->ring.on)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_ring_bfix<R>), grid1((long)g.Ni * g.Nk * g.nG * 2), 256, 0, h->stream, g,
                                    h->clk, P<R>(h->B[S]));
@@ -1430,6 +1560,7 @@ struct gam_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> evs;
+    void *pk_arena = nullptr;   // packed graphs: the members' argument tables (device)
     std::string err;
 };
 
@@ -1446,6 +1577,7 @@ extern "C" int gam_graph_destroy(gam_graph *g) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     for (auto e : g->evs) (void)hipEventDestroy(e);
+    if (g->pk_arena) (void)hipFree(g->pk_arena);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
     return 0;
@@ -1514,6 +1646,130 @@ extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_gr
     return 0;
 }
 
+// Every handle's nsteps fused steps with one launch per kernel for all of
+// them: each handle's launches are recorded (not run), and launches of one
+// kernel by several handles become one packed launch (k_packed) reading each
+// handle's arguments from a device table.  Handles with periodic boundaries
+// and MHD, without ring average or correction.
+extern "C" int gam_graph_create_packed(gam_handle *const *hs, int n, int nsteps, gam_graph **out) {
+    if (!hs || n < 1 || !out || nsteps < 2 || (nsteps & 1)) return -1;
+    *out = nullptr;
+    for (int q = 0; q < n; q++) {
+        gam_handle *h = hs[q];
+        if (!h) return -1;
+        for (int r = 0; r < q; r++) if (hs[r] == h) return -1;
+        int rc = gam_sync(h);
+        if (rc) return rc;
+        if (h->timing) return fail(h, -1, "gam_graph_create_packed: stage timers must be off");
+        if (!all_periodic(h) || h->ring.on || h->corr || !h->mp.doMHD)
+            return fail(h, -1, "gam_graph_create_packed: periodic MHD blocks only (no ring average, no correction)");
+    }
+    std::vector<std::vector<std::unique_ptr<PackRec>>> recs(n);
+    for (int q = 0; q < n; q++) {
+        gam_handle *h = hs[q];
+        h->rec = &recs[q];
+        int rc = 0;
+        for (int k = 0; k < nsteps && !rc; k++) {   // gam_step's sequence; the State swaps return with nsteps even
+            rc = advance(h, 1);
+            if (!rc) launch_dtfinal(h, 1, 1, h->fused_np);
+            if (!rc) launch_cpr(h);
+            if (!rc) rc = bcs(h, true);
+        }
+        h->rec = nullptr;
+        if (rc) return rc;
+    }
+    // Merge the handles' launch sequences: each packed launch takes, of the
+    // kernel that most handles launch next, the next launch of every handle
+    // that launches it next; every handle's launches keep their order, and
+    // handles with the same sequence (the same kind of block) share every
+    // launch.
+    struct Group { std::vector<const PackRec *> r; };
+    std::vector<Group> plan;
+    std::vector<size_t> pos(n, 0);
+    for (;;) {
+        int best = -1, bestc = 0;
+        for (int q = 0; q < n; q++) {
+            if (pos[q] >= recs[q].size()) continue;
+            const PackRec *c = recs[q][pos[q]].get();
+            int cnt = 0;
+            for (int r = 0; r < n; r++)
+                if (pos[r] < recs[r].size() && recs[r][pos[r]]->key() == c->key() && recs[r][pos[r]]->nt == c->nt) cnt++;
+            if (cnt > bestc) { best = q; bestc = cnt; }
+        }
+        if (best < 0) break;
+        const PackRec *c = recs[best][pos[best]].get();
+        Group grp;
+        for (int r = 0; r < n; r++)
+            if (pos[r] < recs[r].size() && recs[r][pos[r]]->key() == c->key() && recs[r][pos[r]]->nt == c->nt)
+                grp.r.push_back(recs[r][pos[r]++].get());
+        plan.push_back(std::move(grp));
+    }
+    const size_t L = plan.size();
+    // per packed launch: the argument table, then first[] and count[], each 256-byte aligned
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    std::vector<size_t> off(L + 1, 0);
+    std::vector<unsigned> tot(L, 0);
+    for (size_t p = 0; p < L; p++) {
+        const size_t m = plan[p].r.size(), ab = plan[p].r[0]->arg_bytes();
+        off[p + 1] = off[p] + al(ab * m) + 2 * al(sizeof(unsigned) * m);
+        for (const PackRec *r : plan[p].r)
+            if (r->grid.y != 1 || r->grid.z != 1) return fail(hs[0], -1, "gam_graph_create_packed: a launch with a 2-D grid");
+    }
+    std::vector<unsigned char> host(off[L]);
+    for (size_t p = 0; p < L; p++) {
+        const size_t m = plan[p].r.size(), ab = plan[p].r[0]->arg_bytes();
+        unsigned char *base = host.data() + off[p];
+        unsigned *first = (unsigned *)(base + al(ab * m)), *count = (unsigned *)(base + al(ab * m) + al(sizeof(unsigned) * m));
+        unsigned long long t = 0;
+        for (size_t q = 0; q < m; q++) {
+            plan[p].r[q]->put_args(base + ab * q);
+            first[q] = (unsigned)t;
+            count[q] = plan[p].r[q]->grid.x;
+            t += (count[q] + 7ull) / 8 * 8;
+        }
+        if (t > 0xffffffffull) return fail(hs[0], -1, "gam_graph_create_packed: more than 2^32 blocks in one launch");
+        tot[p] = first[m - 1] + count[m - 1];
+    }
+    gam_graph *g = new gam_graph();
+    g->hs.assign(hs, hs + n);
+    g->nsteps = nsteps;
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&g->pk_arena, host.size())) != hipSuccess ||
+        (e = hipMemcpy(g->pk_arena, host.data(), host.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        gam_graph_destroy(g);
+        return -2;
+    }
+    if ((e = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+        gam_graph_destroy(g);
+        return -2;
+    }
+    for (size_t p = 0; p < L; p++) {
+        if (!tot[p]) continue;
+        const size_t m = plan[p].r.size(), ab = plan[p].r[0]->arg_bytes();
+        const unsigned char *base = (const unsigned char *)g->pk_arena + off[p];
+        plan[p].r[0]->launch(base, (const unsigned *)(base + al(ab * m)),
+                             (const unsigned *)(base + al(ab * m) + al(sizeof(unsigned) * m)), (int)m, tot[p], g->stream);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t ee = hipStreamEndCapture(g->stream, &graph);
+    g->graph = graph;
+    if (ee != hipSuccess || (e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0)) != hipSuccess) {
+        gam_graph_destroy(g);
+        (void)hipGetLastError();
+        return -2;
+    }
+    g->ts0.assign(n, 0);
+    g->s0.assign(n, 0);
+    for (int q = 0; q < n; q++) {
+        g->s_cap.push_back(hs[q]->s);
+        g->partial_cap.push_back(hs[q]->partial);
+        hs[q]->graph_refs++;
+    }
+    *out = g;
+    return 0;
+}
+
 // Replays the graph; every handle's clock and buffers belong to the graph's
 // stream until gam_graph_sync returns.
 extern "C" int gam_graph_launch(gam_graph *g, int replays) {
@@ -1561,6 +1817,15 @@ extern "C" int gam_graph_sync(gam_graph *g) {
 }
 
 extern "C" const char *gam_graph_error(const gam_graph *g) { return g ? g->err.c_str() : "null graph"; }
+
+extern "C" int gam_graph_nodes(gam_graph *g, int *nodes) {
+    if (!g || !nodes || !g->graph) return -1;
+    size_t k = 0;
+    hipError_t e;
+    if ((e = hipGraphGetNodes(g->graph, nullptr, &k)) != hipSuccess) return gfail(g, -2, "hipGraphGetNodes", e);
+    *nodes = (int)k;
+    return 0;
+}
 
 // ----------------------------------------------------------------- timers API
 extern "C" int gam_set_timing(gam_handle *h, int enable) {

@@ -244,10 +244,10 @@ struct Tile { int ti, tj, tk, fast; };
 #define TILE_LINEAR 3
 #define TILE_NOXCD 4
 
-__device__ __forceinline__ int tile_of_block(const Tile &t, int nt) {
-    if (t.fast & TILE_NOXCD) return (int)blockIdx.x;
+__device__ __forceinline__ int tile_of_block(const Tile &t, int nt, const unsigned vb) {
+    if (t.fast & TILE_NOXCD) return (int)vb;
     const int per = cdiv(nt, 8);
-    return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    return (int)(vb & 7) * per + (int)(vb >> 3);
 }
 
 __host__ __device__ inline int tile_count(const Tile &t, int nx, int ny, int nz) {
@@ -259,11 +259,11 @@ __host__ __device__ inline int tile_blocks(const Tile &t, int nx, int ny, int nz
     return (t.fast & TILE_NOXCD) ? nt : 8 * cdiv(nt, 8);
 }
 
-// (a, b, c) of this thread in the box; false if outside
-__device__ __forceinline__ bool tile_pos(const Tile &t, int nx, int ny, int nz, int &a, int &b, int &c) {
+// (a, b, c) of this thread of block vb in the box; false if outside
+__device__ __forceinline__ bool tile_pos(const unsigned vb, const Tile &t, int nx, int ny, int nz, int &a, int &b, int &c) {
     if ((t.fast & 3) == TILE_LINEAR) {
         const int n = nx * ny * nz, bs = t.ti * t.tj * t.tk, nt = cdiv(n, bs);
-        const int q = tile_of_block(t, nt);
+        const int q = tile_of_block(t, nt, vb);
         const int idx = q * bs + (int)threadIdx.x;
         if (q >= nt || idx >= n) return false;
         a = idx % nx;
@@ -273,7 +273,7 @@ __device__ __forceinline__ bool tile_pos(const Tile &t, int nx, int ny, int nz, 
         return true;
     }
     const int ntx = cdiv(nx, t.ti), nty = cdiv(ny, t.tj), ntz = cdiv(nz, t.tk);
-    const int q = tile_of_block(t, ntx * nty * ntz);
+    const int q = tile_of_block(t, ntx * nty * ntz, vb);
     if (q >= ntx * nty * ntz) return false;
     int tx, ty, tz;
     const int f = t.fast & 3;

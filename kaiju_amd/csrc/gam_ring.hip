@@ -629,7 +629,7 @@ __global__ void __launch_bounds__(256) k_celldt_rows(GVT<R> g, MP m, BGV bg, con
         }
         dtl = cell_dt<BORIS>(g, m, U, Bv, c);
     }
-    block_min_store(dtl, partial);
+    block_min_store(dtl, partial, blockIdx.x);
 }
 
 // ----------------------------------------------------------------- solar-wind BC

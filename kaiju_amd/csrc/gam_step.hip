@@ -108,24 +108,27 @@ __device__ __forceinline__ void cell_bxyz_kc(const GVT<R> &g, const TM *mag, int
 #define UPDATE_KC 4
 
 // ----------------------------------------------------------------- clock
-__global__ void k_clock_begin(DClock *clk) {
+__device__ __forceinline__ void b_clock_begin(const unsigned vb, DClock *clk) {
     // mhdgroup.F90:25-28 trap and :31,459 pdt/odt
-    if (threadIdx.x == 0 && blockIdx.x == 0 && !clk->halt) {
+    if (threadIdx.x == 0 && vb == 0 && !clk->halt) {
         if (fabs(clk->otime - clk->stime) < GAM_TINY) clk->otime = clk->stime - clk->dt;
         clk->odt = clk->stime - clk->otime;
         clk->pdt = 0.5 * clk->dt;
     }
+}
+__global__ void k_clock_begin(DClock *clk) {
+    b_clock_begin(blockIdx.x, clk);
 }
 
 // ----------------------------------------------------------------- predictor
 // magFlux AB2 (FP64 in, bulk out) on every face point; PredictCell on every
 // cell plus the per-point stencil inputs (WKT).
 template <typename R>
-__global__ void __launch_bounds__(256) k_predict(GVT<R> g, MP m, const DClock *__restrict__ clk,
+__device__ __forceinline__ void b_predict(const unsigned vb, GVT<R> g, MP m, const DClock *__restrict__ clk,
                                                  const R *__restrict__ Gas, const R *__restrict__ oGas,
                                                  const double *__restrict__ mag, const double *__restrict__ omag,
                                                  R *__restrict__ hGas, R *__restrict__ hmag, WKT<R> w) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int idx = vb * blockDim.x + threadIdx.x;
     if (idx >= g.nf || clk->halt) return;
     const double pdt = clk->pdt, odt = clk->odt;
     const int nf = g.nf, nc = g.nc;
@@ -171,18 +174,30 @@ __global__ void __launch_bounds__(256) k_predict(GVT<R> g, MP m, const DClock *_
 #pragma unroll
     for (int x = 0; x < 3; x++) w.VdV[(size_t)nc * x + c] = vol * pQ[1 + x] / D;
 }
+template <typename R>
+__global__ void __launch_bounds__(256) k_predict(GVT<R> g, MP m, const DClock *__restrict__ clk,
+                                                 const R *__restrict__ Gas, const R *__restrict__ oGas,
+                                                 const double *__restrict__ mag, const double *__restrict__ omag,
+                                                 R *__restrict__ hGas, R *__restrict__ hmag, WKT<R> w) {
+    b_predict<R>(blockIdx.x, g, m, clk, Gas, oGas, mag, omag, hGas, hmag, w);
+}
 
 // bFlux2Fld over all cells, FP64 arithmetic, stored in the bulk type
 template <typename R, typename TM>
-__global__ void __launch_bounds__(256) k_b2f(GVT<R> g, Tile t, const DClock *__restrict__ clk, const TM *__restrict__ mag,
+__device__ __forceinline__ void b_b2f(const unsigned vb, GVT<R> g, Tile t, const DClock *__restrict__ clk, const TM *__restrict__ mag,
                                              R *__restrict__ B) {
     int i, j, k;
-    if (!tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
+    if (!tile_pos(vb, t, g.Ni, g.Nj, g.Nk, i, j, k) || clk->halt) return;
     const int idx = ci(g, i, j, k);
     double b[3];
     cell_bxyz(g, mag, i, j, k, b);
 #pragma unroll
     for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + idx] = R(b[x]);
+}
+template <typename R, typename TM>
+__global__ void __launch_bounds__(256) k_b2f(GVT<R> g, Tile t, const DClock *__restrict__ clk, const TM *__restrict__ mag,
+                                             R *__restrict__ B) {
+    b_b2f<R, TM>(blockIdx.x, g, t, clk, mag, B);
 }
 
 // bFlux2Fld on the ghost cells only.  Used after the BCs of a fused 3D step:
@@ -228,9 +243,9 @@ __device__ __forceinline__ bool shell_cell(const GVT<R> &g, long idx, int &i, in
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_b2f_shell(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ mag,
+__device__ __forceinline__ void b_b2f_shell(const unsigned vb, GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ mag,
                                                    R *__restrict__ B) {
-    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long idx = (long)vb * blockDim.x + threadIdx.x;
     int i, j, k;
     if (clk->halt || !shell_cell(g, idx, i, j, k)) return;
     const int c = ci(g, i, j, k);
@@ -238,6 +253,11 @@ __global__ void __launch_bounds__(256) k_b2f_shell(GVT<R> g, const DClock *__res
     cell_bxyz(g, mag, i, j, k, b);
 #pragma unroll
     for (int x = 0; x < 3; x++) B[(size_t)g.nc * x + c] = R(b[x]);
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_b2f_shell(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ mag,
+                                                   R *__restrict__ B) {
+    b_b2f_shell<R>(blockIdx.x, g, clk, mag, B);
 }
 
 // ----------------------------------------------------------------- E field
@@ -266,10 +286,10 @@ __device__ __forceinline__ void mom2face(const GVT<R> &g, const R *__restrict__ 
 // window kept in registers, shifted by one point per cell.
 #define VF_KC 8
 template <typename R>
-__global__ void __launch_bounds__(256) k_vf(GVT<R> g, Tile t, const DClock *__restrict__ clk, const R *__restrict__ VdV,
+__device__ __forceinline__ void b_vf(const unsigned vb, GVT<R> g, Tile t, const DClock *__restrict__ clk, const R *__restrict__ VdV,
                                             R *__restrict__ Vf) {
     int i, j, kb;
-    if (!tile_pos(t, g.Ni, g.Nj, cdiv(g.Nk, VF_KC), i, j, kb) || clk->halt) return;
+    if (!tile_pos(vb, t, g.Ni, g.Nj, cdiv(g.Nk, VF_KC), i, j, kb) || clk->halt) return;
     const int a0 = g.nG;
     const size_t nc = g.nc;
     const int sk = cstride(g, 2);
@@ -310,6 +330,11 @@ __global__ void __launch_bounds__(256) k_vf(GVT<R> g, Tile t, const DClock *__re
             st_nt(&Vf[nc * 6 + idx], c8(wX) / dV); st_nt(&Vf[nc * 7 + idx], c8(wY) / dV); st_nt(&Vf[nc * 8 + idx], c8(wZ) / dV);
         }
     }
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_vf(GVT<R> g, Tile t, const DClock *__restrict__ clk, const R *__restrict__ VdV,
+                                            R *__restrict__ Vf) {
+    b_vf<R>(blockIdx.x, g, t, clk, VdV, Vf);
 }
 
 // fields.F90:288-362 EdgeLRs: dT1 faces reconstructed along dT2 at face point f;
@@ -416,13 +441,19 @@ __device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, const BGV &b
 // (CalcElecField zeroes E, fields.F90:89).  One kernel per direction, so each
 // gets a launch brick shaped for its two stencil axes (dT1 and dT2).
 template <int ED, int RM, int BORIS, typename R>
-__global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk, const R *__restrict__ Vf,
+__device__ __forceinline__ void b_emf(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk, const R *__restrict__ Vf,
                                              const R *__restrict__ hGas, const R *__restrict__ bn,
                                              const R *__restrict__ bh, R *__restrict__ E) {
     int a, b, cz;
-    if (!tile_pos(t, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2), a, b, cz) || clk->halt) return;
+    if (!tile_pos(vb, t, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2), a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
     E[(size_t)g.nc * ED + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, ED, i, j, k);
+}
+template <int ED, int RM, int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk, const R *__restrict__ Vf,
+                                             const R *__restrict__ hGas, const R *__restrict__ bn,
+                                             const R *__restrict__ bh, R *__restrict__ E) {
+    b_emf<ED, RM, BORIS, R>(blockIdx.x, g, t, m, bg, clk, Vf, hGas, bn, bh, E);
 }
 
 // J edges walking EMF_KC edges up a k column: the I faces' stencils along k
@@ -430,12 +461,12 @@ __global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, BGV bg, con
 // one point per edge.  Box [is,ie+1]x[js,je]x[ks,ke+1].
 #define EMF_KC 8
 template <int RM, int BORIS, typename R>
-__global__ void __launch_bounds__(256) k_emf_jcol(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+__device__ __forceinline__ void b_emf_jcol(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                   const R *__restrict__ Vf, const R *__restrict__ hGas,
                                                   const R *__restrict__ bn, const R *__restrict__ bh, R *__restrict__ E) {
     const int nz = g.Nkp + 1;
     int a, b, cb;
-    if (!tile_pos(t, g.Nip + 1, g.Njp, cdiv(nz, EMF_KC), a, b, cb) || clk->halt) return;
+    if (!tile_pos(vb, t, g.Nip + 1, g.Njp, cdiv(nz, EMF_KC), a, b, cb) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG;
     const int st = fstride(g, 2);
     const R *fI = g.face, *bhI = bh, *bnI = bn;   // I-face slabs (direction 0)
@@ -461,6 +492,12 @@ __global__ void __launch_bounds__(256) k_emf_jcol(GVT<R> g, Tile t, MP m, BGV bg
         E[(size_t)g.nc * 1 + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, 1, i, j, k,
                                                                    wA, wB, wN);
     }
+}
+template <int RM, int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_emf_jcol(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                                  const R *__restrict__ Vf, const R *__restrict__ hGas,
+                                                  const R *__restrict__ bn, const R *__restrict__ bh, R *__restrict__ E) {
+    b_emf_jcol<RM, BORIS, R>(blockIdx.x, g, t, m, bg, clk, Vf, hGas, bn, bh, E);
 }
 
 // ----------------------------------------------------------------- fluxes
@@ -621,14 +658,14 @@ __device__ __forceinline__ void face_flux(const GVT<R> &g, const MP &m, const BG
 // DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
 // Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
 template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
-__global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+__device__ __forceinline__ void b_flux(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                               const R *__restrict__ Wn, const R *__restrict__ nB,
                                               const R *__restrict__ Wh, const R *__restrict__ hB,
                                               const R *__restrict__ hmag, const R *__restrict__ hGas,
                                               R *__restrict__ Flx) {
     const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
     int a, b, cz;
-    if (!tile_pos(t, nx, ny, nz, a, b, cz) || clk->halt) return;
+    if (!tile_pos(vb, t, nx, ny, nz, a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
     const size_t nc = g.nc;
     const int c = ci(g, i, j, k);
@@ -636,6 +673,14 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, co
     face_flux<DN, RM, BORIS, MHD, BG>(g, m, bg, Wn, nB, Wh, hB, hmag, hGas, c, fi(g, i, j, k), o);
 #pragma unroll
     for (int v = 0; v < (MHD ? 8 : 5); v++) Flx[nc * (8 * DN + v) + c] = o[v];
+}
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+__global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                              const R *__restrict__ Wn, const R *__restrict__ nB,
+                                              const R *__restrict__ Wh, const R *__restrict__ hB,
+                                              const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                              R *__restrict__ Flx) {
+    b_flux<DN, RM, BORIS, MHD, BG, R>(blockIdx.x, g, t, m, bg, clk, Wn, nB, Wh, hB, hmag, hGas, Flx);
 }
 
 // ----------------------------------------------------------------- CT update
@@ -652,9 +697,9 @@ __device__ __forceinline__ double Eval(const GVT<R> &g, const R *E, int d, int i
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_ct(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ magN,
+__device__ __forceinline__ void b_ct(const unsigned vb, GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ magN,
                                             const R *__restrict__ E, double *__restrict__ magO) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    int idx = vb * blockDim.x + threadIdx.x;
     if (idx >= g.nf || clk->halt) return;
     const size_t nf = g.nf;
     int i, j, k;
@@ -672,6 +717,11 @@ __global__ void __launch_bounds__(256) k_ct(GVT<R> g, const DClock *__restrict__
         m2 = m2 - dt * dfk;
     }
     magO[idx] = m0; magO[nf + idx] = m1; magO[2 * nf + idx] = m2;
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_ct(GVT<R> g, const DClock *__restrict__ clk, const double *__restrict__ magN,
+                                            const R *__restrict__ E, double *__restrict__ magO) {
+    b_ct<R>(blockIdx.x, g, clk, magN, E, magO);
 }
 
 // ----------------------------------------------------------------- update (FP64)
@@ -763,11 +813,11 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
-// block min over 256 threads -> partial[blockIdx.x]
-__device__ __forceinline__ void block_min_store(double v, double *partial) {
+// block min over 256 threads -> partial[vb] (vb: the block's index)
+__device__ __forceinline__ void block_min_store(double v, double *partial, const unsigned vb) {
 #ifdef GAM_EMU
     // the CPU emulation runs the threads of a block in order 0..blockDim-1
-    partial[blockIdx.x] = threadIdx.x == 0 ? v : fmin(partial[blockIdx.x], v);
+    partial[vb] = threadIdx.x == 0 ? v : fmin(partial[vb], v);
     return;
 #endif
     __shared__ double red[4];
@@ -778,14 +828,14 @@ __device__ __forceinline__ void block_min_store(double v, double *partial) {
     if (threadIdx.x == 0) {
         double r = red[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); q++) r = fmin(r, red[q]);
-        partial[blockIdx.x] = r;
+        partial[vb] = r;
     }
 }
 
-// block sum over 256 threads -> partial[blockIdx.x]
-__device__ __forceinline__ void block_sum_store(double v, double *partial) {
+// block sum over 256 threads -> partial[vb]
+__device__ __forceinline__ void block_sum_store(double v, double *partial, const unsigned vb) {
 #ifdef GAM_EMU
-    partial[blockIdx.x] = threadIdx.x == 0 ? v : partial[blockIdx.x] + v;
+    partial[vb] = threadIdx.x == 0 ? v : partial[vb] + v;
     return;
 #endif
     __shared__ double reds[4];
@@ -796,7 +846,7 @@ __device__ __forceinline__ void block_sum_store(double v, double *partial) {
     if (threadIdx.x == 0) {
         double r = reds[0];
         for (int q = 1; q < (int)(blockDim.x >> 6); q++) r += reds[q];
-        partial[blockIdx.x] = r;
+        partial[vb] = r;
     }
 }
 
@@ -850,7 +900,7 @@ __device__ __forceinline__ double cell_dt_stored(const GVT<R> &g, const MP &m, c
 // + CellDT min, in FP64 from the bulk-typed inputs.  Writes the new Gas/Bxyz
 // into the O buffers; ghosts copy Gas n.
 template <int BORIS, int MHD, typename R>
-__global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+__device__ __forceinline__ void b_update(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                 const R *__restrict__ nGas, const R *__restrict__ nB,
                                                 const R *__restrict__ Flx, const double *__restrict__ magNew,
                                                 R *__restrict__ oGas, R *__restrict__ oB,
@@ -858,7 +908,7 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
     if (clk->halt) return;
     double dtl = GAM_HUGE;
     int i, j, k;
-    if (tile_pos(t, g.Ni, g.Nj, g.Nk, i, j, k)) {
+    if (tile_pos(vb, t, g.Ni, g.Nj, g.Nk, i, j, k)) {
         const int idx = ci(g, i, j, k);
         const size_t nc = g.nc;
         double U[5], Bn[3] = {0, 0, 0};
@@ -896,14 +946,22 @@ __global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, 
         const bool band = skip_nb > 0 && (j < g.nG - 1 + skip_nb || j > g.nG + g.Njp - skip_nb);
         if (in_dt_box(g, i, j, k) && !band) dtl = cell_dt_stored<BORIS>(g, m, bg, idx, U, Bn);   // CFL bound of the stored state
     }
-    block_min_store(dtl, partial);
+    block_min_store(dtl, partial, vb);
+}
+template <int BORIS, int MHD, typename R>
+__global__ void __launch_bounds__(256) k_update(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                                const R *__restrict__ nGas, const R *__restrict__ nB,
+                                                const R *__restrict__ Flx, const double *__restrict__ magNew,
+                                                R *__restrict__ oGas, R *__restrict__ oB,
+                                                double *__restrict__ partial, int skip_nb) {
+    b_update<BORIS, MHD, R>(blockIdx.x, g, t, m, bg, clk, nGas, nB, Flx, magNew, oGas, oB, partial, skip_nb);
 }
 
 // k_update walking UPDATE_KC cells up a k column per thread, reading each K
 // face's fluxes, magFlux and centre once (the FP64 step's update: less HBM
 // traffic there; in FP32 bulk it measured slower and k_update is used).
 template <int BORIS, int MHD, typename R>
-__global__ void __launch_bounds__(256) k_update_col(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+__device__ __forceinline__ void b_update_col(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                 const R *__restrict__ nGas, const R *__restrict__ nB,
                                                 const R *__restrict__ Flx, const double *__restrict__ magNew,
                                                 R *__restrict__ oGas, R *__restrict__ oB,
@@ -912,7 +970,7 @@ __global__ void __launch_bounds__(256) k_update_col(GVT<R> g, Tile t, MP m, BGV 
     double dtl = GAM_HUGE;
     constexpr int KC = UPDATE_KC;
     int i, j, kb;
-    if (tile_pos(t, g.Ni, g.Nj, cdiv(g.Nk, KC), i, j, kb)) {
+    if (tile_pos(vb, t, g.Ni, g.Nj, cdiv(g.Nk, KC), i, j, kb)) {
         const size_t nc = g.nc;
         const int a0 = g.nG;
         const bool act_ij = i >= a0 && i < a0 + g.Nip && j >= a0 && j < a0 + g.Njp;
@@ -970,7 +1028,15 @@ __global__ void __launch_bounds__(256) k_update_col(GVT<R> g, Tile t, MP m, BGV 
             if (in_dt_box(g, i, j, k) && !band) dtl = fmin(dtl, cell_dt_stored<BORIS>(g, m, bg, idx, U, Bn));   // CFL bound of the stored state
         }
     }
-    block_min_store(dtl, partial);
+    block_min_store(dtl, partial, vb);
+}
+template <int BORIS, int MHD, typename R>
+__global__ void __launch_bounds__(256) k_update_col(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                                const R *__restrict__ nGas, const R *__restrict__ nB,
+                                                const R *__restrict__ Flx, const double *__restrict__ magNew,
+                                                R *__restrict__ oGas, R *__restrict__ oB,
+                                                double *__restrict__ partial, int skip_nb) {
+    b_update_col<BORIS, MHD, R>(blockIdx.x, g, t, m, bg, clk, nGas, nB, Flx, magNew, oGas, oB, partial, skip_nb);
 }
 
 // Stand-alone CellDT min over the DT box (CalcDT without the fused update)
@@ -997,12 +1063,12 @@ __global__ void __launch_bounds__(256) k_celldt(GVT<R> g, MP m, BGV bg, const DC
             dtl = cell_dt<BORIS>(g, m, U, Bv, idx);
         }
     }
-    block_min_store(dtl, partial);
+    block_min_store(dtl, partial, blockIdx.x);
 }
 
 // UpdateStateData time update (gamapp.F90:112-114, mhdgroup.F90:104-105,255)
 // when advance != 0, then CalcDT's checks and clamps (step.F90:103-166).
-__global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int np, int advance, int calc) {
+__device__ __forceinline__ void b_dtfinal(const unsigned vb, DClock *clk, const double *__restrict__ partial, int np, int advance, int calc) {
     if (clk->halt) return;
     if (threadIdx.x == 0) clk->cpr_pending = 0;
 #ifdef GAM_EMU
@@ -1052,6 +1118,9 @@ __global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int n
     // console-output dt0 capture (output.F90:57-59) at the configured cadence
     if (clk->dt0_after > 0 && clk->ts >= clk->dt0_after && clk->dt0 < GAM_TINY) clk->dt0 = clk->dt;
 }
+__global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int np, int advance, int calc) {
+    b_dtfinal(blockIdx.x, clk, partial, np, advance, calc);
+}
 
 // step.F90:279-302 StateCPR after a bad (not disastrous) CalcDT with doCPR:
 // CellCPR (step.F90:305-430, single fluid) on every active cell whose CFL step
@@ -1059,11 +1128,11 @@ __global__ void k_dtfinal(DClock *clk, const double *__restrict__ partial, int n
 // by alpha = 0.9 (velocity) or alpha^2 (pressure); a floored cell loses its
 // velocity; the Alfven branch is off in the reference.  FP64 on the stored state.
 template <int BORIS, typename R>
-__global__ void __launch_bounds__(256) k_cpr(GVT<R> g, MP m, BGV bg, const DClock *__restrict__ clk, R *__restrict__ Gas,
+__device__ __forceinline__ void b_cpr(const unsigned vb, GVT<R> g, MP m, BGV bg, const DClock *__restrict__ clk, R *__restrict__ Gas,
                                              const R *__restrict__ B) {
     if (!clk->cpr_pending || clk->halt) return;
     const int n = g.Nip * g.Njp * g.Nkp;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = vb * blockDim.x + threadIdx.x;
     if (q >= n) return;
     const int i = g.nG + q % g.Nip, r = q / g.Nip, j = g.nG + r % g.Njp, k = g.nG + r / g.Njp;
     const int c = ci(g, i, j, k);
@@ -1098,6 +1167,11 @@ __global__ void __launch_bounds__(256) k_cpr(GVT<R> g, MP m, BGV bg, const DCloc
     cell_p2c(m, pW, Un);
 #pragma unroll
     for (int v = 0; v < 5; v++) Gas[nc * v + c] = R(Un[v]);
+}
+template <int BORIS, typename R>
+__global__ void __launch_bounds__(256) k_cpr(GVT<R> g, MP m, BGV bg, const DClock *__restrict__ clk, R *__restrict__ Gas,
+                                             const R *__restrict__ B) {
+    b_cpr<BORIS, R>(blockIdx.x, g, m, bg, clk, Gas, B);
 }
 
 // ----------------------------------------------------------------- BCs
@@ -1151,10 +1225,10 @@ __host__ __device__ __forceinline__ long bc_shell_count(const BcShell &s) {
 // mask: the periodic axes (bit 0 i, 1 j, 2 k) mapped by this pass; the
 // others are left for their own BCs.
 template <typename R>
-__global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__ clk, int do25D, int mask, R *__restrict__ Gas,
+__device__ __forceinline__ void b_bc(const unsigned vb, GVT<R> g, const DClock *__restrict__ clk, int do25D, int mask, R *__restrict__ Gas,
                                             double *__restrict__ mag, R *__restrict__ B) {
     const BcShell sh = bc_shell(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp, g.nG, mask, do25D);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)vb * blockDim.x + threadIdx.x;
     const long nplane = (long)(sh.np[2] - sh.in[2]) * sh.np[0] * sh.np[1];
     const long nrow = (long)sh.in[2] * (sh.np[1] - sh.in[1]) * sh.np[0];
     if (t >= bc_shell_count(sh) || clk->halt) return;
@@ -1214,6 +1288,11 @@ __global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__
         if (!do25D && d != 2 && k == g.Nk) sk = k;
         if (si != i || sj != j || sk != k) mag[nf * d + idx] = mag[nf * d + fi(g, si, sj, sk)];
     }
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__ clk, int do25D, int mask, R *__restrict__ Gas,
+                                            double *__restrict__ mag, R *__restrict__ B) {
+    b_bc<R>(blockIdx.x, g, clk, do25D, mask, Gas, mag, B);
 }
 
 // One face's non-periodic BC, restating bcs.F90: zeroGrad_{i,o}bc{I,J,K}
@@ -1490,7 +1569,7 @@ __global__ void __launch_bounds__(256) k_corr_cells(GVT<R> g, int do25D, const f
             rG[nc * v + idx] = r;
         }
     }
-    block_sum_store(l, partial);
+    block_sum_store(l, partial, blockIdx.x);
 }
 
 // Pass 2: face residuals rM = 2 wM (dPhi - tM) on the active faces
@@ -1524,7 +1603,7 @@ __global__ void __launch_bounds__(256) k_corr_faces(GVT<R> g, const float *__res
             rM[nf * d + idx] = r;
         }
     }
-    block_sum_store(l, partial);
+    block_sum_store(l, partial, blockIdx.x);
 }
 
 // Pass 3: dL/do per cell (adjoint of passes 1-2).  An active cell gathers the

@@ -144,6 +144,8 @@ def lib():
         "gam_get_background": (C.c_int, [H, _dp, _dp, _dp, _dp, _dp]),
         "gam_background_dipole": (C.c_int, [H, C.c_double, _dp, _dp]),
         "gam_graph_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+        "gam_graph_create_packed": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+        "gam_graph_nodes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
         "gam_graph_launch": (C.c_int, [C.c_void_p, C.c_int]),
         "gam_graph_sync": (C.c_int, [C.c_void_p]),
         "gam_graph_destroy": (C.c_int, [C.c_void_p]),

@@ -39,6 +39,11 @@ STAGE_OF = [
 
 def stage_of(name: str) -> str | None:
     n = name.replace("void ", "").strip()
+    if n.startswith("k_packed<"):   # a packed launch: the stage of the body b_* it runs
+        i = n.find("b_")
+        if i < 0:
+            return None
+        n = "k_" + n[i + 2:]
     for pre, st in STAGE_OF:
         if n.startswith(pre):
             return st

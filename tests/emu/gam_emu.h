@@ -104,6 +104,7 @@ inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
     return hipSuccess;
 }
 inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphGetNodes(hipGraph_t g, void *, size_t *n) { *n = g->size(); return hipSuccess; }
 inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
 inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { *e = (void *)0x1; return hipSuccess; }
 inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }

@@ -224,12 +224,14 @@ def _totals(s: dict, vol: np.ndarray, nG: int):
 def test_ensemble_workload_32_seeded_blocks():
     """The ensemble configuration at its own size: 32 blocks drawn from seed
     31337 (edges 64-160 per axis, half Orszag-Tang, half blast waves with drawn
-    field and pressure ratio, FP32 bulk), 4 steps under one captured graph.
-    Every block steps without a CalcDT halt, stays finite, keeps its mass and
-    momentum (periodic blocks: FP32 storage rounding only) and its div.B."""
+    field and pressure ratio, FP32 bulk), 4 steps under one captured graph
+    with one launch per kernel for all 32 blocks.  Every block steps without a
+    CalcDT halt, stays finite, keeps its mass and momentum (periodic blocks:
+    FP32 storage rounding only) and its div.B."""
     specs = draw_members(32, 31337)
     say(f"[ensemble] C4 workload: {len(specs)} blocks, {sum(s.zones for s in specs)} zones")
-    with Ensemble(specs, tune=False, nsteps=2) as ens:
+    with Ensemble(specs, tune=False, nsteps=2, packed=True) as ens:
+        say(f"   {ens.nodes() // 2} launches per step")
         before = []
         for a in ens.apps:
             vol = a.get_grid(only=("volume",))["volume"]
@@ -247,15 +249,26 @@ def test_ensemble_workload_32_seeded_blocks():
 
 
 @pytest.mark.gpu
+def test_packed_graph_on_seeded_members_matches_the_oracle():
+    """The oracle-sized seeded draw of the test below under the packed graph
+    (FP64): every block agrees with the oracle's two steps of it."""
+    _seeded_members_vs_oracle(packed=True)
+
+
+@pytest.mark.gpu
 def test_seeded_small_members_under_the_graph_match_the_oracle():
     """A seeded draw of oracle-sized blocks (edges 8-20, both kinds, FP64), each
     starting from the oracle's own hatched block, stepped twice under one
     captured graph: every block agrees with the oracle's two steps of it."""
+    _seeded_members_vs_oracle(packed=False)
+
+
+def _seeded_members_vs_oracle(packed):
     specs = draw_members(6, 4242, lo=8, hi=20, precision=0)
-    say(f"[ensemble] seeded small members vs the oracle: {[(s.N, s.ic) for s in specs]}")
+    say(f"[ensemble] seeded small members vs the oracle{' (packed)' if packed else ''}: {[(s.N, s.ic) for s in specs]}")
     ocs = [oracle.OracleCase(s) for s in specs]
     apps = [app_from_oracle(oc) for oc in ocs]
-    with Ensemble(specs, tune=False, nsteps=2, apps=apps) as ens:
+    with Ensemble(specs, tune=False, nsteps=2, apps=apps, packed=packed) as ens:
         assert ens.run(2) == 0
         for a, oc in zip(ens.apps, ocs):
             oc.step(2)
@@ -263,3 +276,67 @@ def test_seeded_small_members_under_the_graph_match_the_oracle():
             say(f"   {a.spec.name}: {e}")
             assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-11, (a.spec.name, e)
             assert a.Model.ts == oc.m.ts
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [1, 0])
+def test_packed_graph_replays_match_eager_steps(precision):
+    """gam_graph_create_packed: one launch per kernel for all blocks (each
+    launch's blocks split among the members, their arguments read from a
+    device table) gives at every replay bit for bit the eager steps' state,
+    clock and status, with eager work between the replays."""
+    specs = small_members(precision)
+    say(f"[ensemble] packed graph vs eager, precision {precision}, {[s.N for s in specs]}")
+    twins = eager_twins(specs)
+    with Ensemble(specs, tune=False, nsteps=2, packed=True) as ens:
+        # blocks of one kind (Boris blast waves, Orszag-Tang vortices) share
+        # every launch, all blocks the ones without a Boris variant
+        gb = ens.other_graph(packed=False)
+        say(f"   {ens.nodes() // 2} launches per step for {len(specs)} blocks ({ens.nodes(gb) // 2} unpacked)")
+        assert ens.nodes() <= 0.4 * ens.nodes(gb)
+        for rep in range(3):
+            ens.launch(1)
+            assert ens.sync() == 0
+            for t in twins:
+                assert t.steps(2) == 0
+            for a, t in zip(ens.apps, twins):
+                same(a, t)
+                a.divb()
+            say(f"   replay {rep} identical")
+    for t in twins:
+        t.close()
+
+
+@pytest.mark.gpu
+def test_packed_graph_block_halts_where_gam_step_halts():
+    """A block whose CalcDT halts inside the packed graph stops where gam_step
+    stops; the other blocks run on."""
+    say("[ensemble] halt inside the packed graph")
+    specs = small_members(1)
+    twins = eager_twins(specs)
+    with Ensemble(specs, tune=False, nsteps=2, packed=True) as ens:
+        # dt0 far above dt: CalcDT's dt/dt0 <= limDT0 check fails after step 1
+        for x in (ens.apps[1], twins[1]):
+            c = x.get_clock()
+            c.dt0 = c.dt * 1e6
+            x.set_clock(c)
+        st = ens.run(4)
+        assert st == 1
+        assert [t.steps(4) for t in twins] == [0, 1, 0, 0]
+        assert [a.Model.ts for a in ens.apps] == [4, 1, 4, 4]
+        for a, t in zip(ens.apps, twins):
+            same(a, t)
+    for t in twins:
+        t.close()
+
+
+@pytest.mark.gpu
+def test_packed_graph_refuses_what_it_cannot_pack():
+    """Blocks the packed graph does not take (here a wall boundary) are refused
+    with a message; the per-block graph takes them."""
+    import dataclasses
+    from kaiju_amd import gamera
+    specs = small_members(1)[:2]
+    specs[0] = dataclasses.replace(specs[0], bcs=("zerograd", "zerograd", "periodic", "periodic", "periodic", "periodic"))
+    with pytest.raises(gamera.HipStepError, match="periodic MHD"):
+        Ensemble(specs, tune=False, nsteps=2, packed=True)
