@@ -208,8 +208,8 @@ def msph_leg():
     log(f"[bench] msph {spec.N}: setup {setup:.1f} s, {spec.steps} steps in {wall:.3f} s, status {st}")
     return dict(workload="magnetosphere-like block: stretched spherical-polar 192x192x256 to the x axis (hole 1e-3 rad), "
                          "r 2-30 R_E, Earth dipole background, constant solar wind on the outer face (WindBC; the wind "
-                         "starts tapered to rest inside ~12 R_E), inner radial face SphIn (zero-gradient copy of the "
-                         "first active cell, standing in for the coupled run's fixed ionospheric boundary), LFM axis "
+                         "starts tapered to rest inside ~12 R_E), fixed inner radial face (lazyBC: its ghosts keep the "
+                         "initial dipole-at-rest state, standing in for the coupled run's ionospheric boundary), LFM axis "
                          "BCs, ring average (12 rings), Boris 1000 km/s, FP32 bulk (FP64 CT/energy/dt)",
                 value=round(spec.steps * spec.zones / wall, 1), unit="zone-updates/s",
                 ms_per_step=round(wall / spec.steps * 1e3, 4), steps=spec.steps, status=st,

@@ -153,7 +153,9 @@ int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt,
  * gam_set_wind or gam_set_wind_series; k periodic; the inner i face keeps its
  * own non-periodic BC),
  * 7 cylindrical pole (cylpole on INI, bcs.F90:987-1017: j goes once around the
- * pole, periodic over an even Njp >= 8; ghosts from the cells half a turn away). 
+ * pole, periodic over an even Njp >= 8; ghosts from the cells half a turn away),
+ * 8 lazy (lazyBC, bcs.F90:268-275: the face's ghosts keep what they hold, the
+ * initial condition's values when nothing else writes them: a fixed boundary). 
  * An axis is periodic on both faces or on neither; 2.5D blocks keep k periodic.
  * Default: all periodic.  The reference's BC objects (Grid%externalBCs) map
  * onto these. */

@@ -20,7 +20,7 @@ CFL_DEFAULT = 0.30000001192092896
 
 
 # the library's boundary types, by code (gam_set_bcs)
-BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere", "lfm", "wind", "cylpole")
+BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere", "lfm", "wind", "cylpole", "lazy")
 
 
 def ring_chunks(Np: int) -> Tuple[int, ...]:
@@ -171,7 +171,8 @@ def cylinder_blast(N=(16, 16, 8), steps: int = 8, r_in: float = 0.5, r_out: floa
 
 def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in: float = 2.0, r_out: float = 30.0,
                   theta0: Optional[float] = None, Ca_kms: float = 1000.0, axis: bool = True,
-                  wind: Optional[bool] = None, chunks: Optional[Tuple[int, ...]] = None, **mkw) -> CaseSpec:
+                  wind: Optional[bool] = None, chunks: Optional[Tuple[int, ...]] = None, inner: str = "lazy",
+                  **mkw) -> CaseSpec:
     """Magnetosphere-like block (kaiju_amd/msph.py): stretched spherical-polar
     grid r in [2, 30] R_E around the x axis, Earth dipole -31000 nT R_E^3 as the
     background field, solar wind rho = 5 cm^-3, v = (-400, 0, 0) km/s (at rest
@@ -184,7 +185,10 @@ def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in:
     (or chunks: the chunks of each ring, innermost first).
     axis = False: a wedge over [theta0, pi - theta0] (default theta0 = 0.15)
     with zero-gradient theta walls.  wind (default: axis): the outer radial face
-    drives the wind in (WindBC with this constant wind) instead of SphOut."""
+    drives the wind in (WindBC with this constant wind) instead of SphOut.
+    inner: the inner radial face's BC, "lazy" (a fixed inner boundary: its
+    ghosts keep the initial state, the dipole in a gas at rest) or "sphere"
+    (SphIn, zero gradient from the first active cell)."""
     from kaiju_amd import msph
     rho, v, p, b = msph.wind_state()
     m = ModelSpec(gamma=5.0 / 3.0, CFL=0.3, doBoris=True, Ca=Ca_kms / 100.0, precision=precision, **mkw)
@@ -200,7 +204,7 @@ def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in:
                     ic_params=dict(r_in=r_in, r_out=r_out, theta0=theta0, rho=rho, v=v, p=p, b=b, axis=axis),
                     model=m, steps=steps, background=dict(M0=msph.dipole_moment(), xc=(0.0, 0.0, 0.0),
                                                           bu=(0.0, 0.0, 0.0)),
-                    bcs=("sphere", "wind" if wind else "sphere", wall, wall, "periodic", "periodic"),
+                    bcs=(inner, "wind" if wind else "sphere", wall, wall, "periodic", "periodic"),
                     ring=(tuple(chunks) if chunks else ring_chunks(nphi)) if axis else None,
                     wind=(rho, p, v, b) if wind else None)
 

@@ -1197,9 +1197,10 @@ static void launch_bc(gam_handle *h, bool withB) {
             continue;
         }
         if (ax == 0 && h->bct[1] == GAM_BC_WIND) {   // the inner face's BC, then WindBC (apps/wind.F90:206-374)
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1((long)g.nG * g.Nj * g.Nk), 256, 0, h->stream, g, h->clk, 0,
-                               h->bct[0], (int)withB, (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF],
-                               P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+            if (h->bct[0] != GAM_BC_LAZY)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1((long)g.nG * g.Nj * g.Nk), 256, 0, h->stream, g, h->clk,
+                                   0, h->bct[0], (int)withB, (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF],
+                                   P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_wind<R>), grid1((long)(g.Nj + 1) * (g.Nk + 1) * g.nG), 256, 0, h->stream, g,
                                h->mp, h->wind, h->clk, (const double *)h->grid[G_XYZCC], (const double *)h->grid[G_XFC],
                                (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF], P<R>(h->Gas[S]), h->mag[S],
@@ -1219,6 +1220,7 @@ static void launch_bc(gam_handle *h, bool withB) {
         }
         const long n = (long)g.nG * ext[ax == 0 ? 1 : 0] * ext[ax == 2 ? 1 : 2];
         for (int side = 0; side < 2; side++) {
+            if (h->bct[2 * ax + side] == GAM_BC_LAZY) continue;   // lazyBC (bcs.F90:268-275): nothing to do
             if (h->bct[2 * ax + side] == GAM_BC_CYLPOLE) {   // cylpole, the inner i face (bcs.F90:987-1017)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_cylpole<R>), grid1(n), 256, 0, h->stream, g, h->clk, (int)withB,
                                    P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
@@ -1236,7 +1238,7 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     if (refuse_if_captured(h, "gam_set_bcs")) return -1;
     for (int ax = 0; ax < 3; ax++) {
         const int a = type[2 * ax], b = type[2 * ax + 1];
-        if (a < 0 || a > GAM_BC_CYLPOLE || b < 0 || b > GAM_BC_CYLPOLE)
+        if (a < 0 || a > GAM_BC_LAZY || b < 0 || b > GAM_BC_LAZY)
             return fail(h, -1, "gam_set_bcs: unknown boundary type");
         if ((a == GAM_BC_CYLPOLE && ax != 0) || b == GAM_BC_CYLPOLE)
             return fail(h, -1, "gam_set_bcs: the cylindrical pole BC is the inner i face's BC");

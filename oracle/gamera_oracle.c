@@ -1313,6 +1313,7 @@ static void bc_cylpole(const orc_grid *g, orc_state *s) {
 }
 
 static void bc_face(const orc_model *m, const orc_grid *g, orc_state *s, int face, int type) {
+    if (type == ORC_BC_LAZY) return;   /* bcs.F90:268-275 lazyBC: the ghosts keep what they hold */
     if (type == ORC_BC_LFM) { bc_lfm(g, s, face & 1); return; }
     if (type == ORC_BC_CYLPOLE) { bc_cylpole(g, s); return; }
     if (type == ORC_BC_WIND) { bc_wind(m, g, s); return; }

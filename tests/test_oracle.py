@@ -1160,3 +1160,45 @@ def test_cylpole_ghosts_are_the_cells_across_the_pole():
                     assert np.array_equal(pts(d, ig, j, k), pts(d, pf, half, k)), (d, n, j, k)
                     want = sign[d] * (area[d][pf, half, k] @ B0)
                     assert abs(s.magFlux[ig, j, k, d] - want) <= 1e-12 * fscale, (d, n, j, k)
+
+
+@pytest.mark.parametrize("ax", [0, 1, 2])
+def test_lazy_bcs_leave_the_ghosts(ax):
+    """lazyBC (bcs.F90:268-275) does nothing: with it on both faces of one axis
+    and periodic BCs on the others, every slot coded by its (i,j,k,v) that lies
+    in that axis's ghost layers at active transverse positions keeps its code,
+    while a periodic axis's ghosts are overwritten from across the box."""
+    spec = configs.orszag_tang(16, 8)
+    spec.N = (10, 9, 8)
+    bcs = ["periodic"] * 6
+    bcs[2 * ax] = bcs[2 * ax + 1] = "lazy"
+    spec.bcs = tuple(bcs)
+    oc = oracle.OracleCase(spec, setup=False)
+    oc.gen_corners()
+    oc.metric()
+    Ni, Nj, Nk = oc.G
+    I, J, K = np.meshgrid(np.arange(Ni + 1), np.arange(Nj + 1), np.arange(Nk + 1), indexing="ij")
+    code = lambda i, j, k, v: v * 1e6 + k * 1e4 + j * 1e2 + i
+    s = oc.state
+    for v in range(5):
+        s.Gas[..., v] = code(I[:-1, :-1, :-1], J[:-1, :-1, :-1], K[:-1, :-1, :-1], v)
+    for d in range(3):
+        s.magFlux[..., d] = code(I, J, K, 20 + d)
+    gas0, mag0 = s.Gas.copy(), s.magFlux.copy()
+    L.orc_enforce_bcs(C.byref(oc.m), C.byref(oc.g), C.byref(s.s))
+    nG = spec.nG
+    act = [slice(nG, nG + n) for n in spec.N]
+    ghosts = [slice(0, nG), slice(nG + spec.N[ax], None)]
+    for gsl in ghosts:
+        sel = list(act)
+        sel[ax] = gsl
+        sel = tuple(sel)
+        assert np.array_equal(s.Gas[sel], gas0[sel])
+        assert np.array_equal(s.magFlux[sel], mag0[sel])
+    other = (ax + 1) % 3   # a periodic axis: its low ghosts now hold the high active cells
+    sel = list(act)
+    sel[other] = slice(0, nG)
+    src = list(act)
+    src[other] = slice(spec.N[other], spec.N[other] + nG)
+    assert np.array_equal(s.Gas[tuple(sel)], gas0[tuple(src)])
+    assert not np.array_equal(s.Gas[tuple(sel)], gas0[tuple(sel)])
