@@ -155,7 +155,11 @@ int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt,
  * 7 cylindrical pole (cylpole on INI, bcs.F90:987-1017: j goes once around the
  * pole, periodic over an even Njp >= 8; ghosts from the cells half a turn away),
  * 8 lazy (lazyBC, bcs.F90:268-275: the face's ghosts keep what they hold, the
- * initial condition's values when nothing else writes them: a fixed boundary). 
+ * initial condition's values when nothing else writes them: a fixed boundary),
+ * 9 ionospheric inner (IonInner on INI, voltron/ICs/earthcmi.F90:378-490, with
+ * no ionosphere attached: the coupling E field zero, so the ghosts take the
+ * density and pressure of the mirrored cell at rest and the ghost faces zero
+ * flux, the field there being the background dipole; k periodic). 
  * An axis is periodic on both faces or on neither; 2.5D blocks keep k periodic.
  * Default: all periodic.  The reference's BC objects (Grid%externalBCs) map
  * onto these. */

@@ -20,7 +20,7 @@ CFL_DEFAULT = 0.30000001192092896
 
 
 # the library's boundary types, by code (gam_set_bcs)
-BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere", "lfm", "wind", "cylpole", "lazy")
+BC_TYPES = ("periodic", "zerograd", "zeroexten", "reflect", "sphere", "lfm", "wind", "cylpole", "lazy", "ion")
 
 
 def ring_chunks(Np: int) -> Tuple[int, ...]:
@@ -171,7 +171,7 @@ def cylinder_blast(N=(16, 16, 8), steps: int = 8, r_in: float = 0.5, r_out: floa
 
 def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in: float = 2.0, r_out: float = 30.0,
                   theta0: Optional[float] = None, Ca_kms: float = 1000.0, axis: bool = True,
-                  wind: Optional[bool] = None, chunks: Optional[Tuple[int, ...]] = None, inner: str = "lazy",
+                  wind: Optional[bool] = None, chunks: Optional[Tuple[int, ...]] = None, inner: str = "ion",
                   **mkw) -> CaseSpec:
     """Magnetosphere-like block (kaiju_amd/msph.py): stretched spherical-polar
     grid r in [2, 30] R_E around the x axis, Earth dipole -31000 nT R_E^3 as the
@@ -186,9 +186,11 @@ def magnetosphere(N=(192, 192, 256), steps: int = 500, precision: int = 1, r_in:
     axis = False: a wedge over [theta0, pi - theta0] (default theta0 = 0.15)
     with zero-gradient theta walls.  wind (default: axis): the outer radial face
     drives the wind in (WindBC with this constant wind) instead of SphOut.
-    inner: the inner radial face's BC, "lazy" (a fixed inner boundary: its
-    ghosts keep the initial state, the dipole in a gas at rest) or "sphere"
-    (SphIn, zero gradient from the first active cell)."""
+    inner: the inner radial face's BC: "ion" (the Earth setup's IonInner,
+    voltron/ICs/earthcmi.F90:378-490, with no ionosphere attached: the gas of
+    the mirrored cell at rest, zero perturbation flux, so the field there is
+    the dipole), "lazy" (the ghosts keep the initial state) or "sphere"
+    (SphIn)."""
     from kaiju_amd import msph
     rho, v, p, b = msph.wind_state()
     m = ModelSpec(gamma=5.0 / 3.0, CFL=0.3, doBoris=True, Ca=Ca_kms / 100.0, precision=precision, **mkw)

@@ -1189,6 +1189,30 @@ static void launch_bc(gam_handle *h, bool withB) {
         return;
     }
     const int ext[3] = {g.Ni, g.Nj, g.Nk};
+    // one face's own (non-periodic) BC
+    auto face_bc = [&](int ax, int side) {
+        const int t = h->bct[2 * ax + side];
+        const long n = (long)g.nG * ext[ax == 0 ? 1 : 0] * ext[ax == 2 ? 1 : 2];
+        if (t == GAM_BC_LAZY) return;   // lazyBC (bcs.F90:268-275): nothing to do
+        if (t == GAM_BC_CYLPOLE) {       // cylpole, the inner i face (bcs.F90:987-1017)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_cylpole<R>), grid1(n), 256, 0, h->stream, g, h->clk, (int)withB,
+                               P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+            return;
+        }
+        if (t == GAM_BC_ION) {           // IonInner, uncoupled (voltron/ICs/earthcmi.F90:378-490)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_ion<R>), grid1((long)(g.Nj + 1) * (g.Nk + 1) * g.nG), 256, 0,
+                               h->stream, g, h->mp, h->clk, h->ring.on, P<R>(h->Gas[S]), h->mag[S]);
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wind_bxyz<R>), grid1((long)g.Nj * g.Nk * g.nG), 256, 0, h->stream, g,
+                               h->clk, h->ring.on, 1, (const double *)h->mag[S], P<R>(h->B[S]));
+            if (h->ring.on)
+                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wind_bfix<R>), grid1((long)g.nG * g.Nk * g.nG * 2), 256, 0,
+                                   h->stream, g, h->clk, 1, P<R>(h->B[S]));
+            return;
+        }
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1(n), 256, 0, h->stream, g, h->clk, 2 * ax + side, t,
+                           (int)withB, (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF],
+                           P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+    };
     for (int ax = 0; ax < 3; ax++) {
         if (h->bct[2 * ax] == GAM_BC_PERIODIC) {
             const long nsh = bc_shell_count(bc_shell(g.Ni, g.Nj, g.Nk, g.Nip, g.Njp, g.Nkp, g.nG, 1 << ax, h->mp.do25D));
@@ -1197,19 +1221,16 @@ static void launch_bc(gam_handle *h, bool withB) {
             continue;
         }
         if (ax == 0 && h->bct[1] == GAM_BC_WIND) {   // the inner face's BC, then WindBC (apps/wind.F90:206-374)
-            if (h->bct[0] != GAM_BC_LAZY)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1((long)g.nG * g.Nj * g.Nk), 256, 0, h->stream, g, h->clk,
-                                   0, h->bct[0], (int)withB, (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF],
-                                   P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
+            face_bc(0, 0);
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_wind<R>), grid1((long)(g.Nj + 1) * (g.Nk + 1) * g.nG), 256, 0, h->stream, g,
                                h->mp, h->wind, h->clk, (const double *)h->grid[G_XYZCC], (const double *)h->grid[G_XFC],
                                (const double *)h->grid[G_FACE], (const double *)h->grid[G_TF], P<R>(h->Gas[S]), h->mag[S],
                                P<R>(h->B[S]));
             hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wind_bxyz<R>), grid1((long)g.Nj * g.Nk * g.nG), 256, 0, h->stream, g, h->clk,
-                               h->ring.on, (const double *)h->mag[S], P<R>(h->B[S]));
+                               h->ring.on, 0, (const double *)h->mag[S], P<R>(h->B[S]));
             if (h->ring.on)
                 hipLaunchKernelGGL(HIP_KERNEL_NAME(k_wind_bfix<R>), grid1((long)g.nG * g.Nk * g.nG * 2), 256, 0, h->stream, g,
-                                   h->clk, P<R>(h->B[S]));
+                                   h->clk, 0, P<R>(h->B[S]));
             continue;
         }
         if (h->bct[2 * ax] == GAM_BC_LFM) {   // lfmIn then lfmOut (bcs.F90:1020-1106)
@@ -1218,18 +1239,7 @@ static void launch_bc(gam_handle *h, bool withB) {
                                    h->stream, g, h->clk, side, P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
             continue;
         }
-        const long n = (long)g.nG * ext[ax == 0 ? 1 : 0] * ext[ax == 2 ? 1 : 2];
-        for (int side = 0; side < 2; side++) {
-            if (h->bct[2 * ax + side] == GAM_BC_LAZY) continue;   // lazyBC (bcs.F90:268-275): nothing to do
-            if (h->bct[2 * ax + side] == GAM_BC_CYLPOLE) {   // cylpole, the inner i face (bcs.F90:987-1017)
-                hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_cylpole<R>), grid1(n), 256, 0, h->stream, g, h->clk, (int)withB,
-                                   P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
-                continue;
-            }
-            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_bc_face<R>), grid1(n), 256, 0, h->stream, g, h->clk, 2 * ax + side,
-                               h->bct[2 * ax + side], (int)withB, (const double *)h->grid[G_FACE],
-                               (const double *)h->grid[G_TF], P<R>(h->Gas[S]), h->mag[S], P<R>(h->B[S]));
-        }
+        for (int side = 0; side < 2; side++) face_bc(ax, side);
     }
 }
 
@@ -1238,8 +1248,10 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
     if (refuse_if_captured(h, "gam_set_bcs")) return -1;
     for (int ax = 0; ax < 3; ax++) {
         const int a = type[2 * ax], b = type[2 * ax + 1];
-        if (a < 0 || a > GAM_BC_LAZY || b < 0 || b > GAM_BC_LAZY)
+        if (a < 0 || a > GAM_BC_ION || b < 0 || b > GAM_BC_ION)
             return fail(h, -1, "gam_set_bcs: unknown boundary type");
+        if ((a == GAM_BC_ION && ax != 0) || b == GAM_BC_ION)
+            return fail(h, -1, "gam_set_bcs: the ionospheric BC is the inner i face's BC");
         if ((a == GAM_BC_CYLPOLE && ax != 0) || b == GAM_BC_CYLPOLE)
             return fail(h, -1, "gam_set_bcs: the cylindrical pole BC is the inner i face's BC");
         if (a == GAM_BC_WIND || (b == GAM_BC_WIND && ax != 0))
@@ -1257,6 +1269,8 @@ extern "C" int gam_set_bcs(gam_handle *h, const int *type) {
         return fail(h, -1, "gam_set_bcs: the LFM axis BCs need k periodic over an even Nkp >= 16");
     if (type[0] == GAM_BC_CYLPOLE && (type[2] != GAM_BC_PERIODIC || h->gv.Njp < 8 || (h->gv.Njp & 1) || type[1] == GAM_BC_WIND))
         return fail(h, -1, "gam_set_bcs: the cylindrical pole BC needs j periodic over an even Njp >= 8 (no wind)");
+    if (type[0] == GAM_BC_ION && type[4] != GAM_BC_PERIODIC)
+        return fail(h, -1, "gam_set_bcs: the ionospheric BC needs k periodic");
     if (type[1] == GAM_BC_WIND && (!h->wind.on || type[4] != GAM_BC_PERIODIC || type[0] == GAM_BC_PERIODIC))
         return fail(h, -1, "gam_set_bcs: the solar-wind BC needs gam_set_wind first, k periodic and a non-periodic i axis");
     if (h->corr && !(type[0] == 0 && type[2] == 0 && type[4] == 0))

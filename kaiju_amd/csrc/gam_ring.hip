@@ -798,14 +798,56 @@ __global__ void __launch_bounds__(256) k_bc_wind(GVT<R> g, MP m, WindP w, const 
     }
 }
 
-// WindBC's ghost Bxyz: bFlux2Fld of the cell the map names (k periodic; j
-// across the axis when jmap); thread = (i, j, k) over the ghost cells i > ie
+// ------------------------------------------------------- ionospheric inner BC
+// voltron/ICs/earthcmi.F90:378-490 IonInner with its coupling field inExyz
+// zero (no ionosphere model attached): the E x B velocity is then zero, and
+// CellPlasmaBC (apps/msphingest.F90:268-372; single fluid, no source terms)
+// gives each ghost cell the density and pressure of the cell mirrored across
+// the inner face (i = is + n - 1; j, k through lfmIJKcc) at rest; every ghost
+// face of the inner layers that bounds a cell carries zero flux (the field
+// there is the background dipole alone, isGoodFaceIJK).  The ghost Bxyz follow
+// from k_wind_bxyz / k_wind_bfix with inner set.  thread = (n, j, k) over
+// j <= Nj, k <= Nk.  FP64 arithmetic.
 template <typename R>
-__global__ void __launch_bounds__(256) k_wind_bxyz(GVT<R> g, const DClock *__restrict__ clk, int jmap,
+__global__ void __launch_bounds__(256) k_bc_ion(GVT<R> g, MP m, const DClock *__restrict__ clk, int jmap,
+                                                R *__restrict__ Gas, double *__restrict__ mag) {
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long)(g.Nj + 1) * (g.Nk + 1) * g.nG || clk->halt) return;
+    const int nl = 1 + (int)(idx % g.nG);
+    const long r = idx / g.nG;
+    const int j = (int)(r % (g.Nj + 1)), k = (int)(r / (g.Nj + 1));
+    const int ig = g.nG - nl, ip = g.nG + nl - 1;
+    const size_t nc = g.nc, nf = g.nf;
+    if (k < g.Nk && j < g.Nj) {
+        int jp = j, kp;
+        if (jmap) lfm_cc(g.nG, g.Njp, g.Nkp, j, k, jp, kp);
+        else kp = k < g.nG ? k + g.Nkp : (k > g.nG + g.Nkp - 1 ? k - g.Nkp : k);
+        const int c = ci(g, ig, j, k), cp = ci(g, ip, jp, kp);
+        double U[5], W[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) U[v] = (double)Gas[nc * v + cp];
+        cell_c2p(m, U, W);
+        const double gW[5] = {W[0], 0.0, 0.0, 0.0, W[4]};
+        cell_p2c(m, gW, U);
+#pragma unroll
+        for (int v = 0; v < 5; v++) Gas[nc * v + c] = R(U[v]);
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if ((d != 1 && j >= g.Nj) || (d != 2 && k >= g.Nk)) continue;
+        mag[nf * d + fi(g, ig, j, k)] = 0.0;
+    }
+}
+
+// WindBC's (IonInner's when inner) ghost Bxyz: bFlux2Fld of the cell the map
+// names (k periodic; j across the axis when jmap); thread = (i, j, k) over the
+// ghost cells i > ie (i < is)
+template <typename R>
+__global__ void __launch_bounds__(256) k_wind_bxyz(GVT<R> g, const DClock *__restrict__ clk, int jmap, int inner,
                                                    const double *__restrict__ mag, R *__restrict__ B) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)g.Nj * g.Nk * g.nG || clk->halt) return;
-    const int ig = g.nG + g.Nip + (int)(idx % g.nG);
+    const int ig = inner ? g.nG - 1 - (int)(idx % g.nG) : g.nG + g.Nip + (int)(idx % g.nG);
     const long r = idx / g.nG;
     const int j = (int)(r % g.Nj), k = (int)(r / g.Nj);
     int jp = j, kp;
@@ -819,12 +861,12 @@ __global__ void __launch_bounds__(256) k_wind_bxyz(GVT<R> g, const DClock *__res
 }
 
 // then, on ring grids, the j ghosts of those i ghost cells from their
-// conjugates; thread = (i > ie, k, n, pole)
+// conjugates; thread = (i > ie (i < is when inner), k, n, pole)
 template <typename R>
-__global__ void __launch_bounds__(256) k_wind_bfix(GVT<R> g, const DClock *__restrict__ clk, R *__restrict__ B) {
+__global__ void __launch_bounds__(256) k_wind_bfix(GVT<R> g, const DClock *__restrict__ clk, int inner, R *__restrict__ B) {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long)g.nG * g.Nk * g.nG * 2 || clk->halt) return;
-    const int i = g.nG + g.Nip + (int)(idx % g.nG);
+    const int i = inner ? (int)(idx % g.nG) : g.nG + g.Nip + (int)(idx % g.nG);
     long r = idx / g.nG;
     const int k = (int)(r % g.Nk);
     r /= g.Nk;

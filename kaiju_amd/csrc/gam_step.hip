@@ -1306,7 +1306,7 @@ __global__ void __launch_bounds__(256) k_bc(GVT<R> g, const DClock *__restrict__
 // GAM_BC_SPHERE: SphIn (INI, the same assignments as zeroExten_ibcI) and SphOut
 // (OUTI: every ghost face and cell from the last cell ie), bcs.F90:934-982.
 enum { GAM_BC_PERIODIC = 0, GAM_BC_ZEROGRAD = 1, GAM_BC_ZEROEXTEN = 2, GAM_BC_REFLECT = 3, GAM_BC_SPHERE = 4, GAM_BC_LFM = 5,
-       GAM_BC_WIND = 6, GAM_BC_CYLPOLE = 7, GAM_BC_LAZY = 8 };
+       GAM_BC_WIND = 6, GAM_BC_CYLPOLE = 7, GAM_BC_LAZY = 8, GAM_BC_ION = 9 };
 template <typename R>
 __global__ void __launch_bounds__(256) k_bc_face(GVT<R> g, const DClock *__restrict__ clk, int face, int type, int withB,
                                                  const double *__restrict__ face64, const double *__restrict__ Tf64,

@@ -1292,6 +1292,7 @@ double orc_calc_dt(orc_model *m, const orc_grid *g, orc_state *s) {
  * axes, as the reference's do loops; the same assignments, including the
  * reference's choices of source component and of which routines write Bxyz. */
 static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s);
+static void bc_ion(const orc_model *m, const orc_grid *g, orc_state *s);
 
 /* bcs.F90:987-1017 cylpole: inner i face of a cylindrical grid, Np = Njp cells
  * around the pole */
@@ -1314,6 +1315,7 @@ static void bc_cylpole(const orc_grid *g, orc_state *s) {
 
 static void bc_face(const orc_model *m, const orc_grid *g, orc_state *s, int face, int type) {
     if (type == ORC_BC_LAZY) return;   /* bcs.F90:268-275 lazyBC: the ghosts keep what they hold */
+    if (type == ORC_BC_ION) { bc_ion(m, g, s); return; }
     if (type == ORC_BC_LFM) { bc_lfm(g, s, face & 1); return; }
     if (type == ORC_BC_CYLPOLE) { bc_cylpole(g, s); return; }
     if (type == ORC_BC_WIND) { bc_wind(m, g, s); return; }
@@ -1599,6 +1601,66 @@ static void bc_wind(const orc_model *m, const orc_grid *g, orc_state *s) {
     if (!g->ring) return;
     for (int k = g->ksg; k <= g->keg; k++)
         for (int i = g->ie + 1; i <= g->ieg; i++)
+            for (int n = 1; n <= g->nG; n++)
+                for (int p = 0; p < 2; p++) {
+                    const int jg = p == 0 ? g->js - n : g->je + n;
+                    int ip_, jp, kp;
+                    lfm_cc(g, i, jg, k, &ip_, &jp, &kp);
+                    const size_t c = CI(g, i, jg, k), cs = CI(g, i, jp, kp);
+                    for (int x = 0; x < 3; x++) s->Bxyz[c + nc * x] = s->Bxyz[cs + nc * x];
+                }
+}
+
+/* ----------------------------------------------------- ionospheric inner BC */
+/* voltron/ICs/earthcmi.F90:378-490 IonInner with the coupling field inExyz
+ * zero (no ionosphere model): Veb = (0 x Bd)/|Bd|^2 = 0, so CellPlasmaBC
+ * (apps/msphingest.F90:268-372, single fluid, doSource off) sets each ghost
+ * (ig = is - n) to the primitives of the cell (is + n - 1, jp, kp) that
+ * lfmIJKcc names, with zero velocity; every ghost face passing
+ * isGoodFaceIJK (gridutils.F90:985-1009) gets zero flux; then the ghost Bxyz
+ * from the mapped cell's faces and, on ring grids, the j ghosts of the i
+ * ghosts from their conjugates. */
+static void bc_ion(const orc_model *m, const orc_grid *g, orc_state *s) {
+    const size_t nc = g->nc, nf = g->nf;
+    for (int k = g->ksg; k <= g->keg + 1; k++)
+        for (int j = g->jsg; j <= g->jeg + 1; j++)
+            for (int n = 1; n <= g->nG; n++) {
+                const int ig = g->is - n, ip = g->is + n - 1;
+                if (k <= g->keg && j <= g->jeg && ig >= g->isg) {
+                    int idip, jp = j, kp = k;
+                    if (g->ring) lfm_cc(g, ig, j, k, &idip, &jp, &kp);
+                    else kp = k < g->ks ? k + g->Nkp : (k > g->ke ? k - g->Nkp : k);
+                    const double Veb[3] = {0.0, 0.0, 0.0};   /* inExyz = 0 */
+                    const size_t c = CI(g, ig, j, k), cp = CI(g, ip, jp, kp);
+                    double pU[5], pW[5], gW[5], U[5];
+                    for (int v = 0; v < 5; v++) pU[v] = s->Gas[cp + nc * v];
+                    orc_cell_c2p(m, pU, pW);
+                    gW[DEN] = pW[DEN];
+                    gW[4] = pW[4];
+                    for (int x = 0; x < 3; x++) gW[1 + x] = Veb[x];
+                    orc_cell_p2c(m, gW, U);
+                    for (int v = 0; v < 5; v++) s->Gas[c + nc * v] = U[v];
+                }
+                for (int d = 0; d < 3; d++) {   /* isGoodFaceIJK */
+                    if ((d != JD && j > g->jeg) || (d != KD && k > g->keg)) continue;
+                    s->magFlux[FI(g, ig, j, k) + nf * d] = 0.0;
+                }
+            }
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int j = g->jsg; j <= g->jeg; j++)
+            for (int n = 1; n <= g->nG; n++) {
+                const int ig = g->is - n;
+                int ip_, jp = j, kp = k;
+                if (g->ring) lfm_cc(g, ig, j, k, &ip_, &jp, &kp);
+                else kp = k < g->ks ? k + g->Nkp : (k > g->ke ? k - g->Nkp : k);
+                double B[3];
+                cell_bxyz(g, s->magFlux, ig, jp, kp, B);
+                const size_t c = CI(g, ig, j, k);
+                for (int x = 0; x < 3; x++) s->Bxyz[c + nc * x] = B[x];
+            }
+    if (!g->ring) return;
+    for (int k = g->ksg; k <= g->keg; k++)
+        for (int i = g->isg; i <= g->is - 1; i++)
             for (int n = 1; n <= g->nG; n++)
                 for (int p = 0; p < 2; p++) {
                     const int jg = p == 0 ? g->js - n : g->je + n;

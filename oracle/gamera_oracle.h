@@ -51,7 +51,7 @@ typedef struct {
  * through-the-axis LFM pole (lfmIn on INJ, lfmOut on OUTJ), solar wind (WindBC on
  * OUTI, apps/wind.F90:206-374, with a wind constant in time) */
 enum { ORC_BC_PERIODIC = 0, ORC_BC_ZEROGRAD = 1, ORC_BC_ZEROEXTEN = 2, ORC_BC_REFLECT = 3, ORC_BC_SPHERE = 4,
-       ORC_BC_LFM = 5, ORC_BC_WIND = 6, ORC_BC_CYLPOLE = 7, ORC_BC_LAZY = 8 };
+       ORC_BC_LFM = 5, ORC_BC_WIND = 6, ORC_BC_CYLPOLE = 7, ORC_BC_LAZY = 8, ORC_BC_ION = 9 };
 #define ORC_MAX_RINGS 32
 
 /* Grid_T subset (reference gamtypes.F90:163-245) */

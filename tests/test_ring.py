@@ -501,6 +501,8 @@ def test_ring_and_wind_argument_checks():
             app.set_wind(*spec.wind)
             with pytest.raises(gamera.HipStepError):   # the wind BC belongs to the outer i face
                 app.set_bcs(("wind", "sphere", "lfm", "lfm", "periodic", "periodic"))
+            with pytest.raises(gamera.HipStepError):   # the ionospheric BC belongs to the inner i face
+                app.set_bcs(("sphere", "ion", "lfm", "lfm", "periodic", "periodic"))
             app.set_bcs(spec.bcs)
             assert app.get_bcs() == spec.bcs
             app.set_corners(gamera.gen_corners(spec))
@@ -649,6 +651,82 @@ def test_axis_block_wind_series_matches_oracle():
     spec = dataclasses.replace(spec, wind_series=_wind_series())
     spec, oc = _hatched_spec(spec)
     with app_from_oracle(oc) as app:
+        assert app.steps(1) == 0
+        oc.step(1)
+        e1 = compare_state(app, oc, 0)
+        assert app.steps(spec.steps - 1) == 0
+        oc.step(spec.steps - 1)
+        e = compare_state(app, oc, 0)
+        say(f"   after 1: {e1}  after {spec.steps}: {e}")
+        assert max(e1["Gas"], e1["magFlux"], e1["Bxyz"]) <= 1e-12, e1
+        assert max(e["Gas"], e["magFlux"], e["Bxyz"]) <= 1e-10, e
+        assert abs(app.Model.dt - oc.m.dt) <= 1e-12 * oc.m.dt
+
+
+# ----------------------------------------------------------------- ionospheric inner BC (uncoupled)
+def test_ion_inner_bc_known_answers():
+    """IonInner (voltron/ICs/earthcmi.F90:378-490) with no ionosphere attached
+    (inExyz = 0, so the E x B velocity is zero) through CellPlasmaBC
+    (apps/msphingest.F90:268-372): every inner ghost cell holds the density
+    and pressure of the cell mirrored across the inner face (is + n - 1, the
+    same j and k for the active rows, the conjugate across the axis for the j
+    ghost rows) at rest; every inner ghost face that bounds a cell carries zero
+    flux; the ghost Bxyz of the active rows is bFlux2Fld of the ghost cell's own
+    faces (computed here from the face fluxes on a Cartesian-free basis: the
+    cells whose six faces are all ghost faces have zero field)."""
+    import dataclasses
+    spec, oc = _oracle_case((12, 12, 32))
+    spec = dataclasses.replace(spec, bcs=("ion",) + spec.bcs[1:])
+    oc.spec = spec
+    oc.m.bc[0] = configs.BC_TYPES.index("ion")
+    oc.metric()
+    rng = np.random.default_rng(3)
+    G, F = oc.G, oc.F
+    gas = np.zeros(G + (5,), order="F")
+    gas[..., 0] = 1.0 + rng.random(G)
+    gas[..., 1:4] = rng.standard_normal(G + (3,))
+    gas[..., 4] = 10.0 + rng.random(G)
+    oc.state.Gas[...] = gas
+    oc.state.magFlux[...] = rng.standard_normal(F + (3,))
+    oc.state.Bxyz[...] = rng.standard_normal(G + (3,))
+    g0 = oc.state.Gas.copy()
+    oracle.lib().orc_enforce_bcs(C.byref(oc.m), C.byref(oc.g), C.byref(oc.state.s))
+    g = oc.g
+    gam = spec.model.gamma
+    is_ = g.is_ - g.isg
+    js, je = g.js - g.jsg, g.je - g.jsg
+    for n in range(1, NG + 1):
+        ig, ip = is_ - n, is_ + n - 1
+        for j in range(g.Nj):
+            for k in range(NG, NG + spec.N[2]):
+                if js <= j <= je:
+                    src = g0[ip, j, k]
+                else:
+                    _, jp, kp = _map(oc, 0, ig + g.isg, j + g.jsg, k + g.ksg)
+                    src = g0[ip, jp - g.jsg, kp - g.ksg]
+                p = max((gam - 1) * (src[4] - 0.5 * np.dot(src[1:4], src[1:4]) / src[0]), spec.model.pFloor)
+                want = np.array([src[0], 0.0, 0.0, 0.0, p / (gam - 1)])
+                got = oc.state.Gas[ig, j, k]
+                assert np.allclose(got, want, rtol=1e-14, atol=0), (n, j, k, got, want)
+        # ghost faces bounding cells: zero (I faces at i <= is - 1, J and K faces of ghost cells)
+        assert not oc.state.magFlux[ig, :-1, :-1, 0].any()
+        assert not oc.state.magFlux[ig, :, :-1, 1].any()
+        assert not oc.state.magFlux[ig, :-1, :, 2].any()
+    # the deepest ghost layer's cells have all six faces zero: zero field (active rows)
+    assert not oc.state.Bxyz[0, js:je + 1, NG:NG + spec.N[2]].any()
+    assert oc.state.Bxyz[is_ - 1, js:je + 1, NG:NG + spec.N[2]].any()   # the n = 1 layer sees the active face
+
+
+@pytest.mark.gpu
+def test_axis_block_ion_inner_matches_oracle():
+    """The axis block 16x12x32 with the uncoupled ionospheric inner BC
+    (IonInner, E = 0) and the solar wind outside: hatch and 4 steps against
+    the oracle."""
+    say("[ion] msph axis 16x12x32 ionospheric inner BC vs oracle")
+    spec = configs.magnetosphere((16, 12, 32), steps=4, precision=0, inner="ion")
+    spec, oc = _hatched_spec(spec)
+    with app_from_oracle(oc) as app:
+        assert app.get_bcs()[0] == "ion"
         assert app.steps(1) == 0
         oc.step(1)
         e1 = compare_state(app, oc, 0)
