@@ -993,13 +993,18 @@ static void launch_flux_bg(gam_handle *h, int d) {
     long nx = g.Nip + (d == 0), ny = g.Njp + (d == 1), nz = (h->mp.do25D ? 1 : g.Nkp) + (d == 2);
     const Tile tl = h->tile[ST_FLUX_I + d];
     dim3 gr = tgrid(h, ST_FLUX_I + d, (int)nx, (int)ny, (int)nz);
+    const dim3 gr2 = tgrid(h, ST_FLUX_I + d, (int)(nx + 1) / 2, (int)ny, (int)nz);   // j, k: interface pairs along i
     const WKT<R> w = wkt<R>(h);
     const R *nB = P<R>(h->B[S]), *hB = P<R>(h->hB), *hmag = P<R>(h->hmag), *hGas = P<R>(h->hGas);
     R *Flx = P<R>(h->Flx);
     StageTimer t(h, ST_FLUX_I + d);
     if (d == 0) PLAUNCH(h, (k_flux<0, RM, BO, MH, BG, R>), (b_flux<0, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-    else if (d == 1) PLAUNCH(h, (k_flux<1, RM, BO, MH, BG, R>), (b_flux<1, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-    else PLAUNCH(h, (k_flux<2, RM, BO, MH, BG, R>), (b_flux<2, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else if (sizeof(R) == sizeof(float)) {   // FP32 bulk: one interface per thread measured as fast as pairs
+        if (d == 1) PLAUNCH(h, (k_flux<1, RM, BO, MH, BG, R>), (b_flux<1, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+        else PLAUNCH(h, (k_flux<2, RM, BO, MH, BG, R>), (b_flux<2, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    }
+    else if (d == 1) PLAUNCH(h, (k_flux2<1, RM, BO, MH, BG, R>), (b_flux2<1, RM, BO, MH, BG, R>), gr2, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else PLAUNCH(h, (k_flux2<2, RM, BO, MH, BG, R>), (b_flux2<2, RM, BO, MH, BG, R>), gr2, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
 }
 
 template <int RM, int BO, int MH, typename R>

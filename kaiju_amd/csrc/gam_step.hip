@@ -566,55 +566,26 @@ __device__ __forceinline__ void gaskin(const MP &m, R sg, const R *P5, const R *
 // point f: out[0..4] the gas fluxes, out[5..7] the magnetic ones (MHD), each
 // times the face area.  BG: the background-field terms are compiled in (a
 // separate instantiation keeps the plain kernel's register count).
-template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
-__device__ __forceinline__ void face_flux(const GVT<R> &g, const MP &m, const BGV &bg,
-                                          const R *__restrict__ Wn, const R *__restrict__ nB,
-                                          const R *__restrict__ Wh, const R *__restrict__ hB,
-                                          const R *__restrict__ hmag, const R *__restrict__ hGas,
-                                          int c, int f, R *out) {
+// The flux of one interface from its reconstructed states (stress.F90
+// Fluxes after BlockStateLRs): kinetic gas and magnetic fluxes, HOGS, the
+// Boris nuke; out[0..4] gas and out[5..7] magnetic fluxes times the face area.
+template <int DN, int BORIS, int MHD, int BG, typename R>
+__device__ __forceinline__ void lr_flux(const GVT<R> &g, const MP &m, const BGV &bg, const R *__restrict__ hmag,
+                                        const R *__restrict__ hGas, int c, int f, const R *PL, const R *PR,
+                                        const R *MagL, const R *MagR, R *out) {
     const size_t nc = g.nc, nf = g.nf;
     const int st = cstride(g, DN);
-    const R pdmb = R(m.pdmb);
-
-    R VolB[8];
-#pragma unroll
-    for (int n = 0; n < 8; n++) VolB[n] = g.vol[c + (n - 4) * st];
-    const R rVi = R(1) / c8(VolB);
     R T[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) T[q] = g.Tf[nf * (q + 9 * DN) + f];
     const R faB = g.face[nf * DN + f];
-
-    R PL[5], PR[5];
-    R MagL[3] = {0, 0, 0}, MagR[3] = {0, 0, 0}, Bn = R(0.0);
-    if (MHD) {
-        const R *limB = m.doLFMLim ? nB : hB;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            R Q[8], nQ[4];
-#pragma unroll
-            for (int n = 0; n < 8; n++) Q[n] = hB[nc * d + c + (n - 4) * st];
-#pragma unroll
-            for (int n = 0; n < 4; n++) nQ[n] = limB[nc * d + c + (n - 2) * st];
-            get_lrs<RM>(pdmb, VolB, nQ, Q, rVi, MagL[d], MagR[d]);
-        }
-        Bn = hmag[nf * DN + f] / faB;
-    }
+    const R Bn = MHD ? hmag[nf * DN + f] / faB : R(0.0);
     R B0f[3] = {0, 0, 0}, B0n = R(0.0);
     const bool bgon = MHD && BG;
     if (bgon) {   // stress.F90:302-305
 #pragma unroll
         for (int x = 0; x < 3; x++) B0f[x] = R(bg.fcB0[nc * (x + 3 * DN) + c]);
         B0n = R(bg.bFlux0[nc * DN + c]) / faB;
-    }
-#pragma unroll
-    for (int v = 0; v < 5; v++) {   // BlockStateLRs on the prepass primitives
-        R Q[8], nQ[4];
-#pragma unroll
-        for (int n = 0; n < 8; n++) Q[n] = Wh[nc * v + c + (n - 4) * st];
-#pragma unroll
-        for (int n = 0; n < 4; n++) nQ[n] = Wn[nc * v + c + (n - 2) * st];
-        get_lrs<RM>(pdmb, VolB, nQ, Q, rVi, PL[v], PR[v]);
     }
     const R Ca = R(m.Ca);
     R VaD = R(0.0);
@@ -654,6 +625,101 @@ __device__ __forceinline__ void face_flux(const GVT<R> &g, const MP &m, const BG
     for (int v = 0; v < 5; v++) out[v] = faB * gF[v];
 }
 
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+__device__ __forceinline__ void face_flux(const GVT<R> &g, const MP &m, const BGV &bg,
+                                          const R *__restrict__ Wn, const R *__restrict__ nB,
+                                          const R *__restrict__ Wh, const R *__restrict__ hB,
+                                          const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                          int c, int f, R *out) {
+    const size_t nc = g.nc;
+    const int st = cstride(g, DN);
+    const R pdmb = R(m.pdmb);
+    R VolB[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) VolB[n] = g.vol[c + (n - 4) * st];
+    const R rVi = R(1) / c8(VolB);
+    R PL[5], PR[5];
+    R MagL[3] = {0, 0, 0}, MagR[3] = {0, 0, 0};
+    if (MHD) {
+        const R *limB = m.doLFMLim ? nB : hB;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            R Q[8], nQ[4];
+#pragma unroll
+            for (int n = 0; n < 8; n++) Q[n] = hB[nc * d + c + (n - 4) * st];
+#pragma unroll
+            for (int n = 0; n < 4; n++) nQ[n] = limB[nc * d + c + (n - 2) * st];
+            get_lrs<RM>(pdmb, VolB, nQ, Q, rVi, MagL[d], MagR[d]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {   // BlockStateLRs on the prepass primitives
+        R Q[8], nQ[4];
+#pragma unroll
+        for (int n = 0; n < 8; n++) Q[n] = Wh[nc * v + c + (n - 4) * st];
+#pragma unroll
+        for (int n = 0; n < 4; n++) nQ[n] = Wn[nc * v + c + (n - 2) * st];
+        get_lrs<RM>(pdmb, VolB, nQ, Q, rVi, PL[v], PR[v]);
+    }
+    lr_flux<DN, BORIS, MHD, BG>(g, m, bg, hmag, hGas, c, f, PL, PR, MagL, MagR, out);
+}
+
+// The same for the two interfaces at cells c and c + 1 (adjacent along i),
+// for a j or k stencil: every stencil point is one 2-wide load.  B's results
+// are formed whether or not the caller keeps them (c + 1 is always a cell of
+// the array: at most the first ghost past the active range).
+#ifdef GAM_EMU
+template <typename R> __device__ __forceinline__ void ld2(const R *p, R &a, R &b) { a = p[0]; b = p[1]; }
+#else
+template <typename R> struct P2 { typedef R T __attribute__((ext_vector_type(2), aligned(sizeof(R)))); };
+template <typename R> __device__ __forceinline__ void ld2(const R *p, R &a, R &b) {
+    const typename P2<R>::T v = *(const typename P2<R>::T *)p;
+    a = v.x;
+    b = v.y;
+}
+#endif
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+__device__ __forceinline__ void face_flux2(const GVT<R> &g, const MP &m, const BGV &bg,
+                                           const R *__restrict__ Wn, const R *__restrict__ nB,
+                                           const R *__restrict__ Wh, const R *__restrict__ hB,
+                                           const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                           int c, int f, R *outA, R *outB) {
+    const size_t nc = g.nc;
+    const int st = cstride(g, DN);
+    const R pdmb = R(m.pdmb);
+    R VA[8], VB[8];
+#pragma unroll
+    for (int n = 0; n < 8; n++) ld2(g.vol + c + (n - 4) * st, VA[n], VB[n]);
+    const R rVA = R(1) / c8(VA), rVB = R(1) / c8(VB);
+    R PLA[5], PRA[5], PLB[5], PRB[5];
+    R MLA[3] = {0, 0, 0}, MRA[3] = {0, 0, 0}, MLB[3] = {0, 0, 0}, MRB[3] = {0, 0, 0};
+    if (MHD) {
+        const R *limB = m.doLFMLim ? nB : hB;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            R QA[8], QB[8], nA[4], nBq[4];
+#pragma unroll
+            for (int n = 0; n < 8; n++) ld2(hB + nc * d + c + (n - 4) * st, QA[n], QB[n]);
+#pragma unroll
+            for (int n = 0; n < 4; n++) ld2(limB + nc * d + c + (n - 2) * st, nA[n], nBq[n]);
+            get_lrs<RM>(pdmb, VA, nA, QA, rVA, MLA[d], MRA[d]);
+            get_lrs<RM>(pdmb, VB, nBq, QB, rVB, MLB[d], MRB[d]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        R QA[8], QB[8], nA[4], nBq[4];
+#pragma unroll
+        for (int n = 0; n < 8; n++) ld2(Wh + nc * v + c + (n - 4) * st, QA[n], QB[n]);
+#pragma unroll
+        for (int n = 0; n < 4; n++) ld2(Wn + nc * v + c + (n - 2) * st, nA[n], nBq[n]);
+        get_lrs<RM>(pdmb, VA, nA, QA, rVA, PLA[v], PRA[v]);
+        get_lrs<RM>(pdmb, VB, nBq, QB, rVB, PLB[v], PRB[v]);
+    }
+    lr_flux<DN, BORIS, MHD, BG>(g, m, bg, hmag, hGas, c, f, PLA, PRA, MLA, MRA, outA);
+    lr_flux<DN, BORIS, MHD, BG>(g, m, bg, hmag, hGas, c + 1, f + 1, PLB, PRB, MLB, MRB, outB);
+}
+
 // Interface fluxes in direction DN for the interface box
 // DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
 // Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
@@ -681,6 +747,39 @@ __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, co
                                               const R *__restrict__ hmag, const R *__restrict__ hGas,
                                               R *__restrict__ Flx) {
     b_flux<DN, RM, BORIS, MHD, BG, R>(blockIdx.x, g, t, m, bg, clk, Wn, nB, Wh, hB, hmag, hGas, Flx);
+}
+
+// j or k interfaces two at a time along i (face_flux2): thread (a, b, c) of
+// the box ceil(nx/2) x ny x nz owns interfaces i = 2a and 2a + 1.
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+__device__ __forceinline__ void b_flux2(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                        const R *__restrict__ Wn, const R *__restrict__ nB,
+                                        const R *__restrict__ Wh, const R *__restrict__ hB,
+                                        const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                        R *__restrict__ Flx) {
+    static_assert(DN != 0, "pairs along i serve the j and k interfaces");
+    const int nx = g.Nip, ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
+    int a, b, cz;
+    if (!tile_pos(vb, t, (nx + 1) / 2, ny, nz, a, b, cz) || clk->halt) return;
+    const int i = 2 * a + g.nG, j = b + g.nG, k = cz + g.nG;
+    const size_t nc = g.nc;
+    const int c = ci(g, i, j, k);
+    R oA[8], oB[8];
+    face_flux2<DN, RM, BORIS, MHD, BG>(g, m, bg, Wn, nB, Wh, hB, hmag, hGas, c, fi(g, i, j, k), oA, oB);
+    const bool two = 2 * a + 1 < nx;
+#pragma unroll
+    for (int v = 0; v < (MHD ? 8 : 5); v++) {
+        Flx[nc * (8 * DN + v) + c] = oA[v];
+        if (two) Flx[nc * (8 * DN + v) + c + 1] = oB[v];
+    }
+}
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+__global__ void __launch_bounds__(256) k_flux2(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
+                                               const R *__restrict__ Wn, const R *__restrict__ nB,
+                                               const R *__restrict__ Wh, const R *__restrict__ hB,
+                                               const R *__restrict__ hmag, const R *__restrict__ hGas,
+                                               R *__restrict__ Flx) {
+    b_flux2<DN, RM, BORIS, MHD, BG, R>(blockIdx.x, g, t, m, bg, clk, Wn, nB, Wh, hB, hmag, hGas, Flx);
 }
 
 // ----------------------------------------------------------------- CT update
