@@ -31,7 +31,7 @@ STAGE_OF = [
     ("k_b2f", "bflux_half"), ("k_ring_", "ring"), ("k_pole_e", "ring"), ("k_fix_e", "ring"),
     ("k_celldt_rows", "celldt"), ("k_bc_wind", "bcs"), ("k_wind_", "bcs"),
     ("k_vf", "mom2face"), ("k_emf_jcol", "emf_j"), ("k_emf<0", "emf_i"), ("k_emf<1", "emf_j"), ("k_emf<2", "emf_k"),
-    ("k_flux<0", "flux_i"), ("k_flux<1", "flux_j"),
+    ("k_flux<0", "flux_i"), ("k_flux<1", "flux_j"), ("k_flux2<1", "flux_j"), ("k_flux2<2", "flux_k"),
     ("k_flux<2", "flux_k"), ("k_ct", "ct_update"), ("k_update", "update"),
     ("k_dtfinal", "dt_final"), ("k_bc", "bcs"), ("k_celldt", "celldt"),
 ]
