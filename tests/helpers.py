@@ -63,14 +63,17 @@ def small_cases():
     nuke = configs.orszag_tang(32, 4, steps=8, doBoris=True, Ca=0.6)
     # cylindrical shell with the through-pole inner BC (cylpole) and zero gradient outside
     cyl = configs.cylinder_blast((12, 16, 8), steps=6)
+    # odd i extents: the paired j/k interfaces end on an unpaired one, and every
+    # other row of the cell arrays starts on an odd index (8-byte aligned pairs)
+    odd = configs.blast_wave((15, 12, 10), steps=6)
     for s, n in ((ot, "ot32x32x4"), (ot8, "ot32x32x4_8cent"), (ot25, "ot32x32x1_25d"), (bw, "bw16_warp"),
                  (bwb, "bw16_warp_boris"), (hyd, "bw16_hydro"), (nolfm, "ot24x24x4_nolfmlim"),
                  (thin, "bw24x2x3_thin"), (dip, "bw16_warp_dipole"), (dipb, "bw16_warp_dipole_boris"),
                  (bcA, "ot24_bc_exten_reflect_grad"), (bcB, "ot24_bc_grad_exten_reflect_boris"),
                  (bcC, "ot24_bc_reflect_grad_exten"), (bcS, "ot24_bc_sphere"), (cpr, "bw16_warp_cpr"),
-                 (nuke, "ot32x32x4_boris_nuke"), (cyl, "cyl12x16x8_cylpole")):
+                 (nuke, "ot32x32x4_boris_nuke"), (cyl, "cyl12x16x8_cylpole"), (odd, "bw15x12x10_odd")):
         s.name = n
-    return [ot, ot8, ot25, bw, bwb, hyd, nolfm, thin, dip, dipb, bcA, bcB, bcC, bcS, cpr, nuke, cyl]
+    return [ot, ot8, ot25, bw, bwb, hyd, nolfm, thin, dip, dipb, bcA, bcB, bcC, bcS, cpr, nuke, cyl, odd]
 
 
 # B0 = (0.5, -1, 0.25) + dipole of moment (0,0,8) at (-0.7, 0.45, 0.6): |B0| ~ 1-20 in the unit cube

@@ -33,7 +33,7 @@ def test_kernels_match_oracle_on_cpu_emulation():
 def test_kernels_in_bounds_under_asan():
     lib = build_emu.build(asan=True)
     env = {"LD_PRELOAD": build_emu.asan_runtime(), "ASAN_OPTIONS": "detect_leaks=0:abort_on_error=1"}
-    out = _run(lib, env, "(test_multistep or test_hatch or test_metric or test_initial) and (bw16_warp_boris or ot32x32x4_8cent or 25d or nolfm or ot24_bc or cyl)", 900)
+    out = _run(lib, env, "(test_multistep or test_hatch or test_metric or test_initial) and (bw16_warp_boris or ot32x32x4_8cent or 25d or nolfm or ot24_bc or cyl or odd)", 900)
     assert " passed" in out
 
 
