@@ -9,7 +9,7 @@ initial condition are built on the GPU before the timed region.
 
 One step = one stepGamera: predictor, E field, stresses, CT/update, CalcDT,
 periodic BCs.  The headline value is the plain FP64 step; the same workload in
-FP32-bulk form (FP64 kept for magFlux/CT, the cell update and dt) is reported
+FP32-bulk form (FP64 kept for magFlux/CT, the energy-to-pressure recovery and dt) is reported
 beside it under "fp32_bulk" with its L1 error per variable against the FP64
 run after the same steps, and the learned-coarse form (128^3 turbulence with the
 trained correction, judged against the 256^3 FP64 solution) under

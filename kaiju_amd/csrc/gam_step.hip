@@ -825,15 +825,19 @@ __global__ void __launch_bounds__(256) k_ct(GVT<R> g, const DClock *__restrict__
 
 // ----------------------------------------------------------------- update (FP64)
 // mhdgroup.F90:300-330 CellMaxwell
-__device__ __forceinline__ void cell_maxwell(const MP &m, double *U, const double *dGM, double dt) {
-    double D = fmax(U[0], m.dFloor);
-    double Mx = U[1], My = U[2], Mz = U[3], TotE = U[4];
-    double KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
-    double P = fmax((m.gamma - 1) * (TotE - KinE), m.pFloor);
+// in the compute type T of the cell update; the total energy to pressure
+// recovery with its floors is FP64 whatever T is
+template <typename T>
+__device__ __forceinline__ void cell_maxwell(const MP &m, T *U, const T *dGM, T dt) {
+    const T D = rmax(U[0], T(m.dFloor));
+    const double Dd = fmax((double)U[0], m.dFloor);
+    const double KinE0 = 0.5 * (sq((double)U[1]) + sq((double)U[2]) + sq((double)U[3])) / Dd;
+    const double P = fmax((m.gamma - 1) * ((double)U[4] - KinE0), m.pFloor);
+    T Mx = U[1], My = U[2], Mz = U[3];
     Mx = Mx + dt * dGM[0]; My = My + dt * dGM[1]; Mz = Mz + dt * dGM[2];
-    KinE = 0.5 * (sq(Mx) + sq(My) + sq(Mz)) / D;
+    const T KinE = T(0.5) * (sq(Mx) + sq(My) + sq(Mz)) / D;
     U[0] = D; U[1] = Mx; U[2] = My; U[3] = Mz;
-    U[4] = KinE + P / (m.gamma - 1);
+    U[4] = T(KinE + P / (m.gamma - 1));
 }
 
 __device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -843,41 +847,51 @@ __device__ __forceinline__ void normvec(const double *a, double *o) {   // math.
     else { o[0] = a[0] / n; o[1] = a[1] / n; o[2] = a[2] / n; }
 }
 
-// mhdgroup.F90:337-389 CellBoris (bulk fluid)
-__device__ __forceinline__ void cell_boris(const MP &m, double *U2, const double *U0, const double *B2, const double *B0,
-                                           const double *dGH, const double *dGM, double Ca, double dt) {
-    double W2[5];
-    cell_c2p(m, U2, W2);
-    double D2 = fmax(W2[0], m.dFloor), P2 = fmax(W2[4], m.pFloor);
-    double D0 = fmax(U0[0], m.dFloor);
-    double V0[3] = {U0[1] / D0, U0[2] / D0, U0[3] / D0};
-    double B1[3] = {0.5 * (B2[0] + B0[0]), 0.5 * (B2[1] + B0[1]), 0.5 * (B2[2] + B0[2])};
-    double alpha = dot3(B1, B1) / (D2 * Ca * Ca);
-    double dRho = dt * dGH[0];
-    double dPGas[3] = {dt * dGH[1], dt * dGH[2], dt * dGH[3]};
-    double dPMag[3] = {dt * dGM[0], dt * dGM[1], dt * dGM[2]};
-    double pCon = 1.0 / (1.0 + alpha);
-    double bh1[3], bh2[3];
-    normvec(B1, bh1);
-    normvec(B2, bh2);
-    double dMom[3];
-    if (alpha > GAM_TINY) {
-        double dp = dot3(dPGas, bh1);
-        double sum[3] = {dPGas[0] + dPMag[0], dPGas[1] + dPMag[1], dPGas[2] + dPMag[2]};
-        double ds = dot3(sum, bh1), dv = dot3(V0, bh1);
-        double s = alpha * pCon * dRho;
+template <typename T> __device__ __forceinline__ T dot3t(const T *a, const T *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template <typename T> __device__ __forceinline__ void normvec_t(const T *a, T *o) {   // math.F90:332-343
+    T n = rroot(dot3t(a, a));
+    if (n <= T(GAM_TINY)) { o[0] = o[1] = o[2] = T(0.0); }
+    else { o[0] = a[0] / n; o[1] = a[1] / n; o[2] = a[2] / n; }
+}
+
+// mhdgroup.F90:337-389 CellBoris (bulk fluid), in the compute type T of the
+// cell update; the total energy to pressure recovery with its floors is FP64
+template <typename T>
+__device__ __forceinline__ void cell_boris(const MP &m, T *U2, const T *U0, const T *B2, const T *B0,
+                                           const T *dGH, const T *dGM, T Ca, T dt) {
+    double U2d[5], W2[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) U2d[v] = U2[v];
+    cell_c2p(m, U2d, W2);
+    const T D2 = T(fmax(W2[0], m.dFloor)), P2 = T(fmax(W2[4], m.pFloor));
+    T D0 = rmax(U0[0], T(m.dFloor));
+    T V0[3] = {U0[1] / D0, U0[2] / D0, U0[3] / D0};
+    T B1[3] = {T(0.5) * (B2[0] + B0[0]), T(0.5) * (B2[1] + B0[1]), T(0.5) * (B2[2] + B0[2])};
+    T alpha = dot3t(B1, B1) / (D2 * Ca * Ca);
+    T dRho = dt * dGH[0];
+    T dPGas[3] = {dt * dGH[1], dt * dGH[2], dt * dGH[3]};
+    T dPMag[3] = {dt * dGM[0], dt * dGM[1], dt * dGM[2]};
+    T pCon = T(1.0) / (T(1.0) + alpha);
+    T bh1[3];
+    normvec_t(B1, bh1);
+    T dMom[3];
+    if (alpha > T(GAM_TINY)) {
+        T dp = dot3t(dPGas, bh1);
+        T sum[3] = {dPGas[0] + dPMag[0], dPGas[1] + dPMag[1], dPGas[2] + dPMag[2]};
+        T ds = dot3t(sum, bh1), dv = dot3t(V0, bh1);
+        T s = alpha * pCon * dRho;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            double a = dp * bh1[d];
-            double b = sum[d] - ds * bh1[d];
-            double cc = V0[d] - dv * bh1[d];
+            T a = dp * bh1[d];
+            T b = sum[d] - ds * bh1[d];
+            T cc = V0[d] - dv * bh1[d];
             dMom[d] = a + pCon * b + s * cc;
         }
     } else {
 #pragma unroll
         for (int d = 0; d < 3; d++) dMom[d] = dPGas[d] + dPMag[d];
     }
-    double W[5];
+    T W[5];
     W[0] = D2;
 #pragma unroll
     for (int d = 0; d < 3; d++) W[1 + d] = (D0 * V0[d] + dMom[d]) / D2;
@@ -958,10 +972,13 @@ template <typename R> __device__ __forceinline__ bool in_dt_box(const GVT<R> &g,
 // applyUpdate of one active cell (mhdgroup.F90:79-199) from its deltas: the
 // Reynolds update of U (State n in, new state out), then Maxwell or Boris with
 // Bn, the cell field of the new face fluxes (total fields with a background).
-template <int BORIS, int MHD, typename R>
-__device__ __forceinline__ void cell_apply(const MP &m, const BGV &bg, size_t nc, int idx, double dt, const R *__restrict__ nB,
-                                           double *U, const double *Bn, const double *dGH, const double *dGM) {
-    double U0[5];
+// Reynolds then Maxwell or Boris (mhdgroup.F90:153-196) in the compute type T
+// of the cell update (the bulk type); the fields' totals with B0 are formed in
+// FP64 and rounded to T
+template <int BORIS, int MHD, typename R, typename T>
+__device__ __forceinline__ void cell_apply(const MP &m, const BGV &bg, size_t nc, int idx, T dt, const R *__restrict__ nB,
+                                           T *U, const double *Bn, const T *dGH, const T *dGM) {
+    T U0[5];
 #pragma unroll
     for (int v = 0; v < 5; v++) { U0[v] = U[v]; U[v] = U[v] + dt * dGH[v]; }   // CellReynolds, Model%dt
     if (MHD) {
@@ -972,7 +989,8 @@ __device__ __forceinline__ void cell_apply(const MP &m, const BGV &bg, size_t nc
 #pragma unroll
                 for (int x = 0; x < 3; x++) { Bt[x] = Bt[x] + bg.B0[nc * x + idx]; oBv[x] = oBv[x] + bg.B0[nc * x + idx]; }
             }
-            cell_boris(m, U, U0, Bt, oBv, dGH, dGM, m.Ca, dt);
+            const T BtT[3] = {T(Bt[0]), T(Bt[1]), T(Bt[2])}, oBT[3] = {T(oBv[0]), T(oBv[1]), T(oBv[2])};
+            cell_boris(m, U, U0, BtT, oBT, dGH, dGM, T(m.Ca), dt);
         } else {
             cell_maxwell(m, U, dGM, dt);
         }
@@ -996,7 +1014,9 @@ __device__ __forceinline__ double cell_dt_stored(const GVT<R> &g, const MP &m, c
 }
 
 // Flux2Deltas + applyUpdate (Reynolds, Bxyz of the new fluxes, Maxwell/Boris)
-// + CellDT min, in FP64 from the bulk-typed inputs.  Writes the new Gas/Bxyz
+// + CellDT min.  Flux2Deltas, Reynolds and Maxwell/Boris compute in the bulk
+// type (FP32 bulk: FP32); the energy to pressure recovery with its floors,
+// bFlux2Fld of the FP64 magFlux and CellDT stay FP64.  Writes the new Gas/Bxyz
 // into the O buffers; ghosts copy Gas n.
 template <int BORIS, int MHD, typename R>
 __device__ __forceinline__ void b_update(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
@@ -1010,18 +1030,19 @@ __device__ __forceinline__ void b_update(const unsigned vb, GVT<R> g, Tile t, MP
     if (tile_pos(vb, t, g.Ni, g.Nj, g.Nk, i, j, k)) {
         const int idx = ci(g, i, j, k);
         const size_t nc = g.nc;
-        double U[5], Bn[3] = {0, 0, 0};
+        R U[5];   // the update computes in the bulk type (FP64 step: double)
+        double Bn[3] = {0, 0, 0};
 #pragma unroll
-        for (int v = 0; v < 5; v++) U[v] = (double)nGas[nc * v + idx];
+        for (int v = 0; v < 5; v++) U[v] = nGas[nc * v + idx];
         cell_bxyz(g, magNew, i, j, k, Bn);
         const int a0 = g.nG;
         bool active = i >= a0 && i < a0 + g.Nip && j >= a0 && j < a0 + g.Njp && k >= a0 && k < a0 + g.Nkp;
         if (active) {
-            const double dt = clk->dt;
+            const R dt = R(clk->dt);
             const int ci1 = idx + 1, cj1 = idx + g.Ni, ck1 = idx + g.Ni * g.Nj;
-            const double dV = g.dvol[idx];
-            auto F = [&](int slab, int c) { return (double)Flx[nc * slab + c]; };
-            double dGH[5], dGM[3] = {0, 0, 0};
+            const R dV = R(g.dvol[idx]);
+            auto F = [&](int slab, int c) { return Flx[nc * slab + c]; };
+            R dGH[5], dGM[3] = {0, 0, 0};
 #pragma unroll
             for (int v = 0; v < 5; v++) {
                 dGH[v] = (F(0 + v, idx) - F(0 + v, ci1) + F(8 + v, idx) - F(8 + v, cj1)
@@ -1032,18 +1053,21 @@ __device__ __forceinline__ void b_update(const unsigned vb, GVT<R> g, Tile t, MP
                 for (int v = 0; v < 3; v++) {
                     dGM[v] = (F(5 + v, idx) - F(5 + v, ci1) + F(13 + v, idx) - F(13 + v, cj1)
                             + F(21 + v, idx) - F(21 + v, ck1)) / dV;
-                    if (bg.on) dGM[v] = dGM[v] + bg.dpB0[nc * v + idx];   // stress.F90:185-188
+                    if (bg.on) dGM[v] = dGM[v] + R(bg.dpB0[nc * v + idx]);   // stress.F90:185-188
                 }
             }
             cell_apply<BORIS, MHD>(m, bg, nc, idx, dt, nB, U, Bn, dGH, dGM);
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++) st_nt(&oGas[nc * v + idx], R(U[v]));
+        for (int v = 0; v < 5; v++) st_nt(&oGas[nc * v + idx], U[v]);
 #pragma unroll
         for (int x = 0; x < 3; x++) st_nt(&oB[nc * x + idx], R(Bn[x]));
         // skip_nb > 0: the ring rows' CFL bound is taken after the ring average (k_celldt_rows)
         const bool band = skip_nb > 0 && (j < g.nG - 1 + skip_nb || j > g.nG + g.Njp - skip_nb);
-        if (in_dt_box(g, i, j, k) && !band) dtl = cell_dt_stored<BORIS>(g, m, bg, idx, U, Bn);   // CFL bound of the stored state
+        if (in_dt_box(g, i, j, k) && !band) {   // CFL bound of the stored state
+            const double Ud[5] = {U[0], U[1], U[2], U[3], U[4]};
+            dtl = cell_dt_stored<BORIS>(g, m, bg, idx, Ud, Bn);
+        }
     }
     block_min_store(dtl, partial, vb);
 }
