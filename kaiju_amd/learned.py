@@ -586,8 +586,8 @@ def _timed_steps(app: GamApp, tr: Trajectory, n: int, corr: Optional[Corrector] 
     return (time.perf_counter() - t0) / n
 
 
-def equal_error_leg(N: int = 128, train_seeds=(0, 1), test_seed: int = 100, steps: int = 8, train_steps: int = 4,
-                    iters: int = 150, batch: int = 4, log=None) -> dict:
+def equal_error_leg(N: int = 128, train_seeds=(0, 1), test_seed: int = 100, steps: int = 8, train_steps: int = 8,
+                    iters: int = 600, batch: int = 4, log=None) -> dict:
     """The learned-coarse form at equal L1 error.  A network is trained on the
     GPU on the solver's (2N)^3 FP64 trajectories of the training seeds,
     downsampled 2x; on a held-out seed, N^3 FP64 runs with and without the
