@@ -506,8 +506,14 @@ def test_ring_and_wind_argument_checks():
             app.set_bcs(spec.bcs)
             assert app.get_bcs() == spec.bcs
             app.set_corners(gamera.gen_corners(spec))
-            with pytest.raises(gamera.HipStepError):   # RingGridFix belongs to the geometry built before
-                app.set_ring((8,))
+            # RingGridFix belongs to the geometry built before: another ring
+            # count, or the same count with other chunks, is refused; the
+            # same rings are accepted again
+            for chunks in ((8,), (8, 8), ()):
+                with pytest.raises(gamera.HipStepError):
+                    app.set_ring(chunks)
+            app.set_ring((8, 16))
+            assert app.get_ring() == (8, 16)
         short = dataclasses.replace(spec, N=(16, 12, 12), ring=None, wind=None,
                                     bcs=("sphere", "sphere", "periodic", "periodic", "periodic", "periodic"))
         with gamera.GamApp(short) as app:
