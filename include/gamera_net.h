@@ -24,6 +24,11 @@ int  gam_net_create(int Ni, int Nj, int Nk, gam_net **out);
 int  gam_net_destroy(gam_net *n);
 const char *gam_net_error(const gam_net *n);
 void *gam_net_stream(gam_net *n);
+/* Queue the net's work on the caller's hipStream_t (e.g. gam_stream(h) of the
+ * block it corrects, so inference, the step and training need no host
+ * synchronisation between them); NULL goes back to the net's own stream.  The
+ * caller's stream must outlive that use. */
+int  gam_net_set_stream(gam_net *n, void *stream);
 int  gam_net_sync(gam_net *n);
 
 int  gam_net_set_params(gam_net *n, const float *w);               /* host FP32 [num_params] */
@@ -38,6 +43,13 @@ int  gam_net_features(gam_net *n, const void *Gas, const void *Bxyz, int precisi
 int  gam_net_set_features(gam_net *n, const float *feat);
 
 int  gam_net_forward(gam_net *n);
+/* Inference only: features of the device state and the three layers in one
+ * call, each layer's input staged in LDS per brick.  The output is stored on
+ * the cells at least `halo` (>= 3) cells from the grid faces and equals
+ * gam_net_forward's there bit for bit; elsewhere it keeps its old values, and
+ * the activations kept for gam_net_backward are not all rewritten (train
+ * through gam_net_features + gam_net_forward). */
+int  gam_net_infer(gam_net *n, const void *Gas, const void *Bxyz, int precision, int halo);
 const float *gam_net_output(gam_net *n);                           /* device FP32 [G][8] */
 int  gam_net_get_output(gam_net *n, float *out);                   /* host copy */
 

@@ -255,24 +255,210 @@ __global__ void __launch_bounds__(256) k_dout(NetDims d, const float *__restrict
     }
 }
 
-// state -> input features ((rho - rho0)/s_rho, v/s_v, (p - p0)/s_p, B/s_B), BF16, both layouts
+// state -> input features ((rho - rho0)/s_rho, v/s_v, (p - p0)/s_p, B/s_B) of cell p
+// from the cell's (rho, rho v, E, B) rounded to FP32 in q[8]
+__device__ __forceinline__ void features_calc(const float (&q)[8], float gamma, float4 sc, float2 off, float (&f)[8]) {
+    float rho = q[0], mx = q[1], my = q[2], mz = q[3];
+    float E = q[4];
+    float r = fmaxf(rho, 1e-12f);
+    float vx = mx / r, vy = my / r, vz = mz / r;
+    float P = (gamma - 1.f) * (E - 0.5f * r * (vx * vx + vy * vy + vz * vz));
+    f[0] = (rho - off.x) / sc.x; f[1] = vx / sc.y; f[2] = vy / sc.y; f[3] = vz / sc.y; f[4] = (P - off.y) / sc.z;
+    f[5] = q[5] / sc.w; f[6] = q[6] / sc.w; f[7] = q[7] / sc.w;
+}
+template <typename R>
+__device__ __forceinline__ void state_of(size_t p, size_t n, const R *__restrict__ Gas, const R *__restrict__ B, float (&q)[8]) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) q[v] = (float)Gas[v * n + p];
+#pragma unroll
+    for (int v = 0; v < 3; v++) q[5 + v] = (float)B[v * n + p];
+}
+template <typename R>
+__device__ __forceinline__ void features_of(size_t p, size_t n, const R *__restrict__ Gas, const R *__restrict__ B, float gamma,
+                                            float4 sc, float2 off, float (&f)[8]) {
+    float q[8];
+    state_of<R>(p, n, Gas, B, q);
+    features_calc(q, gamma, sc, off, f);
+}
+
+// the features, BF16, both layouts
 template <typename R>
 __global__ void __launch_bounds__(256) k_features(int nc, int ncs, const R *__restrict__ Gas, const R *__restrict__ B, float gamma,
                                                   float4 sc, float2 off, u16 *__restrict__ act, u16 *__restrict__ actT) {
     int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= nc) return;
-    const size_t n = nc;
-    float rho = (float)Gas[p], mx = (float)Gas[n + p], my = (float)Gas[2 * n + p], mz = (float)Gas[3 * n + p];
-    float E = (float)Gas[4 * n + p];
-    float r = fmaxf(rho, 1e-12f);
-    float vx = mx / r, vy = my / r, vz = mz / r;
-    float P = (gamma - 1.f) * (E - 0.5f * r * (vx * vx + vy * vy + vz * vz));
-    float f[8] = {(rho - off.x) / sc.x, vx / sc.y, vy / sc.y, vz / sc.y, (P - off.y) / sc.z, (float)B[p] / sc.w, (float)B[n + p] / sc.w,
-                  (float)B[2 * n + p] / sc.w};
+    float f[8];
+    features_of<R>(p, nc, Gas, B, gamma, sc, off, f);
     for (int c = 0; c < NET_C; c++) {
         const u16 b = c < 8 ? f2bf(f[c]) : (u16)0;
         act[(size_t)p * NET_C + c] = b;
         actT[(size_t)c * ncs + p] = b;
+    }
+}
+
+// ----------------------------------------------------------------- inference
+// One layer over a brick of NB_I x NB_J x NB_K output cells with its input (the
+// brick and one cell around it) staged once in LDS: every input cell comes from
+// L2/HBM about 2.5 times instead of once per tap, and the MFMA A operands are
+// LDS reads.  The layer's outputs are stored on a box [lo, lo + n) per axis only
+// (the last layer's box is the cells at least `halo` from the grid faces, each
+// earlier layer's box one cell wider on every side, so every stored output reads
+// stored inputs); staged cells off the grid are zeros and feed only outputs
+// that are not stored.  The first layer (CIN = 8) computes the features of its
+// staged cells from Gas and B itself (features_of, as k_features).  Per output
+// cell the operands, their K order, the bias and the activation are k_conv's,
+// so the stored values equal gam_net_forward's bit for bit.
+// Wave w of the block computes the brick's k row w: NB_J tiles of 16 cells (one
+// per j row).  The results go through LDS to leave as whole cells (64 bytes of
+// BF16 activations, or 32 bytes of FP32 output).
+#define NB_I 16
+#ifndef NB_J
+#define NB_J 4
+#endif
+#define NB_K 4
+#define NB_LI (NB_I + 2)
+#define NB_LJ (NB_J + 2)
+#define NB_LK (NB_K + 2)
+#define NB_LC (NB_LI * NB_LJ * NB_LK)   // staged cells
+#define NB_OC (NB_I * NB_J * NB_K)      // output cells
+
+struct InferBox { int lo[3], n[3], nb[3]; };
+
+template <int CIN, int NCO, int MODE, typename R>
+__global__ void __launch_bounds__(256) k_infer(NetDims d, InferBox bx, const u16 *__restrict__ in, const R *__restrict__ Gas,
+                                               const R *__restrict__ Bc, float gamma, float4 sc, float2 offs,
+                                               const u16 *__restrict__ W, const float *__restrict__ bias,
+                                               u16 *__restrict__ act_out, float *__restrict__ out) {
+    static_assert(CIN == 8 || CIN == 32, "8 or 32 input channels");
+    static_assert(NB_I == 16 && NB_K == 4, "one 16-cell tile per row, one k row per wave");
+    constexpr int TPK = CIN == 8 ? 4 : 1;
+    constexpr int NSTEP = (NET_TAPS + TPK - 1) / TPK;
+    constexpr int OB = MODE == 0 ? 64 : 32;                          // bytes per output cell
+    // a staged cell of 32 channels takes 80 bytes: consecutive cells of an A
+    // operand then start 5 x 16 bytes apart, so the lanes of a 16-byte LDS read
+    // fall on distinct banks (64 bytes apart, every fourth lane would share them)
+    constexpr int CS = CIN == 32 ? 40 : 8;                           // BF16 elements per staged cell
+    constexpr int SB = NB_LC * CS * 2 > NB_OC * OB ? NB_LC * CS * 2 : NB_OC * OB;
+    __shared__ uint4 smem[SB / 16];
+    u16 *sA = reinterpret_cast<u16 *>(smem);
+    const int b = blockIdx.x;
+    const int bi = b % bx.nb[0], bj = (b / bx.nb[0]) % bx.nb[1], bk = b / (bx.nb[0] * bx.nb[1]);
+    const int oi = bx.lo[0] + NB_I * bi, oj = bx.lo[1] + NB_J * bj, ok = bx.lo[2] + NB_K * bk;   // brick origin
+    const size_t nc = d.nc;
+    // stage the input: cell c of the staged box is (oi - 1, oj - 1, ok - 1) + (li, lj, lk).
+    // Every load of the thread is issued before the first LDS store.
+    auto cell_at = [&](int c, size_t &p) -> bool {
+        const int li = c % NB_LI, lj = (c / NB_LI) % NB_LJ, lk = c / (NB_LI * NB_LJ);
+        const int i = oi - 1 + li, j = oj - 1 + lj, k = ok - 1 + lk;
+        p = (size_t)i + (size_t)d.Ni * ((size_t)j + (size_t)d.Nj * k);
+        return i >= 0 && i < d.Ni && j >= 0 && j < d.Nj && k >= 0 && k < d.Nk;
+    };
+    if (CIN == 8) {
+        constexpr int NIT = (NB_LC + 255) / 256;
+        float q[NIT][8];
+        bool ok_[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int c = threadIdx.x + 256 * it;
+            size_t p = 0;
+            ok_[it] = c < NB_LC && cell_at(c, p);
+            if (ok_[it]) state_of<R>(p, nc, Gas, Bc, q[it]);
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int c = threadIdx.x + 256 * it;
+            if (c >= NB_LC) break;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (ok_[it]) {
+                float f[8];
+                features_calc(q[it], gamma, sc, offs, f);
+                v.x = (uint32_t)f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16);
+                v.y = (uint32_t)f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16);
+                v.z = (uint32_t)f2bf(f[4]) | ((uint32_t)f2bf(f[5]) << 16);
+                v.w = (uint32_t)f2bf(f[6]) | ((uint32_t)f2bf(f[7]) << 16);
+            }
+            smem[c] = v;
+        }
+    } else {
+        constexpr int NIT = (NB_LC * 4 + 255) / 256;   // 4 x 16 bytes per cell, consecutive threads along i
+        uint4 v[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int e = threadIdx.x + 256 * it;
+            size_t p = 0;
+            v[it] = uint4{0u, 0u, 0u, 0u};
+            if (e < NB_LC * 4 && cell_at(e >> 2, p)) v[it] = *reinterpret_cast<const uint4 *>(in + p * NET_C + 8 * (e & 3));
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; it++) {
+            const int e = threadIdx.x + 256 * it;
+            if (e < NB_LC * 4) smem[(e >> 2) * (CS / 8) + (e & 3)] = v[it];
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = lane & 15, kg = lane >> 4;
+    f32x4 acc[NB_J][NCO / 16];
+#pragma unroll
+    for (int m = 0; m < NB_J; m++)
+#pragma unroll
+        for (int t = 0; t < NCO / 16; t++) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // the K steps' weights (the same for every block, from L2) are loaded two
+    // steps ahead of their MFMAs, so their latency overlaps the two steps before
+    const u16 *wl = W + ((size_t)row) * NET_C + 8 * kg;
+    bf16x8 wn[2][NCO / 16];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int t = 0; t < NCO / 16; t++) wn[q][t] = ld8(wl + ((size_t)q * NCO + 16 * t) * NET_C);
+    for (int st = 0; st < NSTEP; st++) {
+        const int tap = TPK == 1 ? st : TPK * st + kg;
+        const bool live = TPK == 1 || tap < NET_TAPS;   // K slots past the last tap: zero A (their weights are zero too)
+        const int tp = live ? tap : 13;                 // a staged cell for the dead slots
+        bf16x8 bw[NCO / 16];
+        const int sn = st + 2 < NSTEP ? st + 2 : NSTEP - 1;   // the step whose weights are fetched now
+#pragma unroll
+        for (int t = 0; t < NCO / 16; t++) {
+            bw[t] = wn[0][t];
+            wn[0][t] = wn[1][t];
+            wn[1][t] = ld8(wl + ((size_t)sn * NCO + 16 * t) * NET_C);
+        }
+        const int dx = tp % 3 - 1, dy = (tp / 3) % 3 - 1, dz = tp / 9 - 1;
+        const int c0 = (wave + 1 + dz) * (NB_LJ * NB_LI) + (1 + dy) * NB_LI + 1 + dx + row;
+#pragma unroll
+        for (int m = 0; m < NB_J; m++) {
+            bf16x8 a = *reinterpret_cast<const bf16x8 *>(sA + (size_t)(c0 + m * NB_LI) * CS + (TPK == 1 ? 8 * kg : 0));
+            if (!live) a = bf16x8{};
+#pragma unroll
+            for (int t = 0; t < NCO / 16; t++) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[t], acc[m][t], 0, 0, 0);
+        }
+    }
+    __syncthreads();   // every wave is done with the staged input: the LDS takes the outputs
+    // D layout: lane holds cells 4*kg + r of the tile, channel 16*t + row
+#pragma unroll
+    for (int m = 0; m < NB_J; m++)
+#pragma unroll
+        for (int t = 0; t < NCO / 16; t++) {
+            const int co = 16 * t + row;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int oc = (wave * NB_J + m) * NB_I + 4 * kg + r;
+                const float v = acc[m][t][r] + bias[co < (MODE == 0 ? NCO : 8) ? co : 0];
+                if (MODE == 0) {
+                    sA[oc * NET_C + co] = f2bf(v > 0.f ? v : NET_LEAK * v);
+                } else if (co < 8) {
+                    reinterpret_cast<float *>(smem)[oc * 8 + co] = v;
+                }
+            }
+        }
+    __syncthreads();
+    constexpr int CH = OB / 16;   // 16-byte chunks per output cell
+    for (int e = threadIdx.x; e < NB_OC * CH; e += 256) {
+        const int oc = e / CH, part = e % CH;
+        const int ii = oc % NB_I, jj = (oc / NB_I) % NB_J, kk = oc / (NB_I * NB_J);
+        if (NB_I * bi + ii >= bx.n[0] || NB_J * bj + jj >= bx.n[1] || NB_K * bk + kk >= bx.n[2]) continue;
+        const size_t p = (size_t)(oi + ii) + (size_t)d.Ni * ((size_t)(oj + jj) + (size_t)d.Nj * (ok + kk));
+        if (MODE == 0) *reinterpret_cast<uint4 *>(act_out + p * NET_C + 8 * part) = smem[e];
+        else *reinterpret_cast<uint4 *>(out + p * 8 + 4 * part) = smem[e];
     }
 }
 
@@ -317,7 +503,8 @@ __global__ void __launch_bounds__(256) k_pack0(const float *__restrict__ w, u16 
 // ----------------------------------------------------------------- host side
 struct gam_net {
     NetDims d;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;        // where the net's work goes: its own stream or the caller's
+    hipStream_t own = nullptr;           // the stream the net created
     // parameters: FP32 master, grads, Adam moments
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     u16 *Wf[3] = {}, *Wb[3] = {};
@@ -347,13 +534,13 @@ static int nfail(gam_net *n, int code, const char *msg, hipError_t e) {
 
 extern "C" int gam_net_destroy(gam_net *n) {
     if (!n) return 0;
-    if (n->stream) (void)hipStreamSynchronize(n->stream);
+    if (n->own) (void)hipStreamSynchronize(n->own);   // work on a caller's stream: hipFree below waits for the device
     void *bufs[] = {n->w, n->g, n->m, n->v, n->out, n->dbuf};
     for (void *b : bufs) if (b) (void)hipFree(b);
     for (int l = 0; l < 3; l++) { (void)hipFree(n->Wf[l]); (void)hipFree(n->Wb[l]); }
     for (int q = 0; q < 4; q++) { (void)hipFree(n->act[q]); (void)hipFree(n->actT0[q]); }
     for (int q = 0; q < 2; q++) { (void)hipFree(n->gr[q]); (void)hipFree(n->grT0[q]); }
-    if (n->stream) (void)hipStreamDestroy(n->stream);
+    if (n->own) (void)hipStreamDestroy(n->own);
     delete n;
     return 0;
 }
@@ -375,7 +562,8 @@ extern "C" int gam_net_create(int Ni, int Nj, int Nk, gam_net **out) {
         d.off[t] = dx + Ni * (dy + Nj * dz);
     }
     auto cleanup = [&](int c) { gam_net_destroy(n); return c; };
-    if (hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess) return cleanup(-2);
+    if (hipStreamCreateWithFlags(&n->own, hipStreamNonBlocking) != hipSuccess) return cleanup(-2);
+    n->stream = n->own;
     const int np = gam_net_num_params();
     for (float **p : {&n->w, &n->g, &n->m, &n->v}) {
         if (hipMalloc(p, np * sizeof(float)) != hipSuccess) return cleanup(-3);
@@ -406,6 +594,15 @@ extern "C" int gam_net_create(int Ni, int Nj, int Nk, gam_net **out) {
 
 extern "C" const char *gam_net_error(const gam_net *n) { return n ? n->err.c_str() : "null net"; }
 extern "C" void *gam_net_stream(gam_net *n) { return n ? (void *)n->stream : nullptr; }
+
+extern "C" int gam_net_set_stream(gam_net *n, void *stream) {
+    if (!n) return -1;
+    hipStream_t to = stream ? (hipStream_t)stream : n->own;
+    if (to == n->stream) return 0;
+    NCHK(n, hipStreamSynchronize(n->stream));   // the work already queued ends before any on the new stream
+    n->stream = to;
+    return 0;
+}
 extern "C" const float *gam_net_output(gam_net *n) { return n ? n->out : nullptr; }
 extern "C" float *gam_net_grad_buffer(gam_net *n) { return n ? n->dbuf : nullptr; }
 
@@ -486,6 +683,42 @@ extern "C" int gam_net_forward(gam_net *n) {
                        (const float *)(n->w + boff(1)), (const u16 *)nullptr, n->act[2], n->actT[2], (float *)nullptr);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(k_conv<16, 1, 1>), conv_grid(n), 256, 0, n->stream, d, (const u16 *)n->act[2], (const u16 *)n->Wf[2],
                        (const float *)(n->w + boff(2)), (const u16 *)nullptr, (u16 *)nullptr, (u16 *)nullptr, n->out);
+    NCHK(n, hipGetLastError());
+    return 0;
+}
+
+extern "C" int gam_net_infer(gam_net *n, const void *Gas, const void *B, int precision, int halo) {
+    if (!n || !Gas || !B) return -1;
+    const NetDims &d = n->d;
+    const int N[3] = {d.Ni, d.Nj, d.Nk};
+    if (halo < 3) { n->err = "gam_net_infer: halo must be >= 3 (three layers of 3-point stencils)"; return -1; }
+    for (int a = 0; a < 3; a++)
+        if (N[a] < 2 * halo + 1) { n->err = "gam_net_infer: the grid is thinner than 2*halo + 1 cells"; return -1; }
+    InferBox bx[3];
+    for (int l = 0; l < 3; l++) {   // layer l stores the cells at least halo - (2 - l) from the faces
+        const int lo = halo - (2 - l);
+        const int nb[3] = {NB_I, NB_J, NB_K};
+        for (int a = 0; a < 3; a++) {
+            bx[l].lo[a] = lo;
+            bx[l].n[a] = N[a] - 2 * lo;
+            bx[l].nb[a] = (bx[l].n[a] + nb[a] - 1) / nb[a];
+        }
+    }
+    auto grid = [](const InferBox &b) { return dim3((unsigned)(b.nb[0] * b.nb[1] * b.nb[2])); };
+    if (precision)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_infer<8, 32, 0, float>), grid(bx[0]), 256, 0, n->stream, d, bx[0], (const u16 *)nullptr,
+                           (const float *)Gas, (const float *)B, n->gamma, n->scales, n->offs, (const u16 *)n->Wf[0],
+                           (const float *)(n->w + boff(0)), n->act[1], (float *)nullptr);
+    else
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_infer<8, 32, 0, double>), grid(bx[0]), 256, 0, n->stream, d, bx[0], (const u16 *)nullptr,
+                           (const double *)Gas, (const double *)B, n->gamma, n->scales, n->offs, (const u16 *)n->Wf[0],
+                           (const float *)(n->w + boff(0)), n->act[1], (float *)nullptr);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_infer<32, 32, 0, float>), grid(bx[1]), 256, 0, n->stream, d, bx[1], (const u16 *)n->act[1],
+                       (const float *)nullptr, (const float *)nullptr, n->gamma, n->scales, n->offs, (const u16 *)n->Wf[1],
+                       (const float *)(n->w + boff(1)), n->act[2], (float *)nullptr);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_infer<32, 16, 1, float>), grid(bx[2]), 256, 0, n->stream, d, bx[2], (const u16 *)n->act[2],
+                       (const float *)nullptr, (const float *)nullptr, n->gamma, n->scales, n->offs, (const u16 *)n->Wf[2],
+                       (const float *)(n->w + boff(2)), (u16 *)nullptr, n->out);
     NCHK(n, hipGetLastError());
     return 0;
 }

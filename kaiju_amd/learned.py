@@ -43,7 +43,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from kaiju_amd import configs
-from kaiju_amd.gamera import DeviceArray, GamApp
+from kaiju_amd.gamera import DeviceArray, GamApp, lib as gam_lib
 
 GAMMA = 5.0 / 3.0
 PRIM = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
@@ -383,6 +383,7 @@ class Corrector:
         from kaiju_amd.net import GamNet, init_params
         self.app, self.norm = app, norm
         self.net = GamNet(*app.G)
+        self.net.set_stream(gam_lib().gam_stream(app.h))   # the block's stream: no host waits between net and step
         self.net.set_params(init_params(seed, out_scale=0.02) if params is None else params)
         self.net.set_scales(*norm.feat_scales, gamma=GAMMA)
         self.net.set_offsets(*norm.feat_offsets)
@@ -394,7 +395,17 @@ class Corrector:
         self.zero.free()
 
     def infer(self):
-        """Correction field for the app's current State (device, FP32 [G][8])."""
+        """Correction field for the app's current State (device, FP32 [G][8]),
+        stored on the active cells, the only ones the step reads
+        (gam_net_infer: LDS-staged layers, no activations kept for training).
+        Queued on the block's stream, so a step queued after it reads it."""
+        g, _, b = self.app.state_device(0)
+        self.net.infer(g, b, self.precision, self.app.nG)
+        return self.net.output_ptr()
+
+    def forward(self):
+        """The same field through the training forward, which keeps every
+        layer's activations for gam_net_backward."""
         g, _, b = self.app.state_device(0)
         self.net.features_from_device(g, b, self.precision)
         self.net.forward()
@@ -403,7 +414,7 @@ class Corrector:
 
     def _loss(self, smp: Sample, dt: float, o_ptr=None, backward=False) -> float:
         self.app.put_state_device(0, smp.t, smp.gas.ptr, smp.mag.ptr)
-        o = self.infer() if o_ptr is None else o_ptr
+        o = self.forward() if o_ptr is None else o_ptr
         self.app.correction_target_device(smp.tg.ptr, smp.tm.ptr)
         dbuf = self.net.grad_buffer_ptr()
         loss = self.app.correction_grad(o, self.norm.scale, dt, self.norm.weights, dbuf)
@@ -453,9 +464,11 @@ def rollout(app: GamApp, tr: Trajectory, steps: int, corr: Optional[Corrector] =
     set_time(app, 0.0, tr.dt, 0)
     errs = []
     try:
+        if corr is not None:   # the field's address is fixed; each step's inference is queued before it
+            app.set_correction(corr.net.output_ptr(), corr.norm.scale)
         for n in range(steps):
             if corr is not None:
-                app.set_correction(corr.infer(), corr.norm.scale)
+                corr.infer()
             st = app.steps(1)
             if st != 0:
                 raise RuntimeError(f"coarse rollout halted with status {st} at step {n}")
@@ -573,11 +586,13 @@ def _timed_steps(app: GamApp, tr: Trajectory, n: int, corr: Optional[Corrector] 
     inferred and applied before each step when corr is given)."""
     load_pair(app, tr, 0)
     app.sync()
+    if corr is not None:
+        app.set_correction(corr.net.output_ptr(), corr.norm.scale)
     t0 = time.perf_counter()
     try:
         for _ in range(n):
             if corr is not None:
-                app.set_correction(corr.infer(), corr.norm.scale)
+                corr.infer()
             if app.steps(1) != 0:
                 raise RuntimeError("timed coarse step halted")
         app.sync()

@@ -33,6 +33,7 @@ def lib():
             "gam_net_destroy": (C.c_int, [H]),
             "gam_net_error": (C.c_char_p, [H]),
             "gam_net_stream": (C.c_void_p, [H]),
+            "gam_net_set_stream": (C.c_int, [H, C.c_void_p]),
             "gam_net_sync": (C.c_int, [H]),
             "gam_net_set_params": (C.c_int, [H, _fp]),
             "gam_net_get_params": (C.c_int, [H, _fp, _fp]),
@@ -40,6 +41,7 @@ def lib():
             "gam_net_features": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int]),
             "gam_net_set_features": (C.c_int, [H, _fp]),
             "gam_net_forward": (C.c_int, [H]),
+            "gam_net_infer": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
             "gam_net_output": (C.c_void_p, [H]),
             "gam_net_get_output": (C.c_int, [H, _fp]),
             "gam_net_zero_grad": (C.c_int, [H]),
@@ -133,6 +135,11 @@ class GamNet:
     def forward(self):
         self._chk(lib().gam_net_forward(self.h), "gam_net_forward")
 
+    def infer(self, Gas_ptr: int, B_ptr: int, precision: int, halo: int):
+        """Features and the three layers, output stored on the cells at
+        least `halo` from the grid faces (equal to forward's there)."""
+        self._chk(lib().gam_net_infer(self.h, Gas_ptr, B_ptr, precision, halo), "gam_net_infer")
+
     def output_ptr(self) -> int:
         return lib().gam_net_output(self.h)
 
@@ -162,6 +169,10 @@ class GamNet:
 
     def adam(self, lr: float, t: int, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
         self._chk(lib().gam_net_adam(self.h, lr, b1, b2, eps, t), "gam_net_adam")
+
+    def set_stream(self, stream_ptr):
+        """Queue the net's work on another hipStream_t (None: its own)."""
+        self._chk(lib().gam_net_set_stream(self.h, stream_ptr), "gam_net_set_stream")
 
     def sync(self):
         self._chk(lib().gam_net_sync(self.h), "gam_net_sync")

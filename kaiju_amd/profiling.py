@@ -176,6 +176,12 @@ NET_ROLE = {"k_conv<32, 0, 4>": "forward, layer 1 (8 -> 32, 4 taps per K)", "k_c
             "k_conv<16, 1, 1>": "forward, layer 3 (32 -> 8)", "k_conv<32, 2, 1>": "backward data, layers 3 and 2",
             "k_wgrad<32, 2>": "weight gradient, layer 2", "k_wgrad<16, 2>": "weight gradient, layer 3",
             "k_wgrad<32, 1>": "weight gradient, layer 1"}
+# inference layers (k_infer): MACs per stored cell, role, and the layer's stored
+# box, the active N^3 block widened by this many cells on every side
+NET_INFER = {"k_infer<8, 32, 0, double>": (32 * 8, "inference, features + layer 1 (FP64 state)", 2),
+             "k_infer<8, 32, 0, float>": (32 * 8, "inference, features + layer 1 (FP32 state)", 2),
+             "k_infer<32, 32, 0, float>": (32 * 32, "inference, layer 2", 1),
+             "k_infer<32, 16, 1, float>": (8 * 32, "inference, layer 3", 0)}
 
 
 def mfma(stats_csv: str, N: int, command: str) -> str:
@@ -185,21 +191,27 @@ def mfma(stats_csv: str, N: int, command: str) -> str:
     O = Ni * Ni + Ni + 1
     cells = 16 * ((nc - 2 * O) // 16)   # the cells every convolution computes
     lines = ["# Learned-correction network: MFMA kernels", "", "Command:", "", "```", command, "```", "",
-             f"Block {N}^3 (box {Ni}^3, {cells} cells computed per convolution).  Algorithmic FLOPs per launch = "
+             f"Block {N}^3 (box {Ni}^3, {cells} cells computed per training convolution; the inference layers "
+             f"store ({N}+4)^3, ({N}+2)^3 and {N}^3 cells).  Algorithmic FLOPs per launch = "
              "2 x cells x 27 taps x (output x input channels, real counts 8 -> 32 -> 32 -> 8); rate = FLOPs / mean "
              f"launch time; fraction of the {BF16_PEAK_TFLOPS:.0f} TFLOP/s dense BF16 MFMA peak.", "",
              "| kernel | role | calls | mean (us) | GFLOP/launch | TFLOP/s | of peak |", "|---|---|---:|---:|---:|---:|---:|"]
     tot_f = tot_t = 0.0
     for r in csv.DictReader(open(stats_csv)):
         short = r["Name"].replace("void ", "").split("(")[0]
-        if short not in NET_MACS:
+        if short in NET_MACS:
+            macs, role, nce = NET_MACS[short], NET_ROLE[short], cells
+        elif short in NET_INFER:
+            macs, role, w = NET_INFER[short]
+            nce = (N + 2 * w) ** 3
+        else:
             continue
         t = float(r["AverageNs"]) * 1e-9
-        f = 2.0 * cells * 27 * NET_MACS[short]
+        f = 2.0 * nce * 27 * macs
         n = int(r["Calls"])
         tot_f += f * n
         tot_t += t * n
-        lines.append(f"| `{short}` | {NET_ROLE[short]} | {n} | {t * 1e6:.1f} | {f / 1e9:.1f} | {f / t / 1e12:.1f} | "
+        lines.append(f"| `{short}` | {role} | {n} | {t * 1e6:.1f} | {f / 1e9:.1f} | {f / t / 1e12:.1f} | "
                      f"{f / t / 1e12 / BF16_PEAK_TFLOPS * 100:.2f}% |")
     if tot_t > 0:
         lines += ["", f"All network launches: {tot_f / 1e12:.2f} TFLOP in {tot_t * 1e3:.1f} ms = "

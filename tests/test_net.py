@@ -78,6 +78,51 @@ def test_gpu_forward_layers_and_output():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dims,halo,precision", [((21, 18, 17), 4, 0), ((21, 18, 17), 3, 1), ((40, 13, 11), 4, 1),
+                                                 ((24, 24, 24), 4, 0)])
+def test_gpu_infer_equals_forward_on_its_box(dims, halo, precision):
+    """gam_net_infer (features and the three layers, LDS-staged bricks) stores
+    exactly gam_net_forward's output on the cells at least `halo` from the
+    faces, bit for bit, and nothing elsewhere; grids that are no multiple of
+    the brick, FP64 and FP32 states."""
+    from kaiju_amd.gamera import DeviceArray
+    from kaiju_amd.net import GamNet, HipStepError
+    w, _, _ = _rand_problem(5, dims)
+    rng = np.random.default_rng(11)
+    nc = int(np.prod(dims))
+    rho, pr = rng.uniform(0.5, 1.5, nc), rng.uniform(0.5, 2.0, nc)
+    v, B = rng.normal(0, 0.5, (3, nc)), rng.normal(0, 1.0, (3, nc))
+    gas = np.concatenate([rho, rho * v[0], rho * v[1], rho * v[2], pr / (5.0 / 3.0 - 1) + 0.5 * rho * (v ** 2).sum(0)])
+    dt = np.float64 if precision == 0 else np.float32
+    say(f"[net] infer vs forward {dims} halo {halo} precision {precision}")
+    G, Bd = DeviceArray.of(gas, dt), DeviceArray.of(B.ravel(), dt)
+    try:
+        with GamNet(*dims) as net:
+            net.set_params(w)
+            net.set_scales(1.1, 0.7, 1.3, 0.9)
+            net.set_offsets(1.0, 1.2)
+            for bad in (2, (min(dims) + 1) // 2):   # three stencil layers; a box left on every axis
+                with pytest.raises(HipStepError):
+                    net.infer(G.ptr, Bd.ptr, precision, bad)
+            net.infer(G.ptr, Bd.ptr, precision, halo)
+            out_i = net.get_output()
+            net.features_from_device(G.ptr, Bd.ptr, precision)
+            net.forward()
+            out_f = net.get_output()
+    finally:
+        G.free()
+        Bd.free()
+    Ni, Nj, Nk = dims
+    oi, of = out_i.reshape(Nk, Nj, Ni, 8), out_f.reshape(Nk, Nj, Ni, 8)
+    box = (slice(halo, Nk - halo), slice(halo, Nj - halo), slice(halo, Ni - halo))
+    assert np.abs(of[box]).max() > 0
+    assert np.array_equal(oi[box].view(np.uint32), of[box].view(np.uint32))
+    outside = np.ones((Nk, Nj, Ni), bool)
+    outside[box] = False
+    assert not oi[outside].any()   # the output buffer starts at zero; infer writes the box only
+
+
+@pytest.mark.gpu
 def test_gpu_gradients_match_reference():
     from kaiju_amd.net import GamNet
     w, feat, dout = _rand_problem(4)
