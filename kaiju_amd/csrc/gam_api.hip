@@ -986,6 +986,29 @@ static void launch_cpr(gam_handle *h) {
     if (h->prec) launch_cpr_t<float>(h); else launch_cpr_t<double>(h);
 }
 
+// the flux launches with the learned correction added at the store (a block
+// with a correction is periodic, no ring average, so never packed)
+template <int RM, int BO, int MH, int BG, typename R>
+static void launch_flux_corr(gam_handle *h, int d, dim3 gr, dim3 gr2, Tile tl, const WKT<R> &w, const R *nB, const R *hB,
+                             const R *hmag, const R *hGas, R *Flx) {
+    const GVT<R> &g = view<R>(h);
+#define FLUX_CORR(KERN, GRID) hipLaunchKernelGGL(HIP_KERNEL_NAME KERN, (GRID), 256, 0, h->stream, g, tl, h->mp, h->bgv, h->clk, \
+                                                 w.Wn, nB, w.Wh, hB, hmag, hGas, Flx, h->corr, h->corr_scale)
+    if (d == 0) FLUX_CORR((k_flux<0, RM, BO, MH, BG, R, 1>), gr);
+    else if (sizeof(R) == sizeof(float)) {
+        if (d == 1) FLUX_CORR((k_flux<1, RM, BO, MH, BG, R, 1>), gr);
+        else FLUX_CORR((k_flux<2, RM, BO, MH, BG, R, 1>), gr);
+    }
+    else if (d == 1) {   // the plain pairs, then the correction on its own
+        FLUX_CORR((k_flux2<1, RM, BO, MH, BG, R, 0>), gr2);
+        const long n = (long)g.Nip * (g.Njp + 1) * (h->mp.do25D ? 1 : g.Nkp);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_corr_flux<1, R>), grid1(n), 256, 0, h->stream, g, h->mp, h->clk, Flx, h->corr,
+                           h->corr_scale);
+    }
+    else FLUX_CORR((k_flux2<2, RM, BO, MH, BG, R, 1>), gr2);
+#undef FLUX_CORR
+}
+
 template <int RM, int BO, int MH, int BG, typename R>
 static void launch_flux_bg(gam_handle *h, int d) {
     const GVT<R> &g = view<R>(h);
@@ -998,13 +1021,14 @@ static void launch_flux_bg(gam_handle *h, int d) {
     const R *nB = P<R>(h->B[S]), *hB = P<R>(h->hB), *hmag = P<R>(h->hmag), *hGas = P<R>(h->hGas);
     R *Flx = P<R>(h->Flx);
     StageTimer t(h, ST_FLUX_I + d);
-    if (d == 0) PLAUNCH(h, (k_flux<0, RM, BO, MH, BG, R>), (b_flux<0, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    if (h->corr) launch_flux_corr<RM, BO, MH, BG, R>(h, d, gr, gr2, tl, w, nB, hB, hmag, hGas, Flx);
+    else if (d == 0) PLAUNCH(h, (k_flux<0, RM, BO, MH, BG, R>), (b_flux<0, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx, h->corr, h->corr_scale);
     else if (sizeof(R) == sizeof(float)) {   // FP32 bulk: one interface per thread measured as fast as pairs
-        if (d == 1) PLAUNCH(h, (k_flux<1, RM, BO, MH, BG, R>), (b_flux<1, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-        else PLAUNCH(h, (k_flux<2, RM, BO, MH, BG, R>), (b_flux<2, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+        if (d == 1) PLAUNCH(h, (k_flux<1, RM, BO, MH, BG, R>), (b_flux<1, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx, h->corr, h->corr_scale);
+        else PLAUNCH(h, (k_flux<2, RM, BO, MH, BG, R>), (b_flux<2, RM, BO, MH, BG, R>), gr, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx, h->corr, h->corr_scale);
     }
-    else if (d == 1) PLAUNCH(h, (k_flux2<1, RM, BO, MH, BG, R>), (b_flux2<1, RM, BO, MH, BG, R>), gr2, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
-    else PLAUNCH(h, (k_flux2<2, RM, BO, MH, BG, R>), (b_flux2<2, RM, BO, MH, BG, R>), gr2, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx);
+    else if (d == 1) PLAUNCH(h, (k_flux2<1, RM, BO, MH, BG, R>), (b_flux2<1, RM, BO, MH, BG, R>), gr2, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx, h->corr, h->corr_scale);
+    else PLAUNCH(h, (k_flux2<2, RM, BO, MH, BG, R>), (b_flux2<2, RM, BO, MH, BG, R>), gr2, 256, g, tl, h->mp, h->bgv, h->clk, w.Wn, nB, w.Wh, hB, hmag, hGas, Flx, h->corr, h->corr_scale);
 }
 
 template <int RM, int BO, int MH, typename R>
@@ -1021,10 +1045,10 @@ static void launch_emf(gam_handle *h) {
     StageTimer t(h, st);
     if (ED == 1)   // J edges: k columns with the k stencil in registers
         PLAUNCH(h, (k_emf_jcol<RM, BO, R>), (b_emf_jcol<RM, BO, R>), tgrid(h, st, g.Nip + 1, g.Njp, cdiv(g.Nkp + 1, EMF_KC)),
-                256, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
+                256, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E), h->corr, h->corr_scale);
     else
         PLAUNCH(h, (k_emf<ED, RM, BO, R>), (b_emf<ED, RM, BO, R>), tgrid(h, st, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2)),
-                256, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E));
+                256, g, h->tile[st], h->mp, h->bgv, h->clk, P<R>(h->Vf), P<R>(h->hGas), w.bn, w.bh, P<R>(h->E), h->corr, h->corr_scale);
 }
 
 template <int RM, int BO, typename R>
@@ -1116,9 +1140,6 @@ static int advance_t(gam_handle *h, int fuse_dt) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fix_e<R>), grid1((long)(g.Nip + 1) * (g.Njp + 1)), 256, 0, h->stream, g,
                            h->clk, P<R>(h->E));
     }
-    if (h->corr)   // learned correction of fluxes and EMFs, before CT and the update
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_apply_corr<R>), grid1(g.nc), 256, 0, h->stream, g, h->clk, m.do25D, h->corr,
-                           h->corr_scale, P<R>(h->Flx), P<R>(h->E));
     {
         StageTimer t(h, ST_CT);
         if (m.doMHD) PLAUNCH(h, (k_ct<R>), (b_ct<R>), grid1(g.nf), 256, g, h->clk, h->mag[S], P<R>(h->E), h->mag[O]);

@@ -434,6 +434,20 @@ __device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, const BGV &b
     return e * ed;
 }
 
+// The learned coarse-grid correction, added where an EMF or a flux is stored
+// (periodic blocks only): scale * edge * o[c][5 + d] on the EMF of every active
+// edge of direction d, scale * face * o[c][v] on the gas flux v (0..4) of every
+// interface of the flux box, c the cell at the edge's lower corner or below the
+// interface, o read at c's periodic image (so the two copies of a boundary face
+// or edge get one correction).  The sums are FP64, rounded once to R.
+__device__ __forceinline__ int pwrap(int x, int a0, int n) { return a0 + ((x - a0) % n + n) % n; }
+template <typename R> __device__ __forceinline__ size_t pimg(const GVT<R> &g, int i, int j, int k) {
+    return (size_t)ci(g, pwrap(i, g.nG, g.Nip), pwrap(j, g.nG, g.Njp), pwrap(k, g.nG, g.Nkp));
+}
+template <typename R> __device__ __forceinline__ R corr_add(R x, double s, double len, float o) {
+    return R((double)x + s * len * (double)o);
+}
+
 // Edges of direction ED over their active box (fields.F90:110-225): I edges
 // [is,ie]x[js,je+1]x[ks,ke+1], J edges [is,ie+1]x[js,je]x[ks,ke+1], K edges
 // [is,ie+1]x[js,je+1]x[ks,ke]; E of edge ED at its lower corner (i,j,k).  E
@@ -443,17 +457,21 @@ __device__ __forceinline__ R emf_edge(const GVT<R> &g, const MP &m, const BGV &b
 template <int ED, int RM, int BORIS, typename R>
 __device__ __forceinline__ void b_emf(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk, const R *__restrict__ Vf,
                                              const R *__restrict__ hGas, const R *__restrict__ bn,
-                                             const R *__restrict__ bh, R *__restrict__ E) {
+                                             const R *__restrict__ bh, R *__restrict__ E,
+                                             const float *__restrict__ corr, double cs) {
     int a, b, cz;
     if (!tile_pos(vb, t, g.Nip + (ED != 0), g.Njp + (ED != 1), g.Nkp + (ED != 2), a, b, cz) || clk->halt) return;
     const int i = a + g.nG, j = b + g.nG, k = cz + g.nG;
-    E[(size_t)g.nc * ED + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, ED, i, j, k);
+    R e = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, ED, i, j, k);
+    if (corr) e = corr_add(e, cs, (double)g.edge[(size_t)g.nf * ED + fi(g, i, j, k)], corr[pimg(g, i, j, k) * 8 + 5 + ED]);
+    E[(size_t)g.nc * ED + ci(g, i, j, k)] = e;
 }
 template <int ED, int RM, int BORIS, typename R>
 __global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk, const R *__restrict__ Vf,
                                              const R *__restrict__ hGas, const R *__restrict__ bn,
-                                             const R *__restrict__ bh, R *__restrict__ E) {
-    b_emf<ED, RM, BORIS, R>(blockIdx.x, g, t, m, bg, clk, Vf, hGas, bn, bh, E);
+                                             const R *__restrict__ bh, R *__restrict__ E,
+                                             const float *__restrict__ corr, double cs) {
+    b_emf<ED, RM, BORIS, R>(blockIdx.x, g, t, m, bg, clk, Vf, hGas, bn, bh, E, corr, cs);
 }
 
 // J edges walking EMF_KC edges up a k column: the I faces' stencils along k
@@ -463,7 +481,8 @@ __global__ void __launch_bounds__(256) k_emf(GVT<R> g, Tile t, MP m, BGV bg, con
 template <int RM, int BORIS, typename R>
 __device__ __forceinline__ void b_emf_jcol(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                   const R *__restrict__ Vf, const R *__restrict__ hGas,
-                                                  const R *__restrict__ bn, const R *__restrict__ bh, R *__restrict__ E) {
+                                                  const R *__restrict__ bn, const R *__restrict__ bh, R *__restrict__ E,
+                                                  const float *__restrict__ corr, double cs) {
     const int nz = g.Nkp + 1;
     int a, b, cb;
     if (!tile_pos(vb, t, g.Nip + 1, g.Njp, cdiv(nz, EMF_KC), a, b, cb) || clk->halt) return;
@@ -489,15 +508,17 @@ __device__ __forceinline__ void b_emf_jcol(const unsigned vb, GVT<R> g, Tile t, 
             for (int n = 0; n < 3; n++) wN[n] = wN[n + 1];
             wA[7] = fI[f + 3 * st]; wB[7] = bhI[f + 3 * st]; wN[3] = bnI[f + st];
         }
-        E[(size_t)g.nc * 1 + ci(g, i, j, k)] = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, 1, i, j, k,
-                                                                   wA, wB, wN);
+        R e = emf_edge<RM, BORIS>(g, m, bg, clk->dt, Vf, hGas, bn, bh, 1, i, j, k, wA, wB, wN);
+        if (corr) e = corr_add(e, cs, (double)g.edge[(size_t)g.nf * 1 + f], corr[pimg(g, i, j, k) * 8 + 6]);
+        E[(size_t)g.nc * 1 + ci(g, i, j, k)] = e;
     }
 }
 template <int RM, int BORIS, typename R>
 __global__ void __launch_bounds__(256) k_emf_jcol(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                   const R *__restrict__ Vf, const R *__restrict__ hGas,
-                                                  const R *__restrict__ bn, const R *__restrict__ bh, R *__restrict__ E) {
-    b_emf_jcol<RM, BORIS, R>(blockIdx.x, g, t, m, bg, clk, Vf, hGas, bn, bh, E);
+                                                  const R *__restrict__ bn, const R *__restrict__ bh, R *__restrict__ E,
+                                                  const float *__restrict__ corr, double cs) {
+    b_emf_jcol<RM, BORIS, R>(blockIdx.x, g, t, m, bg, clk, Vf, hGas, bn, bh, E, corr, cs);
 }
 
 // ----------------------------------------------------------------- fluxes
@@ -722,13 +743,15 @@ __device__ __forceinline__ void face_flux2(const GVT<R> &g, const MP &m, const B
 
 // Interface fluxes in direction DN for the interface box
 // DN=I: [is,ie+1]x[js,je]x[ks,ke]  (k range a single slice in 2.5D)
-// Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.
-template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+// Output slabs Flx[(8*DN + v)*nc + c], v = 0..4 gas, 5..7 magnetic.  CORR = 1:
+// with the learned correction (corr, cs) added to the gas fluxes (corr_add); a
+// separate instantiation, so the plain kernels keep their registers.
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R, int CORR = 0>
 __device__ __forceinline__ void b_flux(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                               const R *__restrict__ Wn, const R *__restrict__ nB,
                                               const R *__restrict__ Wh, const R *__restrict__ hB,
                                               const R *__restrict__ hmag, const R *__restrict__ hGas,
-                                              R *__restrict__ Flx) {
+                                              R *__restrict__ Flx, const float *__restrict__ corr, double cs) {
     const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
     int a, b, cz;
     if (!tile_pos(vb, t, nx, ny, nz, a, b, cz) || clk->halt) return;
@@ -737,26 +760,32 @@ __device__ __forceinline__ void b_flux(const unsigned vb, GVT<R> g, Tile t, MP m
     const int c = ci(g, i, j, k);
     R o[8];
     face_flux<DN, RM, BORIS, MHD, BG>(g, m, bg, Wn, nB, Wh, hB, hmag, hGas, c, fi(g, i, j, k), o);
+    if (CORR && !(DN == 2 && m.do25D)) {   // CORR instantiations run with a correction set only
+        const double A = (double)g.face[(size_t)g.nf * DN + fi(g, i, j, k)];
+        const float *oc = corr + pimg(g, i, j, k) * 8;
+#pragma unroll
+        for (int v = 0; v < 5; v++) o[v] = corr_add(o[v], cs, A, oc[v]);
+    }
 #pragma unroll
     for (int v = 0; v < (MHD ? 8 : 5); v++) Flx[nc * (8 * DN + v) + c] = o[v];
 }
-template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R, int CORR = 0>
 __global__ void __launch_bounds__(256) k_flux(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                               const R *__restrict__ Wn, const R *__restrict__ nB,
                                               const R *__restrict__ Wh, const R *__restrict__ hB,
                                               const R *__restrict__ hmag, const R *__restrict__ hGas,
-                                              R *__restrict__ Flx) {
-    b_flux<DN, RM, BORIS, MHD, BG, R>(blockIdx.x, g, t, m, bg, clk, Wn, nB, Wh, hB, hmag, hGas, Flx);
+                                              R *__restrict__ Flx, const float *__restrict__ corr, double cs) {
+    b_flux<DN, RM, BORIS, MHD, BG, R, CORR>(blockIdx.x, g, t, m, bg, clk, Wn, nB, Wh, hB, hmag, hGas, Flx, corr, cs);
 }
 
 // j or k interfaces two at a time along i (face_flux2): thread (a, b, c) of
 // the box ceil(nx/2) x ny x nz owns interfaces i = 2a and 2a + 1.
-template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R, int CORR = 0>
 __device__ __forceinline__ void b_flux2(const unsigned vb, GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                         const R *__restrict__ Wn, const R *__restrict__ nB,
                                         const R *__restrict__ Wh, const R *__restrict__ hB,
                                         const R *__restrict__ hmag, const R *__restrict__ hGas,
-                                        R *__restrict__ Flx) {
+                                        R *__restrict__ Flx, const float *__restrict__ corr, double cs) {
     static_assert(DN != 0, "pairs along i serve the j and k interfaces");
     const int nx = g.Nip, ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
     int a, b, cz;
@@ -767,19 +796,49 @@ __device__ __forceinline__ void b_flux2(const unsigned vb, GVT<R> g, Tile t, MP 
     R oA[8], oB[8];
     face_flux2<DN, RM, BORIS, MHD, BG>(g, m, bg, Wn, nB, Wh, hB, hmag, hGas, c, fi(g, i, j, k), oA, oB);
     const bool two = 2 * a + 1 < nx;
+    if (CORR && !(DN == 2 && m.do25D)) {   // CORR instantiations run with a correction set only
+        const int f = fi(g, i, j, k);
+        const double A = (double)g.face[(size_t)g.nf * DN + f];
+        const float *oc = corr + pimg(g, i, j, k) * 8;
+#pragma unroll
+        for (int v = 0; v < 5; v++) oA[v] = corr_add(oA[v], cs, A, oc[v]);
+        if (two) {
+            const double AB = (double)g.face[(size_t)g.nf * DN + f + 1];
+            const float *ob = corr + pimg(g, i + 1, j, k) * 8;
+#pragma unroll
+            for (int v = 0; v < 5; v++) oB[v] = corr_add(oB[v], cs, AB, ob[v]);
+        }
+    }
 #pragma unroll
     for (int v = 0; v < (MHD ? 8 : 5); v++) {
         Flx[nc * (8 * DN + v) + c] = oA[v];
         if (two) Flx[nc * (8 * DN + v) + c + 1] = oB[v];
     }
 }
-template <int DN, int RM, int BORIS, int MHD, int BG, typename R>
+template <int DN, int RM, int BORIS, int MHD, int BG, typename R, int CORR = 0>
 __global__ void __launch_bounds__(256) k_flux2(GVT<R> g, Tile t, MP m, BGV bg, const DClock *__restrict__ clk,
                                                const R *__restrict__ Wn, const R *__restrict__ nB,
                                                const R *__restrict__ Wh, const R *__restrict__ hB,
                                                const R *__restrict__ hmag, const R *__restrict__ hGas,
-                                               R *__restrict__ Flx) {
-    b_flux2<DN, RM, BORIS, MHD, BG, R>(blockIdx.x, g, t, m, bg, clk, Wn, nB, Wh, hB, hmag, hGas, Flx);
+                                               R *__restrict__ Flx, const float *__restrict__ corr, double cs) {
+    b_flux2<DN, RM, BORIS, MHD, BG, R, CORR>(blockIdx.x, g, t, m, bg, clk, Wn, nB, Wh, hB, hmag, hGas, Flx, corr, cs);
+}
+// The correction of the gas fluxes in direction DN on their own (corr_add on the
+// stored fluxes of the flux box), for the one flux kernel whose correcting
+// instantiation the compiler cannot fit in its plain registers (the FP64 j pairs).
+template <int DN, typename R>
+__global__ void __launch_bounds__(256) k_corr_flux(GVT<R> g, MP m, const DClock *__restrict__ clk, R *__restrict__ Flx,
+                                                   const float *__restrict__ corr, double cs) {
+    const int nx = g.Nip + (DN == 0), ny = g.Njp + (DN == 1), nz = (m.do25D ? 1 : g.Nkp) + (DN == 2);
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= (long)nx * ny * nz || clk->halt || (DN == 2 && m.do25D)) return;
+    const int i = (int)(q % nx) + g.nG, j = (int)((q / nx) % ny) + g.nG, k = (int)(q / ((long)nx * ny)) + g.nG;
+    const size_t nc = g.nc;
+    const int c = ci(g, i, j, k);
+    const double A = (double)g.face[(size_t)g.nf * DN + fi(g, i, j, k)];
+    const float *oc = corr + pimg(g, i, j, k) * 8;
+#pragma unroll
+    for (int v = 0; v < 5; v++) Flx[nc * (8 * DN + v) + c] = corr_add(Flx[nc * (8 * DN + v) + c], cs, A, oc[v]);
 }
 
 // ----------------------------------------------------------------- CT update
@@ -1610,7 +1669,9 @@ __global__ void __launch_bounds__(256) k_sum_ordered(const double *__restrict__ 
 // d at cell c (the edge at the cell's lower corner), before CT and the update:
 // the update stays conservative and E2Flux keeps div.B exact.  A ghost cell
 // uses the o of its periodic image (the step's boundaries are periodic), so
-// the two copies of a boundary face or edge get one correction.
+// the two copies of a boundary face or edge get one correction.  The step
+// adds it where k_emf/k_emf_jcol and k_flux/k_flux2 store (corr_add); the
+// training kernels below use the same faces (flux_box) and edges (edge_active).
 template <typename R> __device__ __forceinline__ bool flux_box(const GVT<R> &g, int do25D, int d, int i, int j, int k) {
     const int a0 = g.nG;
     const int nk = do25D ? 1 : g.Nkp;
@@ -1621,37 +1682,6 @@ template <typename R> __device__ __forceinline__ bool edge_active(const GVT<R> &
     const int a0 = g.nG;
     return i >= a0 && i <= a0 + g.Nip - (d == 0) && j >= a0 && j <= a0 + g.Njp - (d == 1) && k >= a0 &&
            k <= a0 + g.Nkp - (d == 2);
-}
-__device__ __forceinline__ int pwrap(int x, int a0, int n) { return a0 + ((x - a0) % n + n) % n; }
-template <typename R> __device__ __forceinline__ size_t pimg(const GVT<R> &g, int i, int j, int k) {
-    return (size_t)ci(g, pwrap(i, g.nG, g.Nip), pwrap(j, g.nG, g.Njp), pwrap(k, g.nG, g.Nkp));
-}
-
-template <typename R>
-__global__ void __launch_bounds__(256) k_apply_corr(GVT<R> g, const DClock *__restrict__ clk, int do25D,
-                                                    const float *__restrict__ o, double s, R *__restrict__ Flx,
-                                                    R *__restrict__ E) {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.nc || clk->halt) return;
-    int i, j, k;
-    decode_c(g, idx, i, j, k);
-    const size_t nc = g.nc, nf = g.nf;
-    const int f = fi(g, i, j, k);
-    const float *oc = o + pimg(g, i, j, k) * 8;
-    for (int d = 0; d < 3; d++) {
-        if (flux_box(g, do25D, d, i, j, k)) {
-            const double A = (double)g.face[nf * d + f];
-#pragma unroll
-            for (int v = 0; v < 5; v++) {
-                const size_t q = nc * (8 * d + v) + idx;
-                Flx[q] = R((double)Flx[q] + s * A * (double)oc[v]);
-            }
-        }
-        if (edge_active(g, d, i, j, k)) {
-            const size_t q = nc * d + idx;
-            E[q] = R((double)E[q] + s * (double)g.edge[nf * d + f] * (double)oc[5 + d]);
-        }
-    }
 }
 
 // Training: the change the correction o makes to one step, linearised
