@@ -136,6 +136,12 @@ int  gam_memcpy(void *dst, const void *src, size_t bytes);
  * into dout (device FP32 [G][8]). */
 int  gam_state_device(gam_handle *h, int which, void **Gas, double **magFlux, void **Bxyz);
 int  gam_set_correction(gam_handle *h, const float *o, double scale);
+/* A hipEvent_t the step waits for (hipStreamWaitEvent on its stream) right
+ * before its first kernel that reads o, e.g. the event gam_net_infer records
+ * on the net's own stream: the network's inference then overlaps the step's
+ * predictor, bFlux2Fld and Mom2Face.  NULL: no wait.  Graphs refuse handles
+ * with a wait set. */
+int  gam_set_correction_wait(gam_handle *h, void *event);
 int  gam_correction_target(gam_handle *h, const double *dGas, const double *dMag);
 int  gam_correction_target_device(gam_handle *h, const double *dGas, const double *dMag);  /* device FP64 */
 int  gam_correction_grad(gam_handle *h, const float *o, double scale, double dt, const double *w4,

@@ -50,6 +50,10 @@ int  gam_net_forward(gam_net *n);
  * the activations kept for gam_net_backward are not all rewritten (train
  * through gam_net_features + gam_net_forward). */
 int  gam_net_infer(gam_net *n, const void *Gas, const void *Bxyz, int precision, int halo);
+/* The hipEvent_t gam_net_infer records after its last kernel (the net's
+ * lifetime): hand it to gam_set_correction_wait so that a step on another
+ * stream waits for the output only where it reads it. */
+void *gam_net_infer_event(gam_net *n);
 const float *gam_net_output(gam_net *n);                           /* device FP32 [G][8] */
 int  gam_net_get_output(gam_net *n, float *out);                   /* host copy */
 

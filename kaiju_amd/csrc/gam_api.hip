@@ -57,6 +57,7 @@ struct gam_handle {
     double *bg[5] = {};        // background field fcB0, edgB0, B0, bFlux0, dpB0 (G-sized)
     const float *corr = nullptr; // learned correction field [G][8] (device), or none
     double corr_scale = 0.0;
+    hipEvent_t corr_wait = nullptr;    // the step waits for it before the first kernel that reads corr
     int bct[6] = {0, 0, 0, 0, 0, 0};   // boundary type per face INI, OUTI, INJ, OUTJ, INK, OUTK (GAM_BC_*)
     int docpr = 0;                     // params doCPR: StateCPR after a bad CalcDT
     RingP ring = {};                   // ring average about the LFM axis (gam_set_ring)
@@ -845,6 +846,15 @@ extern "C" int gam_state_device(gam_handle *h, int which, void **Gas, double **m
     return 0;
 }
 
+extern "C" int gam_set_correction_wait(gam_handle *h, void *event) {
+    if (!h) return -1;
+    if (refuse_if_captured(h, "gam_set_correction_wait")) return -1;
+    int rc = gam_sync(h);
+    if (rc) return rc;
+    h->corr_wait = (hipEvent_t)event;
+    return 0;
+}
+
 extern "C" int gam_set_correction(gam_handle *h, const float *o, double scale) {
     if (!h) return -1;
     if (refuse_if_captured(h, "gam_set_correction")) return -1;
@@ -1122,9 +1132,10 @@ static int advance_t(gam_handle *h, int fuse_dt) {
         { StageTimer t(h, ST_MOM2FACE);
           PLAUNCH(h, (k_vf<R>), (b_vf<R>), tgrid(h, ST_MOM2FACE, g.Ni, g.Nj, cdiv(g.Nk, VF_KC)), 256, g,
                   h->tile[ST_MOM2FACE], h->clk, P<R>(h->wk[W_VDV]), P<R>(h->Vf)); }
+        if (h->corr && h->corr_wait) HIPCHK(h, hipStreamWaitEvent(h->stream, h->corr_wait, 0));
         if (m.rmeth == 1) { if (m.doBoris) launch_emfs<1, 1, R>(h); else launch_emfs<1, 0, R>(h); }
         else { if (m.doBoris) launch_emfs<0, 1, R>(h); else launch_emfs<0, 0, R>(h); }
-    }
+    } else if (h->corr && h->corr_wait) HIPCHK(h, hipStreamWaitEvent(h->stream, h->corr_wait, 0));
     if (m.doMHD) {
         if (m.rmeth == 1) { if (m.doBoris) launch_fluxes<1, 1, 1, R>(h); else launch_fluxes<1, 0, 1, R>(h); }
         else { if (m.doBoris) launch_fluxes<0, 1, 1, R>(h); else launch_fluxes<0, 0, 1, R>(h); }
@@ -1634,6 +1645,8 @@ extern "C" int gam_graph_create(gam_handle *const *hs, int n, int nsteps, gam_gr
         int rc = gam_sync(hs[q]);
         if (rc) return rc;
         if (hs[q]->timing) return fail(hs[q], -1, "gam_graph_create: stage timers must be off");
+        if (hs[q]->corr_wait)
+            return fail(hs[q], -1, "gam_graph_create: the correction waits for an event of another stream (gam_set_correction_wait)");
     }
     gam_graph *g = new gam_graph();
     g->hs.assign(hs, hs + n);

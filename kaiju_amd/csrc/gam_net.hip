@@ -505,6 +505,7 @@ struct gam_net {
     NetDims d;
     hipStream_t stream = nullptr;        // where the net's work goes: its own stream or the caller's
     hipStream_t own = nullptr;           // the stream the net created
+    hipEvent_t done = nullptr;           // recorded after each gam_net_infer
     // parameters: FP32 master, grads, Adam moments
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     u16 *Wf[3] = {}, *Wb[3] = {};
@@ -540,6 +541,7 @@ extern "C" int gam_net_destroy(gam_net *n) {
     for (int l = 0; l < 3; l++) { (void)hipFree(n->Wf[l]); (void)hipFree(n->Wb[l]); }
     for (int q = 0; q < 4; q++) { (void)hipFree(n->act[q]); (void)hipFree(n->actT0[q]); }
     for (int q = 0; q < 2; q++) { (void)hipFree(n->gr[q]); (void)hipFree(n->grT0[q]); }
+    if (n->done) (void)hipEventDestroy(n->done);
     if (n->own) (void)hipStreamDestroy(n->own);
     delete n;
     return 0;
@@ -564,6 +566,7 @@ extern "C" int gam_net_create(int Ni, int Nj, int Nk, gam_net **out) {
     auto cleanup = [&](int c) { gam_net_destroy(n); return c; };
     if (hipStreamCreateWithFlags(&n->own, hipStreamNonBlocking) != hipSuccess) return cleanup(-2);
     n->stream = n->own;
+    if (hipEventCreateWithFlags(&n->done, hipEventDisableTiming) != hipSuccess) return cleanup(-2);
     const int np = gam_net_num_params();
     for (float **p : {&n->w, &n->g, &n->m, &n->v}) {
         if (hipMalloc(p, np * sizeof(float)) != hipSuccess) return cleanup(-3);
@@ -720,8 +723,11 @@ extern "C" int gam_net_infer(gam_net *n, const void *Gas, const void *B, int pre
                        (const float *)nullptr, (const float *)nullptr, n->gamma, n->scales, n->offs, (const u16 *)n->Wf[2],
                        (const float *)(n->w + boff(2)), (u16 *)nullptr, n->out);
     NCHK(n, hipGetLastError());
+    NCHK(n, hipEventRecord(n->done, n->stream));
     return 0;
 }
+
+extern "C" void *gam_net_infer_event(gam_net *n) { return n ? (void *)n->done : nullptr; }
 
 // dout: device FP32 [nc][8] = dL/d(out).  Accumulates into the gradient buffer
 // (gam_net_zero_grad clears it).  Uses the activations of the last forward.

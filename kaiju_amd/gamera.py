@@ -153,6 +153,7 @@ def lib():
         "gam_state_device": (C.c_int, [H, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_void_p)]),
         "gam_set_correction": (C.c_int, [H, C.c_void_p, C.c_double]),
+        "gam_set_correction_wait": (C.c_int, [H, C.c_void_p]),
         "gam_set_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
         "gam_get_bcs": (C.c_int, [H, C.POINTER(C.c_int)]),
         "gam_set_ring": (C.c_int, [H, C.c_int, C.POINTER(C.c_int)]),
@@ -572,6 +573,11 @@ class GamApp:
     def set_correction(self, o_ptr: Optional[int], scale: float = 1.0):
         """Apply the device FP32 correction field o [G][8] in every following step (None: off)."""
         self._chk(lib().gam_set_correction(self.h, o_ptr, float(scale)), "gam_set_correction")
+
+    def set_correction_wait(self, event_ptr: Optional[int]):
+        """A hipEvent_t each following step waits for before its first kernel
+        that reads the correction (None: no wait)."""
+        self._chk(lib().gam_set_correction_wait(self.h, event_ptr), "gam_set_correction_wait")
 
     def correction_target(self, dGas: np.ndarray, dMag: np.ndarray):
         """Target one-step change (Gas (G,5), magFlux (F,3)) for correction_grad."""

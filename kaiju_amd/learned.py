@@ -43,7 +43,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from kaiju_amd import configs
-from kaiju_amd.gamera import DeviceArray, GamApp, lib as gam_lib
+from kaiju_amd.gamera import DeviceArray, GamApp
 
 GAMMA = 5.0 / 3.0
 PRIM = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
@@ -383,7 +383,10 @@ class Corrector:
         from kaiju_amd.net import GamNet, init_params
         self.app, self.norm = app, norm
         self.net = GamNet(*app.G)
-        self.net.set_stream(gam_lib().gam_stream(app.h))   # the block's stream: no host waits between net and step
+        # the net keeps its own stream; a step waits for its inference only
+        # before the kernels that read the correction, so the inference
+        # overlaps the step's predictor, bFlux2Fld and Mom2Face
+        app.set_correction_wait(self.net.infer_event())
         self.net.set_params(init_params(seed, out_scale=0.02) if params is None else params)
         self.net.set_scales(*norm.feat_scales, gamma=GAMMA)
         self.net.set_offsets(*norm.feat_offsets)
@@ -391,6 +394,7 @@ class Corrector:
         self.zero = DeviceArray.of(np.zeros(int(np.prod(app.G)) * 8, np.float32))
 
     def close(self):
+        self.app.set_correction_wait(None)
         self.net.close()
         self.zero.free()
 
@@ -398,7 +402,9 @@ class Corrector:
         """Correction field for the app's current State (device, FP32 [G][8]),
         stored on the active cells, the only ones the step reads
         (gam_net_infer: LDS-staged layers, no activations kept for training).
-        Queued on the block's stream, so a step queued after it reads it."""
+        Queued on the net's stream after the block's queued work; the step
+        waits for it where it reads the correction."""
+        self.app.sync()
         g, _, b = self.app.state_device(0)
         self.net.infer(g, b, self.precision, self.app.nG)
         return self.net.output_ptr()
@@ -406,6 +412,7 @@ class Corrector:
     def forward(self):
         """The same field through the training forward, which keeps every
         layer's activations for gam_net_backward."""
+        self.app.sync()
         g, _, b = self.app.state_device(0)
         self.net.features_from_device(g, b, self.precision)
         self.net.forward()

@@ -42,6 +42,7 @@ def lib():
             "gam_net_set_features": (C.c_int, [H, _fp]),
             "gam_net_forward": (C.c_int, [H]),
             "gam_net_infer": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+            "gam_net_infer_event": (C.c_void_p, [H]),
             "gam_net_output": (C.c_void_p, [H]),
             "gam_net_get_output": (C.c_int, [H, _fp]),
             "gam_net_zero_grad": (C.c_int, [H]),
@@ -139,6 +140,10 @@ class GamNet:
         """Features and the three layers, output stored on the cells at
         least `halo` from the grid faces (equal to forward's there)."""
         self._chk(lib().gam_net_infer(self.h, Gas_ptr, B_ptr, precision, halo), "gam_net_infer")
+
+    def infer_event(self) -> int:
+        """The hipEvent_t recorded after each infer (for gam_set_correction_wait)."""
+        return lib().gam_net_infer_event(self.h)
 
     def output_ptr(self) -> int:
         return lib().gam_net_output(self.h)
