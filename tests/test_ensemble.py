@@ -210,6 +210,34 @@ def test_graph_refuses_changes_to_held_handles():
     b.close()
 
 
+@pytest.mark.gpu
+def test_graph_refuses_a_correction_wait():
+    """A step that waits for another stream's event (gam_set_correction_wait)
+    cannot be captured: the graph refuses the handle, takes it once the wait
+    is cleared, and refuses a new wait while it holds the handle."""
+    from kaiju_amd.gamera import HipStepError
+    from kaiju_amd.net import GamNet
+    specs = small_members(1)[:1]
+    a = eager_twins(specs)[0]
+    net = GamNet(*a.G)
+    try:
+        a.set_correction_wait(net.infer_event())
+        with pytest.raises(HipStepError, match="gam_set_correction_wait"):
+            Ensemble(specs, tune=False, nsteps=2, apps=[a])
+        a.set_correction_wait(None)
+        ens = Ensemble(specs, tune=False, nsteps=2, apps=[a])
+        try:
+            with pytest.raises(HipStepError):
+                a.set_correction_wait(net.infer_event())
+            assert ens.run(2) == 0
+        finally:
+            ens.apps = []   # the app is closed below
+            ens.close()
+    finally:
+        net.close()
+        a.close()
+
+
 def _totals(s: dict, vol: np.ndarray, nG: int):
     """Mass, momentum and the momentum scale sum |rho v| dV over the active cells."""
     A = tuple(slice(nG, -nG) for _ in range(3))
