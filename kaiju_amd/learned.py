@@ -394,7 +394,8 @@ class Corrector:
         self.zero = DeviceArray.of(np.zeros(int(np.prod(app.G)) * 8, np.float32))
 
     def close(self):
-        self.app.set_correction_wait(None)
+        if getattr(self.app, "h", None) is not None:   # the block forgets the event before the net frees it
+            self.app.set_correction_wait(None)
         self.net.close()
         self.zero.free()
 
