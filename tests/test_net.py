@@ -123,6 +123,34 @@ def test_gpu_infer_equals_forward_on_its_box(dims, halo, precision):
 
 
 @pytest.mark.gpu
+def test_gpu_net_on_a_block_stream():
+    """gam_net_set_stream queues the net on a block's stream (gam_stream) and
+    back on its own: the same output either way, and the infer event exists."""
+    from kaiju_amd import configs
+    from kaiju_amd.gamera import GamApp, lib
+    from kaiju_amd.net import GamNet
+    spec = configs.orszag_tang((16, 16, 8), precision=0)
+    with GamApp(spec) as app:
+        app.InitModel()
+        g, _, b = app.state_device(0)
+        w, _, _ = _rand_problem(6, app.G)
+        with GamNet(*app.G) as net:
+            net.set_params(w)
+            assert net.infer_event()
+            net.infer(g, b, 0, app.nG)
+            own = net.get_output()
+            net.set_stream(lib().gam_stream(app.h))
+            net.infer(g, b, 0, app.nG)
+            shared = net.get_output()
+            net.set_stream(None)
+            net.infer(g, b, 0, app.nG)
+            back = net.get_output()
+    assert np.abs(own).max() > 0
+    assert np.array_equal(own.view(np.uint32), shared.view(np.uint32))
+    assert np.array_equal(own.view(np.uint32), back.view(np.uint32))
+
+
+@pytest.mark.gpu
 def test_gpu_gradients_match_reference():
     from kaiju_amd.net import GamNet
     w, feat, dout = _rand_problem(4)
