@@ -47,6 +47,7 @@ struct gam_handle {
     hipStream_t stream = nullptr;
     double *grid[G_N] = {};
     float *grid32[9] = {};
+    float *b2fc = nullptr;               // FP32 bulk: bFlux2Fld lever arms (18 x G)
     // bulk-typed arrays (double or float by prec); magFlux is always FP64
     void *Gas[2] = {}, *B[2] = {};
     double *mag[2] = {};
@@ -218,6 +219,7 @@ template <typename R> __device__ __forceinline__ GVT<R> pk_glob(const GVT<R> &g)
     o.xyz = pk_glob(g.xyz); o.xyzcc = pk_glob(g.xyzcc); o.xfc = pk_glob(g.xfc); o.dvol = pk_glob(g.dvol);
     o.face = pk_glob(g.face); o.Tf = pk_glob(g.Tf); o.edge = pk_glob(g.edge); o.Te = pk_glob(g.Te);
     o.Teb = pk_glob(g.Teb); o.vol = pk_glob(g.vol); o.di = pk_glob(g.di); o.dj = pk_glob(g.dj); o.dk = pk_glob(g.dk);
+    o.b2fc = pk_glob(g.b2fc);
     return o;
 }
 template <typename R> __device__ __forceinline__ WKT<R> pk_glob(const WKT<R> &w) {
@@ -326,6 +328,8 @@ static int sync_grid32(gam_handle *h) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_cvt<double, float>), dim3((unsigned)((m + 255) / 256)), 256, 0, h->stream,
                            (const double *)h->grid[kG32[q]], h->grid32[q], m);
     }
+    hipLaunchKernelGGL(k_b2f_coef, grid1((long)G), 256, 0, h->stream, h->gv32, (const double *)h->grid[G_XFC],
+                       (const double *)h->grid[G_XYZCC], (const double *)h->grid[G_VOL], h->b2fc);
     HIPCHK(h, hipGetLastError());
     return 0;
 }
@@ -389,9 +393,11 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     const size_t gcomp[G_N] = {3 * F, 3 * G, 9 * F, 3 * F, 27 * F, 3 * F, 27 * F, 12 * F, G, G, G, G};
     for (int q = 0; q < G_N; q++)
         if (hipMalloc(&h->grid[q], gcomp[q] * sizeof(double)) != hipSuccess) return cleanup(-3);
-    if (h->prec)
+    if (h->prec) {
         for (int q = 0; q < 9; q++)
             if (hipMalloc(&h->grid32[q], gcomp[kG32[q]] * sizeof(float)) != hipSuccess) return cleanup(-3);
+        if (hipMalloc(&h->b2fc, 18 * G * sizeof(float)) != hipSuccess) return cleanup(-3);
+    }
     for (int q = 0; q < 2; q++) {
         if (hipMalloc(&h->Gas[q], 5 * G * es) != hipSuccess) return cleanup(-3);
         if (hipMalloc(&h->mag[q], 3 * F * sizeof(double)) != hipSuccess) return cleanup(-3);
@@ -441,6 +447,8 @@ extern "C" int gam_create(const gam_dims *dims, const gam_params *par, gam_handl
     g32.xyz = g.xyz; g32.xyzcc = g.xyzcc; g32.xfc = g.xfc; g32.dvol = g.dvol;
     g32.face = h->grid32[0]; g32.Tf = h->grid32[1]; g32.edge = h->grid32[2]; g32.Te = h->grid32[3];
     g32.Teb = h->grid32[4]; g32.vol = h->grid32[5]; g32.di = h->grid32[6]; g32.dj = h->grid32[7]; g32.dk = h->grid32[8];
+    g32.b2fc = h->b2fc;
+    g.b2fc = nullptr;
     if (sync_grid32(h)) return cleanup(-2);
     if (hipStreamSynchronize(h->stream) != hipSuccess) return cleanup(-2);
     *out = h;
@@ -456,6 +464,7 @@ extern "C" int gam_destroy(gam_handle *h) {
     for (auto e : h->evpool) (void)hipEventDestroy(e);
     for (int q = 0; q < G_N; q++) if (h->grid[q]) (void)hipFree(h->grid[q]);
     for (int q = 0; q < 9; q++) if (h->grid32[q]) (void)hipFree(h->grid32[q]);
+    if (h->b2fc) (void)hipFree(h->b2fc);
     for (int q = 0; q < 5; q++) if (h->bg[q]) (void)hipFree(h->bg[q]);
     for (double *b : {h->ctG, h->ctM, h->crG, h->crM, h->ringSE, h->ringTab, h->ringMean, h->windSer}) if (b) (void)hipFree(b);
     for (int q = 0; q < 2; q++) { (void)hipFree(h->Gas[q]); (void)hipFree(h->mag[q]); (void)hipFree(h->B[q]); }

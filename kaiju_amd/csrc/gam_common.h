@@ -56,12 +56,17 @@ typedef WKT<double> WK;
 
 // Grid view: sizes + geometry pointers (device).  Positions (xyz, xyzcc, xfc)
 // and the FP64 volume dvol stay double in both precisions: bFlux2Fld's lever
-// arms are differences of nearby positions and Flux2Deltas divides by dvol.
+// arms are differences of nearby positions (FP32 bulk reads them from b2fc,
+// formed in FP64 and rounded once) and Flux2Deltas divides by dvol.
 template <typename R> struct GVT {
     int Ni, Nj, Nk, nG, Nip, Njp, Nkp;
     int nc, nf;           // number of cells (G) and face points (F)
     const double *xyz, *xyzcc, *xfc, *dvol;
     const R *face, *Tf, *edge, *Te, *Teb, *vol, *di, *dj, *dk;
+    // FP32 bulk: bFlux2Fld's lever arms per cell, 18 slabs [(3*face + x)*nc + c]
+    // of +-(face centre - cell centre)/volume for its lower/upper I, J, K faces
+    // (differences formed in FP64, then rounded); null in FP64
+    const float *b2fc;
 };
 typedef GVT<double> GV;
 

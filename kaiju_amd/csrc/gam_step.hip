@@ -51,12 +51,30 @@ template <typename T> __device__ __forceinline__ T ab2(T dt, T Q, T oQ, T odt) {
     return Q + dt * dQ;
 }
 
-// gridutils.F90:648-680 gamCellBxyz, in FP64 from a magFlux of type TM
+// gridutils.F90:648-680 gamCellBxyz, in FP64 from a magFlux of type TM.
+// FP32 bulk: the same sum over the six faces with the cell's lever arms from
+// the table g.b2fc (18 floats per cell instead of 13 FP64 positions and the
+// volume; the stored Bxyz is FP32 either way).
 template <typename R, typename TM>
 __device__ __forceinline__ void cell_bxyz(const GVT<R> &g, const TM *mag, int i, int j, int k, double *B) {
     const int nf = g.nf;
     int c = ci(g, i, j, k), f0 = fi(g, i, j, k);
     int fi1 = f0 + 1, fj1 = f0 + (g.Ni + 1), fk1 = f0 + (g.Ni + 1) * (g.Nj + 1);
+    if constexpr (sizeof(R) == sizeof(float)) {
+        if (g.b2fc) {
+            const size_t nc = g.nc;
+            const double fl[6] = {(double)mag[f0], (double)mag[fi1], (double)mag[nf + f0], (double)mag[nf + fj1],
+                                  (double)mag[2 * (size_t)nf + f0], (double)mag[2 * (size_t)nf + fk1]};
+#pragma unroll
+            for (int x = 0; x < 3; x++) {
+                double b = fl[0] * (double)g.b2fc[(size_t)x * nc + c];
+#pragma unroll
+                for (int q = 1; q < 6; q++) b = b + fl[q] * (double)g.b2fc[(size_t)(3 * q + x) * nc + c];
+                B[x] = b;
+            }
+            return;
+        }
+    }
     double dV = g.dvol[c];
     double fI = mag[f0], fJ = mag[nf + f0], fK = mag[2 * nf + f0];
     double fIp = mag[fi1], fJp = mag[nf + fj1], fKp = mag[2 * (size_t)nf + fk1];
@@ -70,6 +88,29 @@ __device__ __forceinline__ void cell_bxyz(const GVT<R> &g, const TM *mag, int i,
         double b = (fIp * xfIp + fJp * xfJp + fKp * xfKp - fI * xfI - fJ * xfJ - fK * xfK) / dV;
         B[x] = b - Div * g.xyzcc[(size_t)g.nc * x + c] / dV;
     }
+}
+
+// FP32 bulk: the lever arms of every cell for cell_bxyz, +-(face centre - cell
+// centre)/volume of its lower (-) and upper (+) I, J, K faces, formed in FP64
+__global__ void __launch_bounds__(256) k_b2f_coef(GVT<float> g, const double *__restrict__ xf, const double *__restrict__ xc,
+                                                  const double *__restrict__ dvol, float *__restrict__ out) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= g.nc) return;
+    int i, j, k;
+    decode_c(g, c, i, j, k);
+    const size_t nc = g.nc, nf = g.nf;
+    const int f0 = fi(g, i, j, k);
+    const int fu[3] = {f0 + 1, f0 + (g.Ni + 1), f0 + (g.Ni + 1) * (g.Nj + 1)};
+    const double dV = dvol[c];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            const double cx = xc[nc * x + c];
+            const double lo = xf[nf * (x + 3 * d) + f0], up = xf[nf * (x + 3 * d) + fu[d]];
+            out[(size_t)(3 * (2 * d) + x) * nc + c] = (float)(-(lo - cx) / dV);
+            out[(size_t)(3 * (2 * d + 1) + x) * nc + c] = (float)((up - cx) / dV);
+        }
 }
 
 // The same, with the lower K face's flux and centre passed in (kc.fK, kc.xfK)
