@@ -1140,7 +1140,7 @@ __device__ __forceinline__ void b_update(const unsigned vb, GVT<R> g, Tile t, MP
         if (active) {
             const R dt = R(clk->dt);
             const int ci1 = idx + 1, cj1 = idx + g.Ni, ck1 = idx + g.Ni * g.Nj;
-            const R dV = R(g.dvol[idx]);
+            const R dV = g.vol[idx];   // FP32 bulk: the FP32 copy of dvol (the same rounding, half the bytes)
             auto F = [&](int slab, int c) { return Flx[nc * slab + c]; };
             R dGH[5], dGM[3] = {0, 0, 0};
 #pragma unroll
